@@ -1,0 +1,216 @@
+// fx_device.hpp — device-side building blocks shared by the gfx950 kernels.
+//
+// All HIP sources are compiled with -ffp-contract=off: a fused multiply-add
+// appears only where fma()/fmaf() is written, which is exactly where the
+// reference CPU backend writes `mul_add` (crates/backend-cpu/src/ops.rs).
+// Stencil and BLAS-1 results are therefore bit-identical to the reference's
+// arithmetic; only reductions differ (a different, more accurate summation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fxd {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;            // 4 waves
+constexpr int NWAVES = BLOCK / WAVE;
+
+// Stencil weights by offset (ops.rs:908-920 extraction, last match wins).
+struct W9 {
+	double c, n, s, w, e, nw, ne, sw, se;
+};
+// Generic radius-1 stencil in entry order (ops.rs:872-883 loop order).
+struct Ent {
+	int n;
+	int dr[32];
+	int dc[32];
+	double w[32];
+};
+
+// ops.rs:924-935 / 1093-1106: w_c·c + (w_n·n + (w_s·s + (w_w·w + w_e·e))).
+__device__ __forceinline__ double chain5(const W9& k, double c, double n, double s, double w, double e) {
+	return fma(k.c, c, fma(k.n, n, fma(k.s, s, fma(k.w, w, k.e * e))));
+}
+__device__ __forceinline__ float chain5f(const float* k, float c, float n, float s, float w, float e) {
+	return fmaf(k[0], c, fmaf(k[1], n, fmaf(k[2], s, fmaf(k[3], w, k[4] * e))));
+}
+// ops.rs:984-1007 / 1058-1083.
+__device__ __forceinline__ double chain9(const W9& k, const double* x, int64_t i, int64_t cols) {
+	return fma(k.c, x[i], fma(k.n, x[i - cols], fma(k.s, x[i + cols], fma(k.w, x[i - 1],
+	       fma(k.e, x[i + 1], fma(k.nw, x[i - cols - 1], fma(k.ne, x[i - cols + 1],
+	       fma(k.sw, x[i + cols - 1], k.se * x[i + cols + 1]))))))));
+}
+__device__ __forceinline__ double chain_generic(const Ent& es, const double* x, int64_t i, int64_t cols) {
+	double acc = 0.0;
+	for (int j = 0; j < es.n; ++j) acc = fma(es.w[j], x[i + es.dr[j] * cols + es.dc[j]], acc);
+	return acc;
+}
+
+// ── compensated (TwoSum) accumulation ────────────────────────────────
+// value = hi + lo.  Deterministic for a fixed launch shape.
+struct Acc {
+	double hi, lo;
+};
+__device__ __forceinline__ void acc_add(Acc& a, double x) {
+	double s = a.hi + x;
+	double bp = s - a.hi;
+	a.lo += (a.hi - (s - bp)) + (x - bp);
+	a.hi = s;
+}
+__device__ __forceinline__ void acc_merge(Acc& a, const Acc& b) {
+	double s = a.hi + b.hi;
+	double bp = s - a.hi;
+	a.lo = (a.lo + b.lo) + ((a.hi - (s - bp)) + (b.hi - bp));
+	a.hi = s;
+}
+__device__ __forceinline__ double acc_value(const Acc& a) { return a.hi + a.lo; }
+
+__device__ __forceinline__ int tid_linear() { return threadIdx.x + threadIdx.y * blockDim.x; }
+__device__ __forceinline__ int bid_linear() { return blockIdx.x + blockIdx.y * gridDim.x; }
+__device__ __forceinline__ int nblocks_linear() { return gridDim.x * gridDim.y; }
+
+template <int NS>
+__device__ __forceinline__ void wave_reduce(Acc (&v)[NS]) {
+#pragma unroll
+	for (int off = WAVE / 2; off > 0; off >>= 1) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) {
+			Acc o;
+			o.hi = __shfl_xor(v[k].hi, off);
+			o.lo = __shfl_xor(v[k].lo, off);
+			acc_merge(v[k], o);
+		}
+	}
+}
+
+// Block reduction over BLOCK threads; result valid in thread 0.
+template <int NS>
+__device__ __forceinline__ void block_reduce(Acc (&v)[NS]) {
+	__shared__ Acc sh[NS][NWAVES];
+	wave_reduce<NS>(v);
+	const int t = tid_linear();
+	if ((t & (WAVE - 1)) == 0) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) sh[k][t / WAVE] = v[k];
+	}
+	__syncthreads();
+	if (t == 0) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) {
+			Acc a = sh[k][0];
+			for (int w = 1; w < NWAVES; ++w) acc_merge(a, sh[k][w]);
+			v[k] = a;
+		}
+	}
+	__syncthreads();
+}
+
+// Reduction scratch handed to every reducing kernel.
+struct RedScratch {
+	Acc* partials;        // [nblocks * NS]
+	unsigned* counter;    // zero between kernels
+};
+
+// Writes this block's partial sums; the last block to arrive merges all
+// partials in block order and returns true in every one of its threads with
+// `out` valid in thread 0.  The counter is reset for the next kernel.
+template <int NS>
+__device__ bool block_partial_and_last(Acc (&v)[NS], const RedScratch& rs, Acc (&out)[NS]) {
+	__shared__ int is_last;
+	const int t = tid_linear();
+	const int nb = nblocks_linear();
+	block_reduce<NS>(v);
+	if (t == 0) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) rs.partials[bid_linear() * NS + k] = v[k];
+		__threadfence();
+		unsigned prev = atomicAdd(rs.counter, 1u);
+		is_last = (prev == (unsigned)(nb - 1));
+	}
+	__syncthreads();
+	if (!is_last) return false;
+	__threadfence();
+	Acc a[NS];
+#pragma unroll
+	for (int k = 0; k < NS; ++k) a[k] = Acc{0.0, 0.0};
+	for (int b = t; b < nb; b += BLOCK) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) {
+			Acc p;
+			p.hi = __builtin_nontemporal_load(&rs.partials[b * NS + k].hi);
+			p.lo = __builtin_nontemporal_load(&rs.partials[b * NS + k].lo);
+			acc_merge(a[k], p);
+		}
+	}
+	block_reduce<NS>(a);
+	if (t == 0) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) out[k] = a[k];
+		*rs.counter = 0u;
+	}
+	return true;
+}
+
+// ── streaming 5-point stencil ────────────────────────────────────────
+// One thread per column, marching down `rpb` rows and keeping the north /
+// centre / south values in registers; west/east come from the cache.  The
+// value source `V` and the epilogue `E` make the same traversal serve
+// stencil_axpy, A·v, residuals, smoothers and the fused CG updates.
+//   V::load(i)            -> value of the (possibly derived) input at i
+//   E::interior(i, lap, c, acc)   E::boundary(i, c, acc)
+//   F::operator()(Acc*)   runs once, in the last block, on the global sums
+struct NoFinal {
+	template <class A>
+	__device__ void operator()(A*) const {}
+};
+
+template <int NS, class V, class E, class F>
+__global__ __launch_bounds__(BLOCK) void k_stream5(V v, E e, F fin, W9 w, int64_t rows, int64_t cols, int rpb,
+                                                   RedScratch rs, const int* done) {
+	if (done && *done) return;
+	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool col_ok = c < cols;
+	const bool col_int = c > 0 && c < cols - 1;
+	Acc acc[NS > 0 ? NS : 1];
+#pragma unroll
+	for (int k = 0; k < (NS > 0 ? NS : 1); ++k) acc[k] = Acc{0.0, 0.0};
+	double up = 0.0, cur = 0.0;
+	if (col_ok) {
+		if (r0 > 0) up = v.load((r0 - 1) * cols + c);
+		cur = v.load(r0 * cols + c);
+	}
+	for (int64_t r = r0; r < r1; ++r) {
+		const int64_t i = r * cols + c;
+		double dn = 0.0;
+		if (col_ok && r + 1 < rows) dn = v.load(i + cols);
+		if (col_ok) {
+			if (col_int && r > 0 && r < rows - 1) {
+				const double wv = v.load(i - 1), ev = v.load(i + 1);
+				e.interior(i, chain5(w, cur, up, dn, wv, ev), cur, acc);
+			} else {
+				e.boundary(i, cur, acc);
+			}
+		}
+		up = cur;
+		cur = dn;
+	}
+	if constexpr (NS > 0) {
+		Acc out[NS];
+		if (block_partial_and_last<NS>(acc, rs, out)) {
+			if (tid_linear() == 0) fin(out);
+		}
+	}
+}
+
+// Rows per block for the streaming kernels: enough blocks to fill 256 CUs
+// several times over, long enough columns to amortise the two halo rows.
+inline int stream_rpb(int64_t rows, int64_t cols) {
+	int64_t cb = (cols + BLOCK - 1) / BLOCK;
+	int rpb = 32;
+	while (rpb > 4 && cb * ((rows + rpb - 1) / rpb) < 2048) rpb /= 2;
+	return rpb;
+}
+
+}  // namespace fxd

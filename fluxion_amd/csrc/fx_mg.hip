@@ -1,0 +1,441 @@
+// fx_mg.hip — geometric multigrid V-cycle (crates/runtime/src/multigrid.rs:12-608).
+//
+// Levels are FP64 (reference) or FP32 (mixed precision).  The per-point
+// arithmetic of every kernel is the same as the oracle's Mg<T> so that FP64
+// and FP32 V-cycles are reproducible on the CPU:
+//   hierarchy          multigrid.rs:92-146 (coarsest threshold is a parameter)
+//   V-cycle order      multigrid.rs:188-230
+//   residual           multigrid.rs:476-491 (apply_stencil, scale −1, axpy rhs)
+//   2x2 restriction    ops.rs:369-399, injection prolongation ops.rs:401-423
+//   Chebyshev          multigrid.rs:317-473
+// The smoother is a two-buffer weighted Jacobi sweep with the ops.rs:454-469
+// point update; the reference CPU sweep is in place (lexicographic).
+#include "fx_mg.hpp"
+
+#include <cmath>
+#include <utility>
+
+using namespace fxd;
+
+namespace {
+
+template <class T> __device__ __forceinline__ T fmaT(T a, T b, T c);
+template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return fma(a, b, c); }
+template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <class T> struct LW { T w[5]; T inv_diag; T omega; };
+
+#define SKIP_RETURN if (skip && *skip) return
+#define PT_INDEX                                                           \
+	int64_t c = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));  \
+	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);      \
+	if (r >= rows || c >= cols) return;                                    \
+	int64_t i = r * cols + c;                                              \
+	bool bnd = r == 0 || c == 0 || r == rows - 1 || c == cols - 1;
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cvt_in(const double* b, const double* x, T* rhs, T* u, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) { rhs[i] = T(b[i]); u[i] = T(x[i]); }
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_cvt_out(const T* u, double* x, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) x[i] = double(u[i]);
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_fillT(T* x, T v, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_axpyT(T a, const T* x, T* y, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) y[i] = fmaT<T>(a, x[i], y[i]);
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_scaleT(T* x, T a, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) x[i] *= a;
+}
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_copyT(const T* s, T* d, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) d[i] = s[i];
+}
+template <class T>
+__global__ void k_zero_bndT(T* x, int64_t rows, int64_t cols, const int* skip) {
+	SKIP_RETURN;
+	int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < cols) { x[t] = T(0); x[(rows - 1) * cols + t] = T(0); }
+	else if (t < cols + rows - 2) { int64_t rr = 1 + (t - cols); x[rr * cols] = T(0); x[rr * cols + cols - 1] = T(0); }
+}
+
+// Two-buffer damped Jacobi: ax accumulated over entries (0,0),(−1,0),(1,0),(0,−1),(0,1).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_jac(const T* xo, T* xn, const T* b, LW<T> L, int64_t rows, int64_t cols,
+                                               const int* skip) {
+	SKIP_RETURN;
+	PT_INDEX;
+	if (bnd) { xn[i] = xo[i]; return; }
+	T ax = T(0);
+	ax = fmaT<T>(L.w[0], xo[i], ax);
+	ax = fmaT<T>(L.w[1], xo[i - cols], ax);
+	ax = fmaT<T>(L.w[2], xo[i + cols], ax);
+	ax = fmaT<T>(L.w[3], xo[i - 1], ax);
+	ax = fmaT<T>(L.w[4], xo[i + 1], ax);
+	T res = b[i] - ax;
+	xn[i] = fmaT<T>(L.omega, L.inv_diag * res, xo[i]);
+}
+
+__device__ double bcv(fx_bc bc, double self_val, double nb, double spacing) {
+	switch (bc.kind) {
+	case FX_BC_DIRICHLET: return bc.a;
+	case FX_BC_NEUMANN: return fma(spacing, bc.a, nb);
+	case FX_BC_ROBIN:
+		if (fabs(bc.b) < 1e-30) return fabs(bc.a) < 1e-30 ? 0.0 : bc.g / bc.a;
+		return fma(spacing, (bc.g - bc.a * self_val) / bc.b, nb);
+	default: return 0.0;
+	}
+}
+
+// Level stencil (ops.rs:891-938 chain) with bc_value boundaries (ops.rs:799-843).
+// mode 0: out = S(u); mode 1: out = rhs − S(u) (multigrid.rs:476-491).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_lstencil(const T* u, const T* rhs, T* out, LW<T> L, fx_boundaries bc,
+                                                    double dx, double dy, int mode, int64_t rows, int64_t cols,
+                                                    const int* skip) {
+	SKIP_RETURN;
+	PT_INDEX;
+	T s;
+	if (bnd) {
+		if (r == 0) s = T(bcv(bc.top, double(u[i]), double(u[cols + c]), dy));
+		else if (r == rows - 1) s = T(bcv(bc.bottom, double(u[i]), double(u[i - cols]), dy));
+		else if (c == 0) s = T(bcv(bc.left, double(u[i]), double(u[i + 1]), dx));
+		else s = T(bcv(bc.right, double(u[i]), double(u[i - 1]), dx));
+	} else {
+		s = fmaT<T>(L.w[0], u[i], fmaT<T>(L.w[1], u[i - cols], fmaT<T>(L.w[2], u[i + cols],
+		    fmaT<T>(L.w[3], u[i - 1], L.w[4] * u[i + 1]))));
+	}
+	out[i] = mode == 0 ? s : fmaT<T>(T(1), rhs[i], -s);
+}
+
+// Linear transfer between level grids spanning the same domain: fine index i
+// sits at coarse coordinate i·(nc−1)/(nf−1) = I0 + w (exact integer split).
+struct Lin1 {
+	int64_t i0;
+	double w;
+};
+__device__ __forceinline__ Lin1 lin1(int64_t i, int64_t nf, int64_t nc) {
+	int64_t num = i * (nc - 1);
+	int64_t den = nf - 1;
+	return {num / den, double(num % den) * (1.0 / double(den))};
+}
+__device__ __forceinline__ double lin_weight(int64_t i, int64_t I, int64_t nf, int64_t nc) {
+	Lin1 l = lin1(i, nf, nc);
+	if (l.i0 == I) return 1.0 - l.w;
+	if (l.i0 + 1 == I) return l.w;
+	return 0.0;
+}
+__device__ __forceinline__ void lin_range(int64_t I, int64_t nf, int64_t nc, int64_t* lo, int64_t* hi) {
+	int64_t den = nc - 1;
+	int64_t a = (I - 1) * (nf - 1);
+	*lo = a <= 0 ? 1 : (a + den - 1) / den;
+	*hi = ((I + 1) * (nf - 1)) / den;
+	if (*lo < 1) *lo = 1;
+	if (*hi > nf - 2) *hi = nf - 2;
+}
+
+// Restriction: reference 2x2 average (ops.rs:369-399) or the transpose of the
+// linear prolongation scaled by (h/H)² (zero on the coarse boundary).
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_restrict(const T* f, T* co, int64_t fr, int64_t fc, int64_t cr, int64_t cc,
+                                                    int lin, const int* skip) {
+	SKIP_RETURN;
+	int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (r >= cr || col >= cc) return;
+	if (!lin) {
+		int64_t a = 2 * r, b = 2 * col;
+		int64_t a1 = a + 1 < fr - 1 ? a + 1 : fr - 1, b1 = b + 1 < fc - 1 ? b + 1 : fc - 1;
+		co[r * cc + col] = T(0.25) * (f[a * fc + b] + f[a1 * fc + b] + f[a * fc + b1] + f[a1 * fc + b1]);
+		return;
+	}
+	if (r == 0 || col == 0 || r == cr - 1 || col == cc - 1) { co[r * cc + col] = T(0); return; }
+	const T sy = T(double(cr - 1) / double(fr - 1)), sx = T(double(cc - 1) / double(fc - 1));
+	int64_t rlo, rhi, clo, chi;
+	lin_range(r, fr, cr, &rlo, &rhi);
+	lin_range(col, fc, cc, &clo, &chi);
+	T acc = T(0);
+	for (int64_t i = rlo; i <= rhi; ++i) {
+		T wy = T(lin_weight(i, r, fr, cr));
+		T row = T(0);
+		for (int64_t j = clo; j <= chi; ++j) row += T(lin_weight(j, col, fc, cc)) * f[i * fc + j];
+		acc += wy * row;
+	}
+	co[r * cc + col] = acc * (sy * sx);
+}
+
+// Prolongation (additive): reference injection over the whole grid
+// (ops.rs:401-423, coarse index clamped for odd sizes) or linear
+// interpolation onto the fine interior.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_prolong(const T* co, T* f, int64_t cr, int64_t cc, int64_t fr, int64_t fc,
+                                                   int lin, const int* skip) {
+	SKIP_RETURN;
+	int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (r >= fr || col >= fc) return;
+	if (!lin) {
+		int64_t a = r / 2 < cr - 1 ? r / 2 : cr - 1, b = col / 2 < cc - 1 ? col / 2 : cc - 1;
+		f[r * fc + col] += co[a * cc + b];
+		return;
+	}
+	if (r == 0 || col == 0 || r == fr - 1 || col == fc - 1) return;
+	Lin1 ly = lin1(r, fr, cr), lx = lin1(col, fc, cc);
+	T wy = T(ly.w), vy = T(1) - wy, wx = T(lx.w), vx = T(1) - wx;
+	int64_t a0 = ly.i0, a1 = ly.i0 + 1 < cr ? ly.i0 + 1 : ly.i0;
+	int64_t b0 = lx.i0, b1 = lx.i0 + 1 < cc ? lx.i0 + 1 : lx.i0;
+	T top = vx * co[a0 * cc + b0] + wx * co[a0 * cc + b1];
+	T bot = vx * co[a1 * cc + b0] + wx * co[a1 * cc + b1];
+	f[r * fc + col] += vy * top + wy * bot;
+}
+
+dim3 pgrid(int64_t rows, int64_t cols) {
+	return dim3((unsigned)((cols + WAVE - 1) / WAVE), (unsigned)((rows + NWAVES - 1) / NWAVES));
+}
+
+template <class T>
+struct Driver {
+	fx_multigrid* mg;
+	const int* skip;
+	hipStream_t st;
+
+	LW<T> lw(const fx_multigrid::Level& L) const {
+		LW<T> o;
+		for (int e = 0; e < 5; ++e) o.w[e] = T(L.w[e]);
+		o.inv_diag = T(L.inv_diag);
+		o.omega = T(mg->opt.omega);
+		return o;
+	}
+	size_t n(const fx_multigrid::Level& L) const { return (size_t)L.g.rows * L.g.cols; }
+
+	fx_status smooth(int l, int count) {
+		if (count <= 0) return FX_OK;
+		fx_multigrid::Level& L = mg->lv[l];
+		if (mg->opt.smoother == FX_SMOOTH_CHEBYSHEV) return cheby(l, count);
+		LW<T> w = lw(L);
+		for (int it = 0; it < count; ++it) {
+			hipLaunchKernelGGL(k_jac<T>, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, st, (const T*)L.u, (T*)L.tmp,
+			                   (const T*)L.rhs, w, (int64_t)L.g.rows, (int64_t)L.g.cols, skip);
+			std::swap(L.u, L.tmp);
+		}
+		return fx::check_launch("mg jacobi");
+	}
+
+	fx_status lstencil(int l, const T* u, T* out, int mode) {
+		fx_multigrid::Level& L = mg->lv[l];
+		hipLaunchKernelGGL(k_lstencil<T>, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, st, u, (const T*)L.rhs, out, lw(L),
+		                   mg->bc, L.g.dx, L.g.dy, mode, (int64_t)L.g.rows, (int64_t)L.g.cols, skip);
+		return fx::check_launch("mg level stencil");
+	}
+
+	unsigned eb(size_t nn) const { return fx::elem_blocks(nn); }
+
+	// multigrid.rs:317-473.
+	fx_status cheby(int l, int count) {
+		fx_multigrid::Level& L = mg->lv[l];
+		const size_t N = n(L);
+		const int64_t rows = L.g.rows, cols = L.g.cols;
+		const double emax = 2.0, emin = 0.1 * emax;
+		const double scl = 2.0 / (emax + emin), alpha = 1.0 - scl * emin, mu = 1.0 / alpha, omegaprod = 2.0 / alpha;
+		const double ix2 = 1.0 / (L.g.dx * L.g.dx), iy2 = 1.0 / (L.g.dy * L.g.dy);
+		const double inv_diag = -1.0 / (2.0 * ix2 + 2.0 * iy2);
+		unsigned zb = (unsigned)((cols + rows - 2 + 255) / 256);
+		auto axpy = [&](T a, const void* x, void* y) {
+			hipLaunchKernelGGL(k_axpyT<T>, dim3(eb(N)), dim3(BLOCK), 0, st, a, (const T*)x, (T*)y, N, skip);
+		};
+		auto scale = [&](void* x, T a) { hipLaunchKernelGGL(k_scaleT<T>, dim3(eb(N)), dim3(BLOCK), 0, st, (T*)x, a, N, skip); };
+		auto copy = [&](const void* s, void* d) {
+			hipLaunchKernelGGL(k_copyT<T>, dim3(eb(N)), dim3(BLOCK), 0, st, (const T*)s, (T*)d, N, skip);
+		};
+		auto zero_bnd = [&](void* x) { hipLaunchKernelGGL(k_zero_bndT<T>, dim3(zb), dim3(256), 0, st, (T*)x, rows, cols, skip); };
+		double c_km1 = 1.0, c_k = mu;
+		FX_CHECK(lstencil(l, (const T*)L.u, (T*)L.tmp, 0));
+		copy(L.rhs, L.w0);
+		axpy(T(-1), L.tmp, L.w0);
+		scale(L.w0, T(scl * inv_diag));
+		axpy(T(1), L.u, L.w0);
+		zero_bnd(L.w0);
+		if (count == 1) { copy(L.w0, L.u); return fx::check_launch("mg chebyshev"); }
+		void** B[3] = {&L.u, &L.w0, &L.w1};
+		int km1 = 0, k = 1, kp1 = 2;
+		for (int i = 1; i < count; ++i) {
+			FX_CHECK(lstencil(l, (const T*)*B[k], (T*)L.tmp, 0));
+			void* p = *B[kp1];
+			copy(L.rhs, p);
+			axpy(T(-1), L.tmp, p);
+			scale(p, T(inv_diag));
+			double c_kp1 = 2.0 * mu * c_k - c_km1;
+			double om = omegaprod * c_k / c_kp1;
+			scale(p, T(om * scl));
+			axpy(T(om), *B[k], p);
+			axpy(T(1.0 - om), *B[km1], p);
+			zero_bnd(p);
+			c_km1 = c_k;
+			c_k = c_kp1;
+			int t = km1; km1 = k; k = kp1; kp1 = t;
+		}
+		if (k != 0) copy(*B[k], L.u);
+		return fx::check_launch("mg chebyshev");
+	}
+
+	fx_status run(double* z, const double* r) {
+		auto& lv = mg->lv;
+		const int nl = (int)lv.size();
+		const auto& o = mg->opt;
+		const int fw = o.transfer == FX_TRANSFER_FW_BILINEAR;  // linear same-domain transfer
+		size_t n0 = n(lv[0]);
+		hipLaunchKernelGGL(k_cvt_in<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, r, (const double*)z, (T*)lv[0].rhs, (T*)lv[0].u,
+		                   n0, skip);
+		for (int l = 0; l < nl - 1; ++l) {
+			FX_CHECK(smooth(l, o.n_pre));
+			FX_CHECK(lstencil(l, (const T*)lv[l].u, (T*)lv[l].tmp, 1));
+			const fx_grid& fg = lv[l].g;
+			const fx_grid& cg = lv[l + 1].g;
+			hipLaunchKernelGGL(k_restrict<T>, pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st, (const T*)lv[l].tmp,
+			                   (T*)lv[l + 1].rhs, (int64_t)fg.rows, (int64_t)fg.cols, (int64_t)cg.rows, (int64_t)cg.cols,
+			                   fw, skip);
+			size_t nc = n(lv[l + 1]);
+			hipLaunchKernelGGL(k_fillT<T>, dim3(eb(nc)), dim3(BLOCK), 0, st, (T*)lv[l + 1].u, T(0), nc, skip);
+		}
+		FX_CHECK(smooth(nl - 1, o.n_pre + o.n_post + o.extra_coarse));
+		for (int l = nl - 2; l >= 0; --l) {
+			const fx_grid& fg = lv[l].g;
+			const fx_grid& cg = lv[l + 1].g;
+			hipLaunchKernelGGL(k_prolong<T>, pgrid(fg.rows, fg.cols), dim3(BLOCK), 0, st, (const T*)lv[l + 1].u, (T*)lv[l].u,
+			                   (int64_t)cg.rows, (int64_t)cg.cols, (int64_t)fg.rows, (int64_t)fg.cols, fw, skip);
+			FX_CHECK(smooth(l, o.n_post));
+		}
+		hipLaunchKernelGGL(k_cvt_out<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, (const T*)lv[0].u, z, n0, skip);
+		return fx::check_launch("mg vcycle");
+	}
+};
+
+void free_levels(fx_multigrid* mg) {
+	for (auto& L : mg->lv) {
+		fx::dev_free(L.u); fx::dev_free(L.rhs); fx::dev_free(L.tmp); fx::dev_free(L.w0); fx::dev_free(L.w1);
+	}
+	mg->lv.clear();
+}
+
+}  // namespace
+
+namespace fx {
+fx_status mg_apply(fx_multigrid* mg, double* z, const double* r, const int* skip) {
+	if (mg->f32) return Driver<float>{mg, skip, mg->be->stream}.run(z, r);
+	return Driver<double>{mg, skip, mg->be->stream}.run(z, r);
+}
+}  // namespace fx
+
+using namespace fx;
+
+extern "C" {
+
+fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_boundaries* bc, const fx_mg_options* opt,
+                                 fx_multigrid** out) {
+	FX_REQUIRE(be && out && opt, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(check_grid(grid));
+	FX_CHECK(check_bc(bc));
+	FX_REQUIRE(opt->precision == FX_F32 || opt->precision == FX_F64, FX_ERR_ARGUMENT, "bad precision");
+	FX_REQUIRE(opt->smoother == FX_SMOOTH_JACOBI || opt->smoother == FX_SMOOTH_CHEBYSHEV, FX_ERR_ARGUMENT, "bad smoother");
+	FX_REQUIRE(opt->transfer == FX_TRANSFER_REFERENCE || opt->transfer == FX_TRANSFER_FW_BILINEAR, FX_ERR_ARGUMENT,
+	           "bad transfer");
+	FX_REQUIRE(opt->k == nullptr, FX_ERR_UNSUPPORTED, "variable-coefficient levels are built by the batched solver");
+	FX_REQUIRE(opt->fused == 0, FX_ERR_UNSUPPORTED, "fused V-cycle is selected through the solver");
+	fx_multigrid* mg = new fx_multigrid();
+	mg->be = be;
+	mg->opt = *opt;
+	mg->bc = *bc;
+	mg->f32 = opt->precision == FX_F32;
+	const size_t es = mg->f32 ? 4 : 8;
+	const uint64_t lim = opt->coarsest > 0 ? (uint64_t)opt->coarsest : 4;
+	fx_grid g = *grid;
+	for (;;) {
+		fx_multigrid::Level L;
+		L.g = g;
+		double ix2 = 1.0 / (g.dx * g.dx), iy2 = 1.0 / (g.dy * g.dy);
+		L.w[0] = opt->sigma + opt->scale * (-2.0 * (ix2 + iy2));
+		L.w[1] = opt->scale * iy2;
+		L.w[2] = opt->scale * iy2;
+		L.w[3] = opt->scale * ix2;
+		L.w[4] = opt->scale * ix2;
+		L.inv_diag = 1.0 / L.w[0];
+		size_t bytes = (size_t)g.rows * g.cols * es;
+		fx_status s = FX_OK;
+		void** bufs[5] = {&L.u, &L.rhs, &L.tmp, &L.w0, &L.w1};
+		int nb = opt->smoother == FX_SMOOTH_CHEBYSHEV ? 5 : 3;
+		for (int i = 0; i < nb && s == FX_OK; ++i) s = dev_alloc(be, bytes, bufs[i]);
+		mg->lv.push_back(L);
+		if (s != FX_OK) { free_levels(mg); delete mg; return s; }
+		if (g.rows <= lim || g.cols <= lim) break;
+		if (opt->transfer == FX_TRANSFER_REFERENCE) {
+			// multigrid.rs:121-135.
+			uint64_t cr = g.rows / 2, cc = g.cols / 2;
+			if (cr < 3 || cc < 3) break;
+			g = fx_grid{cr, cc, g.dx * 2.0, g.dy * 2.0};
+		} else {
+			// Every level spans the fine domain: odd sizes nest ((n+1)/2), even sizes halve.
+			uint64_t cr = g.rows % 2 ? (g.rows + 1) / 2 : g.rows / 2;
+			uint64_t cc = g.cols % 2 ? (g.cols + 1) / 2 : g.cols / 2;
+			if (cr < 3 || cc < 3) break;
+			g = fx_grid{cr, cc, g.dx * double(g.cols - 1) / double(cc - 1), g.dy * double(g.rows - 1) / double(cr - 1)};
+		}
+	}
+	*out = mg;
+	return FX_OK;
+}
+
+fx_status fx_multigrid_create(fx_backend* be, const fx_grid* grid, const fx_boundaries* bc, uint32_t n_pre,
+                              uint32_t n_post, double omega, int32_t smoother, fx_multigrid** out) {
+	fx_mg_options o{};
+	o.n_pre = (int32_t)n_pre;
+	o.n_post = (int32_t)n_post;
+	o.omega = omega;
+	o.smoother = smoother;
+	o.transfer = FX_TRANSFER_REFERENCE;
+	o.coarsest = 4;
+	o.extra_coarse = 10;
+	o.precision = FX_F64;
+	o.fused = 0;
+	o.sigma = 0.0;
+	o.scale = 1.0;
+	o.k = nullptr;
+	return fx_multigrid_create_ex(be, grid, bc, &o, out);
+}
+
+void fx_multigrid_destroy(fx_multigrid* mg) {
+	if (!mg) return;
+	if (mg->be) (void)hipStreamSynchronize(mg->be->stream);
+	free_levels(mg);
+	delete mg;
+}
+
+uint32_t fx_multigrid_depth(const fx_multigrid* mg) { return mg ? (uint32_t)mg->lv.size() : 0; }
+
+fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b) {
+	FX_REQUIRE(mg, FX_ERR_ARGUMENT, "multigrid is null");
+	size_t n = mg->lv[0].g.rows * mg->lv[0].g.cols;
+	FX_CHECK(check_buf(x, n, FX_F64, "x"));
+	FX_CHECK(check_buf(b, n, FX_F64, "b"));
+	return mg_apply(mg, (double*)x->ptr, (const double*)b->ptr, nullptr);
+}
+
+}  // extern "C"

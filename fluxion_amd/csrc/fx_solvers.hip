@@ -1,0 +1,1238 @@
+// fx_solvers.hip — Krylov solvers on the device.
+//
+// The reference drives every solver from the host: one backend call per
+// vector operation and a blocking device-to-host copy per dot product
+// (crates/runtime/src/cg.rs:118-157; backend-hip/src/backend.rs:356-413).
+// Here the recurrence scalars live in device memory.  Each reduction is
+// finished by the last workgroup of the kernel that produced its terms, which
+// then evaluates the solver's scalar update (alpha, beta, convergence and
+// breakdown tests, iteration count) on the device.  The host only enqueues
+// kernels and polls a `done` flag a few iterations behind; kernels launched
+// after `done` is set return immediately.
+//
+// The recurrences, their operation order and their stopping rules follow
+//   CgSolver::solve / solve_preconditioned       cg.rs:96-256
+//   PipelinedCgSolver::solve                     pipelined_cg.rs:96-200
+//   BiCgStabSolver::solve / solve_preconditioned bicgstab.rs:90-346
+//   GmresSolver::solve / solve_preconditioned    gmres.rs:116-490
+#include "fx_mg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace fxd;
+
+namespace {
+
+constexpr int HIST_MAX = 1 << 16;
+constexpr int GM_MAX = 128;   // largest GMRES restart supported
+
+// Solver scalars in device memory.
+struct SState {
+	double tol;
+	double rs;          // CG: r·r; PCG/FCG: r·z; BiCGSTAB: rho
+	double rr;          // r·r
+	double alpha, beta, pap;
+	double gprev, aprev;     // pipelined CG: gamma_prev, alpha_prev
+	double omega, rho_new, snorm;
+	double resid;
+	int64_t iter, max_iter;
+	int32_t done, reason;
+	int32_t pend;        // BiCGSTAB: finish after the x correction
+	int32_t kcur;        // GMRES: Arnoldi steps in the current cycle
+	int32_t cycle_end;   // GMRES: cycle finished
+	int32_t skip;        // done | pend | cycle_end: vector kernels return early
+	int32_t gexit;       // GMRES: 1 = breakdown exit
+	double* hist;
+	int64_t hist_len;
+	// GMRES (H column-major, (m+1) x m; gmres.rs:84-99)
+	int32_t m;
+	int32_t pad2;
+	double gbeta, gcur, ghk;
+	double *H, *cc, *ss, *grs, *y;
+};
+
+__device__ __forceinline__ void record(SState* s, int64_t k, double v) {
+	if (s->hist && k >= 0 && k < s->hist_len) s->hist[k] = v;
+}
+__device__ __forceinline__ void finish(SState* s, int64_t its, double res, int reason) {
+	s->done = 1;
+	s->skip = 1;
+	s->reason = reason;
+	s->iter = its;
+	s->resid = res;
+}
+
+// Element-wise update + reduction; F::load(st) caches scalars, F::operator()(i, acc).
+template <int NS, class F, class Fin>
+__global__ __launch_bounds__(BLOCK) void k_ered(F f, Fin fin, size_t n, RedScratch rs, const int* skip) {
+	if (skip && *skip) return;
+	f.load();
+	Acc acc[NS];
+#pragma unroll
+	for (int k = 0; k < NS; ++k) acc[k] = Acc{0.0, 0.0};
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) f(i, acc);
+	Acc out[NS];
+	if (block_partial_and_last<NS>(acc, rs, out) && tid_linear() == 0) fin(out);
+}
+template <class F>
+__global__ __launch_bounds__(BLOCK) void k_eupd(F f, size_t n, const int* skip) {
+	if (skip && *skip) return;
+	f.load();
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) f(i);
+}
+template <class Fin>
+__global__ void k_scalar(Fin fin, const int* skip) {
+	if (skip && *skip) return;
+	fin();
+}
+
+// ── element-wise functors ──────────────────────────────────────────────
+// r = b − ap (copy + axpy(−1): cg.rs:110-111), with r·r.
+struct FResid {
+	const double *b, *ap;
+	double* r;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		double v = b[i] - ap[i];
+		r[i] = v;
+		acc_add(a[0], v * v);
+	}
+};
+// x += α·p; r −= α·ap (cg.rs:145-147), with r·r.
+struct FXR {
+	const SState* st;
+	double* x;
+	double* r;
+	const double *p, *ap;
+	double al;
+	__device__ void load() { al = st->alpha; }
+	__device__ void operator()(size_t i, Acc* a) const {
+		x[i] = fma(al, p[i], x[i]);
+		double v = fma(-al, ap[i], r[i]);
+		r[i] = v;
+		acc_add(a[0], v * v);
+	}
+};
+struct FDot {
+	const double *x, *y;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const { acc_add(a[0], x[i] * y[i]); }
+};
+struct FDot2 {
+	const double *a0, *b0, *a1, *b1;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		acc_add(a[0], a0[i] * b0[i]);
+		acc_add(a[1], a1[i] * b1[i]);
+	}
+};
+// p_new = src + β·p_old (scale + axpy(1, src): cg.rs:153-154), written to p_out.
+struct FPUpd {
+	const SState* st;
+	const double *src, *pold;
+	double* pout;
+	double be;
+	__device__ void load() { be = st->beta; }
+	__device__ void operator()(size_t i) const { pout[i] = src[i] + be * pold[i]; }
+};
+struct FCopy {
+	const double* s;
+	double* d;
+	__device__ void load() {}
+	__device__ void operator()(size_t i) const { d[i] = s[i]; }
+};
+struct FZero {
+	double* d;
+	__device__ void load() {}
+	__device__ void operator()(size_t i) const { d[i] = 0.0; }
+};
+
+// ── streaming stencil pieces for the fused CG step ─────────────────────
+// Value source p_new = src + β·p_old, recomputed at every neighbour.
+struct VPNew {
+	const double *src, *pold;
+	double be;
+	__device__ double load(int64_t i) const { return src[i] + be * pold[i]; }
+};
+// ap = fma(coeff, lap, p) inside, ap = p on the boundary; writes p_new; p·ap.
+struct EpiCG {
+	double coeff;
+	double *pout, *ap;
+	__device__ void interior(int64_t i, double lap, double c, Acc* a) const {
+		double v = fma(coeff, lap, c);
+		pout[i] = c;
+		ap[i] = v;
+		acc_add(a[0], c * v);
+	}
+	__device__ void boundary(int64_t i, double c, Acc* a) const {
+		pout[i] = c;
+		ap[i] = c;
+		acc_add(a[0], c * c);
+	}
+};
+
+template <int NS, class V, class E, class F>
+__global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, F fin, W9 w, int64_t rows, int64_t cols, int rpb,
+                                                     RedScratch rs, const SState* st) {
+	// Same traversal as fxd::k_stream5, reading β from the device state first.
+	if (st->done) return;
+	v.be = st->beta;
+	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool col_ok = c < cols;
+	const bool col_int = c > 0 && c < cols - 1;
+	Acc acc[NS];
+#pragma unroll
+	for (int k = 0; k < NS; ++k) acc[k] = Acc{0.0, 0.0};
+	double up = 0.0, cur = 0.0;
+	if (col_ok) {
+		if (r0 > 0) up = v.load((r0 - 1) * cols + c);
+		cur = v.load(r0 * cols + c);
+	}
+	for (int64_t r = r0; r < r1; ++r) {
+		const int64_t i = r * cols + c;
+		double dn = 0.0;
+		if (col_ok && r + 1 < rows) dn = v.load(i + cols);
+		if (col_ok) {
+			if (col_int && r > 0 && r < rows - 1) {
+				const double wv = v.load(i - 1), ev = v.load(i + 1);
+				e.interior(i, chain5(w, cur, up, dn, wv, ev), cur, acc);
+			} else {
+				e.boundary(i, cur, acc);
+			}
+		}
+		up = cur;
+		cur = dn;
+	}
+	Acc out[NS];
+	if (block_partial_and_last<NS>(acc, rs, out)) {
+		if (tid_linear() == 0) fin(out);
+	}
+}
+
+// ── scalar epilogues ───────────────────────────────────────────────────
+// cg.rs:113-125: rs_old = r·r; converged at iteration 0 if ‖r‖ < tol.
+struct FinCGInit {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double rs = acc_value(o[0]);
+		st->rs = rs;
+		st->rr = rs;
+		st->beta = 0.0;
+		st->iter = 0;
+		record(st, 0, sqrt(rs));
+		if (st->max_iter == 0) finish(st, 0, sqrt(rs), FX_MAX_ITERATIONS);
+		else if (sqrt(rs) < st->tol) finish(st, 0, sqrt(rs), FX_CONVERGED);
+	}
+};
+// cg.rs:134-142 (CG) / 223-231 (PCG): α = rs/(p·Ap), breakdown below 1e−30.
+struct FinAlpha {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double pap = acc_value(o[0]);
+		st->pap = pap;
+		if (fabs(pap) < 1e-30) finish(st, st->iter, sqrt(st->rr), FX_BREAKDOWN);
+		else st->alpha = st->rs / pap;
+	}
+};
+// cg.rs:149-156 + the check at the top of the next iteration (cg.rs:118-125).
+struct FinCGBeta {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double rs_new = acc_value(o[0]);
+		st->beta = rs_new / st->rs;
+		st->rs = rs_new;
+		st->rr = rs_new;
+		int64_t it = st->iter + 1;
+		st->iter = it;
+		record(st, it, sqrt(rs_new));
+		if (it >= st->max_iter) finish(st, it, sqrt(rs_new), FX_MAX_ITERATIONS);
+		else if (sqrt(rs_new) < st->tol) finish(st, it, sqrt(rs_new), FX_CONVERGED);
+	}
+};
+// PCG: r·r after the x/r update (cg.rs:205-211 of the next iteration).
+struct FinPcgRR {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double rr = acc_value(o[0]);
+		st->rr = rr;
+		int64_t it = st->iter + 1;
+		st->iter = it;
+		record(st, it, sqrt(rr));
+		if (it >= st->max_iter) finish(st, it, sqrt(rr), FX_MAX_ITERATIONS);
+		else if (sqrt(rr) < st->tol) finish(st, it, sqrt(rr), FX_CONVERGED);
+	}
+};
+// PCG init: r = b − Ax with r·r (no check yet: cg.rs:188-202 computes z first).
+struct FinPcgInitRR {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		st->rr = acc_value(o[0]);
+		st->beta = 0.0;
+		st->iter = 0;
+	}
+};
+// PCG: rz_old = r·z at start, check ‖r‖ (cg.rs:202-211).
+struct FinPcgInitRZ {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		st->rs = acc_value(o[0]);
+		double rn = sqrt(st->rr);
+		record(st, 0, rn);
+		if (st->max_iter == 0) finish(st, 0, rn, FX_MAX_ITERATIONS);
+		else if (rn < st->tol) finish(st, 0, rn, FX_CONVERGED);
+	}
+};
+// PCG: β = rz_new/rz_old (cg.rs:240-245).
+struct FinPcgBeta {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double rz = acc_value(o[0]);
+		st->beta = rz / st->rs;
+		st->rs = rz;
+	}
+};
+// FCG: (r·z, z·Ap) -> α numerator and the flexible β = −(z·Ap)/(p·Ap).
+struct FinFcgBeta {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		st->rs = acc_value(o[0]);
+		st->beta = -acc_value(o[1]) / st->pap;
+	}
+};
+
+// ── pipelined CG (pipelined_cg.rs:96-200) ──────────────────────────────
+// Fused 6-way update in the CPU order (pipelined_cg.rs:182-199) followed by
+// the next iteration's (γ, δ) = (r·r, r·w).
+struct FPipeUpd {
+	const SState* st;
+	double *z, *t, *p, *x, *r, *w;
+	const double* q;
+	double a, b;
+	__device__ void load() { a = st->alpha; b = st->beta; }
+	__device__ void operator()(size_t i, Acc* acc) const {
+		double zi = q[i] + b * z[i];
+		double ti = w[i] + b * t[i];
+		double pi = r[i] + b * p[i];
+		x[i] = fma(a, pi, x[i]);
+		double ri = fma(-a, ti, r[i]);
+		double wi = fma(-a, zi, w[i]);
+		r[i] = ri;
+		w[i] = wi;
+		z[i] = zi;
+		t[i] = ti;
+		p[i] = pi;
+		acc_add(acc[0], ri * ri);
+		acc_add(acc[1], ri * wi);
+	}
+};
+struct FinPipe {
+	SState* st;
+	bool first;
+	__device__ void operator()(Acc* o) const {
+		double gamma = acc_value(o[0]), delta = acc_value(o[1]);
+		int64_t k = first ? 0 : st->iter + 1;
+		st->iter = k;
+		st->rr = gamma;
+		record(st, k, sqrt(gamma));
+		if (k >= st->max_iter) { finish(st, k, sqrt(gamma), FX_MAX_ITERATIONS); return; }
+		if (sqrt(gamma) < st->tol) { finish(st, k, sqrt(gamma), FX_CONVERGED); return; }
+		double gp = first ? INFINITY : st->gprev, ap = first ? INFINITY : st->aprev;
+		double beta = gamma / gp;
+		double alpha = gamma / (delta - beta * gamma / ap);
+		st->beta = beta;
+		st->alpha = alpha;
+		st->gprev = gamma;
+		st->aprev = alpha;
+	}
+};
+
+// ── preconditioned pipelined CG (Ghysels & Vanroose 2014, Alg. 3) ──────
+struct FPPUpd {
+	const SState* st;
+	double *z, *q, *s, *p, *x, *r, *u, *w;
+	const double *nn, *m;
+	double a, b;
+	__device__ void load() { a = st->alpha; b = st->beta; }
+	__device__ void operator()(size_t i, Acc* acc) const {
+		double zi = nn[i] + b * z[i];
+		double qi = m[i] + b * q[i];
+		double si = w[i] + b * s[i];
+		double pi = u[i] + b * p[i];
+		x[i] = fma(a, pi, x[i]);
+		double ri = fma(-a, si, r[i]);
+		double ui = fma(-a, qi, u[i]);
+		double wi = fma(-a, zi, w[i]);
+		z[i] = zi; q[i] = qi; s[i] = si; p[i] = pi;
+		r[i] = ri; u[i] = ui; w[i] = wi;
+		acc_add(acc[0], ri * ui);
+		acc_add(acc[1], wi * ui);
+		acc_add(acc[2], ri * ri);
+	}
+};
+struct FDot3 {
+	const double *r, *u, *w;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		acc_add(a[0], r[i] * u[i]);
+		acc_add(a[1], w[i] * u[i]);
+		acc_add(a[2], r[i] * r[i]);
+	}
+};
+// (γ, δ, ‖r‖²) -> check, then α, β (the new β/α are applied by the next update).
+struct FinPP {
+	SState* st;
+	bool first;
+	__device__ void operator()(Acc* o) const {
+		double gamma = acc_value(o[0]), delta = acc_value(o[1]), rr = acc_value(o[2]);
+		int64_t k = first ? 0 : st->iter + 1;
+		st->iter = k;
+		st->rr = rr;
+		record(st, k, sqrt(rr));
+		if (k >= st->max_iter) { finish(st, k, sqrt(rr), FX_MAX_ITERATIONS); return; }
+		if (sqrt(rr) < st->tol) { finish(st, k, sqrt(rr), FX_CONVERGED); return; }
+		double beta, alpha;
+		if (first) { beta = 0.0; alpha = gamma / delta; }
+		else { beta = gamma / st->gprev; alpha = gamma / (delta - beta * gamma / st->aprev); }
+		if (!isfinite(alpha)) { finish(st, k, sqrt(rr), FX_BREAKDOWN); return; }
+		st->beta = beta;
+		st->alpha = alpha;
+		st->gprev = gamma;
+		st->aprev = alpha;
+	}
+};
+
+// ── BiCGSTAB (bicgstab.rs:90-346) ──────────────────────────────────────
+// p = r + β·(p − ω·v)  (axpy(−ω, v), scale(β), axpy(1, r): bicgstab.rs:134-137)
+struct FBiP {
+	const SState* st;
+	double* p;
+	const double *r, *v;
+	double be, om;
+	__device__ void load() { be = st->beta; om = st->omega; }
+	__device__ void operator()(size_t i) const {
+		double t = fma(-om, v[i], p[i]);
+		p[i] = r[i] + be * t;
+	}
+};
+// s = r − α·v (copy + axpy: bicgstab.rs:156-158), with s·s.
+struct FBiS {
+	const SState* st;
+	double* s;
+	const double *r, *v;
+	double al;
+	__device__ void load() { al = st->alpha; }
+	__device__ void operator()(size_t i, Acc* a) const {
+		double t = fma(-al, v[i], r[i]);
+		s[i] = t;
+		acc_add(a[0], t * t);
+	}
+};
+// x += α·p̂ + ω·ŝ; r = s − ω·t (bicgstab.rs:190-196), then (r·r, r̂·r) for the next iteration.
+// While `pend` is set only x += α·p̂ is applied (bicgstab.rs:160-172, 180-188).
+struct FBiX {
+	const SState* st;
+	double *x, *r;
+	const double *pa, *sa, *s, *t, *rh;
+	double al, om;
+	int pend;
+	__device__ void load() { al = st->alpha; om = st->omega; pend = st->pend; }
+	__device__ void operator()(size_t i, Acc* a) const {
+		double xv = fma(al, pa[i], x[i]);
+		if (pend) { x[i] = xv; return; }
+		x[i] = fma(om, sa[i], xv);
+		double rv = fma(-om, t[i], s[i]);
+		r[i] = rv;
+		acc_add(a[0], rv * rv);
+		acc_add(a[1], rh[i] * rv);
+	}
+};
+// Top of iteration i (bicgstab.rs:115-132): ‖r‖ check, ρ_new = r̂·r, β.
+struct FinBiTop {
+	SState* st;
+	bool first;
+	__device__ void operator()(Acc* o) const {
+		if (!first && st->pend) {   // finish the early exit with the values recorded by FinBiS / FinBiOm
+			st->done = 1;
+			st->skip = 1;
+			return;
+		}
+		double rr = acc_value(o[0]), rho_new = acc_value(o[1]);
+		if (!first) st->rs = st->rho_new;   // rho = rho_new (bicgstab.rs:198)
+		int64_t i = first ? 0 : st->iter + 1;
+		st->iter = i;
+		st->rr = rr;
+		double rn = sqrt(rr);
+		record(st, i, rn);
+		if (i >= st->max_iter) { finish(st, i, rn, FX_MAX_ITERATIONS); return; }
+		if (rn < st->tol) { finish(st, i, rn, FX_CONVERGED); return; }
+		if (fabs(rho_new) < 1e-30) { finish(st, i, rn, FX_BREAKDOWN); return; }
+		st->rho_new = rho_new;
+		st->beta = (rho_new / st->rs) * (st->alpha / st->omega);
+	}
+};
+// α = ρ_new / (r̂·v) (bicgstab.rs:145-154).
+struct FinBiAlpha {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double rv = acc_value(o[0]);
+		if (fabs(rv) < 1e-30) { finish(st, st->iter, sqrt(st->rr), FX_BREAKDOWN); return; }
+		st->alpha = st->rho_new / rv;
+	}
+};
+// ‖s‖ < tol -> x += α·p̂ and stop (bicgstab.rs:160-172).
+struct FinBiS {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double sn = sqrt(acc_value(o[0]));
+		st->snorm = sn;
+		if (sn < st->tol) {
+			st->pend = 1;
+			st->skip = 1;
+			st->reason = FX_CONVERGED;
+			st->resid = sn;
+		}
+	}
+};
+// ω = (t·s)/(t·t); breakdown -> x += α·p̂ and stop (bicgstab.rs:178-189).
+struct FinBiOm {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double ts = acc_value(o[0]), tt = acc_value(o[1]);
+		if (fabs(tt) < 1e-30) {
+			st->pend = 1;
+			st->skip = 1;
+			st->reason = FX_BREAKDOWN;
+			st->resid = st->snorm;
+			return;
+		}
+		st->omega = ts / tt;
+	}
+};
+struct FinNone {
+	__device__ void operator()(Acc*) const {}
+};
+struct FinBiInit {
+	SState* st;
+	__device__ void operator()(Acc*) const {
+		st->rs = 1.0;      // rho
+		st->alpha = 1.0;
+		st->omega = 1.0;
+		st->iter = 0;
+	}
+};
+
+// ── GMRES(m) (gmres.rs:116-490) ────────────────────────────────────────
+// v0 = b − A·x (scale(−1) + axpy(1, b): gmres.rs:127-134) with ‖v0‖².
+struct FGmR0 {
+	const double* b;
+	double* v;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		double t = b[i] + (-v[i]);
+		v[i] = t;
+		acc_add(a[0], t * t);
+	}
+};
+struct FSumSq {
+	const double* v;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const { acc_add(a[0], v[i] * v[i]); }
+};
+// v *= 1/β (gmres.rs:146) or v = w·(1/h) (gmres.rs:183-186); the factor is 1/state value.
+struct FGmScale {
+	const double* den;
+	const double* src;
+	double* dst;
+	double inv;
+	__device__ void load() { inv = 1.0 / *den; }
+	__device__ void operator()(size_t i) const { dst[i] = src[i] * inv; }
+};
+// MGS (gmres.rs:171-176): w −= h_{j−1}·v_{j−1} (if j > 0) then w·v_j.
+struct FGmMgs {
+	double* w;
+	const double *vprev, *vj;
+	const double* hprev;
+	double h;
+	int has_prev;
+	__device__ void load() { h = has_prev ? *hprev : 0.0; }
+	__device__ void operator()(size_t i, Acc* a) const {
+		double t = w[i];
+		if (has_prev) {
+			t = fma(-h, vprev[i], t);
+			w[i] = t;
+		}
+		acc_add(a[0], t * (vj ? vj[i] : t));
+	}
+};
+__device__ __forceinline__ double& Hm(SState* s, int i, int j) { return s->H[j * (s->m + 1) + i]; }
+// Start of a cycle (gmres.rs:136-150): β = ‖v0‖.
+struct FinGmStart {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		double beta = sqrt(acc_value(o[0]));
+		if (beta < st->tol) { finish(st, st->iter, beta, FX_CONVERGED); return; }
+		// gmres.rs:155-164: the iteration budget is checked before the first Arnoldi step.
+		if (st->iter >= st->max_iter) { finish(st, st->iter, beta, FX_MAX_ITERATIONS); return; }
+		st->gbeta = beta;
+		st->gcur = beta;
+		for (int i = 0; i <= st->m; ++i) st->grs[i] = 0.0;
+		st->grs[0] = beta;
+		for (int i = 0; i < (st->m + 1) * st->m; ++i) st->H[i] = 0.0;
+		st->kcur = 0;
+		st->cycle_end = 0;
+		st->gexit = 0;
+		st->skip = 0;
+	}
+};
+struct FinGmH {
+	SState* st;
+	int j, k;
+	__device__ void operator()(Acc* o) const { Hm(st, j, k) = acc_value(o[0]); }
+};
+// End of an Arnoldi step (gmres.rs:178-247): h_{k+1,k}, Givens, |g_{k+1}|, stopping tests.
+struct FinGmStep {
+	SState* st;
+	int k;
+	__device__ void operator()(Acc* o) const {
+		double hk = sqrt(acc_value(o[0]));
+		Hm(st, k + 1, k) = hk;
+		st->ghk = hk;
+		for (int j = 0; j < k; ++j) {
+			double hj = Hm(st, j, k), hj1 = Hm(st, j + 1, k), cj = st->cc[j], sj = st->ss[j];
+			Hm(st, j, k) = fma(cj, hj, sj * hj1);
+			Hm(st, j + 1, k) = fma(cj, hj1, -sj * hj);
+		}
+		double a = Hm(st, k, k), b = Hm(st, k + 1, k);
+		double r = hypot(a, b);
+		if (fabs(r) < 1e-30) {
+			st->gexit = 1;
+			st->cycle_end = 1;
+			st->skip = 1;
+			return;
+		}
+		st->cc[k] = a / r;
+		st->ss[k] = b / r;
+		Hm(st, k, k) = r;
+		Hm(st, k + 1, k) = 0.0;
+		double gk = st->grs[k];
+		st->grs[k] = st->cc[k] * gk;
+		st->grs[k + 1] = -st->ss[k] * gk;
+		st->gcur = fabs(st->grs[k + 1]);
+		st->iter += 1;
+		st->kcur = k + 1;
+		record(st, st->iter, st->gcur);
+		if (st->gcur < st->tol || hk <= 1e-30 || k + 1 >= st->m || st->iter >= st->max_iter) {
+			st->cycle_end = 1;
+			st->skip = 1;
+		}
+	}
+};
+// v_{k+1} = w/h only when h > 1e−30 (gmres.rs:181-187).
+struct FGmNorm {
+	const SState* st;
+	const double* w;
+	double* dst;
+	double inv;
+	int ok;
+	__device__ void load() { ok = st->ghk > 1e-30; inv = ok ? 1.0 / st->ghk : 0.0; }
+	__device__ void operator()(size_t i) const {
+		if (ok) dst[i] = w[i] * inv;
+	}
+};
+// Back-substitution (gmres.rs:461-481).
+struct FinGmBack {
+	SState* st;
+	__device__ void operator()() const {
+		int k = st->kcur, mp1 = st->m + 1;
+		for (int i = 0; i < k; ++i) st->y[i] = 0.0;
+		for (int i = k - 1; i >= 0; --i) {
+			double s = st->grs[i];
+			for (int j = i + 1; j < k; ++j) s -= st->H[j * mp1 + i] * st->y[j];
+			double diag = st->H[i * mp1 + i];
+			if (fabs(diag) < 1e-30) break;
+			st->y[i] = s / diag;
+		}
+	}
+};
+// x += Σ y_i·v_i in order (gmres.rs:483-487).
+struct VecList {
+	const double* v[GM_MAX + 1];
+};
+struct FGmX {
+	const SState* st;
+	double* x;
+	VecList vl;
+	int k;
+	const double* y;
+	__device__ void load() { k = st->kcur; y = st->y; }
+	__device__ void operator()(size_t i) const {
+		double t = x[i];
+		for (int j = 0; j < k; ++j) t = fma(y[j], vl.v[j][i], t);
+		x[i] = t;
+	}
+};
+// After build_solution (gmres.rs:250-283): stop or restart.
+struct FinGmEnd {
+	SState* st;
+	__device__ void operator()() const {
+		if (st->gexit) { finish(st, st->iter, st->gcur, FX_BREAKDOWN); return; }
+		if (st->gcur < st->tol) { finish(st, st->iter, st->gcur, FX_CONVERGED); return; }
+		if (st->iter >= st->max_iter) { finish(st, st->iter, st->gcur, FX_MAX_ITERATIONS); return; }
+		st->cycle_end = 0;
+		st->skip = 0;
+	}
+};
+struct FinMaxCheck {
+	SState* st;
+	__device__ void operator()() const {
+		if (st->iter >= st->max_iter) finish(st, st->iter, st->gcur, FX_MAX_ITERATIONS);
+	}
+};
+
+}  // namespace
+
+// ════════════════════════════════════════════════════════════════════════
+struct fx_solver {
+	fx_backend* be = nullptr;
+	int kind = 0;
+	fx_grid g{};
+	size_t n = 0;
+	uint32_t restart = 0;
+	double tol = 0;
+	uint64_t max_iter = 0;
+	std::vector<double*> v;      // work vectors
+	std::vector<double*> basis;  // GMRES Krylov basis (m+1)
+	SState* dst = nullptr;       // device state
+	SState* hst = nullptr;       // pinned host copy
+	double* dhist = nullptr;
+	double* dgm = nullptr;       // GMRES H, cc, ss, grs, y
+	int* hflag = nullptr;        // pinned ring of done flags
+	hipEvent_t ev[8] = {};
+	std::vector<double> hist;
+	double* vec(int i) {
+		while ((int)v.size() <= i) {
+			void* p = nullptr;
+			if (fx::dev_alloc(be, n * sizeof(double), &p) != FX_OK) return nullptr;
+			v.push_back((double*)p);
+		}
+		return v[i];
+	}
+};
+
+namespace {
+
+using fx::check_launch;
+
+const char* solver_name(int kind) {
+	switch (kind) {
+	case FX_SOLVER_CG: return "CG";
+	case FX_SOLVER_PIPELINED_CG: return "PipelinedCG";
+	case FX_SOLVER_BICGSTAB: return "BiCGSTAB";
+	case FX_SOLVER_GMRES: return "GMRES";
+	case FX_SOLVER_FCG: return "FCG";
+	default: return "?";
+	}
+}
+
+struct Ctx {
+	fx_solver* s;
+	fx_backend* be;
+	hipStream_t st;
+	const fx_operator* op;
+	const fx_precond* pc;
+	size_t n;
+	int64_t rows, cols;
+	SState* d;
+	const int* done;   // &d->done
+	const int* skip;   // &d->skip
+	bool fused5;       // operator is the reference 5-point stencil kind
+	W9 w9;
+
+	unsigned eb() const { return fx::elem_blocks(n); }
+	unsigned rb() const { return fx::red_blocks(n); }
+
+	template <int NS, class F, class Fin>
+	fx_status ered(F f, Fin fin, const char* what, const int* sk) {
+		unsigned nb = rb();
+		FX_CHECK(fx::ensure_partials(be, (size_t)nb * NS));
+		hipLaunchKernelGGL((k_ered<NS, F, Fin>), dim3(nb), dim3(BLOCK), 0, st, f, fin, n, fx::red_scratch(be), sk);
+		return check_launch(what);
+	}
+	template <int NS, class F, class Fin>
+	fx_status ered(F f, Fin fin, const char* what) { return ered<NS>(f, fin, what, skip); }
+	template <class F>
+	fx_status eupd(F f, const char* what, const int* sk) {
+		hipLaunchKernelGGL(k_eupd<F>, dim3(eb()), dim3(BLOCK), 0, st, f, n, sk);
+		return check_launch(what);
+	}
+	template <class F>
+	fx_status eupd(F f, const char* what) { return eupd(f, what, skip); }
+	template <class Fin>
+	fx_status scalar(Fin fin, const char* what, const int* sk) {
+		hipLaunchKernelGGL(k_scalar<Fin>, dim3(1), dim3(1), 0, st, fin, sk);
+		return check_launch(what);
+	}
+	fx_status apply_op(const double* x, double* y, const int* sk) {
+		fx_grid g = s->g;
+		return fx::k_apply_operator(be, op, x, y, &g, sk);
+	}
+	fx_status apply_op(const double* x, double* y) { return apply_op(x, y, skip); }
+	// z = M·r, z zeroed first (cg.rs:193-194); identity copies (preconditioner.rs:29-37).
+	fx_status precond(double* z, const double* r, const int* sk) {
+		if (!pc || pc->kind == FX_PC_IDENTITY) return eupd(FCopy{r, z}, "pc identity", sk);
+		if (pc->kind == FX_PC_MULTIGRID) {
+			FX_CHECK(eupd(FZero{z}, "pc zero", sk));
+			return fx::mg_apply((fx_multigrid*)pc->handle, z, r, sk);
+		}
+		fx::set_error("preconditioner kind %d not supported here", pc->kind);
+		return FX_ERR_UNSUPPORTED;
+	}
+	fx_status precond(double* z, const double* r) { return precond(z, r, skip); }
+
+	// p_out = src + β·p_old; ap = A·p_out; p·ap -> α (FinAlpha).
+	fx_status cg_step1(const double* src, const double* pold, double* pout, double* ap) {
+		if (fused5) {
+			int rpb = stream_rpb(rows, cols);
+			dim3 grid = fx::stream_grid(rows, cols, rpb);
+			FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
+			hipLaunchKernelGGL((k_stream_st<1, VPNew, EpiCG, FinAlpha>), grid, dim3(BLOCK), 0, st, VPNew{src, pold, 0.0},
+			                   EpiCG{op->coeff, pout, ap}, FinAlpha{d}, w9, rows, cols, rpb, fx::red_scratch(be),
+			                   (const SState*)d);
+			return check_launch("cg fused step");
+		}
+		FX_CHECK(eupd(FPUpd{d, src, pold, pout, 0.0}, "p update"));
+		FX_CHECK(apply_op(pout, ap));
+		return ered<1>(FDot{pout, ap}, FinAlpha{d}, "p.ap");
+	}
+};
+
+fx_status make_ctx(fx_solver* s, const fx_operator* op, const fx_precond* pc, Ctx* c) {
+	c->s = s;
+	c->be = s->be;
+	c->st = s->be->stream;
+	c->op = op;
+	c->pc = pc;
+	c->n = s->n;
+	c->rows = (int64_t)s->g.rows;
+	c->cols = (int64_t)s->g.cols;
+	c->d = s->dst;
+	c->done = &s->dst->done;
+	c->skip = &s->dst->skip;
+	c->fused5 = false;
+	if (op->kind == FX_OP_STENCIL) {
+		FX_CHECK(fx::check_bc(op->bc));
+		FX_CHECK(fx::stencil_args(op->stencil, &c->w9, nullptr));
+		c->fused5 = op->stencil->n == 5;
+	}
+	FX_REQUIRE(!pc || pc->kind == FX_PC_IDENTITY || pc->kind == FX_PC_MULTIGRID, FX_ERR_UNSUPPORTED,
+	           "preconditioner kind %d is not supported", pc ? pc->kind : -1);
+	if (pc && pc->kind == FX_PC_MULTIGRID) {
+		fx_multigrid* mg = (fx_multigrid*)pc->handle;
+		FX_REQUIRE(mg && mg->lv[0].g.rows == s->g.rows && mg->lv[0].g.cols == s->g.cols, FX_ERR_DIMENSION,
+		           "multigrid grid does not match the solver grid");
+	}
+	return FX_OK;
+}
+
+// Resets the device state for a new solve.
+fx_status reset_state(fx_solver* s, int64_t max_iter) {
+	SState h;
+	std::memset(&h, 0, sizeof h);
+	h.tol = s->tol;
+	h.max_iter = max_iter;
+	h.hist = s->dhist;
+	h.hist_len = HIST_MAX;
+	h.m = (int32_t)s->restart;
+	if (s->dgm) {
+		size_t m = s->restart;
+		h.H = s->dgm;
+		h.cc = h.H + (m + 1) * m;
+		h.ss = h.cc + m;
+		h.grs = h.ss + m;
+		h.y = h.grs + m + 1;
+	}
+	*s->hst = h;
+	FX_HIP(hipMemcpyAsync(s->dst, s->hst, sizeof(SState), hipMemcpyHostToDevice, s->be->stream));
+	FX_HIP(hipStreamSynchronize(s->be->stream));
+	return FX_OK;
+}
+
+// Enqueues `step(i)` until the device reports done, polling `look` steps behind.
+template <class Step>
+fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
+	constexpr int RING = 8;
+	hipStream_t st = s->be->stream;
+	for (int64_t i = 0;; ++i) {
+		if (i > max_steps + look + 2) break;
+		FX_CHECK(step(i));
+		FX_HIP(hipMemcpyAsync(&s->hflag[i % RING], &s->dst->done, sizeof(int), hipMemcpyDeviceToHost, st));
+		FX_HIP(hipEventRecord(s->ev[i % RING], st));
+		if (i >= look) {
+			FX_HIP(hipEventSynchronize(s->ev[(i - look) % RING]));
+			if (s->hflag[(i - look) % RING]) break;
+		}
+	}
+	FX_HIP(hipStreamSynchronize(st));
+	return FX_OK;
+}
+
+fx_status fetch_stats(fx_solver* s, fx_solve_stats* out) {
+	FX_HIP(hipMemcpyAsync(s->hst, s->dst, sizeof(SState), hipMemcpyDeviceToHost, s->be->stream));
+	FX_HIP(hipStreamSynchronize(s->be->stream));
+	int64_t nh = std::min<int64_t>(s->hst->iter + 1, HIST_MAX);
+	s->hist.assign((size_t)std::max<int64_t>(nh, 0), 0.0);
+	if (nh > 0) {
+		FX_HIP(hipMemcpyAsync(s->hist.data(), s->dhist, sizeof(double) * nh, hipMemcpyDeviceToHost, s->be->stream));
+		FX_HIP(hipStreamSynchronize(s->be->stream));
+	}
+	if (out) {
+		out->iterations = (uint64_t)s->hst->iter;
+		out->residual = s->hst->resid;
+		out->reason = s->hst->reason;
+		out->true_residual = NAN;
+	}
+	FX_REQUIRE(s->hst->done, FX_ERR_HIP, "solver did not reach a stopping state");
+	return FX_OK;
+}
+
+// ‖b − A·x‖₂ in FP64 (overwrites work vector `tmp`).
+fx_status true_residual(Ctx& c, const double* x, const double* b, double* tmp, double* out) {
+	FX_CHECK(c.apply_op(x, tmp, nullptr));
+	FX_CHECK(c.ered<1>(FResid{b, tmp, tmp}, FinPcgInitRR{c.d}, "true residual", nullptr));
+	FX_HIP(hipMemcpyAsync(c.s->hst, c.d, sizeof(SState), hipMemcpyDeviceToHost, c.st));
+	FX_HIP(hipStreamSynchronize(c.st));
+	*out = std::sqrt(c.s->hst->rr);
+	return FX_OK;
+}
+
+// ── CG / PCG (cg.rs:96-256) ───────────────────────────────────────────
+fx_status solve_cg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *p0 = s->vec(1), *p1 = s->vec(2), *ap = s->vec(3);
+	FX_REQUIRE(r && p0 && p1 && ap, FX_ERR_ALLOC, "work vector allocation failed");
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	FX_CHECK(c.apply_op(x, ap, nullptr));
+	FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
+	double* pp[2] = {p0, p1};
+	if (!c.pc) {
+		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinCGInit{c.d}, "cg init", nullptr));
+		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+			double* pold = pp[i & 1];
+			double* pnew = pp[(i + 1) & 1];
+			FX_CHECK(c.cg_step1(r, pold, pnew, ap));
+			return c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinCGBeta{c.d}, "cg x/r update");
+		}, (int64_t)s->max_iter));
+	} else {
+		double* z = s->vec(4);
+		FX_REQUIRE(z, FX_ERR_ALLOC, "work vector allocation failed");
+		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "pcg init", nullptr));
+		FX_CHECK(c.precond(z, r));
+		FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "pcg r.z"));
+		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+			double* pold = pp[i & 1];
+			double* pnew = pp[(i + 1) & 1];
+			FX_CHECK(c.cg_step1(z, pold, pnew, ap));
+			FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "pcg x/r update"));
+			FX_CHECK(c.precond(z, r));
+			return c.ered<1>(FDot{r, z}, FinPcgBeta{c.d}, "pcg r.z");
+		}, (int64_t)s->max_iter));
+	}
+	FX_CHECK(fetch_stats(s, stats));
+	if (stats) FX_CHECK(true_residual(c, x, b, ap, &stats->true_residual));
+	return FX_OK;
+}
+
+// ── FCG: flexible PCG with FP64 true-residual refinement (extension) ──
+fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *p0 = s->vec(1), *p1 = s->vec(2), *ap = s->vec(3), *z = s->vec(4);
+	FX_REQUIRE(r && p0 && p1 && ap && z, FX_ERR_ALLOC, "work vector allocation failed");
+	uint64_t used = 0;
+	fx_solve_stats st{};
+	double* pp[2] = {p0, p1};
+	for (int pass = 0; pass < 8; ++pass) {
+		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used)));
+		FX_CHECK(c.apply_op(x, ap, nullptr));
+		FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
+		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "fcg init", nullptr));
+		FX_CHECK(c.precond(z, r));
+		FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "fcg r.z"));
+		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+			double* pold = pp[i & 1];
+			double* pnew = pp[(i + 1) & 1];
+			FX_CHECK(c.cg_step1(z, pold, pnew, ap));
+			FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
+			FX_CHECK(c.precond(z, r));
+			return c.ered<2>(FDot2{r, z, z, ap}, FinFcgBeta{c.d}, "fcg r.z z.ap");
+		}, (int64_t)(s->max_iter - used)));
+		FX_CHECK(fetch_stats(s, &st));
+		used += st.iterations;
+		double tr = 0;
+		FX_CHECK(true_residual(c, x, b, ap, &tr));
+		st.true_residual = tr;
+		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
+	}
+	st.iterations = used;
+	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol)) st.reason = FX_MAX_ITERATIONS;
+	if (stats) *stats = st;
+	return FX_OK;
+}
+
+// ── PipelinedCgSolver::solve (pipelined_cg.rs:96-167) ─────────────────
+fx_status solve_pipelined(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *p = s->vec(1), *w = s->vec(2), *t = s->vec(3), *z = s->vec(4), *q = s->vec(5);
+	FX_REQUIRE(r && p && w && t && z && q, FX_ERR_ALLOC, "work vector allocation failed");
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	FX_CHECK(c.apply_op(x, q, nullptr));
+	FX_CHECK(c.ered<1>(FResid{b, q, r}, FinPcgInitRR{c.d}, "pipecg init", nullptr));
+	FX_CHECK(c.apply_op(r, w, nullptr));
+	FX_CHECK(fx::k_fill(c.be, p, 0.0, c.n));
+	FX_CHECK(fx::k_fill(c.be, t, 0.0, c.n));
+	FX_CHECK(fx::k_fill(c.be, z, 0.0, c.n));
+	FX_CHECK(c.ered<2>(FDot2{r, r, r, w}, FinPipe{c.d, true}, "pipecg dot2"));
+	FX_CHECK(drive(s, [&](int64_t) -> fx_status {
+		// q = A·w (pipelined_cg.rs:143-147), fused update + next (γ, δ).
+		FX_CHECK(c.apply_op(w, q));
+		return c.ered<2>(FPipeUpd{c.d, z, t, p, x, r, w, q, 0.0, 0.0}, FinPipe{c.d, false}, "pipecg update");
+	}, (int64_t)s->max_iter));
+	FX_CHECK(fetch_stats(s, stats));
+	if (stats) FX_CHECK(true_residual(c, x, b, q, &stats->true_residual));
+	return FX_OK;
+}
+
+// ── preconditioned pipelined CG (extension, Ghysels & Vanroose Alg. 3) ─
+fx_status solve_ppcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *u = s->vec(1), *w = s->vec(2), *m = s->vec(3), *nn = s->vec(4), *z = s->vec(5),
+	       *q = s->vec(6), *sv = s->vec(7), *p = s->vec(8);
+	FX_REQUIRE(r && u && w && m && nn && z && q && sv && p, FX_ERR_ALLOC, "work vector allocation failed");
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	FX_CHECK(c.apply_op(x, sv, nullptr));
+	FX_CHECK(c.ered<1>(FResid{b, sv, r}, FinPcgInitRR{c.d}, "ppcg init", nullptr));
+	FX_CHECK(c.precond(u, r));
+	FX_CHECK(c.apply_op(u, w, nullptr));
+	for (double* v : {z, q, sv, p}) FX_CHECK(fx::k_fill(c.be, v, 0.0, c.n));
+	FX_CHECK(c.ered<3>(FDot3{r, u, w}, FinPP{c.d, true}, "ppcg dot3"));
+	FX_CHECK(drive(s, [&](int64_t) -> fx_status {
+		FX_CHECK(c.precond(m, w));
+		FX_CHECK(c.apply_op(m, nn));
+		return c.ered<3>(FPPUpd{c.d, z, q, sv, p, x, r, u, w, nn, m, 0.0, 0.0}, FinPP{c.d, false}, "ppcg update");
+	}, (int64_t)s->max_iter));
+	FX_CHECK(fetch_stats(s, stats));
+	if (stats) FX_CHECK(true_residual(c, x, b, m, &stats->true_residual));
+	return FX_OK;
+}
+
+// ── BiCgStabSolver (bicgstab.rs:90-346) ───────────────────────────────
+fx_status solve_bicgstab(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *rh = s->vec(1), *p = s->vec(2), *v = s->vec(3), *sv = s->vec(4), *t = s->vec(5),
+	       *ph = s->vec(6), *sh = s->vec(7);
+	FX_REQUIRE(r && rh && p && v && sv && t && ph && sh, FX_ERR_ALLOC, "work vector allocation failed");
+	const bool pre = c.pc != nullptr;
+	const double* pa = pre ? ph : p;
+	const double* sa = pre ? sh : sv;
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	// r = b − A·x; r̂ = r; v = p = 0 (bicgstab.rs:98-113).
+	FX_CHECK(c.apply_op(x, v, nullptr));
+	FX_CHECK(c.ered<1>(FResid{b, v, r}, FinBiInit{c.d}, "bicgstab init", nullptr));
+	FX_CHECK(c.eupd(FCopy{r, rh}, "copy", nullptr));
+	FX_CHECK(fx::k_fill(c.be, v, 0.0, c.n));
+	FX_CHECK(fx::k_fill(c.be, p, 0.0, c.n));
+	FX_CHECK(c.ered<2>(FDot2{r, r, rh, r}, FinBiTop{c.d, true}, "bicgstab top", nullptr));
+	FX_CHECK(drive(s, [&](int64_t) -> fx_status {
+		FX_CHECK(c.eupd(FBiP{c.d, p, r, v, 0.0, 0.0}, "bicgstab p"));
+		if (pre) FX_CHECK(c.precond(ph, p));
+		FX_CHECK(c.apply_op(pa, v));
+		FX_CHECK(c.ered<1>(FDot{rh, v}, FinBiAlpha{c.d}, "bicgstab rh.v"));
+		FX_CHECK(c.ered<1>(FBiS{c.d, sv, r, v, 0.0}, FinBiS{c.d}, "bicgstab s"));
+		if (pre) FX_CHECK(c.precond(sh, sv));
+		FX_CHECK(c.apply_op(sa, t));
+		FX_CHECK(c.ered<2>(FDot2{t, sv, t, t}, FinBiOm{c.d}, "bicgstab omega"));
+		return c.ered<2>(FBiX{c.d, x, r, pa, sa, sv, t, rh, 0.0, 0.0, 0}, FinBiTop{c.d, false}, "bicgstab x/r", c.done);
+	}, (int64_t)s->max_iter));
+	FX_CHECK(fetch_stats(s, stats));
+	if (stats) FX_CHECK(true_residual(c, x, b, t, &stats->true_residual));
+	return FX_OK;
+}
+
+// ── GmresSolver (gmres.rs:116-490) ────────────────────────────────────
+fx_status solve_gmres(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	const int m = (int)s->restart;
+	const bool pre = c.pc != nullptr;
+	double* w = s->vec(0);
+	FX_REQUIRE(w && (int)s->basis.size() == m + 1, FX_ERR_ALLOC, "work vector allocation failed");
+	auto& vv = s->basis;
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	VecList vl{};
+	for (int i = 0; i <= m; ++i) vl.v[i] = vv[i];
+	const SState* dd = c.d;
+	FX_CHECK(drive(s, [&](int64_t) -> fx_status {
+		// Cycle start (gmres.rs:124-157 / 305-337).
+		if (!pre) {
+			FX_CHECK(c.apply_op(x, vv[0], c.done));
+			FX_CHECK(c.ered<1>(FGmR0{b, vv[0]}, FinGmStart{c.d}, "gmres r0", c.done));
+		} else {
+			FX_CHECK(c.apply_op(x, w, c.done));
+			FX_CHECK(c.ered<1>(FGmR0{b, w}, FinNone{}, "gmres r0", c.done));
+			FX_CHECK(c.precond(vv[0], w, c.done));
+			FX_CHECK(c.ered<1>(FSumSq{vv[0]}, FinGmStart{c.d}, "gmres beta", c.done));
+		}
+		FX_CHECK(c.eupd(FGmScale{&dd->gbeta, vv[0], vv[0], 0.0}, "gmres v0 scale"));
+		for (int k = 0; k < m; ++k) {
+			FX_CHECK(c.apply_op(vv[k], w));
+			if (pre) {
+				FX_CHECK(c.precond(vv[k + 1], w));
+				FX_CHECK(c.eupd(FCopy{vv[k + 1], w}, "gmres copy"));
+			}
+			// H(i, k) lives at dgm[k·(m+1) + i] (column-major, gmres.rs:84-99).
+			const double* Hk = s->dgm + (size_t)k * (m + 1);
+			for (int j = 0; j <= k; ++j)
+				FX_CHECK(c.ered<1>(FGmMgs{w, j ? vv[j - 1] : nullptr, vv[j], j ? Hk + (j - 1) : nullptr, 0.0, j > 0},
+				                   FinGmH{c.d, j, k}, "gmres mgs"));
+			FX_CHECK(c.ered<1>(FGmMgs{w, vv[k], nullptr, Hk + k, 0.0, 1}, FinGmStep{c.d, k}, "gmres h"));
+			FX_CHECK(c.eupd(FGmNorm{c.d, w, vv[k + 1], 0.0, 0}, "gmres normalize"));
+		}
+		FX_CHECK(c.scalar(FinGmBack{c.d}, "gmres back", c.done));
+		FX_CHECK(c.eupd(FGmX{c.d, x, vl, 0, nullptr}, "gmres x", c.done));
+		return c.scalar(FinGmEnd{c.d}, "gmres end", c.done);
+	}, (int64_t)s->max_iter + 1, 1));
+	FX_CHECK(fetch_stats(s, stats));
+	if (stats) FX_CHECK(true_residual(c, x, b, w, &stats->true_residual));
+	return FX_OK;
+}
+
+}  // namespace
+
+using namespace fx;
+
+extern "C" {
+
+fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, uint32_t restart, double tol,
+                           uint64_t max_iter, fx_solver** out) {
+	FX_REQUIRE(be && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(check_grid(grid));
+	FX_REQUIRE(kind >= FX_SOLVER_CG && kind <= FX_SOLVER_FCG, FX_ERR_ARGUMENT, "unknown solver kind %d", kind);
+	FX_REQUIRE(kind != FX_SOLVER_GMRES || (restart >= 1 && restart <= GM_MAX), FX_ERR_ARGUMENT,
+	           "GMRES restart must be in 1..%d", GM_MAX);
+	fx_solver* s = new fx_solver();
+	s->be = be;
+	s->kind = kind;
+	s->g = *grid;
+	s->n = grid->rows * grid->cols;
+	s->restart = restart;
+	s->tol = tol;
+	s->max_iter = max_iter;
+	auto fail = [&](fx_status st) {
+		fx_solver_destroy(s);
+		return st;
+	};
+	void* p = nullptr;
+	fx_status st;
+	if ((st = dev_alloc(be, sizeof(SState), &p)) != FX_OK) return fail(st);
+	s->dst = (SState*)p;
+	if ((st = dev_alloc(be, sizeof(double) * HIST_MAX, &p)) != FX_OK) return fail(st);
+	s->dhist = (double*)p;
+	if (hipHostMalloc((void**)&s->hst, sizeof(SState), hipHostMallocDefault) != hipSuccess) return fail(FX_ERR_ALLOC);
+	if (hipHostMalloc((void**)&s->hflag, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess) return fail(FX_ERR_ALLOC);
+	for (auto& e : s->ev)
+		if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(FX_ERR_HIP);
+	// Work vectors up front, as the reference constructors do (cg.rs:79-84).
+	int nvec = kind == FX_SOLVER_CG || kind == FX_SOLVER_FCG ? 5
+	         : kind == FX_SOLVER_PIPELINED_CG                ? 6
+	         : kind == FX_SOLVER_BICGSTAB                    ? 8
+	                                                         : 1;
+	for (int i = 0; i < nvec; ++i)
+		if (!s->vec(i)) return fail(FX_ERR_ALLOC);
+	if (kind == FX_SOLVER_GMRES) {
+		for (uint32_t i = 0; i <= restart; ++i) {
+			if (dev_alloc(be, s->n * sizeof(double), &p) != FX_OK) return fail(FX_ERR_ALLOC);
+			s->basis.push_back((double*)p);
+		}
+		size_t m = restart;
+		if (dev_alloc(be, sizeof(double) * ((m + 1) * m + 3 * m + 1 + m), &p) != FX_OK) return fail(FX_ERR_ALLOC);
+		s->dgm = (double*)p;
+	}
+	if (hipStreamSynchronize(be->stream) != hipSuccess) return fail(FX_ERR_HIP);
+	*out = s;
+	return FX_OK;
+}
+
+void fx_solver_destroy(fx_solver* s) {
+	if (!s) return;
+	if (s->be) (void)hipStreamSynchronize(s->be->stream);
+	for (double* p : s->v) dev_free(p);
+	for (double* p : s->basis) dev_free(p);
+	dev_free(s->dst);
+	dev_free(s->dhist);
+	dev_free(s->dgm);
+	if (s->hst) (void)hipHostFree(s->hst);
+	if (s->hflag) (void)hipHostFree(s->hflag);
+	for (auto& e : s->ev)
+		if (e) (void)hipEventDestroy(e);
+	delete s;
+}
+
+const char* fx_solver_name(const fx_solver* s) { return s ? solver_name(s->kind) : "?"; }
+
+size_t fx_solver_history(const fx_solver* s, double* out, size_t len) {
+	if (!s) return 0;
+	size_t n = std::min(len, s->hist.size());
+	if (out && n) std::memcpy(out, s->hist.data(), n * sizeof(double));
+	return s->hist.size();
+}
+
+fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_operator* op, const fx_precond* pc,
+                             fx_solve_stats* stats) {
+	FX_REQUIRE(s && op, FX_ERR_ARGUMENT, "null argument");
+	FX_CHECK(check_buf(x, s->n, FX_F64, "x"));
+	FX_CHECK(check_buf(b, s->n, FX_F64, "b"));
+	Ctx c;
+	FX_CHECK(make_ctx(s, op, pc, &c));
+	double* xp = (double*)x->ptr;
+	const double* bp = (const double*)b->ptr;
+	switch (s->kind) {
+	case FX_SOLVER_CG: return solve_cg(c, xp, bp, stats);
+	case FX_SOLVER_FCG: return solve_fcg(c, xp, bp, stats);
+	case FX_SOLVER_PIPELINED_CG: return pc ? solve_ppcg(c, xp, bp, stats) : solve_pipelined(c, xp, bp, stats);
+	case FX_SOLVER_BICGSTAB: return solve_bicgstab(c, xp, bp, stats);
+	case FX_SOLVER_GMRES: return solve_gmres(c, xp, bp, stats);
+	default: break;
+	}
+	set_error("solver kind %d not implemented", s->kind);
+	return FX_ERR_UNSUPPORTED;
+}
+
+static fx_operator stencil_op(double coeff, const fx_stencil* st, const fx_boundaries* bc) {
+	fx_operator op{};
+	op.kind = FX_OP_STENCIL;
+	op.coeff = coeff;
+	op.sigma = 1.0;
+	op.stencil = st;
+	op.bc = bc;
+	return op;
+}
+
+fx_status fx_solver_solve(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff, const fx_stencil* st,
+                          const fx_boundaries* bc, fx_solve_stats* stats) {
+	fx_operator op = stencil_op(coeff, st, bc);
+	return fx_solver_solve_op(s, x, b, &op, nullptr, stats);
+}
+
+fx_status fx_solver_solve_preconditioned(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff, const fx_stencil* st,
+                                         const fx_boundaries* bc, const fx_precond* pc, fx_solve_stats* stats) {
+	FX_REQUIRE(pc, FX_ERR_ARGUMENT, "preconditioner is null");
+	fx_operator op = stencil_op(coeff, st, bc);
+	return fx_solver_solve_op(s, x, b, &op, pc, stats);
+}
+
+}  // extern "C"

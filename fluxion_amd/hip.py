@@ -1,0 +1,479 @@
+"""Python mirror of the reference runtime API over the HIP C ABI.
+
+HipBackend    ~ crates/backend-hip/src/backend.rs (the `Backend` trait, core/src/backend.rs:43-406)
+Field         ~ crates/runtime/src/field.rs (buffer + grid with per-op methods)
+CgSolver, PipelinedCgSolver, BiCgStabSolver, GmresSolver
+              ~ crates/runtime/src/{cg,pipelined_cg,bicgstab,gmres}.rs
+Multigrid, Identity
+              ~ crates/runtime/src/{multigrid,preconditioner}.rs
+
+Every method calls one C entry point of libfluxion_hip.so; nothing here computes
+on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .core import (Boundaries, ConvergenceReason, DType, DimensionMismatch, Grid, SmootherKind, SolveStats,
+                   Stencil, Transfer)
+
+
+class HipBackend:
+    """One device, one stream (backend-hip/src/backend.rs:61-264)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        L.check(self._lib.fx_backend_create(device, C.byref(h)), "fx_backend_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.fx_backend_destroy(h)
+            self.h = None
+
+    @property
+    def name(self) -> str:
+        return self._lib.fx_backend_name(self.h).decode()
+
+    def synchronize(self):
+        L.check(self._lib.fx_synchronize(self.h), "synchronize")
+
+    @property
+    def stream(self) -> int:
+        return self._lib.fx_backend_stream(self.h) or 0
+
+    # ── memory ────────────────────────────────────────────────────────
+    def allocate(self, n: int, dtype: DType = DType.F64) -> "Buffer":
+        h = C.c_void_p()
+        L.check(self._lib.fx_allocate(self.h, n, int(dtype), C.byref(h)), "allocate")
+        return Buffer(self, h)
+
+    def upload(self, data) -> "Buffer":
+        a = np.ascontiguousarray(data)
+        h = C.c_void_p()
+        if a.dtype == np.float32:
+            L.check(self._lib.fx_upload_f32(self.h, a.ctypes.data, a.size, C.byref(h)), "upload_f32")
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            L.check(self._lib.fx_upload_f64(self.h, a.ctypes.data, a.size, C.byref(h)), "upload_f64")
+        return Buffer(self, h)
+
+
+class Buffer:
+    """Device buffer owned by the library (backend-hip/src/buffer.rs:16-123)."""
+
+    def __init__(self, be: HipBackend, h: C.c_void_p):
+        self.be = be
+        self.h = h
+        self._lib = be._lib
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.fx_buffer_free(h)
+            self.h = None
+
+    def __len__(self):
+        return int(self._lib.fx_buffer_len(self.h))
+
+    @property
+    def dtype(self) -> DType:
+        return DType(self._lib.fx_buffer_dtype(self.h))
+
+    @property
+    def device_ptr(self) -> int:
+        return self._lib.fx_buffer_device_ptr(self.h) or 0
+
+    def to_numpy(self, dtype=np.float64) -> np.ndarray:
+        n = len(self)
+        out = np.empty(n, dtype=dtype)
+        if dtype == np.float32:
+            L.check(self._lib.fx_copy_to_host_f32(self.be.h, self.h, out.ctypes.data, n), "copy_to_host_f32")
+        else:
+            L.check(self._lib.fx_copy_to_host_f64(self.be.h, self.h, out.ctypes.data, n), "copy_to_host_f64")
+        return out
+
+    def write(self, data) -> None:
+        a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+        L.check(self._lib.fx_write_f64(self.be.h, self.h, a.ctypes.data, a.size), "write_f64")
+
+
+def _backend(be: Optional[HipBackend]) -> HipBackend:
+    global _DEFAULT
+    if be is not None:
+        return be
+    if _DEFAULT is None:
+        _DEFAULT = HipBackend(0)
+    return _DEFAULT
+
+
+_DEFAULT: Optional[HipBackend] = None
+
+
+def default_backend() -> HipBackend:
+    """Process-wide backend, like the reference's OnceLock singleton (storage.rs:120-131)."""
+    return _backend(None)
+
+
+class Field:
+    """A buffer with its grid (crates/runtime/src/field.rs:15-1050)."""
+
+    def __init__(self, grid: Grid, buf: Buffer):
+        if len(buf) != len(grid):
+            raise DimensionMismatch(f"expected {len(grid)}, got {len(buf)}")
+        self.grid = grid
+        self.buf = buf
+        self.be = buf.be
+        self._lib = buf._lib
+
+    # constructors (field.rs:27-110)
+    @staticmethod
+    def zeros(grid: Grid, dtype: DType = DType.F64, be: Optional[HipBackend] = None) -> "Field":
+        be = _backend(be)
+        return Field(grid, be.allocate(len(grid), dtype))
+
+    @staticmethod
+    def from_f64(grid: Grid, data, be: Optional[HipBackend] = None) -> "Field":
+        a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+        if a.size != len(grid):
+            raise DimensionMismatch(f"expected {len(grid)}, got {a.size}")
+        return Field(grid, _backend(be).upload(a))
+
+    def to_numpy(self) -> np.ndarray:
+        return self.buf.to_numpy().reshape(self.grid.rows, self.grid.cols)
+
+    def to_vec_f64(self) -> np.ndarray:
+        return self.buf.to_numpy()
+
+    def write(self, data) -> None:
+        self.buf.write(data)
+
+    @property
+    def h(self):
+        return self.buf.h
+
+    def _bh(self):
+        return self.be.h
+
+    # ── ops (field.rs:147-1050) ───────────────────────────────────────
+    def copy_from(self, other: "Field"):
+        L.check(self._lib.fx_copy(self._bh(), other.h, self.h), "copy")
+
+    def fill(self, value: float):
+        L.check(self._lib.fx_fill(self._bh(), self.h, value), "fill")
+
+    def scale(self, alpha: float):
+        L.check(self._lib.fx_scale(self._bh(), self.h, alpha), "scale")
+
+    def axpy(self, alpha: float, x: "Field"):
+        L.check(self._lib.fx_axpy(self._bh(), alpha, x.h, self.h), "axpy")
+
+    def waxpy(self, alpha: float, x: "Field", beta: float, y: "Field"):
+        L.check(self._lib.fx_waxpy(self._bh(), alpha, x.h, beta, y.h, self.h), "waxpy")
+
+    def aypx(self, alpha: float, x: "Field"):
+        L.check(self._lib.fx_aypx(self._bh(), alpha, x.h, self.h), "aypx")
+
+    def pointwise_mult(self, x: "Field", y: "Field"):
+        L.check(self._lib.fx_pointwise_mult(self._bh(), x.h, y.h, self.h), "pointwise_mult")
+
+    def pointwise_div(self, x: "Field", y: "Field"):
+        L.check(self._lib.fx_pointwise_div(self._bh(), x.h, y.h, self.h), "pointwise_div")
+
+    def pointwise_max(self, x: "Field", y: "Field"):
+        L.check(self._lib.fx_pointwise_max(self._bh(), x.h, y.h, self.h), "pointwise_max")
+
+    def pointwise_min(self, x: "Field", y: "Field"):
+        L.check(self._lib.fx_pointwise_min(self._bh(), x.h, y.h, self.h), "pointwise_min")
+
+    def reciprocal(self):
+        L.check(self._lib.fx_reciprocal(self._bh(), self.h), "reciprocal")
+
+    def abs_val(self):
+        L.check(self._lib.fx_abs_val(self._bh(), self.h), "abs_val")
+
+    def zero_boundaries(self):
+        L.check(self._lib.fx_zero_boundaries(self._bh(), self.h, self.grid.rows, self.grid.cols), "zero_boundaries")
+
+    def _red(self, fn, *args) -> float:
+        out = C.c_double()
+        L.check(fn(self._bh(), *args, C.byref(out)), fn.__name__)
+        return out.value
+
+    def norm_l2(self) -> float:
+        return self._red(self._lib.fx_norm_l2, self.h)
+
+    def sum(self) -> float:
+        return self._red(self._lib.fx_reduce_sum, self.h)
+
+    def max(self) -> float:
+        return self._red(self._lib.fx_reduce_max, self.h)
+
+    def min(self) -> float:
+        return self._red(self._lib.fx_reduce_min, self.h)
+
+    def dot(self, other: "Field") -> float:
+        return self._red(self._lib.fx_dot, self.h, other.h)
+
+    def dot2(self, a: "Field", b: "Field", c: "Field"):
+        """(self·a, b·c) in one pass (field.rs:637-679)."""
+        out = (C.c_double * 2)()
+        L.check(self._lib.fx_dot2(self._bh(), self.h, a.h, b.h, c.h, out), "dot2")
+        return out[0], out[1]
+
+    def integral(self) -> float:
+        """field.rs:686-690: sum·dx·dy."""
+        return self.sum() * self.grid.dx * self.grid.dy
+
+    def apply_stencil_into(self, stencil: Stencil, bc: Boundaries, output: "Field"):
+        st = L.StencilC(stencil)
+        g, b = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_apply_stencil(self._bh(), self.h, output.h, C.byref(g), C.byref(st.c), C.byref(b)),
+                "apply_stencil")
+
+    def apply_stencil_var_into(self, stencil: Stencil, bc: Boundaries, coeff: "Field", output: "Field"):
+        st = L.StencilC(stencil)
+        g, b = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_apply_stencil_var(self._bh(), self.h, output.h, coeff.h, C.byref(g), C.byref(st.c),
+                                               C.byref(b)), "apply_stencil_var")
+
+    def stencil_axpy(self, alpha: float, x: "Field", stencil: Stencil, bc: Boundaries):
+        """self += alpha·S(x) on interior points (field.rs:309-351)."""
+        st = L.StencilC(stencil)
+        g, b = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_stencil_axpy(self._bh(), alpha, x.h, self.h, C.byref(g), C.byref(st.c), C.byref(b)),
+                "stencil_axpy")
+
+    def weighted_jacobi(self, b: "Field", omega: float, stencil: Stencil, bc: Boundaries):
+        st = L.StencilC(stencil)
+        g, bb = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_weighted_jacobi(self._bh(), self.h, b.h, omega, C.byref(g), C.byref(st.c),
+                                             C.byref(bb)), "weighted_jacobi")
+
+    def apply_conv_diff_into(self, kappa: "Field", vx: "Field", vy: "Field", bc: Boundaries, output: "Field"):
+        g, b = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_apply_conv_diff(self._bh(), self.h, output.h, kappa.h, vx.h, vy.h, C.byref(g),
+                                             C.byref(b)), "apply_conv_diff")
+
+    def conv_diff_axpy(self, alpha: float, u: "Field", kappa: "Field", vx: "Field", vy: "Field", bc: Boundaries):
+        g, b = L.grid_c(self.grid), L.bcs_c(bc)
+        L.check(self._lib.fx_conv_diff_axpy(self._bh(), alpha, u.h, self.h, kappa.h, vx.h, vy.h, C.byref(g),
+                                            C.byref(b)), "conv_diff_axpy")
+
+
+def restrict(fine: Field, coarse: Field):
+    """Backend::restrict (core/src/backend.rs:240-246)."""
+    fg, cg = L.grid_c(fine.grid), L.grid_c(coarse.grid)
+    L.check(fine._lib.fx_restrict(fine._bh(), fine.h, coarse.h, C.byref(fg), C.byref(cg)), "restrict")
+
+
+def prolong(coarse: Field, fine: Field):
+    """Backend::prolong (core/src/backend.rs:254-260)."""
+    fg, cg = L.grid_c(fine.grid), L.grid_c(coarse.grid)
+    L.check(fine._lib.fx_prolong(fine._bh(), coarse.h, fine.h, C.byref(cg), C.byref(fg)), "prolong")
+
+
+def pipelined_cg_fused(alpha, beta, z, t, p, x, r, w, q):
+    """HipBackend::pipelined_cg_fused (backend-hip/src/backend.rs:469-504)."""
+    L.check(z._lib.fx_pipelined_cg_fused(z._bh(), alpha, beta, z.h, t.h, p.h, x.h, r.h, w.h, q.h),
+            "pipelined_cg_fused")
+
+
+# ── operators (extension) ─────────────────────────────────────────────
+class Operator:
+    """A·v = sigma·v + coeff·L(v) on interior points, identity rows on the boundary.
+
+    kind 0 reproduces the reference operator v + coeff·S(v) (cg.rs:104-111)."""
+
+    STENCIL, CONV_DIFF, VARK = 0, 1, 2
+
+    def __init__(self, kind, coeff, sigma=1.0, stencil=None, bc=None, kappa=None, kappa_scalar=0.0, vx=None,
+                 vy=None, k=None):
+        self.kind, self.coeff, self.sigma = kind, coeff, sigma
+        self._st = L.StencilC(stencil) if stencil is not None else None
+        self._bc = L.bcs_c(bc) if bc is not None else None
+        self._keep = (kappa, vx, vy, k)
+        self.c = L.FxOperator(
+            kind, 0, coeff, sigma,
+            C.pointer(self._st.c) if self._st else None,
+            C.pointer(self._bc) if self._bc else None,
+            kappa.h if kappa is not None else None, kappa_scalar,
+            vx.h if vx is not None else None, vy.h if vy is not None else None,
+            k.h if k is not None else None)
+
+    @staticmethod
+    def stencil(coeff: float, stencil: Stencil, bc: Boundaries) -> "Operator":
+        return Operator(Operator.STENCIL, coeff, 1.0, stencil=stencil, bc=bc)
+
+    @staticmethod
+    def conv_diff(vx: Field, vy: Field, kappa: Optional[Field] = None, kappa_scalar: float = 0.0,
+                  coeff: float = 1.0, sigma: float = 0.0) -> "Operator":
+        """sigma·u + coeff·(κΔu + v·∇⁻u) with the reference upwind discretisation (ops.rs:739-791)."""
+        return Operator(Operator.CONV_DIFF, coeff, sigma, kappa=kappa, kappa_scalar=kappa_scalar, vx=vx, vy=vy)
+
+    @staticmethod
+    def var_k(k: Field, coeff: float = 1.0, sigma: float = 0.0) -> "Operator":
+        """sigma·u + coeff·(−∇·(k∇u)), harmonic face averages of nodal k."""
+        return Operator(Operator.VARK, coeff, sigma, k=k)
+
+    def apply(self, x: Field, y: Field):
+        g = L.grid_c(x.grid)
+        L.check(x._lib.fx_apply_operator(x._bh(), C.byref(self.c), x.h, y.h, C.byref(g)), "apply_operator")
+
+
+# ── preconditioners ───────────────────────────────────────────────────
+class Identity:
+    """preconditioner.rs:29-37."""
+
+    def _pc(self):
+        return L.FxPrecond(0, 0, None)
+
+
+class Multigrid:
+    """Geometric multigrid V-cycle (multigrid.rs:12-608).
+
+    Multigrid(grid, bc, n_pre, n_post, omega) is Multigrid::new; the keyword
+    options select the extensions (transfer operators, coarsest size, FP32
+    levels, level operator sigma·I + scale·Δ_h)."""
+
+    def __init__(self, grid: Grid, bc: Boundaries, n_pre: int = 2, n_post: int = 2, omega: float = 2.0 / 3.0,
+                 smoother: SmootherKind = SmootherKind.JACOBI, *, transfer: Transfer = Transfer.REFERENCE,
+                 coarsest: int = 4, extra_coarse: int = 10, precision: DType = DType.F64, fused: bool = False,
+                 sigma: float = 0.0, scale: float = 1.0, be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        opt = L.FxMgOptions(n_pre, n_post, omega, int(smoother), int(transfer), coarsest, extra_coarse,
+                            int(precision), int(bool(fused)), sigma, scale, None)
+        g, b = L.grid_c(grid), L.bcs_c(bc)
+        h = C.c_void_p()
+        L.check(self._lib.fx_multigrid_create_ex(self.be.h, C.byref(g), C.byref(b), C.byref(opt), C.byref(h)),
+                "multigrid_create")
+        self.h = h
+
+    @staticmethod
+    def build(grid, bc, n_pre, n_post, omega, smoother, be=None) -> "Multigrid":
+        return Multigrid(grid, bc, n_pre, n_post, omega, smoother, be=be)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.fx_multigrid_destroy(h)
+            self.h = None
+
+    def depth(self) -> int:
+        return int(self._lib.fx_multigrid_depth(self.h))
+
+    def v_cycle(self, x: Field, b: Field):
+        L.check(self._lib.fx_multigrid_vcycle(self.h, x.h, b.h), "v_cycle")
+
+    def apply(self, z: Field, r: Field):
+        self.v_cycle(z, r)
+
+    def _pc(self):
+        return L.FxPrecond(1, 0, self.h)
+
+
+# ── solvers ───────────────────────────────────────────────────────────
+class _Solver:
+    KIND = 0
+    NAME = "?"
+
+    def __init__(self, grid: Grid, tol: float, max_iter: int, restart: int = 0, be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        self.tol = tol
+        self.max_iter = max_iter
+        g = L.grid_c(grid)
+        h = C.c_void_p()
+        L.check(self._lib.fx_solver_create(self.be.h, self.KIND, C.byref(g), restart, tol, max_iter, C.byref(h)),
+                "solver_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._lib.fx_solver_destroy(h)
+            self.h = None
+
+    def name(self) -> str:
+        return self._lib.fx_solver_name(self.h).decode()
+
+    def _stats(self, st: L.FxSolveStats) -> SolveStats:
+        n = self._lib.fx_solver_history(self.h, None, 0)
+        hist = (C.c_double * max(n, 1))()
+        self._lib.fx_solver_history(self.h, hist, n)
+        return SolveStats(int(st.iterations), st.residual, ConvergenceReason(st.reason), st.true_residual,
+                          list(hist[:n]))
+
+    def solve(self, x: Field, b: Field, coeff: float, stencil: Stencil, bc: Boundaries) -> SolveStats:
+        st = L.StencilC(stencil)
+        bb = L.bcs_c(bc)
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_solver_solve(self.h, x.h, b.h, coeff, C.byref(st.c), C.byref(bb), C.byref(out)),
+                "solve")
+        return self._stats(out)
+
+    def solve_preconditioned(self, x: Field, b: Field, coeff: float, stencil: Stencil, bc: Boundaries,
+                             pc) -> SolveStats:
+        st = L.StencilC(stencil)
+        bb = L.bcs_c(bc)
+        p = pc._pc()
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_solver_solve_preconditioned(self.h, x.h, b.h, coeff, C.byref(st.c), C.byref(bb),
+                                                         C.byref(p), C.byref(out)), "solve_preconditioned")
+        return self._stats(out)
+
+    def solve_op(self, x: Field, b: Field, op: Operator, pc=None) -> SolveStats:
+        p = pc._pc() if pc is not None else None
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_solver_solve_op(self.h, x.h, b.h, C.byref(op.c), C.byref(p) if p is not None else None,
+                                             C.byref(out)), "solve_op")
+        return self._stats(out)
+
+
+class CgSolver(_Solver):
+    """cg.rs:52-256."""
+    KIND = 0
+
+    def __init__(self, grid, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, 0, be)
+
+
+class PipelinedCgSolver(_Solver):
+    """pipelined_cg.rs:25-238 (+ preconditioned Ghysels–Vanroose variant)."""
+    KIND = 1
+
+    def __init__(self, grid, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, 0, be)
+
+
+class BiCgStabSolver(_Solver):
+    """bicgstab.rs:19-346."""
+    KIND = 2
+
+    def __init__(self, grid, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, 0, be)
+
+
+class GmresSolver(_Solver):
+    """gmres.rs:24-490: GmresSolver::new(grid, device, restart, tol, max_iter)."""
+    KIND = 3
+
+    def __init__(self, grid, restart, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, restart, be)
+
+
+class FcgSolver(_Solver):
+    """Flexible CG with FP64 true-residual refinement (extension)."""
+    KIND = 4
+
+    def __init__(self, grid, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, 0, be)

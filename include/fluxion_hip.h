@@ -1,0 +1,248 @@
+/*
+ * fluxion_hip.h — C ABI of libfluxion_hip.so, a gfx950 (MI355X) backend for
+ * fluxion's matrix-free structured-grid linear solve path.
+ *
+ * Every entry point replaces one method of the reference's backend plugin
+ * contract `fluxion_core::Backend` (crates/core/src/backend.rs:43-406) or one
+ * solver / preconditioner type of `fluxion_runtime` (crates/runtime/src/).
+ * The reference file:line each declaration stands in for is given beside it.
+ *
+ * Conventions
+ *   - Plain C types only: opaque handles, pointers, sizes.  No C++ or torch types.
+ *   - Every call returns an fx_status (0 = FX_OK).  On failure a message is
+ *     available from fx_last_error() (thread-local).  Every kernel launch is
+ *     checked, unlike the reference (backend-hip/src/backend.rs:293-307).
+ *   - All operations are ordered on the backend's single HIP stream.  Calls
+ *     that return a host value (reductions, downloads) synchronise the stream,
+ *     as the reference's reductions do (backend.rs:356-413).
+ *   - Grids are row-major, index = row*cols + col (core/src/grid.rs:59-61).
+ *   - Buffers are owned by the library; callers release them with
+ *     fx_buffer_free.  Input and output buffers of one call may alias only
+ *     where stated (pointwise ops with z == x or z == y).
+ *   - One backend per host thread, or an external lock (the reference relies
+ *     on a process-wide singleton: runtime/src/storage.rs:120-138).
+ */
+#ifndef FLUXION_HIP_H
+#define FLUXION_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ── status codes (core/src/error.rs:3-20) ───────────────────────────── */
+typedef enum {
+	FX_OK = 0,
+	FX_ERR_PERIODIC = 1,    /* ops.rs:11-23 reject_periodic */
+	FX_ERR_DIMENSION = 2,   /* CoreError::DimensionMismatch */
+	FX_ERR_GRID = 3,        /* CoreError::InvalidGrid (grid.rs:15-36) */
+	FX_ERR_STENCIL = 4,     /* CoreError::InvalidStencil (stencil.rs:23-30); also |offset| > 1 */
+	FX_ERR_ARGUMENT = 5,    /* null handle, wrong dtype, bad option */
+	FX_ERR_HIP = 6,         /* CoreError::BackendError from the HIP runtime */
+	FX_ERR_ALLOC = 7,       /* device allocation failed */
+	FX_ERR_NO_DEVICE = 8,   /* no HIP device / not gfx950 */
+	FX_ERR_UNSUPPORTED = 9
+} fx_status;
+
+typedef enum { FX_F32 = 0, FX_F64 = 1 } fx_dtype;            /* core/src/dtype.rs:1-16 */
+
+typedef struct fx_backend fx_backend;  /* HipBackend (backend-hip/src/backend.rs:61-79) */
+typedef struct fx_buf fx_buf;          /* HipBuffer (backend-hip/src/buffer.rs:16-123) */
+
+typedef struct { uint64_t rows, cols; double dx, dy; } fx_grid;            /* grid.rs:4-10 */
+typedef struct { int32_t dr, dc; double weight; } fx_stencil_entry;       /* stencil.rs:3-9 */
+typedef struct { uint32_t n; const fx_stencil_entry* entries; } fx_stencil; /* stencil.rs:16-18 */
+
+typedef enum { FX_BC_DIRICHLET = 0, FX_BC_NEUMANN = 1, FX_BC_ROBIN = 2, FX_BC_PERIODIC = 3 } fx_bc_kind;
+/* boundary.rs:1-15: Dirichlet(a) | Neumann(a) | Robin{alpha=a, beta=b, g} | Periodic */
+typedef struct { int32_t kind; double a, b, g; } fx_bc;
+typedef struct { fx_bc top, bottom, left, right; } fx_boundaries;          /* boundary.rs:18-24 */
+
+/* ── backend lifetime (backend.rs:103-264, storage.rs:120-131) ───────── */
+fx_status fx_backend_create(int device, fx_backend** out);
+void fx_backend_destroy(fx_backend* be);
+const char* fx_backend_name(const fx_backend* be);                        /* Backend::name backend.rs:47 */
+fx_status fx_synchronize(fx_backend* be);
+size_t fx_last_error(char* buf, size_t len);
+/* Raw HIP stream of the backend (hipStream_t), for callers that time or order work around it. */
+void* fx_backend_stream(fx_backend* be);
+
+/* ── memory (backend.rs:50-79, BackendBuffer 13-36) ──────────────────── */
+fx_status fx_allocate(fx_backend* be, size_t len, fx_dtype dtype, fx_buf** out);   /* zero-filled */
+fx_status fx_upload_f64(fx_backend* be, const double* data, size_t len, fx_buf** out);
+fx_status fx_upload_f32(fx_backend* be, const float* data, size_t len, fx_buf** out);
+void fx_buffer_free(fx_buf* buf);                                          /* Drop for HipBuffer */
+size_t fx_buffer_len(const fx_buf* buf);
+fx_dtype fx_buffer_dtype(const fx_buf* buf);
+void* fx_buffer_device_ptr(fx_buf* buf);
+fx_status fx_copy_to_host_f64(fx_backend* be, const fx_buf* buf, double* dst, size_t len);
+fx_status fx_copy_to_host_f32(fx_backend* be, const fx_buf* buf, float* dst, size_t len);
+/* Overwrite an existing buffer from host memory (upload without allocation). */
+fx_status fx_write_f64(fx_backend* be, fx_buf* buf, const double* src, size_t len);
+
+/* ── stencil operations (backend.rs:88-174) ──────────────────────────── */
+fx_status fx_apply_stencil(fx_backend* be, const fx_buf* input, fx_buf* output, const fx_grid* grid,
+                           const fx_stencil* st, const fx_boundaries* bc);            /* backend.rs:88-95 */
+fx_status fx_apply_stencil_var(fx_backend* be, const fx_buf* input, fx_buf* output, const fx_buf* coeff,
+                               const fx_grid* grid, const fx_stencil* st,
+                               const fx_boundaries* bc);                              /* backend.rs:148-156 */
+fx_status fx_stencil_axpy(fx_backend* be, double alpha, const fx_buf* x, fx_buf* y, const fx_grid* grid,
+                          const fx_stencil* st, const fx_boundaries* bc);             /* backend.rs:166-174 */
+
+/* ── BLAS-1 (backend.rs:101-136, 285-361) ────────────────────────────── */
+fx_status fx_copy(fx_backend* be, const fx_buf* src, fx_buf* dst);                   /* backend.rs:101-105 */
+fx_status fx_fill(fx_backend* be, fx_buf* buf, double value);                        /* backend.rs:111-115 */
+fx_status fx_axpy(fx_backend* be, double alpha, const fx_buf* x, fx_buf* y);         /* backend.rs:121-126 */
+fx_status fx_scale(fx_backend* be, fx_buf* buf, double alpha);                       /* backend.rs:132-136 */
+fx_status fx_waxpy(fx_backend* be, double alpha, const fx_buf* x, double beta, const fx_buf* y,
+                   fx_buf* w);                                                        /* backend.rs:309-316 */
+fx_status fx_aypx(fx_backend* be, double alpha, const fx_buf* x, fx_buf* y);         /* backend.rs:322-327 */
+fx_status fx_pointwise_mult(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* z); /* backend.rs:285-290 */
+fx_status fx_pointwise_div(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* z);  /* backend.rs:296-301 */
+fx_status fx_pointwise_max(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* z);  /* backend.rs:348-354 */
+fx_status fx_pointwise_min(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* z);  /* backend.rs:356-361 */
+fx_status fx_reciprocal(fx_backend* be, fx_buf* buf);                                /* backend.rs:333-336 */
+fx_status fx_abs_val(fx_backend* be, fx_buf* buf);                                   /* backend.rs:342-345 */
+
+/* ── reductions (backend.rs:180-230); results are copied to the host ── */
+fx_status fx_norm_l2(fx_backend* be, const fx_buf* buf, double* out);                /* backend.rs:180 */
+fx_status fx_reduce_sum(fx_backend* be, const fx_buf* buf, double* out);             /* backend.rs:188 */
+fx_status fx_reduce_max(fx_backend* be, const fx_buf* buf, double* out);             /* backend.rs:194 */
+fx_status fx_reduce_min(fx_backend* be, const fx_buf* buf, double* out);             /* backend.rs:200 */
+fx_status fx_dot(fx_backend* be, const fx_buf* x, const fx_buf* y, double* out);     /* backend.rs:206-210 */
+/* out[0] = a·b, out[1] = c·d in one pass (backend.rs:219-230; HIP dot2 backend-hip/src/backend.rs:1143-1210) */
+fx_status fx_dot2(fx_backend* be, const fx_buf* a, const fx_buf* b, const fx_buf* c, const fx_buf* d,
+                  double* out);
+
+/* ── multigrid primitives (backend.rs:240-277) ───────────────────────── */
+fx_status fx_restrict(fx_backend* be, const fx_buf* fine, fx_buf* coarse, const fx_grid* fine_grid,
+                      const fx_grid* coarse_grid);                                    /* backend.rs:240-246 */
+fx_status fx_prolong(fx_backend* be, const fx_buf* coarse, fx_buf* fine, const fx_grid* coarse_grid,
+                     const fx_grid* fine_grid);                                       /* backend.rs:254-260 */
+/* One damped sweep x += ω·D⁻¹·(b − S·x) on interior points (backend.rs:269-277).  The reference CPU
+ * sweep is in place and lexicographic (ops.rs:447-471); this one is a two-buffer Jacobi sweep: every
+ * point reads the previous iterate only. */
+fx_status fx_weighted_jacobi(fx_backend* be, fx_buf* x, const fx_buf* b, double omega, const fx_grid* grid,
+                             const fx_stencil* st, const fx_boundaries* bc);
+
+/* ── convection-diffusion (backend.rs:375-405) ───────────────────────── */
+fx_status fx_apply_conv_diff(fx_backend* be, const fx_buf* u, fx_buf* output, const fx_buf* kappa,
+                             const fx_buf* vx, const fx_buf* vy, const fx_grid* grid,
+                             const fx_boundaries* bc);                                /* backend.rs:375-384 */
+fx_status fx_conv_diff_axpy(fx_backend* be, double alpha, const fx_buf* u, fx_buf* output, const fx_buf* kappa,
+                            const fx_buf* vx, const fx_buf* vy, const fx_grid* grid,
+                            const fx_boundaries* bc);                                 /* backend.rs:395-405 */
+
+/* ── HIP-only extras (backend-hip/src/backend.rs:463-531) ────────────── */
+fx_status fx_zero_boundaries(fx_backend* be, fx_buf* buf, uint64_t rows, uint64_t cols); /* backend.rs:510-531 */
+/* z=q+βz; t=w+βt; p=r+βp; x+=αp; r−=αt; w−=αz (backend.rs:469-504, pipelined_cg.rs:172-200) */
+fx_status fx_pipelined_cg_fused(fx_backend* be, double alpha, double beta, fx_buf* z, fx_buf* t, fx_buf* p,
+                                fx_buf* x, fx_buf* r, fx_buf* w, const fx_buf* q);
+
+/* ════════════════════════════════════════════════════════════════════
+ * Solvers (crates/runtime/src/{cg,pipelined_cg,bicgstab,gmres}.rs)
+ * ════════════════════════════════════════════════════════════════════ */
+
+typedef enum { FX_CONVERGED = 0, FX_MAX_ITERATIONS = 1, FX_BREAKDOWN = 2 } fx_reason;  /* cg.rs:11-18 */
+typedef struct {                                                                       /* cg.rs:22-29 */
+	uint64_t iterations;
+	double residual;
+	int32_t reason;
+	int32_t pad;
+	/* Extension: ‖b − A·x‖₂ recomputed in FP64 after the solve (the reference never recomputes it). */
+	double true_residual;
+} fx_solve_stats;
+
+/* Operator A.  FX_OP_STENCIL is the reference operator A·v = v + coeff·S(v) with identity rows on the
+ * boundary (cg.rs:104-111, copy + stencil_axpy).  The other kinds are extensions with no reference
+ * solver counterpart: conv-diff sigma·v + coeff·(κΔv + w·∇⁻v) (ops.rs:674-791 with an identity weight)
+ * and variable coefficient sigma·v + coeff·(−∇·(k∇v)) with harmonic face averages of nodal k. */
+typedef enum { FX_OP_STENCIL = 0, FX_OP_CONV_DIFF = 1, FX_OP_VARK = 2 } fx_op_kind;
+typedef struct {
+	int32_t kind;
+	int32_t pad;
+	double coeff;
+	double sigma;                 /* identity weight (kinds 1, 2) */
+	const fx_stencil* stencil;    /* kind 0 */
+	const fx_boundaries* bc;      /* kind 0 */
+	const fx_buf* kappa;          /* kind 1: diffusivity field, or NULL for kappa_scalar */
+	double kappa_scalar;
+	const fx_buf* vx;             /* kind 1 */
+	const fx_buf* vy;             /* kind 1 */
+	const fx_buf* k;              /* kind 2: nodal coefficient */
+} fx_operator;
+
+/* Multigrid V-cycle preconditioner (multigrid.rs:12-608). */
+typedef struct fx_multigrid fx_multigrid;
+typedef enum { FX_SMOOTH_JACOBI = 0, FX_SMOOTH_CHEBYSHEV = 1 } fx_smoother;          /* multigrid.rs:12-20 */
+typedef enum { FX_TRANSFER_REFERENCE = 0, FX_TRANSFER_FW_BILINEAR = 1 } fx_transfer;
+typedef struct {
+	int32_t n_pre, n_post;
+	double omega;
+	int32_t smoother;        /* fx_smoother */
+	int32_t transfer;        /* FX_TRANSFER_REFERENCE: 2x2 average + injection, halving sizes (ops.rs:369-423,
+	                          * multigrid.rs:121-135).  FX_TRANSFER_FW_BILINEAR: every level spans the fine domain
+	                          * (sizes (n+1)/2 for odd n, n/2 for even n); linear interpolation and its transpose
+	                          * scaled by (h/H)², i.e. textbook full weighting + bilinear on nested 2^k+1 grids. */
+	int32_t coarsest;        /* stop coarsening once rows or cols <= coarsest (reference: 4, multigrid.rs:125) */
+	int32_t extra_coarse;    /* extra sweeps on the coarsest level (reference: 10, multigrid.rs:215-216) */
+	int32_t precision;       /* FX_F64 (reference) or FX_F32 (mixed-precision V-cycle) */
+	int32_t fused;           /* 1: fused FP32 kernels (smoother+residual+restriction); needs FX_F32, Jacobi, FW */
+	double sigma, scale;     /* level operator sigma·I + scale·Δ_h (reference: 0, 1 — multigrid.rs:105-106) */
+	const fx_buf* k;         /* optional nodal coefficient: level operator sigma·I + scale·(−∇·k∇) */
+} fx_mg_options;
+
+/* Multigrid::new(grid, bc, device, n_pre, n_post, omega) / Multigrid::build(.., smoother)
+ * (multigrid.rs:69-146): reference options. */
+fx_status fx_multigrid_create(fx_backend* be, const fx_grid* grid, const fx_boundaries* bc, uint32_t n_pre,
+                              uint32_t n_post, double omega, int32_t smoother, fx_multigrid** out);
+fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_boundaries* bc,
+                                 const fx_mg_options* opt, fx_multigrid** out);
+void fx_multigrid_destroy(fx_multigrid* mg);
+uint32_t fx_multigrid_depth(const fx_multigrid* mg);                                  /* multigrid.rs:172-175 */
+/* Multigrid::v_cycle(x, b) (multigrid.rs:188-230): x is the initial guess, overwritten. */
+fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b);
+
+/* Preconditioner handle (preconditioner.rs:10-37). */
+typedef enum { FX_PC_IDENTITY = 0, FX_PC_MULTIGRID = 1, FX_PC_LEARNED = 2 } fx_pc_kind;
+typedef struct { int32_t kind; int32_t pad; void* handle; } fx_precond;
+
+typedef struct fx_solver fx_solver;
+typedef enum {
+	FX_SOLVER_CG = 0,            /* CgSolver (cg.rs:52-256) */
+	FX_SOLVER_PIPELINED_CG = 1,  /* PipelinedCgSolver (pipelined_cg.rs:25-238) */
+	FX_SOLVER_BICGSTAB = 2,      /* BiCgStabSolver (bicgstab.rs:19-346) */
+	FX_SOLVER_GMRES = 3,         /* GmresSolver (gmres.rs:24-490) */
+	FX_SOLVER_FCG = 4            /* extension: flexible CG with FP64 true-residual refinement */
+} fx_solver_kind;
+
+/* XxxSolver::new(grid, device, [restart,] tol, max_iter): all work vectors allocated once
+ * (cg.rs:72-86, pipelined_cg.rs:50-70, bicgstab.rs:61-82, gmres.rs:56-81). restart is used by GMRES only. */
+fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, uint32_t restart, double tol,
+                           uint64_t max_iter, fx_solver** out);
+void fx_solver_destroy(fx_solver* s);
+/* LinearSolver::name (linear_solver.rs:30): "CG", "PipelinedCG", "BiCGSTAB", "GMRES", "FCG". */
+const char* fx_solver_name(const fx_solver* s);
+/* LinearSolver::solve(x, b, coeff, stencil, bc) (linear_solver.rs:13-27). */
+fx_status fx_solver_solve(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff, const fx_stencil* st,
+                          const fx_boundaries* bc, fx_solve_stats* stats);
+/* solve_preconditioned (cg.rs:178-256, bicgstab.rs:218-346, gmres.rs:294-454). */
+fx_status fx_solver_solve_preconditioned(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff,
+                                         const fx_stencil* st, const fx_boundaries* bc, const fx_precond* pc,
+                                         fx_solve_stats* stats);
+/* Extension: any operator kind, optional preconditioner (NULL = unpreconditioned). */
+fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_operator* op,
+                             const fx_precond* pc, fx_solve_stats* stats);
+/* Extension: per-iteration residual history of the last solve (entries copied: min(len, its+1)). */
+size_t fx_solver_history(const fx_solver* s, double* out, size_t len);
+
+/* Extension: A·x for an operator (same arithmetic as the solvers use). */
+fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y,
+                            const fx_grid* grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUXION_HIP_H */

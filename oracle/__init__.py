@@ -1,0 +1,354 @@
+"""ctypes wrapper of the CPU oracle (oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the fluxion_amd product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from fluxion_amd.core import Boundaries, ConvergenceReason, SolveStats, Stencil
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE), "-B", "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB_PATH
+
+
+class OStencil(C.Structure):
+    _fields_ = [("n", C.c_int32), ("dr", C.c_int32 * 32), ("dc", C.c_int32 * 32), ("w", C.c_double * 32)]
+
+
+class OBcs(C.Structure):
+    _fields_ = [("kind", C.c_int32 * 4), ("param", C.c_double * 12)]
+
+
+class OOp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("coeff", C.c_double), ("sigma", C.c_double),
+                ("st", OStencil), ("bc", OBcs), ("kappa", C.c_void_p), ("kappa_scalar", C.c_double),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("k", C.c_void_p)]
+
+
+class OMg(C.Structure):
+    _fields_ = [("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", C.c_double), ("smoother", C.c_int32),
+                ("transfer", C.c_int32), ("coarsest", C.c_int32), ("extra_coarse", C.c_int32),
+                ("precision", C.c_int32), ("pad", C.c_int32), ("sigma", C.c_double), ("scale", C.c_double),
+                ("bc", OBcs)]
+
+
+class OStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("residual", C.c_double), ("reason", C.c_int32), ("pad", C.c_int32)]
+
+
+# Smoother codes of the oracle's V-cycle.
+SMOOTH_REF_GS = 0      # in-place lexicographic sweep (ops.rs:447-471)
+SMOOTH_JACOBI = 1      # two-buffer Jacobi (the HIP library's smoother)
+SMOOTH_CHEBYSHEV = 2   # multigrid.rs:317-473
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        for name in ("orc_dot", "orc_norm_l2", "orc_reduce_sum", "orc_reduce_max", "orc_reduce_min"):
+            getattr(_lib, name).restype = C.c_double
+        for name in ("orc_axpy", "orc_scale", "orc_fill"):
+            pass
+        _lib.orc_axpy.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_int64]
+        _lib.orc_scale.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+        _lib.orc_fill.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+        _lib.orc_waxpy.argtypes = [C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int64]
+        _lib.orc_aypx.argtypes = [C.c_double, C.c_void_p, C.c_void_p, C.c_int64]
+        _lib.orc_dot.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        for name in ("orc_norm_l2", "orc_reduce_sum", "orc_reduce_max", "orc_reduce_min"):
+            getattr(_lib, name).argtypes = [C.c_void_p, C.c_int64]
+        _lib.orc_conv_diff.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                       C.c_double]
+        _lib.orc_vark_apply.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.c_int64, C.c_double, C.c_double]
+        _lib.orc_weighted_jacobi.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int64, C.c_int64, C.c_double,
+                                             C.c_double, C.POINTER(OStencil), C.c_int]
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous and a.dtype == np.float64
+    return a.ctypes.data
+
+
+def ostencil(st: Stencil) -> OStencil:
+    s = OStencil()
+    s.n = len(st.entries)
+    for i, e in enumerate(st.entries):
+        s.dr[i], s.dc[i], s.w[i] = e.dr, e.dc, e.weight
+    return s
+
+
+def obcs(bc: Boundaries) -> OBcs:
+    b = OBcs()
+    for i, s in enumerate(bc.sides()):
+        b.kind[i] = int(s.kind)
+        b.param[3 * i], b.param[3 * i + 1], b.param[3 * i + 2] = s.a, s.b, s.g
+    return b
+
+
+def mg_cfg(n_pre=2, n_post=2, omega=2.0 / 3.0, smoother=SMOOTH_JACOBI, transfer=0, coarsest=4, extra_coarse=10,
+           precision=0, sigma=0.0, scale=1.0, bc: Boundaries = None) -> OMg:
+    c = OMg()
+    c.n_pre, c.n_post, c.omega, c.smoother = n_pre, n_post, omega, smoother
+    c.transfer, c.coarsest, c.extra_coarse, c.precision = transfer, coarsest, extra_coarse, precision
+    c.sigma, c.scale = sigma, scale
+    c.bc = obcs(bc or Boundaries.zero_dirichlet())
+    return c
+
+
+class Op:
+    """Oracle operator (see orc_op in oracle.cpp)."""
+
+    def __init__(self, kind=0, coeff=1.0, sigma=1.0, stencil=None, bc=None, kappa=None, kappa_scalar=0.0,
+                 vx=None, vy=None, k=None):
+        self._keep = [a for a in (kappa, vx, vy, k) if a is not None]
+        self.c = OOp()
+        self.c.kind, self.c.coeff, self.c.sigma = kind, coeff, sigma
+        if stencil is not None:
+            self.c.st = ostencil(stencil)
+        self.c.bc = obcs(bc or Boundaries.zero_dirichlet())
+        self.c.kappa = _p(kappa) if kappa is not None else None
+        self.c.kappa_scalar = kappa_scalar
+        self.c.vx = _p(vx) if vx is not None else None
+        self.c.vy = _p(vy) if vy is not None else None
+        self.c.k = _p(k) if k is not None else None
+
+    @staticmethod
+    def stencil(coeff, st, bc=None):
+        return Op(0, coeff, 1.0, stencil=st, bc=bc)
+
+    @staticmethod
+    def conv_diff(vx, vy, kappa=None, kappa_scalar=0.0, coeff=1.0, sigma=0.0):
+        return Op(1, coeff, sigma, kappa=kappa, kappa_scalar=kappa_scalar, vx=vx, vy=vy)
+
+    @staticmethod
+    def var_k(k, coeff=1.0, sigma=0.0):
+        return Op(2, coeff, sigma, k=k)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+# ── ops ──────────────────────────────────────────────────────────────
+def apply_stencil(x, grid, st, bc):
+    out = np.zeros_like(x)
+    s, b = ostencil(st), obcs(bc)
+    _check(lib().orc_apply_stencil(C.c_void_p(_p(x)), C.c_void_p(_p(out)), C.c_int64(grid.rows),
+                                   C.c_int64(grid.cols), C.c_double(grid.dx), C.c_double(grid.dy), C.byref(s),
+                                   C.byref(b)), "apply_stencil")
+    return out
+
+
+def apply_stencil_var(x, coeff, grid, st, bc):
+    out = np.zeros_like(x)
+    s, b = ostencil(st), obcs(bc)
+    _check(lib().orc_apply_stencil_var(C.c_void_p(_p(x)), C.c_void_p(_p(out)), C.c_void_p(_p(coeff)),
+                                       C.c_int64(grid.rows), C.c_int64(grid.cols), C.c_double(grid.dx),
+                                       C.c_double(grid.dy), C.byref(s), C.byref(b)), "apply_stencil_var")
+    return out
+
+
+def stencil_axpy(alpha, x, y, grid, st, bc):
+    """Returns a new array y + alpha·S(x) (interior)."""
+    y = np.array(y, dtype=np.float64, copy=True)
+    s, b = ostencil(st), obcs(bc)
+    _check(lib().orc_stencil_axpy(C.c_double(alpha), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_int64(grid.rows),
+                                  C.c_int64(grid.cols), C.c_double(grid.dx), C.c_double(grid.dy), C.byref(s),
+                                  C.byref(b)), "stencil_axpy")
+    return y
+
+
+def axpy(alpha, x, y):
+    y = y.copy()
+    lib().orc_axpy(alpha, _p(x), _p(y), x.size)
+    return y
+
+
+def scale(x, alpha):
+    x = x.copy()
+    lib().orc_scale(_p(x), alpha, x.size)
+    return x
+
+
+def waxpy(alpha, x, beta, y):
+    w = np.empty_like(x)
+    lib().orc_waxpy(alpha, _p(x), beta, _p(y), _p(w), x.size)
+    return w
+
+
+def aypx(alpha, x, y):
+    y = y.copy()
+    lib().orc_aypx(alpha, _p(x), _p(y), x.size)
+    return y
+
+
+def pointwise(op, x, y):
+    z = np.empty_like(x)
+    lib().orc_pointwise(C.c_int(op), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_void_p(_p(z)), C.c_int64(x.size))
+    return z
+
+
+def dot(x, y):
+    return lib().orc_dot(_p(x), _p(y), x.size)
+
+
+def norm_l2(x):
+    return lib().orc_norm_l2(_p(x), x.size)
+
+
+def reduce_sum(x):
+    return lib().orc_reduce_sum(_p(x), x.size)
+
+
+def reduce_max(x):
+    return lib().orc_reduce_max(_p(x), x.size)
+
+
+def reduce_min(x):
+    return lib().orc_reduce_min(_p(x), x.size)
+
+
+def restrict(f, fg, cg, mode=0):
+    c = np.zeros(cg.rows * cg.cols)
+    lib().orc_restrict(C.c_void_p(_p(f)), C.c_void_p(_p(c)), C.c_int64(fg.rows), C.c_int64(fg.cols),
+                       C.c_int64(cg.rows), C.c_int64(cg.cols), C.c_int(mode))
+    return c
+
+
+def prolong(c, f, cg, fg, mode=0):
+    f = f.copy()
+    lib().orc_prolong(C.c_void_p(_p(c)), C.c_void_p(_p(f)), C.c_int64(cg.rows), C.c_int64(cg.cols),
+                      C.c_int64(fg.rows), C.c_int64(fg.cols), C.c_int(mode))
+    return f
+
+
+def weighted_jacobi(x, b, omega, grid, st, mode=0):
+    """mode 0: reference in-place lexicographic sweep; 1: two-buffer Jacobi."""
+    x = x.copy()
+    s = ostencil(st)
+    lib().orc_weighted_jacobi(_p(x), _p(b), omega, grid.rows, grid.cols, grid.dx, grid.dy, C.byref(s), mode)
+    return x
+
+
+def conv_diff(mode, alpha, u, kappa, vx, vy, grid, out=None, kappa_scalar=0.0, sigma=1.0):
+    out = np.zeros_like(u) if out is None else out.copy()
+    lib().orc_conv_diff(mode, alpha, _p(u), _p(out), _p(kappa) if kappa is not None else None, kappa_scalar,
+                        _p(vx), _p(vy), grid.rows, grid.cols, grid.dx, grid.dy, sigma)
+    return out
+
+
+def vark_apply(sigma, alpha, k, u, grid):
+    out = np.zeros_like(u)
+    lib().orc_vark_apply(sigma, alpha, _p(k), _p(u), _p(out), grid.rows, grid.cols, grid.dx, grid.dy)
+    return out
+
+
+def apply_op(op: Op, x, grid):
+    y = np.zeros_like(x)
+    _check(lib().orc_apply_op(C.byref(op.c), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_int64(grid.rows),
+                              C.c_int64(grid.cols), C.c_double(grid.dx), C.c_double(grid.dy)), "apply_op")
+    return y
+
+
+def zero_boundaries(x, rows, cols):
+    x = x.copy()
+    lib().orc_zero_boundaries(C.c_void_p(_p(x)), C.c_int64(rows), C.c_int64(cols))
+    return x
+
+
+def mg_depth(grid, cfg: OMg) -> int:
+    return lib().orc_mg_depth(C.c_int64(grid.rows), C.c_int64(grid.cols), C.c_double(grid.dx),
+                              C.c_double(grid.dy), C.byref(cfg))
+
+
+def vcycle(x, b, grid, cfg: OMg, n_cycles=1):
+    x = np.array(x, dtype=np.float64, copy=True)
+    lib().orc_vcycle(C.c_void_p(_p(x)), C.c_void_p(_p(b)), C.c_int64(grid.rows), C.c_int64(grid.cols),
+                     C.c_double(grid.dx), C.c_double(grid.dy), C.byref(cfg), C.c_int(n_cycles))
+    return x
+
+
+# ── solvers ──────────────────────────────────────────────────────────
+def _solve(fn, x, b, grid, op, tol, max_iter, extra=(), hist=True):
+    x = np.array(x, dtype=np.float64, copy=True)
+    st = OStats()
+    h = np.zeros(max(1, max_iter + 2)) if hist else None
+    args = [C.c_void_p(_p(x)), C.c_void_p(_p(b)), C.c_int64(grid.rows), C.c_int64(grid.cols),
+            C.c_double(grid.dx), C.c_double(grid.dy), C.byref(op.c)]
+    args += list(extra)
+    args += [C.c_double(tol), C.c_int64(max_iter), C.byref(st), C.c_void_p(_p(h)) if h is not None else None]
+    _check(fn(*args), fn.__name__)
+    n = min(int(st.iterations) + 1, h.size) if h is not None else 0
+    stats = SolveStats(int(st.iterations), st.residual, ConvergenceReason(st.reason),
+                       history=list(h[:n]) if h is not None else [])
+    return x, stats
+
+
+def _pc(mg):
+    return (C.c_int(1 if mg is not None else 0), C.byref(mg) if mg is not None else None)
+
+
+def cg(x, b, grid, op, tol, max_iter):
+    """CgSolver::solve (cg.rs:96-164)."""
+    return _solve(lib().orc_cg, x, b, grid, op, tol, max_iter)
+
+
+def pcg(x, b, grid, op, tol, max_iter, mg: OMg = None, identity=False):
+    """CgSolver::solve_preconditioned (cg.rs:178-256); mg=None & identity → Identity."""
+    kind = 0 if mg is None else 1
+    return _solve(lib().orc_pcg, x, b, grid, op, tol, max_iter,
+                  (C.c_int(kind), C.byref(mg) if mg is not None else None))
+
+
+def fcg(x, b, grid, op, tol, max_iter, mg: OMg = None):
+    return _solve(lib().orc_fcg, x, b, grid, op, tol, max_iter, _pc(mg))
+
+
+def pipelined_cg(x, b, grid, op, tol, max_iter):
+    return _solve(lib().orc_pipelined_cg, x, b, grid, op, tol, max_iter)
+
+
+def ppcg(x, b, grid, op, tol, max_iter, mg: OMg = None):
+    return _solve(lib().orc_ppcg, x, b, grid, op, tol, max_iter, _pc(mg))
+
+
+def bicgstab(x, b, grid, op, tol, max_iter, mg: OMg = None, identity=False):
+    if mg is None and identity:
+        extra = (C.c_int(3), None)   # any non-zero kind without a config = identity preconditioner
+    else:
+        extra = _pc(mg)
+    return _solve(lib().orc_bicgstab, x, b, grid, op, tol, max_iter, extra)
+
+
+def gmres(x, b, grid, op, restart, tol, max_iter, mg: OMg = None, identity=False):
+    if mg is None and identity:
+        pc = (C.c_int(3), None)
+    else:
+        pc = _pc(mg)
+    return _solve(lib().orc_gmres, x, b, grid, op, tol, max_iter, (C.c_int64(restart),) + tuple(pc))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,317 @@
+"""Krylov solvers and the multigrid V-cycle on the GPU against the CPU oracle.
+
+Parity bar for the FP64 paths: iteration counts within ±2 and solution relative
+L2 difference ≤ 1e-10 against the oracle on the same seeded inputs.  The
+V-cycle is bit-identical to the oracle's (same per-point arithmetic).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fluxion_amd.core import Boundaries, ConvergenceReason, DType, Grid, SmootherKind, Stencil, Transfer
+from fluxion_amd.inputs import gaussian_bumps, grf_log_k, streamfunction_velocity
+from tests.conftest import gaussian_ic, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ZD = Boundaries.zero_dirichlet()
+SOL_TOL = 1e-10
+IT_TOL = 2
+
+
+def H():
+    from fluxion_amd import hip
+    return hip
+
+
+def lap(h):
+    return Stencil.laplacian_2d_5pt(h, h)
+
+
+def poisson_c1(n, seed=20260101):
+    """Config-1 formulation: −Δu = f via coeff = 1, S = −Δ_h − I; f = 8 Gaussian bumps, zero on the boundary."""
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    f = gaussian_bumps(n, seed).ravel()
+    return g, h, f, Stencil.poisson_minus_identity(h, h), 1.0
+
+
+def cn_system(n, dt=0.1):
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    u = gaussian_ic(n, h)
+    half = 0.5 * dt
+    b = O.stencil_axpy(half, u, u, g, lap(h), ZD)
+    return g, h, u, b, -half
+
+
+def check_parity(x_gpu, st_gpu, x_ref, st_ref, sol_tol=SOL_TOL, it_tol=IT_TOL):
+    assert st_gpu.reason == st_ref.reason, (st_gpu, st_ref)
+    assert abs(st_gpu.iterations - st_ref.iterations) <= it_tol, (st_gpu.iterations, st_ref.iterations)
+    assert rel_l2(x_gpu, x_ref) <= sol_tol, rel_l2(x_gpu, x_ref)
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_cg_config1_parity(be, n):
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(n)
+    tol = 1e-8 * np.linalg.norm(f)
+    xr, sr = O.cg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), tol, 5000)
+    x = hip.Field.zeros(g, be=be)
+    b = hip.Field.from_f64(g, f, be)
+    s = hip.CgSolver(g, tol, 5000, be=be).solve(x, b, coeff, st, ZD)
+    print(f"cg n={n}: gpu {s.iterations} its, oracle {sr.iterations} its, rel diff {rel_l2(x.to_vec_f64(), xr):.2e}")
+    check_parity(x.to_vec_f64(), s, xr, sr)
+    assert s.true_residual < 1.01 * tol
+
+
+def test_cg_reference_cases(be):
+    hip = H()
+    # solvers.rs:80-97 zero RHS -> 0 iterations.
+    g = Grid.square(16, 1.0 / 15)
+    st = lap(g.dx)
+    s = hip.CgSolver(g, 1e-10, 100, be=be).solve(hip.Field.zeros(g, be=be), hip.Field.zeros(g, be=be), -0.01, st, ZD)
+    assert s.converged() and s.iterations == 0
+    # solvers.rs:1153-1176 max_iter = 2.
+    n = 32
+    g = Grid.square(n, 1.0 / (n - 1))
+    b = hip.Field.from_f64(g, gaussian_ic(n, g.dx), be)
+    s = hip.CgSolver(g, 1e-15, 2, be=be).solve(hip.Field.zeros(g, be=be), b, -0.01, lap(g.dx), ZD)
+    assert s.reason == ConvergenceReason.MAX_ITERATIONS and s.iterations == 2
+    # max_iter = 0 -> MaxIterations at 0 without a check (cg.rs:118 loop never runs).
+    s = hip.CgSolver(g, 1e30, 0, be=be).solve(hip.Field.zeros(g, be=be), b, -0.01, lap(g.dx), ZD)
+    assert s.reason == ConvergenceReason.MAX_ITERATIONS and s.iterations == 0
+    # Breakdown: A = I + 1·(−I) = 0 on the interior (zero-boundary RHS) -> p·Ap = 0.
+    f = gaussian_bumps(n).ravel()
+    s = hip.CgSolver(g, 1e-12, 50, be=be).solve(hip.Field.zeros(g, be=be), hip.Field.from_f64(g, f, be), 1.0,
+                                                Stencil([(0, 0, -1.0)]), ZD)
+    _, so = O.cg(np.zeros_like(f), f, g, O.Op.stencil(1.0, Stencil([(0, 0, -1.0)])), 1e-12, 50)
+    assert s.reason == ConvergenceReason.BREAKDOWN == so.reason and s.iterations == so.iterations == 0
+
+
+def test_cg_solver_reuse_and_history(be):
+    # solvers.rs:1182-1217: one solver, several solves, same answer.
+    hip = H()
+    g, h, u, b, coeff = cn_system(48)
+    solver = hip.CgSolver(g, 1e-9, 500, be=be)
+    B = hip.Field.from_f64(g, b, be)
+    outs = []
+    for _ in range(2):
+        x = hip.Field.from_f64(g, u, be)
+        s = solver.solve(x, B, coeff, lap(h), ZD)
+        outs.append((x.to_vec_f64(), s))
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1].iterations == outs[1][1].iterations
+    _, so = O.cg(u, b, g, O.Op.stencil(coeff, lap(h)), 1e-9, 500)
+    hist = outs[0][1].history
+    assert len(hist) == outs[0][1].iterations + 1
+    assert np.allclose(hist, so.history[:len(hist)], rtol=1e-8)
+
+
+def test_cg_nine_point_and_rectangular(be):
+    hip = H()
+    rows, cols = 61, 90
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    st = Stencil.laplacian_2d_9pt(g.dx, g.dy)
+    f = O.zero_boundaries(gaussian_bumps(rows, cols=cols).ravel(), rows, cols)
+    xr, sr = O.cg(np.zeros_like(f), f, g, O.Op.stencil(-2e-4, st), 1e-10, 2000)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.CgSolver(g, 1e-10, 2000, be=be).solve(x, hip.Field.from_f64(g, f, be), -2e-4, st, ZD)
+    check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+def test_pcg_identity_matches_cg(be):
+    hip = H()
+    g, h, u, b, coeff = cn_system(64)
+    B = hip.Field.from_f64(g, b, be)
+    x1, x2 = hip.Field.from_f64(g, u, be), hip.Field.from_f64(g, u, be)
+    s1 = hip.CgSolver(g, 1e-8, 500, be=be).solve(x1, B, coeff, lap(h), ZD)
+    s2 = hip.CgSolver(g, 1e-8, 500, be=be).solve_preconditioned(x2, B, coeff, lap(h), ZD, hip.Identity())
+    assert s1.converged() and s2.converged() and abs(s1.iterations - s2.iterations) <= 2
+    xr, sr = O.pcg(u, b, g, O.Op.stencil(coeff, lap(h)), 1e-8, 500)
+    check_parity(x2.to_vec_f64(), s2, xr, sr)
+
+
+@pytest.mark.parametrize("transfer,precision", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("shape", [(64, 64), (129, 97)])
+def test_vcycle_bitwise(be, transfer, precision, shape):
+    hip = H()
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    r = O.zero_boundaries(np.random.default_rng(7).standard_normal(rows * cols), rows, cols)
+    x0 = np.zeros(rows * cols)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer(transfer), coarsest=4,
+                       precision=DType.F32 if precision else DType.F64, be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=transfer, coarsest=4, precision=precision)
+    assert mg.depth() == O.mg_depth(g, cfg)
+    X = hip.Field.from_f64(g, x0, be)
+    mg.v_cycle(X, hip.Field.from_f64(g, r, be))
+    assert np.array_equal(X.to_vec_f64(), O.vcycle(x0, r, g, cfg))
+
+
+def test_vcycle_chebyshev_bitwise(be):
+    hip = H()
+    n = 64
+    g = Grid.square(n, 1.0 / (n - 1))
+    r = O.zero_boundaries(np.random.default_rng(8).standard_normal(n * n), n, n)
+    mg = hip.Multigrid(g, ZD, 3, 3, 2.0 / 3.0, SmootherKind.CHEBYSHEV, be=be)
+    X = hip.Field.zeros(g, be=be)
+    mg.v_cycle(X, hip.Field.from_f64(g, r, be))
+    ref = O.vcycle(np.zeros(n * n), r, g, O.mg_cfg(3, 3, 2.0 / 3.0, smoother=O.SMOOTH_CHEBYSHEV))
+    assert np.array_equal(X.to_vec_f64(), ref)
+
+
+def test_mg_depth(be):
+    hip = H()
+    for n, d in ((64, 5), (32, 4), (8, 2)):     # solvers.rs:814-840
+        assert hip.Multigrid(Grid.square(n, 1.0 / (n - 1)), ZD, 2, 2, 2.0 / 3.0, be=be).depth() == d
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_pcg_multigrid_parity(be, n):
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(n)
+    tol = 1e-10 * np.linalg.norm(f)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI)
+    xr, sr = O.pcg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), tol, 500, mg=cfg)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, be=be)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.CgSolver(g, tol, 500, be=be).solve_preconditioned(x, hip.Field.from_f64(g, f, be), coeff, st, ZD, mg)
+    print(f"pcg-mg n={n}: gpu {s.iterations}, oracle {sr.iterations}, diff {rel_l2(x.to_vec_f64(), xr):.2e}")
+    check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+def test_pipelined_cg_parity(be):
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(128)
+    tol = 1e-8 * np.linalg.norm(f)
+    xr, sr = O.pipelined_cg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), tol, 3000)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.PipelinedCgSolver(g, tol, 3000, be=be).solve(x, hip.Field.from_f64(g, f, be), coeff, st, ZD)
+    check_parity(x.to_vec_f64(), s, xr, sr)
+    # pipelined vs standard CG (solvers.rs:324-377)
+    x2 = hip.Field.zeros(g, be=be)
+    hip.CgSolver(g, tol, 3000, be=be).solve(x2, hip.Field.from_f64(g, f, be), coeff, st, ZD)
+    assert np.max(np.abs(x.to_vec_f64() - x2.to_vec_f64())) < 1e-6 * np.max(np.abs(x2.to_vec_f64()))
+
+
+def test_preconditioned_pipelined_cg(be):
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(128)
+    tol = 1e-10 * np.linalg.norm(f)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI)
+    xr, sr = O.ppcg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), tol, 500, mg=cfg)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, be=be)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.PipelinedCgSolver(g, tol, 500, be=be).solve_preconditioned(x, hip.Field.from_f64(g, f, be), coeff, st,
+                                                                       ZD, mg)
+    check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+@pytest.mark.parametrize("pre", [False, True])
+def test_bicgstab_parity(be, pre):
+    hip = H()
+    g, h, u, b, coeff = cn_system(96)
+    op = O.Op.stencil(coeff, lap(h))
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI) if pre else None
+    xr, sr = O.bicgstab(u, b, g, op, 1e-10, 500, mg=cfg)
+    x = hip.Field.from_f64(g, u, be)
+    B = hip.Field.from_f64(g, b, be)
+    solver = hip.BiCgStabSolver(g, 1e-10, 500, be=be)
+    if pre:
+        s = solver.solve_preconditioned(x, B, coeff, lap(h), ZD, hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, be=be))
+    else:
+        s = solver.solve(x, B, coeff, lap(h), ZD)
+    check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+@pytest.mark.parametrize("restart,pre", [(30, False), (5, False), (30, True)])
+def test_gmres_parity(be, restart, pre):
+    hip = H()
+    g, h, u, b, coeff = cn_system(80)
+    b = O.zero_boundaries(b, 80, 80)
+    op = O.Op.stencil(coeff, lap(h))
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI) if pre else None
+    tol = 1e-9
+    xr, sr = O.gmres(np.zeros_like(b), b, g, op, restart, tol, 3000, mg=cfg)
+    x = hip.Field.zeros(g, be=be)
+    B = hip.Field.from_f64(g, b, be)
+    solver = hip.GmresSolver(g, restart, tol, 3000, be=be)
+    if pre:
+        s = solver.solve_preconditioned(x, B, coeff, lap(h), ZD, hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, be=be))
+    else:
+        s = solver.solve(x, B, coeff, lap(h), ZD)
+    print(f"gmres({restart}, pre={pre}): gpu {s.iterations}, oracle {sr.iterations}")
+    check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+def test_gmres_max_iter_and_zero_rhs(be):
+    hip = H()
+    g, h, u, b, coeff = cn_system(40)
+    B = hip.Field.from_f64(g, O.zero_boundaries(b, 40, 40), be)
+    s = hip.GmresSolver(g, 10, 1e-14, 7, be=be).solve(hip.Field.zeros(g, be=be), B, coeff, lap(h), ZD)
+    _, so = O.gmres(np.zeros_like(b), O.zero_boundaries(b, 40, 40), g, O.Op.stencil(coeff, lap(h)), 10, 1e-14, 7)
+    assert s.reason == so.reason == ConvergenceReason.MAX_ITERATIONS and s.iterations == so.iterations == 7
+    s = hip.GmresSolver(g, 10, 1e-10, 50, be=be).solve(hip.Field.zeros(g, be=be), hip.Field.zeros(g, be=be), coeff,
+                                                       lap(h), ZD)
+    assert s.converged() and s.iterations == 0
+
+
+def test_conv_diff_bicgstab_gmres(be):
+    # Config-3 style operator −κΔu + v·∇⁻u at a small size, unpreconditioned, against the oracle.
+    hip = H()
+    n = 96
+    kappa = 1e-3
+    g = Grid.square(n, 1.0 / (n - 1))
+    vx, vy, pe = streamfunction_velocity(n, kappa)
+    vx, vy = vx.ravel(), vy.ravel()
+    f = gaussian_bumps(n).ravel()
+    tol = 1e-8 * np.linalg.norm(f)
+    op_r = O.Op.conv_diff(vx, vy, kappa_scalar=-kappa, coeff=1.0, sigma=0.0)
+    FX, FY = hip.Field.from_f64(g, vx, be), hip.Field.from_f64(g, vy, be)
+    op = hip.Operator.conv_diff(FX, FY, kappa_scalar=-kappa, coeff=1.0, sigma=0.0)
+    B = hip.Field.from_f64(g, f, be)
+    xr, sr = O.bicgstab(np.zeros_like(f), f, g, op_r, tol, 3000)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.BiCgStabSolver(g, tol, 3000, be=be).solve_op(x, B, op)
+    assert s.converged() and sr.converged()
+    assert abs(s.iterations - sr.iterations) <= max(IT_TOL, sr.iterations // 10)
+    assert rel_l2(x.to_vec_f64(), xr) < 1e-6
+    xr, sr = O.gmres(np.zeros_like(f), f, g, op_r, 30, tol, 3000)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.GmresSolver(g, 30, tol, 3000, be=be).solve_op(x, B, op)
+    assert s.converged() and sr.converged() and abs(s.iterations - sr.iterations) <= IT_TOL
+    assert rel_l2(x.to_vec_f64(), xr) < 1e-8
+
+
+def test_fcg_mixed_precision(be):
+    # FP32 V-cycle (full weighting, bilinear, coarsest 16) inside FP64 flexible CG to 1e-10.
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(256)
+    tol = 1e-10 * np.linalg.norm(f)
+    xr, _ = O.cg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), 1e-14 * np.linalg.norm(f), 20000)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                       scale=-1.0, be=be)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.FcgSolver(g, tol, 200, be=be).solve_preconditioned(x, hip.Field.from_f64(g, f, be), coeff, st, ZD, mg)
+    print(f"fcg f32: {s.iterations} its, true res {s.true_residual / np.linalg.norm(f):.2e}, "
+          f"err vs fp64 oracle {rel_l2(x.to_vec_f64(), xr):.2e}")
+    assert s.converged() and s.true_residual < tol
+    assert rel_l2(x.to_vec_f64(), xr) < 1e-7
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=-1.0)
+    _, sr = O.fcg(np.zeros_like(f), f, g, O.Op.stencil(coeff, st), tol, 200, mg=cfg)
+    assert abs(s.iterations - sr.iterations) <= IT_TOL
+
+
+def test_vark_operator_pcg(be):
+    # Variable coefficient −∇·(k∇u) with log-normal k: CG on the GPU against the oracle.
+    hip = H()
+    n = 96
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    f = gaussian_bumps(n).ravel()
+    tol = 1e-8 * np.linalg.norm(f)
+    xr, sr = O.cg(np.zeros_like(f), f, g, O.Op.var_k(k), tol, 5000)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.CgSolver(g, tol, 5000, be=be).solve_op(x, hip.Field.from_f64(g, f, be),
+                                                   hip.Operator.var_k(hip.Field.from_f64(g, k, be)))
+    check_parity(x.to_vec_f64(), s, xr, sr)
