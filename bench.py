@@ -97,11 +97,11 @@ def main():
     B = hip.Field.from_f64(g, f, be)
     X = hip.Field.zeros(g, be=be)
 
-    def mg(precision):
+    def mg(precision, fused):
         return hip.Multigrid(g, bc, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
-                             precision=precision, scale=-1.0, be=be)
+                             precision=precision, fused=fused, scale=-1.0, be=be)
 
-    mg64, mg32 = mg(DType.F64), mg(DType.F32)
+    mg64, mg32 = mg(DType.F64, False), mg(DType.F32, True)
     max_iter = 200
     pcg = hip.CgSolver(g, tol, max_iter, be=be)
     fcg = hip.FcgSolver(g, tol, max_iter, be=be)

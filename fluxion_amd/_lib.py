@@ -145,6 +145,20 @@ _SIGS = {
                                      _P(FxSolveStats)]),
     "fx_solver_history": (C.c_size_t, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "fx_apply_operator": (C.c_int, [c_be_p, _P(FxOperator), c_buf_p, c_buf_p, _P(FxGrid)]),
+    "fx_unet_create": (C.c_int, [c_be_p, C.c_int32, _P(C.c_void_p)]),
+    "fx_unet_destroy": (None, [C.c_void_p]),
+    "fx_unet_nparams": (C.c_size_t, [C.c_void_p]),
+    "fx_unet_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "fx_unet_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "fx_unet_get_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "fx_unet_forward": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_int32, C.c_int32, C.c_int32]),
+    "fx_unet_loss_grad": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_int32, C.c_int32, C.c_int32,
+                                    _P(C.c_double)]),
+    "fx_unet_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "fx_learned_create": (C.c_int, [c_be_p, C.c_void_p, C.c_void_p, c_buf_p, _P(FxGrid), C.c_double,
+                                    _P(C.c_void_p)]),
+    "fx_learned_destroy": (None, [C.c_void_p]),
+    "fx_learned_fell_back": (C.c_int32, [C.c_void_p]),
 }
 
 _lib = None

@@ -106,4 +106,8 @@ fx_status k_apply_operator(fx_backend* be, const fx_operator* op, const double* 
 // Host-returning dot of device arrays (synchronises).
 fx_status k_dot_host(fx_backend* be, const double* x, const double* y, size_t n, double* out);
 
+// Network inference on a device FP32 input [B][H][W][cin]; returns the FP32 output
+// (channel 0 of a [B][H][W][16] array) in *y.  All kernels return early while *skip != 0.
+fx_status unet_infer(fx_unet* u, const float* x, int B, int H, int W, const int* skip, const float** y);
+
 }  // namespace fx

@@ -11,6 +11,7 @@
 // The smoother is a two-buffer weighted Jacobi sweep with the ops.rs:454-469
 // point update; the reference CPU sweep is in place (lexicographic).
 #include "fx_mg.hpp"
+#include "fx_mgfast.hpp"
 
 #include <cmath>
 #include <utility>
@@ -23,7 +24,29 @@ template <class T> __device__ __forceinline__ T fmaT(T a, T b, T c);
 template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return fma(a, b, c); }
 template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
-template <class T> struct LW { T w[5]; T inv_diag; T omega; };
+template <class T> struct LW {
+	T w[5];
+	T inv_diag;
+	T omega;
+	// Variable-coefficient level operator sigma·u + scale·(−∇·(k∇u)) when k != nullptr.
+	const T* k;
+	T ix2, iy2, sigma, scale;
+};
+
+template <class T> __device__ __forceinline__ T hmT(T a, T b) { return T(2) * a * b / (a + b); }
+// sigma·u + scale·(−∇·(k∇u)) at interior point i with harmonic face means; *diag gets the diagonal.
+template <class T>
+__device__ __forceinline__ T vk_apply(const T* u, const LW<T>& L, int64_t i, int64_t cols, T* diag) {
+	const T* k = L.k;
+	T kc = k[i], c = u[i];
+	T kw = hmT<T>(kc, k[i - 1]), ke = hmT<T>(kc, k[i + 1]);
+	T kn = hmT<T>(kc, k[i - cols]), ks = hmT<T>(kc, k[i + cols]);
+	T fx = ke * (u[i + 1] - c) - kw * (c - u[i - 1]);
+	T fy = ks * (u[i + cols] - c) - kn * (c - u[i - cols]);
+	T Lv = -(fx * L.ix2 + fy * L.iy2);
+	*diag = fmaT<T>(L.scale, (kw + ke) * L.ix2 + (kn + ks) * L.iy2, L.sigma);
+	return fmaT<T>(L.scale, Lv, L.sigma * c);
+}
 
 #define SKIP_RETURN if (skip && *skip) return
 #define PT_INDEX                                                           \
@@ -84,6 +107,12 @@ __global__ __launch_bounds__(BLOCK) void k_jac(const T* xo, T* xn, const T* b, L
 	SKIP_RETURN;
 	PT_INDEX;
 	if (bnd) { xn[i] = xo[i]; return; }
+	if (L.k) {
+		T d;
+		T au = vk_apply<T>(xo, L, i, cols, &d);
+		xn[i] = fmaT<T>(L.omega, (b[i] - au) / d, xo[i]);
+		return;
+	}
 	T ax = T(0);
 	ax = fmaT<T>(L.w[0], xo[i], ax);
 	ax = fmaT<T>(L.w[1], xo[i - cols], ax);
@@ -119,6 +148,9 @@ __global__ __launch_bounds__(BLOCK) void k_lstencil(const T* u, const T* rhs, T*
 		else if (r == rows - 1) s = T(bcv(bc.bottom, double(u[i]), double(u[i - cols]), dy));
 		else if (c == 0) s = T(bcv(bc.left, double(u[i]), double(u[i + 1]), dx));
 		else s = T(bcv(bc.right, double(u[i]), double(u[i - 1]), dx));
+	} else if (L.k) {
+		T d;
+		s = vk_apply<T>(u, L, i, cols, &d);
 	} else {
 		s = fmaT<T>(L.w[0], u[i], fmaT<T>(L.w[1], u[i - cols], fmaT<T>(L.w[2], u[i + cols],
 		    fmaT<T>(L.w[3], u[i - 1], L.w[4] * u[i + 1]))));
@@ -212,6 +244,32 @@ dim3 pgrid(int64_t rows, int64_t cols) {
 }
 
 template <class T>
+__global__ void k_cvt_k(const double* k, T* out, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = T(k[i]);
+}
+// Coarse-level coefficient: injection for halving levels, linear interpolation
+// at the coarse node positions for same-domain levels.
+template <class T>
+__global__ void k_coarsen_k(const T* kf, T* kc, int64_t fr, int64_t fc, int64_t cr, int64_t cc, int lin) {
+	int64_t C = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	int64_t R = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (R >= cr || C >= cc) return;
+	if (!lin) {
+		int64_t a = 2 * R < fr - 1 ? 2 * R : fr - 1, b = 2 * C < fc - 1 ? 2 * C : fc - 1;
+		kc[R * cc + C] = kf[a * fc + b];
+		return;
+	}
+	Lin1 ly = lin1(R, cr, fr), lx = lin1(C, cc, fc);
+	int64_t a0 = ly.i0, a1 = ly.i0 + 1 < fr - 1 ? ly.i0 + 1 : fr - 1;
+	int64_t b0 = lx.i0, b1 = lx.i0 + 1 < fc - 1 ? lx.i0 + 1 : fc - 1;
+	T wy = T(ly.w), vy = T(1) - wy, wx = T(lx.w), vx = T(1) - wx;
+	T top = vx * kf[a0 * fc + b0] + wx * kf[a0 * fc + b1];
+	T bot = vx * kf[a1 * fc + b0] + wx * kf[a1 * fc + b1];
+	kc[R * cc + C] = vy * top + wy * bot;
+}
+
+template <class T>
 struct Driver {
 	fx_multigrid* mg;
 	const int* skip;
@@ -222,6 +280,11 @@ struct Driver {
 		for (int e = 0; e < 5; ++e) o.w[e] = T(L.w[e]);
 		o.inv_diag = T(L.inv_diag);
 		o.omega = T(mg->opt.omega);
+		o.k = (const T*)L.k;
+		o.ix2 = T(1.0 / (L.g.dx * L.g.dx));
+		o.iy2 = T(1.0 / (L.g.dy * L.g.dy));
+		o.sigma = T(mg->opt.sigma);
+		o.scale = T(mg->opt.scale);
 		return o;
 	}
 	size_t n(const fx_multigrid::Level& L) const { return (size_t)L.g.rows * L.g.cols; }
@@ -331,14 +394,31 @@ struct Driver {
 void free_levels(fx_multigrid* mg) {
 	for (auto& L : mg->lv) {
 		fx::dev_free(L.u); fx::dev_free(L.rhs); fx::dev_free(L.tmp); fx::dev_free(L.w0); fx::dev_free(L.w1);
+		fx::dev_free(L.k);
 	}
 	mg->lv.clear();
+}
+
+__global__ __launch_bounds__(BLOCK) void k_to_f32(const double* r, float* b, size_t n, const int* skip) {
+	SKIP_RETURN;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) b[i] = float(r[i]);
 }
 
 }  // namespace
 
 namespace fx {
 fx_status mg_apply(fx_multigrid* mg, double* z, const double* r, const int* skip) {
+	if (mg->opt.fused) {
+		// Fused cycle: the initial guess is taken as zero.
+		size_t n0 = (size_t)mg->lv[0].g.rows * mg->lv[0].g.cols;
+		hipStream_t st = mg->be->stream;
+		hipLaunchKernelGGL(k_to_f32, dim3(fx::elem_blocks(n0)), dim3(BLOCK), 0, st, r, (float*)mg->lv[0].rhs, n0, skip);
+		FX_CHECK(fmg::fast_cycle(mg, skip, fmg::NoEpi{}));
+		hipLaunchKernelGGL(k_cvt_out<float>, dim3(fx::elem_blocks(n0)), dim3(BLOCK), 0, st, (const float*)mg->lv[0].tmp, z,
+		                   n0, skip);
+		return fx::check_launch("mg fused");
+	}
 	if (mg->f32) return Driver<float>{mg, skip, mg->be->stream}.run(z, r);
 	return Driver<double>{mg, skip, mg->be->stream}.run(z, r);
 }
@@ -358,8 +438,11 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 	FX_REQUIRE(opt->smoother == FX_SMOOTH_JACOBI || opt->smoother == FX_SMOOTH_CHEBYSHEV, FX_ERR_ARGUMENT, "bad smoother");
 	FX_REQUIRE(opt->transfer == FX_TRANSFER_REFERENCE || opt->transfer == FX_TRANSFER_FW_BILINEAR, FX_ERR_ARGUMENT,
 	           "bad transfer");
-	FX_REQUIRE(opt->k == nullptr, FX_ERR_UNSUPPORTED, "variable-coefficient levels are built by the batched solver");
-	FX_REQUIRE(opt->fused == 0, FX_ERR_UNSUPPORTED, "fused V-cycle is selected through the solver");
+	if (opt->k) {
+		FX_CHECK(check_buf(opt->k, grid->rows * grid->cols, FX_F64, "k"));
+		FX_REQUIRE(opt->smoother == FX_SMOOTH_JACOBI, FX_ERR_UNSUPPORTED,
+		           "variable-coefficient levels use the Jacobi smoother");
+	}
 	fx_multigrid* mg = new fx_multigrid();
 	mg->be = be;
 	mg->opt = *opt;
@@ -383,6 +466,28 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 		void** bufs[5] = {&L.u, &L.rhs, &L.tmp, &L.w0, &L.w1};
 		int nb = opt->smoother == FX_SMOOTH_CHEBYSHEV ? 5 : 3;
 		for (int i = 0; i < nb && s == FX_OK; ++i) s = dev_alloc(be, bytes, bufs[i]);
+		if (s == FX_OK && opt->k) {
+			s = dev_alloc(be, bytes, &L.k);
+			if (s == FX_OK) {
+				size_t n = (size_t)g.rows * g.cols;
+				if (mg->lv.empty()) {
+					if (mg->f32) hipLaunchKernelGGL(k_cvt_k<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+					                                be->stream, (const double*)opt->k->ptr, (float*)L.k, n);
+					else hipLaunchKernelGGL(k_cvt_k<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+					                        be->stream, (const double*)opt->k->ptr, (double*)L.k, n);
+				} else {
+					const fx_multigrid::Level& F = mg->lv.back();
+					int lin = opt->transfer == FX_TRANSFER_FW_BILINEAR;
+					if (mg->f32) hipLaunchKernelGGL(k_coarsen_k<float>, pgrid(g.rows, g.cols), dim3(BLOCK), 0, be->stream,
+					                                (const float*)F.k, (float*)L.k, (int64_t)F.g.rows, (int64_t)F.g.cols,
+					                                (int64_t)g.rows, (int64_t)g.cols, lin);
+					else hipLaunchKernelGGL(k_coarsen_k<double>, pgrid(g.rows, g.cols), dim3(BLOCK), 0, be->stream,
+					                        (const double*)F.k, (double*)L.k, (int64_t)F.g.rows, (int64_t)F.g.cols,
+					                        (int64_t)g.rows, (int64_t)g.cols, lin);
+				}
+				s = check_launch("multigrid coefficient");
+			}
+		}
 		mg->lv.push_back(L);
 		if (s != FX_OK) { free_levels(mg); delete mg; return s; }
 		if (g.rows <= lim || g.cols <= lim) break;
@@ -398,6 +503,13 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 			if (cr < 3 || cc < 3) break;
 			g = fx_grid{cr, cc, g.dx * double(g.cols - 1) / double(cc - 1), g.dy * double(g.rows - 1) / double(cr - 1)};
 		}
+	}
+	if (opt->fused && !fmg::fused_ok(mg)) {
+		free_levels(mg);
+		delete mg;
+		set_error("the fused V-cycle needs FP32 levels, Jacobi smoothing, linear transfers, 1-2 pre/post sweeps "
+		          "and a coarsest level of at most 4096 points");
+		return FX_ERR_UNSUPPORTED;
 	}
 	*out = mg;
 	return FX_OK;

@@ -10,6 +10,7 @@ struct fx_multigrid {
 		void* tmp = nullptr;   // T[n]
 		void* w0 = nullptr;    // Chebyshev work (T[n])
 		void* w1 = nullptr;
+		void* k = nullptr;     // nodal coefficient (T[n]) of a variable-coefficient hierarchy
 		double w[5];           // sigma·I + scale·Δ_h weights, laplacian_2d_5pt order
 		double inv_diag;
 	};

@@ -16,6 +16,7 @@
 //   BiCgStabSolver::solve / solve_preconditioned bicgstab.rs:90-346
 //   GmresSolver::solve / solve_preconditioned    gmres.rs:116-490
 #include "fx_mg.hpp"
+#include "fx_mgfast.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -51,6 +52,10 @@ struct SState {
 	int32_t pad2;
 	double gbeta, gcur, ghk;
 	double *H, *cc, *ss, *grs, *y;
+	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once
+	// ‖r‖² fails to drop below stall2·(previous ‖r‖²).
+	int32_t lmode, lplain, lskip, pad3;
+	double stall2;
 };
 
 __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
@@ -59,6 +64,7 @@ __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
 __device__ __forceinline__ void finish(SState* s, int64_t its, double res, int reason) {
 	s->done = 1;
 	s->skip = 1;
+	s->lskip = 1;
 	s->reason = reason;
 	s->iter = its;
 	s->resid = res;
@@ -260,6 +266,10 @@ struct FinPcgRR {
 	SState* st;
 	__device__ void operator()(Acc* o) const {
 		double rr = acc_value(o[0]);
+		if (st->lmode && !st->lplain && rr > st->stall2 * st->rr) {
+			st->lplain = 1;
+			st->lskip = 1;
+		}
 		st->rr = rr;
 		int64_t it = st->iter + 1;
 		st->iter = it;
@@ -405,6 +415,114 @@ struct FinPP {
 		st->gprev = gamma;
 		st->aprev = alpha;
 	}
+};
+
+// ── learned preconditioner pieces ─────────────────────────────────────
+// r1 = r − A·z (in place over the A·z buffer), with Σ r1².
+struct FLr1 {
+	const double* r;
+	double* az;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		double v = r[i] - az[i];
+		az[i] = v;
+		acc_add(a[0], v * v);
+	}
+};
+// s = rms(r1) (1 when r1 = 0).
+struct FinScale {
+	double* s;
+	double n;
+	__device__ void operator()(Acc* o) const {
+		double v = sqrt(acc_value(o[0]) / n);
+		*s = v > 0.0 ? v : 1.0;
+	}
+};
+// Network input [p][2] = (r1/s, log k).
+struct FPackIn {
+	const double* r1;
+	const double* logk;
+	const double* s;
+	float* x;
+	double inv;
+	__device__ void load() { inv = 1.0 / *s; }
+	__device__ void operator()(size_t i) const {
+		x[2 * i] = float(r1[i] * inv);
+		x[2 * i + 1] = logk ? float(logk[i]) : 0.0f;
+	}
+};
+// z += s·h²·y on interior points (y = channel 0 of the [p][16] network output).
+struct FAddNet {
+	double* z;
+	const float* y;
+	const double* s;
+	double h2;
+	int64_t rows, cols;
+	double sc;
+	__device__ void load() { sc = *s * h2; }
+	__device__ void operator()(size_t i) const {
+		int64_t r = (int64_t)i / cols, c = (int64_t)i % cols;
+		if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) return;
+		z[i] += sc * double(y[16 * i]);
+	}
+};
+
+// ── fused mixed-precision FCG pieces ───────────────────────────────────
+// p_new = z + β·p_old with z held in FP32 (exact widening, same value as the FP64 copy).
+struct VPNewF {
+	const float* z;
+	const double* pold;
+	double be;
+	__device__ double load(int64_t i) const { return double(z[i]) + be * pold[i]; }
+};
+// r = b − ap with r·r, and the FP32 copy of r for the V-cycle.
+struct FResid32 {
+	const double *b, *ap;
+	double* r;
+	float* r32;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		double v = b[i] - ap[i];
+		r[i] = v;
+		r32[i] = float(v);
+		acc_add(a[0], v * v);
+	}
+};
+// x += α·p; r −= α·ap with r·r, and the FP32 copy of r.
+struct FXR32 {
+	const SState* st;
+	double* x;
+	double* r;
+	float* r32;
+	const double *p, *ap;
+	double al;
+	__device__ void load() { al = st->alpha; }
+	__device__ void operator()(size_t i, Acc* a) const {
+		x[i] = fma(al, p[i], x[i]);
+		double v = fma(-al, ap[i], r[i]);
+		r[i] = v;
+		r32[i] = float(v);
+		acc_add(a[0], v * v);
+	}
+};
+// Level-0 epilogues of the fused V-cycle: r·z (start) and (r·z, z·Ap) (iterations).
+struct EpiRZ {
+	static constexpr int NS = 1;
+	const double* r;
+	SState* st;
+	__device__ void term(int64_t j, float z, Acc* a) const { acc_add(a[0], r[j] * double(z)); }
+	__device__ void fin(Acc* o) const { FinPcgInitRZ{st}(o); }
+};
+struct EpiFcg {
+	static constexpr int NS = 2;
+	const double *r, *ap;
+	SState* st;
+	__device__ void term(int64_t j, float z, Acc* a) const {
+		double zd = double(z);
+		acc_add(a[0], r[j] * zd);
+		acc_add(a[1], zd * ap[j]);
+	}
+	__device__ void fin(Acc* o) const { FinFcgBeta{st}(o); }
 };
 
 // ── BiCGSTAB (bicgstab.rs:90-346) ──────────────────────────────────────
@@ -576,6 +694,7 @@ struct FinGmStart {
 	SState* st;
 	__device__ void operator()(Acc* o) const {
 		double beta = sqrt(acc_value(o[0]));
+		record(st, st->iter, beta);
 		if (beta < st->tol) { finish(st, st->iter, beta, FX_CONVERGED); return; }
 		// gmres.rs:155-164: the iteration budget is checked before the first Arnoldi step.
 		if (st->iter >= st->max_iter) { finish(st, st->iter, beta, FX_MAX_ITERATIONS); return; }
@@ -725,6 +844,20 @@ struct fx_solver {
 	}
 };
 
+// Learned preconditioner z = V(r) + s·h²·N([r1/s, log k]), r1 = r − A·V(r), s = rms(r1).
+struct fx_learned {
+	fx_backend* be = nullptr;
+	fx_unet* net = nullptr;
+	fx_multigrid* mg = nullptr;
+	const fx_buf* logk = nullptr;
+	fx_grid g{};
+	double stall = 0.9;
+	double* s_dev = nullptr;
+	double* r1 = nullptr;
+	float* xin = nullptr;
+	int fell_back = 0;
+};
+
 namespace {
 
 using fx::check_launch;
@@ -790,10 +923,27 @@ struct Ctx {
 			FX_CHECK(eupd(FZero{z}, "pc zero", sk));
 			return fx::mg_apply((fx_multigrid*)pc->handle, z, r, sk);
 		}
+		if (pc->kind == FX_PC_LEARNED) return learned((fx_learned*)pc->handle, z, r, sk, &d->lskip);
 		fx::set_error("preconditioner kind %d not supported here", pc->kind);
 		return FX_ERR_UNSUPPORTED;
 	}
 	fx_status precond(double* z, const double* r) { return precond(z, r, skip); }
+
+	// z = V(r), then the network correction unless *lsk is set.
+	fx_status learned(fx_learned* L, double* z, const double* r, const int* sk, const int* lsk) {
+		FX_CHECK(eupd(FZero{z}, "pc zero", sk));
+		FX_CHECK(fx::mg_apply(L->mg, z, r, sk));
+		FX_CHECK(apply_op(z, L->r1, lsk));
+		FX_CHECK(ered<1>(FLr1{r, L->r1}, FinScale{L->s_dev, (double)n}, "learned r1", lsk));
+		hipLaunchKernelGGL(k_eupd<FPackIn>, dim3(eb()), dim3(BLOCK), 0, st,
+		                   FPackIn{L->r1, L->logk ? (const double*)L->logk->ptr : nullptr, L->s_dev, L->xin, 0.0}, n, lsk);
+		FX_CHECK(check_launch("learned pack"));
+		const float* y = nullptr;
+		FX_CHECK(fx::unet_infer(L->net, L->xin, 1, (int)rows, (int)cols, lsk, &y));
+		hipLaunchKernelGGL(k_eupd<FAddNet>, dim3(eb()), dim3(BLOCK), 0, st,
+		                   FAddNet{z, y, L->s_dev, s->g.dx * s->g.dy, rows, cols, 0.0}, n, lsk);
+		return check_launch("learned add");
+	}
 
 	// p_out = src + β·p_old; ap = A·p_out; p·ap -> α (FinAlpha).
 	fx_status cg_step1(const double* src, const double* pold, double* pout, double* ap) {
@@ -830,8 +980,13 @@ fx_status make_ctx(fx_solver* s, const fx_operator* op, const fx_precond* pc, Ct
 		FX_CHECK(fx::stencil_args(op->stencil, &c->w9, nullptr));
 		c->fused5 = op->stencil->n == 5;
 	}
-	FX_REQUIRE(!pc || pc->kind == FX_PC_IDENTITY || pc->kind == FX_PC_MULTIGRID, FX_ERR_UNSUPPORTED,
-	           "preconditioner kind %d is not supported", pc ? pc->kind : -1);
+	FX_REQUIRE(!pc || pc->kind == FX_PC_IDENTITY || pc->kind == FX_PC_MULTIGRID || pc->kind == FX_PC_LEARNED,
+	           FX_ERR_UNSUPPORTED, "preconditioner kind %d is not supported", pc ? pc->kind : -1);
+	if (pc && pc->kind == FX_PC_LEARNED) {
+		fx_learned* L = (fx_learned*)pc->handle;
+		FX_REQUIRE(L && L->g.rows == s->g.rows && L->g.cols == s->g.cols, FX_ERR_DIMENSION,
+		           "learned preconditioner grid does not match the solver grid");
+	}
 	if (pc && pc->kind == FX_PC_MULTIGRID) {
 		fx_multigrid* mg = (fx_multigrid*)pc->handle;
 		FX_REQUIRE(mg && mg->lv[0].g.rows == s->g.rows && mg->lv[0].g.cols == s->g.cols, FX_ERR_DIMENSION,
@@ -841,9 +996,14 @@ fx_status make_ctx(fx_solver* s, const fx_operator* op, const fx_precond* pc, Ct
 }
 
 // Resets the device state for a new solve.
-fx_status reset_state(fx_solver* s, int64_t max_iter) {
+fx_status reset_state(fx_solver* s, int64_t max_iter, const fx_precond* pc = nullptr) {
 	SState h;
 	std::memset(&h, 0, sizeof h);
+	if (pc && pc->kind == FX_PC_LEARNED) {
+		h.lmode = 1;
+		double st = ((fx_learned*)pc->handle)->stall;
+		h.stall2 = st * st;
+	}
 	h.tol = s->tol;
 	h.max_iter = max_iter;
 	h.hist = s->dhist;
@@ -916,7 +1076,7 @@ fx_status solve_cg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	fx_solver* s = c.s;
 	double *r = s->vec(0), *p0 = s->vec(1), *p1 = s->vec(2), *ap = s->vec(3);
 	FX_REQUIRE(r && p0 && p1 && ap, FX_ERR_ALLOC, "work vector allocation failed");
-	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
+	FX_CHECK(reset_state(s, (int64_t)s->max_iter, c.pc));
 	FX_CHECK(c.apply_op(x, ap, nullptr));
 	FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
 	double* pp[2] = {p0, p1};
@@ -950,28 +1110,58 @@ fx_status solve_cg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 
 // ── FCG: flexible PCG with FP64 true-residual refinement (extension) ──
 fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	// With a fused FP32 multigrid and the 5-point stencil operator an iteration is
+	// the fused p-update/A·p/p·Ap step, the x/r update (which also writes the FP32
+	// residual) and the fused V-cycle, whose level-0 kernel produces z and the
+	// (r·z, z·Ap) reduction.
 	fx_solver* s = c.s;
 	double *r = s->vec(0), *p0 = s->vec(1), *p1 = s->vec(2), *ap = s->vec(3), *z = s->vec(4);
 	FX_REQUIRE(r && p0 && p1 && ap && z, FX_ERR_ALLOC, "work vector allocation failed");
+	fx_multigrid* fm = nullptr;
+	if (c.pc && c.pc->kind == FX_PC_MULTIGRID && c.fused5) {
+		fx_multigrid* m = (fx_multigrid*)c.pc->handle;
+		if (m->opt.fused) fm = m;
+	}
 	uint64_t used = 0;
 	fx_solve_stats st{};
 	double* pp[2] = {p0, p1};
 	for (int pass = 0; pass < 8; ++pass) {
-		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used)));
+		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used), c.pc));
 		FX_CHECK(c.apply_op(x, ap, nullptr));
 		FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
-		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "fcg init", nullptr));
-		FX_CHECK(c.precond(z, r));
-		FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "fcg r.z"));
-		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
-			double* pold = pp[i & 1];
-			double* pnew = pp[(i + 1) & 1];
-			FX_CHECK(c.cg_step1(z, pold, pnew, ap));
-			FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
+		if (fm) {
+			float* r32 = (float*)fm->lv[0].rhs;
+			const float* z32 = (const float*)fm->lv[0].tmp;
+			FX_CHECK(c.ered<1>(FResid32{b, ap, r, r32}, FinPcgInitRR{c.d}, "fcg init", nullptr));
+			FX_CHECK(fmg::fast_cycle(fm, c.skip, EpiRZ{r, c.d}));
+			FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+				double* pold = pp[i & 1];
+				double* pnew = pp[(i + 1) & 1];
+				int rpb = stream_rpb(c.rows, c.cols);
+				dim3 grid = fx::stream_grid(c.rows, c.cols, rpb);
+				FX_CHECK(fx::ensure_partials(c.be, (size_t)grid.x * grid.y));
+				hipLaunchKernelGGL((k_stream_st<1, VPNewF, EpiCG, FinAlpha>), grid, dim3(BLOCK), 0, c.st,
+				                   VPNewF{z32, pold, 0.0}, EpiCG{c.op->coeff, pnew, ap}, FinAlpha{c.d}, c.w9, c.rows,
+				                   c.cols, rpb, fx::red_scratch(c.be), (const SState*)c.d);
+				FX_CHECK(check_launch("fcg fused step"));
+				FX_CHECK(c.ered<1>(FXR32{c.d, x, r, r32, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
+				return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
+			}, (int64_t)(s->max_iter - used)));
+		} else {
+			FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "fcg init", nullptr));
 			FX_CHECK(c.precond(z, r));
-			return c.ered<2>(FDot2{r, z, z, ap}, FinFcgBeta{c.d}, "fcg r.z z.ap");
-		}, (int64_t)(s->max_iter - used)));
+			FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "fcg r.z"));
+			FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+				double* pold = pp[i & 1];
+				double* pnew = pp[(i + 1) & 1];
+				FX_CHECK(c.cg_step1(z, pold, pnew, ap));
+				FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
+				FX_CHECK(c.precond(z, r));
+				return c.ered<2>(FDot2{r, z, z, ap}, FinFcgBeta{c.d}, "fcg r.z z.ap");
+			}, (int64_t)(s->max_iter - used)));
+		}
 		FX_CHECK(fetch_stats(s, &st));
+		if (c.pc && c.pc->kind == FX_PC_LEARNED) ((fx_learned*)c.pc->handle)->fell_back |= s->hst->lplain;
 		used += st.iterations;
 		double tr = 0;
 		FX_CHECK(true_residual(c, x, b, ap, &tr));
@@ -1210,6 +1400,52 @@ fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_
 	}
 	set_error("solver kind %d not implemented", s->kind);
 	return FX_ERR_UNSUPPORTED;
+}
+
+fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, const fx_buf* logk, const fx_grid* grid,
+                            double stall, fx_learned** out) {
+	FX_REQUIRE(be && net && mg && out, FX_ERR_ARGUMENT, "null argument");
+	FX_CHECK(check_grid(grid));
+	FX_REQUIRE(grid->rows % 8 == 0 && grid->cols % 8 == 0, FX_ERR_ARGUMENT,
+	           "the network needs grid sides divisible by 8, got %llux%llu", (unsigned long long)grid->rows,
+	           (unsigned long long)grid->cols);
+	FX_REQUIRE(mg->lv[0].g.rows == grid->rows && mg->lv[0].g.cols == grid->cols, FX_ERR_DIMENSION,
+	           "multigrid grid does not match");
+	size_t n = grid->rows * grid->cols;
+	if (logk) FX_CHECK(check_buf(logk, n, FX_F64, "logk"));
+	fx_learned* L = new fx_learned();
+	L->be = be;
+	L->net = net;
+	L->mg = mg;
+	L->logk = logk;
+	L->g = *grid;
+	L->stall = stall;
+	void* p;
+	fx_status st;
+	if ((st = dev_alloc(be, sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->s_dev = (double*)p;
+	if ((st = dev_alloc(be, n * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->r1 = (double*)p;
+	if ((st = dev_alloc(be, n * 2 * sizeof(float), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->xin = (float*)p;
+	*out = L;
+	return FX_OK;
+}
+
+void fx_learned_destroy(fx_learned* L) {
+	if (!L) return;
+	if (L->be) (void)hipStreamSynchronize(L->be->stream);
+	dev_free(L->s_dev);
+	dev_free(L->r1);
+	dev_free(L->xin);
+	delete L;
+}
+
+int32_t fx_learned_fell_back(fx_learned* L) {
+	if (!L) return 0;
+	int v = L->fell_back;
+	L->fell_back = 0;
+	return v;
 }
 
 static fx_operator stencil_op(double coeff, const fx_stencil* st, const fx_boundaries* bc) {

@@ -345,12 +345,14 @@ class Multigrid:
     def __init__(self, grid: Grid, bc: Boundaries, n_pre: int = 2, n_post: int = 2, omega: float = 2.0 / 3.0,
                  smoother: SmootherKind = SmootherKind.JACOBI, *, transfer: Transfer = Transfer.REFERENCE,
                  coarsest: int = 4, extra_coarse: int = 10, precision: DType = DType.F64, fused: bool = False,
-                 sigma: float = 0.0, scale: float = 1.0, be: Optional[HipBackend] = None):
+                 sigma: float = 0.0, scale: float = 1.0, k: Optional["Field"] = None,
+                 be: Optional[HipBackend] = None):
         self.be = _backend(be)
         self._lib = self.be._lib
         self.grid = grid
+        self._k = k
         opt = L.FxMgOptions(n_pre, n_post, omega, int(smoother), int(transfer), coarsest, extra_coarse,
-                            int(precision), int(bool(fused)), sigma, scale, None)
+                            int(precision), int(bool(fused)), sigma, scale, k.h if k is not None else None)
         g, b = L.grid_c(grid), L.bcs_c(bc)
         h = C.c_void_p()
         L.check(self._lib.fx_multigrid_create_ex(self.be.h, C.byref(g), C.byref(b), C.byref(opt), C.byref(h)),

@@ -238,6 +238,38 @@ fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_
 /* Extension: per-iteration residual history of the last solve (entries copied: min(len, its+1)). */
 size_t fx_solver_history(const fx_solver* s, double* out, size_t len);
 
+/* ════════════════════════════════════════════════════════════════════
+ * Learned preconditioner network (extension; no reference counterpart).
+ * A 4-level U-Net (3×3 convs, 16/32/48/64 channels, BF16 weights and
+ * activations, FP32 accumulation on MFMA).  Tensors are FX_F32 buffers in
+ * NHWC order: x has B·H·W·cin values, y and t have B·H·W; H, W divisible by 8.
+ * Parameters: conv weights W_l[tap][cout][cin] in layer order, then biases
+ * (the layout of oracle/unet.cpp).
+ * ════════════════════════════════════════════════════════════════════ */
+typedef struct fx_unet fx_unet;
+fx_status fx_unet_create(fx_backend* be, int32_t cin, fx_unet** out);
+void fx_unet_destroy(fx_unet* u);
+size_t fx_unet_nparams(const fx_unet* u);
+fx_status fx_unet_set_params(fx_unet* u, const float* host, size_t n);   /* also resets the Adam state */
+fx_status fx_unet_get_params(fx_unet* u, float* host, size_t n);
+fx_status fx_unet_get_grad(fx_unet* u, float* host, size_t n);
+fx_status fx_unet_forward(fx_unet* u, const fx_buf* x, fx_buf* y, int32_t B, int32_t H, int32_t W);
+/* Σ_b ‖y_b − t_b‖²/‖t_b‖² and its gradient (kept on the device). */
+fx_status fx_unet_loss_grad(fx_unet* u, const fx_buf* x, const fx_buf* t, int32_t B, int32_t H, int32_t W,
+                            double* loss);
+fx_status fx_unet_adam_step(fx_unet* u, float lr, float beta1, float beta2, float eps);
+
+/* Learned preconditioner (FX_PC_LEARNED): z = V(r) + s·h²·N([r1/s, log k]) with r1 = r − A·V(r),
+ * s = rms(r1), V the multigrid V-cycle, N the network (h² = dx·dy, logk may be NULL for k ≡ 1).
+ * Inside a solve it falls back to z = V(r) for the remaining iterations once ‖r‖ fails to drop by the
+ * factor `stall` in one iteration (flexible CG judges by its FP64 residual). */
+typedef struct fx_learned fx_learned;
+fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, const fx_buf* logk, const fx_grid* grid,
+                            double stall, fx_learned** out);
+void fx_learned_destroy(fx_learned* l);
+/* 1 if a solve since the previous call switched to the plain V-cycle (and clears the flag). */
+int32_t fx_learned_fell_back(fx_learned* l);
+
 /* Extension: A·x for an operator (same arithmetic as the solvers use). */
 fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y,
                             const fx_grid* grid);

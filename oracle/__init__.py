@@ -42,7 +42,7 @@ class OMg(C.Structure):
     _fields_ = [("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", C.c_double), ("smoother", C.c_int32),
                 ("transfer", C.c_int32), ("coarsest", C.c_int32), ("extra_coarse", C.c_int32),
                 ("precision", C.c_int32), ("pad", C.c_int32), ("sigma", C.c_double), ("scale", C.c_double),
-                ("bc", OBcs)]
+                ("bc", OBcs), ("k", C.c_void_p)]
 
 
 class OStats(C.Structure):
@@ -107,8 +107,13 @@ def obcs(bc: Boundaries) -> OBcs:
 
 
 def mg_cfg(n_pre=2, n_post=2, omega=2.0 / 3.0, smoother=SMOOTH_JACOBI, transfer=0, coarsest=4, extra_coarse=10,
-           precision=0, sigma=0.0, scale=1.0, bc: Boundaries = None) -> OMg:
+           precision=0, sigma=0.0, scale=1.0, bc: Boundaries = None, k=None) -> OMg:
+    """Multigrid options; `k` (nodal coefficient, kept alive by the returned object) selects the
+    level operator sigma·I + scale·(−∇·k∇)."""
     c = OMg()
+    if k is not None:
+        c._k = np.ascontiguousarray(k, dtype=np.float64)
+        c.k = c._k.ctypes.data
     c.n_pre, c.n_post, c.omega, c.smoother = n_pre, n_post, omega, smoother
     c.transfer, c.coarsest, c.extra_coarse, c.precision = transfer, coarsest, extra_coarse, precision
     c.sigma, c.scale = sigma, scale
@@ -352,3 +357,44 @@ def gmres(x, b, grid, op, restart, tol, max_iter, mg: OMg = None, identity=False
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+# ── learned preconditioner network (unet.cpp) ─────────────────────────
+def unet_nparams(cin=2) -> int:
+    L = lib()
+    L.orc_unet_nparams.restype = C.c_size_t
+    return int(L.orc_unet_nparams(C.c_int(cin)))
+
+
+def unet_layout(cin=2):
+    woff = (C.c_int64 * 15)()
+    boff = (C.c_int64 * 15)()
+    lcin = (C.c_int * 15)()
+    lcout = (C.c_int * 15)()
+    lib().orc_unet_layout(C.c_int(cin), woff, boff, lcin, lcout)
+    return list(woff), list(boff), list(lcin), list(lcout)
+
+
+def unet_forward(params, x):
+    """x: (B, H, W, CIN) float64 -> (B, H, W)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    b, h, w, cin = x.shape
+    y = np.zeros((b, h, w))
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    lib().orc_unet_forward(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_int(b), C.c_int(h),
+                           C.c_int(w), C.c_int(cin))
+    return y
+
+
+def unet_loss_grad(params, x, t):
+    """Relative squared error summed over the batch, and its gradient w.r.t. the parameters."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    b, h, w, cin = x.shape
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    g = np.zeros_like(p)
+    L = lib()
+    L.orc_unet_loss_grad.restype = C.c_double
+    loss = L.orc_unet_loss_grad(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(t)), C.c_void_p(_p(g)),
+                                C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin))
+    return loss, g

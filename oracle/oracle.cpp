@@ -482,6 +482,7 @@ struct orc_mg_cfg {
 	int32_t pad;
 	double sigma, scale;   // level operator sigma·I + scale·Δ_h (reference: 0, 1)
 	orc_bcs bc;
+	const double* k;       // EXTENSION: nodal coefficient -> level operator sigma·I + scale·(−∇·k∇)
 };
 
 struct orc_stats { int64_t iterations; double residual; int32_t reason; int32_t pad; };
@@ -529,7 +530,43 @@ struct MgLevel {
 	Grid g;
 	Lvl5<T> L;
 	std::vector<T> u, rhs, tmp, w0, w1;
+	std::vector<T> k;   // variable-coefficient levels only
 };
+
+// EXTENSION: variable-coefficient level operator sigma·u + scale·(−∇·(k∇u)) at an interior
+// point, harmonic face means; `diag` receives its diagonal.
+template <class T> inline T hmT(T a, T b) { return T(2) * a * b / (a + b); }
+template <class T>
+inline T vk_apply(const T* u, const T* k, int64_t i, int64_t cols, T ix2, T iy2, T sigma, T scale, T* diag) {
+	T kc = k[i], c = u[i];
+	T kw = hmT<T>(kc, k[i - 1]), ke = hmT<T>(kc, k[i + 1]);
+	T kn = hmT<T>(kc, k[i - cols]), ks = hmT<T>(kc, k[i + cols]);
+	T fx = ke * (u[i + 1] - c) - kw * (c - u[i - 1]);
+	T fy = ks * (u[i + cols] - c) - kn * (c - u[i - cols]);
+	T L = -(fx * ix2 + fy * iy2);
+	*diag = fmaT<T>(scale, (kw + ke) * ix2 + (kn + ks) * iy2, sigma);
+	return fmaT<T>(scale, L, sigma * c);
+}
+
+// Coefficient on a coarse level: injection for halving levels, linear
+// interpolation at the coarse node positions for same-domain levels.
+template <class T>
+void coarsen_k(const T* kf, const Grid& fg, T* kc, const Grid& cg, int transfer) {
+	for (int64_t R = 0; R < cg.rows; ++R)
+		for (int64_t Cc = 0; Cc < cg.cols; ++Cc) {
+			if (transfer == 0) {
+				kc[R * cg.cols + Cc] = kf[std::min(2 * R, fg.rows - 1) * fg.cols + std::min(2 * Cc, fg.cols - 1)];
+				continue;
+			}
+			Lin1 ly = lin1(R, cg.rows, fg.rows), lx = lin1(Cc, cg.cols, fg.cols);
+			int64_t a0 = ly.i0, a1 = std::min(ly.i0 + 1, fg.rows - 1);
+			int64_t b0 = lx.i0, b1 = std::min(lx.i0 + 1, fg.cols - 1);
+			T wy = T(ly.w), vy = T(1) - wy, wx = T(lx.w), vx = T(1) - wx;
+			T top = vx * kf[a0 * fg.cols + b0] + wx * kf[a0 * fg.cols + b1];
+			T bot = vx * kf[a1 * fg.cols + b0] + wx * kf[a1 * fg.cols + b1];
+			kc[R * cg.cols + Cc] = vy * top + wy * bot;
+		}
+}
 
 template <class T>
 struct Mg {
@@ -552,6 +589,15 @@ struct Mg {
 			size_t n = (size_t)g.len();
 			L.u.assign(n, T(0)); L.rhs.assign(n, T(0)); L.tmp.assign(n, T(0));
 			L.w0.assign(n, T(0)); L.w1.assign(n, T(0));
+			if (c.k) {
+				L.k.assign(n, T(0));
+				if (lv.empty()) {
+					for (size_t i = 0; i < n; ++i) L.k[i] = T(c.k[i]);
+				} else {
+					const MgLevel<T>& F = lv.back();
+					coarsen_k(F.k.data(), F.g, L.k.data(), g, c.transfer);
+				}
+			}
 			lv.push_back(std::move(L));
 			if (g.rows <= lim || g.cols <= lim) break;
 			if (c.transfer == 0) {
@@ -573,12 +619,33 @@ struct Mg {
 		T om = T(cfg.omega);
 		if (cfg.smoother == 2) { smooth_cheby(l, n); return; }
 		for (int it = 0; it < n; ++it) {
-			if (cfg.smoother == 0) smooth_ref_gs<T>(L.u.data(), L.rhs.data(), om, L.g, L.L);
+			if (!L.k.empty()) {
+				smooth_jacobi_k(l, L.u.data(), L.tmp.data());
+				std::swap(L.u, L.tmp);
+			} else if (cfg.smoother == 0) smooth_ref_gs<T>(L.u.data(), L.rhs.data(), om, L.g, L.L);
 			else {
 				smooth_jacobi<T>(L.u.data(), L.tmp.data(), L.rhs.data(), om, L.g, L.L);
 				std::swap(L.u, L.tmp);
 			}
 		}
+	}
+
+	// EXTENSION: two-buffer Jacobi with the variable-coefficient level operator:
+	// x_new = fma(ω, (b − A·x)/diag, x).
+	void smooth_jacobi_k(int l, const T* xo, T* xn) {
+		MgLevel<T>& L = lv[l];
+		const int64_t rows = L.g.rows, cols = L.g.cols;
+		const T ix2 = T(1.0 / (L.g.dx * L.g.dx)), iy2 = T(1.0 / (L.g.dy * L.g.dy));
+		const T sg = T(cfg.sigma), sc = T(cfg.scale), om = T(cfg.omega);
+#pragma omp parallel for schedule(static)
+		for (int64_t r = 0; r < rows; ++r)
+			for (int64_t c = 0; c < cols; ++c) {
+				int64_t i = r * cols + c;
+				if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) { xn[i] = xo[i]; continue; }
+				T d;
+				T au = vk_apply<T>(xo, L.k.data(), i, cols, ix2, iy2, sg, sc, &d);
+				xn[i] = fmaT<T>(om, (L.rhs[i] - au) / d, xo[i]);
+			}
 	}
 
 	// apply_stencil on a level (5-pt path, ops.rs:891-938), boundary via bc_value.
@@ -599,10 +666,17 @@ struct Mg {
 			out[r * cols + lc] = bcv(bc.right, in[r * cols + lc], in[r * cols + lc - 1], L.g.dx);
 		}
 		const T* w = L.L.w;
+		const T ix2 = T(1.0 / (L.g.dx * L.g.dx)), iy2 = T(1.0 / (L.g.dy * L.g.dy));
+		const T sg = T(cfg.sigma), sc = T(cfg.scale);
 #pragma omp parallel for schedule(static)
 		for (int64_t r = 1; r < lr; ++r)
 			for (int64_t c = 1; c < lc; ++c) {
 				int64_t i = r * cols + c;
+				if (!L.k.empty()) {
+					T d;
+					out[i] = vk_apply<T>(in, L.k.data(), i, cols, ix2, iy2, sg, sc, &d);
+					continue;
+				}
 				out[i] = fmaT<T>(w[0], in[i], fmaT<T>(w[1], in[i - cols], fmaT<T>(w[2], in[i + cols],
 				         fmaT<T>(w[3], in[i - 1], w[4] * in[i + 1]))));
 			}

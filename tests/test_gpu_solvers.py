@@ -276,11 +276,15 @@ def test_conv_diff_bicgstab_gmres(be):
     assert s.converged() and sr.converged()
     assert abs(s.iterations - sr.iterations) <= max(IT_TOL, sr.iterations // 10)
     assert rel_l2(x.to_vec_f64(), xr) < 1e-6
-    xr, sr = O.gmres(np.zeros_like(f), f, g, op_r, 30, tol, 3000)
+    # Unpreconditioned GMRES(30) converges slowly at cell Péclet 50 (about 2500 steps to 1e-6):
+    # compare a fixed budget of 300 steps instead.
+    xr, sr = O.gmres(np.zeros_like(f), f, g, op_r, 30, tol, 300)
     x = hip.Field.zeros(g, be=be)
-    s = hip.GmresSolver(g, 30, tol, 3000, be=be).solve_op(x, B, op)
-    assert s.converged() and sr.converged() and abs(s.iterations - sr.iterations) <= IT_TOL
-    assert rel_l2(x.to_vec_f64(), xr) < 1e-8
+    s = hip.GmresSolver(g, 30, tol, 300, be=be).solve_op(x, B, op)
+    assert s.reason == sr.reason == ConvergenceReason.MAX_ITERATIONS and s.iterations == sr.iterations == 300
+    assert rel_l2(x.to_vec_f64(), xr) < 1e-10
+    # The oracle leaves the estimate after the last step unrecorded (gmres.rs records at the loop top).
+    assert np.allclose(s.history[:-1], sr.history[:len(s.history) - 1], rtol=1e-9)
 
 
 def test_fcg_mixed_precision(be):
@@ -303,15 +307,90 @@ def test_fcg_mixed_precision(be):
 
 
 def test_vark_operator_pcg(be):
-    # Variable coefficient −∇·(k∇u) with log-normal k: CG on the GPU against the oracle.
+    # Variable coefficient −∇·(k∇u) with log-normal k: MG-preconditioned CG (coefficient-aware FP64
+    # levels) on the GPU against the oracle.  (Unpreconditioned CG needs ~1300 irregular iterations
+    # here, where the summation order alone moves the count by a few.)
     hip = H()
     n = 96
     g = Grid.square(n, 1.0 / (n - 1))
     k = grf_log_k(n).ravel()
     f = gaussian_bumps(n).ravel()
-    tol = 1e-8 * np.linalg.norm(f)
-    xr, sr = O.cg(np.zeros_like(f), f, g, O.Op.var_k(k), tol, 5000)
+    tol = 1e-10 * np.linalg.norm(f)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, scale=1.0, k=k)
+    xr, sr = O.pcg(np.zeros_like(f), f, g, O.Op.var_k(k), tol, 500, mg=cfg)
+    K = hip.Field.from_f64(g, k, be)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, scale=1.0, k=K, be=be)
     x = hip.Field.zeros(g, be=be)
-    s = hip.CgSolver(g, tol, 5000, be=be).solve_op(x, hip.Field.from_f64(g, f, be),
-                                                   hip.Operator.var_k(hip.Field.from_f64(g, k, be)))
+    s = hip.CgSolver(g, tol, 500, be=be).solve_op(x, hip.Field.from_f64(g, f, be), hip.Operator.var_k(K), mg)
     check_parity(x.to_vec_f64(), s, xr, sr)
+
+
+@pytest.mark.parametrize("shape,npre,npost", [((64, 64), 2, 2), ((129, 97), 2, 2), ((200, 131), 1, 2),
+                                              ((96, 96), 2, 1), ((4096, 4096), 2, 2)])
+def test_fused_vcycle_bitwise(be, shape, npre, npost):
+    # The fused FP32 V-cycle (zero initial guess) equals the oracle's FP32 V-cycle bit for bit.
+    hip = H()
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    r = O.zero_boundaries(np.random.default_rng(11).standard_normal(rows * cols), rows, cols)
+    mg = hip.Multigrid(g, ZD, npre, npost, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
+                       precision=DType.F32, fused=True, scale=-1.0, be=be)
+    cfg = O.mg_cfg(npre, npost, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=-1.0)
+    X = hip.Field.zeros(g, be=be)
+    mg.v_cycle(X, hip.Field.from_f64(g, r, be))
+    assert np.array_equal(X.to_vec_f64(), O.vcycle(np.zeros(rows * cols), r, g, cfg))
+
+
+def test_fused_fcg_matches_unfused(be):
+    hip = H()
+    g, h, f, st, coeff = poisson_c1(512)
+    tol = 1e-10 * np.linalg.norm(f)
+    B = hip.Field.from_f64(g, f, be)
+    out = []
+    for fused in (False, True):
+        mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                           fused=fused, scale=-1.0, be=be)
+        x = hip.Field.zeros(g, be=be)
+        s = hip.FcgSolver(g, tol, 200, be=be).solve_preconditioned(x, B, coeff, st, ZD, mg)
+        assert s.converged() and s.true_residual < tol
+        out.append((x.to_vec_f64(), s))
+    assert abs(out[0][1].iterations - out[1][1].iterations) <= 1
+    assert rel_l2(out[0][0], out[1][0]) < 1e-9
+
+
+@pytest.mark.parametrize("precision,fused,transfer", [(0, False, 1), (1, False, 1), (1, True, 1), (0, False, 0)])
+def test_vark_vcycle_bitwise(be, precision, fused, transfer):
+    # Coefficient-aware levels (level operator −∇·(k∇·), coarse k interpolated) match the oracle bit for bit.
+    hip = H()
+    n = 129
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    r = O.zero_boundaries(np.random.default_rng(13).standard_normal(n * n), n, n)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer(transfer), coarsest=16,
+                       precision=DType.F32 if precision else DType.F64, fused=fused, scale=1.0,
+                       k=hip.Field.from_f64(g, k, be), be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=transfer, coarsest=16, precision=precision,
+                   scale=1.0, k=k)
+    X = hip.Field.zeros(g, be=be)
+    mg.v_cycle(X, hip.Field.from_f64(g, r, be))
+    assert np.array_equal(X.to_vec_f64(), O.vcycle(np.zeros(n * n), r, g, cfg))
+
+
+def test_vark_fcg_fused_mg(be):
+    # −∇·(k∇u) = f with log-normal k: FCG + fused FP32 coefficient-aware V-cycle against the oracle.
+    hip = H()
+    n = 256
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    f = gaussian_bumps(n).ravel()
+    tol = 1e-10 * np.linalg.norm(f)
+    K = hip.Field.from_f64(g, k, be)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                       fused=True, scale=1.0, k=K, be=be)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.FcgSolver(g, tol, 200, be=be).solve_op(x, hip.Field.from_f64(g, f, be), hip.Operator.var_k(K), mg)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=1.0, k=k)
+    xr, sr = O.fcg(np.zeros_like(f), f, g, O.Op.var_k(k), tol, 200, mg=cfg)
+    assert s.converged() and s.true_residual < tol
+    assert abs(s.iterations - sr.iterations) <= IT_TOL
+    assert rel_l2(x.to_vec_f64(), xr) < 1e-8

@@ -1,0 +1,70 @@
+"""The BF16/MFMA U-Net kernels against the FP64 oracle network (GPU)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def init_params(cin=2, seed=0):
+    woff, boff, lcin, lcout = O.unet_layout(cin)
+    p = np.zeros(O.unet_nparams(cin))
+    rng = np.random.default_rng(seed)
+    for i in range(15):
+        n = 9 * lcin[i] * lcout[i]
+        p[woff[i]:woff[i] + n] = rng.standard_normal(n) * np.sqrt(2.0 / (9 * lcin[i]))
+        p[boff[i]:boff[i] + lcout[i]] = 0.05 * rng.standard_normal(lcout[i])
+    return p
+
+
+def bf16_round(a):
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def test_unet_forward_and_gradient(be):
+    from fluxion_amd.learned import UNet
+    cin, B, H, W = 2, 2, 32, 48
+    p = bf16_round(init_params(cin, 3))
+    rng = np.random.default_rng(4)
+    x = bf16_round(rng.standard_normal((B, H, W, cin)))
+    t = rng.standard_normal((B, H, W))
+    net = UNet(be, cin)
+    net.set_params(p)
+    y = net.forward(x)
+    yr = O.unet_forward(p, x)
+    rel = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f"forward rel diff {rel:.3e}")
+    assert rel < 0.05
+    loss, g = net.loss_grad(x, t)
+    lr, gr = O.unet_loss_grad(p, x, t)
+    relg = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    cos = np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr))
+    print(f"loss {loss:.6f} vs {lr:.6f}; grad rel diff {relg:.3e}, cosine {cos:.6f}")
+    assert abs(loss - lr) < 0.05 * lr
+    assert cos > 0.995 and relg < 0.1
+    woff, boff, lcin, lcout = O.unet_layout(cin)
+    for i in range(15):   # every layer's weight gradient, not only the large ones
+        sl = slice(woff[i], woff[i] + 9 * lcin[i] * lcout[i])
+        c = np.dot(g[sl], gr[sl]) / (np.linalg.norm(g[sl]) * np.linalg.norm(gr[sl]) + 1e-300)
+        assert c > 0.98, (i, c)
+
+
+def test_unet_adam_decreases_loss(be):
+    from fluxion_amd.learned import UNet
+    cin, B, H, W = 2, 4, 16, 16
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((B, H, W, cin))
+    t = np.tanh(x[..., 0])
+    net = UNet(be, cin)
+    net.set_params(init_params(cin, 6))
+    l0, _ = net.loss_grad(x, t)
+    for _ in range(60):
+        net.loss_grad(x, t)
+        net.adam_step(2e-3)
+    l1, _ = net.loss_grad(x, t)
+    print(f"loss {l0:.4f} -> {l1:.4f}")
+    assert l1 < 0.5 * l0

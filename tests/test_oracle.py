@@ -468,3 +468,34 @@ def test_errors():
     g = Grid.square(8, 1.0 / 7)
     with pytest.raises(RuntimeError):
         O.apply_stencil(np.zeros(64), g, lap(1.0 / 7), Boundaries.uniform(BoundaryCondition.periodic()))
+
+
+def test_vark_multigrid_unit_k_matches_constant():
+    # With k ≡ 1 the variable-coefficient level operator is −Δ_h: the V-cycle agrees with the
+    # constant-coefficient one (scale −1) up to rounding.
+    n = 65
+    g = Grid.square(n, 1.0 / (n - 1))
+    r = O.zero_boundaries(np.random.default_rng(12).standard_normal(n * n), n, n)
+    a = O.vcycle(np.zeros(n * n), r, g, O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16, scale=-1.0))
+    b = O.vcycle(np.zeros(n * n), r, g, O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16, scale=1.0,
+                                                  k=np.ones(n * n)))
+    assert rel_l2(a, b) < 1e-12
+
+
+def test_vark_multigrid_pcg_converges():
+    # Log-normal k (correlation 0.1, log-variance 1): MG-PCG with coefficient-aware levels converges
+    # in far fewer iterations than CG and than MG built on the constant-coefficient Laplacian.
+    from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+    n = 129
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    f = gaussian_bumps(n).ravel()
+    op = O.Op.var_k(k)
+    tol = 1e-10 * np.linalg.norm(f)
+    _, s_cg = O.cg(np.zeros_like(f), f, g, op, tol, 5000)
+    x, s_k = O.pcg(np.zeros_like(f), f, g, op, tol, 300, mg=O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16,
+                                                                      scale=1.0, k=k))
+    _, s_c = O.pcg(np.zeros_like(f), f, g, op, tol, 300, mg=O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16,
+                                                                      scale=-1.0))
+    assert s_k.converged() and s_k.iterations < s_c.iterations < s_cg.iterations
+    assert np.linalg.norm(f - O.apply_op(op, x, g)) < tol
