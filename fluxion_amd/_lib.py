@@ -83,6 +83,8 @@ _SIGS = {
     "fx_synchronize": (C.c_int, [c_be_p]),
     "fx_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "fx_backend_stream": (C.c_void_p, [c_be_p]),
+    "fx_probe_set": (C.c_int, [c_be_p, C.c_int32]),
+    "fx_probe_read": (C.c_int, [c_be_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fx_allocate": (C.c_int, [c_be_p, C.c_size_t, C.c_int, _P(c_buf_p)]),
     "fx_upload_f64": (C.c_int, [c_be_p, C.c_void_p, C.c_size_t, _P(c_buf_p)]),
     "fx_upload_f32": (C.c_int, [c_be_p, C.c_void_p, C.c_size_t, _P(c_buf_p)]),

@@ -108,47 +108,67 @@ __device__ __forceinline__ void block_reduce(Acc (&v)[NS]) {
 // Reduction scratch handed to every reducing kernel.
 struct RedScratch {
 	Acc* partials;        // [nblocks * NS]
-	unsigned* counter;    // zero between kernels
 };
 
-// Writes this block's partial sums; the last block to arrive merges all
-// partials in block order and returns true in every one of its threads with
-// `out` valid in thread 0.  The counter is reset for the next kernel.
+// Writes this block's partial sums to partials[block * NS + k].  The global
+// sums are formed by k_finish in the next launch on the same stream: the
+// kernel boundary orders the partial stores before their loads, so no
+// device-scope fence or atomic is needed (on gfx950 such a fence writes back
+// the XCD's L2, which serialises a grid of thousands of workgroups).
 template <int NS>
-__device__ bool block_partial_and_last(Acc (&v)[NS], const RedScratch& rs, Acc (&out)[NS]) {
-	__shared__ int is_last;
-	const int t = tid_linear();
-	const int nb = nblocks_linear();
+__device__ __forceinline__ void block_partial(Acc (&v)[NS], const RedScratch& rs) {
 	block_reduce<NS>(v);
-	if (t == 0) {
+	if (tid_linear() == 0) {
 #pragma unroll
 		for (int k = 0; k < NS; ++k) rs.partials[bid_linear() * NS + k] = v[k];
-		__threadfence();
-		unsigned prev = atomicAdd(rs.counter, 1u);
-		is_last = (prev == (unsigned)(nb - 1));
 	}
-	__syncthreads();
-	if (!is_last) return false;
-	__threadfence();
+}
+
+// One workgroup of FIN_BLOCK threads merges `nb` block partials in a fixed
+// order (thread t takes blocks t, t+FIN_BLOCK, …, four loads in flight, then
+// the wave and workgroup trees) and runs fin(out) in thread 0.  Deterministic
+// for a fixed nb.  Skipped, like the kernel that wrote the partials, when
+// *skip is set.
+constexpr int FIN_BLOCK = 1024;
+template <int NS, class Fin>
+__global__ __launch_bounds__(FIN_BLOCK) void k_finish(RedScratch rs, unsigned nb, Fin fin, const int* skip) {
+	if (skip && *skip) return;
+	const int t = threadIdx.x;
 	Acc a[NS];
 #pragma unroll
 	for (int k = 0; k < NS; ++k) a[k] = Acc{0.0, 0.0};
-	for (int b = t; b < nb; b += BLOCK) {
+	unsigned b = t;
+	for (; b + 3 * FIN_BLOCK < nb; b += 4 * FIN_BLOCK) {
+		Acc p[4][NS];
 #pragma unroll
-		for (int k = 0; k < NS; ++k) {
-			Acc p;
-			p.hi = __builtin_nontemporal_load(&rs.partials[b * NS + k].hi);
-			p.lo = __builtin_nontemporal_load(&rs.partials[b * NS + k].lo);
-			acc_merge(a[k], p);
-		}
+		for (int u = 0; u < 4; ++u)
+#pragma unroll
+			for (int k = 0; k < NS; ++k) p[u][k] = rs.partials[(size_t)(b + u * FIN_BLOCK) * NS + k];
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+#pragma unroll
+			for (int k = 0; k < NS; ++k) acc_merge(a[k], p[u][k]);
 	}
-	block_reduce<NS>(a);
+	for (; b < nb; b += FIN_BLOCK) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) acc_merge(a[k], rs.partials[(size_t)b * NS + k]);
+	}
+	wave_reduce<NS>(a);
+	__shared__ Acc sh[NS][FIN_BLOCK / WAVE];
+	if ((t & (WAVE - 1)) == 0) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) sh[k][t / WAVE] = a[k];
+	}
+	__syncthreads();
 	if (t == 0) {
 #pragma unroll
-		for (int k = 0; k < NS; ++k) out[k] = a[k];
-		*rs.counter = 0u;
+		for (int k = 0; k < NS; ++k) {
+			Acc v = sh[k][0];
+			for (int w = 1; w < FIN_BLOCK / WAVE; ++w) acc_merge(v, sh[k][w]);
+			a[k] = v;
+		}
+		fin(a);
 	}
-	return true;
 }
 
 // ── streaming 5-point stencil ────────────────────────────────────────
@@ -196,12 +216,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream5(V v, E e, F fin, W9 w, int64_
 		up = cur;
 		cur = dn;
 	}
-	if constexpr (NS > 0) {
-		Acc out[NS];
-		if (block_partial_and_last<NS>(acc, rs, out)) {
-			if (tid_linear() == 0) fin(out);
-		}
-	}
+	if constexpr (NS > 0) block_partial<NS>(acc, rs);
 }
 
 // Rows per block for the streaming kernels: enough blocks to fill 256 CUs

@@ -19,9 +19,13 @@ struct fx_backend {
 	char name[96] = {0};
 	fxd::Acc* partials = nullptr;   // reduction partials
 	size_t partials_cap = 0;        // in Acc entries
-	unsigned* counter = nullptr;    // last-block counter (kept at zero between kernels)
 	double* dres = nullptr;         // device result slots for host-returning reductions
 	double* hres = nullptr;         // pinned host mirror of dres
+	// Kernel probe (fx_probe_set): event pairs around launches of one family.
+	int probe_kind = 0;
+	std::vector<hipEvent_t> probe_a, probe_b;
+	size_t probe_used = 0;
+	std::vector<float> probe_ms;
 };
 
 // HipBuffer analogue (backend-hip/src/buffer.rs:16-123).
@@ -65,7 +69,11 @@ fx_status check_launch(const char* what);
 
 // Grows the reduction partials buffer to hold `n` Acc entries.
 fx_status ensure_partials(fx_backend* be, size_t n);
-inline fxd::RedScratch red_scratch(fx_backend* be) { return fxd::RedScratch{be->partials, be->counter}; }
+// Event records around a launch of probe family `kind` (no-ops unless selected).
+void probe_begin(fx_backend* be, int kind);
+void probe_end(fx_backend* be, int kind);
+fx_status probe_drain(fx_backend* be);
+inline fxd::RedScratch red_scratch(fx_backend* be) { return fxd::RedScratch{be->partials}; }
 
 // Device allocation on the backend's device; zero-filled on the stream.
 fx_status dev_alloc(fx_backend* be, size_t bytes, void** out);

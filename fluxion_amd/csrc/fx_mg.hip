@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <utility>
+#include <vector>
 
 using namespace fxd;
 
@@ -169,26 +170,20 @@ __device__ __forceinline__ Lin1 lin1(int64_t i, int64_t nf, int64_t nc) {
 	int64_t den = nf - 1;
 	return {num / den, double(num % den) * (1.0 / double(den))};
 }
-__device__ __forceinline__ double lin_weight(int64_t i, int64_t I, int64_t nf, int64_t nc) {
-	Lin1 l = lin1(i, nf, nc);
-	if (l.i0 == I) return 1.0 - l.w;
-	if (l.i0 + 1 == I) return l.w;
+// Weight of fine index i toward coarse index I in the linear transfer
+// (tabulated split i0 + w): 1 − w, w or 0.
+__device__ __forceinline__ double lin_weight(const int32_t* i0, const double* w, int64_t i, int64_t I) {
+	const int64_t a = i0[i];
+	if (a == I) return 1.0 - w[i];
+	if (a + 1 == I) return w[i];
 	return 0.0;
-}
-__device__ __forceinline__ void lin_range(int64_t I, int64_t nf, int64_t nc, int64_t* lo, int64_t* hi) {
-	int64_t den = nc - 1;
-	int64_t a = (I - 1) * (nf - 1);
-	*lo = a <= 0 ? 1 : (a + den - 1) / den;
-	*hi = ((I + 1) * (nf - 1)) / den;
-	if (*lo < 1) *lo = 1;
-	if (*hi > nf - 2) *hi = nf - 2;
 }
 
 // Restriction: reference 2x2 average (ops.rs:369-399) or the transpose of the
 // linear prolongation scaled by (h/H)² (zero on the coarse boundary).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_restrict(const T* f, T* co, int64_t fr, int64_t fc, int64_t cr, int64_t cc,
-                                                    int lin, const int* skip) {
+                                                    int lin, fx_xtab X, const int* skip) {
 	SKIP_RETURN;
 	int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
 	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
@@ -201,14 +196,12 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(const T* f, T* co, int64_t f
 	}
 	if (r == 0 || col == 0 || r == cr - 1 || col == cc - 1) { co[r * cc + col] = T(0); return; }
 	const T sy = T(double(cr - 1) / double(fr - 1)), sx = T(double(cc - 1) / double(fc - 1));
-	int64_t rlo, rhi, clo, chi;
-	lin_range(r, fr, cr, &rlo, &rhi);
-	lin_range(col, fc, cc, &clo, &chi);
+	const int64_t rlo = X.lor[r], rhi = X.hir[r], clo = X.loc[col], chi = X.hic[col];
 	T acc = T(0);
 	for (int64_t i = rlo; i <= rhi; ++i) {
-		T wy = T(lin_weight(i, r, fr, cr));
+		T wy = T(lin_weight(X.i0r, X.wr, i, r));
 		T row = T(0);
-		for (int64_t j = clo; j <= chi; ++j) row += T(lin_weight(j, col, fc, cc)) * f[i * fc + j];
+		for (int64_t j = clo; j <= chi; ++j) row += T(lin_weight(X.i0c, X.wc, j, col)) * f[i * fc + j];
 		acc += wy * row;
 	}
 	co[r * cc + col] = acc * (sy * sx);
@@ -219,7 +212,7 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(const T* f, T* co, int64_t f
 // interpolation onto the fine interior.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_prolong(const T* co, T* f, int64_t cr, int64_t cc, int64_t fr, int64_t fc,
-                                                   int lin, const int* skip) {
+                                                   int lin, fx_xtab X, const int* skip) {
 	SKIP_RETURN;
 	int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
 	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
@@ -230,10 +223,9 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(const T* co, T* f, int64_t cr
 		return;
 	}
 	if (r == 0 || col == 0 || r == fr - 1 || col == fc - 1) return;
-	Lin1 ly = lin1(r, fr, cr), lx = lin1(col, fc, cc);
-	T wy = T(ly.w), vy = T(1) - wy, wx = T(lx.w), vx = T(1) - wx;
-	int64_t a0 = ly.i0, a1 = ly.i0 + 1 < cr ? ly.i0 + 1 : ly.i0;
-	int64_t b0 = lx.i0, b1 = lx.i0 + 1 < cc ? lx.i0 + 1 : lx.i0;
+	T wy = T(X.wr[r]), vy = T(1) - wy, wx = T(X.wc[col]), vx = T(1) - wx;
+	int64_t a0 = X.i0r[r], a1 = a0 + 1 < cr ? a0 + 1 : a0;
+	int64_t b0 = X.i0c[col], b1 = b0 + 1 < cc ? b0 + 1 : b0;
 	T top = vx * co[a0 * cc + b0] + wx * co[a0 * cc + b1];
 	T bot = vx * co[a1 * cc + b0] + wx * co[a1 * cc + b1];
 	f[r * fc + col] += vy * top + wy * bot;
@@ -374,7 +366,7 @@ struct Driver {
 			const fx_grid& cg = lv[l + 1].g;
 			hipLaunchKernelGGL(k_restrict<T>, pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st, (const T*)lv[l].tmp,
 			                   (T*)lv[l + 1].rhs, (int64_t)fg.rows, (int64_t)fg.cols, (int64_t)cg.rows, (int64_t)cg.cols,
-			                   fw, skip);
+			                   fw, lv[l].xt, skip);
 			size_t nc = n(lv[l + 1]);
 			hipLaunchKernelGGL(k_fillT<T>, dim3(eb(nc)), dim3(BLOCK), 0, st, (T*)lv[l + 1].u, T(0), nc, skip);
 		}
@@ -383,7 +375,7 @@ struct Driver {
 			const fx_grid& fg = lv[l].g;
 			const fx_grid& cg = lv[l + 1].g;
 			hipLaunchKernelGGL(k_prolong<T>, pgrid(fg.rows, fg.cols), dim3(BLOCK), 0, st, (const T*)lv[l + 1].u, (T*)lv[l].u,
-			                   (int64_t)cg.rows, (int64_t)cg.cols, (int64_t)fg.rows, (int64_t)fg.cols, fw, skip);
+			                   (int64_t)cg.rows, (int64_t)cg.cols, (int64_t)fg.rows, (int64_t)fg.cols, fw, lv[l].xt, skip);
 			FX_CHECK(smooth(l, o.n_post));
 		}
 		hipLaunchKernelGGL(k_cvt_out<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, (const T*)lv[0].u, z, n0, skip);
@@ -395,8 +387,79 @@ void free_levels(fx_multigrid* mg) {
 	for (auto& L : mg->lv) {
 		fx::dev_free(L.u); fx::dev_free(L.rhs); fx::dev_free(L.tmp); fx::dev_free(L.w0); fx::dev_free(L.w1);
 		fx::dev_free(L.k);
+		fx::dev_free(L.xbuf);
 	}
 	mg->lv.clear();
+}
+
+// Tabulates the linear transfer from level grid f to the next coarser grid c
+// (fine index i → coarse coordinate i·(nc−1)/(nf−1) = i0 + w, split exactly in
+// integers; coarse index I gathers fine interior indices lo..hi) and the LDS
+// region of the fused restriction's widest tile.
+fx_status build_xtab(fx_backend* be, fx_multigrid::Level& L, const fx_grid& c) {
+	const int64_t fr = (int64_t)L.g.rows, fc = (int64_t)L.g.cols, cr = (int64_t)c.rows, cc = (int64_t)c.cols;
+	FX_REQUIRE(fr < (int64_t)1 << 30 && fc < (int64_t)1 << 30, FX_ERR_GRID, "grid too large for transfer tables");
+	const size_t nd = (size_t)(fr + fc), ni = (size_t)(fr + fc + 2 * cr + 2 * cc);
+	std::vector<double> dv(nd);
+	std::vector<int32_t> iv(ni);
+	double* wr = dv.data();
+	double* wc = wr + fr;
+	int32_t* i0r = iv.data();
+	int32_t* i0c = i0r + fr;
+	int32_t* lor = i0c + fc;
+	int32_t* hir = lor + cr;
+	int32_t* loc = hir + cr;
+	int32_t* hic = loc + cc;
+	auto axis = [](int64_t nf, int64_t nc, int32_t* i0, double* w, int32_t* lo, int32_t* hi) {
+		const int64_t den = nf - 1, dc = nc - 1;
+		for (int64_t i = 0; i < nf; ++i) {
+			const int64_t num = i * (nc - 1);
+			i0[i] = (int32_t)(num / den);
+			w[i] = double(num % den) * (1.0 / double(den));
+		}
+		for (int64_t I = 0; I < nc; ++I) {
+			const int64_t a = (I - 1) * (nf - 1);
+			int64_t l = a <= 0 ? 1 : (a + dc - 1) / dc;
+			int64_t h = ((I + 1) * (nf - 1)) / dc;
+			if (l < 1) l = 1;
+			if (h > nf - 2) h = nf - 2;
+			lo[I] = (int32_t)l;
+			hi[I] = (int32_t)h;
+		}
+	};
+	axis(fr, cr, i0r, wr, lor, hir);
+	axis(fc, cc, i0c, wc, loc, hic);
+	auto span = [](int64_t nc, int tile, const int32_t* lo, const int32_t* hi) {
+		int64_t m = 1;
+		for (int64_t I0 = 0; I0 < nc; I0 += tile) {
+			const int64_t a = I0 > 1 ? I0 : 1, b = I0 + tile - 1 < nc - 2 ? I0 + tile - 1 : nc - 2;
+			if (a <= b && hi[b] >= lo[a] && hi[b] - lo[a] + 1 > m) m = hi[b] - lo[a] + 1;
+		}
+		return m;
+	};
+	void* p = nullptr;
+	FX_CHECK(fx::dev_alloc(be, nd * sizeof(double) + ni * sizeof(int32_t), &p));
+	L.xbuf = p;
+	FX_HIP(hipMemcpyAsync(p, dv.data(), nd * sizeof(double), hipMemcpyHostToDevice, be->stream));
+	FX_HIP(hipMemcpyAsync((char*)p + nd * sizeof(double), iv.data(), ni * sizeof(int32_t), hipMemcpyHostToDevice,
+	                      be->stream));
+	FX_HIP(hipStreamSynchronize(be->stream));
+	const double* dw = (const double*)p;
+	const int32_t* di = (const int32_t*)((char*)p + nd * sizeof(double));
+	fx_xtab& X = L.xt;
+	X.wr = dw;
+	X.wc = dw + fr;
+	X.i0r = di;
+	X.i0c = di + fr;
+	X.lor = X.i0c + fc;
+	X.hir = X.lor + cr;
+	X.loc = X.hir + cr;
+	X.hic = X.loc + cc;
+	X.sy = float(double(cr - 1) / double(fr - 1));
+	X.sx = float(double(cc - 1) / double(fc - 1));
+	X.ry = (int)span(cr, fmg::RCY, lor, hir) + 4;
+	X.rx = (int)span(cc, fmg::RCX, loc, hic) + 4;
+	return FX_OK;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_to_f32(const double* r, float* b, size_t n, const int* skip) {
@@ -502,6 +565,12 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 			uint64_t cc = g.cols % 2 ? (g.cols + 1) / 2 : g.cols / 2;
 			if (cr < 3 || cc < 3) break;
 			g = fx_grid{cr, cc, g.dx * double(g.cols - 1) / double(cc - 1), g.dy * double(g.rows - 1) / double(cr - 1)};
+		}
+	}
+	if (opt->transfer == FX_TRANSFER_FW_BILINEAR) {
+		for (size_t l = 0; l + 1 < mg->lv.size(); ++l) {
+			fx_status s = build_xtab(be, mg->lv[l], mg->lv[l + 1].g);
+			if (s != FX_OK) { free_levels(mg); delete mg; return s; }
 		}
 	}
 	if (opt->fused && !fmg::fused_ok(mg)) {

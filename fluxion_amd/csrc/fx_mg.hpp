@@ -2,6 +2,23 @@
 #pragma once
 #include "fx_internal.hpp"
 
+// Linear same-domain transfer from one level to the next coarser one,
+// tabulated per axis on the host so that kernels do no integer division.
+// Fine index i sits at coarse coordinate i·(nc−1)/(nf−1) = i0[i] + w[i];
+// coarse index I gathers the fine interior indices lo[I]..hi[I].
+struct fx_xtab {
+	const int32_t* i0r = nullptr;   // [fine rows]
+	const double* wr = nullptr;     // [fine rows]
+	const int32_t* i0c = nullptr;   // [fine cols]
+	const double* wc = nullptr;     // [fine cols]
+	const int32_t* lor = nullptr;   // [coarse rows]
+	const int32_t* hir = nullptr;
+	const int32_t* loc = nullptr;   // [coarse cols]
+	const int32_t* hic = nullptr;
+	float sy = 0.0f, sx = 0.0f;     // float((nc−1)/(nf−1)) per axis: the restriction scale (h/H)
+	int ry = 0, rx = 0;             // LDS region (halo included) of the fused restriction's widest tile
+};
+
 struct fx_multigrid {
 	struct Level {
 		fx_grid g;
@@ -13,6 +30,8 @@ struct fx_multigrid {
 		void* k = nullptr;     // nodal coefficient (T[n]) of a variable-coefficient hierarchy
 		double w[5];           // sigma·I + scale·Δ_h weights, laplacian_2d_5pt order
 		double inv_diag;
+		fx_xtab xt;            // transfer to the next coarser level (linear transfers only)
+		void* xbuf = nullptr;  // device storage behind xt
 	};
 	fx_backend* be = nullptr;
 	fx_mg_options opt{};

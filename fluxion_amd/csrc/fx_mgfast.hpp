@@ -1,12 +1,16 @@
 // fx_mgfast.hpp — fused FP32 V-cycle kernels (no reference counterpart).
 //
 // The V-cycle of multigrid.rs:188-230 with two-buffer Jacobi, linear same-domain
-// transfers and a zero initial guess, regrouped into three kernels per level:
-//   k_fpre           n_pre (1 or 2) sweeps from u = 0: reads b, writes u
-//   k_fres_restrict  coarse rhs = R(b − A·u): reads u, b, writes the coarse rhs
-//   k_fpost          u += P·e_c then n_post (1 or 2) sweeps in an LDS tile:
-//                    reads u, b, e_c, writes the level's result
-// and one single-workgroup kernel for the coarsest level's sweeps.  Each point
+// transfers and a zero initial guess, regrouped into two LDS-tiled kernels per
+// level:
+//   k_frestrict  coarse rhs = R(b − A·J^npre(0; b)): reads b (and k), writes the
+//                coarse rhs.  Pre-smoothing, residual and restriction happen in
+//                LDS; each fine residual is computed once per tile.
+//   k_fpost      out = J^npost(J^npre(0; b) + P·e_c): reads b (and k) and the
+//                coarse correction, writes the level's result.  The pre-smoothed
+//                iterate is recomputed from b in LDS rather than stored, so a
+//                level's iterate never travels through HBM.
+// plus one single-workgroup kernel for the coarsest level's sweeps.  Each point
 // is computed with exactly the operations of the unfused kernels (fx_mg.hip),
 // so the fused cycle is bit-identical to the oracle's FP32 V-cycle.
 #pragma once
@@ -44,31 +48,6 @@ __device__ __forceinline__ float jack(const LWf& L, float kc, float kn, float ks
 	float au = vkf(L, kc, kn, ks, kw, ke, c, n, s, w, e, &d);
 	return fmaf(L.omega, (b - au) / d, c);
 }
-
-struct Lin1 {
-	int64_t i0;
-	double w;
-};
-__device__ __forceinline__ Lin1 lin1(int64_t i, int64_t nf, int64_t nc) {
-	int64_t num = i * (nc - 1);
-	int64_t den = nf - 1;
-	return {num / den, double(num % den) * (1.0 / double(den))};
-}
-__device__ __forceinline__ double lin_weight(int64_t i, int64_t I, int64_t nf, int64_t nc) {
-	Lin1 l = lin1(i, nf, nc);
-	if (l.i0 == I) return 1.0 - l.w;
-	if (l.i0 + 1 == I) return l.w;
-	return 0.0;
-}
-__device__ __forceinline__ void lin_range(int64_t I, int64_t nf, int64_t nc, int64_t* lo, int64_t* hi) {
-	int64_t den = nc - 1;
-	int64_t a = (I - 1) * (nf - 1);
-	*lo = a <= 0 ? 1 : (a + den - 1) / den;
-	*hi = ((I + 1) * (nf - 1)) / den;
-	if (*lo < 1) *lo = 1;
-	if (*hi > nf - 2) *hi = nf - 2;
-}
-
 // Jacobi point update (fx_mg.hip k_jac): ax over (0,0),(−1,0),(1,0),(0,−1),(0,1).
 __device__ __forceinline__ float jac(const LWf& L, float c, float n, float s, float w, float e, float b) {
 	float ax = 0.0f;
@@ -91,75 +70,195 @@ __device__ __forceinline__ float jac0(const LWf& L, float b) {
 	float res = b - ax;
 	return fmaf(L.omega, L.inv_diag * res, 0.0f);
 }
-
-#define FMG_PT                                                              \
-	int64_t c = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));   \
-	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);       \
-	if (r >= rows || c >= cols) return;                                     \
-	int64_t i = r * cols + c;
-
-// n_pre sweeps (1 or 2) from a zero initial guess.
-__global__ __launch_bounds__(BLOCK) void k_fpre(const float* __restrict__ b, float* __restrict__ u, LWf L, int npre,
-                                                int64_t rows, int64_t cols, const int* skip) {
-	if (skip && *skip) return;
-	FMG_PT;
-	if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) { u[i] = 0.0f; return; }
-	if (L.k) {
-		const float* k = L.k;
-		auto first = [&](int64_t rr, int64_t cc, int64_t j) -> float {
-			if (rr == 0 || cc == 0 || rr == rows - 1 || cc == cols - 1) return 0.0f;
-			return jack(L, k[j], k[j - cols], k[j + cols], k[j - 1], k[j + 1], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, b[j]);
-		};
-		float uc = first(r, c, i);
-		if (npre == 1) { u[i] = uc; return; }
-		u[i] = jack(L, k[i], k[i - cols], k[i + cols], k[i - 1], k[i + 1], uc, first(r - 1, c, i - cols),
-		            first(r + 1, c, i + cols), first(r, c - 1, i - 1), first(r, c + 1, i + 1), b[i]);
-		return;
-	}
-	if (npre == 1) { u[i] = jac0(L, b[i]); return; }
-	auto u1 = [&](int64_t rr, int64_t cc, int64_t j) -> float {
-		return (rr == 0 || cc == 0 || rr == rows - 1 || cc == cols - 1) ? 0.0f : jac0(L, b[j]);
-	};
-	float bc = b[i];
-	float uc = jac0(L, bc);
-	u[i] = jac(L, uc, u1(r - 1, c, i - cols), u1(r + 1, c, i + cols), u1(r, c - 1, i - 1), u1(r, c + 1, i + 1), bc);
+// S(u) of the level operator (fx_mg.hip k_lstencil interior).
+__device__ __forceinline__ float sop(const LWf& L, float c, float n, float s, float w, float e) {
+	return fmaf(L.w[0], c, fmaf(L.w[1], n, fmaf(L.w[2], s, fmaf(L.w[3], w, L.w[4] * e))));
+}
+// Restriction weight of fine index i toward coarse index I (the transpose of
+// the linear interpolation), as float(1 − w) or float(w).
+__device__ __forceinline__ float xw(const int32_t* i0, const double* w, int64_t i, int64_t I) {
+	const int64_t a = i0[i];
+	if (a == I) return float(1.0 - w[i]);
+	if (a + 1 == I) return float(w[i]);
+	return 0.0f;
 }
 
-// Coarse rhs = restriction of the fine residual rhs − S(u) (fx_mg.hip k_lstencil mode 1 + k_restrict lin).
-__global__ __launch_bounds__(BLOCK) void k_fres_restrict(const float* __restrict__ u, const float* __restrict__ b,
-                                                         float* __restrict__ bc_out, LWf L, int64_t fr, int64_t fc,
-                                                         int64_t cr, int64_t cc, const int* skip) {
+// ── k_frestrict ──────────────────────────────────────────────────────
+// Coarse tile RCX × RCY (one coarse point per thread, RCX·RCY ≤ BLOCK); the
+// fine window it gathers from (about 2·RCX + 7 ≤ 64 columns, one wave per row)
+// is staged in LDS with a halo of 2 (pre-smoothing + stencil), together with
+// the window's transfer weights.
+constexpr int RCX = 28, RCY = 8;
+static_assert(RCX * RCY <= BLOCK, "one coarse point per thread");
+
+// Dynamic LDS bytes of k_frestrict for a level.
+inline size_t frestrict_lds(const fx_xtab& X, bool vk) {
+	size_t f = 3 * (size_t)X.ry * X.rx;                  // b, two work planes
+	if (vk) f += (size_t)(X.ry + 2) * (X.rx + 2);        // k with one more ring
+	f += 2 * (size_t)(X.ry + X.rx);                      // window weights (1 − w, w)
+	size_t i = (size_t)(X.ry + X.rx) + 2 * (RCY + RCX);  // window i0, tile lo/hi
+	return f * sizeof(float) + i * sizeof(int32_t);
+}
+
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
+                                                     fx_xtab X, int64_t fr, int64_t fc, int64_t cr, int64_t cc,
+                                                     const int* skip) {
 	if (skip && *skip) return;
-	int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
-	int64_t R = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
-	if (R >= cr || col >= cc) return;
-	if (R == 0 || col == 0 || R == cr - 1 || col == cc - 1) { bc_out[R * cc + col] = 0.0f; return; }
-	const float sy = float(double(cr - 1) / double(fr - 1)), sx = float(double(cc - 1) / double(fc - 1));
-	int64_t rlo, rhi, clo, chi;
-	lin_range(R, fr, cr, &rlo, &rhi);
-	lin_range(col, fc, cc, &clo, &chi);
-	float acc = 0.0f;
-	for (int64_t ii = rlo; ii <= rhi; ++ii) {
-		float wy = float(lin_weight(ii, R, fr, cr));
-		float row = 0.0f;
-		for (int64_t jj = clo; jj <= chi; ++jj) {
-			int64_t q = ii * fc + jj;
-			float s;
-			if (L.k) {
-				float d;
-				const float* k = L.k;
-				s = vkf(L, k[q], k[q - fc], k[q + fc], k[q - 1], k[q + 1], u[q], u[q - fc], u[q + fc], u[q - 1],
-				        u[q + 1], &d);
-			} else {
-				s = fmaf(L.w[0], u[q], fmaf(L.w[1], u[q - fc], fmaf(L.w[2], u[q + fc], fmaf(L.w[3], u[q - 1],
-				    L.w[4] * u[q + 1]))));
+	extern __shared__ float smem[];
+	const int RY = X.ry, RX = X.rx, KX = RX + 2;
+	float* sb = smem;
+	float* sa = sb + RY * RX;
+	float* su = sa + RY * RX;
+	float* sk = su + RY * RX;                    // VK only
+	float* wlr = sk + (VK ? (RY + 2) * KX : 0);  // [RY] float(1 − w) of the window's fine rows
+	float* whr = wlr + RY;                       // [RY] float(w)
+	float* wlc = whr + RY;                       // [RX]
+	float* whc = wlc + RX;
+	int32_t* i0r = (int32_t*)(whc + RX);         // [RY]
+	int32_t* i0c = i0r + RY;                     // [RX]
+	int32_t* tlr = i0c + RX;                     // [RCY] lo/hi of the tile's coarse rows
+	int32_t* thr = tlr + RCY;
+	int32_t* tlc = thr + RCY;                    // [RCX]
+	int32_t* thc = tlc + RCX;
+	const int t = threadIdx.x;
+	const int64_t I0 = (int64_t)blockIdx.y * RCY, J0 = (int64_t)blockIdx.x * RCX;
+	const int ly = t / RCX, lx = t % RCX;
+	const int64_t I = I0 + ly, J = J0 + lx;
+	const bool mine = t < RCX * RCY && I < cr && J < cc;
+	const int64_t Ia = I0 > 1 ? I0 : 1, Ib = I0 + RCY - 1 < cr - 2 ? I0 + RCY - 1 : cr - 2;
+	const int64_t Ja = J0 > 1 ? J0 : 1, Jb = J0 + RCX - 1 < cc - 2 ? J0 + RCX - 1 : cc - 2;
+	int64_t F0 = 0, F1 = -1, G0 = 0, G1 = -1;
+	if (Ia <= Ib && Ja <= Jb) {
+		F0 = X.lor[Ia];
+		F1 = X.hir[Ib];
+		G0 = X.loc[Ja];
+		G1 = X.hic[Jb];
+	}
+	if (F1 < F0 || G1 < G0) {  // only boundary coarse points in this tile
+		if (mine) bco[I * cc + J] = 0.0f;
+		return;
+	}
+	const int64_t R0 = F0 - 2, C0 = G0 - 2;  // region origin (fine indices)
+	const int ny = (int)(F1 - F0) + 5, nx = (int)(G1 - G0) + 5;
+	const int tx = t & (WAVE - 1), ty = t / WAVE;
+	for (int qy = ty; qy < ny; qy += NWAVES) {
+		const int64_t gr = R0 + qy;
+		for (int qx = tx; qx < nx; qx += WAVE) {
+			const int64_t gc = C0 + qx;
+			const bool in = gr >= 0 && gr < fr && gc >= 0 && gc < fc;
+			sb[qy * RX + qx] = in ? b[gr * fc + gc] : 0.0f;
+		}
+	}
+	if constexpr (VK) {
+		for (int qy = ty; qy < ny + 2; qy += NWAVES) {
+			const int64_t gr = R0 - 1 + qy;
+			for (int qx = tx; qx < nx + 2; qx += WAVE) {
+				const int64_t gc = C0 - 1 + qx;
+				const bool in = gr >= 0 && gr < fr && gc >= 0 && gc < fc;
+				sk[qy * KX + qx] = in ? L.k[gr * fc + gc] : 1.0f;
 			}
-			float res = fmaf(1.0f, b[q], -s);
-			row += float(lin_weight(jj, col, fc, cc)) * res;
+		}
+	}
+	// Transfer weights of the window's fine rows/columns and the tile's ranges.
+	for (int q = t; q < ny - 4; q += BLOCK) {
+		const int64_t f = F0 + q;
+		const double w = X.wr[f];
+		i0r[q] = X.i0r[f];
+		wlr[q] = float(1.0 - w);
+		whr[q] = float(w);
+	}
+	for (int q = t; q < nx - 4; q += BLOCK) {
+		const int64_t f = G0 + q;
+		const double w = X.wc[f];
+		i0c[q] = X.i0c[f];
+		wlc[q] = float(1.0 - w);
+		whc[q] = float(w);
+	}
+	if (t < RCY) {
+		const int64_t c = I0 + t;
+		tlr[t] = c < cr ? X.lor[c] : 0;
+		thr[t] = c < cr ? X.hir[c] : -1;
+	} else if (t >= WAVE && t < WAVE + RCX) {
+		const int64_t c = J0 + (t - WAVE);
+		tlc[t - WAVE] = c < cc ? X.loc[c] : 0;
+		thc[t - WAVE] = c < cc ? X.hic[c] : -1;
+	}
+	__syncthreads();
+	// 1 ≤ row ≤ fr − 2 and 1 ≤ col ≤ fc − 2, as 32-bit unsigned range tests.
+	const int iR0 = (int)R0 - 1, iC0 = (int)C0 - 1;
+	const unsigned ur = (unsigned)(fr - 2), uc = (unsigned)(fc - 2);
+	auto interior = [&](int qy, int qx) { return (unsigned)(iR0 + qy) < ur && (unsigned)(iC0 + qx) < uc; };
+	// k at (qy, qx) and its four neighbours, in the (c, n, s, w, e) order of vkf.
+#define FMG_K5(y, x) sk[((y) + 1) * KX + (x) + 1], sk[(y) * KX + (x) + 1], sk[((y) + 2) * KX + (x) + 1], \
+	sk[((y) + 1) * KX + (x)], sk[((y) + 1) * KX + (x) + 2]
+	// Pre-smoothing sweep 1 from zero over the whole region.
+	float* s1 = NPRE == 2 ? sa : su;
+	for (int qy = ty; qy < ny; qy += NWAVES) {
+		for (int qx = tx; qx < nx; qx += WAVE) {
+			const int q = qy * RX + qx;
+			float v = 0.0f;
+			if (interior(qy, qx)) {
+				if constexpr (VK) v = jack(L, FMG_K5(qy, qx), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[q]);
+				else v = jac0(L, sb[q]);
+			}
+			s1[q] = v;
+		}
+	}
+	__syncthreads();
+	if constexpr (NPRE == 2) {
+		for (int qy = 1 + ty; qy < ny - 1; qy += NWAVES) {
+			for (int qx = 1 + tx; qx < nx - 1; qx += WAVE) {
+				const int q = qy * RX + qx;
+				float v = 0.0f;
+				if (interior(qy, qx)) {
+					if constexpr (VK) v = jack(L, FMG_K5(qy, qx), sa[q], sa[q - RX], sa[q + RX], sa[q - 1], sa[q + 1], sb[q]);
+					else v = jac(L, sa[q], sa[q - RX], sa[q + RX], sa[q - 1], sa[q + 1], sb[q]);
+				}
+				su[q] = v;
+			}
+		}
+		__syncthreads();
+	}
+	// Residual b − S(u) on the fine window (all fine-interior points).
+	for (int qy = 2 + ty; qy < ny - 2; qy += NWAVES) {
+		for (int qx = 2 + tx; qx < nx - 2; qx += WAVE) {
+			const int q = qy * RX + qx;
+			float s;
+			if constexpr (VK) {
+				float d;
+				s = vkf(L, FMG_K5(qy, qx), su[q], su[q - RX], su[q + RX], su[q - 1], su[q + 1], &d);
+			} else {
+				s = sop(L, su[q], su[q - RX], su[q + RX], su[q - 1], su[q + 1]);
+			}
+			sa[q] = fmaf(1.0f, sb[q], -s);
+		}
+	}
+#undef FMG_K5
+	__syncthreads();
+	if (!mine) return;
+	if (I == 0 || J == 0 || I == cr - 1 || J == cc - 1) {
+		bco[I * cc + J] = 0.0f;
+		return;
+	}
+	// Window-relative ranges: fine row ii ↔ window index ii − F0, LDS row ii − R0.
+	const int rlo = tlr[ly] - (int)F0, rhi = thr[ly] - (int)F0;
+	const int clo = tlc[lx] - (int)G0, chi = thc[lx] - (int)G0;
+	const int iI = (int)I, iJ = (int)J;
+	float acc = 0.0f;
+	for (int ii = rlo; ii <= rhi; ++ii) {
+		const int ar = i0r[ii];
+		const float wy = ar == iI ? wlr[ii] : (ar + 1 == iI ? whr[ii] : 0.0f);
+		const float* rr = sa + (ii + 2) * RX + 2;
+		float row = 0.0f;
+		for (int jj = clo; jj <= chi; ++jj) {
+			const int ac = i0c[jj];
+			const float wx = ac == iJ ? wlc[jj] : (ac + 1 == iJ ? whc[jj] : 0.0f);
+			row += wx * rr[jj];
 		}
 		acc += wy * row;
 	}
-	bc_out[R * cc + col] = acc * (sy * sx);
+	bco[I * cc + J] = acc * (X.sy * X.sx);
 }
 
 // Coarsest level: `sweeps` Jacobi sweeps from zero in one workgroup (n ≤ 4096 points).
@@ -190,121 +289,152 @@ __global__ __launch_bounds__(1024) void k_fcoarse(const float* __restrict__ b, f
 	for (int q = threadIdx.x; q < n; q += blockDim.x) out[q] = cur[q];
 }
 
-// Post-smoothing tile: 64 columns × 16 rows per 256-thread workgroup.
-constexpr int PTX = 64, PTY = 16;
+// ── k_fpost ──────────────────────────────────────────────────────────
+// Tile PTX × PTY of fine points per 256-thread workgroup; b (and k) are staged
+// with a halo of PH = 3 ≥ npre + npost − 1 (k one ring more).
+constexpr int PTX = 64, PTY = 16, PH = 3;
+constexpr int PRY = PTY + 2 * PH, PRX = PTX + 2 * PH;
+static_assert(PTX == WAVE, "one tile column per lane in the final sweep");
 
 struct NoEpi {
 	static constexpr int NS = 0;
 	__device__ void term(int64_t, float, Acc*) const {}
-	template <class A> __device__ void fin(A*) const {}
 };
 
-// u_fin = Jacobi^npost(u + P·e) on the tile.  Epi adds per-point reduction
-// terms for the level-0 result and finishes them in the last workgroup.
-template <class Epi>
-__global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ u, const float* __restrict__ b,
-                                                 const float* __restrict__ ec, float* __restrict__ out, LWf L, int npost,
-                                                 int64_t rows, int64_t cols, int64_t cr, int64_t cc, Epi epi,
-                                                 RedScratch rs, const int* skip) {
+// Calls f(y, x) for every point of the NY × NX block at (OY, OX) of an LDS region.
+template <int OY, int OX, int NY, int NX, class F>
+__device__ __forceinline__ void region(F&& f) {
+	for (int q = threadIdx.x; q < NY * NX; q += BLOCK) f(OY + q / NX, OX + q % NX);
+}
+
+// out = J^NPOST(J^NPRE(0; b) + P·e) on the tile.  Epi adds per-point reduction
+// terms of the level-0 result (block partials; k_finish forms the sums).
+template <bool VK, int NPRE, int NPOST, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, const float* __restrict__ ec,
+                                                 float* __restrict__ out, LWf L, fx_xtab X, int64_t rows, int64_t cols,
+                                                 int64_t cr, int64_t cc, Epi epi, RedScratch rs, const int* skip) {
+	static_assert(NPRE + NPOST - 1 <= PH, "halo too small");
 	if (skip && *skip) return;
-	constexpr int H = 2;
-	constexpr int RX = PTX + 2 * H, RY = PTY + 2 * H;
-	__shared__ float s3[RY][RX];
-	__shared__ float s4[RY][RX];
-	__shared__ float sbv[RY][RX];
-	__shared__ float skv[RY][RX];
-	__shared__ int64_t ry0[RY], ry1[RY], rx0[RX], rx1[RX];
-	__shared__ float rwy[RY], rwx[RX];
-	const int64_t r0 = (int64_t)blockIdx.y * PTY - H, c0 = (int64_t)blockIdx.x * PTX - H;
+	// Offsets into the region: the pre-smoothed iterate is needed on the tile ±
+	// NPOST (OU), its first sweep one ring wider (O1), b on the widest (OB).
+	constexpr int OU = PH - NPOST, O1 = OU - 1, OB = NPRE == 2 ? O1 : OU;
+	__shared__ float sb[PRY][PRX];
+	__shared__ float sa[PRY][PRX];
+	__shared__ float su[PRY][PRX];
+	__shared__ float sk[VK ? PRY + 2 : 1][VK ? PRX + 2 : 1];
+	__shared__ int32_t ry0[PRY], ry1[PRY], rx0[PRX], rx1[PRX];
+	__shared__ float rwy[PRY], rwx[PRX];
+	const int64_t r0 = (int64_t)blockIdx.y * PTY - PH, c0 = (int64_t)blockIdx.x * PTX - PH;
 	const int t = threadIdx.x;
 	// Interpolation tables for the region's rows and columns.
-	for (int q = t; q < RY; q += BLOCK) {
-		int64_t rr = r0 + q;
+	for (int q = t; q < PRY; q += BLOCK) {
+		const int64_t rr = r0 + q;
 		if (rr > 0 && rr < rows - 1) {
-			Lin1 l = lin1(rr, rows, cr);
-			ry0[q] = l.i0;
-			ry1[q] = l.i0 + 1 < cr ? l.i0 + 1 : l.i0;
-			rwy[q] = float(l.w);
+			const int32_t a = X.i0r[rr];
+			ry0[q] = a;
+			ry1[q] = a + 1 < cr ? a + 1 : a;
+			rwy[q] = float(X.wr[rr]);
 		}
 	}
-	for (int q = t; q < RX; q += BLOCK) {
-		int64_t cl = c0 + q;
+	for (int q = t; q < PRX; q += BLOCK) {
+		const int64_t cl = c0 + q;
 		if (cl > 0 && cl < cols - 1) {
-			Lin1 l = lin1(cl, cols, cc);
-			rx0[q] = l.i0;
-			rx1[q] = l.i0 + 1 < cc ? l.i0 + 1 : l.i0;
-			rwx[q] = float(l.w);
+			const int32_t a = X.i0c[cl];
+			rx0[q] = a;
+			rx1[q] = a + 1 < cc ? a + 1 : a;
+			rwx[q] = float(X.wc[cl]);
 		}
 	}
-	__syncthreads();
-	// u3 = u + P·e (fx_mg.hip k_prolong lin), b into LDS.
-	for (int q = t; q < RX * RY; q += BLOCK) {
-		int qy = q / RX, qx = q % RX;
-		int64_t rr = r0 + qy, cl = c0 + qx;
-		float v = 0.0f, bv = 0.0f;
-		if (rr >= 0 && rr < rows && cl >= 0 && cl < cols) {
-			int64_t j = rr * cols + cl;
-			v = u[j];
-			bv = b[j];
-			if (rr > 0 && rr < rows - 1 && cl > 0 && cl < cols - 1) {
-				float wy = rwy[qy], vy = 1.0f - wy, wx = rwx[qx], vx = 1.0f - wx;
-				int64_t a0 = ry0[qy], a1 = ry1[qy], b0 = rx0[qx], b1 = rx1[qx];
-				float top = vx * ec[a0 * cc + b0] + wx * ec[a0 * cc + b1];
-				float bot = vx * ec[a1 * cc + b0] + wx * ec[a1 * cc + b1];
-				v += vy * top + wy * bot;
-			}
-		}
-		s3[qy][qx] = v;
-		sbv[qy][qx] = bv;
-		if (L.k) skv[qy][qx] = (rr >= 0 && rr < rows && cl >= 0 && cl < cols) ? L.k[rr * cols + cl] : 1.0f;
+	region<OB, OB, PRY - 2 * OB, PRX - 2 * OB>([&](int y, int x) {
+		const int64_t rr = r0 + y, cl = c0 + x;
+		const bool in = rr >= 0 && rr < rows && cl >= 0 && cl < cols;
+		sb[y][x] = in ? b[rr * cols + cl] : 0.0f;
+	});
+	if constexpr (VK) {
+		region<OB, OB, PRY + 2 - 2 * OB, PRX + 2 - 2 * OB>([&](int y, int x) {
+			const int64_t rr = r0 - 1 + y, cl = c0 - 1 + x;
+			const bool in = rr >= 0 && rr < rows && cl >= 0 && cl < cols;
+			sk[y][x] = in ? L.k[rr * cols + cl] : 1.0f;
+		});
 	}
 	__syncthreads();
-	float (*last)[RX] = s3;
-	if (npost == 2) {
-		// First sweep on the region minus its outer ring.
-		for (int q = t; q < (RX - 2) * (RY - 2); q += BLOCK) {
-			int qy = 1 + q / (RX - 2), qx = 1 + q % (RX - 2);
-			int64_t rr = r0 + qy, cl = c0 + qx;
-			float v = s3[qy][qx];
-			if (rr > 0 && rr < rows - 1 && cl > 0 && cl < cols - 1) {
-				if (L.k)
-					v = jack(L, skv[qy][qx], skv[qy - 1][qx], skv[qy + 1][qx], skv[qy][qx - 1], skv[qy][qx + 1], v,
-					         s3[qy - 1][qx], s3[qy + 1][qx], s3[qy][qx - 1], s3[qy][qx + 1], sbv[qy][qx]);
-				else
-					v = jac(L, v, s3[qy - 1][qx], s3[qy + 1][qx], s3[qy][qx - 1], s3[qy][qx + 1], sbv[qy][qx]);
+	// 1 ≤ row ≤ rows − 2 and 1 ≤ col ≤ cols − 2, as 32-bit unsigned range tests.
+	const int ir0 = (int)r0 - 1, ic0 = (int)c0 - 1;
+	const unsigned ur = (unsigned)(rows - 2), uc = (unsigned)(cols - 2);
+	auto interior = [&](int qy, int qx) { return (unsigned)(ir0 + qy) < ur && (unsigned)(ic0 + qx) < uc; };
+	// k at (qy, qx) and its four neighbours, in the (c, n, s, w, e) order of vkf.
+#define FMG_K5(y, x) sk[(y) + 1][(x) + 1], sk[(y)][(x) + 1], sk[(y) + 2][(x) + 1], sk[(y) + 1][(x)], sk[(y) + 1][(x) + 2]
+	if constexpr (NPRE == 2) {
+		region<O1, O1, PRY - 2 * O1, PRX - 2 * O1>([&](int y, int x) {
+			float v = 0.0f;
+			if (interior(y, x)) {
+				if constexpr (VK) v = jack(L, FMG_K5(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
+				else v = jac0(L, sb[y][x]);
 			}
-			s4[qy][qx] = v;
-		}
+			sa[y][x] = v;
+		});
 		__syncthreads();
-		last = s4;
+	}
+	// u = J^NPRE(0; b), then u += P·e (fx_mg.hip k_prolong lin) on interior points.
+	region<OU, OU, PRY - 2 * OU, PRX - 2 * OU>([&](int y, int x) {
+		float v = 0.0f;
+		if (interior(y, x)) {
+			if constexpr (NPRE == 2) {
+				if constexpr (VK)
+					v = jack(L, FMG_K5(y, x), sa[y][x], sa[y - 1][x], sa[y + 1][x], sa[y][x - 1], sa[y][x + 1], sb[y][x]);
+				else
+					v = jac(L, sa[y][x], sa[y - 1][x], sa[y + 1][x], sa[y][x - 1], sa[y][x + 1], sb[y][x]);
+			} else {
+				if constexpr (VK) v = jack(L, FMG_K5(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
+				else v = jac0(L, sb[y][x]);
+			}
+			const float wy = rwy[y], vy = 1.0f - wy, wx = rwx[x], vx = 1.0f - wx;
+			const int64_t a0 = ry0[y], a1 = ry1[y], b0 = rx0[x], b1 = rx1[x];
+			const float top = vx * ec[a0 * cc + b0] + wx * ec[a0 * cc + b1];
+			const float bot = vx * ec[a1 * cc + b0] + wx * ec[a1 * cc + b1];
+			v += vy * top + wy * bot;
+		}
+		su[y][x] = v;
+	});
+	__syncthreads();
+	float (*last)[PRX] = su;
+	if constexpr (NPOST == 2) {
+		region<PH - 1, PH - 1, PTY + 2, PTX + 2>([&](int y, int x) {
+			float v = su[y][x];
+			if (interior(y, x)) {
+				if constexpr (VK)
+					v = jack(L, FMG_K5(y, x), v, su[y - 1][x], su[y + 1][x], su[y][x - 1], su[y][x + 1], sb[y][x]);
+				else
+					v = jac(L, v, su[y - 1][x], su[y + 1][x], su[y][x - 1], su[y][x + 1], sb[y][x]);
+			}
+			sa[y][x] = v;
+		});
+		__syncthreads();
+		last = sa;
 	}
 	// Final sweep on the tile.
 	Acc acc[Epi::NS > 0 ? Epi::NS : 1];
 	for (int k = 0; k < (Epi::NS > 0 ? Epi::NS : 1); ++k) acc[k] = Acc{0.0, 0.0};
-	for (int q = t; q < PTX * PTY; q += BLOCK) {
-		int qy = H + q / PTX, qx = H + q % PTX;
-		int64_t rr = r0 + qy, cl = c0 + qx;
-		if (rr >= rows || cl >= cols) continue;
-		float v = last[qy][qx];
-		if (rr > 0 && rr < rows - 1 && cl > 0 && cl < cols - 1) {
-			if (L.k)
-				v = jack(L, skv[qy][qx], skv[qy - 1][qx], skv[qy + 1][qx], skv[qy][qx - 1], skv[qy][qx + 1], v,
-				         last[qy - 1][qx], last[qy + 1][qx], last[qy][qx - 1], last[qy][qx + 1], sbv[qy][qx]);
+	region<PH, PH, PTY, PTX>([&](int y, int x) {
+		const int64_t rr = r0 + y, cl = c0 + x;
+		if (rr >= rows || cl >= cols) return;
+		float v = last[y][x];
+		if (interior(y, x)) {
+			if constexpr (VK)
+				v = jack(L, FMG_K5(y, x), v, last[y - 1][x], last[y + 1][x], last[y][x - 1], last[y][x + 1], sb[y][x]);
 			else
-				v = jac(L, v, last[qy - 1][qx], last[qy + 1][qx], last[qy][qx - 1], last[qy][qx + 1], sbv[qy][qx]);
+				v = jac(L, v, last[y - 1][x], last[y + 1][x], last[y][x - 1], last[y][x + 1], sb[y][x]);
 		}
-		int64_t j = rr * cols + cl;
+		const int64_t j = rr * cols + cl;
 		out[j] = v;
 		epi.term(j, v, acc);
-	}
-	if constexpr (Epi::NS > 0) {
-		Acc res[Epi::NS];
-		if (block_partial_and_last<Epi::NS>(acc, rs, res) && tid_linear() == 0) epi.fin(res);
-	}
+	});
+#undef FMG_K5
+	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
 }
 
-inline dim3 pgrid(int64_t rows, int64_t cols) {
-	return dim3((unsigned)((cols + WAVE - 1) / WAVE), (unsigned)((rows + NWAVES - 1) / NWAVES));
+inline dim3 rgrid(int64_t cr, int64_t cc) {
+	return dim3((unsigned)((cc + RCX - 1) / RCX), (unsigned)((cr + RCY - 1) / RCY));
 }
 inline dim3 tgrid(int64_t rows, int64_t cols) {
 	return dim3((unsigned)((cols + PTX - 1) / PTX), (unsigned)((rows + PTY - 1) / PTY));
@@ -329,15 +459,14 @@ inline bool fused_ok(const fx_multigrid* mg) {
 	if (!mg->f32 || o.smoother != FX_SMOOTH_JACOBI || o.transfer != FX_TRANSFER_FW_BILINEAR) return false;
 	if (o.n_pre < 1 || o.n_pre > 2 || o.n_post < 1 || o.n_post > 2) return false;
 	if (mg->lv.size() < 2) return false;
+	for (size_t l = 0; l + 1 < mg->lv.size(); ++l)
+		if (!mg->lv[l].xbuf || frestrict_lds(mg->lv[l].xt, mg->lv[l].k != nullptr) > 64 * 1024) return false;
 	const fx_grid& g = mg->lv.back().g;
 	return g.rows * g.cols <= 4096;
 }
 
-// One fused V-cycle with a zero initial guess.  The level-0 rhs must already be
-// in lv[0].rhs (FP32); the result is written to lv[0].tmp, with epi0's
-// per-point terms reduced over level 0 by the last workgroup of its kernel.
-template <class Epi>
-fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0) {
+template <bool VK, int NPRE, int NPOST, class Epi>
+fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 	fx_backend* be = mg->be;
 	hipStream_t st = be->stream;
 	auto& lv = mg->lv;
@@ -346,12 +475,12 @@ fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0) {
 	for (int l = 0; l < nl - 1; ++l) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
-		LWf w = lwf(mg, lv[l]);
-		hipLaunchKernelGGL(k_fpre, pgrid(g.rows, g.cols), dim3(BLOCK), 0, st, (const float*)lv[l].rhs, (float*)lv[l].u, w,
-		                   o.n_pre, (int64_t)g.rows, (int64_t)g.cols, skip);
-		hipLaunchKernelGGL(k_fres_restrict, pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st, (const float*)lv[l].u,
-		                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, w, (int64_t)g.rows, (int64_t)g.cols,
-		                   (int64_t)cg.rows, (int64_t)cg.cols, skip);
+		if (l == 0) fx::probe_begin(be, FX_PROBE_VC_RESTRICT0);
+		hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
+		                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs,
+		                   lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows,
+		                   (int64_t)cg.cols, skip);
+		if (l == 0) fx::probe_end(be, FX_PROBE_VC_RESTRICT0);
 	}
 	{
 		const auto& L = lv[nl - 1];
@@ -361,23 +490,46 @@ fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0) {
 	for (int l = nl - 2; l >= 1; --l) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
-		hipLaunchKernelGGL(k_fpost<NoEpi>, tgrid(g.rows, g.cols), dim3(BLOCK), 0, st, (const float*)lv[l].u,
-		                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]), o.n_post,
-		                   (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, NoEpi{},
-		                   RedScratch{nullptr, nullptr}, skip);
+		hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, NoEpi>), tgrid(g.rows, g.cols), dim3(BLOCK), 0, st,
+		                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
+		                   lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, NoEpi{},
+		                   RedScratch{nullptr}, skip);
 	}
 	const fx_grid& g = lv[0].g;
 	const fx_grid& cg = lv[1].g;
 	dim3 tg = tgrid(g.rows, g.cols);
-	RedScratch rs{nullptr, nullptr};
+	RedScratch rs{nullptr};
 	if (Epi::NS > 0) {
 		FX_CHECK(fx::ensure_partials(be, (size_t)tg.x * tg.y * Epi::NS));
 		rs = fx::red_scratch(be);
 	}
-	hipLaunchKernelGGL(k_fpost<Epi>, tg, dim3(BLOCK), 0, st, (const float*)lv[0].u, (const float*)lv[0].rhs,
-	                   (const float*)lv[1].tmp, (float*)lv[0].tmp, lwf(mg, lv[0]), o.n_post, (int64_t)g.rows,
+	fx::probe_begin(be, FX_PROBE_VC_POST0);
+	hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, Epi>), tg, dim3(BLOCK), 0, st, (const float*)lv[0].rhs,
+	                   (const float*)lv[1].tmp, (float*)lv[0].tmp, lwf(mg, lv[0]), lv[0].xt, (int64_t)g.rows,
 	                   (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, epi0, rs, skip);
+	fx::probe_end(be, FX_PROBE_VC_POST0);
+	if constexpr (Epi::NS > 0)
+		hipLaunchKernelGGL((k_finish<Epi::NS, typename Epi::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, rs, tg.x * tg.y,
+		                   epi0.fin(), skip);
 	return fx::check_launch("fused V-cycle");
+}
+
+template <bool VK, class Epi>
+fx_status fast_cycle_v(fx_multigrid* mg, const int* skip, Epi epi0) {
+	const int a = mg->opt.n_pre, b = mg->opt.n_post;
+	if (a == 2 && b == 2) return fast_cycle_t<VK, 2, 2>(mg, skip, epi0);
+	if (a == 1 && b == 1) return fast_cycle_t<VK, 1, 1>(mg, skip, epi0);
+	if (a == 1 && b == 2) return fast_cycle_t<VK, 1, 2>(mg, skip, epi0);
+	return fast_cycle_t<VK, 2, 1>(mg, skip, epi0);
+}
+
+// One fused V-cycle with a zero initial guess.  The level-0 rhs must already be
+// in lv[0].rhs (FP32); the result is written to lv[0].tmp, with epi0's
+// per-point terms reduced over level 0 (k_finish runs epi0.fin() on the sums).
+template <class Epi>
+fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0) {
+	if (mg->lv[0].k) return fast_cycle_v<true>(mg, skip, epi0);
+	return fast_cycle_v<false>(mg, skip, epi0);
 }
 
 }  // namespace fmg

@@ -68,7 +68,7 @@ struct PcgFusedF {
 
 // ── reductions ───────────────────────────────────────────────────────
 template <int NS, class T>
-__global__ __launch_bounds__(BLOCK) void k_reduce(T term, size_t n, RedScratch rs, double* out) {
+__global__ __launch_bounds__(BLOCK) void k_reduce(T term, size_t n, RedScratch rs) {
 	Acc acc[NS];
 #pragma unroll
 	for (int k = 0; k < NS; ++k) acc[k] = Acc{0.0, 0.0};
@@ -79,12 +79,17 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(T term, size_t n, RedScratch r
 #pragma unroll
 		for (int k = 0; k < NS; ++k) acc_add(acc[k], v[k]);
 	}
-	Acc res[NS];
-	if (block_partial_and_last<NS>(acc, rs, res) && threadIdx.x == 0) {
-#pragma unroll
-		for (int k = 0; k < NS; ++k) out[k] = acc_value(res[k]);
-	}
+	block_partial<NS>(acc, rs);
 }
+// Writes the NS global sums to out[0..NS).
+template <int NS>
+struct FinStore {
+	double* out;
+	__device__ void operator()(Acc* a) const {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) out[k] = acc_value(a[k]);
+	}
+};
 // ops.rs:341-356: prod = x*y (no fma).
 struct DotT { const double* x; const double* y; __device__ void operator()(size_t i, double* v) const { v[0] = x[i] * y[i]; } };
 struct Dot2T { const double *a, *b, *c, *d;
@@ -93,55 +98,46 @@ struct SumT { const double* x; __device__ void operator()(size_t i, double* v) c
 struct SqT { const double* x; __device__ void operator()(size_t i, double* v) const { v[0] = x[i] * x[i]; } };
 
 // max / min (ops.rs:318-331, 587-601): NaN-ignoring, identities ∓inf.
+// Block results go to partials[b].hi; k_minmax_finish combines them.
 template <int MAX>
-__global__ __launch_bounds__(BLOCK) void k_minmax(const double* x, size_t n, RedScratch rs, double* out) {
-	const double ident = MAX ? -INFINITY : INFINITY;
-	double m = ident;
-	size_t stride = (size_t)gridDim.x * BLOCK;
-	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) m = MAX ? fmax(m, x[i]) : fmin(m, x[i]);
-	// Reuse the pair reduction by carrying the value in `hi` and merging with fmax/fmin.
+__device__ __forceinline__ double mm_block(double m) {
 	for (int off = WAVE / 2; off > 0; off >>= 1) {
 		double o = __shfl_xor(m, off);
 		m = MAX ? fmax(m, o) : fmin(m, o);
 	}
 	__shared__ double sh[NWAVES];
-	__shared__ int is_last;
 	if ((threadIdx.x & (WAVE - 1)) == 0) sh[threadIdx.x / WAVE] = m;
 	__syncthreads();
-	if (threadIdx.x == 0) {
+	if (threadIdx.x == 0)
 		for (int w = 1; w < NWAVES; ++w) m = MAX ? fmax(m, sh[w]) : fmin(m, sh[w]);
-		rs.partials[blockIdx.x].hi = m;
-		__threadfence();
-		unsigned prev = atomicAdd(rs.counter, 1u);
-		is_last = prev == gridDim.x - 1;
-	}
-	__syncthreads();
-	if (!is_last) return;
-	__threadfence();
-	m = ident;
-	for (unsigned b = threadIdx.x; b < gridDim.x; b += BLOCK) {
-		double p = __builtin_nontemporal_load(&rs.partials[b].hi);
+	return m;
+}
+template <int MAX>
+__global__ __launch_bounds__(BLOCK) void k_minmax(const double* x, size_t n, RedScratch rs) {
+	double m = MAX ? -INFINITY : INFINITY;
+	size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) m = MAX ? fmax(m, x[i]) : fmin(m, x[i]);
+	m = mm_block<MAX>(m);
+	if (threadIdx.x == 0) rs.partials[blockIdx.x].hi = m;
+}
+template <int MAX>
+__global__ __launch_bounds__(BLOCK) void k_minmax_finish(RedScratch rs, unsigned nb, double* out) {
+	double m = MAX ? -INFINITY : INFINITY;
+	for (unsigned b = threadIdx.x; b < nb; b += BLOCK) {
+		double p = rs.partials[b].hi;
 		m = MAX ? fmax(m, p) : fmin(m, p);
 	}
-	for (int off = WAVE / 2; off > 0; off >>= 1) {
-		double o = __shfl_xor(m, off);
-		m = MAX ? fmax(m, o) : fmin(m, o);
-	}
-	__syncthreads();
-	if ((threadIdx.x & (WAVE - 1)) == 0) sh[threadIdx.x / WAVE] = m;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		for (int w = 1; w < NWAVES; ++w) m = MAX ? fmax(m, sh[w]) : fmin(m, sh[w]);
-		*out = m;
-		*rs.counter = 0u;
-	}
+	m = mm_block<MAX>(m);
+	if (threadIdx.x == 0) *out = m;
 }
 
 template <int NS, class T>
 fx_status reduce_host(fx_backend* be, T term, size_t n, double* out, const char* what) {
 	unsigned nb = fx::red_blocks(n);
 	FX_CHECK(fx::ensure_partials(be, (size_t)nb * NS));
-	hipLaunchKernelGGL((k_reduce<NS, T>), dim3(nb), dim3(BLOCK), 0, be->stream, term, n, fx::red_scratch(be), be->dres);
+	hipLaunchKernelGGL((k_reduce<NS, T>), dim3(nb), dim3(BLOCK), 0, be->stream, term, n, fx::red_scratch(be));
+	hipLaunchKernelGGL((k_finish<NS, FinStore<NS>>), dim3(1), dim3(FIN_BLOCK), 0, be->stream, fx::red_scratch(be), nb,
+	                   FinStore<NS>{be->dres}, (const int*)nullptr);
 	FX_CHECK(fx::check_launch(what));
 	FX_HIP(hipMemcpyAsync(be->hres, be->dres, sizeof(double) * NS, hipMemcpyDeviceToHost, be->stream));
 	FX_HIP(hipStreamSynchronize(be->stream));
@@ -197,7 +193,7 @@ fx_status launch_stencil(fx_backend* be, const double* x, E e, const fx_stencil*
 	if (st->n == 5) {
 		int rpb = stream_rpb(rows, cols);
 		hipLaunchKernelGGL((k_stream5<0, LoadX, E, NoFinal>), fx::stream_grid(rows, cols, rpb), dim3(BLOCK), 0,
-		                   be->stream, LoadX{x}, e, NoFinal{}, w, rows, cols, rpb, RedScratch{nullptr, nullptr}, skip);
+		                   be->stream, LoadX{x}, e, NoFinal{}, w, rows, cols, rpb, RedScratch{nullptr}, skip);
 	} else {
 		dim3 grid((unsigned)((cols + WAVE - 1) / WAVE), (unsigned)((rows + NWAVES - 1) / NWAVES));
 		hipLaunchKernelGGL(k_point3<E>, grid, dim3(BLOCK), 0, be->stream, x, e, w, ent, st->n == 9 ? 9 : 0, rows, cols,
@@ -497,8 +493,13 @@ static fx_status minmax(fx_backend* be, const fx_buf* x, double* out, bool mx) {
 	FX_REQUIRE(be && out && x && x->ptr && x->dtype == FX_F64, FX_ERR_ARGUMENT, "fp64 buffer required");
 	unsigned nb = red_blocks(x->len);
 	FX_CHECK(ensure_partials(be, nb));
-	if (mx) hipLaunchKernelGGL(k_minmax<1>, dim3(nb), dim3(BLOCK), 0, be->stream, CD(x), x->len, red_scratch(be), be->dres);
-	else hipLaunchKernelGGL(k_minmax<0>, dim3(nb), dim3(BLOCK), 0, be->stream, CD(x), x->len, red_scratch(be), be->dres);
+	if (mx) {
+		hipLaunchKernelGGL(k_minmax<1>, dim3(nb), dim3(BLOCK), 0, be->stream, CD(x), x->len, red_scratch(be));
+		hipLaunchKernelGGL(k_minmax_finish<1>, dim3(1), dim3(BLOCK), 0, be->stream, red_scratch(be), nb, be->dres);
+	} else {
+		hipLaunchKernelGGL(k_minmax<0>, dim3(nb), dim3(BLOCK), 0, be->stream, CD(x), x->len, red_scratch(be));
+		hipLaunchKernelGGL(k_minmax_finish<0>, dim3(1), dim3(BLOCK), 0, be->stream, red_scratch(be), nb, be->dres);
+	}
 	FX_CHECK(check_launch("minmax"));
 	FX_HIP(hipMemcpyAsync(be->hres, be->dres, sizeof(double), hipMemcpyDeviceToHost, be->stream));
 	FX_HIP(hipStreamSynchronize(be->stream));
@@ -543,7 +544,10 @@ fx_status fx_stencil_axpy(fx_backend* be, double alpha, const fx_buf* x, fx_buf*
 	size_t n = g->rows * g->cols;
 	FX_CHECK(check_buf(x, n, FX_F64, "x"));
 	FX_CHECK(check_buf(y, n, FX_F64, "y"));
-	return launch_stencil(be, CD(x), EpiAxpy{alpha, D(y)}, st, (int64_t)g->rows, (int64_t)g->cols, "stencil_axpy");
+	probe_begin(be, FX_PROBE_STENCIL_AXPY);
+	fx_status s = launch_stencil(be, CD(x), EpiAxpy{alpha, D(y)}, st, (int64_t)g->rows, (int64_t)g->cols, "stencil_axpy");
+	probe_end(be, FX_PROBE_STENCIL_AXPY);
+	return s;
 }
 
 fx_status fx_restrict(fx_backend* be, const fx_buf* fine, fx_buf* coarse, const fx_grid* fg, const fx_grid* cg) {

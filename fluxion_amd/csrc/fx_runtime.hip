@@ -3,6 +3,7 @@
 // Replaces HipBackend::new / HipBuffer (crates/backend-hip/src/backend.rs:103-264,
 // buffer.rs:16-123) with a C ABI.  Unlike the reference there is no embedded
 // code-object loading: kernels are linked into this library for gfx950.
+#include <algorithm>
 #include "fx_internal.hpp"
 
 #include <cmath>
@@ -59,6 +60,28 @@ fx_status ensure_partials(fx_backend* be, size_t n) {
 	be->partials = (fxd::Acc*)p;
 	be->partials_cap = cap;
 	return FX_OK;
+}
+
+fx_status probe_drain(fx_backend* be) {
+	if (be->probe_used == 0) return FX_OK;
+	FX_HIP(hipEventSynchronize(be->probe_b[be->probe_used - 1]));
+	for (size_t i = 0; i < be->probe_used; ++i) {
+		float ms = 0.0f;
+		FX_HIP(hipEventElapsedTime(&ms, be->probe_a[i], be->probe_b[i]));
+		be->probe_ms.push_back(ms);
+	}
+	be->probe_used = 0;
+	return FX_OK;
+}
+void probe_begin(fx_backend* be, int kind) {
+	if (be->probe_kind == 0 || be->probe_kind != kind) return;
+	if (be->probe_used == be->probe_a.size()) (void)probe_drain(be);
+	(void)hipEventRecord(be->probe_a[be->probe_used], be->stream);
+}
+void probe_end(fx_backend* be, int kind) {
+	if (be->probe_kind == 0 || be->probe_kind != kind) return;
+	(void)hipEventRecord(be->probe_b[be->probe_used], be->stream);
+	++be->probe_used;
 }
 
 // grid.rs:15-36.
@@ -164,8 +187,6 @@ fx_status fx_backend_create(int device, fx_backend** out) {
 	}
 	void* p = nullptr;
 	fx_status s;
-	if ((s = dev_alloc(be, sizeof(unsigned) * 64, &p)) != FX_OK) { fx_backend_destroy(be); return s; }
-	be->counter = (unsigned*)p;
 	if ((s = dev_alloc(be, sizeof(double) * 64, &p)) != FX_OK) { fx_backend_destroy(be); return s; }
 	be->dres = (double*)p;
 	if (hipHostMalloc((void**)&be->hres, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) {
@@ -182,8 +203,9 @@ fx_status fx_backend_create(int device, fx_backend** out) {
 void fx_backend_destroy(fx_backend* be) {
 	if (!be) return;
 	if (be->stream) (void)hipStreamSynchronize(be->stream);
+	for (hipEvent_t e : be->probe_a) (void)hipEventDestroy(e);
+	for (hipEvent_t e : be->probe_b) (void)hipEventDestroy(e);
 	dev_free(be->partials);
-	dev_free(be->counter);
 	dev_free(be->dres);
 	if (be->hres) (void)hipHostFree(be->hres);
 	if (be->stream) (void)hipStreamDestroy(be->stream);
@@ -192,6 +214,44 @@ void fx_backend_destroy(fx_backend* be) {
 
 const char* fx_backend_name(const fx_backend* be) { return be ? be->name : "hip"; }
 void* fx_backend_stream(fx_backend* be) { return be ? (void*)be->stream : nullptr; }
+
+fx_status fx_probe_set(fx_backend* be, int32_t kind) {
+	FX_REQUIRE(be, FX_ERR_ARGUMENT, "backend is null");
+	FX_REQUIRE(kind >= FX_PROBE_OFF && kind <= FX_PROBE_STENCIL_AXPY, FX_ERR_ARGUMENT, "bad probe kind %d", kind);
+	FX_HIP(hipStreamSynchronize(be->stream));
+	if (kind != FX_PROBE_OFF && be->probe_a.empty()) {
+		be->probe_a.resize(256);
+		be->probe_b.resize(256);
+		for (size_t i = 0; i < be->probe_a.size(); ++i) {
+			FX_HIP(hipEventCreate(&be->probe_a[i]));
+			FX_HIP(hipEventCreate(&be->probe_b[i]));
+		}
+	}
+	be->probe_kind = kind;
+	be->probe_used = 0;
+	be->probe_ms.clear();
+	return FX_OK;
+}
+
+fx_status fx_probe_read(fx_backend* be, double* total_ms, uint64_t* launches) {
+	FX_REQUIRE(be && total_ms && launches, FX_ERR_ARGUMENT, "null argument");
+	FX_CHECK(fx::probe_drain(be));
+	std::vector<float> v = be->probe_ms;
+	double tot = 0.0;
+	uint64_t cnt = 0;
+	if (!v.empty()) {
+		std::sort(v.begin(), v.end());
+		const float cut = 0.1f * v[v.size() / 2];
+		for (float m : v)
+			if (m >= cut) {
+				tot += m;
+				++cnt;
+			}
+	}
+	*total_ms = tot;
+	*launches = cnt;
+	return FX_OK;
+}
 
 fx_status fx_synchronize(fx_backend* be) {
 	FX_REQUIRE(be, FX_ERR_ARGUMENT, "backend is null");

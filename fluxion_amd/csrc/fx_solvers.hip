@@ -71,8 +71,8 @@ __device__ __forceinline__ void finish(SState* s, int64_t its, double res, int r
 }
 
 // Element-wise update + reduction; F::load(st) caches scalars, F::operator()(i, acc).
-template <int NS, class F, class Fin>
-__global__ __launch_bounds__(BLOCK) void k_ered(F f, Fin fin, size_t n, RedScratch rs, const int* skip) {
+template <int NS, class F>
+__global__ __launch_bounds__(BLOCK) void k_ered(F f, size_t n, RedScratch rs, const int* skip) {
 	if (skip && *skip) return;
 	f.load();
 	Acc acc[NS];
@@ -80,8 +80,7 @@ __global__ __launch_bounds__(BLOCK) void k_ered(F f, Fin fin, size_t n, RedScrat
 	for (int k = 0; k < NS; ++k) acc[k] = Acc{0.0, 0.0};
 	size_t stride = (size_t)gridDim.x * BLOCK;
 	for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) f(i, acc);
-	Acc out[NS];
-	if (block_partial_and_last<NS>(acc, rs, out) && tid_linear() == 0) fin(out);
+	block_partial<NS>(acc, rs);
 }
 template <class F>
 __global__ __launch_bounds__(BLOCK) void k_eupd(F f, size_t n, const int* skip) {
@@ -181,8 +180,8 @@ struct EpiCG {
 	}
 };
 
-template <int NS, class V, class E, class F>
-__global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, F fin, W9 w, int64_t rows, int64_t cols, int rpb,
+template <int NS, class V, class E>
+__global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, W9 w, int64_t rows, int64_t cols, int rpb,
                                                      RedScratch rs, const SState* st) {
 	// Same traversal as fxd::k_stream5, reading β from the device state first.
 	if (st->done) return;
@@ -215,10 +214,179 @@ __global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, F fin, W9 w, int6
 		up = cur;
 		cur = dn;
 	}
-	Acc out[NS];
-	if (block_partial_and_last<NS>(acc, rs, out)) {
-		if (tid_linear() == 0) fin(out);
+	block_partial<NS>(acc, rs);
+}
+
+// The same traversal for the variable-coefficient operator
+// sigma·p + alpha·(−∇·(k∇p)) with harmonic face means (fx_ops.hip k_vark), k
+// streamed alongside p; E::interior receives the operator value.
+__device__ __forceinline__ double hmean_d(double a, double b) { return 2.0 * a * b / (a + b); }
+template <int NS, class V, class E>
+__global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __restrict__ k, double sigma, double alpha,
+                                                     double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
+                                                     RedScratch rs, const SState* st) {
+	if (st->done) return;
+	v.be = st->beta;
+	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool col_ok = c < cols;
+	const bool col_int = c > 0 && c < cols - 1;
+	Acc acc[NS];
+#pragma unroll
+	for (int q = 0; q < NS; ++q) acc[q] = Acc{0.0, 0.0};
+	double up = 0.0, cur = 0.0, kup = 1.0, kcur = 1.0;
+	if (col_ok) {
+		if (r0 > 0) {
+			up = v.load((r0 - 1) * cols + c);
+			kup = k[(r0 - 1) * cols + c];
+		}
+		cur = v.load(r0 * cols + c);
+		kcur = k[r0 * cols + c];
 	}
+	for (int64_t r = r0; r < r1; ++r) {
+		const int64_t i = r * cols + c;
+		double dn = 0.0, kdn = 1.0;
+		if (col_ok && r + 1 < rows) {
+			dn = v.load(i + cols);
+			kdn = k[i + cols];
+		}
+		if (col_ok) {
+			if (col_int && r > 0 && r < rows - 1) {
+				const double wv = v.load(i - 1), ev = v.load(i + 1);
+				const double kw = hmean_d(kcur, k[i - 1]), ke = hmean_d(kcur, k[i + 1]);
+				const double kn = hmean_d(kcur, kup), ks = hmean_d(kcur, kdn);
+				const double fx = ke * (ev - cur) - kw * (cur - wv);
+				const double fy = ks * (dn - cur) - kn * (cur - up);
+				const double L = -(fx * ix2 + fy * iy2);
+				e.interior(i, fma(alpha, L, sigma * cur), cur, acc);
+			} else {
+				e.boundary(i, cur, acc);
+			}
+		}
+		up = cur;
+		cur = dn;
+		kup = kcur;
+		kcur = kdn;
+	}
+	block_partial<NS>(acc, rs);
+}
+// ap = A·p inside (operator value given), ap = p on the boundary; writes p_new; p·ap.
+struct EpiCGv {
+	double *pout, *ap;
+	__device__ void interior(int64_t i, double av, double c, Acc* a) const {
+		pout[i] = c;
+		ap[i] = av;
+		acc_add(a[0], c * av);
+	}
+	__device__ void boundary(int64_t i, double c, Acc* a) const {
+		pout[i] = c;
+		ap[i] = c;
+		acc_add(a[0], c * c);
+	}
+};
+
+// FCG step of the fused loop on grids with an even column count, two columns
+// per thread (16-byte loads and stores): p = double(z) + β·p_old, ap = A·p
+// (identity boundary rows), p·ap.  A is the 5-point stencil operator
+// p + coeff·S(p) (chain order of fxd::chain5) or, with VK, the
+// variable-coefficient operator of k_stream_vk.  Per-point arithmetic equals
+// k_stream_st / k_stream_vk.
+template <bool VK>
+__global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z, const double* __restrict__ pold,
+                                                     double* __restrict__ pnew, double* __restrict__ ap, W9 w,
+                                                     double coeff, const double* __restrict__ k, double sigma,
+                                                     double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
+                                                     RedScratch rs, const SState* st) {
+	if (st->done) return;
+	const double be = st->beta;
+	const int64_t c0 = 2 * ((int64_t)blockIdx.x * BLOCK + threadIdx.x);
+	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool ok = c0 < cols;
+	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;   // column interior tests (c0 + 1 > 0 always)
+	auto P2 = [&](int64_t i) {
+		const float2 zz = *(const float2*)(z + i);
+		const double2 pp = *(const double2*)(pold + i);
+		return double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
+	};
+	auto P1 = [&](int64_t i) { return double(z[i]) + be * pold[i]; };
+	auto K2 = [&](int64_t i) { return *(const double2*)(k + i); };
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	double2 up{0.0, 0.0}, cur{0.0, 0.0}, kup{1.0, 1.0}, kcur{1.0, 1.0};
+	if (ok) {
+		if (r0 > 0) {
+			up = P2((r0 - 1) * cols + c0);
+			if constexpr (VK) kup = K2((r0 - 1) * cols + c0);
+		}
+		cur = P2(r0 * cols + c0);
+		if constexpr (VK) kcur = K2(r0 * cols + c0);
+	}
+	for (int64_t r = r0; r < r1 && ok; ++r) {
+		const int64_t i = r * cols + c0;
+		double2 dn{0.0, 0.0}, kdn{1.0, 1.0};
+		if (r + 1 < rows) {
+			dn = P2(i + cols);
+			if constexpr (VK) kdn = K2(i + cols);
+		}
+		const bool rint = r > 0 && r < rows - 1;
+		double2 av = cur;   // boundary rows and columns: identity
+		if (rint) {
+			const double wv = intA ? P1(i - 1) : 0.0;
+			const double ev = intB ? P1(i + 2) : 0.0;
+			if constexpr (VK) {
+				auto op = [&](double kc, double kn, double ks, double kw, double ke, double c, double n, double s_,
+				              double w_, double e_) {
+					const double hw = hmean_d(kc, kw), he = hmean_d(kc, ke);
+					const double hn = hmean_d(kc, kn), hs = hmean_d(kc, ks);
+					const double fx = he * (e_ - c) - hw * (c - w_);
+					const double fy = hs * (s_ - c) - hn * (c - n);
+					const double L = -(fx * ix2 + fy * iy2);
+					return fma(coeff, L, sigma * c);
+				};
+				if (intA) av.x = op(kcur.x, kup.x, kdn.x, k[i - 1], kcur.y, cur.x, up.x, dn.x, wv, cur.y);
+				if (intB) av.y = op(kcur.y, kup.y, kdn.y, kcur.x, k[i + 2], cur.y, up.y, dn.y, cur.x, ev);
+			} else {
+				if (intA) av.x = fma(coeff, chain5(w, cur.x, up.x, dn.x, wv, cur.y), cur.x);
+				if (intB) av.y = fma(coeff, chain5(w, cur.y, up.y, dn.y, cur.x, ev), cur.y);
+			}
+		}
+		*(double2*)(pnew + i) = cur;
+		*(double2*)(ap + i) = av;
+		acc_add(acc[0], cur.x * av.x);
+		acc_add(acc[0], cur.y * av.y);
+		up = cur;
+		cur = dn;
+		if constexpr (VK) {
+			kup = kcur;
+			kcur = kdn;
+		}
+	}
+	block_partial<1>(acc, rs);
+}
+
+// x += α·p; r −= α·ap with r·r, and the FP32 copy of r; two points per thread
+// per step (n even).  Per-point arithmetic equals FXR32.
+__global__ __launch_bounds__(BLOCK) void k_fxr32v(const SState* st, double* __restrict__ x, double* __restrict__ r,
+                                                  float* __restrict__ r32, const double* __restrict__ p,
+                                                  const double* __restrict__ ap, size_t n2, RedScratch rs,
+                                                  const int* skip) {
+	if (skip && *skip) return;
+	const double al = st->alpha;
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	const size_t stride = (size_t)gridDim.x * BLOCK;
+	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n2; j += stride) {
+		const double2 xv = ((const double2*)x)[j], pv = ((const double2*)p)[j];
+		const double2 rv = ((const double2*)r)[j], av = ((const double2*)ap)[j];
+		const double2 xn{fma(al, pv.x, xv.x), fma(al, pv.y, xv.y)};
+		const double2 rn{fma(-al, av.x, rv.x), fma(-al, av.y, rv.y)};
+		((double2*)x)[j] = xn;
+		((double2*)r)[j] = rn;
+		((float2*)r32)[j] = float2{float(rn.x), float(rn.y)};
+		acc_add(acc[0], rn.x * rn.x);
+		acc_add(acc[0], rn.y * rn.y);
+	}
+	block_partial<1>(acc, rs);
 }
 
 // ── scalar epilogues ───────────────────────────────────────────────────
@@ -511,7 +679,8 @@ struct EpiRZ {
 	const double* r;
 	SState* st;
 	__device__ void term(int64_t j, float z, Acc* a) const { acc_add(a[0], r[j] * double(z)); }
-	__device__ void fin(Acc* o) const { FinPcgInitRZ{st}(o); }
+	using Fin = FinPcgInitRZ;
+	Fin fin() const { return Fin{st}; }
 };
 struct EpiFcg {
 	static constexpr int NS = 2;
@@ -522,7 +691,8 @@ struct EpiFcg {
 		acc_add(a[0], r[j] * zd);
 		acc_add(a[1], zd * ap[j]);
 	}
-	__device__ void fin(Acc* o) const { FinFcgBeta{st}(o); }
+	using Fin = FinFcgBeta;
+	Fin fin() const { return Fin{st}; }
 };
 
 // ── BiCGSTAB (bicgstab.rs:90-346) ──────────────────────────────────────
@@ -894,7 +1064,8 @@ struct Ctx {
 	fx_status ered(F f, Fin fin, const char* what, const int* sk) {
 		unsigned nb = rb();
 		FX_CHECK(fx::ensure_partials(be, (size_t)nb * NS));
-		hipLaunchKernelGGL((k_ered<NS, F, Fin>), dim3(nb), dim3(BLOCK), 0, st, f, fin, n, fx::red_scratch(be), sk);
+		hipLaunchKernelGGL((k_ered<NS, F>), dim3(nb), dim3(BLOCK), 0, st, f, n, fx::red_scratch(be), sk);
+		hipLaunchKernelGGL((k_finish<NS, Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be), nb, fin, sk);
 		return check_launch(what);
 	}
 	template <int NS, class F, class Fin>
@@ -951,9 +1122,10 @@ struct Ctx {
 			int rpb = stream_rpb(rows, cols);
 			dim3 grid = fx::stream_grid(rows, cols, rpb);
 			FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
-			hipLaunchKernelGGL((k_stream_st<1, VPNew, EpiCG, FinAlpha>), grid, dim3(BLOCK), 0, st, VPNew{src, pold, 0.0},
-			                   EpiCG{op->coeff, pout, ap}, FinAlpha{d}, w9, rows, cols, rpb, fx::red_scratch(be),
-			                   (const SState*)d);
+			hipLaunchKernelGGL((k_stream_st<1, VPNew, EpiCG>), grid, dim3(BLOCK), 0, st, VPNew{src, pold, 0.0},
+			                   EpiCG{op->coeff, pout, ap}, w9, rows, cols, rpb, fx::red_scratch(be), (const SState*)d);
+			hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be),
+			                   grid.x * grid.y, FinAlpha{d}, done);
 			return check_launch("cg fused step");
 		}
 		FX_CHECK(eupd(FPUpd{d, src, pold, pout, 0.0}, "p update"));
@@ -1118,13 +1290,15 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	double *r = s->vec(0), *p0 = s->vec(1), *p1 = s->vec(2), *ap = s->vec(3), *z = s->vec(4);
 	FX_REQUIRE(r && p0 && p1 && ap && z, FX_ERR_ALLOC, "work vector allocation failed");
 	fx_multigrid* fm = nullptr;
-	if (c.pc && c.pc->kind == FX_PC_MULTIGRID && c.fused5) {
+	const bool vk = c.op->kind == FX_OP_VARK;
+	if (c.pc && c.pc->kind == FX_PC_MULTIGRID && (c.fused5 || vk)) {
 		fx_multigrid* m = (fx_multigrid*)c.pc->handle;
 		if (m->opt.fused) fm = m;
 	}
 	uint64_t used = 0;
 	fx_solve_stats st{};
 	double* pp[2] = {p0, p1};
+	double prev_tr = INFINITY;
 	for (int pass = 0; pass < 8; ++pass) {
 		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used), c.pc));
 		FX_CHECK(c.apply_op(x, ap, nullptr));
@@ -1134,15 +1308,55 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 			const float* z32 = (const float*)fm->lv[0].tmp;
 			FX_CHECK(c.ered<1>(FResid32{b, ap, r, r32}, FinPcgInitRR{c.d}, "fcg init", nullptr));
 			FX_CHECK(fmg::fast_cycle(fm, c.skip, EpiRZ{r, c.d}));
+			const bool even = c.cols % 2 == 0;
 			FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
 				double* pold = pp[i & 1];
 				double* pnew = pp[(i + 1) & 1];
+				if (even) {
+					// Two columns per thread: half as many threads per row.
+					int rpb = stream_rpb(c.rows, c.cols / 2);
+					dim3 grid = fx::stream_grid(c.rows, c.cols / 2, rpb);
+					FX_CHECK(fx::ensure_partials(c.be, (size_t)grid.x * grid.y));
+					const double* kp = vk ? (const double*)c.op->k->ptr : nullptr;
+					const double ix2 = 1.0 / (s->g.dx * s->g.dx), iy2 = 1.0 / (s->g.dy * s->g.dy);
+					fx::probe_begin(c.be, FX_PROBE_FCG_STEP);
+					if (vk)
+						hipLaunchKernelGGL(k_fcg_step2<true>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
+						                   c.op->coeff, kp, c.op->sigma, ix2, iy2, c.rows, c.cols, rpb,
+						                   fx::red_scratch(c.be), (const SState*)c.d);
+					else
+						hipLaunchKernelGGL(k_fcg_step2<false>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
+						                   c.op->coeff, kp, 0.0, ix2, iy2, c.rows, c.cols, rpb, fx::red_scratch(c.be),
+						                   (const SState*)c.d);
+					fx::probe_end(c.be, FX_PROBE_FCG_STEP);
+					hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
+					                   grid.x * grid.y, FinAlpha{c.d}, c.done);
+					FX_CHECK(check_launch("fcg fused step"));
+					const size_t n2 = c.n / 2;
+					const unsigned nb = fx::red_blocks(c.n);
+					fx::probe_begin(c.be, FX_PROBE_FCG_UPDATE);
+					hipLaunchKernelGGL(k_fxr32v, dim3(nb), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, r, r32,
+					                   (const double*)pnew, (const double*)ap, n2, fx::red_scratch(c.be), c.skip);
+					fx::probe_end(c.be, FX_PROBE_FCG_UPDATE);
+					hipLaunchKernelGGL((k_finish<1, FinPcgRR>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
+					                   nb, FinPcgRR{c.d}, c.skip);
+					FX_CHECK(check_launch("fcg x/r update"));
+					return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
+				}
 				int rpb = stream_rpb(c.rows, c.cols);
 				dim3 grid = fx::stream_grid(c.rows, c.cols, rpb);
 				FX_CHECK(fx::ensure_partials(c.be, (size_t)grid.x * grid.y));
-				hipLaunchKernelGGL((k_stream_st<1, VPNewF, EpiCG, FinAlpha>), grid, dim3(BLOCK), 0, c.st,
-				                   VPNewF{z32, pold, 0.0}, EpiCG{c.op->coeff, pnew, ap}, FinAlpha{c.d}, c.w9, c.rows,
-				                   c.cols, rpb, fx::red_scratch(c.be), (const SState*)c.d);
+				if (vk)
+					hipLaunchKernelGGL((k_stream_vk<1, VPNewF, EpiCGv>), grid, dim3(BLOCK), 0, c.st,
+					                   VPNewF{z32, pold, 0.0}, EpiCGv{pnew, ap}, (const double*)c.op->k->ptr, c.op->sigma,
+					                   c.op->coeff, 1.0 / (s->g.dx * s->g.dx), 1.0 / (s->g.dy * s->g.dy), c.rows, c.cols,
+					                   rpb, fx::red_scratch(c.be), (const SState*)c.d);
+				else
+					hipLaunchKernelGGL((k_stream_st<1, VPNewF, EpiCG>), grid, dim3(BLOCK), 0, c.st,
+					                   VPNewF{z32, pold, 0.0}, EpiCG{c.op->coeff, pnew, ap}, c.w9, c.rows, c.cols, rpb,
+					                   fx::red_scratch(c.be), (const SState*)c.d);
+				hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
+				                   grid.x * grid.y, FinAlpha{c.d}, c.done);
 				FX_CHECK(check_launch("fcg fused step"));
 				FX_CHECK(c.ered<1>(FXR32{c.d, x, r, r32, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
 				return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
@@ -1167,6 +1381,10 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 		FX_CHECK(true_residual(c, x, b, ap, &tr));
 		st.true_residual = tr;
 		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
+		// A restart that did not halve the true residual has reached the FP64
+		// attainable accuracy (about eps·‖A‖·‖x‖); further passes only repeat it.
+		if (!(tr < 0.5 * prev_tr)) break;
+		prev_tr = tr;
 	}
 	st.iterations = used;
 	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol)) st.reason = FX_MAX_ITERATIONS;

@@ -48,6 +48,19 @@ class HipBackend:
     def stream(self) -> int:
         return self._lib.fx_backend_stream(self.h) or 0
 
+    # ── kernel probe (fx_probe_set / fx_probe_read) ───────────────────
+    PROBE_OFF, PROBE_FCG_UPDATE, PROBE_FCG_STEP, PROBE_VC_POST0, PROBE_VC_RESTRICT0, PROBE_STENCIL_AXPY = range(6)
+
+    def probe(self, kind: int) -> None:
+        """Time every launch of one kernel family with HIP events on this stream (0 = off)."""
+        L.check(self._lib.fx_probe_set(self.h, int(kind)), "probe_set")
+
+    def probe_read(self) -> tuple[float, int]:
+        """(summed ms, launches) of the probed family since probe()."""
+        ms, cnt = C.c_double(), C.c_uint64()
+        L.check(self._lib.fx_probe_read(self.h, C.byref(ms), C.byref(cnt)), "probe_read")
+        return ms.value, cnt.value
+
     # ── memory ────────────────────────────────────────────────────────
     def allocate(self, n: int, dtype: DType = DType.F64) -> "Buffer":
         h = C.c_void_p()

@@ -110,15 +110,16 @@ def grf_log_k(n: int, seed: int = SEED0, corr: float = 0.1, modes: int = 256) ->
     g = √(2/M)·Σ cos(ω·x + φ)."""
     rng = SplitMix64(seed)
     x, y = node_coords(n, n)
-    g = np.zeros((n, n))
-    for _ in range(modes):
-        wx = rng.normal() / corr
-        wy = rng.normal() / corr
-        ph = rng.uniform(0.0, 2.0 * math.pi)
-        # cos(wx·x + wy·y + ph) = cos(a)cos(b) − sin(a)sin(b), a = wx·x + ph, b = wy·y
-        a = wx * x + ph
-        b = wy * y
-        g += np.outer(np.cos(b), np.cos(a)) - np.outer(np.sin(b), np.sin(a))
+    w = np.empty((modes, 3))
+    for m in range(modes):
+        w[m, 0] = rng.normal() / corr
+        w[m, 1] = rng.normal() / corr
+        w[m, 2] = rng.uniform(0.0, 2.0 * math.pi)
+    # cos(wx·x + wy·y + ph) = cos(a)cos(b) − sin(a)sin(b), a = wx·x + ph, b = wy·y; the sum over
+    # modes is two (n×M)·(M×n) products.
+    a = w[:, 0:1] * x[None, :] + w[:, 2:3]
+    b = w[:, 1:2] * y[None, :]
+    g = np.cos(b).T @ np.cos(a) - np.sin(b).T @ np.sin(a)
     g *= math.sqrt(2.0 / modes)
     return np.exp(g)
 

@@ -69,6 +69,24 @@ size_t fx_last_error(char* buf, size_t len);
 /* Raw HIP stream of the backend (hipStream_t), for callers that time or order work around it. */
 void* fx_backend_stream(fx_backend* be);
 
+/* ── kernel probe (measurement extension; no reference counterpart) ─────
+ * HIP events recorded on the backend's stream immediately before and after
+ * every launch of one kernel family.  fx_probe_set selects the family (OFF
+ * disables) and clears the accumulator; fx_probe_read synchronises and returns
+ * the summed duration and count of the probed launches.  Launches that exited
+ * at once on a solver's convergence flag (shorter than a tenth of the median)
+ * are left out of both figures. */
+typedef enum {
+	FX_PROBE_OFF = 0,
+	FX_PROBE_FCG_UPDATE = 1,      /* FCG x/r update + FP32 residual copy + r·r        */
+	FX_PROBE_FCG_STEP = 2,        /* FCG p update + A·p + p·Ap                        */
+	FX_PROBE_VC_POST0 = 3,        /* fused V-cycle, finest-level post kernel          */
+	FX_PROBE_VC_RESTRICT0 = 4,    /* fused V-cycle, finest-level residual+restriction */
+	FX_PROBE_STENCIL_AXPY = 5     /* Backend::stencil_axpy, 5-point streaming kernel  */
+} fx_probe_kind;
+fx_status fx_probe_set(fx_backend* be, int32_t kind);
+fx_status fx_probe_read(fx_backend* be, double* total_ms, uint64_t* launches);
+
 /* ── memory (backend.rs:50-79, BackendBuffer 13-36) ──────────────────── */
 fx_status fx_allocate(fx_backend* be, size_t len, fx_dtype dtype, fx_buf** out);   /* zero-filled */
 fx_status fx_upload_f64(fx_backend* be, const double* data, size_t len, fx_buf** out);

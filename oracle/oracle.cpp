@@ -987,13 +987,17 @@ int orc_fcg(double* x, const double* b, int64_t rows, int64_t cols, double dx, d
 	Pc pc(pc_kind, g, cfg);
 	std::vector<double> r(n), z(n), p(n), ap(n);
 	int64_t it = 0;
-	double r_norm = 0.0;
+	double r_norm = 0.0, prev_tr = INFINITY;
 	for (int restart = 0; restart < 8; ++restart) {
 		apply_op(*op, x, ap.data(), g);
 		copy(b, r.data(), n);
 		axpy(-1.0, ap.data(), r.data(), n);
 		r_norm = std::sqrt(dot(r.data(), r.data(), n));
 		if (r_norm < tol) { set_stats(st, it, r_norm, CONVERGED); return OK; }
+		// A restart that did not halve the true residual has reached the FP64
+		// attainable accuracy; further passes only repeat it.
+		if (restart > 0 && !(r_norm < 0.5 * prev_tr)) break;
+		prev_tr = r_norm;
 		fill(z.data(), 0.0, n);
 		pc.apply(z.data(), r.data(), n);
 		copy(z.data(), p.data(), n);

@@ -341,9 +341,11 @@ def test_fused_vcycle_bitwise(be, shape, npre, npost):
     assert np.array_equal(X.to_vec_f64(), O.vcycle(np.zeros(rows * cols), r, g, cfg))
 
 
-def test_fused_fcg_matches_unfused(be):
+@pytest.mark.parametrize("n", [512, 257])
+def test_fused_fcg_matches_unfused(be, n):
+    # Even column counts take the two-columns-per-thread step kernels, odd ones the scalar ones.
     hip = H()
-    g, h, f, st, coeff = poisson_c1(512)
+    g, h, f, st, coeff = poisson_c1(n)
     tol = 1e-10 * np.linalg.norm(f)
     B = hip.Field.from_f64(g, f, be)
     out = []
@@ -376,10 +378,10 @@ def test_vark_vcycle_bitwise(be, precision, fused, transfer):
     assert np.array_equal(X.to_vec_f64(), O.vcycle(np.zeros(n * n), r, g, cfg))
 
 
-def test_vark_fcg_fused_mg(be):
+@pytest.mark.parametrize("n", [256, 255])
+def test_vark_fcg_fused_mg(be, n):
     # −∇·(k∇u) = f with log-normal k: FCG + fused FP32 coefficient-aware V-cycle against the oracle.
     hip = H()
-    n = 256
     g = Grid.square(n, 1.0 / (n - 1))
     k = grf_log_k(n).ravel()
     f = gaussian_bumps(n).ravel()
