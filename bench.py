@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run Backend::copy of 2^24 doubles 20 times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -78,6 +80,18 @@ def main():
     from fluxion_amd import hip
     from fluxion_amd.core import Boundaries, DType, Grid, Stencil, Transfer
     from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+
+    if args.copy_calibration:
+        # A copy of known size: 2^24 doubles read and written per launch (128 MiB + 128 MiB).
+        be = hip.HipBackend(0)
+        src = hip.Field.zeros(Grid(4096, 4096, 1.0, 1.0), be=be)
+        dst = hip.Field.zeros(Grid(4096, 4096, 1.0, 1.0), be=be)
+        src.fill(1.0)
+        for _ in range(20):
+            dst.copy_from(src)
+        be.synchronize()
+        log("bench: copy calibration done (20 launches of 2^24 doubles)")
+        return
 
     n = args.size
     h = 1.0 / (n - 1)

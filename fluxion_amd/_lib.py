@@ -4,6 +4,7 @@ Loading fails loudly: there is no CPU fallback behind this module.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 from pathlib import Path
 
@@ -156,6 +157,8 @@ _SIGS = {
     "fx_unet_forward": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_int32, C.c_int32, C.c_int32]),
     "fx_unet_loss_grad": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_int32, C.c_int32, C.c_int32,
                                     _P(C.c_double)]),
+    "fx_unet_loss_grad_energy": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_int32, C.c_int32, C.c_int32,
+                                    _P(C.c_double)]),
     "fx_unet_adam_step": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
     "fx_learned_create": (C.c_int, [c_be_p, C.c_void_p, C.c_void_p, c_buf_p, _P(FxGrid), C.c_double,
                                     _P(C.c_void_p)]),
@@ -164,6 +167,22 @@ _SIGS = {
 }
 
 _lib = None
+_exiting = False
+
+
+def _mark_exiting() -> None:
+    global _exiting
+    _exiting = True
+
+
+# Native handles still alive when the interpreter exits are left to the driver: destroy calls made
+# during interpreter teardown can run after the HIP runtime has begun its own shutdown.
+atexit.register(_mark_exiting)
+
+
+def live() -> bool:
+    """False once the interpreter has begun exiting (destructors then skip native calls)."""
+    return not _exiting
 
 
 def lib() -> C.CDLL:

@@ -105,6 +105,23 @@ __device__ __forceinline__ void block_reduce(Acc (&v)[NS]) {
 	__syncthreads();
 }
 
+// Workgroups are dealt round-robin to the eight XCDs in dispatch order (x
+// fastest).  xcd_remap maps the hardware block index to a logical tile index so
+// that each XCD receives one contiguous run of tiles (neighbouring tiles then
+// share that XCD's L2 instead of re-reading halos from HBM); a bijection on
+// [0, nb).  xcd_tile applies it to a 2-D grid, row-major.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+	constexpr int NX = 8;
+	const int q = nb / NX, r = nb % NX;
+	const int i = b % NX, k = b / NX;
+	return i * q + (i < r ? i : r) + k;
+}
+__device__ __forceinline__ void xcd_tile(int* bx, int* by) {
+	const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+	*bx = lid % gridDim.x;
+	*by = lid / gridDim.x;
+}
+
 // Reduction scratch handed to every reducing kernel.
 struct RedScratch {
 	Acc* partials;        // [nblocks * NS]
@@ -188,8 +205,10 @@ template <int NS, class V, class E, class F>
 __global__ __launch_bounds__(BLOCK) void k_stream5(V v, E e, F fin, W9 w, int64_t rows, int64_t cols, int rpb,
                                                    RedScratch rs, const int* done) {
 	if (done && *done) return;
-	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t c = (int64_t)bx * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool col_ok = c < cols;
 	const bool col_int = c > 0 && c < cols - 1;

@@ -261,6 +261,17 @@ __global__ void k_coarsen_k(const T* kf, T* kc, int64_t fr, int64_t fc, int64_t 
 	kc[R * cc + C] = vy * top + wy * bot;
 }
 
+// Face coefficients of a level: kx[i] = hm(k[i], k[i+1]), ky[i] = hm(k[i], k[i+cols])
+// (the harmonic mean is symmetric bit for bit: 2·a·b and a + b are exact in either order).
+__global__ void k_faces(const float* k, float* kx, float* ky, int64_t rows, int64_t cols) {
+	int64_t c = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (r >= rows || c >= cols) return;
+	int64_t i = r * cols + c;
+	kx[i] = c + 1 < cols ? fmg::hmf(k[i], k[i + 1]) : 0.0f;
+	ky[i] = r + 1 < rows ? fmg::hmf(k[i], k[i + cols]) : 0.0f;
+}
+
 template <class T>
 struct Driver {
 	fx_multigrid* mg;
@@ -387,6 +398,8 @@ void free_levels(fx_multigrid* mg) {
 	for (auto& L : mg->lv) {
 		fx::dev_free(L.u); fx::dev_free(L.rhs); fx::dev_free(L.tmp); fx::dev_free(L.w0); fx::dev_free(L.w1);
 		fx::dev_free(L.k);
+		fx::dev_free(L.kx);
+		fx::dev_free(L.ky);
 		fx::dev_free(L.xbuf);
 	}
 	mg->lv.clear();
@@ -570,6 +583,19 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 	if (opt->transfer == FX_TRANSFER_FW_BILINEAR) {
 		for (size_t l = 0; l + 1 < mg->lv.size(); ++l) {
 			fx_status s = build_xtab(be, mg->lv[l], mg->lv[l + 1].g);
+			if (s != FX_OK) { free_levels(mg); delete mg; return s; }
+		}
+	}
+	if (opt->fused && opt->k && mg->f32) {
+		for (auto& L : mg->lv) {
+			const size_t bytes = (size_t)L.g.rows * L.g.cols * sizeof(float);
+			fx_status s = dev_alloc(be, bytes, &L.kx);
+			if (s == FX_OK) s = dev_alloc(be, bytes, &L.ky);
+			if (s == FX_OK) {
+				hipLaunchKernelGGL(k_faces, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, be->stream, (const float*)L.k,
+				                   (float*)L.kx, (float*)L.ky, (int64_t)L.g.rows, (int64_t)L.g.cols);
+				s = check_launch("multigrid face coefficients");
+			}
 			if (s != FX_OK) { free_levels(mg); delete mg; return s; }
 		}
 	}

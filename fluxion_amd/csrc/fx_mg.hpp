@@ -28,6 +28,8 @@ struct fx_multigrid {
 		void* w0 = nullptr;    // Chebyshev work (T[n])
 		void* w1 = nullptr;
 		void* k = nullptr;     // nodal coefficient (T[n]) of a variable-coefficient hierarchy
+		void* kx = nullptr;    // FP32 fused hierarchies: harmonic-mean face coefficient east of each point
+		void* ky = nullptr;    //   and south of each point (float[n]; 0 past the last column / row)
 		double w[5];           // sigma·I + scale·Δ_h weights, laplacian_2d_5pt order
 		double inv_diag;
 		fx_xtab xt;            // transfer to the next coarser level (linear transfers only)
@@ -37,6 +39,7 @@ struct fx_multigrid {
 	fx_mg_options opt{};
 	fx_boundaries bc{};
 	bool f32 = false;
+	bool streaming = true;   // fused cycle: streaming post kernel (k_spost) instead of the tiled one
 	std::vector<Level> lv;
 };
 

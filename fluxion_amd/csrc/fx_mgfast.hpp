@@ -25,8 +25,11 @@ struct LWf {
 	float w[5];
 	float inv_diag;
 	float omega;
-	// Variable-coefficient level operator when k != nullptr (fx_mg.hip vk_apply).
+	// Variable-coefficient level operator when k != nullptr (fx_mg.hip vk_apply);
+	// kx / ky: its face coefficients (fused hierarchies).
 	const float* k;
+	const float* kx;
+	const float* ky;
 	float ix2, iy2, sigma, scale;
 };
 
@@ -40,6 +43,21 @@ __device__ __forceinline__ float vkf(const LWf& L, float kc, float kn, float ks,
 	float Lv = -(fx * L.ix2 + fy * L.iy2);
 	*diag = fmaf(L.scale, (hw + he) * L.ix2 + (hn + hs) * L.iy2, L.sigma);
 	return fmaf(L.scale, Lv, L.sigma * c);
+}
+// vkf from the four face coefficients hw = hm(kc, kw), he, hn, hs (same arithmetic).
+__device__ __forceinline__ float vkff(const LWf& L, float hw, float he, float hn, float hs, float c, float n, float s,
+                                      float w, float e, float* diag) {
+	float fx = he * (e - c) - hw * (c - w);
+	float fy = hs * (s - c) - hn * (c - n);
+	float Lv = -(fx * L.ix2 + fy * L.iy2);
+	*diag = fmaf(L.scale, (hw + he) * L.ix2 + (hn + hs) * L.iy2, L.sigma);
+	return fmaf(L.scale, Lv, L.sigma * c);
+}
+__device__ __forceinline__ float jackf(const LWf& L, float hw, float he, float hn, float hs, float c, float n, float s,
+                                       float w, float e, float b) {
+	float d;
+	float au = vkff(L, hw, he, hn, hs, c, n, s, w, e, &d);
+	return fmaf(L.omega, (b - au) / d, c);
 }
 // Variable-coefficient Jacobi point update (fx_mg.hip k_jac with k).
 __device__ __forceinline__ float jack(const LWf& L, float kc, float kn, float ks, float kw, float ke, float c, float n,
@@ -94,7 +112,7 @@ static_assert(RCX * RCY <= BLOCK, "one coarse point per thread");
 // Dynamic LDS bytes of k_frestrict for a level.
 inline size_t frestrict_lds(const fx_xtab& X, bool vk) {
 	size_t f = 3 * (size_t)X.ry * X.rx;                  // b, two work planes
-	if (vk) f += (size_t)(X.ry + 2) * (X.rx + 2);        // k with one more ring
+	if (vk) f += (size_t)X.ry * (X.rx + 1) + (size_t)(X.ry + 1) * X.rx;  // west/east and north/south faces
 	f += 2 * (size_t)(X.ry + X.rx);                      // window weights (1 − w, w)
 	size_t i = (size_t)(X.ry + X.rx) + 2 * (RCY + RCX);  // window i0, tile lo/hi
 	return f * sizeof(float) + i * sizeof(int32_t);
@@ -106,12 +124,13 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
                                                      const int* skip) {
 	if (skip && *skip) return;
 	extern __shared__ float smem[];
-	const int RY = X.ry, RX = X.rx, KX = RX + 2;
+	const int RY = X.ry, RX = X.rx, FX = RX + 1;
 	float* sb = smem;
 	float* sa = sb + RY * RX;
 	float* su = sa + RY * RX;
-	float* sk = su + RY * RX;                    // VK only
-	float* wlr = sk + (VK ? (RY + 2) * KX : 0);  // [RY] float(1 − w) of the window's fine rows
+	float* sfx = su + RY * RX;                   // VK: [RY][RX + 1] face left of region column x
+	float* sfy = sfx + (VK ? RY * FX : 0);       // VK: [RY + 1][RX] face above region row y
+	float* wlr = sfy + (VK ? (RY + 1) * RX : 0); // [RY] float(1 − w) of the window's fine rows
 	float* whr = wlr + RY;                       // [RY] float(w)
 	float* wlc = whr + RY;                       // [RX]
 	float* whc = wlc + RX;
@@ -151,12 +170,21 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 		}
 	}
 	if constexpr (VK) {
-		for (int qy = ty; qy < ny + 2; qy += NWAVES) {
-			const int64_t gr = R0 - 1 + qy;
-			for (int qx = tx; qx < nx + 2; qx += WAVE) {
-				const int64_t gc = C0 - 1 + qx;
-				const bool in = gr >= 0 && gr < fr && gc >= 0 && gc < fc;
-				sk[qy * KX + qx] = in ? L.k[gr * fc + gc] : 1.0f;
+		// sfx[y][x] = face between region columns x − 1 and x; sfy[y][x] = between rows y − 1 and y.
+		for (int qy = ty; qy < ny; qy += NWAVES) {
+			const int64_t gr = R0 + qy;
+			for (int qx = tx; qx < nx + 1; qx += WAVE) {
+				const int64_t gc = C0 + qx - 1;
+				const bool in = gr >= 0 && gr < fr && gc >= 0 && gc < fc - 1;
+				sfx[qy * FX + qx] = in ? L.kx[gr * fc + gc] : 1.0f;
+			}
+		}
+		for (int qy = ty; qy < ny + 1; qy += NWAVES) {
+			const int64_t gr = R0 + qy - 1;
+			for (int qx = tx; qx < nx; qx += WAVE) {
+				const int64_t gc = C0 + qx;
+				const bool in = gr >= 0 && gr < fr - 1 && gc >= 0 && gc < fc;
+				sfy[qy * RX + qx] = in ? L.ky[gr * fc + gc] : 1.0f;
 			}
 		}
 	}
@@ -189,9 +217,8 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 	const int iR0 = (int)R0 - 1, iC0 = (int)C0 - 1;
 	const unsigned ur = (unsigned)(fr - 2), uc = (unsigned)(fc - 2);
 	auto interior = [&](int qy, int qx) { return (unsigned)(iR0 + qy) < ur && (unsigned)(iC0 + qx) < uc; };
-	// k at (qy, qx) and its four neighbours, in the (c, n, s, w, e) order of vkf.
-#define FMG_K5(y, x) sk[((y) + 1) * KX + (x) + 1], sk[(y) * KX + (x) + 1], sk[((y) + 2) * KX + (x) + 1], \
-	sk[((y) + 1) * KX + (x)], sk[((y) + 1) * KX + (x) + 2]
+	// Face coefficients of (qy, qx) in the (west, east, north, south) order of vkff.
+#define FMG_F4(y, x) sfx[(y) * FX + (x)], sfx[(y) * FX + (x) + 1], sfy[(y) * RX + (x)], sfy[((y) + 1) * RX + (x)]
 	// Pre-smoothing sweep 1 from zero over the whole region.
 	float* s1 = NPRE == 2 ? sa : su;
 	for (int qy = ty; qy < ny; qy += NWAVES) {
@@ -199,7 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 			const int q = qy * RX + qx;
 			float v = 0.0f;
 			if (interior(qy, qx)) {
-				if constexpr (VK) v = jack(L, FMG_K5(qy, qx), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[q]);
+				if constexpr (VK) v = jackf(L, FMG_F4(qy, qx), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[q]);
 				else v = jac0(L, sb[q]);
 			}
 			s1[q] = v;
@@ -212,7 +239,7 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 				const int q = qy * RX + qx;
 				float v = 0.0f;
 				if (interior(qy, qx)) {
-					if constexpr (VK) v = jack(L, FMG_K5(qy, qx), sa[q], sa[q - RX], sa[q + RX], sa[q - 1], sa[q + 1], sb[q]);
+					if constexpr (VK) v = jackf(L, FMG_F4(qy, qx), sa[q], sa[q - RX], sa[q + RX], sa[q - 1], sa[q + 1], sb[q]);
 					else v = jac(L, sa[q], sa[q - RX], sa[q + RX], sa[q - 1], sa[q + 1], sb[q]);
 				}
 				su[q] = v;
@@ -227,14 +254,14 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 			float s;
 			if constexpr (VK) {
 				float d;
-				s = vkf(L, FMG_K5(qy, qx), su[q], su[q - RX], su[q + RX], su[q - 1], su[q + 1], &d);
+				s = vkff(L, FMG_F4(qy, qx), su[q], su[q - RX], su[q + RX], su[q - 1], su[q + 1], &d);
 			} else {
 				s = sop(L, su[q], su[q - RX], su[q + RX], su[q - 1], su[q + 1]);
 			}
 			sa[q] = fmaf(1.0f, sb[q], -s);
 		}
 	}
-#undef FMG_K5
+#undef FMG_F4
 	__syncthreads();
 	if (!mine) return;
 	if (I == 0 || J == 0 || I == cr - 1 || J == cc - 1) {
@@ -298,7 +325,10 @@ static_assert(PTX == WAVE, "one tile column per lane in the final sweep");
 
 struct NoEpi {
 	static constexpr int NS = 0;
+	struct Pf {};
 	__device__ void term(int64_t, float, Acc*) const {}
+	__device__ Pf load(int64_t) const { return Pf{}; }
+	__device__ void use(const Pf&, float, Acc*) const {}
 };
 
 // Calls f(y, x) for every point of the NY × NX block at (OY, OX) of an LDS region.
@@ -321,7 +351,8 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 	__shared__ float sb[PRY][PRX];
 	__shared__ float sa[PRY][PRX];
 	__shared__ float su[PRY][PRX];
-	__shared__ float sk[VK ? PRY + 2 : 1][VK ? PRX + 2 : 1];
+	__shared__ float sfx[VK ? PRY : 1][VK ? PRX + 1 : 1];   // face left of region column x
+	__shared__ float sfy[VK ? PRY + 1 : 1][VK ? PRX : 1];   // face above region row y
 	__shared__ int32_t ry0[PRY], ry1[PRY], rx0[PRX], rx1[PRX];
 	__shared__ float rwy[PRY], rwx[PRX];
 	const int64_t r0 = (int64_t)blockIdx.y * PTY - PH, c0 = (int64_t)blockIdx.x * PTX - PH;
@@ -351,10 +382,15 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		sb[y][x] = in ? b[rr * cols + cl] : 0.0f;
 	});
 	if constexpr (VK) {
-		region<OB, OB, PRY + 2 - 2 * OB, PRX + 2 - 2 * OB>([&](int y, int x) {
-			const int64_t rr = r0 - 1 + y, cl = c0 - 1 + x;
-			const bool in = rr >= 0 && rr < rows && cl >= 0 && cl < cols;
-			sk[y][x] = in ? L.k[rr * cols + cl] : 1.0f;
+		region<OB, OB, PRY - 2 * OB, PRX + 1 - 2 * OB>([&](int y, int x) {
+			const int64_t rr = r0 + y, cl = c0 + x - 1;
+			const bool in = rr >= 0 && rr < rows && cl >= 0 && cl < cols - 1;
+			sfx[y][x] = in ? L.kx[rr * cols + cl] : 1.0f;
+		});
+		region<OB, OB, PRY + 1 - 2 * OB, PRX - 2 * OB>([&](int y, int x) {
+			const int64_t rr = r0 + y - 1, cl = c0 + x;
+			const bool in = rr >= 0 && rr < rows - 1 && cl >= 0 && cl < cols;
+			sfy[y][x] = in ? L.ky[rr * cols + cl] : 1.0f;
 		});
 	}
 	__syncthreads();
@@ -362,13 +398,13 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 	const int ir0 = (int)r0 - 1, ic0 = (int)c0 - 1;
 	const unsigned ur = (unsigned)(rows - 2), uc = (unsigned)(cols - 2);
 	auto interior = [&](int qy, int qx) { return (unsigned)(ir0 + qy) < ur && (unsigned)(ic0 + qx) < uc; };
-	// k at (qy, qx) and its four neighbours, in the (c, n, s, w, e) order of vkf.
-#define FMG_K5(y, x) sk[(y) + 1][(x) + 1], sk[(y)][(x) + 1], sk[(y) + 2][(x) + 1], sk[(y) + 1][(x)], sk[(y) + 1][(x) + 2]
+	// Face coefficients of (y, x) in the (west, east, north, south) order of vkff.
+#define FMG_F4(y, x) sfx[y][x], sfx[y][(x) + 1], sfy[y][x], sfy[(y) + 1][x]
 	if constexpr (NPRE == 2) {
 		region<O1, O1, PRY - 2 * O1, PRX - 2 * O1>([&](int y, int x) {
 			float v = 0.0f;
 			if (interior(y, x)) {
-				if constexpr (VK) v = jack(L, FMG_K5(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
+				if constexpr (VK) v = jackf(L, FMG_F4(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
 				else v = jac0(L, sb[y][x]);
 			}
 			sa[y][x] = v;
@@ -381,11 +417,11 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		if (interior(y, x)) {
 			if constexpr (NPRE == 2) {
 				if constexpr (VK)
-					v = jack(L, FMG_K5(y, x), sa[y][x], sa[y - 1][x], sa[y + 1][x], sa[y][x - 1], sa[y][x + 1], sb[y][x]);
+					v = jackf(L, FMG_F4(y, x), sa[y][x], sa[y - 1][x], sa[y + 1][x], sa[y][x - 1], sa[y][x + 1], sb[y][x]);
 				else
 					v = jac(L, sa[y][x], sa[y - 1][x], sa[y + 1][x], sa[y][x - 1], sa[y][x + 1], sb[y][x]);
 			} else {
-				if constexpr (VK) v = jack(L, FMG_K5(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
+				if constexpr (VK) v = jackf(L, FMG_F4(y, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, sb[y][x]);
 				else v = jac0(L, sb[y][x]);
 			}
 			const float wy = rwy[y], vy = 1.0f - wy, wx = rwx[x], vx = 1.0f - wx;
@@ -403,7 +439,7 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 			float v = su[y][x];
 			if (interior(y, x)) {
 				if constexpr (VK)
-					v = jack(L, FMG_K5(y, x), v, su[y - 1][x], su[y + 1][x], su[y][x - 1], su[y][x + 1], sb[y][x]);
+					v = jackf(L, FMG_F4(y, x), v, su[y - 1][x], su[y + 1][x], su[y][x - 1], su[y][x + 1], sb[y][x]);
 				else
 					v = jac(L, v, su[y - 1][x], su[y + 1][x], su[y][x - 1], su[y][x + 1], sb[y][x]);
 			}
@@ -421,7 +457,7 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		float v = last[y][x];
 		if (interior(y, x)) {
 			if constexpr (VK)
-				v = jack(L, FMG_K5(y, x), v, last[y - 1][x], last[y + 1][x], last[y][x - 1], last[y][x + 1], sb[y][x]);
+				v = jackf(L, FMG_F4(y, x), v, last[y - 1][x], last[y + 1][x], last[y][x - 1], last[y][x + 1], sb[y][x]);
 			else
 				v = jac(L, v, last[y - 1][x], last[y + 1][x], last[y][x - 1], last[y][x + 1], sb[y][x]);
 		}
@@ -429,8 +465,445 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		out[j] = v;
 		epi.term(j, v, acc);
 	});
-#undef FMG_K5
+#undef FMG_F4
 	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
+}
+
+// ── k_spost: streaming post kernel ───────────────────────────────────
+// The work of k_fpost reorganised for HBM streaming.  A workgroup owns a strip
+// of SSW = 58 output columns (a region of SRW = 64 columns, one wave wide, with
+// a halo of 3 each side) and a segment of SSH output rows, which it walks down
+// in chunks of SCH rows.  Each stage (b, first pre-sweep, pre-smoothed iterate
+// + P·e, first post-sweep, final sweep) runs a fixed number of rows ahead of
+// the next and keeps its recent rows in an LDS ring, so no row is computed
+// twice inside a segment; one extra chunk before the segment fills the rings.
+// The global loads of the next chunk are issued before the current chunk's
+// stages run.  Per-point arithmetic is that of k_fpost.
+constexpr int SRW = 64, SHX = 3, SSW = SRW - 2 * SHX, SCH = 8, SRB = 16;
+static_assert(SCH == 2 * NWAVES, "two rows per wave per chunk");
+
+template <bool VK, int NPRE, int NPOST, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spost(const float* __restrict__ b, const float* __restrict__ ec,
+                                                 float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
+                                                 int cc, int nstrips, int sh, Epi epi, RedScratch rs, const int* skip) {
+	static_assert(NPRE + NPOST - 1 <= SHX, "halo too small");
+	if (skip && *skip) return;
+	// Rows each stage runs ahead of the output rows of the chunk.
+	constexpr int LP1 = 1, LU = NPOST, LU1 = NPOST + 1, LB = NPRE == 2 ? LU1 : LU;
+	__shared__ float rb[SRB][SRW];
+	__shared__ float r1[NPRE == 2 ? SRB : 1][SRW];
+	__shared__ float ru[SRB][SRW];
+	__shared__ float rp[NPOST == 2 ? SRB : 1][SRW];
+	__shared__ float rfx[VK ? SRB : 1][VK ? SRW + 1 : 1];  // face west of region column x (x = 0..SRW)
+	__shared__ float rfy[VK ? SRB : 1][VK ? SRW : 1];      // face south of the row
+	const int lid = xcd_remap(blockIdx.x, gridDim.x);
+	const int strip = lid % nstrips, seg = lid / nstrips;
+	const int c0 = strip * SSW - SHX;
+	const int S0 = seg * sh, S1 = S0 + sh < rows ? S0 + sh : rows;
+	const int t = threadIdx.x, x = t & (WAVE - 1), w = t / WAVE;
+	const int gc = c0 + x;
+	const bool colin = gc >= 0 && gc < cols;
+	const unsigned ucols = (unsigned)(cols - 2), urows = (unsigned)(rows - 2);
+	const bool colint = (unsigned)(gc - 1) < ucols;
+	auto slot = [](int r) { return (r + 8 * SRB) & (SRB - 1); };
+	auto interior = [&](int r) { return colint && (unsigned)(r - 1) < urows; };
+	// Column interpolation data of this lane (fixed for the workgroup).
+	int b0 = 0, b1 = 0;
+	float wx = 0.0f, vx = 0.0f;
+	if (colint) {
+		const int a = X.i0c[gc];
+		b0 = a;
+		b1 = a + 1 < cc ? a + 1 : a;
+		wx = float(X.wc[gc]);
+		vx = 1.0f - wx;
+	}
+	// Loads of row r into registers (b, the west face of this lane's column and, for lane 0,
+	// the face east of the region, the south face) and their store into the rings.
+	struct Row {
+		float b, fx, fx2, fy;
+	};
+	auto load_row = [&](int r) {
+		Row v{0.0f, 1.0f, 1.0f, 1.0f};
+		if (r >= 0 && r < rows) {
+			const size_t o = (size_t)r * cols;
+			if (colin) v.b = b[o + gc];
+			if constexpr (VK) {
+				const int cw = gc - 1;
+				if (cw >= 0 && cw < cols - 1) v.fx = L.kx[o + cw];
+				if (x == 0) {
+					const int ce = c0 + SRW - 1;
+					if (ce >= 0 && ce < cols - 1) v.fx2 = L.kx[o + ce];
+				}
+				if (colin && r < rows - 1) v.fy = L.ky[o + gc];
+			}
+		}
+		return v;
+	};
+	auto store_row = [&](int r, const Row& v) {
+		const int s = slot(r);
+		rb[s][x] = v.b;
+		if constexpr (VK) {
+			rfx[s][x] = v.fx;
+			if (x == 0) rfx[s][SRW] = v.fx2;
+			rfy[s][x] = v.fy;
+		}
+	};
+	// Face coefficients of row r at lane column xx (west, east, north, south).
+#define FMG_SF4(r, xx) rfx[slot(r)][xx], rfx[slot(r)][(xx) + 1], rfy[slot((r) - 1)][xx], rfy[slot(r)][xx]
+	Acc acc[Epi::NS > 0 ? Epi::NS : 1];
+	for (int k = 0; k < (Epi::NS > 0 ? Epi::NS : 1); ++k) acc[k] = Acc{0.0, 0.0};
+	const int R0 = S0 - SCH;
+	// The first chunk also needs the row above its loads (north faces, and the stages' upper halo).
+	if (w == 0) store_row(R0 + LB - 1, load_row(R0 + LB - 1));
+	Row nx0 = load_row(R0 + LB + w), nx1 = load_row(R0 + LB + w + NWAVES);
+	for (int R = R0; R < S1; R += SCH) {
+		__syncthreads();  // the previous chunk's stages are done with the ring rows reused below
+		store_row(R + LB + w, nx0);
+		store_row(R + LB + w + NWAVES, nx1);
+		// This chunk's coarse-correction values (rows of the u stage) and epilogue operands,
+		// issued before the next chunk's loads so that waiting on them never waits on those.
+		float e00[2], e01[2], e10[2], e11[2];
+		float wyv[2];
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + LU + w + h * NWAVES;
+			e00[h] = e01[h] = e10[h] = e11[h] = 0.0f;
+			wyv[h] = 0.0f;
+			if (interior(r)) {
+				const int a0 = X.i0r[r], a1 = a0 + 1 < cr ? a0 + 1 : a0;
+				wyv[h] = float(X.wr[r]);
+				e00[h] = ec[(size_t)a0 * cc + b0];
+				e01[h] = ec[(size_t)a0 * cc + b1];
+				e10[h] = ec[(size_t)a1 * cc + b0];
+				e11[h] = ec[(size_t)a1 * cc + b1];
+			}
+		}
+		typename Epi::Pf pf[2];
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + w + h * NWAVES;
+			if (r >= S0 && r < S1 && x >= SHX && x < SRW - SHX && colin) pf[h] = epi.load((int64_t)r * cols + gc);
+		}
+		nx0 = load_row(R + SCH + LB + w);
+		nx1 = load_row(R + SCH + LB + w + NWAVES);
+		__syncthreads();
+		if constexpr (NPRE == 2) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const int r = R + LU1 + w + h * NWAVES;
+				float v = 0.0f;
+				if (interior(r)) {
+					if constexpr (VK) v = jackf(L, FMG_SF4(r, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, rb[slot(r)][x]);
+					else v = jac0(L, rb[slot(r)][x]);
+				}
+				r1[slot(r)][x] = v;
+			}
+			__syncthreads();
+		}
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + LU + w + h * NWAVES;
+			float v = 0.0f;
+			if (interior(r) && x > 0 && x < SRW - 1) {
+				const int s = slot(r);
+				if constexpr (NPRE == 2) {
+					const int sn = slot(r - 1), ss = slot(r + 1);
+					if constexpr (VK)
+						v = jackf(L, FMG_SF4(r, x), r1[s][x], r1[sn][x], r1[ss][x], r1[s][x - 1], r1[s][x + 1], rb[s][x]);
+					else
+						v = jac(L, r1[s][x], r1[sn][x], r1[ss][x], r1[s][x - 1], r1[s][x + 1], rb[s][x]);
+				} else {
+					if constexpr (VK) v = jackf(L, FMG_SF4(r, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, rb[s][x]);
+					else v = jac0(L, rb[s][x]);
+				}
+				const float wy = wyv[h], vy = 1.0f - wy;
+				const float top = vx * e00[h] + wx * e01[h];
+				const float bot = vx * e10[h] + wx * e11[h];
+				v += vy * top + wy * bot;
+			}
+			ru[slot(r)][x] = v;
+		}
+		__syncthreads();
+		float (*last)[SRW] = ru;
+		if constexpr (NPOST == 2) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const int r = R + LP1 + w + h * NWAVES;
+				const int s = slot(r);
+				float v = ru[s][x];
+				if (interior(r) && x > 1 && x < SRW - 2) {
+					const int sn = slot(r - 1), ss = slot(r + 1);
+					if constexpr (VK)
+						v = jackf(L, FMG_SF4(r, x), v, ru[sn][x], ru[ss][x], ru[s][x - 1], ru[s][x + 1], rb[s][x]);
+					else
+						v = jac(L, v, ru[sn][x], ru[ss][x], ru[s][x - 1], ru[s][x + 1], rb[s][x]);
+				}
+				rp[s][x] = v;
+			}
+			__syncthreads();
+			last = rp;
+		}
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + w + h * NWAVES;
+			if (r < S0 || r >= S1 || x < SHX || x >= SRW - SHX || !colin) continue;
+			const int s = slot(r);
+			float v = last[s][x];
+			if (interior(r)) {
+				const int sn = slot(r - 1), ss = slot(r + 1);
+				if constexpr (VK)
+					v = jackf(L, FMG_SF4(r, x), v, last[sn][x], last[ss][x], last[s][x - 1], last[s][x + 1], rb[s][x]);
+				else
+					v = jac(L, v, last[sn][x], last[ss][x], last[s][x - 1], last[s][x + 1], rb[s][x]);
+			}
+			const size_t j = (size_t)r * cols + gc;
+			out[j] = v;
+			epi.use(pf[h], v, acc);
+		}
+	}
+#undef FMG_SF4
+	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
+}
+
+inline unsigned spost_blocks(int64_t rows, int64_t cols, int sh) {
+	return (unsigned)(((cols + SSW - 1) / SSW) * ((rows + sh - 1) / sh));
+}
+// Segment height (rows, a multiple of SCH) for a level: the tallest of 128, 64, 32 that still gives
+// about 2048 workgroups; 0 when even 32 gives fewer than 1024 (the tiled kernel is used then).
+inline int spost_height(int64_t rows, int64_t cols) {
+	int sh = 128;
+	while (sh > 32 && spost_blocks(rows, cols, sh) < 2048) sh /= 2;
+	return spost_blocks(rows, cols, sh) >= 1024 ? sh : 0;
+}
+
+// ── k_srestrict: streaming residual + restriction ─────────────────────
+// The work of k_frestrict reorganised like k_spost.  A workgroup owns RCX
+// coarse columns (a fine window of at most SRW columns, one wave wide) and a
+// segment of SCR coarse rows, and walks the fine rows down in chunks of SCH:
+// b (and faces) → first pre-sweep → pre-smoothed iterate → residual, each stage
+// a few rows ahead of the next in LDS rings; after each chunk every coarse row
+// whose fine rows are all available is gathered.  Per-point arithmetic and the
+// gather order are those of k_frestrict.
+
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_srestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
+                                                     fx_xtab X, int fr, int fc, int cr, int cc, int nstrips,
+                                                     int scr, const int* skip) {
+	if (skip && *skip) return;
+	constexpr int LR = 0, LU = 1, LU1 = 2, LB = NPRE == 2 ? LU1 : LU;
+	__shared__ float rb[SRB][SRW];
+	__shared__ float r1[NPRE == 2 ? SRB : 1][SRW];
+	__shared__ float ru[SRB][SRW];
+	__shared__ float rr[SRB][SRW];
+	__shared__ float rfx[VK ? SRB : 1][VK ? SRW + 1 : 1];
+	__shared__ float rfy[VK ? SRB : 1][VK ? SRW : 1];
+	__shared__ int32_t ci0[SRW];
+	__shared__ float cwl[SRW], cwh[SRW];
+	__shared__ int32_t clo[RCX], chi[RCX];
+	__shared__ float rsr[SRB][RCX];   // per fine row: Σ_jj w(jj, J)·res(row, jj) for the strip's coarse columns
+	__shared__ int32_t ri0[SRB];      // per fine row: coarse row below it and its two weights
+	__shared__ float rwl[SRB], rwh[SRB];
+	const int lid = xcd_remap(blockIdx.x, gridDim.x);
+	const int strip = lid % nstrips, seg = lid / nstrips;
+	const int J0 = strip * RCX, I0 = seg * scr;
+	const int Jend = J0 + RCX < cc ? J0 + RCX : cc, Iend = I0 + scr < cr ? I0 + scr : cr;
+	const int t = threadIdx.x, x = t & (WAVE - 1), w = t / WAVE;
+	// Boundary coarse rows and columns are zero.
+	for (int q = t; q < (Jend - J0); q += BLOCK) {
+		if (I0 == 0) bco[(size_t)J0 + q] = 0.0f;
+		if (Iend == cr) bco[(size_t)(cr - 1) * cc + J0 + q] = 0.0f;
+	}
+	const int Ia = I0 > 1 ? I0 : 1, Ib = Iend - 1 < cr - 2 ? Iend - 1 : cr - 2;
+	const int Ja = J0 > 1 ? J0 : 1, Jb = Jend - 1 < cc - 2 ? Jend - 1 : cc - 2;
+	if (Ia > Ib) return;
+	const int G0 = Ja <= Jb ? X.loc[Ja] : 0, G1 = Ja <= Jb ? X.hic[Jb] : -1;
+	const int F0 = X.lor[Ia], F1 = X.hir[Ib];
+	if (G1 < G0 || F1 < F0) {  // nothing to gather: the strip's interior rows are zero
+		const int nc = Jend - J0;
+		for (int q = t; q < (Ib - Ia + 1) * nc; q += BLOCK) bco[(size_t)(Ia + q / nc) * cc + J0 + q % nc] = 0.0f;
+		return;
+	}
+	const int C0 = G0 - 2;  // fine column of lane 0
+	const int gc = C0 + x;
+	const bool colin = gc >= 0 && gc < fc;
+	const unsigned ucols = (unsigned)(fc - 2), urows = (unsigned)(fr - 2);
+	const bool colint = (unsigned)(gc - 1) < ucols;
+	auto slot = [](int r) { return (r + 8 * SRB) & (SRB - 1); };
+	auto interior = [&](int r) { return colint && (unsigned)(r - 1) < urows; };
+	// Column weights of the lane's fine column and the strip's coarse column ranges.
+	if (colint) {
+		const double wv = X.wc[gc];
+		ci0[x] = X.i0c[gc];
+		cwl[x] = float(1.0 - wv);
+		cwh[x] = float(wv);
+	} else if (w == 0) {
+		ci0[x] = -2;
+		cwl[x] = cwh[x] = 0.0f;
+	}
+	if (t >= WAVE && t < WAVE + RCX) {
+		const int J = J0 + t - WAVE;
+		clo[t - WAVE] = J < cc ? X.loc[J] : 0;
+		chi[t - WAVE] = J < cc ? X.hic[J] : -1;
+	}
+	struct Row {
+		float b, fx, fx2, fy;
+	};
+	auto load_row = [&](int r) {
+		Row v{0.0f, 1.0f, 1.0f, 1.0f};
+		if (r >= 0 && r < fr) {
+			const size_t o = (size_t)r * fc;
+			if (colin) v.b = b[o + gc];
+			if constexpr (VK) {
+				const int cw = gc - 1;
+				if (cw >= 0 && cw < fc - 1) v.fx = L.kx[o + cw];
+				if (x == 0) {
+					const int ce = C0 + SRW - 1;
+					if (ce >= 0 && ce < fc - 1) v.fx2 = L.kx[o + ce];
+				}
+				if (colin && r < fr - 1) v.fy = L.ky[o + gc];
+			}
+		}
+		return v;
+	};
+	auto store_row = [&](int r, const Row& v) {
+		const int s = slot(r);
+		rb[s][x] = v.b;
+		if constexpr (VK) {
+			rfx[s][x] = v.fx;
+			if (x == 0) rfx[s][SRW] = v.fx2;
+			rfy[s][x] = v.fy;
+		}
+	};
+#define FMG_SF4(r, xx) rfx[slot(r)][xx], rfx[slot(r)][(xx) + 1], rfy[slot((r) - 1)][xx], rfy[slot(r)][xx]
+	const int R0 = F0 - SCH;
+	if (w == 0) store_row(R0 + LB - 1, load_row(R0 + LB - 1));
+	Row nx0 = load_row(R0 + LB + w), nx1 = load_row(R0 + LB + w + NWAVES);
+	int Inext = Ia;  // next coarse row to gather
+	const float scale = X.sy * X.sx;
+	for (int R = R0; Inext <= Ib; R += SCH) {
+		__syncthreads();
+		store_row(R + LB + w, nx0);
+		store_row(R + LB + w + NWAVES, nx1);
+		nx0 = load_row(R + SCH + LB + w);
+		nx1 = load_row(R + SCH + LB + w + NWAVES);
+		__syncthreads();
+		if constexpr (NPRE == 2) {
+#pragma unroll
+			for (int h = 0; h < 2; ++h) {
+				const int r = R + LU1 + w + h * NWAVES;
+				float v = 0.0f;
+				if (interior(r)) {
+					if constexpr (VK) v = jackf(L, FMG_SF4(r, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, rb[slot(r)][x]);
+					else v = jac0(L, rb[slot(r)][x]);
+				}
+				r1[slot(r)][x] = v;
+			}
+			__syncthreads();
+		}
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + LU + w + h * NWAVES;
+			const int s = slot(r);
+			float v = 0.0f;
+			if constexpr (NPRE == 2) {
+				if (interior(r) && x > 0 && x < SRW - 1) {
+					const int sn = slot(r - 1), ss = slot(r + 1);
+					if constexpr (VK)
+						v = jackf(L, FMG_SF4(r, x), r1[s][x], r1[sn][x], r1[ss][x], r1[s][x - 1], r1[s][x + 1], rb[s][x]);
+					else
+						v = jac(L, r1[s][x], r1[sn][x], r1[ss][x], r1[s][x - 1], r1[s][x + 1], rb[s][x]);
+				}
+			} else {
+				if (interior(r)) {
+					if constexpr (VK) v = jackf(L, FMG_SF4(r, x), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, rb[s][x]);
+					else v = jac0(L, rb[s][x]);
+				}
+			}
+			ru[s][x] = v;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int r = R + LR + w + h * NWAVES;
+			const int s = slot(r);
+			float res = 0.0f;
+			if (interior(r) && x > 0 && x < SRW - 1) {
+				const int sn = slot(r - 1), ss = slot(r + 1);
+				float sv;
+				if constexpr (VK) {
+					float d;
+					sv = vkff(L, FMG_SF4(r, x), ru[s][x], ru[sn][x], ru[ss][x], ru[s][x - 1], ru[s][x + 1], &d);
+				} else {
+					sv = sop(L, ru[s][x], ru[sn][x], ru[ss][x], ru[s][x - 1], ru[s][x + 1]);
+				}
+				res = fmaf(1.0f, rb[s][x], -sv);
+			}
+			rr[s][x] = res;
+		}
+		__syncthreads();
+		// Row pass: each fine row of the chunk against each coarse column of the strip (the inner sums of
+		// the restriction, in the fine-column order of k_frestrict), plus the row's interpolation data.
+		const int ncol = Jend - J0;
+		if (t < SCH * RCX) {
+			const int hr = t / RCX, jx = t % RCX;
+			const int r = R + LR + hr, J = J0 + jx;
+			const int s = slot(r);
+			if (jx < ncol && J > 0 && J < cc - 1) {
+				const int lo = clo[jx] - C0, hi = chi[jx] - C0;
+				const float* row = rr[s];
+				float rs = 0.0f;
+				for (int jj = lo; jj <= hi; ++jj) {
+					const int ac = ci0[jj];
+					const float wxv = ac == J ? cwl[jj] : (ac + 1 == J ? cwh[jj] : 0.0f);
+					rs += wxv * row[jj];
+				}
+				rsr[s][jx] = rs;
+			}
+			if (jx == 0) {
+				const bool in = (unsigned)(r - 1) < urows;
+				const double wr = in ? X.wr[r] : 0.0;
+				ri0[s] = in ? X.i0r[r] : -2;
+				rwl[s] = float(1.0 - wr);
+				rwh[s] = float(wr);
+			}
+		}
+		__syncthreads();
+		// Column pass: coarse rows whose fine rows lo..hi have all been computed.
+		const int last = R + SCH - 1;
+		int Iready = Inext;
+		while (Iready <= Ib && X.hir[Iready] <= last) ++Iready;
+		const int nrow = Iready - Inext;
+		for (int q = t; q < nrow * ncol; q += BLOCK) {
+			const int I = Inext + q / ncol, jx = q % ncol, J = J0 + jx;
+			float outv = 0.0f;
+			if (J > 0 && J < cc - 1) {
+				const int rlo = X.lor[I], rhi = X.hir[I];
+				float acc = 0.0f;
+				for (int ii = rlo; ii <= rhi; ++ii) {
+					const int s = slot(ii);
+					const int ar = ri0[s];
+					const float wy = ar == I ? rwl[s] : (ar + 1 == I ? rwh[s] : 0.0f);
+					acc += wy * rsr[s][jx];
+				}
+				outv = acc * scale;
+			}
+			bco[(size_t)I * cc + J] = outv;
+		}
+		Inext = Iready;
+	}
+#undef FMG_SF4
+}
+
+inline unsigned srestrict_blocks(int64_t cr, int64_t cc, int scr) {
+	return (unsigned)(((cc + RCX - 1) / RCX) * ((cr + scr - 1) / scr));
+}
+// Coarse rows per segment (64, 32 or 16) for about 2048 workgroups; 0 when the level's fine window
+// is wider than a wave or even 16 rows give fewer than 1024 workgroups (tiled kernel then).
+inline int srestrict_height(const fx_xtab& X, int64_t cr, int64_t cc) {
+	if (X.rx > SRW) return 0;
+	int scr = 64;
+	while (scr > 16 && srestrict_blocks(cr, cc, scr) < 2048) scr /= 2;
+	return srestrict_blocks(cr, cc, scr) >= 1024 ? scr : 0;
 }
 
 inline dim3 rgrid(int64_t cr, int64_t cc) {
@@ -446,6 +919,8 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 	o.inv_diag = float(L.inv_diag);
 	o.omega = float(mg->opt.omega);
 	o.k = (const float*)L.k;
+	o.kx = (const float*)L.kx;
+	o.ky = (const float*)L.ky;
 	o.ix2 = float(1.0 / (L.g.dx * L.g.dx));
 	o.iy2 = float(1.0 / (L.g.dy * L.g.dy));
 	o.sigma = float(mg->opt.sigma);
@@ -476,10 +951,16 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
 		if (l == 0) fx::probe_begin(be, FX_PROBE_VC_RESTRICT0);
-		hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
-		                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs,
-		                   lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows,
-		                   (int64_t)cg.cols, skip);
+		const int scr = mg->streaming ? srestrict_height(lv[l].xt, cg.rows, cg.cols) : 0;
+		if (scr > 0)
+			hipLaunchKernelGGL((k_srestrict<VK, NPRE>), dim3(srestrict_blocks(cg.rows, cg.cols, scr)), dim3(BLOCK), 0, st,
+			                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lwf(mg, lv[l]), lv[l].xt, (int)g.rows,
+			                   (int)g.cols, (int)cg.rows, (int)cg.cols, (int)((cg.cols + RCX - 1) / RCX), scr, skip);
+		else
+			hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
+			                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs,
+			                   lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows,
+			                   (int64_t)cg.cols, skip);
 		if (l == 0) fx::probe_end(be, FX_PROBE_VC_RESTRICT0);
 	}
 	{
@@ -487,29 +968,39 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		hipLaunchKernelGGL(k_fcoarse, dim3(1), dim3(1024), 0, st, (const float*)L.rhs, (float*)L.tmp, lwf(mg, L),
 		                   o.n_pre + o.n_post + o.extra_coarse, (int)L.g.rows, (int)L.g.cols, skip);
 	}
-	for (int l = nl - 2; l >= 1; --l) {
+	auto post = [&](int l, auto epi, RedScratch rs, unsigned* nblocks) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
-		hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, NoEpi>), tgrid(g.rows, g.cols), dim3(BLOCK), 0, st,
-		                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
-		                   lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, NoEpi{},
-		                   RedScratch{nullptr}, skip);
-	}
-	const fx_grid& g = lv[0].g;
-	const fx_grid& cg = lv[1].g;
-	dim3 tg = tgrid(g.rows, g.cols);
+		const int sh = mg->streaming ? spost_height(g.rows, g.cols) : 0;
+		if (sh > 0) {
+			const unsigned nbk = spost_blocks(g.rows, g.cols, sh);
+			const int ns = (int)((g.cols + SSW - 1) / SSW);
+			hipLaunchKernelGGL((k_spost<VK, NPRE, NPOST, decltype(epi)>), dim3(nbk), dim3(BLOCK), 0, st,
+			                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
+			                   lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, ns, sh, epi, rs, skip);
+			*nblocks = nbk;
+		} else {
+			const dim3 tg = tgrid(g.rows, g.cols);
+			hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, decltype(epi)>), tg, dim3(BLOCK), 0, st, (const float*)lv[l].rhs,
+			                   (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows,
+			                   (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, epi, rs, skip);
+			*nblocks = tg.x * tg.y;
+		}
+	};
+	unsigned nbk = 0;
+	for (int l = nl - 2; l >= 1; --l) post(l, NoEpi{}, RedScratch{nullptr}, &nbk);
 	RedScratch rs{nullptr};
 	if (Epi::NS > 0) {
-		FX_CHECK(fx::ensure_partials(be, (size_t)tg.x * tg.y * Epi::NS));
+		const fx_grid& g = lv[0].g;
+		const dim3 tg = tgrid(g.rows, g.cols);
+		FX_CHECK(fx::ensure_partials(be, (size_t)(tg.x * tg.y + spost_blocks(g.rows, g.cols, 32)) * Epi::NS));
 		rs = fx::red_scratch(be);
 	}
 	fx::probe_begin(be, FX_PROBE_VC_POST0);
-	hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, Epi>), tg, dim3(BLOCK), 0, st, (const float*)lv[0].rhs,
-	                   (const float*)lv[1].tmp, (float*)lv[0].tmp, lwf(mg, lv[0]), lv[0].xt, (int64_t)g.rows,
-	                   (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, epi0, rs, skip);
+	post(0, epi0, rs, &nbk);
 	fx::probe_end(be, FX_PROBE_VC_POST0);
 	if constexpr (Epi::NS > 0)
-		hipLaunchKernelGGL((k_finish<Epi::NS, typename Epi::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, rs, tg.x * tg.y,
+		hipLaunchKernelGGL((k_finish<Epi::NS, typename Epi::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, rs, nbk,
 		                   epi0.fin(), skip);
 	return fx::check_launch("fused V-cycle");
 }

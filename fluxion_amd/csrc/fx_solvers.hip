@@ -186,8 +186,10 @@ __global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, W9 w, int64_t row
 	// Same traversal as fxd::k_stream5, reading β from the device state first.
 	if (st->done) return;
 	v.be = st->beta;
-	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t c = (int64_t)bx * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool col_ok = c < cols;
 	const bool col_int = c > 0 && c < cols - 1;
@@ -227,8 +229,10 @@ __global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __r
                                                      RedScratch rs, const SState* st) {
 	if (st->done) return;
 	v.be = st->beta;
-	const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t c = (int64_t)bx * BLOCK + threadIdx.x;
+	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool col_ok = c < cols;
 	const bool col_int = c > 0 && c < cols - 1;
@@ -300,8 +304,10 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
                                                      RedScratch rs, const SState* st) {
 	if (st->done) return;
 	const double be = st->beta;
-	const int64_t c0 = 2 * ((int64_t)blockIdx.x * BLOCK + threadIdx.x);
-	const int64_t r0 = (int64_t)blockIdx.y * rpb;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t c0 = 2 * ((int64_t)bx * BLOCK + threadIdx.x);
+	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool ok = c0 < cols;
 	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;   // column interior tests (c0 + 1 > 0 always)
@@ -619,6 +625,38 @@ struct FPackIn {
 		x[2 * i + 1] = logk ? float(logk[i]) : 0.0f;
 	}
 };
+// c = s·h²·y on interior points, 0 on the boundary (y = channel 0 of the [p][16] network output).
+struct FNetC {
+	double* c;
+	const float* y;
+	const double* s;
+	double h2;
+	int64_t rows, cols;
+	double sc;
+	__device__ void load() { sc = *s * h2; }
+	__device__ void operator()(size_t i) const {
+		int64_t r = (int64_t)i / cols, col = (int64_t)i % cols;
+		c[i] = (r == 0 || col == 0 || r == rows - 1 || col == cols - 1) ? 0.0 : sc * double(y[16 * i]);
+	}
+};
+// α* = (c·r1)/(c·Ac), the step along c that minimises the A-norm of the error left after the V-cycle
+// (r1 = A·e1), clipped at 0: adding α*·c never increases that error.
+struct FinStep {
+	double* a;
+	__device__ void operator()(Acc* o) const {
+		const double cr = acc_value(o[0]), cac = acc_value(o[1]);
+		*a = (cac > 0.0 && cr > 0.0) ? cr / cac : 0.0;
+	}
+};
+// z += α*·c.
+struct FAddStep {
+	double* z;
+	const double* c;
+	const double* a;
+	double al;
+	__device__ void load() { al = *a; }
+	__device__ void operator()(size_t i) const { z[i] = fma(al, c[i], z[i]); }
+};
 // z += s·h²·y on interior points (y = channel 0 of the [p][16] network output).
 struct FAddNet {
 	double* z;
@@ -679,6 +717,11 @@ struct EpiRZ {
 	const double* r;
 	SState* st;
 	__device__ void term(int64_t j, float z, Acc* a) const { acc_add(a[0], r[j] * double(z)); }
+	struct Pf {
+		double r;
+	};
+	__device__ Pf load(int64_t j) const { return Pf{r[j]}; }
+	__device__ void use(const Pf& p, float z, Acc* a) const { acc_add(a[0], p.r * double(z)); }
 	using Fin = FinPcgInitRZ;
 	Fin fin() const { return Fin{st}; }
 };
@@ -690,6 +733,15 @@ struct EpiFcg {
 		double zd = double(z);
 		acc_add(a[0], r[j] * zd);
 		acc_add(a[1], zd * ap[j]);
+	}
+	struct Pf {
+		double r, ap;
+	};
+	__device__ Pf load(int64_t j) const { return Pf{r[j], ap[j]}; }
+	__device__ void use(const Pf& p, float z, Acc* a) const {
+		const double zd = double(z);
+		acc_add(a[0], p.r * zd);
+		acc_add(a[1], zd * p.ap);
 	}
 	using Fin = FinFcgBeta;
 	Fin fin() const { return Fin{st}; }
@@ -1022,8 +1074,10 @@ struct fx_learned {
 	const fx_buf* logk = nullptr;
 	fx_grid g{};
 	double stall = 0.9;
-	double* s_dev = nullptr;
+	double* s_dev = nullptr;   // [0] rms(r1), [1] the step α* of the network correction
 	double* r1 = nullptr;
+	double* c = nullptr;       // the network correction
+	double* ac = nullptr;      // A·c
 	float* xin = nullptr;
 	int fell_back = 0;
 };
@@ -1111,9 +1165,12 @@ struct Ctx {
 		FX_CHECK(check_launch("learned pack"));
 		const float* y = nullptr;
 		FX_CHECK(fx::unet_infer(L->net, L->xin, 1, (int)rows, (int)cols, lsk, &y));
-		hipLaunchKernelGGL(k_eupd<FAddNet>, dim3(eb()), dim3(BLOCK), 0, st,
-		                   FAddNet{z, y, L->s_dev, s->g.dx * s->g.dy, rows, cols, 0.0}, n, lsk);
-		return check_launch("learned add");
+		hipLaunchKernelGGL(k_eupd<FNetC>, dim3(eb()), dim3(BLOCK), 0, st,
+		                   FNetC{L->c, y, L->s_dev, s->g.dx * s->g.dy, rows, cols, 0.0}, n, lsk);
+		FX_CHECK(check_launch("learned correction"));
+		FX_CHECK(apply_op(L->c, L->ac, lsk));
+		FX_CHECK(ered<2>(FDot2{L->c, L->r1, L->c, L->ac}, FinStep{L->s_dev + 1}, "learned step", lsk));
+		return eupd(FAddStep{z, L->c, L->s_dev + 1, 0.0}, "learned add", lsk);
 	}
 
 	// p_out = src + β·p_old; ap = A·p_out; p·ap -> α (FinAlpha).
@@ -1640,10 +1697,14 @@ fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, cons
 	L->stall = stall;
 	void* p;
 	fx_status st;
-	if ((st = dev_alloc(be, sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	if ((st = dev_alloc(be, 2 * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
 	L->s_dev = (double*)p;
 	if ((st = dev_alloc(be, n * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
 	L->r1 = (double*)p;
+	if ((st = dev_alloc(be, n * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->c = (double*)p;
+	if ((st = dev_alloc(be, n * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->ac = (double*)p;
 	if ((st = dev_alloc(be, n * 2 * sizeof(float), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
 	L->xin = (float*)p;
 	*out = L;
@@ -1655,6 +1716,8 @@ void fx_learned_destroy(fx_learned* L) {
 	if (L->be) (void)hipStreamSynchronize(L->be->stream);
 	dev_free(L->s_dev);
 	dev_free(L->r1);
+	dev_free(L->c);
+	dev_free(L->ac);
 	dev_free(L->xin);
 	delete L;
 }

@@ -254,6 +254,50 @@ __global__ void k_loss_grad(const float* y, int ystride, const float* t, const f
 	float tt = sums[2 * b + 1];
 	dy[q * ystride] = 2.f * (y[q * ystride] - t[q]) / tt;
 }
+// Relative energy-norm loss per sample: dᵀA d / tᵀA t with d = y − t and A the unit-spacing
+// 5-point −∇·(k∇·) on interior pixels (k = exp of input channel 1, harmonic face means,
+// zero outside the interior).  Writes A·d to dy (k_eloss_scale turns it into the gradient).
+__device__ __forceinline__ float hmk(float a, float b) { return 2.f * a * b / (a + b); }
+__global__ void k_eloss(const float* x, int cin, const float* y, int ystride, const float* t, float* sums, float* dy,
+                        int H, int W) {
+	const int b = blockIdx.y;
+	const long hw = (long)H * W;
+	const size_t base = (size_t)b * hw;
+	auto inner = [&](int r, int c) { return r > 0 && c > 0 && r < H - 1 && c < W - 1; };
+	auto K = [&](long p) { return expf(x[(base + p) * cin + 1]); };
+	auto D = [&](int r, int c) { return inner(r, c) ? y[(base + (long)r * W + c) * ystride] - t[base + (long)r * W + c] : 0.f; };
+	auto T = [&](int r, int c) { return inner(r, c) ? t[base + (long)r * W + c] : 0.f; };
+	float e = 0.f, tt = 0.f;
+	for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < hw; p += (long)gridDim.x * blockDim.x) {
+		const int r = (int)(p / W), c = (int)(p % W);
+		if (!inner(r, c)) {
+			dy[(base + p) * ystride] = 0.f;
+			continue;
+		}
+		const float kc = K(p);
+		const float hw_ = hmk(kc, K(p - 1)), he = hmk(kc, K(p + 1)), hn = hmk(kc, K(p - W)), hs = hmk(kc, K(p + W));
+		const float dc = D(r, c), tc = T(r, c);
+		const float ad = hw_ * (dc - D(r, c - 1)) + he * (dc - D(r, c + 1)) + hn * (dc - D(r - 1, c)) + hs * (dc - D(r + 1, c));
+		const float at = hw_ * (tc - T(r, c - 1)) + he * (tc - T(r, c + 1)) + hn * (tc - T(r - 1, c)) + hs * (tc - T(r + 1, c));
+		dy[(base + p) * ystride] = ad;
+		e += dc * ad;
+		tt += tc * at;
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		e += __shfl_xor(e, off);
+		tt += __shfl_xor(tt, off);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		atomicAdd(&sums[2 * b], e);
+		atomicAdd(&sums[2 * b + 1], tt);
+	}
+}
+__global__ void k_eloss_scale(float* dy, int ystride, const float* sums, int B, long hw) {
+	long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= (long)B * hw) return;
+	int b = q / hw;
+	dy[q * ystride] *= 2.f / sums[2 * b + 1];
+}
 // FP32 master weights [tap][cout][cin] -> BF16 padded [tap][cout_p][cin_p] and the
 // transposed copy [tap][cin_p][cout_p] used for the data gradient.
 __global__ void k_pack_w(const float* w, bf16* wp, bf16* wt, int cin, int cout, int cin_p, int cout_p) {
@@ -610,6 +654,8 @@ static fx_status run_forward(fx_unet* u, const float* x, int32_t B, int32_t H, i
 
 static fx_status check_tensors(fx_unet* u, const fx_buf* x, const fx_buf* y, int32_t B, int32_t H, int32_t W) {
 	FX_REQUIRE(u && x && y && B > 0 && H > 0 && W > 0, FX_ERR_ARGUMENT, "bad arguments");
+	FX_REQUIRE(H % 8 == 0 && W % 8 == 0, FX_ERR_DIMENSION,
+	           "the U-Net pools three times: image sides must be multiples of 8, got %dx%d", H, W);
 	size_t npix = (size_t)B * H * W;
 	FX_CHECK(check_buf(x, npix * u->cin, FX_F32, "x"));
 	FX_CHECK(check_buf(y, npix, FX_F32, "y/t"));
@@ -649,6 +695,31 @@ fx_status fx_unet_loss_grad(fx_unet* u, const fx_buf* xb, const fx_buf* tb, int3
 	return FX_OK;
 }
 
+// Relative energy-norm loss (k_eloss) and its gradient (kept on the device).
+fx_status fx_unet_loss_grad_energy(fx_unet* u, const fx_buf* xb, const fx_buf* tb, int32_t B, int32_t H, int32_t W,
+                                   double* loss) {
+	FX_CHECK(check_tensors(u, xb, tb, B, H, W));
+	FX_REQUIRE(u->cin >= 2, FX_ERR_ARGUMENT, "the energy loss reads log k from input channel 1");
+	const float* t = (const float*)tb->ptr;
+	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W));
+	hipStream_t st = u->be->stream;
+	long hw = (long)H * W;
+	FX_HIP(hipMemsetAsync(u->sums, 0, (size_t)B * 2 * 4, st));
+	FX_HIP(hipMemsetAsync(u->dy, 0, (size_t)B * hw * 16 * 4, st));
+	hipLaunchKernelGGL(k_eloss, dim3(64, B), dim3(256), 0, st, (const float*)xb->ptr, u->cin, u->yout, 16, t, u->sums,
+	                   u->dy, H, W);
+	hipLaunchKernelGGL(k_eloss_scale, dim3(nb(B * hw)), dim3(256), 0, st, u->dy, 16, u->sums, B, hw);
+	FX_CHECK(check_launch("unet energy loss"));
+	FX_CHECK(backward(u));
+	std::vector<float> sums((size_t)B * 2);
+	FX_HIP(hipMemcpyAsync(sums.data(), u->sums, sums.size() * 4, hipMemcpyDeviceToHost, st));
+	FX_HIP(hipStreamSynchronize(st));
+	double L = 0;
+	for (int b = 0; b < B; ++b) L += sums[2 * b] / sums[2 * b + 1];
+	if (loss) *loss = L;
+	return FX_OK;
+}
+
 fx_status fx_unet_adam_step(fx_unet* u, float lr, float b1, float b2, float eps) {
 	FX_REQUIRE(u, FX_ERR_ARGUMENT, "null");
 	u->step += 1;
@@ -664,6 +735,7 @@ fx_status fx_unet_adam_step(fx_unet* u, float lr, float b1, float b2, float eps)
 
 namespace fx {
 fx_status unet_infer(fx_unet* u, const float* x, int B, int H, int W, const int* skip, const float** y) {
+	FX_REQUIRE(H % 8 == 0 && W % 8 == 0, FX_ERR_DIMENSION, "U-Net image sides must be multiples of 8, got %dx%d", H, W);
 	u->skip = skip;
 	fx_status s = run_forward(u, x, B, H, W);
 	u->skip = nullptr;

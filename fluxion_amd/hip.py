@@ -33,7 +33,7 @@ class HipBackend:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_backend_destroy(h)
             self.h = None
 
@@ -88,7 +88,7 @@ class Buffer:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_buffer_free(h)
             self.h = None
 
@@ -378,7 +378,7 @@ class Multigrid:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_multigrid_destroy(h)
             self.h = None
 
@@ -414,7 +414,7 @@ class _Solver:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_solver_destroy(h)
             self.h = None
 

@@ -31,7 +31,7 @@ class UNet:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_unet_destroy(h)
             self.h = None
 
@@ -74,18 +74,18 @@ class UNet:
     def forward_device(self, bx, by, B, H, W):
         L.check(self._lib.fx_unet_forward(self.h, bx.h, by.h, B, H, W), "unet_forward")
 
-    def loss_grad(self, x, t):
-        """Σ_b ‖y_b − t_b‖²/‖t_b‖² and its gradient (host copy)."""
+    def loss_grad(self, x, t, energy: bool = False):
+        """Σ_b ‖y_b − t_b‖²/‖t_b‖² (or the relative energy norm, fx_unet_loss_grad_energy) and its
+        gradient (host copy)."""
         B, H, W, cin = x.shape
         bx = self._upload("x", x)
         bt = self._upload("t", t)
-        out = C.c_double()
-        L.check(self._lib.fx_unet_loss_grad(self.h, bx.h, bt.h, B, H, W, C.byref(out)), "unet_loss_grad")
-        return out.value, self.grad().astype(np.float64)
+        return self.loss_grad_device(bx, bt, B, H, W, energy), self.grad().astype(np.float64)
 
-    def loss_grad_device(self, bx, bt, B, H, W) -> float:
+    def loss_grad_device(self, bx, bt, B, H, W, energy: bool = False) -> float:
         out = C.c_double()
-        L.check(self._lib.fx_unet_loss_grad(self.h, bx.h, bt.h, B, H, W, C.byref(out)), "unet_loss_grad")
+        fn = self._lib.fx_unet_loss_grad_energy if energy else self._lib.fx_unet_loss_grad
+        L.check(fn(self.h, bx.h, bt.h, B, H, W, C.byref(out)), "unet_loss_grad")
         return out.value
 
     def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
@@ -124,7 +124,7 @@ class LearnedPreconditioner:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and L.live():
             self._lib.fx_learned_destroy(h)
             self.h = None
 
@@ -186,8 +186,10 @@ def solver_pairs(be, n: int, seed: int, max_its: int = 12, stride: int = 1) -> P
     return PairSet(np.stack(xs), np.stack(ts))
 
 
-def train(net: UNet, sets, steps: int = 400, batch: int = 8, lr: float = 1e-3, seed: int = 0, log=None):
-    """Adam on the relative squared error over minibatches drawn from same-size pair sets."""
+def train(net: UNet, sets, steps: int = 400, batch: int = 8, lr: float = 1e-3, seed: int = 0, log=None,
+          energy: bool = False):
+    """Adam on the relative squared error (or relative energy norm) over minibatches drawn from
+    same-size pair sets, with a cosine learning-rate schedule."""
     rng = np.random.default_rng(seed)
     losses = []
     for step in range(steps):
@@ -196,7 +198,7 @@ def train(net: UNet, sets, steps: int = 400, batch: int = 8, lr: float = 1e-3, s
         x, t = ps.x[idx], ps.t[idx]
         lr_t = lr * 0.5 * (1.0 + np.cos(np.pi * step / steps))
         B, H, W, _ = x.shape
-        loss = net.loss_grad_device(net._upload("x", x), net._upload("t", t), B, H, W)
+        loss = net.loss_grad_device(net._upload("x", x), net._upload("t", t), B, H, W, energy)
         net.adam_step(lr_t)
         losses.append(loss / len(idx))
         if log and (step % 50 == 0 or step == steps - 1):

@@ -275,6 +275,10 @@ fx_status fx_unet_forward(fx_unet* u, const fx_buf* x, fx_buf* y, int32_t B, int
 /* Σ_b ‖y_b − t_b‖²/‖t_b‖² and its gradient (kept on the device). */
 fx_status fx_unet_loss_grad(fx_unet* u, const fx_buf* x, const fx_buf* t, int32_t B, int32_t H, int32_t W,
                             double* loss);
+/* Relative energy-norm loss Σ_b dᵀA_b d / tᵀA_b t (d = y − t, A_b the unit-spacing 5-point
+ * −∇·(k∇·) on interior pixels with k = exp of input channel 1) and its gradient. */
+fx_status fx_unet_loss_grad_energy(fx_unet* u, const fx_buf* x, const fx_buf* t, int32_t B, int32_t H, int32_t W,
+                                   double* loss);
 fx_status fx_unet_adam_step(fx_unet* u, float lr, float beta1, float beta2, float eps);
 
 /* Learned preconditioner (FX_PC_LEARNED): z = V(r) + s·h²·N([r1/s, log k]) with r1 = r − A·V(r),

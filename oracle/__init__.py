@@ -379,6 +379,8 @@ def unet_forward(params, x):
     """x: (B, H, W, CIN) float64 -> (B, H, W)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     b, h, w, cin = x.shape
+    if h % 8 or w % 8:
+        raise ValueError(f"the U-Net pools three times: image sides must be multiples of 8, got {h}x{w}")
     y = np.zeros((b, h, w))
     p = np.ascontiguousarray(params, dtype=np.float64)
     lib().orc_unet_forward(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_int(b), C.c_int(h),
@@ -386,15 +388,19 @@ def unet_forward(params, x):
     return y
 
 
-def unet_loss_grad(params, x, t):
-    """Relative squared error summed over the batch, and its gradient w.r.t. the parameters."""
+def unet_loss_grad(params, x, t, energy=False):
+    """Relative squared error (or, with energy=True, relative energy norm under the unit-spacing
+    −∇·(k∇·) with k = exp(x[..., 1])) summed over the batch, and its gradient w.r.t. the parameters."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     t = np.ascontiguousarray(t, dtype=np.float64)
     b, h, w, cin = x.shape
+    if h % 8 or w % 8:
+        raise ValueError(f"the U-Net pools three times: image sides must be multiples of 8, got {h}x{w}")
     p = np.ascontiguousarray(params, dtype=np.float64)
     g = np.zeros_like(p)
     L = lib()
-    L.orc_unet_loss_grad.restype = C.c_double
-    loss = L.orc_unet_loss_grad(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(t)), C.c_void_p(_p(g)),
-                                C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin))
+    fn = L.orc_unet_loss_grad_energy if energy else L.orc_unet_loss_grad
+    fn.restype = C.c_double
+    loss = fn(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(t)), C.c_void_p(_p(g)),
+              C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin))
     return loss, g

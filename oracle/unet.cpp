@@ -262,4 +262,58 @@ double orc_unet_loss_grad(const double* P, const double* x, const double* t, dou
 	return loss;
 }
 
+// Relative energy-norm loss: Σ_b dᵀA_b d / tᵀA_b t with d = y − t and A_b the unit-spacing
+// 5-point −∇·(k∇·) on interior pixels (k = exp of input channel 1, harmonic face means,
+// values outside the interior taken as zero); dLoss/dy = 2·A_b d / tᵀA_b t on interior pixels.
+double orc_unet_loss_grad_energy(const double* P, const double* x, const double* t, double* G, int batch, int h,
+                                 int w, int cin) {
+	Net N(cin);
+	std::memset(G, 0, sizeof(double) * N.nparams());
+	double loss = 0.0;
+	for (int b = 0; b < batch; ++b) {
+		T4 xi(h, w, cin), yo;
+		std::memcpy(xi.v.data(), x + (size_t)b * h * w * cin, sizeof(double) * h * w * cin);
+		Tape tp;
+		forward(N, P, xi, yo, tp);
+		const double* tb = t + (size_t)b * h * w;
+		auto inner = [&](int r, int c) { return r > 0 && c > 0 && r < h - 1 && c < w - 1; };
+		auto kk = [&](int r, int c) { return std::exp(xi.v[((size_t)r * w + c) * cin + 1]); };
+		auto hm = [](double a, double c) { return 2.0 * a * c / (a + c); };
+		auto apply = [&](const std::vector<double>& v, std::vector<double>& out) {
+			for (int r = 0; r < h; ++r)
+				for (int c = 0; c < w; ++c) {
+					double o = 0.0;
+					if (inner(r, c)) {
+						const double kc = kk(r, c), vc = v[(size_t)r * w + c];
+						const int dr[4] = {0, 0, -1, 1}, dc[4] = {-1, 1, 0, 0};
+						for (int e = 0; e < 4; ++e) {
+							const int rr = r + dr[e], cc = c + dc[e];
+							const double vn = inner(rr, cc) ? v[(size_t)rr * w + cc] : 0.0;
+							o += hm(kc, kk(rr, cc)) * (vc - vn);
+						}
+					}
+					out[(size_t)r * w + c] = o;
+				}
+		};
+		std::vector<double> d((size_t)h * w), tv((size_t)h * w), ad((size_t)h * w), at((size_t)h * w);
+		for (int q = 0; q < h * w; ++q) {
+			const int r = q / w, c = q % w;
+			d[q] = inner(r, c) ? yo.v[q] - tb[q] : 0.0;
+			tv[q] = inner(r, c) ? tb[q] : 0.0;
+		}
+		apply(d, ad);
+		apply(tv, at);
+		double ee = 0.0, tt = 0.0;
+		for (int q = 0; q < h * w; ++q) {
+			ee += d[q] * ad[q];
+			tt += tv[q] * at[q];
+		}
+		T4 dy(h, w, 1);
+		for (int q = 0; q < h * w; ++q) dy.v[q] = 2.0 * ad[q] / tt;
+		loss += ee / tt;
+		backward(N, P, tp, dy, G, nullptr);
+	}
+	return loss;
+}
+
 }  // extern "C"

@@ -68,3 +68,22 @@ def test_unet_adam_decreases_loss(be):
     l1, _ = net.loss_grad(x, t)
     print(f"loss {l0:.4f} -> {l1:.4f}")
     assert l1 < 0.5 * l0
+
+
+def test_unet_energy_gradient(be):
+    # The energy-norm loss and its gradient on the GPU (FP32 loss kernel, BF16 network) against the
+    # FP64 oracle.
+    from fluxion_amd.learned import UNet
+    cin, B, H, W = 2, 2, 32, 40
+    p = bf16_round(init_params(cin, 7))
+    rng = np.random.default_rng(8)
+    x = bf16_round(rng.standard_normal((B, H, W, cin)))
+    t = rng.standard_normal((B, H, W))
+    net = UNet(be, cin)
+    net.set_params(p)
+    loss, g = net.loss_grad(x, t, energy=True)
+    lr, gr = O.unet_loss_grad(p, x, t, energy=True)
+    cos = np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr))
+    print(f"energy loss {loss:.6f} vs {lr:.6f}; grad cosine {cos:.6f}")
+    assert abs(loss - lr) < 0.05 * lr
+    assert cos > 0.995
