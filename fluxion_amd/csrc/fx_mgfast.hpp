@@ -118,11 +118,11 @@ inline size_t frestrict_lds(const fx_xtab& X, bool vk) {
 	return f * sizeof(float) + i * sizeof(int32_t);
 }
 
+// One coarse tile (bx, by) of the residual + restriction (dynamic LDS frestrict_lds()).
 template <bool VK, int NPRE>
-__global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
-                                                     fx_xtab X, int64_t fr, int64_t fc, int64_t cr, int64_t cc,
-                                                     const int* skip) {
-	if (skip && *skip) return;
+__device__ __forceinline__ void frestrict_tile(int bx, int by, const float* __restrict__ b, float* __restrict__ bco,
+                                               const LWf& L, const fx_xtab& X, int64_t fr, int64_t fc, int64_t cr,
+                                               int64_t cc) {
 	extern __shared__ float smem[];
 	const int RY = X.ry, RX = X.rx, FX = RX + 1;
 	float* sb = smem;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 	int32_t* tlc = thr + RCY;                    // [RCX]
 	int32_t* thc = tlc + RCX;
 	const int t = threadIdx.x;
-	const int64_t I0 = (int64_t)blockIdx.y * RCY, J0 = (int64_t)blockIdx.x * RCX;
+	const int64_t I0 = (int64_t)by * RCY, J0 = (int64_t)bx * RCX;
 	const int ly = t / RCX, lx = t % RCX;
 	const int64_t I = I0 + ly, J = J0 + lx;
 	const bool mine = t < RCX * RCY && I < cr && J < cc;
@@ -288,10 +288,17 @@ __global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b
 	bco[I * cc + J] = acc * (X.sy * X.sx);
 }
 
-// Coarsest level: `sweeps` Jacobi sweeps from zero in one workgroup (n ≤ 4096 points).
-__global__ __launch_bounds__(1024) void k_fcoarse(const float* __restrict__ b, float* __restrict__ out, LWf L, int sweeps,
-                                                  int rows, int cols, const int* skip) {
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_frestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
+                                                     fx_xtab X, int64_t fr, int64_t fc, int64_t cr, int64_t cc,
+                                                     const int* skip) {
 	if (skip && *skip) return;
+	frestrict_tile<VK, NPRE>(blockIdx.x, blockIdx.y, b, bco, L, X, fr, fc, cr, cc);
+}
+
+// Coarsest level: `sweeps` Jacobi sweeps from zero in one workgroup (n ≤ 4096 points).
+__device__ __forceinline__ void fcoarse_body(const float* __restrict__ b, float* __restrict__ out, const LWf& L,
+                                             int sweeps, int rows, int cols) {
 	__shared__ float sa[4096], sb[4096];
 	const int n = rows * cols;
 	for (int q = threadIdx.x; q < n; q += blockDim.x) sa[q] = 0.0f;
@@ -315,6 +322,11 @@ __global__ __launch_bounds__(1024) void k_fcoarse(const float* __restrict__ b, f
 	}
 	for (int q = threadIdx.x; q < n; q += blockDim.x) out[q] = cur[q];
 }
+__global__ __launch_bounds__(1024) void k_fcoarse(const float* __restrict__ b, float* __restrict__ out, LWf L, int sweeps,
+                                                  int rows, int cols, const int* skip) {
+	if (skip && *skip) return;
+	fcoarse_body(b, out, L, sweeps, rows, cols);
+}
 
 // ── k_fpost ──────────────────────────────────────────────────────────
 // Tile PTX × PTY of fine points per 256-thread workgroup; b (and k) are staged
@@ -337,14 +349,13 @@ __device__ __forceinline__ void region(F&& f) {
 	for (int q = threadIdx.x; q < NY * NX; q += BLOCK) f(OY + q / NX, OX + q % NX);
 }
 
-// out = J^NPOST(J^NPRE(0; b) + P·e) on the tile.  Epi adds per-point reduction
-// terms of the level-0 result (block partials; k_finish forms the sums).
+// out = J^NPOST(J^NPRE(0; b) + P·e) on tile (bx, by).  Epi adds per-point
+// reduction terms of the level-0 result to acc.
 template <bool VK, int NPRE, int NPOST, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, const float* __restrict__ ec,
-                                                 float* __restrict__ out, LWf L, fx_xtab X, int64_t rows, int64_t cols,
-                                                 int64_t cr, int64_t cc, Epi epi, RedScratch rs, const int* skip) {
+__device__ __forceinline__ void fpost_tile(int bx, int by, const float* __restrict__ b, const float* __restrict__ ec,
+                                           float* __restrict__ out, const LWf& L, const fx_xtab& X, int64_t rows,
+                                           int64_t cols, int64_t cr, int64_t cc, const Epi& epi, Acc* acc) {
 	static_assert(NPRE + NPOST - 1 <= PH, "halo too small");
-	if (skip && *skip) return;
 	// Offsets into the region: the pre-smoothed iterate is needed on the tile ±
 	// NPOST (OU), its first sweep one ring wider (O1), b on the widest (OB).
 	constexpr int OU = PH - NPOST, O1 = OU - 1, OB = NPRE == 2 ? O1 : OU;
@@ -355,7 +366,7 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 	__shared__ float sfy[VK ? PRY + 1 : 1][VK ? PRX : 1];   // face above region row y
 	__shared__ int32_t ry0[PRY], ry1[PRY], rx0[PRX], rx1[PRX];
 	__shared__ float rwy[PRY], rwx[PRX];
-	const int64_t r0 = (int64_t)blockIdx.y * PTY - PH, c0 = (int64_t)blockIdx.x * PTX - PH;
+	const int64_t r0 = (int64_t)by * PTY - PH, c0 = (int64_t)bx * PTX - PH;
 	const int t = threadIdx.x;
 	// Interpolation tables for the region's rows and columns.
 	for (int q = t; q < PRY; q += BLOCK) {
@@ -449,8 +460,6 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		last = sa;
 	}
 	// Final sweep on the tile.
-	Acc acc[Epi::NS > 0 ? Epi::NS : 1];
-	for (int k = 0; k < (Epi::NS > 0 ? Epi::NS : 1); ++k) acc[k] = Acc{0.0, 0.0};
 	region<PH, PH, PTY, PTX>([&](int y, int x) {
 		const int64_t rr = r0 + y, cl = c0 + x;
 		if (rr >= rows || cl >= cols) return;
@@ -466,6 +475,18 @@ __global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, co
 		epi.term(j, v, acc);
 	});
 #undef FMG_F4
+}
+
+// out = J^NPOST(J^NPRE(0; b) + P·e) on the tile.  Epi adds per-point reduction
+// terms of the level-0 result (block partials; k_finish forms the sums).
+template <bool VK, int NPRE, int NPOST, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_fpost(const float* __restrict__ b, const float* __restrict__ ec,
+                                                 float* __restrict__ out, LWf L, fx_xtab X, int64_t rows, int64_t cols,
+                                                 int64_t cr, int64_t cc, Epi epi, RedScratch rs, const int* skip) {
+	if (skip && *skip) return;
+	Acc acc[Epi::NS > 0 ? Epi::NS : 1];
+	for (int k = 0; k < (Epi::NS > 0 ? Epi::NS : 1); ++k) acc[k] = Acc{0.0, 0.0};
+	fpost_tile<VK, NPRE, NPOST, Epi>(blockIdx.x, blockIdx.y, b, ec, out, L, X, rows, cols, cr, cc, epi, acc);
 	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
 }
 

@@ -223,22 +223,16 @@ __global__ __launch_bounds__(BLOCK) void k_stream_st(V v, E e, W9 w, int64_t row
 // sigma·p + alpha·(−∇·(k∇p)) with harmonic face means (fx_ops.hip k_vark), k
 // streamed alongside p; E::interior receives the operator value.
 __device__ __forceinline__ double hmean_d(double a, double b) { return 2.0 * a * b / (a + b); }
+// Column block bx, row block by of the traversal; the terms go to acc.
 template <int NS, class V, class E>
-__global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __restrict__ k, double sigma, double alpha,
-                                                     double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
-                                                     RedScratch rs, const SState* st) {
-	if (st->done) return;
-	v.be = st->beta;
-	int bx, by;
-	xcd_tile(&bx, &by);
+__device__ __forceinline__ void stream_vk_body(int bx, int by, const V& v, const E& e, const double* __restrict__ k,
+                                               double sigma, double alpha, double ix2, double iy2, int64_t rows,
+                                               int64_t cols, int rpb, Acc* acc) {
 	const int64_t c = (int64_t)bx * BLOCK + threadIdx.x;
 	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool col_ok = c < cols;
 	const bool col_int = c > 0 && c < cols - 1;
-	Acc acc[NS];
-#pragma unroll
-	for (int q = 0; q < NS; ++q) acc[q] = Acc{0.0, 0.0};
 	double up = 0.0, cur = 0.0, kup = 1.0, kcur = 1.0;
 	if (col_ok) {
 		if (r0 > 0) {
@@ -273,6 +267,19 @@ __global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __r
 		kup = kcur;
 		kcur = kdn;
 	}
+}
+template <int NS, class V, class E>
+__global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __restrict__ k, double sigma, double alpha,
+                                                     double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
+                                                     RedScratch rs, const SState* st) {
+	if (st->done) return;
+	v.be = st->beta;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	Acc acc[NS];
+#pragma unroll
+	for (int q = 0; q < NS; ++q) acc[q] = Acc{0.0, 0.0};
+	stream_vk_body<NS>(bx, by, v, e, k, sigma, alpha, ix2, iy2, rows, cols, rpb, acc);
 	block_partial<NS>(acc, rs);
 }
 // ap = A·p inside (operator value given), ap = p on the boundary; writes p_new; p·ap.
@@ -1575,6 +1582,357 @@ fx_status solve_gmres(Ctx& c, double* x, const double* b, fx_solve_stats* stats)
 	return FX_OK;
 }
 
+// ══════════════════════════════════════════════════════════════════════
+// Batched multi-problem solver (extension; no reference counterpart).
+//
+// Many independent −∇·(k∇u) = f problems on grids of uneven sizes, each solved
+// by FP64 flexible CG around its own fused FP32 coefficient-aware V-cycle, all
+// advancing together.  Every kernel of an iteration is one launch over a
+// prebuilt work list of (problem, level, tile) items, so an iteration is a
+// fixed launch sequence over all problems; it is captured once in a hipGraph
+// and replayed until every problem has converged.  Each problem has its own
+// device scalar state (SState) and done flag; blocks of finished problems
+// return at once, and each problem's reductions are finished by its own
+// workgroup of a per-problem finish kernel.  Launch shapes that leave every
+// per-point result unchanged (rows per block of the step kernel, points per
+// block of the update kernel) are chosen by timing candidates at creation.
+// ══════════════════════════════════════════════════════════════════════
+
+struct BProb {
+	int64_t off;          // first point of the problem in the concatenated FP64 vectors
+	int32_t rows, cols;
+	double ix2, iy2;      // 1/dx², 1/dy² of the operator
+	float* r32;           // level-0 rhs of the problem's hierarchy (FP32 residual)
+	const float* z32;     // level-0 result of the problem's V-cycle
+	int32_t stp0, nstp;   // its block range in the step list
+	int32_t upd0, nupd;   // … in the update list
+	int32_t pst0, npst;   // … in the level-0 post list
+	int32_t lev0, nlev;   // its levels in the level table
+	int32_t sweeps, pad;  // coarsest-level sweeps
+};
+struct BLev {
+	float* rhs;
+	float* tmp;
+	fmg::LWf L;
+	fx_xtab X;
+	int32_t rows, cols, cr, cc;   // this level and the next coarser one
+};
+struct BItem {
+	int32_t p, l, bx, by;         // problem, level-table index, tile
+};
+
+constexpr int BUPD_THREAD = 8;    // points per thread of one update chunk before tuning
+
+// p = double(z) + β·p_old, ap = A·p, p·ap (the traversal of k_stream_vk) for one (problem, tile).
+__global__ __launch_bounds__(BLOCK) void k_b_step(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                  const SState* __restrict__ st, const double* __restrict__ k,
+                                                  const double* __restrict__ pold, double* __restrict__ pnew,
+                                                  double* __restrict__ ap, double sigma, double alpha, int rpb,
+                                                  RedScratch rs) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BProb P = pr[it.p];
+	VPNewF v{P.z32, pold + P.off, st[it.p].beta};
+	EpiCGv e{pnew + P.off, ap + P.off};
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	stream_vk_body<1>(it.bx, it.by, v, e, k + P.off, sigma, alpha, P.ix2, P.iy2, P.rows, P.cols, rpb, acc);
+	block_partial<1>(acc, rs);
+}
+
+// x += α·p; r −= α·ap; r32 = float(r); r·r (FXR32) over one chunk of a problem.
+__global__ __launch_bounds__(BLOCK) void k_b_update(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                    const SState* __restrict__ st, double* __restrict__ x,
+                                                    double* __restrict__ r, const double* __restrict__ p,
+                                                    const double* __restrict__ ap, int chunk, RedScratch rs) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BProb P = pr[it.p];
+	const double al = st[it.p].alpha;
+	const int64_t n = (int64_t)P.rows * P.cols;
+	const int64_t i0 = (int64_t)it.bx * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	for (int64_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+		const int64_t g = P.off + i;
+		x[g] = fma(al, p[g], x[g]);
+		const double v = fma(-al, ap[g], r[g]);
+		r[g] = v;
+		P.r32[i] = float(v);
+		acc_add(acc[0], v * v);
+	}
+	block_partial<1>(acc, rs);
+}
+
+// r = b − A·x, r32 = float(r), r·r (mode 0); or only ‖b − A·x‖² (mode 1), over one chunk.
+__global__ __launch_bounds__(BLOCK) void k_b_resid(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                   const double* __restrict__ k, const double* __restrict__ x,
+                                                   const double* __restrict__ b, double* __restrict__ r, double sigma,
+                                                   double alpha, int chunk, int mode, RedScratch rs) {
+	const BItem it = items[blockIdx.x];
+	const BProb P = pr[it.p];
+	const int64_t n = (int64_t)P.rows * P.cols;
+	const int64_t i0 = (int64_t)it.bx * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+	const double* kk = k + P.off;
+	const double* u = x + P.off;
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	for (int64_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+		const int rr = (int)(i / P.cols), c = (int)(i % P.cols);
+		double av;
+		if (rr == 0 || c == 0 || rr == P.rows - 1 || c == P.cols - 1) {
+			av = u[i];
+		} else {
+			const double kc = kk[i], cv = u[i];
+			const double kw = hmean_d(kc, kk[i - 1]), ke = hmean_d(kc, kk[i + 1]);
+			const double kn = hmean_d(kc, kk[i - P.cols]), ks = hmean_d(kc, kk[i + P.cols]);
+			const double fx = ke * (u[i + 1] - cv) - kw * (cv - u[i - 1]);
+			const double fy = ks * (u[i + P.cols] - cv) - kn * (cv - u[i - P.cols]);
+			av = fma(alpha, -(fx * P.ix2 + fy * P.iy2), sigma * cv);
+		}
+		const double v = b[P.off + i] - av;
+		if (mode == 0) {
+			r[P.off + i] = v;
+			P.r32[i] = float(v);
+		}
+		acc_add(acc[0], v * v);
+	}
+	block_partial<1>(acc, rs);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_b_zero(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                  double* __restrict__ v, int chunk) {
+	const BItem it = items[blockIdx.x];
+	const BProb P = pr[it.p];
+	const int64_t n = (int64_t)P.rows * P.cols;
+	const int64_t i0 = (int64_t)it.bx * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+	for (int64_t i = i0 + threadIdx.x; i < i1; i += BLOCK) v[P.off + i] = 0.0;
+}
+
+// V-cycle pieces over (problem, level, tile) items.
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_b_restrict(const BItem* __restrict__ items, const BLev* __restrict__ lv,
+                                                      const SState* __restrict__ st, const float* __restrict__ dummy) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BLev& L = lv[it.l];
+	fmg::frestrict_tile<VK, NPRE>(it.bx, it.by, L.rhs, lv[it.l + 1].rhs, L.L, L.X, L.rows, L.cols, L.cr, L.cc);
+}
+__global__ __launch_bounds__(1024) void k_b_coarse(const BProb* __restrict__ pr, const BLev* __restrict__ lv,
+                                                   const SState* __restrict__ st) {
+	const int p = blockIdx.x;
+	if (st[p].skip) return;
+	const BLev& L = lv[pr[p].lev0 + pr[p].nlev - 1];
+	fmg::fcoarse_body(L.rhs, L.tmp, L.L, pr[p].sweeps, L.rows, L.cols);
+}
+template <bool VK, int NPRE, int NPOST>
+__global__ __launch_bounds__(BLOCK) void k_b_post(const BItem* __restrict__ items, const BLev* __restrict__ lv,
+                                                  const SState* __restrict__ st) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BLev& L = lv[it.l];
+	Acc acc[1];
+	fmg::fpost_tile<VK, NPRE, NPOST, fmg::NoEpi>(it.bx, it.by, L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols,
+	                                             L.cr, L.cc, fmg::NoEpi{}, acc);
+}
+// Level-0 post with the FCG reductions of each problem: (r·z, z·Ap) at iterations, r·z at the start.
+template <bool VK, int NPRE, int NPOST, int NS>
+__global__ __launch_bounds__(BLOCK) void k_b_post0(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                   const BLev* __restrict__ lv, SState* __restrict__ st,
+                                                   const double* __restrict__ r, const double* __restrict__ ap,
+                                                   RedScratch rs) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BLev& L = lv[it.l];
+	const BProb& P = pr[it.p];
+	Acc acc[NS];
+#pragma unroll
+	for (int q = 0; q < NS; ++q) acc[q] = Acc{0.0, 0.0};
+	if constexpr (NS == 2)
+		fmg::fpost_tile<VK, NPRE, NPOST, EpiFcg>(it.bx, it.by, L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols,
+		                                         L.cr, L.cc, EpiFcg{r + P.off, ap + P.off, st + it.p}, acc);
+	else
+		fmg::fpost_tile<VK, NPRE, NPOST, EpiRZ>(it.bx, it.by, L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols,
+		                                        L.cr, L.cc, EpiRZ{r + P.off, st + it.p}, acc);
+	block_partial<NS>(acc, rs);
+}
+
+// Per-problem finish: workgroup p merges its blocks' partials (fixed order) and runs Fin on its state.
+template <int NS, class Fin>
+__global__ __launch_bounds__(BLOCK) void k_b_finish(const BProb* __restrict__ pr, SState* __restrict__ st,
+                                                    RedScratch rs, int which, int force) {
+	const int p = blockIdx.x;
+	if (!force && st[p].skip) return;
+	const BProb& P = pr[p];
+	const int b0 = which == 0 ? P.stp0 : which == 1 ? P.upd0 : P.pst0;
+	const int nb = which == 0 ? P.nstp : which == 1 ? P.nupd : P.npst;
+	Acc a[NS];
+#pragma unroll
+	for (int k = 0; k < NS; ++k) a[k] = Acc{0.0, 0.0};
+	for (int b = threadIdx.x; b < nb; b += BLOCK) {
+#pragma unroll
+		for (int k = 0; k < NS; ++k) acc_merge(a[k], rs.partials[(size_t)(b0 + b) * NS + k]);
+	}
+	block_reduce<NS>(a);
+	if (threadIdx.x == 0) Fin{st + p}(a);
+}
+// ‖b − A·x‖ of each problem into its `resid` slot (after the solve).
+struct FinTrue {
+	SState* st;
+	__device__ void operator()(Acc* o) const { st->omega = sqrt(acc_value(o[0])); }
+};
+// Count of problems still running.
+__global__ void k_b_active(const SState* __restrict__ st, int nprob, int* out) {
+	__shared__ int cnt;
+	if (threadIdx.x == 0) cnt = 0;
+	__syncthreads();
+	int c = 0;
+	for (int p = threadIdx.x; p < nprob; p += blockDim.x) c += st[p].done ? 0 : 1;
+	atomicAdd(&cnt, c);
+	__syncthreads();
+	if (threadIdx.x == 0) *out = cnt;
+}
+
+}  // namespace
+
+struct fx_batch {
+	fx_backend* be = nullptr;
+	int nprob = 0;
+	int64_t ntot = 0;
+	fx_mg_options opt{};
+	std::vector<fx_grid> grids;
+	std::vector<fx_multigrid*> mgs;
+	std::vector<BProb> hp;
+	const fx_buf* k = nullptr;
+	// device tables
+	BProb* dp = nullptr;
+	BLev* dl = nullptr;
+	SState* dst = nullptr;
+	BItem *step = nullptr, *upd = nullptr, *post0 = nullptr;
+	std::vector<BItem*> restr, post;         // per level depth d (restrict d → d+1; post at d ≥ 1)
+	std::vector<int> nrestr, npost;
+	std::vector<size_t> restr_lds;
+	int nstep = 0, nupd = 0, npost0 = 0, maxdepth = 0;
+	int rpb = 16, chunk = BLOCK * BUPD_THREAD;
+	// work vectors
+	double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *ap = nullptr;
+	fxd::Acc* partials = nullptr;
+	int* dactive = nullptr;
+	int* hactive = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipGraph_t graph = nullptr;
+	int graph_iters = 2;
+};
+
+namespace {
+
+void batch_free(fx_batch* B) {
+	if (!B) return;
+	if (B->be) (void)hipStreamSynchronize(B->be->stream);
+	if (B->exec) (void)hipGraphExecDestroy(B->exec);
+	if (B->graph) (void)hipGraphDestroy(B->graph);
+	for (auto* m : B->mgs) fx_multigrid_destroy(m);
+	fx::dev_free(B->dp);
+	fx::dev_free(B->dl);
+	fx::dev_free(B->dst);
+	fx::dev_free(B->step);
+	fx::dev_free(B->upd);
+	fx::dev_free(B->post0);
+	for (auto* q : B->restr) fx::dev_free(q);
+	for (auto* q : B->post) fx::dev_free(q);
+	fx::dev_free(B->x);
+	fx::dev_free(B->r);
+	fx::dev_free(B->p0);
+	fx::dev_free(B->p1);
+	fx::dev_free(B->ap);
+	fx::dev_free(B->partials);
+	fx::dev_free(B->dactive);
+	if (B->hactive) (void)hipHostFree(B->hactive);
+	delete B;
+}
+
+template <class T>
+fx_status upload_vec(fx_backend* be, const std::vector<T>& v, T** out) {
+	void* p = nullptr;
+	FX_CHECK(fx::dev_alloc(be, std::max<size_t>(1, v.size()) * sizeof(T), &p));
+	if (!v.empty()) FX_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	*out = (T*)p;
+	return FX_OK;
+}
+
+// Step-list items for a rows-per-block choice; block ranges go into hp.
+std::vector<BItem> step_items(fx_batch* B, int rpb) {
+	std::vector<BItem> v;
+	for (int p = 0; p < B->nprob; ++p) {
+		const fx_grid& g = B->grids[p];
+		B->hp[p].stp0 = (int32_t)v.size();
+		const int nbx = (int)((g.cols + BLOCK - 1) / BLOCK), nby = (int)((g.rows + rpb - 1) / rpb);
+		for (int by = 0; by < nby; ++by)
+			for (int bx = 0; bx < nbx; ++bx) v.push_back(BItem{p, 0, bx, by});
+		B->hp[p].nstp = (int32_t)v.size() - B->hp[p].stp0;
+	}
+	return v;
+}
+std::vector<BItem> upd_items(fx_batch* B, int chunk) {
+	std::vector<BItem> v;
+	for (int p = 0; p < B->nprob; ++p) {
+		const fx_grid& g = B->grids[p];
+		const int64_t n = (int64_t)g.rows * g.cols;
+		B->hp[p].upd0 = (int32_t)v.size();
+		for (int64_t c = 0; c * chunk < n; ++c) v.push_back(BItem{p, 0, (int32_t)c, 0});
+		B->hp[p].nupd = (int32_t)v.size() - B->hp[p].upd0;
+	}
+	return v;
+}
+
+template <bool VK, int NPRE, int NPOST>
+fx_status batch_vcycle(fx_batch* B, bool start) {
+	hipStream_t st = B->be->stream;
+	for (int d = 0; d + 1 <= B->maxdepth - 1; ++d) {
+		if (B->nrestr[d] == 0) continue;
+		hipLaunchKernelGGL((k_b_restrict<VK, NPRE>), dim3(B->nrestr[d]), dim3(BLOCK), (unsigned)B->restr_lds[d], st,
+		                   B->restr[d], B->dl, B->dst, (const float*)nullptr);
+	}
+	hipLaunchKernelGGL(k_b_coarse, dim3(B->nprob), dim3(1024), 0, st, B->dp, B->dl, B->dst);
+	for (int d = B->maxdepth - 2; d >= 1; --d) {
+		if (B->npost[d] == 0) continue;
+		hipLaunchKernelGGL((k_b_post<VK, NPRE, NPOST>), dim3(B->npost[d]), dim3(BLOCK), 0, st, B->post[d], B->dl, B->dst);
+	}
+	RedScratch rs{B->partials};
+	if (start) {
+		hipLaunchKernelGGL((k_b_post0<VK, NPRE, NPOST, 1>), dim3(B->npost0), dim3(BLOCK), 0, st, B->post0, B->dp, B->dl,
+		                   B->dst, (const double*)B->r, (const double*)B->ap, rs);
+		hipLaunchKernelGGL((k_b_finish<1, FinPcgInitRZ>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 2, 0);
+	} else {
+		hipLaunchKernelGGL((k_b_post0<VK, NPRE, NPOST, 2>), dim3(B->npost0), dim3(BLOCK), 0, st, B->post0, B->dp, B->dl,
+		                   B->dst, (const double*)B->r, (const double*)B->ap, rs);
+		hipLaunchKernelGGL((k_b_finish<2, FinFcgBeta>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 2, 0);
+	}
+	return fx::check_launch("batched V-cycle");
+}
+
+fx_status batch_vcycle_any(fx_batch* B, bool start) {
+	const int a = B->opt.n_pre, b = B->opt.n_post;
+	if (a == 2 && b == 2) return batch_vcycle<true, 2, 2>(B, start);
+	if (a == 1 && b == 1) return batch_vcycle<true, 1, 1>(B, start);
+	if (a == 1 && b == 2) return batch_vcycle<true, 1, 2>(B, start);
+	return batch_vcycle<true, 2, 1>(B, start);
+}
+
+// One FCG iteration of every running problem: p_new = z + β·p_old, A·p, α; x/r update and the
+// convergence test; V-cycle with (r·z, z·Ap), β.  Even and odd iterations swap p0 and p1.
+fx_status batch_iteration(fx_batch* B, int parity) {
+	hipStream_t st = B->be->stream;
+	RedScratch rs{B->partials};
+	double* pold = parity ? B->p1 : B->p0;
+	double* pnew = parity ? B->p0 : B->p1;
+	hipLaunchKernelGGL(k_b_step, dim3(B->nstep), dim3(BLOCK), 0, st, B->step, B->dp, (const SState*)B->dst,
+	                   (const double*)B->k->ptr, (const double*)pold, pnew, B->ap, 0.0, 1.0, B->rpb, rs);
+	hipLaunchKernelGGL((k_b_finish<1, FinAlpha>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 0, 0);
+	hipLaunchKernelGGL(k_b_update, dim3(B->nupd), dim3(BLOCK), 0, st, B->upd, B->dp, (const SState*)B->dst, B->x, B->r,
+	                   (const double*)pnew, (const double*)B->ap, B->chunk, rs);
+	hipLaunchKernelGGL((k_b_finish<1, FinPcgRR>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 1, 0);
+	FX_CHECK(fx::check_launch("batched FCG step"));
+	return batch_vcycle_any(B, false);
+}
+
 }  // namespace
 
 using namespace fx;
@@ -1750,6 +2108,286 @@ fx_status fx_solver_solve_preconditioned(fx_solver* s, fx_buf* x, const fx_buf* 
 	FX_REQUIRE(pc, FX_ERR_ARGUMENT, "preconditioner is null");
 	fx_operator op = stencil_op(coeff, st, bc);
 	return fx_solver_solve_op(s, x, b, &op, pc, stats);
+}
+
+fx_status fx_batch_create(fx_backend* be, uint32_t nprob, const fx_grid* grids, const fx_buf* k,
+                          const fx_mg_options* opt, fx_batch** out) {
+	FX_REQUIRE(be && grids && k && opt && out && nprob > 0, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_REQUIRE(opt->precision == FX_F32 && opt->fused && opt->smoother == FX_SMOOTH_JACOBI &&
+	               opt->transfer == FX_TRANSFER_FW_BILINEAR,
+	           FX_ERR_UNSUPPORTED, "the batched solver runs the fused FP32 V-cycle (FX_F32, fused, Jacobi, FW)");
+	int64_t ntot = 0;
+	for (uint32_t p = 0; p < nprob; ++p) {
+		FX_CHECK(check_grid(&grids[p]));
+		FX_REQUIRE(grids[p].rows < (1u << 30) && grids[p].cols < (1u << 30), FX_ERR_GRID, "grid too large");
+		ntot += (int64_t)grids[p].rows * grids[p].cols;
+	}
+	FX_CHECK(check_buf(k, (size_t)ntot, FX_F64, "k"));
+	fx_batch* B = new fx_batch();
+	B->be = be;
+	B->nprob = (int)nprob;
+	B->ntot = ntot;
+	B->opt = *opt;
+	B->k = k;
+	B->grids.assign(grids, grids + nprob);
+	B->hp.resize(nprob);
+	fx_status s = FX_OK;
+	auto fail = [&](fx_status e) { batch_free(B); return e; };
+	// Per-problem hierarchies (coefficient-aware FP32 levels built from a view of k).
+	fx_boundaries zd;
+	zd.top = zd.bottom = zd.left = zd.right = fx_bc{FX_BC_DIRICHLET, 0.0, 0.0, 0.0};
+	std::vector<BLev> lev;
+	int64_t off = 0;
+	for (int p = 0; p < B->nprob; ++p) {
+		const fx_grid& g = grids[p];
+		fx_buf view;
+		view.be = be;
+		view.ptr = (double*)k->ptr + off;
+		view.len = (size_t)g.rows * g.cols;
+		view.dtype = FX_F64;
+		fx_mg_options o = *opt;
+		o.k = &view;
+		fx_multigrid* mg = nullptr;
+		if ((s = fx_multigrid_create_ex(be, &g, &zd, &o, &mg)) != FX_OK) return fail(s);
+		mg->opt.k = nullptr;
+		B->mgs.push_back(mg);
+		BProb& P = B->hp[p];
+		P.off = off;
+		P.rows = (int32_t)g.rows;
+		P.cols = (int32_t)g.cols;
+		P.ix2 = 1.0 / (g.dx * g.dx);
+		P.iy2 = 1.0 / (g.dy * g.dy);
+		P.r32 = (float*)mg->lv[0].rhs;
+		P.z32 = (const float*)mg->lv[0].tmp;
+		P.lev0 = (int32_t)lev.size();
+		P.nlev = (int32_t)mg->lv.size();
+		P.sweeps = opt->n_pre + opt->n_post + opt->extra_coarse;
+		for (size_t l = 0; l < mg->lv.size(); ++l) {
+			const auto& L = mg->lv[l];
+			BLev d{};
+			d.rhs = (float*)L.rhs;
+			d.tmp = (float*)L.tmp;
+			d.L = fmg::lwf(mg, L);
+			d.X = L.xt;
+			d.rows = (int32_t)L.g.rows;
+			d.cols = (int32_t)L.g.cols;
+			d.cr = l + 1 < mg->lv.size() ? (int32_t)mg->lv[l + 1].g.rows : 0;
+			d.cc = l + 1 < mg->lv.size() ? (int32_t)mg->lv[l + 1].g.cols : 0;
+			lev.push_back(d);
+		}
+		B->maxdepth = std::max(B->maxdepth, P.nlev);
+		off += (int64_t)g.rows * g.cols;
+	}
+	// Work lists: V-cycle tiles per level depth and the level-0 post tiles.
+	B->restr.assign(B->maxdepth, nullptr);
+	B->post.assign(B->maxdepth, nullptr);
+	B->nrestr.assign(B->maxdepth, 0);
+	B->npost.assign(B->maxdepth, 0);
+	B->restr_lds.assign(B->maxdepth, 0);
+	std::vector<BItem> p0items;
+	for (int d = 0; d < B->maxdepth; ++d) {
+		std::vector<BItem> ri, pi;
+		for (int p = 0; p < B->nprob; ++p) {
+			const BProb& P = B->hp[p];
+			if (d + 1 >= P.nlev) continue;   // level d is the coarsest (or absent) for this problem
+			const BLev& L = lev[P.lev0 + d];
+			const dim3 rg = fmg::rgrid(L.cr, L.cc), tg = fmg::tgrid(L.rows, L.cols);
+			for (unsigned by = 0; by < rg.y; ++by)
+				for (unsigned bx = 0; bx < rg.x; ++bx) ri.push_back(BItem{p, P.lev0 + d, (int32_t)bx, (int32_t)by});
+			B->restr_lds[d] = std::max(B->restr_lds[d], fmg::frestrict_lds(L.X, true));
+			std::vector<BItem>& dst = d == 0 ? p0items : pi;
+			if (d == 0) B->hp[p].pst0 = (int32_t)p0items.size();
+			for (unsigned by = 0; by < tg.y; ++by)
+				for (unsigned bx = 0; bx < tg.x; ++bx) dst.push_back(BItem{p, P.lev0 + d, (int32_t)bx, (int32_t)by});
+			if (d == 0) B->hp[p].npst = (int32_t)p0items.size() - B->hp[p].pst0;
+		}
+		if (B->restr_lds[d] > 64 * 1024) {
+			set_error("restriction tile too large for level depth %d", d);
+			return fail(FX_ERR_UNSUPPORTED);
+		}
+		B->nrestr[d] = (int)ri.size();
+		B->npost[d] = (int)pi.size();
+		if ((s = upload_vec(be, ri, &B->restr[d])) != FX_OK) return fail(s);
+		if ((s = upload_vec(be, pi, &B->post[d])) != FX_OK) return fail(s);
+	}
+	for (int p = 0; p < B->nprob; ++p) {
+		if (B->hp[p].nlev < 2) {
+			set_error("problem %d is too small for a two-level hierarchy", p);
+			return fail(FX_ERR_GRID);
+		}
+	}
+	B->npost0 = (int)p0items.size();
+	if ((s = upload_vec(be, p0items, &B->post0)) != FX_OK) return fail(s);
+	if ((s = upload_vec(be, lev, &B->dl)) != FX_OK) return fail(s);
+	// Work vectors and state.
+	void* q = nullptr;
+	double** vecs[5] = {&B->x, &B->r, &B->p0, &B->p1, &B->ap};
+	for (double** v : vecs) {
+		if ((s = fx::dev_alloc(be, (size_t)ntot * sizeof(double), &q)) != FX_OK) return fail(s);
+		*v = (double*)q;
+	}
+	if ((s = fx::dev_alloc(be, (size_t)B->nprob * sizeof(SState), &q)) != FX_OK) return fail(s);
+	B->dst = (SState*)q;
+	if ((s = fx::dev_alloc(be, sizeof(int), &q)) != FX_OK) return fail(s);
+	B->dactive = (int*)q;
+	if (hipHostMalloc((void**)&B->hactive, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+		set_error("hipHostMalloc failed");
+		return fail(FX_ERR_ALLOC);
+	}
+	// Launch-shape tuning: rows per block of the step kernel and points per block of the update kernel,
+	// each timed on this batch with zero states (β = α = 0, nothing skipped; only scratch vectors move).
+	FX_HIP(hipMemsetAsync(B->dst, 0, (size_t)B->nprob * sizeof(SState), be->stream));
+	size_t maxparts = (size_t)B->npost0 * 2;
+	for (int rpb : {4, 8, 16, 32}) maxparts = std::max(maxparts, step_items(B, rpb).size());
+	for (int ch : {BLOCK * 4, BLOCK * 8, BLOCK * 16}) maxparts = std::max(maxparts, upd_items(B, ch).size());
+	if ((s = fx::dev_alloc(be, maxparts * sizeof(fxd::Acc), &q)) != FX_OK) return fail(s);
+	B->partials = (fxd::Acc*)q;
+	if ((s = fx::dev_alloc(be, (size_t)B->nprob * sizeof(BProb), &q)) != FX_OK) return fail(s);
+	B->dp = (BProb*)q;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	FX_HIP(hipEventCreate(&e0));
+	FX_HIP(hipEventCreate(&e1));
+	auto timed = [&](auto launch, float* ms) -> fx_status {
+		FX_HIP(hipMemcpyAsync(B->dp, B->hp.data(), B->hp.size() * sizeof(BProb), hipMemcpyHostToDevice, be->stream));
+		launch();
+		FX_HIP(hipEventRecord(e0, be->stream));
+		for (int i = 0; i < 3; ++i) launch();
+		FX_HIP(hipEventRecord(e1, be->stream));
+		FX_HIP(hipEventSynchronize(e1));
+		FX_CHECK(fx::check_launch("batch tuning"));
+		FX_HIP(hipEventElapsedTime(ms, e0, e1));
+		return FX_OK;
+	};
+	float best = 1e30f;
+	for (int rpb : {4, 8, 16, 32}) {
+		std::vector<BItem> it = step_items(B, rpb);
+		BItem* d = nullptr;
+		if ((s = upload_vec(be, it, &d)) != FX_OK) return fail(s);
+		float ms = 0.0f;
+		s = timed([&] {
+			hipLaunchKernelGGL(k_b_step, dim3((unsigned)it.size()), dim3(BLOCK), 0, be->stream, (const BItem*)d,
+			                   (const BProb*)B->dp, (const SState*)B->dst, (const double*)k->ptr, (const double*)B->p0,
+			                   B->p1, B->ap, 0.0, 1.0, rpb, RedScratch{B->partials});
+		}, &ms);
+		fx::dev_free(d);
+		if (s != FX_OK) return fail(s);
+		if (ms < best) {
+			best = ms;
+			B->rpb = rpb;
+		}
+	}
+	best = 1e30f;
+	for (int ch : {BLOCK * 4, BLOCK * 8, BLOCK * 16}) {
+		std::vector<BItem> it = upd_items(B, ch);
+		BItem* d = nullptr;
+		if ((s = upload_vec(be, it, &d)) != FX_OK) return fail(s);
+		float ms = 0.0f;
+		s = timed([&] {
+			hipLaunchKernelGGL(k_b_update, dim3((unsigned)it.size()), dim3(BLOCK), 0, be->stream, (const BItem*)d,
+			                   (const BProb*)B->dp, (const SState*)B->dst, B->x, B->r, (const double*)B->p1,
+			                   (const double*)B->ap, ch, RedScratch{B->partials});
+		}, &ms);
+		fx::dev_free(d);
+		if (s != FX_OK) return fail(s);
+		if (ms < best) {
+			best = ms;
+			B->chunk = ch;
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	std::vector<BItem> sit = step_items(B, B->rpb), uit = upd_items(B, B->chunk);
+	B->nstep = (int)sit.size();
+	B->nupd = (int)uit.size();
+	if ((s = upload_vec(be, sit, &B->step)) != FX_OK) return fail(s);
+	if ((s = upload_vec(be, uit, &B->upd)) != FX_OK) return fail(s);
+	FX_HIP(hipMemcpyAsync(B->dp, B->hp.data(), B->hp.size() * sizeof(BProb), hipMemcpyHostToDevice, be->stream));
+	// The iteration graph: graph_iters FCG iterations of every problem, then the count of running problems.
+	FX_HIP(hipStreamSynchronize(be->stream));
+	FX_HIP(hipStreamBeginCapture(be->stream, hipStreamCaptureModeRelaxed));
+	fx_status cs = FX_OK;
+	for (int it = 0; it < B->graph_iters && cs == FX_OK; ++it) cs = batch_iteration(B, it & 1);
+	hipLaunchKernelGGL(k_b_active, dim3(1), dim3(BLOCK), 0, be->stream, (const SState*)B->dst, B->nprob, B->dactive);
+	hipError_t ce = hipStreamEndCapture(be->stream, &B->graph);
+	if (cs != FX_OK) return fail(cs);
+	if (ce != hipSuccess) {
+		set_error("graph capture failed: %s", hipGetErrorString(ce));
+		return fail(FX_ERR_HIP);
+	}
+	if (hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
+		set_error("graph instantiation failed");
+		return fail(FX_ERR_HIP);
+	}
+	*out = B;
+	return FX_OK;
+}
+
+fx_status fx_batch_solve(fx_batch* B, fx_buf* x, const fx_buf* b, const double* tol, uint64_t max_iter,
+                         fx_solve_stats* stats) {
+	FX_REQUIRE(B && tol && stats, FX_ERR_ARGUMENT, "null argument");
+	FX_CHECK(check_buf(x, (size_t)B->ntot, FX_F64, "x"));
+	FX_CHECK(check_buf(b, (size_t)B->ntot, FX_F64, "b"));
+	fx_backend* be = B->be;
+	hipStream_t st = be->stream;
+	const size_t bytes = (size_t)B->ntot * sizeof(double);
+	FX_HIP(hipMemcpyAsync(B->x, x->ptr, bytes, hipMemcpyDeviceToDevice, st));
+	std::vector<SState> h((size_t)B->nprob);
+	std::memset(h.data(), 0, h.size() * sizeof(SState));
+	for (int p = 0; p < B->nprob; ++p) {
+		FX_REQUIRE(tol[p] >= 0.0, FX_ERR_ARGUMENT, "negative tolerance for problem %d", p);
+		h[p].tol = tol[p];
+		h[p].max_iter = (int64_t)max_iter;
+	}
+	FX_HIP(hipMemcpyAsync(B->dst, h.data(), h.size() * sizeof(SState), hipMemcpyHostToDevice, st));
+	RedScratch rs{B->partials};
+	// r = b − A·x, r·r; p = 0; z = V(r) with r·z and the iteration-0 test.
+	hipLaunchKernelGGL(k_b_resid, dim3(B->nupd), dim3(BLOCK), 0, st, (const BItem*)B->upd, (const BProb*)B->dp,
+	                   (const double*)B->k->ptr, (const double*)B->x, (const double*)b->ptr, B->r, 0.0, 1.0, B->chunk, 0,
+	                   rs);
+	hipLaunchKernelGGL((k_b_finish<1, FinPcgInitRR>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 1, 1);
+	hipLaunchKernelGGL(k_b_zero, dim3(B->nupd), dim3(BLOCK), 0, st, (const BItem*)B->upd, (const BProb*)B->dp, B->p0,
+	                   B->chunk);
+	FX_CHECK(fx::check_launch("batch init"));
+	FX_CHECK(batch_vcycle_any(B, true));
+	hipLaunchKernelGGL(k_b_active, dim3(1), dim3(BLOCK), 0, st, (const SState*)B->dst, B->nprob, B->dactive);
+	FX_HIP(hipMemcpyAsync(B->hactive, B->dactive, sizeof(int), hipMemcpyDeviceToHost, st));
+	FX_HIP(hipStreamSynchronize(st));
+	// Replay the captured iterations until no problem is running (every problem stops by max_iter).
+	for (uint64_t done = 0; *B->hactive > 0 && done <= max_iter; done += (uint64_t)B->graph_iters) {
+		FX_HIP(hipGraphLaunch(B->exec, st));
+		FX_HIP(hipMemcpyAsync(B->hactive, B->dactive, sizeof(int), hipMemcpyDeviceToHost, st));
+		FX_HIP(hipStreamSynchronize(st));
+	}
+	// FP64 true residual of every problem.
+	hipLaunchKernelGGL(k_b_resid, dim3(B->nupd), dim3(BLOCK), 0, st, (const BItem*)B->upd, (const BProb*)B->dp,
+	                   (const double*)B->k->ptr, (const double*)B->x, (const double*)b->ptr, B->r, 0.0, 1.0, B->chunk, 1,
+	                   rs);
+	hipLaunchKernelGGL((k_b_finish<1, FinTrue>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 1, 1);
+	FX_CHECK(fx::check_launch("batch true residual"));
+	FX_HIP(hipMemcpyAsync(x->ptr, B->x, bytes, hipMemcpyDeviceToDevice, st));
+	FX_HIP(hipMemcpyAsync(h.data(), B->dst, h.size() * sizeof(SState), hipMemcpyDeviceToHost, st));
+	FX_HIP(hipStreamSynchronize(st));
+	for (int p = 0; p < B->nprob; ++p) {
+		stats[p].iterations = (uint64_t)h[p].iter;
+		stats[p].residual = h[p].resid;
+		stats[p].reason = h[p].done ? h[p].reason : FX_MAX_ITERATIONS;
+		stats[p].pad = 0;
+		stats[p].true_residual = h[p].omega;
+	}
+	return FX_OK;
+}
+
+void fx_batch_destroy(fx_batch* B) { batch_free(B); }
+
+fx_status fx_batch_info(const fx_batch* B, int32_t* rows_per_block, int32_t* points_per_block, uint64_t* points,
+                        int32_t* graph_iterations) {
+	FX_REQUIRE(B, FX_ERR_ARGUMENT, "null batch");
+	if (rows_per_block) *rows_per_block = B->rpb;
+	if (points_per_block) *points_per_block = B->chunk;
+	if (points) *points = (uint64_t)B->ntot;
+	if (graph_iterations) *graph_iterations = B->graph_iters;
+	return FX_OK;
 }
 
 }  // extern "C"

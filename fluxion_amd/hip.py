@@ -492,3 +492,50 @@ class FcgSolver(_Solver):
 
     def __init__(self, grid, tol, max_iter, be=None):
         super().__init__(grid, tol, max_iter, 0, be)
+
+
+# ── batched multi-problem solver (fx_batch_*; no reference counterpart) ──
+class BatchSolver:
+    """Many −∇·(k∇u) = f problems (zero Dirichlet) on grids of uneven sizes, each by FP64 flexible CG
+    around its own fused FP32 coefficient-aware V-cycle, all advancing together from a captured HIP
+    graph.  Vectors are the problems' points concatenated in order (offsets()).  `k` is an FP64
+    Buffer of the concatenated coefficients and must outlive the solver."""
+
+    def __init__(self, grids, k: "Buffer", n_pre: int = 2, n_post: int = 2, omega: float = 2.0 / 3.0,
+                 coarsest: int = 16, extra_coarse: int = 10, be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grids = list(grids)
+        self._k = k
+        n = len(self.grids)
+        arr = (L.FxGrid * n)(*[L.grid_c(g) for g in self.grids])
+        opt = L.FxMgOptions(n_pre, n_post, omega, int(SmootherKind.JACOBI), int(Transfer.FW_BILINEAR), coarsest,
+                            extra_coarse, int(DType.F32), 1, 0.0, 1.0, None)
+        h = C.c_void_p()
+        L.check(self._lib.fx_batch_create(self.be.h, n, C.cast(arr, C.c_void_p), k.h, C.byref(opt), C.byref(h)),
+                "batch_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and L.live():
+            self._lib.fx_batch_destroy(h)
+            self.h = None
+
+    def offsets(self) -> np.ndarray:
+        sizes = np.array([g.rows * g.cols for g in self.grids], dtype=np.int64)
+        return np.concatenate([[0], np.cumsum(sizes)])
+
+    def info(self) -> dict:
+        rpb, ppb, pts, gi = C.c_int32(), C.c_int32(), C.c_uint64(), C.c_int32()
+        L.check(self._lib.fx_batch_info(self.h, C.byref(rpb), C.byref(ppb), C.byref(pts), C.byref(gi)), "batch_info")
+        return {"rows_per_block": rpb.value, "points_per_block": ppb.value, "points": pts.value,
+                "graph_iterations": gi.value}
+
+    def solve(self, x: "Buffer", b: "Buffer", tol, max_iter: int) -> list:
+        n = len(self.grids)
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, dtype=np.float64), (n,)))
+        st = (L.FxSolveStats * n)()
+        L.check(self._lib.fx_batch_solve(self.h, x.h, b.h, t.ctypes.data, int(max_iter), C.cast(st, C.c_void_p)),
+                "batch_solve")
+        return [SolveStats(s.iterations, s.residual, ConvergenceReason(s.reason), s.true_residual) for s in st]

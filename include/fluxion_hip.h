@@ -296,6 +296,25 @@ int32_t fx_learned_fell_back(fx_learned* l);
 fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y,
                             const fx_grid* grid);
 
+/* ── batched multi-problem solver (extension; no reference counterpart) ───────
+ * nprob independent problems −∇·(k_i∇u_i) = f_i (zero Dirichlet) on grids of uneven sizes, each
+ * solved by FP64 flexible CG around its own fused FP32 coefficient-aware V-cycle (options `opt`:
+ * FX_F32, fused, Jacobi, FX_TRANSFER_FW_BILINEAR), all problems advancing together: every kernel of
+ * an iteration is one launch over a work list of (problem, level, tile) items and the iterations are
+ * replayed from a captured HIP graph until every problem has stopped.  Problem i's points occupy
+ * [off_i, off_i + rows_i·cols_i) of every concatenated vector (problems in order, row-major).  `k`
+ * (FP64, concatenated) must stay allocated while the batch is used.  Launch shapes that leave the
+ * per-point results unchanged are picked by timing candidates at creation (fx_batch_info). */
+typedef struct fx_batch fx_batch;
+fx_status fx_batch_create(fx_backend* be, uint32_t nprob, const fx_grid* grids, const fx_buf* k,
+                          const fx_mg_options* opt, fx_batch** out);
+/* x: initial guesses in, solutions out; tol[i] is problem i's absolute ‖r‖ tolerance; stats[nprob]. */
+fx_status fx_batch_solve(fx_batch* bt, fx_buf* x, const fx_buf* b, const double* tol, uint64_t max_iter,
+                         fx_solve_stats* stats);
+void fx_batch_destroy(fx_batch* bt);
+fx_status fx_batch_info(const fx_batch* bt, int32_t* rows_per_block, int32_t* points_per_block, uint64_t* points,
+                        int32_t* graph_iterations);
+
 #ifdef __cplusplus
 }
 #endif
