@@ -51,12 +51,25 @@ def kernel_bytes(vk: bool) -> dict:
         "fcg_update": 4 * 8 + 2 * 8 + 4,
         # z (FP32) and p_old read, p_new and A·p written (FP64); k read for the var-k operator.
         "fcg_step": 4 + 8 + 8 + 8 + (8 if vk else 0),
-        # level-0 rhs (FP32), coarse correction (¼ point, FP32) and z (FP32); r, A·p (FP64) for the dots.
-        "vc_post0": 4 + 1 + 4 + 8 + 8 + (4 if vk else 0),
-        # level-0 rhs (FP32) read, coarse rhs (¼ point, FP32) written.
-        "vc_restrict0": 4 + 1 + (4 if vk else 0),
+        # level-0 rhs (FP32), coarse correction (¼ point, FP32) and z (FP32); r, A·p (FP64) for the dots;
+        # the level's FP32 face coefficients kx, ky for the var-k operator.
+        "vc_post0": 4 + 1 + 4 + 8 + 8 + (8 if vk else 0),
+        # level-0 rhs (FP32) and face coefficients read, coarse rhs (¼ point, FP32) written.
+        "vc_restrict0": 4 + 1 + (8 if vk else 0),
         "stencil_axpy": 24,
     }
+
+
+# HBM bytes per launch of the probed level-0 kernels on the 4096² var-k workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by the same counters' reading of a copy of
+# 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
+# profiles/vark4096_mixed_fcg_lds_ring.md.  Only valid for --size 4096.
+MEASURED_TRAFFIC_4096 = {
+    "vc_restrict0": (113073 * 2 + 16400) * 1024,
+    "vc_post0": (248798 * 2 + 66270) * 1024,
+    "fcg_step": (284807 * 2 + 262208) * 1024,
+    "fcg_update": (262163 * 2 + 327744) * 1024,
+}
 
 
 def main():
@@ -284,7 +297,9 @@ def main():
         "kernels": kernels,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kd["frac"], "traffic": None, "bytes_per_point": kd["bytes_per_point"],
+                     "frac": kd["frac"],
+                     "traffic": MEASURED_TRAFFIC_4096.get(dom) if n == 4096 else None,
+                     "bytes_per_point": kd["bytes_per_point"],
                      "points_per_launch": n * n, "us_per_launch": kd["us_per_launch"]},
         "cpu_baseline": cpu,
     }
