@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
+    ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
+                    help="level kernels of the fused FP32 V-cycle (all bit-identical)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run Backend::copy of 2^24 doubles 20 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -144,6 +146,9 @@ def main():
                           fused=False, scale=1.0, k=K, be=be)
     mgv32 = hip.Multigrid(g, bc, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
                           fused=True, scale=1.0, k=K, be=be)
+    from fluxion_amd.core import MgKernels
+    mgk = {"wave": MgKernels.WAVE, "ring": MgKernels.RING, "tiled": MgKernels.TILED}[args.mg_kernels]
+    mgv32.set_kernels(mgk)
     pcg = hip.CgSolver(g, tol, max_iter, be=be)
     fcg = hip.FcgSolver(g, tol, max_iter, be=be)
 
@@ -225,6 +230,7 @@ def main():
             return hip.Multigrid(g, bc, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
                                  precision=precision, fused=fused, scale=-1.0, be=be)
         mp64, mp32 = pmg(DType.F64, False), pmg(DType.F32, True)
+        mp32.set_kernels(mgk)
         ppcg = hip.CgSolver(g, args.tol * fpn, max_iter, be=be)
         pfcg = hip.FcgSolver(g, args.tol * fpn, max_iter, be=be)
 

@@ -233,6 +233,14 @@ class SmootherKind(enum.IntEnum):
     CHEBYSHEV = 1
 
 
+class MgKernels(enum.IntEnum):
+    """Level kernels of the fused FP32 V-cycle (fx_mg_kernels); all give bit-identical results."""
+
+    TILED = 0            # LDS tiles
+    RING = 1             # LDS-ring row streams
+    WAVE = 2             # independent wave-wide row streams (default)
+
+
 class Transfer(enum.IntEnum):
     REFERENCE = 0        # 2x2 average restriction + injection prolongation (ops.rs:369-423)
     FW_BILINEAR = 1      # full-weighting restriction + bilinear prolongation

@@ -472,6 +472,8 @@ fx_status build_xtab(fx_backend* be, fx_multigrid::Level& L, const fx_grid& c) {
 	X.sx = float(double(cc - 1) / double(fc - 1));
 	X.ry = (int)span(cr, fmg::RCY, lor, hir) + 4;
 	X.rx = (int)span(cc, fmg::RCX, loc, hic) + 4;
+	X.gmax = 0;
+	for (int64_t J = 1; J + 1 < cc; ++J) X.gmax = std::max<int>(X.gmax, hic[J] - loc[J] + 1);
 	return FX_OK;
 }
 
@@ -643,6 +645,27 @@ fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b) {
 	FX_CHECK(check_buf(x, n, FX_F64, "x"));
 	FX_CHECK(check_buf(b, n, FX_F64, "b"));
 	return mg_apply(mg, (double*)x->ptr, (const double*)b->ptr, nullptr);
+}
+
+fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels) {
+	FX_REQUIRE(mg, FX_ERR_ARGUMENT, "multigrid is null");
+	FX_REQUIRE(kernels == FX_MGK_TILED || kernels == FX_MGK_RING || kernels == FX_MGK_WAVE, FX_ERR_ARGUMENT,
+	           "unknown level-kernel variant %d", kernels);
+	mg->kernels = kernels;
+	return FX_OK;
+}
+
+fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows) {
+	FX_REQUIRE(mg, FX_ERR_ARGUMENT, "multigrid is null");
+	FX_REQUIRE(level + 1 < mg->lv.size(), FX_ERR_ARGUMENT, "level %u has no coarser level (depth %zu)", level,
+	           mg->lv.size());
+	FX_REQUIRE(post_rows >= 0 && post_rows <= 4096 && restrict_rows >= 0 && restrict_rows <= 4096, FX_ERR_ARGUMENT,
+	           "segment heights must lie in [0, 4096]");
+	mg->wrows_post.resize(mg->lv.size(), 0);
+	mg->wrows_restrict.resize(mg->lv.size(), 0);
+	mg->wrows_post[level] = post_rows;
+	mg->wrows_restrict[level] = restrict_rows;
+	return FX_OK;
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@ struct fx_xtab {
 	const int32_t* hic = nullptr;
 	float sy = 0.0f, sx = 0.0f;     // float((nc−1)/(nf−1)) per axis: the restriction scale (h/H)
 	int ry = 0, rx = 0;             // LDS region (halo included) of the fused restriction's widest tile
+	int gmax = 0;                   // most fine columns any coarse column gathers from
 };
 
 struct fx_multigrid {
@@ -39,7 +40,10 @@ struct fx_multigrid {
 	fx_mg_options opt{};
 	fx_boundaries bc{};
 	bool f32 = false;
-	bool streaming = true;   // fused cycle: streaming post kernel (k_spost) instead of the tiled one
+	// Fused-cycle level kernels: FX_MGK_WAVE (wave-streaming, default), FX_MGK_RING (LDS-ring
+	// streaming) or FX_MGK_TILED; levels a variant cannot run fall back to the tiled kernels.
+	int kernels = 2;
+	std::vector<int> wrows_post, wrows_restrict;   // tuned wave segment heights per level (0: heuristic)
 	std::vector<Level> lv;
 };
 

@@ -949,6 +949,427 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 	return o;
 }
 
+// ── wave-streaming level kernels (k_wrestrict, k_wpost) ───────────────
+// The work of k_frestrict / k_fpost for a whole level, with no LDS and no
+// workgroup barrier.  Each wave is independent: it owns a strip of SRW = 64
+// fine columns (one per lane) and a segment of rows, which it walks down one
+// row per step.  Every stage (first pre-sweep, pre-smoothed iterate + P·e,
+// post-sweeps, residual) runs a fixed number of rows behind the row being
+// loaded and keeps its last three rows in registers; west and east neighbours
+// come from the adjacent lanes through DPP wavefront shifts.  Rows are loaded
+// WD steps ahead from clamped (always in-bounds) addresses, so the loads are
+// unconditional and the steps straight-line code.  Per-point arithmetic, and
+// the restriction's summation order, are those of the unfused kernels.
+constexpr int WD = 4;     // rows of loads in flight per wave
+constexpr int WNMAX = 5;  // most fine columns (or rows) a coarse point gathers from (2·(h/H)⁻¹ + 1 for H ≈ 2h)
+
+// v of lane x − 1 (lane 0: 0) and of lane x + 1 (lane 63: 0), wave-wide.
+__device__ __forceinline__ float lane_west(float v) {
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_east(float v) {
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// One row of a level's inputs at lane column gc: b and, for variable
+// coefficients, the faces west (kx[gc − 1]), east (kx[gc]) and south (ky[gc])
+// of the point.  wload reads them unconditionally from clamped addresses;
+// wfix, applied when the row is consumed (not when it is loaded, so that the
+// compiler cannot tie the load to a branch and wait on it at once), puts 0 and
+// 1 outside the grid, as the LDS kernels stage them.
+struct WRow {
+	float b, kw, ke, ky;
+};
+template <bool VK>
+__device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols, int gcc) {
+	const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
+	const size_t o = (size_t)rc * (size_t)cols;
+	WRow v;
+	v.b = b[o + gcc];
+	if constexpr (VK) {
+		v.kw = L.kx[o + (gcc > 0 ? gcc - 1 : 0)];
+		v.ke = L.kx[o + gcc];
+		v.ky = L.ky[o + gcc];
+	} else {
+		v.kw = v.ke = v.ky = 1.0f;
+	}
+	return v;
+}
+template <bool VK>
+__device__ __forceinline__ WRow wfix(const WRow& v, int r, int rows, bool colin, bool kwok, bool keok) {
+	const bool rin = r >= 0 && r < rows;
+	WRow o;
+	o.b = rin && colin ? v.b : 0.0f;
+	if constexpr (VK) {
+		o.kw = rin && kwok ? v.kw : 1.0f;
+		o.ke = rin && keok ? v.ke : 1.0f;
+		o.ky = rin && colin && r < rows - 1 ? v.ky : 1.0f;
+	} else {
+		o.kw = o.ke = o.ky = 1.0f;
+	}
+	return o;
+}
+
+// lin1(r, nf, nc) of consecutive rows r (clamped to [0, nf − 1]): the coarse
+// row i0 below fine row r and the remainder rem of r·(nc − 1) = i0·(nf − 1) + rem,
+// advanced by exact integer steps, so w = double(rem)·(1/(nf − 1)) equals the
+// transfer table's weight without a memory access.
+struct LinRow {
+	int r, i0, rem, nf, step, den;
+	__device__ void init(int r0, int nf_, int nc) {
+		nf = nf_, step = nc - 1, den = nf_ - 1, r = r0;
+		const int rc = r0 < 0 ? 0 : (r0 < nf_ ? r0 : nf_ - 1);
+		const int64_t num = (int64_t)rc * step;
+		i0 = (int)(num / den);
+		rem = (int)(num % den);
+	}
+	__device__ void next() {
+		++r;
+		if (r >= 1 && r <= nf - 1) {
+			rem += step;
+			while (rem >= den) rem -= den, ++i0;
+		}
+	}
+	__device__ double w(double inv) const { return double(rem) * inv; }
+};
+
+// Waves of a wave-streaming launch over a level: strips × segments.
+inline int64_t wave_strips_post(int64_t cols) { return (cols + SSW - 1) / SSW; }
+inline int64_t wave_strips_restrict(int64_t cc) { return (cc + RCX - 1) / RCX; }
+inline unsigned wave_blocks(int64_t nwaves) { return (unsigned)((nwaves + NWAVES - 1) / NWAVES); }
+
+// out = J^NPOST(J^NPRE(0; b) + P·e) over the level (k_fpost's result), with Epi's
+// per-point reduction terms.  Wave gw takes strip gw % nstrips and row segment
+// gw / nstrips of sh rows.
+template <bool VK, int NPRE, int NPOST, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, const float* __restrict__ ec,
+                                                 float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
+                                                 int cc, int nstrips, int nseg, int sh, Epi epi, RedScratch rs,
+                                                 const int* skip) {
+	if (skip && *skip) return;
+	constexpr int LU = NPRE - 1;     // rows the iterate u = J^npre(0; b) + P·e lags the loaded row
+	constexpr int LS = LU + 1;       // first post-sweep (NPOST == 2)
+	constexpr int LZ = LU + NPOST;   // the level's result
+	static_assert(LZ <= SHX, "halo too small");
+	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
+	Acc acc[NA];
+#pragma unroll
+	for (int k = 0; k < NA; ++k) acc[k] = Acc{0.0, 0.0};
+	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	const int x = (int)(threadIdx.x & (WAVE - 1));
+	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
+	if (gw < nstrips * nseg) {
+		const int strip = gw % nstrips, seg = gw / nstrips;
+		const int gc = strip * SSW - SHX + x;
+		const int gcc = gc < 0 ? 0 : (gc < cols ? gc : cols - 1);
+		const bool colin = gc >= 0 && gc < cols;
+		const bool colint = gc >= 1 && gc <= cols - 2;
+		const bool kwok = gc >= 1 && gc <= cols - 1, keok = gc >= 0 && gc <= cols - 2;
+		const bool outlane = colin && x >= SHX && x < SRW - SHX;
+		const int S0 = seg * sh, S1 = S0 + sh < rows ? S0 + sh : rows;
+		// Column interpolation data of this lane.
+		int b0 = 0, b1 = 0;
+		float wx = 0.0f, vx = 0.0f;
+		if (colint) {
+			const int a = X.i0c[gc];
+			b0 = a;
+			b1 = a + 1 < cc ? a + 1 : a;
+			wx = float(X.wc[gc]);
+			vx = 1.0f - wx;
+		}
+		auto inter = [&](int r) { return colint && r >= 1 && r <= rows - 2; };
+		// Coarse-correction operands of row r (read when row r + LU is loaded).
+		struct EV {
+			float e00, e01, e10, e11, wy;
+		};
+		const double rinv = 1.0 / double(rows - 1);
+		LinRow le;  // row of the next coarse-correction load
+		auto eload = [&]() {
+			const int a0 = le.i0;
+			const int a1 = a0 + 1 < cr ? a0 + 1 : a0;
+			const float* e0 = ec + (size_t)a0 * (size_t)cc;
+			const float* e1 = ec + (size_t)a1 * (size_t)cc;
+			const EV v{e0[b0], e0[b1], e1[b0], e1[b1], float(le.w(rinv))};
+			le.next();
+			return v;
+		};
+		auto pload = [&](int r) {
+			const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
+			return epi.load((int64_t)rc * cols + gcc);
+		};
+		float B[LZ + 1], KW[LZ + 1], KE[LZ + 1], KY[LZ + 2];
+#pragma unroll
+		for (int i = 0; i <= LZ; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f;
+#pragma unroll
+		for (int i = 0; i <= LZ + 1; ++i) KY[i] = 1.0f;
+		float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
+		// Steps load rows T0 … T1 − 1: the first output row needs the first pre-sweep from row S0 − LZ on,
+		// and that sweep needs the south face of the row above it.
+		const int T0 = S0 - LZ - 1, T1 = S1 + LZ;
+		le.init(T0 - LU, rows, cr);
+		WRow inb[WD];
+		EV evb[WD];
+		typename Epi::Pf pfb[WD];
+#pragma unroll
+		for (int k = 0; k < WD; ++k) {
+			inb[k] = wload<VK>(b, L, T0 + k, rows, cols, gcc);
+			evb[k] = eload();
+			pfb[k] = pload(T0 + k - LZ);
+		}
+		for (int t0 = T0; t0 < T1; t0 += WD) {
+#pragma unroll
+			for (int k = 0; k < WD; ++k) {
+				const int t = t0 + k;
+				// Row windows: index i holds row t − i.
+#pragma unroll
+				for (int i = LZ; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1];
+#pragma unroll
+				for (int i = LZ + 1; i > 0; --i) KY[i] = KY[i - 1];
+				{
+					const WRow in = wfix<VK>(inb[k], t, rows, colin, kwok, keok);
+					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky;
+				}
+				const EV ev = evb[k];
+				// Faces of row t − i: west KW[i], east KE[i], north KY[i + 1], south KY[i].
+				float u = 0.0f;  // iterate at row t − LU
+				if constexpr (NPRE == 2) {
+					float u1 = 0.0f;
+					if (inter(t)) {
+						if constexpr (VK) u1 = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						else u1 = jac0(L, B[0]);
+					}
+					U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
+					const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
+					if (inter(t - 1)) {
+						if constexpr (VK) u = jackf(L, KW[1], KE[1], KY[2], KY[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
+						else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
+					}
+				} else {
+					if (inter(t)) {
+						if constexpr (VK) u = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						else u = jac0(L, B[0]);
+					}
+				}
+				if (inter(t - LU)) {
+					const float wy = ev.wy, vy = 1.0f - wy;
+					const float top = vx * ev.e00 + wx * ev.e01;
+					const float bot = vx * ev.e10 + wx * ev.e11;
+					u += vy * top + wy * bot;
+				}
+				U[2] = U[1], U[1] = U[0], U[0] = u;
+				const float* C = U;
+				if constexpr (NPOST == 2) {
+					float s = U[1];
+					const float wv = lane_west(U[1]), evv = lane_east(U[1]);
+					if (inter(t - LS)) {
+						if constexpr (VK)
+							s = jackf(L, KW[LS], KE[LS], KY[LS + 1], KY[LS], U[1], U[2], U[0], wv, evv, B[LS]);
+						else
+							s = jac(L, U[1], U[2], U[0], wv, evv, B[LS]);
+					}
+					S[2] = S[1], S[1] = S[0], S[0] = s;
+					C = S;
+				}
+				float z = C[1];
+				const float wv = lane_west(C[1]), evv = lane_east(C[1]);
+				const int rz = t - LZ;
+				if (inter(rz)) {
+					if constexpr (VK) z = jackf(L, KW[LZ], KE[LZ], KY[LZ + 1], KY[LZ], C[1], C[2], C[0], wv, evv, B[LZ]);
+					else z = jac(L, C[1], C[2], C[0], wv, evv, B[LZ]);
+				}
+				if (outlane && rz >= S0 && rz < S1) {
+					out[(size_t)rz * (size_t)cols + gc] = z;
+					epi.use(pfb[k], z, acc);
+				}
+				// Refill this step's slots (after their last use, so each slot keeps its registers).
+				inb[k] = wload<VK>(b, L, t + WD, rows, cols, gcc);
+				evb[k] = eload();
+				pfb[k] = pload(t + WD - LZ);
+			}
+		}
+	}
+	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
+}
+
+// coarse rhs = R(b − A·J^npre(0; b)) over the level (k_frestrict's result).
+// Wave gw takes RCX coarse columns (strip gw % nstrips; lane l < RCX owns
+// column J0 + l) and scr coarse rows (segment gw / nstrips); its 64 lanes hold
+// the fine window from two columns left of the strip's first gathered fine
+// column.  Each fine row's residual is summed over the coarse column's fine
+// columns in ascending order (cross-lane permutes), then added, in ascending
+// fine-row order, to the (at most two) coarse rows the fine row contributes
+// to: the order of k_restrict's loops.
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
+                                                     fx_xtab X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
+                                                     int scr, const int* skip) {
+	if (skip && *skip) return;
+	constexpr int LU = NPRE - 1, LR = NPRE;  // lags of the pre-smoothed iterate and of the residual
+	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	const int x = (int)(threadIdx.x & (WAVE - 1));
+	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
+	if (gw >= nstrips * nseg) return;  // no workgroup-wide operation follows
+	const int strip = gw % nstrips, seg = gw / nstrips;
+	const int J0 = strip * RCX, I0 = seg * scr;
+	const int Jend = J0 + RCX < cc ? J0 + RCX : cc, Iend = I0 + scr < cr ? I0 + scr : cr;
+	const int J = J0 + x;
+	const bool own = x < RCX && J < Jend;
+	// Boundary coarse rows and columns are zero.
+	if (own) {
+		if (I0 == 0) bco[J] = 0.0f;
+		if (Iend == cr) bco[(size_t)(cr - 1) * cc + J] = 0.0f;
+		if (J == 0 || J == cc - 1)
+			for (int I = I0; I < Iend; ++I) bco[(size_t)I * cc + J] = 0.0f;
+	}
+	const int Ia = I0 > 1 ? I0 : 1, Ib = Iend - 1 < cr - 2 ? Iend - 1 : cr - 2;
+	const int Ja = J0 > 1 ? J0 : 1, Jb = Jend - 1 < cc - 2 ? Jend - 1 : cc - 2;
+	if (Ia > Ib || Ja > Jb) return;
+	const int G0 = X.loc[Ja];
+	const int F0 = X.lor[Ia], F1 = X.hir[Ib];
+	const bool jint = own && J >= 1 && J <= cc - 2;
+	if (G0 > X.hic[Jb] || F0 > F1) {  // nothing to gather: the strip's interior rows are zero
+		if (jint)
+			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + J] = 0.0f;
+		return;
+	}
+	const int C0 = G0 - 2;  // fine column of lane 0
+	const int gc = C0 + x;
+	const int gcc = gc < 0 ? 0 : (gc < fc ? gc : fc - 1);
+	const bool colin = gc >= 0 && gc < fc;
+	const bool colint = gc >= 1 && gc <= fc - 2;
+	const bool kwok = gc >= 1 && gc <= fc - 1, keok = gc >= 0 && gc <= fc - 2;
+	auto inter = [&](int r) { return colint && r >= 1 && r <= fr - 2; };
+	// The owned coarse column's fine columns lo..hi, their weights and source lanes.
+	int cnt = 0;
+	float wk[WNMAX];
+	int sk[WNMAX];
+	{
+		const int lo = jint ? X.loc[J] : 0, hi = jint ? X.hic[J] : -1;
+		cnt = hi - lo + 1;
+#pragma unroll
+		for (int k = 0; k < WNMAX; ++k) {
+			const bool v = k < cnt;
+			const int jj = v ? lo + k : 0;
+			const int ac = v ? X.i0c[jj] : -4;
+			const double wv = v ? X.wc[jj] : 0.0;
+			wk[k] = ac == J ? float(1.0 - wv) : (ac + 1 == J ? float(wv) : 0.0f);
+			sk[k] = v ? jj - C0 : 0;
+		}
+	}
+	const float scale = X.sy * X.sx;
+	float B[LR + 1], KW[LR + 1], KE[LR + 1], KY[LR + 2];
+#pragma unroll
+	for (int i = 0; i <= LR; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f;
+#pragma unroll
+	for (int i = 0; i <= LR + 1; ++i) KY[i] = 1.0f;
+	float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f};
+	int Icur = X.i0r[F0];  // coarse row accumulated in alo (ahi: Icur + 1)
+	float alo = 0.0f, ahi = 0.0f;
+	auto emit = [&](int I, float v) {
+		if (jint && I >= Ia && I <= Ib) bco[(size_t)I * cc + J] = v * scale;
+	};
+	// Steps load rows T0 … T1 − 1: the residual of row F0 needs the first pre-sweep from row F0 − LR on,
+	// and that sweep needs the south face of the row above it.
+	const int T0 = F0 - LR - 1, T1 = F1 + LR + 1;
+	const double rinv = 1.0 / double(fr - 1);
+	LinRow lr;  // fine row whose residual the step produces
+	lr.init(T0 - LR, fr, cr);
+	WRow inb[WD];
+#pragma unroll
+	for (int k = 0; k < WD; ++k) inb[k] = wload<VK>(b, L, T0 + k, fr, fc, gcc);
+	for (int t0 = T0; t0 < T1; t0 += WD) {
+#pragma unroll
+		for (int k = 0; k < WD; ++k) {
+			const int t = t0 + k;
+#pragma unroll
+			for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1];
+#pragma unroll
+			for (int i = LR + 1; i > 0; --i) KY[i] = KY[i - 1];
+			{
+				const WRow in = wfix<VK>(inb[k], t, fr, colin, kwok, keok);
+				B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky;
+			}
+			inb[k] = wload<VK>(b, L, t + WD, fr, fc, gcc);  // slot k refilled
+			float u = 0.0f;  // pre-smoothed iterate at row t − LU
+			if constexpr (NPRE == 2) {
+				float u1 = 0.0f;
+				if (inter(t)) {
+					if constexpr (VK) u1 = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+					else u1 = jac0(L, B[0]);
+				}
+				U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
+				const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
+				if (inter(t - 1)) {
+					if constexpr (VK) u = jackf(L, KW[1], KE[1], KY[2], KY[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
+					else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
+				}
+			} else {
+				if (inter(t)) {
+					if constexpr (VK) u = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+					else u = jac0(L, B[0]);
+				}
+			}
+			U[2] = U[1], U[1] = U[0], U[0] = u;
+			// Residual at fine row i = t − LR.
+			const int i = t - LR;
+			const float wv = lane_west(U[1]), evv = lane_east(U[1]);
+			float res = 0.0f;
+			if (inter(i)) {
+				float sv;
+				if constexpr (VK) {
+					float d;
+					sv = vkff(L, KW[LR], KE[LR], KY[LR + 1], KY[LR], U[1], U[2], U[0], wv, evv, &d);
+				} else {
+					sv = sop(L, U[1], U[2], U[0], wv, evv);
+				}
+				res = fmaf(1.0f, B[LR], -sv);
+			}
+			// Row sum over the owned coarse column's fine columns.
+			float row = 0.0f;
+#pragma unroll
+			for (int k2 = 0; k2 < WNMAX; ++k2) {
+				const float v = __shfl(res, sk[k2]);
+				if (k2 < cnt) row += wk[k2] * v;
+			}
+			const int a = lr.i0;
+			const double wrv = lr.w(rinv);
+			lr.next();
+			if (i >= F0 && i <= F1) {
+				while (Icur < a) {
+					emit(Icur, alo);
+					alo = ahi;
+					ahi = 0.0f;
+					++Icur;
+				}
+				alo += float(1.0 - wrv) * row;
+				ahi += float(wrv) * row;
+			}
+		}
+	}
+	emit(Icur, alo);
+	emit(Icur + 1, ahi);
+}
+
+// Segment heights of the wave-streaming kernels for a level: the tallest of
+// 256 … 8 fine rows (post) or 128 … 4 coarse rows (restrict) that still gives
+// WAVE_TARGET waves (eight per CU), unless a tuned height is set.
+constexpr int64_t WAVE_TARGET = 2048;
+inline int wpost_rows(int64_t rows, int64_t cols) {
+	const int64_t ns = wave_strips_post(cols);
+	int sh = 256;
+	while (sh > 8 && ns * ((rows + sh - 1) / sh) < WAVE_TARGET) sh /= 2;
+	return sh;
+}
+inline int wrestrict_rows(int64_t cr, int64_t cc) {
+	const int64_t ns = wave_strips_restrict(cc);
+	int s = 128;
+	while (s > 4 && ns * ((cr + s - 1) / s) < WAVE_TARGET) s /= 2;
+	return s;
+}
+// The wave restriction needs each strip's fine window (two columns of halo
+// each side) within one wave and at most WNMAX fine columns per coarse column.
+inline bool wrestrict_ok(const fx_xtab& X) { return X.rx <= SRW && X.gmax <= WNMAX; }
+
 // Whether the fused cycle can run this hierarchy.
 inline bool fused_ok(const fx_multigrid* mg) {
 	const auto& o = mg->opt;
@@ -968,20 +1389,31 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 	auto& lv = mg->lv;
 	const int nl = (int)lv.size();
 	const auto& o = mg->opt;
+	auto tuned = [](const std::vector<int>& v, int l) { return l < (int)v.size() ? v[l] : 0; };
 	for (int l = 0; l < nl - 1; ++l) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
 		if (l == 0) fx::probe_begin(be, FX_PROBE_VC_RESTRICT0);
-		const int scr = mg->streaming ? srestrict_height(lv[l].xt, cg.rows, cg.cols) : 0;
-		if (scr > 0)
-			hipLaunchKernelGGL((k_srestrict<VK, NPRE>), dim3(srestrict_blocks(cg.rows, cg.cols, scr)), dim3(BLOCK), 0, st,
+		if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt)) {
+			int scr = tuned(mg->wrows_restrict, l);
+			if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
+			const int64_t ns = wave_strips_restrict(cg.cols), nseg = (cg.rows + scr - 1) / scr;
+			hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
 			                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lwf(mg, lv[l]), lv[l].xt, (int)g.rows,
-			                   (int)g.cols, (int)cg.rows, (int)cg.cols, (int)((cg.cols + RCX - 1) / RCX), scr, skip);
-		else
-			hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
-			                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs,
-			                   lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows,
-			                   (int64_t)cg.cols, skip);
+			                   (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, scr, skip);
+		} else {
+			const int scr = mg->kernels == FX_MGK_RING ? srestrict_height(lv[l].xt, cg.rows, cg.cols) : 0;
+			if (scr > 0)
+				hipLaunchKernelGGL((k_srestrict<VK, NPRE>), dim3(srestrict_blocks(cg.rows, cg.cols, scr)), dim3(BLOCK), 0,
+				                   st, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lwf(mg, lv[l]), lv[l].xt,
+				                   (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols,
+				                   (int)((cg.cols + RCX - 1) / RCX), scr, skip);
+			else
+				hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
+				                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs,
+				                   (float*)lv[l + 1].rhs, lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows, (int64_t)g.cols,
+				                   (int64_t)cg.rows, (int64_t)cg.cols, skip);
+		}
 		if (l == 0) fx::probe_end(be, FX_PROBE_VC_RESTRICT0);
 	}
 	{
@@ -989,43 +1421,59 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		hipLaunchKernelGGL(k_fcoarse, dim3(1), dim3(1024), 0, st, (const float*)L.rhs, (float*)L.tmp, lwf(mg, L),
 		                   o.n_pre + o.n_post + o.extra_coarse, (int)L.g.rows, (int)L.g.cols, skip);
 	}
-	auto post = [&](int l, auto epi, RedScratch rs, unsigned* nblocks) {
+	// Launch shape of a level's post kernel: {variant, blocks, wave strips, segments, rows per segment}.
+	struct PostShape {
+		int kind;
+		unsigned blocks;
+		int ns, nseg, sh;
+	};
+	auto post_shape = [&](int l) {
+		const fx_grid& g = lv[l].g;
+		if (mg->kernels == FX_MGK_WAVE) {
+			int sh = tuned(mg->wrows_post, l);
+			if (sh <= 0) sh = wpost_rows(g.rows, g.cols);
+			const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
+			return PostShape{FX_MGK_WAVE, wave_blocks(ns * nseg), (int)ns, (int)nseg, sh};
+		}
+		const int sh = mg->kernels == FX_MGK_RING ? spost_height(g.rows, g.cols) : 0;
+		if (sh > 0) return PostShape{FX_MGK_RING, spost_blocks(g.rows, g.cols, sh), (int)((g.cols + SSW - 1) / SSW), 0, sh};
+		const dim3 tg = tgrid(g.rows, g.cols);
+		return PostShape{FX_MGK_TILED, tg.x * tg.y, 0, 0, 0};
+	};
+	auto post = [&](int l, auto epi, RedScratch rs, const PostShape& ps) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
-		const int sh = mg->streaming ? spost_height(g.rows, g.cols) : 0;
-		if (sh > 0) {
-			const unsigned nbk = spost_blocks(g.rows, g.cols, sh);
-			const int ns = (int)((g.cols + SSW - 1) / SSW);
-			hipLaunchKernelGGL((k_spost<VK, NPRE, NPOST, decltype(epi)>), dim3(nbk), dim3(BLOCK), 0, st,
+		if (ps.kind == FX_MGK_WAVE) {
+			hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, decltype(epi)>), dim3(ps.blocks), dim3(BLOCK), 0, st,
 			                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
-			                   lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, ns, sh, epi, rs, skip);
-			*nblocks = nbk;
+			                   lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, ps.ns, ps.nseg, ps.sh, epi,
+			                   rs, skip);
+		} else if (ps.kind == FX_MGK_RING) {
+			hipLaunchKernelGGL((k_spost<VK, NPRE, NPOST, decltype(epi)>), dim3(ps.blocks), dim3(BLOCK), 0, st,
+			                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
+			                   lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, ps.ns, ps.sh, epi, rs, skip);
 		} else {
 			const dim3 tg = tgrid(g.rows, g.cols);
 			hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, decltype(epi)>), tg, dim3(BLOCK), 0, st, (const float*)lv[l].rhs,
 			                   (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]), lv[l].xt, (int64_t)g.rows,
 			                   (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, epi, rs, skip);
-			*nblocks = tg.x * tg.y;
 		}
 	};
-	unsigned nbk = 0;
-	for (int l = nl - 2; l >= 1; --l) post(l, NoEpi{}, RedScratch{nullptr}, &nbk);
+	for (int l = nl - 2; l >= 1; --l) post(l, NoEpi{}, RedScratch{nullptr}, post_shape(l));
+	const PostShape ps0 = post_shape(0);
 	RedScratch rs{nullptr};
 	if (Epi::NS > 0) {
-		const fx_grid& g = lv[0].g;
-		const dim3 tg = tgrid(g.rows, g.cols);
-		FX_CHECK(fx::ensure_partials(be, (size_t)(tg.x * tg.y + spost_blocks(g.rows, g.cols, 32)) * Epi::NS));
+		FX_CHECK(fx::ensure_partials(be, (size_t)ps0.blocks * Epi::NS));
 		rs = fx::red_scratch(be);
 	}
 	fx::probe_begin(be, FX_PROBE_VC_POST0);
-	post(0, epi0, rs, &nbk);
+	post(0, epi0, rs, ps0);
 	fx::probe_end(be, FX_PROBE_VC_POST0);
 	if constexpr (Epi::NS > 0)
-		hipLaunchKernelGGL((k_finish<Epi::NS, typename Epi::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, rs, nbk,
+		hipLaunchKernelGGL((k_finish<Epi::NS, typename Epi::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, rs, ps0.blocks,
 		                   epi0.fin(), skip);
 	return fx::check_launch("fused V-cycle");
 }
-
 template <bool VK, class Epi>
 fx_status fast_cycle_v(fx_multigrid* mg, const int* skip, Epi epi0) {
 	const int a = mg->opt.n_pre, b = mg->opt.n_post;

@@ -388,6 +388,15 @@ class Multigrid:
     def v_cycle(self, x: Field, b: Field):
         L.check(self._lib.fx_multigrid_vcycle(self.h, x.h, b.h), "v_cycle")
 
+    def set_kernels(self, kernels: int):
+        """Level kernels of the fused FP32 cycle (MgKernels)."""
+        L.check(self._lib.fx_multigrid_set_kernels(self.h, int(kernels)), "multigrid_set_kernels")
+
+    def set_wave_rows(self, level: int, post_rows: int, restrict_rows: int):
+        """Wave segment heights of one level (0: built-in choice)."""
+        L.check(self._lib.fx_multigrid_set_wave_rows(self.h, level, post_rows, restrict_rows),
+                "multigrid_set_wave_rows")
+
     def apply(self, z: Field, r: Field):
         self.v_cycle(z, r)
 

@@ -222,6 +222,14 @@ void fx_multigrid_destroy(fx_multigrid* mg);
 uint32_t fx_multigrid_depth(const fx_multigrid* mg);                                  /* multigrid.rs:172-175 */
 /* Multigrid::v_cycle(x, b) (multigrid.rs:188-230): x is the initial guess, overwritten. */
 fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b);
+/* Level kernels of the fused FP32 V-cycle (extension; no reference counterpart).  FX_MGK_WAVE (the
+ * default) runs each level as independent wave-wide row streams, FX_MGK_RING as LDS-ring row streams and
+ * FX_MGK_TILED as LDS tiles; all three give bit-identical results.  fx_multigrid_set_wave_rows sets
+ * level `level`'s wave segment heights (fine rows per wave of its post kernel, coarse rows per wave of
+ * its restriction; 0 restores the built-in choice). */
+typedef enum { FX_MGK_TILED = 0, FX_MGK_RING = 1, FX_MGK_WAVE = 2 } fx_mg_kernels;
+fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels);
+fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows);
 
 /* Preconditioner handle (preconditioner.rs:10-37). */
 typedef enum { FX_PC_IDENTITY = 0, FX_PC_MULTIGRID = 1, FX_PC_LEARNED = 2 } fx_pc_kind;

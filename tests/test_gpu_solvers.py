@@ -378,6 +378,43 @@ def test_vark_vcycle_bitwise(be, precision, fused, transfer):
     assert np.array_equal(X.to_vec_f64(), O.vcycle(np.zeros(n * n), r, g, cfg))
 
 
+@pytest.mark.parametrize("shape,vk", [((129, 129), True), ((200, 131), True), ((1000, 777), True),
+                                      ((4096, 4096), True), ((257, 300), False), ((2048, 2048), False)])
+def test_fused_vcycle_kernel_variants(be, shape, vk):
+    # The tiled, LDS-ring and wave-streaming level kernels, and wave segment heights from 4 to 256 rows,
+    # all reproduce the oracle's FP32 V-cycle bit for bit.
+    hip = H()
+    from fluxion_amd.core import MgKernels
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    r = O.zero_boundaries(np.random.default_rng(17).standard_normal(rows * cols), rows, cols)
+    k = None
+    if vk:
+        y, x = np.linspace(0, 1, rows)[:, None], np.linspace(0, 1, cols)[None, :]
+        k = np.exp(1.2 * np.sin(7 * x + 1) * np.cos(5 * y) + 0.5 * np.sin(23 * x * y)).ravel()
+    scale = 1.0 if vk else -1.0
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                       fused=True, scale=scale, k=hip.Field.from_f64(g, k, be) if vk else None, be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=scale, k=k)
+    ref = O.vcycle(np.zeros(rows * cols), r, g, cfg)
+    B = hip.Field.from_f64(g, r, be)
+
+    def run():
+        X = hip.Field.zeros(g, be=be)
+        mg.v_cycle(X, B)
+        return X.to_vec_f64()
+
+    for kern in (MgKernels.TILED, MgKernels.RING, MgKernels.WAVE):
+        mg.set_kernels(kern)
+        assert np.array_equal(run(), ref), kern.name
+    for post_rows, restrict_rows in ((8, 4), (256, 128), (24, 7)):
+        for lev in range(mg.depth() - 1):
+            mg.set_wave_rows(lev, post_rows, restrict_rows)
+        assert np.array_equal(run(), ref), (post_rows, restrict_rows)
+    with pytest.raises(Exception):
+        mg.set_wave_rows(mg.depth() - 1, 8, 4)   # the coarsest level has no level kernels
+
+
 @pytest.mark.parametrize("n", [256, 255])
 def test_vark_fcg_fused_mg(be, n):
     # −∇·(k∇u) = f with log-normal k: FCG + fused FP32 coefficient-aware V-cycle against the oracle.
