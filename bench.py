@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in wave segment heights")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run Backend::copy of 2^24 doubles 20 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -149,6 +150,8 @@ def main():
     from fluxion_amd.core import MgKernels
     mgk = {"wave": MgKernels.WAVE, "ring": MgKernels.RING, "tiled": MgKernels.TILED}[args.mg_kernels]
     mgv32.set_kernels(mgk)
+    if mgk == MgKernels.WAVE and not args.no_autotune:
+        log(f"bench: autotuned wave segment heights, {mgv32.autotune(5):.3f} ms per cycle of level kernels")
     pcg = hip.CgSolver(g, tol, max_iter, be=be)
     fcg = hip.FcgSolver(g, tol, max_iter, be=be)
 
@@ -231,6 +234,8 @@ def main():
                                  precision=precision, fused=fused, scale=-1.0, be=be)
         mp64, mp32 = pmg(DType.F64, False), pmg(DType.F32, True)
         mp32.set_kernels(mgk)
+        if mgk == MgKernels.WAVE and not args.no_autotune:
+            mp32.autotune(5)
         ppcg = hip.CgSolver(g, args.tol * fpn, max_iter, be=be)
         pfcg = hip.FcgSolver(g, args.tol * fpn, max_iter, be=be)
 

@@ -142,6 +142,7 @@ _SIGS = {
     "fx_multigrid_vcycle": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p]),
     "fx_multigrid_set_kernels": (C.c_int, [C.c_void_p, C.c_int32]),
     "fx_multigrid_set_wave_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_int32]),
+    "fx_multigrid_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "fx_solver_create": (C.c_int, [c_be_p, C.c_int32, _P(FxGrid), C.c_uint32, C.c_double, C.c_uint64,
                                    _P(C.c_void_p)]),
     "fx_solver_destroy": (None, [C.c_void_p]),

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_jac(const T* xo, T* xn, const T* b, L
 	if (L.k) {
 		T d;
 		T au = vk_apply<T>(xo, L, i, cols, &d);
-		xn[i] = fmaT<T>(L.omega, (b[i] - au) / d, xo[i]);
+		xn[i] = fmaT<T>(L.omega / d, b[i] - au, xo[i]);
 		return;
 	}
 	T ax = T(0);
@@ -272,6 +272,22 @@ __global__ void k_faces(const float* k, float* kx, float* ky, int64_t rows, int6
 	ky[i] = r + 1 < rows ? fmg::hmf(k[i], k[i + cols]) : 0.0f;
 }
 
+// ω/diagonal of the variable-coefficient level operator on interior points, from the
+// face coefficients (the d of fmg::vkff, so wd equals the ω/d of fmg::jackf).
+__global__ void k_wdiag(const float* kx, const float* ky, float* wd, fmg::LWf L, int64_t rows, int64_t cols) {
+	int64_t c = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (r >= rows || c >= cols) return;
+	int64_t i = r * cols + c;
+	if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) {
+		wd[i] = 0.0f;
+		return;
+	}
+	float d;
+	(void)fmg::vkff(L, kx[i - 1], kx[i], ky[i - cols], ky[i], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, &d);
+	wd[i] = L.omega / d;
+}
+
 template <class T>
 struct Driver {
 	fx_multigrid* mg;
@@ -400,6 +416,7 @@ void free_levels(fx_multigrid* mg) {
 		fx::dev_free(L.k);
 		fx::dev_free(L.kx);
 		fx::dev_free(L.ky);
+		fx::dev_free(L.wd);
 		fx::dev_free(L.xbuf);
 	}
 	mg->lv.clear();
@@ -593,10 +610,17 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 			const size_t bytes = (size_t)L.g.rows * L.g.cols * sizeof(float);
 			fx_status s = dev_alloc(be, bytes, &L.kx);
 			if (s == FX_OK) s = dev_alloc(be, bytes, &L.ky);
+			if (s == FX_OK) s = dev_alloc(be, bytes, &L.wd);
 			if (s == FX_OK) {
 				hipLaunchKernelGGL(k_faces, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, be->stream, (const float*)L.k,
 				                   (float*)L.kx, (float*)L.ky, (int64_t)L.g.rows, (int64_t)L.g.cols);
 				s = check_launch("multigrid face coefficients");
+			}
+			if (s == FX_OK) {
+				hipLaunchKernelGGL(k_wdiag, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, be->stream, (const float*)L.kx,
+				                   (const float*)L.ky, (float*)L.wd, fmg::lwf(mg, L), (int64_t)L.g.rows,
+				                   (int64_t)L.g.cols);
+				s = check_launch("multigrid omega/diagonal");
 			}
 			if (s != FX_OK) { free_levels(mg); delete mg; return s; }
 		}
@@ -653,6 +677,13 @@ fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels) {
 	           "unknown level-kernel variant %d", kernels);
 	mg->kernels = kernels;
 	return FX_OK;
+}
+
+fx_status fx_multigrid_autotune(fx_multigrid* mg, int32_t reps, double* ms_out) {
+	FX_REQUIRE(mg, FX_ERR_ARGUMENT, "multigrid is null");
+	FX_REQUIRE(reps >= 1 && reps <= 1000, FX_ERR_ARGUMENT, "reps must lie in [1, 1000]");
+	FX_REQUIRE(fmg::fused_ok(mg), FX_ERR_UNSUPPORTED, "autotuning needs a fused FP32 hierarchy");
+	return fmg::autotune(mg, reps, ms_out);
 }
 
 fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows) {

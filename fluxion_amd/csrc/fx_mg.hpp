@@ -31,6 +31,7 @@ struct fx_multigrid {
 		void* k = nullptr;     // nodal coefficient (T[n]) of a variable-coefficient hierarchy
 		void* kx = nullptr;    // FP32 fused hierarchies: harmonic-mean face coefficient east of each point
 		void* ky = nullptr;    //   and south of each point (float[n]; 0 past the last column / row)
+		void* wd = nullptr;    //   and ω/diagonal of the level operator (float[n]; 0 on the boundary)
 		double w[5];           // sigma·I + scale·Δ_h weights, laplacian_2d_5pt order
 		double inv_diag;
 		fx_xtab xt;            // transfer to the next coarser level (linear transfers only)

@@ -26,10 +26,11 @@ struct LWf {
 	float inv_diag;
 	float omega;
 	// Variable-coefficient level operator when k != nullptr (fx_mg.hip vk_apply);
-	// kx / ky: its face coefficients (fused hierarchies).
+	// kx / ky: its face coefficients and wd: ω/diagonal on interior points (fused hierarchies).
 	const float* k;
 	const float* kx;
 	const float* ky;
+	const float* wd;
 	float ix2, iy2, sigma, scale;
 };
 
@@ -53,18 +54,26 @@ __device__ __forceinline__ float vkff(const LWf& L, float hw, float he, float hn
 	*diag = fmaf(L.scale, (hw + he) * L.ix2 + (hn + hs) * L.iy2, L.sigma);
 	return fmaf(L.scale, Lv, L.sigma * c);
 }
+// Variable-coefficient Jacobi point update u + (ω/d)·(b − A·u) (fx_mg.hip k_jac with k).
 __device__ __forceinline__ float jackf(const LWf& L, float hw, float he, float hn, float hs, float c, float n, float s,
                                        float w, float e, float b) {
 	float d;
 	float au = vkff(L, hw, he, hn, hs, c, n, s, w, e, &d);
-	return fmaf(L.omega, (b - au) / d, c);
+	return fmaf(L.omega / d, b - au, c);
+}
+// jackf with ω/d read from the level's wd plane (k_wdiag: the same ω/d).
+__device__ __forceinline__ float jackw(const LWf& L, float hw, float he, float hn, float hs, float wd, float c, float n,
+                                       float s, float w, float e, float b) {
+	float d;
+	float au = vkff(L, hw, he, hn, hs, c, n, s, w, e, &d);
+	return fmaf(wd, b - au, c);
 }
 // Variable-coefficient Jacobi point update (fx_mg.hip k_jac with k).
 __device__ __forceinline__ float jack(const LWf& L, float kc, float kn, float ks, float kw, float ke, float c, float n,
                                       float s, float w, float e, float b) {
 	float d;
 	float au = vkf(L, kc, kn, ks, kw, ke, c, n, s, w, e, &d);
-	return fmaf(L.omega, (b - au) / d, c);
+	return fmaf(L.omega / d, b - au, c);
 }
 // Jacobi point update (fx_mg.hip k_jac): ax over (0,0),(−1,0),(1,0),(0,−1),(0,1).
 __device__ __forceinline__ float jac(const LWf& L, float c, float n, float s, float w, float e, float b) {
@@ -942,6 +951,7 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 	o.k = (const float*)L.k;
 	o.kx = (const float*)L.kx;
 	o.ky = (const float*)L.ky;
+	o.wd = (const float*)L.wd;
 	o.ix2 = float(1.0 / (L.g.dx * L.g.dx));
 	o.iy2 = float(1.0 / (L.g.dy * L.g.dy));
 	o.sigma = float(mg->opt.sigma);
@@ -960,7 +970,7 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 // WD steps ahead from clamped (always in-bounds) addresses, so the loads are
 // unconditional and the steps straight-line code.  Per-point arithmetic, and
 // the restriction's summation order, are those of the unfused kernels.
-constexpr int WD = 4;     // rows of loads in flight per wave
+constexpr int WD = 8;     // rows of loads in flight per wave
 constexpr int WNMAX = 5;  // most fine columns (or rows) a coarse point gathers from (2·(h/H)⁻¹ + 1 for H ≈ 2h)
 
 // v of lane x − 1 (lane 0: 0) and of lane x + 1 (lane 63: 0), wave-wide.
@@ -978,7 +988,7 @@ __device__ __forceinline__ float lane_east(float v) {
 // compiler cannot tie the load to a branch and wait on it at once), puts 0 and
 // 1 outside the grid, as the LDS kernels stage them.
 struct WRow {
-	float b, kw, ke, ky;
+	float b, kw, ke, ky, wd;
 };
 template <bool VK>
 __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols, int gcc) {
@@ -990,8 +1000,10 @@ __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L,
 		v.kw = L.kx[o + (gcc > 0 ? gcc - 1 : 0)];
 		v.ke = L.kx[o + gcc];
 		v.ky = L.ky[o + gcc];
+		v.wd = L.wd[o + gcc];
 	} else {
 		v.kw = v.ke = v.ky = 1.0f;
+		v.wd = 0.0f;
 	}
 	return v;
 }
@@ -1004,8 +1016,10 @@ __device__ __forceinline__ WRow wfix(const WRow& v, int r, int rows, bool colin,
 		o.kw = rin && kwok ? v.kw : 1.0f;
 		o.ke = rin && keok ? v.ke : 1.0f;
 		o.ky = rin && colin && r < rows - 1 ? v.ky : 1.0f;
+		o.wd = v.wd;  // used on interior points only
 	} else {
 		o.kw = o.ke = o.ky = 1.0f;
+		o.wd = 0.0f;
 	}
 	return o;
 }
@@ -1097,9 +1111,9 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 			const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
 			return epi.load((int64_t)rc * cols + gcc);
 		};
-		float B[LZ + 1], KW[LZ + 1], KE[LZ + 1], KY[LZ + 2];
+		float B[LZ + 1], KW[LZ + 1], KE[LZ + 1], KY[LZ + 2], QW[LZ + 1];
 #pragma unroll
-		for (int i = 0; i <= LZ; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f;
+		for (int i = 0; i <= LZ; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f, QW[i] = 0.0f;
 #pragma unroll
 		for (int i = 0; i <= LZ + 1; ++i) KY[i] = 1.0f;
 		float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
@@ -1122,12 +1136,12 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				const int t = t0 + k;
 				// Row windows: index i holds row t − i.
 #pragma unroll
-				for (int i = LZ; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1];
+				for (int i = LZ; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
 #pragma unroll
 				for (int i = LZ + 1; i > 0; --i) KY[i] = KY[i - 1];
 				{
 					const WRow in = wfix<VK>(inb[k], t, rows, colin, kwok, keok);
-					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky;
+					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
 				}
 				const EV ev = evb[k];
 				// Faces of row t − i: west KW[i], east KE[i], north KY[i + 1], south KY[i].
@@ -1135,18 +1149,18 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				if constexpr (NPRE == 2) {
 					float u1 = 0.0f;
 					if (inter(t)) {
-						if constexpr (VK) u1 = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
 						else u1 = jac0(L, B[0]);
 					}
 					U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
 					const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
 					if (inter(t - 1)) {
-						if constexpr (VK) u = jackf(L, KW[1], KE[1], KY[2], KY[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
+						if constexpr (VK) u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
 						else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
 					}
 				} else {
 					if (inter(t)) {
-						if constexpr (VK) u = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
 						else u = jac0(L, B[0]);
 					}
 				}
@@ -1163,7 +1177,7 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 					const float wv = lane_west(U[1]), evv = lane_east(U[1]);
 					if (inter(t - LS)) {
 						if constexpr (VK)
-							s = jackf(L, KW[LS], KE[LS], KY[LS + 1], KY[LS], U[1], U[2], U[0], wv, evv, B[LS]);
+							s = jackw(L, KW[LS], KE[LS], KY[LS + 1], KY[LS], QW[LS], U[1], U[2], U[0], wv, evv, B[LS]);
 						else
 							s = jac(L, U[1], U[2], U[0], wv, evv, B[LS]);
 					}
@@ -1174,7 +1188,7 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				const float wv = lane_west(C[1]), evv = lane_east(C[1]);
 				const int rz = t - LZ;
 				if (inter(rz)) {
-					if constexpr (VK) z = jackf(L, KW[LZ], KE[LZ], KY[LZ + 1], KY[LZ], C[1], C[2], C[0], wv, evv, B[LZ]);
+					if constexpr (VK) z = jackw(L, KW[LZ], KE[LZ], KY[LZ + 1], KY[LZ], QW[LZ], C[1], C[2], C[0], wv, evv, B[LZ]);
 					else z = jac(L, C[1], C[2], C[0], wv, evv, B[LZ]);
 				}
 				if (outlane && rz >= S0 && rz < S1) {
@@ -1257,9 +1271,9 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 		}
 	}
 	const float scale = X.sy * X.sx;
-	float B[LR + 1], KW[LR + 1], KE[LR + 1], KY[LR + 2];
+	float B[LR + 1], KW[LR + 1], KE[LR + 1], KY[LR + 2], QW[LR + 1];
 #pragma unroll
-	for (int i = 0; i <= LR; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f;
+	for (int i = 0; i <= LR; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f, QW[i] = 0.0f;
 #pragma unroll
 	for (int i = 0; i <= LR + 1; ++i) KY[i] = 1.0f;
 	float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f};
@@ -1282,30 +1296,30 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 		for (int k = 0; k < WD; ++k) {
 			const int t = t0 + k;
 #pragma unroll
-			for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1];
+			for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
 #pragma unroll
 			for (int i = LR + 1; i > 0; --i) KY[i] = KY[i - 1];
 			{
 				const WRow in = wfix<VK>(inb[k], t, fr, colin, kwok, keok);
-				B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky;
+				B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
 			}
 			inb[k] = wload<VK>(b, L, t + WD, fr, fc, gcc);  // slot k refilled
 			float u = 0.0f;  // pre-smoothed iterate at row t − LU
 			if constexpr (NPRE == 2) {
 				float u1 = 0.0f;
 				if (inter(t)) {
-					if constexpr (VK) u1 = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+					if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
 					else u1 = jac0(L, B[0]);
 				}
 				U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
 				const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
 				if (inter(t - 1)) {
-					if constexpr (VK) u = jackf(L, KW[1], KE[1], KY[2], KY[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
+					if constexpr (VK) u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
 					else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
 				}
 			} else {
 				if (inter(t)) {
-					if constexpr (VK) u = jackf(L, KW[0], KE[0], KY[1], KY[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+					if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
 					else u = jac0(L, B[0]);
 				}
 			}
@@ -1350,6 +1364,301 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	emit(Icur + 1, ahi);
 }
 
+// ── k_ftail: the V-cycle below a small level, in one workgroup ─────────
+// The levels from the first one whose whole sub-hierarchy fits in LDS down to
+// the coarsest are cycled by one workgroup of TAIL_BLOCK threads, phase by
+// phase with workgroup barriers in between: pre-sweeps, residual, restriction
+// (fine-row sums over each coarse column's fine columns, then the weighted
+// sums of those over each coarse row's fine rows, the order of k_restrict),
+// the coarsest level's sweeps, prolongation and post-sweeps.  Every plane
+// (face coefficients, ω/d, b, iterates) and transfer table lives in LDS, so a
+// phase costs a few LDS round trips instead of a kernel launch.  Per-point
+// arithmetic is that of the other fused kernels, so the result is
+// bit-identical.
+constexpr int TAIL_MAXL = 6;
+constexpr int TAIL_BLOCK = 1024;
+constexpr size_t TAIL_LDS_MAX = 160 * 1024;
+struct TailLevel {
+	LWf L;
+	fx_xtab X;          // transfer to the next tail level (levels above the coarsest)
+	int rows, cols;
+	int off;            // offset of the level in each point plane
+	int tr, tc;         // offsets of its fine-row / fine-column tables (levels above the coarsest)
+	int tcr, tcc;       // offsets of the next level's coarse-row / coarse-column ranges
+};
+struct TailArgs {
+	TailLevel lv[TAIL_MAXL];
+	int nl;             // tail levels; lv[nl − 1] is the hierarchy's coarsest
+	int sweeps;         // coarsest-level sweeps from zero
+	int npts;           // points over all tail levels (plane length)
+	int nrs;            // row-sum scratch length: max over levels of rows · (next level's cols)
+	int ntab;           // table entries (fine rows + fine cols of levels above the coarsest)
+	int nrng;           // range entries (coarse rows + coarse cols of levels below the first)
+	const float* rhs0;  // rhs of the first tail level (global)
+	float* out0;        // result of the first tail level (global)
+};
+// LDS bytes of k_ftail for the given plane, scratch and table sizes.
+inline size_t tail_lds(bool vk, int npts, int nrs, int ntab, int nrng) {
+	return (size_t)(vk ? 6 : 3) * npts * sizeof(float) + (size_t)nrs * sizeof(float) +
+	       (size_t)ntab * (sizeof(int32_t) + 2 * sizeof(float)) + (size_t)nrng * 2 * sizeof(int32_t);
+}
+
+template <bool VK, int NPRE, int NPOST>
+__global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* skip) {
+	if (skip && *skip) return;
+	extern __shared__ float tsm[];
+	const int P = A.npts;
+	float* pb = tsm;            // b
+	float* pu = pb + P;         // iterate (pre-smoothed, then corrected)
+	float* pv = pu + P;         // work plane
+	float* kxp = pv + P;        // VK: faces and ω/d
+	float* kyp = kxp + (VK ? P : 0);
+	float* wdp = kyp + (VK ? P : 0);
+	float* rsum = VK ? wdp + P : kxp;  // restriction row sums
+	int32_t* ti0 = (int32_t*)(rsum + A.nrs);   // per fine row / column: coarse index below it
+	float* twl = (float*)(ti0 + A.ntab);       // its weights float(1 − w), float(w)
+	float* twh = twl + A.ntab;
+	int32_t* rlo = (int32_t*)(twh + A.ntab);   // per coarse row / column: fine range lo..hi
+	int32_t* rhi = rlo + A.nrng;
+	const int t = threadIdx.x;
+	// Stage the planes and tables.
+	for (int l = 0; l < A.nl; ++l) {
+		const TailLevel& T = A.lv[l];
+		const int n = T.rows * T.cols;
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			if constexpr (VK) {
+				kxp[T.off + q] = T.L.kx[q];
+				kyp[T.off + q] = T.L.ky[q];
+				wdp[T.off + q] = T.L.wd[q];
+			}
+			if (l == 0) pb[q] = A.rhs0[q];
+		}
+		if (l + 1 < A.nl) {
+			const TailLevel& C = A.lv[l + 1];
+			for (int q = t; q < T.rows; q += TAIL_BLOCK) {
+				const double w = T.X.wr[q];
+				ti0[T.tr + q] = T.X.i0r[q];
+				twl[T.tr + q] = float(1.0 - w);
+				twh[T.tr + q] = float(w);
+			}
+			for (int q = t; q < T.cols; q += TAIL_BLOCK) {
+				const double w = T.X.wc[q];
+				ti0[T.tc + q] = T.X.i0c[q];
+				twl[T.tc + q] = float(1.0 - w);
+				twh[T.tc + q] = float(w);
+			}
+			for (int q = t; q < C.rows; q += TAIL_BLOCK) rlo[T.tcr + q] = T.X.lor[q], rhi[T.tcr + q] = T.X.hir[q];
+			for (int q = t; q < C.cols; q += TAIL_BLOCK) rlo[T.tcc + q] = T.X.loc[q], rhi[T.tcc + q] = T.X.hic[q];
+		}
+	}
+	__syncthreads();
+	// One Jacobi point update of level T at q from plane `in` (interior points only).
+	auto jacq = [&](const TailLevel& T, const float* in, int q, float c) {
+		const int cols = T.cols;
+		const float* u = in + T.off;
+		if constexpr (VK) {
+			const int o = T.off + q;
+			return jackw(T.L, kxp[o - 1], kxp[o], kyp[o - cols], kyp[o], wdp[o], c, u[q - cols], u[q + cols], u[q - 1],
+			             u[q + 1], pb[o]);
+		} else {
+			return jac(T.L, c, u[q - cols], u[q + cols], u[q - 1], u[q + 1], pb[T.off + q]);
+		}
+	};
+	auto inner = [](const TailLevel& T, int q) {
+		const int r = q / T.cols, c = q % T.cols;
+		return r > 0 && c > 0 && r < T.rows - 1 && c < T.cols - 1;
+	};
+	// ── down: pre-smoothing, residual, restriction ──
+	for (int l = 0; l + 1 < A.nl; ++l) {
+		const TailLevel& T = A.lv[l];
+		const TailLevel& C = A.lv[l + 1];
+		const int n = T.rows * T.cols;
+		// First sweep from zero into pv (NPRE == 2) or pu (NPRE == 1).
+		float* first = NPRE == 2 ? pv : pu;
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			float v = 0.0f;
+			if (inner(T, q)) {
+				if constexpr (VK) {
+					const int o = T.off + q;
+					v = jackw(T.L, kxp[o - 1], kxp[o], kyp[o - T.cols], kyp[o], wdp[o], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, pb[o]);
+				} else {
+					v = jac0(T.L, pb[T.off + q]);
+				}
+			}
+			first[T.off + q] = v;
+		}
+		__syncthreads();
+		if constexpr (NPRE == 2) {
+			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = inner(T, q) ? jacq(T, pv, q, pv[T.off + q]) : 0.0f;
+			__syncthreads();
+		}
+		// Residual into pv.
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			float res = 0.0f;
+			if (inner(T, q)) {
+				const float* u = pu + T.off;
+				const int cols = T.cols;
+				float sv;
+				if constexpr (VK) {
+					const int o = T.off + q;
+					float d;
+					sv = vkff(T.L, kxp[o - 1], kxp[o], kyp[o - cols], kyp[o], u[q], u[q - cols], u[q + cols], u[q - 1],
+					          u[q + 1], &d);
+				} else {
+					sv = sop(T.L, u[q], u[q - cols], u[q + cols], u[q - 1], u[q + 1]);
+				}
+				res = fmaf(1.0f, pb[T.off + q], -sv);
+			}
+			pv[T.off + q] = res;
+		}
+		__syncthreads();
+		// Row pass: rsum[i·cc + J] = Σ_jj w(jj, J)·res(i, jj) over J's fine columns.
+		const int cr = C.rows, cc = C.cols;
+		for (int q = t; q < T.rows * cc; q += TAIL_BLOCK) {
+			const int i = q / cc, J = q % cc;
+			if (i < 1 || i > T.rows - 2 || J < 1 || J > cc - 2) continue;
+			const int lo = rlo[T.tcc + J], hi = rhi[T.tcc + J];
+			const float* row = pv + T.off + i * T.cols;
+			float rs = 0.0f;
+			for (int jj = lo; jj <= hi; ++jj) {
+				const int ac = ti0[T.tc + jj];
+				const float wxv = ac == J ? twl[T.tc + jj] : (ac + 1 == J ? twh[T.tc + jj] : 0.0f);
+				rs += wxv * row[jj];
+			}
+			rsum[q] = rs;
+		}
+		__syncthreads();
+		// Column pass: the coarse rhs.
+		const float scale = T.X.sy * T.X.sx;
+		for (int q = t; q < cr * cc; q += TAIL_BLOCK) {
+			const int I = q / cc, J = q % cc;
+			float outv = 0.0f;
+			if (I > 0 && I < cr - 1 && J > 0 && J < cc - 1) {
+				const int lo = rlo[T.tcr + I], hi = rhi[T.tcr + I];
+				float acc = 0.0f;
+				for (int ii = lo; ii <= hi; ++ii) {
+					const int ar = ti0[T.tr + ii];
+					const float wy = ar == I ? twl[T.tr + ii] : (ar + 1 == I ? twh[T.tr + ii] : 0.0f);
+					acc += wy * rsum[ii * cc + J];
+				}
+				outv = acc * scale;
+			}
+			pb[C.off + q] = outv;
+		}
+		__syncthreads();
+	}
+	// ── coarsest: sweeps from zero (k_fcoarse), result in pu ──
+	{
+		const TailLevel& T = A.lv[A.nl - 1];
+		const int n = T.rows * T.cols;
+		for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = 0.0f;
+		__syncthreads();
+		float* cur = pu;
+		float* nxt = pv;
+		for (int it = 0; it < A.sweeps; ++it) {
+			for (int q = t; q < n; q += TAIL_BLOCK) {
+				const float c = cur[T.off + q];
+				nxt[T.off + q] = inner(T, q) ? jacq(T, cur, q, c) : c;
+			}
+			__syncthreads();
+			float* x = cur;
+			cur = nxt;
+			nxt = x;
+		}
+		if (cur != pu) {
+			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = cur[T.off + q];
+			__syncthreads();
+		}
+	}
+	// ── up: prolongation and post-smoothing, each level's result in pu ──
+	for (int l = A.nl - 2; l >= 0; --l) {
+		const TailLevel& T = A.lv[l];
+		const TailLevel& C = A.lv[l + 1];
+		const int n = T.rows * T.cols;
+		const int cr = C.rows, cc = C.cols;
+		const float* e = pu + C.off;
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			if (!inner(T, q)) continue;
+			const int r = q / T.cols, c = q % T.cols;
+			const int a0 = ti0[T.tr + r], a1 = a0 + 1 < cr ? a0 + 1 : a0;
+			const int b0 = ti0[T.tc + c], b1 = b0 + 1 < cc ? b0 + 1 : b0;
+			const float wy = twh[T.tr + r], vy = 1.0f - wy;
+			const float wx = twh[T.tc + c], vx = 1.0f - wx;
+			const float top = vx * e[a0 * cc + b0] + wx * e[a0 * cc + b1];
+			const float bot = vx * e[a1 * cc + b0] + wx * e[a1 * cc + b1];
+			pu[T.off + q] += vy * top + wy * bot;
+		}
+		__syncthreads();
+		const float* last = pu;
+		if constexpr (NPOST == 2) {
+			for (int q = t; q < n; q += TAIL_BLOCK) {
+				const float c = pu[T.off + q];
+				pv[T.off + q] = inner(T, q) ? jacq(T, pu, q, c) : c;
+			}
+			__syncthreads();
+			last = pv;
+		}
+		float* dst = l == 0 ? A.out0 : pu + T.off;
+		if (NPOST == 1 && l > 0) dst = pv + T.off;
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			const float c = last[T.off + q];
+			dst[q] = inner(T, q) ? jacq(T, last, q, c) : c;
+		}
+		__syncthreads();
+		if (NPOST == 1 && l > 0) {
+			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = pv[T.off + q];
+			__syncthreads();
+		}
+	}
+}
+
+// First level of the tail kernel (≥ 1: level 0 carries the solver's reductions),
+// the smallest index whose sub-hierarchy fits TAIL_MAXL levels and the LDS;
+// nl − 1 when only the coarsest level does (then k_fcoarse runs alone).
+// Fills *A (except rhs0 / out0) for that level.
+inline int tail_plan(const fx_multigrid* mg, bool vk, TailArgs* A, size_t* lds) {
+	const auto& lv = mg->lv;
+	const int nl = (int)lv.size();
+	for (int lt = 1; lt < nl - 1; ++lt) {
+		if (nl - lt > TAIL_MAXL) continue;
+		int npts = 0, nrs = 0, ntab = 0, nrng = 0;
+		for (int l = lt; l < nl; ++l) {
+			npts += (int)(lv[l].g.rows * lv[l].g.cols);
+			if (l + 1 < nl) {
+				nrs = std::max<int>(nrs, (int)(lv[l].g.rows * lv[l + 1].g.cols));
+				ntab += (int)(lv[l].g.rows + lv[l].g.cols);
+				nrng += (int)(lv[l + 1].g.rows + lv[l + 1].g.cols);
+			}
+		}
+		const size_t b = tail_lds(vk, npts, nrs, ntab, nrng);
+		if (b > TAIL_LDS_MAX) continue;
+		A->nl = nl - lt;
+		A->sweeps = mg->opt.n_pre + mg->opt.n_post + mg->opt.extra_coarse;
+		A->npts = npts, A->nrs = nrs, A->ntab = ntab, A->nrng = nrng;
+		int off = 0, tab = 0, rng = 0;
+		for (int l = lt; l < nl; ++l) {
+			TailLevel& T = A->lv[l - lt];
+			T.L = lwf(mg, lv[l]);
+			T.rows = (int)lv[l].g.rows, T.cols = (int)lv[l].g.cols;
+			T.off = off;
+			off += T.rows * T.cols;
+			if (l + 1 < nl) {
+				T.X = lv[l].xt;
+				T.tr = tab, tab += T.rows;
+				T.tc = tab, tab += T.cols;
+				T.tcr = rng, rng += (int)lv[l + 1].g.rows;
+				T.tcc = rng, rng += (int)lv[l + 1].g.cols;
+			} else {
+				T.X = fx_xtab{};
+				T.tr = T.tc = T.tcr = T.tcc = 0;
+			}
+		}
+		*lds = b;
+		return lt;
+	}
+	return nl - 1;
+}
+
 // Segment heights of the wave-streaming kernels for a level: the tallest of
 // 256 … 8 fine rows (post) or 128 … 4 coarse rows (restrict) that still gives
 // WAVE_TARGET waves (eight per CU), unless a tuned height is set.
@@ -1390,7 +1699,11 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 	const int nl = (int)lv.size();
 	const auto& o = mg->opt;
 	auto tuned = [](const std::vector<int>& v, int l) { return l < (int)v.size() ? v[l] : 0; };
-	for (int l = 0; l < nl - 1; ++l) {
+	// Levels lt … nl − 1 run in the single-workgroup tail kernel (wave variant only).
+	TailArgs ta{};
+	size_t tlds = 0;
+	const int lt = mg->kernels == FX_MGK_WAVE ? tail_plan(mg, VK, &ta, &tlds) : nl - 1;
+	for (int l = 0; l < lt; ++l) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
 		if (l == 0) fx::probe_begin(be, FX_PROBE_VC_RESTRICT0);
@@ -1416,7 +1729,18 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		}
 		if (l == 0) fx::probe_end(be, FX_PROBE_VC_RESTRICT0);
 	}
-	{
+	if (lt < nl - 1) {
+		ta.rhs0 = (const float*)lv[lt].rhs;
+		ta.out0 = (float*)lv[lt].tmp;
+		static bool lds_set[2][3][3] = {};
+		bool& set = lds_set[VK][NPRE][NPOST];
+		if (!set) {
+			FX_HIP(hipFuncSetAttribute((const void*)k_ftail<VK, NPRE, NPOST>,
+			                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)TAIL_LDS_MAX));
+			set = true;
+		}
+		hipLaunchKernelGGL((k_ftail<VK, NPRE, NPOST>), dim3(1), dim3(TAIL_BLOCK), (unsigned)tlds, st, ta, skip);
+	} else {
 		const auto& L = lv[nl - 1];
 		hipLaunchKernelGGL(k_fcoarse, dim3(1), dim3(1024), 0, st, (const float*)L.rhs, (float*)L.tmp, lwf(mg, L),
 		                   o.n_pre + o.n_post + o.extra_coarse, (int)L.g.rows, (int)L.g.cols, skip);
@@ -1459,7 +1783,7 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 			                   (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols, epi, rs, skip);
 		}
 	};
-	for (int l = nl - 2; l >= 1; --l) post(l, NoEpi{}, RedScratch{nullptr}, post_shape(l));
+	for (int l = lt - 1; l >= 1; --l) post(l, NoEpi{}, RedScratch{nullptr}, post_shape(l));
 	const PostShape ps0 = post_shape(0);
 	RedScratch rs{nullptr};
 	if (Epi::NS > 0) {
@@ -1474,6 +1798,90 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		                   epi0.fin(), skip);
 	return fx::check_launch("fused V-cycle");
 }
+// Autotuner of the wave kernels' segment heights: for every level, each
+// candidate height is timed with HIP events over `reps` launches of the level's
+// kernel on the level's own buffers (the kernels' speed does not depend on the
+// values), and the fastest is kept in wrows_post / wrows_restrict.  *ms_out
+// (if given) receives the sum over levels of the chosen per-launch times.
+template <bool VK, int NPRE, int NPOST>
+fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
+	fx_backend* be = mg->be;
+	hipStream_t st = be->stream;
+	auto& lv = mg->lv;
+	const int nl = (int)lv.size();
+	mg->wrows_post.assign(nl, 0);
+	mg->wrows_restrict.assign(nl, 0);
+	hipEvent_t e0, e1;
+	FX_HIP(hipEventCreate(&e0));
+	FX_HIP(hipEventCreate(&e1));
+	double total = 0.0;
+	fx_status status = FX_OK;
+	auto time_it = [&](auto launch, float* ms) -> fx_status {
+		launch();
+		FX_HIP(hipEventRecord(e0, st));
+		for (int i = 0; i < reps; ++i) launch();
+		FX_HIP(hipEventRecord(e1, st));
+		FX_HIP(hipEventSynchronize(e1));
+		FX_HIP(hipEventElapsedTime(ms, e0, e1));
+		*ms /= (float)reps;
+		return fx::check_launch("multigrid autotune");
+	};
+	for (int l = 0; l + 1 < nl && status == FX_OK; ++l) {
+		const fx_grid& g = lv[l].g;
+		const fx_grid& cg = lv[l + 1].g;
+		const LWf lw = lwf(mg, lv[l]);
+		float best = INFINITY;
+		int pick = 0;
+		for (int sh = 8; sh <= 512 && status == FX_OK; sh *= 2) {
+			if (sh > 8 && sh / 2 >= g.rows) break;
+			const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
+			float ms = 0.0f;
+			status = time_it([&] {
+				hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, NoEpi>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
+				                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lw, lv[l].xt,
+				                   (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, sh, NoEpi{},
+				                   RedScratch{nullptr}, nullptr);
+			}, &ms);
+			if (ms < best) best = ms, pick = sh;
+		}
+		mg->wrows_post[l] = pick;
+		total += best;
+		if (!wrestrict_ok(lv[l].xt)) continue;
+		best = INFINITY;
+		pick = 0;
+		for (int scr = 4; scr <= 256 && status == FX_OK; scr *= 2) {
+			if (scr > 4 && scr / 2 >= cg.rows) break;
+			const int64_t ns = wave_strips_restrict(cg.cols), nseg = (cg.rows + scr - 1) / scr;
+			float ms = 0.0f;
+			status = time_it([&] {
+				hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
+				                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lw, lv[l].xt, (int)g.rows, (int)g.cols,
+				                   (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, scr, nullptr);
+			}, &ms);
+			if (ms < best) best = ms, pick = scr;
+		}
+		mg->wrows_restrict[l] = pick;
+		total += best;
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (ms_out) *ms_out = total;
+	return status;
+}
+inline fx_status autotune(fx_multigrid* mg, int reps, double* ms_out) {
+	const int a = mg->opt.n_pre, b = mg->opt.n_post;
+	const bool vk = mg->lv[0].k != nullptr;
+#define FMG_TUNE(A, B)                                                                       \
+	if (a == A && b == B) return vk ? autotune_t<true, A, B>(mg, reps, ms_out)               \
+	                                : autotune_t<false, A, B>(mg, reps, ms_out);
+	FMG_TUNE(2, 2)
+	FMG_TUNE(1, 1)
+	FMG_TUNE(1, 2)
+	FMG_TUNE(2, 1)
+#undef FMG_TUNE
+	return FX_ERR_UNSUPPORTED;
+}
+
 template <bool VK, class Epi>
 fx_status fast_cycle_v(fx_multigrid* mg, const int* skip, Epi epi0) {
 	const int a = mg->opt.n_pre, b = mg->opt.n_post;

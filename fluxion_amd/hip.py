@@ -392,6 +392,13 @@ class Multigrid:
         """Level kernels of the fused FP32 cycle (MgKernels)."""
         L.check(self._lib.fx_multigrid_set_kernels(self.h, int(kernels)), "multigrid_set_kernels")
 
+    def autotune(self, reps: int = 5) -> float:
+        """Picks each level's wave segment heights by timing them (fx_multigrid_autotune); returns the
+        summed per-launch time (ms) of the chosen shapes."""
+        ms = C.c_double()
+        L.check(self._lib.fx_multigrid_autotune(self.h, reps, C.byref(ms)), "multigrid_autotune")
+        return ms.value
+
     def set_wave_rows(self, level: int, post_rows: int, restrict_rows: int):
         """Wave segment heights of one level (0: built-in choice)."""
         L.check(self._lib.fx_multigrid_set_wave_rows(self.h, level, post_rows, restrict_rows),

@@ -230,6 +230,10 @@ fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b);
 typedef enum { FX_MGK_TILED = 0, FX_MGK_RING = 1, FX_MGK_WAVE = 2 } fx_mg_kernels;
 fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels);
 fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows);
+/* Autotuner (extension): times every level's wave kernels over candidate segment heights with HIP
+ * events (reps launches each, on the hierarchy's own buffers, whose contents it overwrites) and keeps
+ * the fastest per level; *ms_out (may be NULL) gets the summed per-launch time of the chosen shapes. */
+fx_status fx_multigrid_autotune(fx_multigrid* mg, int32_t reps, double* ms_out);
 
 /* Preconditioner handle (preconditioner.rs:10-37). */
 typedef enum { FX_PC_IDENTITY = 0, FX_PC_MULTIGRID = 1, FX_PC_LEARNED = 2 } fx_pc_kind;

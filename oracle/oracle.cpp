@@ -644,7 +644,7 @@ struct Mg {
 				if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) { xn[i] = xo[i]; continue; }
 				T d;
 				T au = vk_apply<T>(xo, L.k.data(), i, cols, ix2, iy2, sg, sc, &d);
-				xn[i] = fmaT<T>(om, (L.rhs[i] - au) / d, xo[i]);
+				xn[i] = fmaT<T>(om / d, L.rhs[i] - au, xo[i]);
 			}
 	}
 

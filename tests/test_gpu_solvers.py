@@ -413,6 +413,8 @@ def test_fused_vcycle_kernel_variants(be, shape, vk):
         assert np.array_equal(run(), ref), (post_rows, restrict_rows)
     with pytest.raises(Exception):
         mg.set_wave_rows(mg.depth() - 1, 8, 4)   # the coarsest level has no level kernels
+    assert mg.autotune(2) > 0.0                  # the autotuner's picks give the same bits
+    assert np.array_equal(run(), ref)
 
 
 @pytest.mark.parametrize("n", [256, 255])
