@@ -52,10 +52,10 @@ def kernel_bytes(vk: bool) -> dict:
         # z (FP32) and p_old read, p_new and A·p written (FP64); k read for the var-k operator.
         "fcg_step": 4 + 8 + 8 + 8 + (8 if vk else 0),
         # level-0 rhs (FP32), coarse correction (¼ point, FP32) and z (FP32); r, A·p (FP64) for the dots;
-        # the level's FP32 face coefficients kx, ky for the var-k operator.
-        "vc_post0": 4 + 1 + 4 + 8 + 8 + (8 if vk else 0),
-        # level-0 rhs (FP32) and face coefficients read, coarse rhs (¼ point, FP32) written.
-        "vc_restrict0": 4 + 1 + (8 if vk else 0),
+        # the level's FP32 face coefficients kx, ky and ω/d for the var-k operator.
+        "vc_post0": 4 + 1 + 4 + 8 + 8 + (12 if vk else 0),
+        # level-0 rhs (FP32), face coefficients and ω/d read, coarse rhs (¼ point, FP32) written.
+        "vc_restrict0": 4 + 1 + (12 if vk else 0),
         "stencil_axpy": 24,
     }
 
@@ -200,8 +200,9 @@ def main():
         if cnt == 0:
             continue
         us = 1e3 * ms / cnt
-        gbs = kb[name] * n * n / (us * 1e-6) / 1e9
-        kernels[name] = {"us_per_launch": us, "launches": cnt, "bytes_per_point": kb[name], "GBs": gbs,
+        bpp = kb[name]
+        gbs = bpp * n * n / (us * 1e-6) / 1e9
+        kernels[name] = {"us_per_launch": us, "launches": cnt, "bytes_per_point": bpp, "GBs": gbs,
                          "frac": gbs / HBM_PEAK_GBS, "ms_per_solve": ms / (args.steps if name == "fcg_update" else 2)}
         log(f"bench: kernel {name}: {us:.1f} us/launch, {gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.0f}%)")
     dom = max(kernels, key=lambda m: kernels[m]["ms_per_solve"])

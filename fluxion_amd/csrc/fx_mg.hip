@@ -491,6 +491,14 @@ fx_status build_xtab(fx_backend* be, fx_multigrid::Level& L, const fx_grid& c) {
 	X.rx = (int)span(cc, fmg::RCX, loc, hic) + 4;
 	X.gmax = 0;
 	for (int64_t J = 1; J + 1 < cc; ++J) X.gmax = std::max<int>(X.gmax, hic[J] - loc[J] + 1);
+	// Fine window (two columns of halo each side) of the wave restriction's strips of RCX interior
+	// coarse columns.
+	int64_t wspan = 1;
+	for (int64_t Ja = 1; Ja + 1 < cc; Ja += fmg::RCX) {
+		const int64_t Jb = std::min<int64_t>(Ja + fmg::RCX - 1, cc - 2);
+		if (hic[Jb] >= loc[Ja]) wspan = std::max<int64_t>(wspan, hic[Jb] - loc[Ja] + 1);
+	}
+	X.wx = (int)wspan + 4;
 	return FX_OK;
 }
 

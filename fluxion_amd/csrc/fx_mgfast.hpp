@@ -970,7 +970,11 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 // WD steps ahead from clamped (always in-bounds) addresses, so the loads are
 // unconditional and the steps straight-line code.  Per-point arithmetic, and
 // the restriction's summation order, are those of the unfused kernels.
-constexpr int WD = 8;     // rows of loads in flight per wave
+// WD (a template parameter of the kernels): rows of loads in flight per wave,
+// 8 on large levels, 4 on small ones, whose short segments would otherwise be
+// padded to a multiple of 8 steps.
+constexpr int WD_BIG = 8, WD_SMALL = 4;
+constexpr int64_t WD_BIG_POINTS = 1 << 20;   // levels with at least this many points use WD_BIG
 constexpr int WNMAX = 5;  // most fine columns (or rows) a coarse point gathers from (2·(h/H)⁻¹ + 1 for H ≈ 2h)
 
 // v of lane x − 1 (lane 0: 0) and of lane x + 1 (lane 63: 0), wave-wide.
@@ -1049,13 +1053,14 @@ struct LinRow {
 
 // Waves of a wave-streaming launch over a level: strips × segments.
 inline int64_t wave_strips_post(int64_t cols) { return (cols + SSW - 1) / SSW; }
-inline int64_t wave_strips_restrict(int64_t cc) { return (cc + RCX - 1) / RCX; }
+inline int64_t wave_strips_restrict(int64_t cc) { return (cc - 2 + RCX - 1) / RCX; }  // interior coarse columns
+inline int64_t wave_segs_restrict(int64_t cr, int scr) { return (cr - 2 + scr - 1) / scr; }   // interior coarse rows
 inline unsigned wave_blocks(int64_t nwaves) { return (unsigned)((nwaves + NWAVES - 1) / NWAVES); }
 
 // out = J^NPOST(J^NPRE(0; b) + P·e) over the level (k_fpost's result), with Epi's
 // per-point reduction terms.  Wave gw takes strip gw % nstrips and row segment
 // gw / nstrips of sh rows.
-template <bool VK, int NPRE, int NPOST, class Epi>
+template <bool VK, int NPRE, int NPOST, class Epi, int WD>
 __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, const float* __restrict__ ec,
                                                  float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
                                                  int cc, int nstrips, int nseg, int sh, Epi epi, RedScratch rs,
@@ -1206,14 +1211,15 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 }
 
 // coarse rhs = R(b − A·J^npre(0; b)) over the level (k_frestrict's result).
-// Wave gw takes RCX coarse columns (strip gw % nstrips; lane l < RCX owns
-// column J0 + l) and scr coarse rows (segment gw / nstrips); its 64 lanes hold
-// the fine window from two columns left of the strip's first gathered fine
-// column.  Each fine row's residual is summed over the coarse column's fine
-// columns in ascending order (cross-lane permutes), then added, in ascending
-// fine-row order, to the (at most two) coarse rows the fine row contributes
-// to: the order of k_restrict's loops.
-template <bool VK, int NPRE>
+// Wave gw takes RCX interior coarse columns Ja … Jb (strip gw % nstrips; lane
+// l < RCX owns column Ja + l) and scr interior coarse rows Ia … Ib (segment
+// gw / nstrips); the first and last strips and segments also write the zero
+// boundary.  Its 64 lanes hold the fine window from two columns left of the
+// strip's first gathered fine column.  Each fine row's residual is summed over
+// the coarse column's fine columns in ascending order (cross-lane permutes),
+// then added, in ascending fine-row order, to the (at most two) coarse rows the
+// fine row contributes to: the order of k_restrict's loops.
+template <bool VK, int NPRE, int WD>
 __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
                                                      fx_xtab X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
                                                      int scr, const int* skip) {
@@ -1222,30 +1228,32 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
 	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
-	if (gw >= nstrips * nseg) return;  // no workgroup-wide operation follows
-	const int strip = gw % nstrips, seg = gw / nstrips;
-	const int J0 = strip * RCX, I0 = seg * scr;
-	const int Jend = J0 + RCX < cc ? J0 + RCX : cc, Iend = I0 + scr < cr ? I0 + scr : cr;
-	const int J = J0 + x;
-	const bool own = x < RCX && J < Jend;
-	// Boundary coarse rows and columns are zero.
-	if (own) {
-		if (I0 == 0) bco[J] = 0.0f;
-		if (Iend == cr) bco[(size_t)(cr - 1) * cc + J] = 0.0f;
-		if (J == 0 || J == cc - 1)
-			for (int I = I0; I < Iend; ++I) bco[(size_t)I * cc + J] = 0.0f;
+	bool active = gw < nstrips * nseg;
+	const int strip = active ? gw % nstrips : 0, seg = active ? gw / nstrips : 0;
+	const bool sfirst = strip == 0, slast = strip == nstrips - 1, gfirst = seg == 0, glast = seg == nseg - 1;
+	const int Ja = 1 + strip * RCX, Jb = Ja + RCX - 1 < cc - 2 ? Ja + RCX - 1 : cc - 2;
+	const int Ia = 1 + seg * scr, Ib = Ia + scr - 1 < cr - 2 ? Ia + scr - 1 : cr - 2;
+	const int J = Ja + x;
+	const bool jint = x < RCX && J <= Jb;  // the lane's interior coarse column
+	if (active) {
+		// Zero boundary rows (first / last segment) and columns (first / last strip).
+		if (gfirst || glast) {
+			const size_t row = gfirst ? 0 : (size_t)(cr - 1) * cc;
+			if (jint) bco[row + J] = 0.0f;
+			if (x == 0 && sfirst) bco[row] = 0.0f;
+			if (x == 0 && slast) bco[row + cc - 1] = 0.0f;
+			if (gfirst && glast && jint) bco[(size_t)(cr - 1) * cc + J] = 0.0f;
+			if (gfirst && glast && x == 0 && sfirst) bco[(size_t)(cr - 1) * cc] = 0.0f;
+			if (gfirst && glast && x == 0 && slast) bco[(size_t)(cr - 1) * cc + cc - 1] = 0.0f;
+		}
+		if ((sfirst || slast) && x < 2) {
+			const bool left = x == 0;
+			if ((left && sfirst) || (!left && slast))
+				for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + (left ? 0 : cc - 1)] = 0.0f;
+		}
 	}
-	const int Ia = I0 > 1 ? I0 : 1, Ib = Iend - 1 < cr - 2 ? Iend - 1 : cr - 2;
-	const int Ja = J0 > 1 ? J0 : 1, Jb = Jend - 1 < cc - 2 ? Jend - 1 : cc - 2;
-	if (Ia > Ib || Ja > Jb) return;
 	const int G0 = X.loc[Ja];
 	const int F0 = X.lor[Ia], F1 = X.hir[Ib];
-	const bool jint = own && J >= 1 && J <= cc - 2;
-	if (G0 > X.hic[Jb] || F0 > F1) {  // nothing to gather: the strip's interior rows are zero
-		if (jint)
-			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + J] = 0.0f;
-		return;
-	}
 	const int C0 = G0 - 2;  // fine column of lane 0
 	const int gc = C0 + x;
 	const int gcc = gc < 0 ? 0 : (gc < fc ? gc : fc - 1);
@@ -1276,7 +1284,7 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	for (int i = 0; i <= LR; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f, QW[i] = 0.0f;
 #pragma unroll
 	for (int i = 0; i <= LR + 1; ++i) KY[i] = 1.0f;
-	float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f};
+	float U1[3] = {0.0f, 0.0f, 0.0f}, Uv[3] = {0.0f, 0.0f, 0.0f};
 	int Icur = X.i0r[F0];  // coarse row accumulated in alo (ahi: Icur + 1)
 	float alo = 0.0f, ahi = 0.0f;
 	auto emit = [&](int I, float v) {
@@ -1288,80 +1296,89 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	const double rinv = 1.0 / double(fr - 1);
 	LinRow lr;  // fine row whose residual the step produces
 	lr.init(T0 - LR, fr, cr);
-	WRow inb[WD];
+	if (G0 > X.hic[Jb] || F0 > F1) {  // nothing to gather (tiny grids): the strip's interior rows are zero
+		if (active && jint)
+			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + J] = 0.0f;
+		active = false;
+	}
+	if (active) {
+		WRow inb[WD];
 #pragma unroll
-	for (int k = 0; k < WD; ++k) inb[k] = wload<VK>(b, L, T0 + k, fr, fc, gcc);
-	for (int t0 = T0; t0 < T1; t0 += WD) {
+		for (int k = 0; k < WD; ++k) inb[k] = wload<VK>(b, L, T0 + k, fr, fc, gcc);
+		for (int t0 = T0; t0 < T1; t0 += WD) {
 #pragma unroll
-		for (int k = 0; k < WD; ++k) {
-			const int t = t0 + k;
+			for (int k = 0; k < WD; ++k) {
+				const int t = t0 + k;
 #pragma unroll
-			for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
+				for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
 #pragma unroll
-			for (int i = LR + 1; i > 0; --i) KY[i] = KY[i - 1];
-			{
-				const WRow in = wfix<VK>(inb[k], t, fr, colin, kwok, keok);
-				B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
-			}
-			inb[k] = wload<VK>(b, L, t + WD, fr, fc, gcc);  // slot k refilled
-			float u = 0.0f;  // pre-smoothed iterate at row t − LU
-			if constexpr (NPRE == 2) {
-				float u1 = 0.0f;
-				if (inter(t)) {
-					if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-					else u1 = jac0(L, B[0]);
+				for (int i = LR + 1; i > 0; --i) KY[i] = KY[i - 1];
+				{
+					const WRow in = wfix<VK>(inb[k], t, fr, colin, kwok, keok);
+					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
 				}
-				U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
-				const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
-				if (inter(t - 1)) {
-					if constexpr (VK) u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
-					else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
-				}
-			} else {
-				if (inter(t)) {
-					if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-					else u = jac0(L, B[0]);
-				}
-			}
-			U[2] = U[1], U[1] = U[0], U[0] = u;
-			// Residual at fine row i = t − LR.
-			const int i = t - LR;
-			const float wv = lane_west(U[1]), evv = lane_east(U[1]);
-			float res = 0.0f;
-			if (inter(i)) {
-				float sv;
-				if constexpr (VK) {
-					float d;
-					sv = vkff(L, KW[LR], KE[LR], KY[LR + 1], KY[LR], U[1], U[2], U[0], wv, evv, &d);
+				inb[k] = wload<VK>(b, L, t + WD, fr, fc, gcc);  // slot k refilled
+				float u = 0.0f;  // pre-smoothed iterate at row t − LU
+				if constexpr (NPRE == 2) {
+					float u1 = 0.0f;
+					if (inter(t)) {
+						if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						else u1 = jac0(L, B[0]);
+					}
+					U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
+					const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
+					if (inter(t - 1)) {
+						if constexpr (VK)
+							u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
+						else
+							u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
+					}
 				} else {
-					sv = sop(L, U[1], U[2], U[0], wv, evv);
+					if (inter(t)) {
+						if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
+						else u = jac0(L, B[0]);
+					}
 				}
-				res = fmaf(1.0f, B[LR], -sv);
-			}
-			// Row sum over the owned coarse column's fine columns.
-			float row = 0.0f;
+				Uv[2] = Uv[1], Uv[1] = Uv[0], Uv[0] = u;
+				// Residual at fine row i = t − LR.
+				const int i = t - LR;
+				const float wv = lane_west(Uv[1]), evv = lane_east(Uv[1]);
+				float res = 0.0f;
+				if (inter(i)) {
+					float sv;
+					if constexpr (VK) {
+						float d;
+						sv = vkff(L, KW[LR], KE[LR], KY[LR + 1], KY[LR], Uv[1], Uv[2], Uv[0], wv, evv, &d);
+					} else {
+						sv = sop(L, Uv[1], Uv[2], Uv[0], wv, evv);
+					}
+					res = fmaf(1.0f, B[LR], -sv);
+				}
+				// Row sum over the owned coarse column's fine columns.
+				float row = 0.0f;
 #pragma unroll
-			for (int k2 = 0; k2 < WNMAX; ++k2) {
-				const float v = __shfl(res, sk[k2]);
-				if (k2 < cnt) row += wk[k2] * v;
-			}
-			const int a = lr.i0;
-			const double wrv = lr.w(rinv);
-			lr.next();
-			if (i >= F0 && i <= F1) {
-				while (Icur < a) {
-					emit(Icur, alo);
-					alo = ahi;
-					ahi = 0.0f;
-					++Icur;
+				for (int k2 = 0; k2 < WNMAX; ++k2) {
+					const float v = __shfl(res, sk[k2]);
+					if (k2 < cnt) row += wk[k2] * v;
 				}
-				alo += float(1.0 - wrv) * row;
-				ahi += float(wrv) * row;
+				const int a = lr.i0;
+				const double wrv = lr.w(rinv);
+				lr.next();
+				if (i >= F0 && i <= F1) {
+					while (Icur < a) {
+						emit(Icur, alo);
+						alo = ahi;
+						ahi = 0.0f;
+						++Icur;
+					}
+					alo += float(1.0 - wrv) * row;
+					ahi += float(wrv) * row;
+				}
 			}
 		}
+		emit(Icur, alo);
+		emit(Icur + 1, ahi);
 	}
-	emit(Icur, alo);
-	emit(Icur + 1, ahi);
 }
 
 // ── k_ftail: the V-cycle below a small level, in one workgroup ─────────
@@ -1659,6 +1676,29 @@ inline int tail_plan(const fx_multigrid* mg, bool vk, TailArgs* A, size_t* lds) 
 	return nl - 1;
 }
 
+// Launches of the wave kernels with the level's prefetch depth.
+template <bool VK, int NPRE, int NPOST, class Epi>
+void launch_wpost(hipStream_t st, unsigned blocks, const float* b, const float* ec, float* out, const LWf& L,
+                  const fx_xtab& X, int rows, int cols, int cr, int cc, int ns, int nseg, int sh, Epi epi, RedScratch rs,
+                  const int* skip) {
+	if ((int64_t)rows * cols >= WD_BIG_POINTS)
+		hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi, WD_BIG>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X,
+		                   rows, cols, cr, cc, ns, nseg, sh, epi, rs, skip);
+	else
+		hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi, WD_SMALL>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X,
+		                   rows, cols, cr, cc, ns, nseg, sh, epi, rs, skip);
+}
+template <bool VK, int NPRE>
+void launch_wrestrict(hipStream_t st, unsigned blocks, const float* b, float* bco, const LWf& L, const fx_xtab& X,
+                      int fr, int fc, int cr, int cc, int ns, int nseg, int scr, const int* skip) {
+	if ((int64_t)fr * fc >= WD_BIG_POINTS)
+		hipLaunchKernelGGL((k_wrestrict<VK, NPRE, WD_BIG>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr,
+		                   cc, ns, nseg, scr, skip);
+	else
+		hipLaunchKernelGGL((k_wrestrict<VK, NPRE, WD_SMALL>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr,
+		                   cc, ns, nseg, scr, skip);
+}
+
 // Segment heights of the wave-streaming kernels for a level: the tallest of
 // 256 … 8 fine rows (post) or 128 … 4 coarse rows (restrict) that still gives
 // WAVE_TARGET waves (eight per CU), unless a tuned height is set.
@@ -1672,12 +1712,12 @@ inline int wpost_rows(int64_t rows, int64_t cols) {
 inline int wrestrict_rows(int64_t cr, int64_t cc) {
 	const int64_t ns = wave_strips_restrict(cc);
 	int s = 128;
-	while (s > 4 && ns * ((cr + s - 1) / s) < WAVE_TARGET) s /= 2;
+	while (s > 4 && ns * wave_segs_restrict(cr, s) < WAVE_TARGET) s /= 2;
 	return s;
 }
 // The wave restriction needs each strip's fine window (two columns of halo
 // each side) within one wave and at most WNMAX fine columns per coarse column.
-inline bool wrestrict_ok(const fx_xtab& X) { return X.rx <= SRW && X.gmax <= WNMAX; }
+inline bool wrestrict_ok(const fx_xtab& X) { return X.wx <= SRW && X.gmax <= WNMAX; }
 
 // Whether the fused cycle can run this hierarchy.
 inline bool fused_ok(const fx_multigrid* mg) {
@@ -1710,10 +1750,10 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt)) {
 			int scr = tuned(mg->wrows_restrict, l);
 			if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
-			const int64_t ns = wave_strips_restrict(cg.cols), nseg = (cg.rows + scr - 1) / scr;
-			hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
-			                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lwf(mg, lv[l]), lv[l].xt, (int)g.rows,
-			                   (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, scr, skip);
+			const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
+			launch_wrestrict<VK, NPRE>(st, wave_blocks(ns * nseg), (const float*)lv[l].rhs, (float*)lv[l + 1].rhs,
+			                           lwf(mg, lv[l]), lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols,
+			                           (int)ns, (int)nseg, scr, skip);
 		} else {
 			const int scr = mg->kernels == FX_MGK_RING ? srestrict_height(lv[l].xt, cg.rows, cg.cols) : 0;
 			if (scr > 0)
@@ -1768,10 +1808,9 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
 		if (ps.kind == FX_MGK_WAVE) {
-			hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, decltype(epi)>), dim3(ps.blocks), dim3(BLOCK), 0, st,
-			                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
-			                   lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, ps.ns, ps.nseg, ps.sh, epi,
-			                   rs, skip);
+			launch_wpost<VK, NPRE, NPOST>(st, ps.blocks, (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp,
+			                              (float*)lv[l].tmp, lwf(mg, lv[l]), lv[l].xt, (int)g.rows, (int)g.cols,
+			                              (int)cg.rows, (int)cg.cols, ps.ns, ps.nseg, ps.sh, epi, rs, skip);
 		} else if (ps.kind == FX_MGK_RING) {
 			hipLaunchKernelGGL((k_spost<VK, NPRE, NPOST, decltype(epi)>), dim3(ps.blocks), dim3(BLOCK), 0, st,
 			                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lwf(mg, lv[l]),
@@ -1832,15 +1871,15 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 		const LWf lw = lwf(mg, lv[l]);
 		float best = INFINITY;
 		int pick = 0;
-		for (int sh = 8; sh <= 512 && status == FX_OK; sh *= 2) {
-			if (sh > 8 && sh / 2 >= g.rows) break;
+		for (int sh = 2; sh <= 512 && status == FX_OK; sh *= 2) {
+			if (sh > 2 && sh / 2 >= g.rows) break;
 			const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
 			float ms = 0.0f;
 			status = time_it([&] {
-				hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, NoEpi>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
-				                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lw, lv[l].xt,
-				                   (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, sh, NoEpi{},
-				                   RedScratch{nullptr}, nullptr);
+				launch_wpost<VK, NPRE, NPOST>(st, wave_blocks(ns * nseg), (const float*)lv[l].rhs,
+				                              (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lw, lv[l].xt, (int)g.rows,
+				                              (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, sh, NoEpi{},
+				                              RedScratch{nullptr}, nullptr);
 			}, &ms);
 			if (ms < best) best = ms, pick = sh;
 		}
@@ -1849,14 +1888,14 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 		if (!wrestrict_ok(lv[l].xt)) continue;
 		best = INFINITY;
 		pick = 0;
-		for (int scr = 4; scr <= 256 && status == FX_OK; scr *= 2) {
-			if (scr > 4 && scr / 2 >= cg.rows) break;
-			const int64_t ns = wave_strips_restrict(cg.cols), nseg = (cg.rows + scr - 1) / scr;
+		for (int scr = 1; scr <= 256 && status == FX_OK; scr *= 2) {
+			if (scr > 1 && scr / 2 >= cg.rows - 2) break;
+			const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
 			float ms = 0.0f;
 			status = time_it([&] {
-				hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(wave_blocks(ns * nseg)), dim3(BLOCK), 0, st,
-				                   (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lw, lv[l].xt, (int)g.rows, (int)g.cols,
-				                   (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg, scr, nullptr);
+				launch_wrestrict<VK, NPRE>(st, wave_blocks(ns * nseg), (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lw,
+				                           lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns,
+				                           (int)nseg, scr, nullptr);
 			}, &ms);
 			if (ms < best) best = ms, pick = scr;
 		}

@@ -56,13 +56,20 @@ struct SState {
 	// ‖r‖² fails to drop below stall2·(previous ‖r‖²).
 	int32_t lmode, lplain, lskip, pad3;
 	double stall2;
+	// Host-visible copy of `done` (coherent pinned memory), set together with it so that the host's
+	// enqueue loop can stop without a per-iteration device-to-host copy.
+	int* hdone;
 };
 
 __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
 	if (s->hist && k >= 0 && k < s->hist_len) s->hist[k] = v;
 }
-__device__ __forceinline__ void finish(SState* s, int64_t its, double res, int reason) {
+__device__ __forceinline__ void set_done(SState* s) {
 	s->done = 1;
+	if (s->hdone) __hip_atomic_store(s->hdone, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void finish(SState* s, int64_t its, double res, int reason) {
+	set_done(s);
 	s->skip = 1;
 	s->lskip = 1;
 	s->reason = reason;
@@ -297,12 +304,26 @@ struct EpiCGv {
 	}
 };
 
+// v of lane x − 1 (ctrl 0x138, wave_shr:1) or x + 1 (0x130, wave_shl:1) for a
+// double, as two 32-bit DPP moves; 0 past the wave.
+template <int CTRL>
+__device__ __forceinline__ double dlane(double v) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // FCG step of the fused loop on grids with an even column count, two columns
 // per thread (16-byte loads and stores): p = double(z) + β·p_old, ap = A·p
 // (identity boundary rows), p·ap.  A is the 5-point stencil operator
 // p + coeff·S(p) (chain order of fxd::chain5) or, with VK, the
-// variable-coefficient operator of k_stream_vk.  Per-point arithmetic equals
-// k_stream_st / k_stream_vk.
+// variable-coefficient operator of k_stream_vk.  The west and east neighbours
+// of a thread's pair come from the adjacent lanes (DPP); only the first and
+// last lane of a wave read the column beyond the wave, and do so together with
+// the row's other loads.  Every lane walks the rows (addresses clamped into
+// the grid), so the cross-lane moves always see all 64 lanes.  Per-point
+// arithmetic equals k_stream_st / k_stream_vk.
 template <bool VK>
 __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z, const double* __restrict__ pold,
                                                      double* __restrict__ pnew, double* __restrict__ ap, W9 w,
@@ -313,67 +334,78 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 	const double be = st->beta;
 	int bx, by;
 	xcd_tile(&bx, &by);
+	const int lane = (int)(threadIdx.x & (WAVE - 1));
 	const int64_t c0 = 2 * ((int64_t)bx * BLOCK + threadIdx.x);
 	const int64_t r0 = (int64_t)by * rpb;
 	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
 	const bool ok = c0 < cols;
+	const int64_t cl = ok ? c0 : cols - 2;   // clamped pair column
 	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;   // column interior tests (c0 + 1 > 0 always)
-	auto P2 = [&](int64_t i) {
+	const bool edgeW = lane == 0 && ok && c0 > 0, edgeE = lane == WAVE - 1 && ok && c0 + 2 < cols;
+	struct Row {
+		double2 p, k;
+		double pw, pe, kw, ke;   // p and k west of the pair (first lane) and east of it (last lane)
+	};
+	auto ld = [&](int64_t r) {
+		Row v;
+		const int64_t i = r * cols + cl;
 		const float2 zz = *(const float2*)(z + i);
 		const double2 pp = *(const double2*)(pold + i);
-		return double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
+		v.p = double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
+		v.k = VK ? *(const double2*)(k + i) : double2{1.0, 1.0};
+		v.pw = v.pe = 0.0;
+		v.kw = v.ke = 1.0;
+		if (edgeW) {
+			v.pw = double(z[i - 1]) + be * pold[i - 1];
+			if constexpr (VK) v.kw = k[i - 1];
+		}
+		if (edgeE) {
+			v.pe = double(z[i + 2]) + be * pold[i + 2];
+			if constexpr (VK) v.ke = k[i + 2];
+		}
+		return v;
 	};
-	auto P1 = [&](int64_t i) { return double(z[i]) + be * pold[i]; };
-	auto K2 = [&](int64_t i) { return *(const double2*)(k + i); };
+	const Row none{double2{0.0, 0.0}, double2{1.0, 1.0}, 0.0, 0.0, 1.0, 1.0};
 	Acc acc[1] = {Acc{0.0, 0.0}};
-	double2 up{0.0, 0.0}, cur{0.0, 0.0}, kup{1.0, 1.0}, kcur{1.0, 1.0};
-	if (ok) {
-		if (r0 > 0) {
-			up = P2((r0 - 1) * cols + c0);
-			if constexpr (VK) kup = K2((r0 - 1) * cols + c0);
-		}
-		cur = P2(r0 * cols + c0);
-		if constexpr (VK) kcur = K2(r0 * cols + c0);
-	}
-	for (int64_t r = r0; r < r1 && ok; ++r) {
+	Row up = r0 > 0 ? ld(r0 - 1) : none;
+	Row cur = ld(r0 < rows ? r0 : rows - 1);
+	for (int64_t r = r0; r < r1; ++r) {
 		const int64_t i = r * cols + c0;
-		double2 dn{0.0, 0.0}, kdn{1.0, 1.0};
-		if (r + 1 < rows) {
-			dn = P2(i + cols);
-			if constexpr (VK) kdn = K2(i + cols);
-		}
+		const Row dn = r + 1 < rows ? ld(r + 1) : none;
+		double pw = dlane<0x138>(cur.p.y), pe = dlane<0x130>(cur.p.x);
+		double kw = dlane<0x138>(cur.k.y), ke = dlane<0x130>(cur.k.x);
+		if (lane == 0) pw = cur.pw, kw = cur.kw;
+		if (lane == WAVE - 1) pe = cur.pe, ke = cur.ke;
 		const bool rint = r > 0 && r < rows - 1;
-		double2 av = cur;   // boundary rows and columns: identity
+		double2 av = cur.p;   // boundary rows and columns: identity
 		if (rint) {
-			const double wv = intA ? P1(i - 1) : 0.0;
-			const double ev = intB ? P1(i + 2) : 0.0;
+			const double wv = intA ? pw : 0.0;
+			const double ev = intB ? pe : 0.0;
 			if constexpr (VK) {
-				auto op = [&](double kc, double kn, double ks, double kw, double ke, double c, double n, double s_,
+				auto op = [&](double kc, double kn, double ks, double kwv, double kev, double c, double n, double s_,
 				              double w_, double e_) {
-					const double hw = hmean_d(kc, kw), he = hmean_d(kc, ke);
+					const double hw = hmean_d(kc, kwv), he = hmean_d(kc, kev);
 					const double hn = hmean_d(kc, kn), hs = hmean_d(kc, ks);
 					const double fx = he * (e_ - c) - hw * (c - w_);
 					const double fy = hs * (s_ - c) - hn * (c - n);
 					const double L = -(fx * ix2 + fy * iy2);
 					return fma(coeff, L, sigma * c);
 				};
-				if (intA) av.x = op(kcur.x, kup.x, kdn.x, k[i - 1], kcur.y, cur.x, up.x, dn.x, wv, cur.y);
-				if (intB) av.y = op(kcur.y, kup.y, kdn.y, kcur.x, k[i + 2], cur.y, up.y, dn.y, cur.x, ev);
+				if (intA) av.x = op(cur.k.x, up.k.x, dn.k.x, kw, cur.k.y, cur.p.x, up.p.x, dn.p.x, wv, cur.p.y);
+				if (intB) av.y = op(cur.k.y, up.k.y, dn.k.y, cur.k.x, ke, cur.p.y, up.p.y, dn.p.y, cur.p.x, ev);
 			} else {
-				if (intA) av.x = fma(coeff, chain5(w, cur.x, up.x, dn.x, wv, cur.y), cur.x);
-				if (intB) av.y = fma(coeff, chain5(w, cur.y, up.y, dn.y, cur.x, ev), cur.y);
+				if (intA) av.x = fma(coeff, chain5(w, cur.p.x, up.p.x, dn.p.x, wv, cur.p.y), cur.p.x);
+				if (intB) av.y = fma(coeff, chain5(w, cur.p.y, up.p.y, dn.p.y, cur.p.x, ev), cur.p.y);
 			}
 		}
-		*(double2*)(pnew + i) = cur;
-		*(double2*)(ap + i) = av;
-		acc_add(acc[0], cur.x * av.x);
-		acc_add(acc[0], cur.y * av.y);
+		if (ok) {
+			*(double2*)(pnew + i) = cur.p;
+			*(double2*)(ap + i) = av;
+			acc_add(acc[0], cur.p.x * av.x);
+			acc_add(acc[0], cur.p.y * av.y);
+		}
 		up = cur;
 		cur = dn;
-		if constexpr (VK) {
-			kup = kcur;
-			kcur = kdn;
-		}
 	}
 	block_partial<1>(acc, rs);
 }
@@ -805,7 +837,7 @@ struct FinBiTop {
 	bool first;
 	__device__ void operator()(Acc* o) const {
 		if (!first && st->pend) {   // finish the early exit with the values recorded by FinBiS / FinBiOm
-			st->done = 1;
+			set_done(st);
 			st->skip = 1;
 			return;
 		}
@@ -1060,7 +1092,7 @@ struct fx_solver {
 	SState* hst = nullptr;       // pinned host copy
 	double* dhist = nullptr;
 	double* dgm = nullptr;       // GMRES H, cc, ss, grs, y
-	int* hflag = nullptr;        // pinned ring of done flags
+	int* hdone = nullptr;        // coherent pinned copy of the device `done` flag
 	hipEvent_t ev[8] = {};
 	std::vector<double> hist;
 	double* vec(int i) {
@@ -1242,6 +1274,10 @@ fx_status reset_state(fx_solver* s, int64_t max_iter, const fx_precond* pc = nul
 	}
 	h.tol = s->tol;
 	h.max_iter = max_iter;
+	__atomic_store_n(s->hdone, 0, __ATOMIC_RELEASE);
+	void* hd = nullptr;
+	FX_HIP(hipHostGetDevicePointer(&hd, s->hdone, 0));
+	h.hdone = (int*)hd;
 	h.hist = s->dhist;
 	h.hist_len = HIST_MAX;
 	h.m = (int32_t)s->restart;
@@ -1267,11 +1303,11 @@ fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
 	for (int64_t i = 0;; ++i) {
 		if (i > max_steps + look + 2) break;
 		FX_CHECK(step(i));
-		FX_HIP(hipMemcpyAsync(&s->hflag[i % RING], &s->dst->done, sizeof(int), hipMemcpyDeviceToHost, st));
 		FX_HIP(hipEventRecord(s->ev[i % RING], st));
 		if (i >= look) {
+			// Once step i − look has completed, its kernels' stores to *hdone are visible here.
 			FX_HIP(hipEventSynchronize(s->ev[(i - look) % RING]));
-			if (s->hflag[(i - look) % RING]) break;
+			if (__atomic_load_n(s->hdone, __ATOMIC_ACQUIRE)) break;
 		}
 	}
 	FX_HIP(hipStreamSynchronize(st));
@@ -1966,7 +2002,8 @@ fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, ui
 	if ((st = dev_alloc(be, sizeof(double) * HIST_MAX, &p)) != FX_OK) return fail(st);
 	s->dhist = (double*)p;
 	if (hipHostMalloc((void**)&s->hst, sizeof(SState), hipHostMallocDefault) != hipSuccess) return fail(FX_ERR_ALLOC);
-	if (hipHostMalloc((void**)&s->hflag, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess) return fail(FX_ERR_ALLOC);
+	if (hipHostMalloc((void**)&s->hdone, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+		return fail(FX_ERR_ALLOC);
 	for (auto& e : s->ev)
 		if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(FX_ERR_HIP);
 	// Work vectors up front, as the reference constructors do (cg.rs:79-84).
@@ -1999,7 +2036,7 @@ void fx_solver_destroy(fx_solver* s) {
 	dev_free(s->dhist);
 	dev_free(s->dgm);
 	if (s->hst) (void)hipHostFree(s->hst);
-	if (s->hflag) (void)hipHostFree(s->hflag);
+	if (s->hdone) (void)hipHostFree(s->hdone);
 	for (auto& e : s->ev)
 		if (e) (void)hipEventDestroy(e);
 	delete s;
