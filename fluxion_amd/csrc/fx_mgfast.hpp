@@ -14,6 +14,8 @@
 // is computed with exactly the operations of the unfused kernels (fx_mg.hip),
 // so the fused cycle is bit-identical to the oracle's FP32 V-cycle.
 #pragma once
+#include <type_traits>
+
 #include "fx_mg.hpp"
 
 namespace {
@@ -350,6 +352,8 @@ struct NoEpi {
 	__device__ void term(int64_t, float, Acc*) const {}
 	__device__ Pf load(int64_t) const { return Pf{}; }
 	__device__ void use(const Pf&, float, Acc*) const {}
+	__device__ Pf load_at(size_t, unsigned) const { return Pf{}; }
+	__device__ void use_lane(const Pf&, float, double*) const {}
 };
 
 // Calls f(y, x) for every point of the NY × NX block at (OY, OX) of an LDS region.
@@ -970,11 +974,6 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 // WD steps ahead from clamped (always in-bounds) addresses, so the loads are
 // unconditional and the steps straight-line code.  Per-point arithmetic, and
 // the restriction's summation order, are those of the unfused kernels.
-// WD (a template parameter of the kernels): rows of loads in flight per wave,
-// 8 on large levels, 4 on small ones, whose short segments would otherwise be
-// padded to a multiple of 8 steps.
-constexpr int WD_BIG = 8, WD_SMALL = 4;
-constexpr int64_t WD_BIG_POINTS = 1 << 20;   // levels with at least this many points use WD_BIG
 constexpr int WNMAX = 5;  // most fine columns (or rows) a coarse point gathers from (2·(h/H)⁻¹ + 1 for H ≈ 2h)
 
 // v of lane x − 1 (lane 0: 0) and of lane x + 1 (lane 63: 0), wave-wide.
@@ -986,27 +985,31 @@ __device__ __forceinline__ float lane_east(float v) {
 }
 
 // One row of a level's inputs at lane column gc: b and, for variable
-// coefficients, the faces west (kx[gc − 1]), east (kx[gc]) and south (ky[gc])
-// of the point.  wload reads them unconditionally from clamped addresses;
-// wfix, applied when the row is consumed (not when it is loaded, so that the
-// compiler cannot tie the load to a branch and wait on it at once), puts 0 and
-// 1 outside the grid, as the LDS kernels stage them.
+// coefficients, the faces west (kx[gc − 1]), east (kx[gc]), north (ky one row
+// up) and south (ky[gc]) of the point, and ω/d.  wload reads them
+// unconditionally from clamped addresses (a uniform row offset plus the lane's
+// 32-bit column); wfix, applied when the row is consumed (not when it is
+// loaded, so that the compiler cannot tie the load to a branch and wait on it
+// at once), puts 0 and 1 outside the grid, as the LDS kernels stage them.
 struct WRow {
-	float b, kw, ke, ky, wd;
+	float b, kw, ke, kn, ks, wd;
 };
 template <bool VK>
-__device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols, int gcc) {
+__device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols,
+                                      unsigned c, unsigned cw) {
 	const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
 	const size_t o = (size_t)rc * (size_t)cols;
 	WRow v;
-	v.b = b[o + gcc];
+	v.b = (b + o)[c];
 	if constexpr (VK) {
-		v.kw = L.kx[o + (gcc > 0 ? gcc - 1 : 0)];
-		v.ke = L.kx[o + gcc];
-		v.ky = L.ky[o + gcc];
-		v.wd = L.wd[o + gcc];
+		const size_t on = (size_t)(rc > 0 ? rc - 1 : 0) * (size_t)cols;
+		v.kw = (L.kx + o)[cw];
+		v.ke = (L.kx + o)[c];
+		v.kn = (L.ky + on)[c];
+		v.ks = (L.ky + o)[c];
+		v.wd = (L.wd + o)[c];
 	} else {
-		v.kw = v.ke = v.ky = 1.0f;
+		v.kw = v.ke = v.kn = v.ks = 1.0f;
 		v.wd = 0.0f;
 	}
 	return v;
@@ -1019,13 +1022,48 @@ __device__ __forceinline__ WRow wfix(const WRow& v, int r, int rows, bool colin,
 	if constexpr (VK) {
 		o.kw = rin && kwok ? v.kw : 1.0f;
 		o.ke = rin && keok ? v.ke : 1.0f;
-		o.ky = rin && colin && r < rows - 1 ? v.ky : 1.0f;
+		o.kn = colin && r >= 1 && r <= rows - 1 ? v.kn : 1.0f;
+		o.ks = rin && colin && r < rows - 1 ? v.ks : 1.0f;
 		o.wd = v.wd;  // used on interior points only
 	} else {
-		o.kw = o.ke = o.ky = 1.0f;
+		o.kw = o.ke = o.kn = o.ks = 1.0f;
 		o.wd = 0.0f;
 	}
 	return o;
+}
+
+// Row rings.  A wave's steps are unrolled by WD = 8, and row t of unrolled step
+// k sits in slot (k − i) mod W of a W-slot ring when the current row is t + i,
+// a compile-time index, so advancing a row moves no data.  The inputs' ring has
+// W = WD slots: the stages read the last few rows in place and each slot is
+// refilled, WD − (rows kept) steps ahead, once its row is no longer read.  The
+// stages' own rings have W = 4 slots.
+constexpr int WD = 8;
+constexpr int WR = 4;
+template <int W, int K, int I>
+struct Slot {
+	static constexpr int v = ((K - I) % W + W) % W;
+};
+// Jacobi update / operator value of the point whose inputs are v, from its
+// centre c, north n, south s, west w and east e values.
+template <bool VK>
+__device__ __forceinline__ float wjac(const LWf& L, const WRow& v, float c, float n, float s, float w, float e) {
+	if constexpr (VK) return jackw(L, v.kw, v.ke, v.kn, v.ks, v.wd, c, n, s, w, e, v.b);
+	else return jac(L, c, n, s, w, e, v.b);
+}
+template <bool VK>
+__device__ __forceinline__ float wjac0(const LWf& L, const WRow& v) {
+	if constexpr (VK) return jackw(L, v.kw, v.ke, v.kn, v.ks, v.wd, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, v.b);
+	else return jac0(L, v.b);
+}
+template <bool VK>
+__device__ __forceinline__ float wop(const LWf& L, const WRow& v, float c, float n, float s, float w, float e) {
+	if constexpr (VK) {
+		float d;
+		return vkff(L, v.kw, v.ke, v.kn, v.ks, c, n, s, w, e, &d);
+	} else {
+		return sop(L, c, n, s, w, e);
+	}
 }
 
 // lin1(r, nf, nc) of consecutive rows r (clamped to [0, nf − 1]): the coarse
@@ -1041,11 +1079,12 @@ struct LinRow {
 		i0 = (int)(num / den);
 		rem = (int)(num % den);
 	}
+	// Coarsening levels have nc ≤ nf, so step < den and one subtraction suffices.
 	__device__ void next() {
 		++r;
 		if (r >= 1 && r <= nf - 1) {
 			rem += step;
-			while (rem >= den) rem -= den, ++i0;
+			if (rem >= den) rem -= den, ++i0;
 		}
 	}
 	__device__ double w(double inv) const { return double(rem) * inv; }
@@ -1057,10 +1096,43 @@ inline int64_t wave_strips_restrict(int64_t cc) { return (cc - 2 + RCX - 1) / RC
 inline int64_t wave_segs_restrict(int64_t cr, int scr) { return (cr - 2 + scr - 1) / scr; }   // interior coarse rows
 inline unsigned wave_blocks(int64_t nwaves) { return (unsigned)((nwaves + NWAVES - 1) / NWAVES); }
 
+// Runs step(integral_constant<k>, t) for t = t0 … t1 − 1, k = (t − t0) mod WD:
+// whole blocks of WD steps without conditions (so the compiler's memory-wait
+// counts carry across the loop), then the remainder.
+template <class F>
+__device__ __forceinline__ void run_steps(F&& step, int t0, int t1) {
+	auto all = [&](auto kc, int b) { step(kc, b + decltype(kc)::value); };
+	int b = t0;
+	for (; b + WD <= t1; b += WD) {
+		all(std::integral_constant<int, 0>{}, b);
+		all(std::integral_constant<int, 1>{}, b);
+		all(std::integral_constant<int, 2>{}, b);
+		all(std::integral_constant<int, 3>{}, b);
+		all(std::integral_constant<int, 4>{}, b);
+		all(std::integral_constant<int, 5>{}, b);
+		all(std::integral_constant<int, 6>{}, b);
+		all(std::integral_constant<int, 7>{}, b);
+	}
+	auto one = [&](auto kc) {
+		constexpr int k = decltype(kc)::value;
+		if (b + k < t1) step(kc, b + k);
+	};
+	one(std::integral_constant<int, 0>{});
+	one(std::integral_constant<int, 1>{});
+	one(std::integral_constant<int, 2>{});
+	one(std::integral_constant<int, 3>{});
+	one(std::integral_constant<int, 4>{});
+	one(std::integral_constant<int, 5>{});
+	one(std::integral_constant<int, 6>{});
+	one(std::integral_constant<int, 7>{});
+}
+
 // out = J^NPOST(J^NPRE(0; b) + P·e) over the level (k_fpost's result), with Epi's
-// per-point reduction terms.  Wave gw takes strip gw % nstrips and row segment
-// gw / nstrips of sh rows.
-template <bool VK, int NPRE, int NPOST, class Epi, int WD>
+// per-point reduction terms (summed per lane in FP64, then merged with
+// compensation across lanes and workgroups).  Wave gw takes strip gw % nstrips
+// and row segment gw / nstrips of sh rows.  Stage values are computed on every
+// lane and row and then selected, so a step is branch-free up to the store.
+template <bool VK, int NPRE, int NPOST, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, const float* __restrict__ ec,
                                                  float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
                                                  int cc, int nstrips, int nseg, int sh, Epi epi, RedScratch rs,
@@ -1069,35 +1141,36 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 	constexpr int LU = NPRE - 1;     // rows the iterate u = J^npre(0; b) + P·e lags the loaded row
 	constexpr int LS = LU + 1;       // first post-sweep (NPOST == 2)
 	constexpr int LZ = LU + NPOST;   // the level's result
-	static_assert(LZ <= SHX, "halo too small");
+	static_assert(LZ <= SHX && LZ < WR && LZ < WD, "halo or rings too small");
 	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
-	Acc acc[NA];
+	double lsum[NA];
 #pragma unroll
-	for (int k = 0; k < NA; ++k) acc[k] = Acc{0.0, 0.0};
+	for (int k = 0; k < NA; ++k) lsum[k] = 0.0;
 	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
 	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
 	if (gw < nstrips * nseg) {
 		const int strip = gw % nstrips, seg = gw / nstrips;
 		const int gc = strip * SSW - SHX + x;
-		const int gcc = gc < 0 ? 0 : (gc < cols ? gc : cols - 1);
+		const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < cols ? gc : cols - 1));
+		const unsigned ucw = uc > 0 ? uc - 1 : 0;
 		const bool colin = gc >= 0 && gc < cols;
 		const bool colint = gc >= 1 && gc <= cols - 2;
 		const bool kwok = gc >= 1 && gc <= cols - 1, keok = gc >= 0 && gc <= cols - 2;
 		const bool outlane = colin && x >= SHX && x < SRW - SHX;
 		const int S0 = seg * sh, S1 = S0 + sh < rows ? S0 + sh : rows;
 		// Column interpolation data of this lane.
-		int b0 = 0, b1 = 0;
+		unsigned b0 = 0, b1 = 0;
 		float wx = 0.0f, vx = 0.0f;
 		if (colint) {
 			const int a = X.i0c[gc];
-			b0 = a;
-			b1 = a + 1 < cc ? a + 1 : a;
+			b0 = (unsigned)a;
+			b1 = (unsigned)(a + 1 < cc ? a + 1 : a);
 			wx = float(X.wc[gc]);
 			vx = 1.0f - wx;
 		}
 		auto inter = [&](int r) { return colint && r >= 1 && r <= rows - 2; };
-		// Coarse-correction operands of row r (read when row r + LU is loaded).
+		// Coarse-correction operands of a row (read WD steps before the row's iterate is formed).
 		struct EV {
 			float e00, e01, e10, e11, wy;
 		};
@@ -1114,100 +1187,73 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 		};
 		auto pload = [&](int r) {
 			const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
-			return epi.load((int64_t)rc * cols + gcc);
+			return epi.load_at((size_t)rc * (size_t)cols, uc);
 		};
-		float B[LZ + 1], KW[LZ + 1], KE[LZ + 1], KY[LZ + 2], QW[LZ + 1];
-#pragma unroll
-		for (int i = 0; i <= LZ; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f, QW[i] = 0.0f;
-#pragma unroll
-		for (int i = 0; i <= LZ + 1; ++i) KY[i] = 1.0f;
-		float U1[3] = {0.0f, 0.0f, 0.0f}, U[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
-		// Steps load rows T0 … T1 − 1: the first output row needs the first pre-sweep from row S0 − LZ on,
-		// and that sweep needs the south face of the row above it.
-		const int T0 = S0 - LZ - 1, T1 = S1 + LZ;
+		const int T0 = S0 - LZ, T1 = S1 + LZ;  // steps: rows loaded
 		le.init(T0 - LU, rows, cr);
-		WRow inb[WD];
+		// Input ring: row T0 + j in slot j mod WD; the first WD − LZ rows now, the rest as slots free up.
+		WRow rb[WD];
 		EV evb[WD];
 		typename Epi::Pf pfb[WD];
 #pragma unroll
 		for (int k = 0; k < WD; ++k) {
-			inb[k] = wload<VK>(b, L, T0 + k, rows, cols, gcc);
+			rb[k] = k < WD - LZ ? wload<VK>(b, L, T0 + k, rows, cols, uc, ucw) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
 			evb[k] = eload();
 			pfb[k] = pload(T0 + k - LZ);
 		}
-		for (int t0 = T0; t0 < T1; t0 += WD) {
-#pragma unroll
-			for (int k = 0; k < WD; ++k) {
-				const int t = t0 + k;
-				// Row windows: index i holds row t − i.
-#pragma unroll
-				for (int i = LZ; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
-#pragma unroll
-				for (int i = LZ + 1; i > 0; --i) KY[i] = KY[i - 1];
-				{
-					const WRow in = wfix<VK>(inb[k], t, rows, colin, kwok, keok);
-					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
-				}
-				const EV ev = evb[k];
-				// Faces of row t − i: west KW[i], east KE[i], north KY[i + 1], south KY[i].
-				float u = 0.0f;  // iterate at row t − LU
-				if constexpr (NPRE == 2) {
-					float u1 = 0.0f;
-					if (inter(t)) {
-						if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-						else u1 = jac0(L, B[0]);
-					}
-					U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
-					const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
-					if (inter(t - 1)) {
-						if constexpr (VK) u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
-						else u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
-					}
-				} else {
-					if (inter(t)) {
-						if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-						else u = jac0(L, B[0]);
-					}
-				}
-				if (inter(t - LU)) {
-					const float wy = ev.wy, vy = 1.0f - wy;
-					const float top = vx * ev.e00 + wx * ev.e01;
-					const float bot = vx * ev.e10 + wx * ev.e11;
-					u += vy * top + wy * bot;
-				}
-				U[2] = U[1], U[1] = U[0], U[0] = u;
-				const float* C = U;
-				if constexpr (NPOST == 2) {
-					float s = U[1];
-					const float wv = lane_west(U[1]), evv = lane_east(U[1]);
-					if (inter(t - LS)) {
-						if constexpr (VK)
-							s = jackw(L, KW[LS], KE[LS], KY[LS + 1], KY[LS], QW[LS], U[1], U[2], U[0], wv, evv, B[LS]);
-						else
-							s = jac(L, U[1], U[2], U[0], wv, evv, B[LS]);
-					}
-					S[2] = S[1], S[1] = S[0], S[0] = s;
-					C = S;
-				}
-				float z = C[1];
-				const float wv = lane_west(C[1]), evv = lane_east(C[1]);
-				const int rz = t - LZ;
-				if (inter(rz)) {
-					if constexpr (VK) z = jackw(L, KW[LZ], KE[LZ], KY[LZ + 1], KY[LZ], QW[LZ], C[1], C[2], C[0], wv, evv, B[LZ]);
-					else z = jac(L, C[1], C[2], C[0], wv, evv, B[LZ]);
-				}
-				if (outlane && rz >= S0 && rz < S1) {
-					out[(size_t)rz * (size_t)cols + gc] = z;
-					epi.use(pfb[k], z, acc);
-				}
-				// Refill this step's slots (after their last use, so each slot keeps its registers).
-				inb[k] = wload<VK>(b, L, t + WD, rows, cols, gcc);
-				evb[k] = eload();
-				pfb[k] = pload(t + WD - LZ);
+		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, U[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, S[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
+		auto step = [&](auto kc, int t) {
+			constexpr int k = decltype(kc)::value, q = k % WR;
+			rb[k] = wfix<VK>(rb[k], t, rows, colin, kwok, keok);
+			// Inputs of row t − i: rb[Slot<WD, k, i>].
+			float u;  // iterate at row t − LU
+			if constexpr (NPRE == 2) {
+				const float u1 = wjac0<VK>(L, rb[k]);
+				U1[q] = inter(t) ? u1 : 0.0f;
+				const float c = U1[Slot<WR, q, 1>::v];
+				const float wv = lane_west(c), evv = lane_east(c);
+				u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+			} else {
+				u = wjac0<VK>(L, rb[k]);
 			}
-		}
+			{
+				const EV& ev = evb[k];
+				const float wy = ev.wy, vy = 1.0f - wy;
+				const float top = vx * ev.e00 + wx * ev.e01;
+				const float bot = vx * ev.e10 + wx * ev.e11;
+				u += vy * top + wy * bot;
+			}
+			U[q] = inter(t - LU) ? u : 0.0f;
+			const float* C = U;
+			if constexpr (NPOST == 2) {
+				const float c = U[Slot<WR, q, 1>::v];
+				const float wv = lane_west(c), evv = lane_east(c);
+				const float sv = wjac<VK>(L, rb[Slot<WD, k, LS>::v], c, U[Slot<WR, q, 2>::v], U[q], wv, evv);
+				S[q] = inter(t - LS) ? sv : c;
+				C = S;
+			}
+			const float c = C[Slot<WR, q, 1>::v];
+			const float wv = lane_west(c), evv = lane_east(c);
+			const float zv = wjac<VK>(L, rb[Slot<WD, k, LZ>::v], c, C[Slot<WR, q, 2>::v], C[q], wv, evv);
+			const int rz = t - LZ;
+			const float z = inter(rz) ? zv : c;
+			if (outlane && rz >= S0 && rz < S1) {
+				(out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
+				epi.use_lane(pfb[k], z, lsum);
+			}
+			// Refill: the slot of row t − LZ (no longer read) with row t + WD − LZ, and this step's operands.
+			rb[Slot<WD, k, LZ>::v] = wload<VK>(b, L, t + WD - LZ, rows, cols, uc, ucw);
+			evb[k] = eload();
+			pfb[k] = pload(t + WD - LZ);
+		};
+		run_steps(step, T0, T1);
 	}
-	if constexpr (Epi::NS > 0) block_partial<Epi::NS>(acc, rs);
+	if constexpr (Epi::NS > 0) {
+		Acc acc[NA];
+#pragma unroll
+		for (int k = 0; k < NA; ++k) acc[k] = Acc{lsum[k], 0.0};
+		block_partial<Epi::NS>(acc, rs);
+	}
 }
 
 // coarse rhs = R(b − A·J^npre(0; b)) over the level (k_frestrict's result).
@@ -1219,44 +1265,49 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 // the coarse column's fine columns in ascending order (cross-lane permutes),
 // then added, in ascending fine-row order, to the (at most two) coarse rows the
 // fine row contributes to: the order of k_restrict's loops.
-template <bool VK, int NPRE, int WD>
+template <bool VK, int NPRE>
 __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
                                                      fx_xtab X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
                                                      int scr, const int* skip) {
 	if (skip && *skip) return;
-	constexpr int LU = NPRE - 1, LR = NPRE;  // lags of the pre-smoothed iterate and of the residual
+	constexpr int LU = NPRE - 1, LR = NPRE;  // rows the pre-smoothed iterate and the residual lag the loaded row
+	static_assert(LR < WR && LR < WD, "rings too small");
 	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
 	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
-	bool active = gw < nstrips * nseg;
-	const int strip = active ? gw % nstrips : 0, seg = active ? gw / nstrips : 0;
+	if (gw >= nstrips * nseg) return;  // no workgroup-wide operation follows
+	const int strip = gw % nstrips, seg = gw / nstrips;
 	const bool sfirst = strip == 0, slast = strip == nstrips - 1, gfirst = seg == 0, glast = seg == nseg - 1;
 	const int Ja = 1 + strip * RCX, Jb = Ja + RCX - 1 < cc - 2 ? Ja + RCX - 1 : cc - 2;
 	const int Ia = 1 + seg * scr, Ib = Ia + scr - 1 < cr - 2 ? Ia + scr - 1 : cr - 2;
 	const int J = Ja + x;
 	const bool jint = x < RCX && J <= Jb;  // the lane's interior coarse column
-	if (active) {
-		// Zero boundary rows (first / last segment) and columns (first / last strip).
-		if (gfirst || glast) {
-			const size_t row = gfirst ? 0 : (size_t)(cr - 1) * cc;
-			if (jint) bco[row + J] = 0.0f;
-			if (x == 0 && sfirst) bco[row] = 0.0f;
-			if (x == 0 && slast) bco[row + cc - 1] = 0.0f;
-			if (gfirst && glast && jint) bco[(size_t)(cr - 1) * cc + J] = 0.0f;
-			if (gfirst && glast && x == 0 && sfirst) bco[(size_t)(cr - 1) * cc] = 0.0f;
-			if (gfirst && glast && x == 0 && slast) bco[(size_t)(cr - 1) * cc + cc - 1] = 0.0f;
-		}
-		if ((sfirst || slast) && x < 2) {
-			const bool left = x == 0;
-			if ((left && sfirst) || (!left && slast))
-				for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + (left ? 0 : cc - 1)] = 0.0f;
-		}
+	// Zero boundary rows (first / last segment) and columns (first / last strip).
+	if (gfirst || glast) {
+		const size_t row = gfirst ? 0 : (size_t)(cr - 1) * cc;
+		if (jint) bco[row + J] = 0.0f;
+		if (x == 0 && sfirst) bco[row] = 0.0f;
+		if (x == 0 && slast) bco[row + cc - 1] = 0.0f;
+		if (gfirst && glast && jint) bco[(size_t)(cr - 1) * cc + J] = 0.0f;
+		if (gfirst && glast && x == 0 && sfirst) bco[(size_t)(cr - 1) * cc] = 0.0f;
+		if (gfirst && glast && x == 0 && slast) bco[(size_t)(cr - 1) * cc + cc - 1] = 0.0f;
+	}
+	if ((sfirst || slast) && x < 2) {
+		const bool left = x == 0;
+		if ((left && sfirst) || (!left && slast))
+			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + (left ? 0 : cc - 1)] = 0.0f;
 	}
 	const int G0 = X.loc[Ja];
 	const int F0 = X.lor[Ia], F1 = X.hir[Ib];
+	if (G0 > X.hic[Jb] || F0 > F1) {  // nothing to gather (tiny grids): the strip's interior rows are zero
+		if (jint)
+			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + J] = 0.0f;
+		return;
+	}
 	const int C0 = G0 - 2;  // fine column of lane 0
 	const int gc = C0 + x;
-	const int gcc = gc < 0 ? 0 : (gc < fc ? gc : fc - 1);
+	const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < fc ? gc : fc - 1));
+	const unsigned ucw = uc > 0 ? uc - 1 : 0;
 	const bool colin = gc >= 0 && gc < fc;
 	const bool colint = gc >= 1 && gc <= fc - 2;
 	const bool kwok = gc >= 1 && gc <= fc - 1, keok = gc >= 0 && gc <= fc - 2;
@@ -1279,106 +1330,68 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 		}
 	}
 	const float scale = X.sy * X.sx;
-	float B[LR + 1], KW[LR + 1], KE[LR + 1], KY[LR + 2], QW[LR + 1];
-#pragma unroll
-	for (int i = 0; i <= LR; ++i) B[i] = 0.0f, KW[i] = KE[i] = 1.0f, QW[i] = 0.0f;
-#pragma unroll
-	for (int i = 0; i <= LR + 1; ++i) KY[i] = 1.0f;
-	float U1[3] = {0.0f, 0.0f, 0.0f}, Uv[3] = {0.0f, 0.0f, 0.0f};
 	int Icur = X.i0r[F0];  // coarse row accumulated in alo (ahi: Icur + 1)
 	float alo = 0.0f, ahi = 0.0f;
 	auto emit = [&](int I, float v) {
 		if (jint && I >= Ia && I <= Ib) bco[(size_t)I * cc + J] = v * scale;
 	};
-	// Steps load rows T0 … T1 − 1: the residual of row F0 needs the first pre-sweep from row F0 − LR on,
-	// and that sweep needs the south face of the row above it.
-	const int T0 = F0 - LR - 1, T1 = F1 + LR + 1;
+	const int T0 = F0 - LR, T1 = F1 + LR + 1;  // steps: rows loaded
 	const double rinv = 1.0 / double(fr - 1);
 	LinRow lr;  // fine row whose residual the step produces
 	lr.init(T0 - LR, fr, cr);
-	if (G0 > X.hic[Jb] || F0 > F1) {  // nothing to gather (tiny grids): the strip's interior rows are zero
-		if (active && jint)
-			for (int I = Ia; I <= Ib; ++I) bco[(size_t)I * cc + J] = 0.0f;
-		active = false;
-	}
-	if (active) {
-		WRow inb[WD];
+	WRow rb[WD];
 #pragma unroll
-		for (int k = 0; k < WD; ++k) inb[k] = wload<VK>(b, L, T0 + k, fr, fc, gcc);
-		for (int t0 = T0; t0 < T1; t0 += WD) {
-#pragma unroll
-			for (int k = 0; k < WD; ++k) {
-				const int t = t0 + k;
-#pragma unroll
-				for (int i = LR; i > 0; --i) B[i] = B[i - 1], KW[i] = KW[i - 1], KE[i] = KE[i - 1], QW[i] = QW[i - 1];
-#pragma unroll
-				for (int i = LR + 1; i > 0; --i) KY[i] = KY[i - 1];
-				{
-					const WRow in = wfix<VK>(inb[k], t, fr, colin, kwok, keok);
-					B[0] = in.b, KW[0] = in.kw, KE[0] = in.ke, KY[0] = in.ky, QW[0] = in.wd;
-				}
-				inb[k] = wload<VK>(b, L, t + WD, fr, fc, gcc);  // slot k refilled
-				float u = 0.0f;  // pre-smoothed iterate at row t − LU
-				if constexpr (NPRE == 2) {
-					float u1 = 0.0f;
-					if (inter(t)) {
-						if constexpr (VK) u1 = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-						else u1 = jac0(L, B[0]);
-					}
-					U1[2] = U1[1], U1[1] = U1[0], U1[0] = u1;
-					const float wv = lane_west(U1[1]), evv = lane_east(U1[1]);
-					if (inter(t - 1)) {
-						if constexpr (VK)
-							u = jackw(L, KW[1], KE[1], KY[2], KY[1], QW[1], U1[1], U1[2], U1[0], wv, evv, B[1]);
-						else
-							u = jac(L, U1[1], U1[2], U1[0], wv, evv, B[1]);
-					}
-				} else {
-					if (inter(t)) {
-						if constexpr (VK) u = jackw(L, KW[0], KE[0], KY[1], KY[0], QW[0], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, B[0]);
-						else u = jac0(L, B[0]);
-					}
-				}
-				Uv[2] = Uv[1], Uv[1] = Uv[0], Uv[0] = u;
-				// Residual at fine row i = t − LR.
-				const int i = t - LR;
-				const float wv = lane_west(Uv[1]), evv = lane_east(Uv[1]);
-				float res = 0.0f;
-				if (inter(i)) {
-					float sv;
-					if constexpr (VK) {
-						float d;
-						sv = vkff(L, KW[LR], KE[LR], KY[LR + 1], KY[LR], Uv[1], Uv[2], Uv[0], wv, evv, &d);
-					} else {
-						sv = sop(L, Uv[1], Uv[2], Uv[0], wv, evv);
-					}
-					res = fmaf(1.0f, B[LR], -sv);
-				}
-				// Row sum over the owned coarse column's fine columns.
-				float row = 0.0f;
-#pragma unroll
-				for (int k2 = 0; k2 < WNMAX; ++k2) {
-					const float v = __shfl(res, sk[k2]);
-					if (k2 < cnt) row += wk[k2] * v;
-				}
-				const int a = lr.i0;
-				const double wrv = lr.w(rinv);
-				lr.next();
-				if (i >= F0 && i <= F1) {
-					while (Icur < a) {
-						emit(Icur, alo);
-						alo = ahi;
-						ahi = 0.0f;
-						++Icur;
-					}
-					alo += float(1.0 - wrv) * row;
-					ahi += float(wrv) * row;
-				}
-			}
+	for (int k = 0; k < WD; ++k)
+		rb[k] = k < WD - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc, ucw) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+	float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
+	auto step = [&](auto kc, int t) {
+		constexpr int k = decltype(kc)::value, q = k % WR;
+		rb[k] = wfix<VK>(rb[k], t, fr, colin, kwok, keok);
+		float u;  // pre-smoothed iterate at row t − LU
+		if constexpr (NPRE == 2) {
+			const float u1 = wjac0<VK>(L, rb[k]);
+			U1[q] = inter(t) ? u1 : 0.0f;
+			const float c = U1[Slot<WR, q, 1>::v];
+			const float wv = lane_west(c), evv = lane_east(c);
+			u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+		} else {
+			u = wjac0<VK>(L, rb[k]);
 		}
-		emit(Icur, alo);
-		emit(Icur + 1, ahi);
-	}
+		Uv[q] = inter(t - LU) ? u : 0.0f;
+		// Residual at fine row i = t − LR.
+		const int i = t - LR;
+		const float c = Uv[Slot<WR, q, 1>::v];
+		const float wv = lane_west(c), evv = lane_east(c);
+		const WRow& ri = rb[Slot<WD, k, LR>::v];
+		const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
+		const float res = inter(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
+		// Refill the slot of row t − LR (no longer read) with row t + WD − LR.
+		rb[Slot<WD, k, LR>::v] = wload<VK>(b, L, t + WD - LR, fr, fc, uc, ucw);
+		// Row sum over the owned coarse column's fine columns.
+		float row = 0.0f;
+#pragma unroll
+		for (int k2 = 0; k2 < WNMAX; ++k2) {
+			const float v = __shfl(res, sk[k2]);
+			if (k2 < cnt) row += wk[k2] * v;
+		}
+		const int a = lr.i0;
+		const double wrv = lr.w(rinv);
+		lr.next();
+		if (i >= F0 && i <= F1) {
+			// Coarse rows advance by at most one per fine row (nc ≤ nf).
+			if (Icur < a) {
+				emit(Icur, alo);
+				alo = ahi;
+				ahi = 0.0f;
+				++Icur;
+			}
+			alo += float(1.0 - wrv) * row;
+			ahi += float(wrv) * row;
+		}
+	};
+	run_steps(step, T0, T1);
+	emit(Icur, alo);
+	emit(Icur + 1, ahi);
 }
 
 // ── k_ftail: the V-cycle below a small level, in one workgroup ─────────
@@ -1676,27 +1689,19 @@ inline int tail_plan(const fx_multigrid* mg, bool vk, TailArgs* A, size_t* lds) 
 	return nl - 1;
 }
 
-// Launches of the wave kernels with the level's prefetch depth.
+// Launches of the wave kernels.
 template <bool VK, int NPRE, int NPOST, class Epi>
 void launch_wpost(hipStream_t st, unsigned blocks, const float* b, const float* ec, float* out, const LWf& L,
                   const fx_xtab& X, int rows, int cols, int cr, int cc, int ns, int nseg, int sh, Epi epi, RedScratch rs,
                   const int* skip) {
-	if ((int64_t)rows * cols >= WD_BIG_POINTS)
-		hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi, WD_BIG>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X,
-		                   rows, cols, cr, cc, ns, nseg, sh, epi, rs, skip);
-	else
-		hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi, WD_SMALL>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X,
-		                   rows, cols, cr, cc, ns, nseg, sh, epi, rs, skip);
+	hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X, rows, cols,
+	                   cr, cc, ns, nseg, sh, epi, rs, skip);
 }
 template <bool VK, int NPRE>
 void launch_wrestrict(hipStream_t st, unsigned blocks, const float* b, float* bco, const LWf& L, const fx_xtab& X,
                       int fr, int fc, int cr, int cc, int ns, int nseg, int scr, const int* skip) {
-	if ((int64_t)fr * fc >= WD_BIG_POINTS)
-		hipLaunchKernelGGL((k_wrestrict<VK, NPRE, WD_BIG>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr,
-		                   cc, ns, nseg, scr, skip);
-	else
-		hipLaunchKernelGGL((k_wrestrict<VK, NPRE, WD_SMALL>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr,
-		                   cc, ns, nseg, scr, skip);
+	hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr, cc, ns, nseg,
+	                   scr, skip);
 }
 
 // Segment heights of the wave-streaming kernels for a level: the tallest of

@@ -761,6 +761,9 @@ struct EpiRZ {
 	};
 	__device__ Pf load(int64_t j) const { return Pf{r[j]}; }
 	__device__ void use(const Pf& p, float z, Acc* a) const { acc_add(a[0], p.r * double(z)); }
+	// Row-offset form (uniform row offset o, lane column c) and plain per-lane sums (wave kernels).
+	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c]}; }
+	__device__ void use_lane(const Pf& p, float z, double* s) const { s[0] += p.r * double(z); }
 	using Fin = FinPcgInitRZ;
 	Fin fin() const { return Fin{st}; }
 };
@@ -781,6 +784,12 @@ struct EpiFcg {
 		const double zd = double(z);
 		acc_add(a[0], p.r * zd);
 		acc_add(a[1], zd * p.ap);
+	}
+	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c], (ap + o)[c]}; }
+	__device__ void use_lane(const Pf& p, float z, double* s) const {
+		const double zd = double(z);
+		s[0] += p.r * zd;
+		s[1] += zd * p.ap;
 	}
 	using Fin = FinFcgBeta;
 	Fin fin() const { return Fin{st}; }
