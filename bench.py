@@ -63,12 +63,12 @@ def kernel_bytes(vk: bool) -> dict:
 # HBM bytes per launch of the probed level-0 kernels on the 4096² var-k workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by the same counters' reading of a copy of
 # 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
-# profiles/vark4096_mixed_fcg_lds_ring.md.  Only valid for --size 4096.
+# profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels.
 MEASURED_TRAFFIC_4096 = {
-    "vc_restrict0": (113073 * 2 + 16400) * 1024,
-    "vc_post0": (248798 * 2 + 66270) * 1024,
-    "fcg_step": (284807 * 2 + 262208) * 1024,
-    "fcg_update": (262163 * 2 + 327744) * 1024,
+    "vc_restrict0": (159304 * 2 + 16400) * 1024,
+    "vc_post0": (331539 * 2 + 67647) * 1024,
+    "fcg_step": (186508 * 2 + 262208) * 1024,
+    "fcg_update": (262167 * 2 + 327744) * 1024,
 }
 
 
@@ -310,7 +310,7 @@ def main():
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["frac"],
-                     "traffic": MEASURED_TRAFFIC_4096.get(dom) if n == 4096 else None,
+                     "traffic": MEASURED_TRAFFIC_4096.get(dom) if n == 4096 and args.mg_kernels == "wave" else None,
                      "bytes_per_point": kd["bytes_per_point"],
                      "points_per_launch": n * n, "us_per_launch": kd["us_per_launch"]},
         "cpu_baseline": cpu,
