@@ -1494,20 +1494,23 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 			return jac(T.L, c, u[q - cols], u[q + cols], u[q - 1], u[q + 1], pb[T.off + q]);
 		}
 	};
-	auto inner = [](const TailLevel& T, int q) {
-		const int r = q / T.cols, c = q % T.cols;
-		return r > 0 && c > 0 && r < T.rows - 1 && c < T.cols - 1;
+	// Point loops: lanes take columns, waves take rows (no index division).
+	const int tx = t & (WAVE - 1), ty = t / WAVE;
+	constexpr int NYW = TAIL_BLOCK / WAVE;
+	auto pts = [&](int rows, int cols, auto f) {
+		for (int r = ty; r < rows; r += NYW)
+			for (int c = tx; c < cols; c += WAVE) f(r, c, r * cols + c);
 	};
+	auto inner = [](const TailLevel& T, int r, int c) { return r > 0 && c > 0 && r < T.rows - 1 && c < T.cols - 1; };
 	// ── down: pre-smoothing, residual, restriction ──
 	for (int l = 0; l + 1 < A.nl; ++l) {
 		const TailLevel& T = A.lv[l];
 		const TailLevel& C = A.lv[l + 1];
-		const int n = T.rows * T.cols;
 		// First sweep from zero into pv (NPRE == 2) or pu (NPRE == 1).
 		float* first = NPRE == 2 ? pv : pu;
-		for (int q = t; q < n; q += TAIL_BLOCK) {
+		pts(T.rows, T.cols, [&](int r, int c, int q) {
 			float v = 0.0f;
-			if (inner(T, q)) {
+			if (inner(T, r, c)) {
 				if constexpr (VK) {
 					const int o = T.off + q;
 					v = jackw(T.L, kxp[o - 1], kxp[o], kyp[o - T.cols], kyp[o], wdp[o], 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, pb[o]);
@@ -1516,16 +1519,18 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 				}
 			}
 			first[T.off + q] = v;
-		}
+		});
 		__syncthreads();
 		if constexpr (NPRE == 2) {
-			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = inner(T, q) ? jacq(T, pv, q, pv[T.off + q]) : 0.0f;
+			pts(T.rows, T.cols, [&](int r, int c, int q) {
+				pu[T.off + q] = inner(T, r, c) ? jacq(T, pv, q, pv[T.off + q]) : 0.0f;
+			});
 			__syncthreads();
 		}
 		// Residual into pv.
-		for (int q = t; q < n; q += TAIL_BLOCK) {
+		pts(T.rows, T.cols, [&](int r, int c, int q) {
 			float res = 0.0f;
-			if (inner(T, q)) {
+			if (inner(T, r, c)) {
 				const float* u = pu + T.off;
 				const int cols = T.cols;
 				float sv;
@@ -1540,13 +1545,12 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 				res = fmaf(1.0f, pb[T.off + q], -sv);
 			}
 			pv[T.off + q] = res;
-		}
+		});
 		__syncthreads();
 		// Row pass: rsum[i·cc + J] = Σ_jj w(jj, J)·res(i, jj) over J's fine columns.
 		const int cr = C.rows, cc = C.cols;
-		for (int q = t; q < T.rows * cc; q += TAIL_BLOCK) {
-			const int i = q / cc, J = q % cc;
-			if (i < 1 || i > T.rows - 2 || J < 1 || J > cc - 2) continue;
+		pts(T.rows, cc, [&](int i, int J, int q) {
+			if (i < 1 || i > T.rows - 2 || J < 1 || J > cc - 2) return;
 			const int lo = rlo[T.tcc + J], hi = rhi[T.tcc + J];
 			const float* row = pv + T.off + i * T.cols;
 			float rs = 0.0f;
@@ -1556,12 +1560,11 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 				rs += wxv * row[jj];
 			}
 			rsum[q] = rs;
-		}
+		});
 		__syncthreads();
 		// Column pass: the coarse rhs.
 		const float scale = T.X.sy * T.X.sx;
-		for (int q = t; q < cr * cc; q += TAIL_BLOCK) {
-			const int I = q / cc, J = q % cc;
+		pts(cr, cc, [&](int I, int J, int q) {
 			float outv = 0.0f;
 			if (I > 0 && I < cr - 1 && J > 0 && J < cc - 1) {
 				const int lo = rlo[T.tcr + I], hi = rhi[T.tcr + I];
@@ -1574,29 +1577,28 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 				outv = acc * scale;
 			}
 			pb[C.off + q] = outv;
-		}
+		});
 		__syncthreads();
 	}
 	// ── coarsest: sweeps from zero (k_fcoarse), result in pu ──
 	{
 		const TailLevel& T = A.lv[A.nl - 1];
-		const int n = T.rows * T.cols;
-		for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = 0.0f;
+		pts(T.rows, T.cols, [&](int, int, int q) { pu[T.off + q] = 0.0f; });
 		__syncthreads();
 		float* cur = pu;
 		float* nxt = pv;
 		for (int it = 0; it < A.sweeps; ++it) {
-			for (int q = t; q < n; q += TAIL_BLOCK) {
-				const float c = cur[T.off + q];
-				nxt[T.off + q] = inner(T, q) ? jacq(T, cur, q, c) : c;
-			}
+			pts(T.rows, T.cols, [&](int r, int c, int q) {
+				const float v = cur[T.off + q];
+				nxt[T.off + q] = inner(T, r, c) ? jacq(T, cur, q, v) : v;
+			});
 			__syncthreads();
 			float* x = cur;
 			cur = nxt;
 			nxt = x;
 		}
 		if (cur != pu) {
-			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = cur[T.off + q];
+			pts(T.rows, T.cols, [&](int, int, int q) { pu[T.off + q] = cur[T.off + q]; });
 			__syncthreads();
 		}
 	}
@@ -1604,12 +1606,10 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 	for (int l = A.nl - 2; l >= 0; --l) {
 		const TailLevel& T = A.lv[l];
 		const TailLevel& C = A.lv[l + 1];
-		const int n = T.rows * T.cols;
 		const int cr = C.rows, cc = C.cols;
 		const float* e = pu + C.off;
-		for (int q = t; q < n; q += TAIL_BLOCK) {
-			if (!inner(T, q)) continue;
-			const int r = q / T.cols, c = q % T.cols;
+		pts(T.rows, T.cols, [&](int r, int c, int q) {
+			if (!inner(T, r, c)) return;
 			const int a0 = ti0[T.tr + r], a1 = a0 + 1 < cr ? a0 + 1 : a0;
 			const int b0 = ti0[T.tc + c], b1 = b0 + 1 < cc ? b0 + 1 : b0;
 			const float wy = twh[T.tr + r], vy = 1.0f - wy;
@@ -1617,26 +1617,26 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 			const float top = vx * e[a0 * cc + b0] + wx * e[a0 * cc + b1];
 			const float bot = vx * e[a1 * cc + b0] + wx * e[a1 * cc + b1];
 			pu[T.off + q] += vy * top + wy * bot;
-		}
+		});
 		__syncthreads();
 		const float* last = pu;
 		if constexpr (NPOST == 2) {
-			for (int q = t; q < n; q += TAIL_BLOCK) {
-				const float c = pu[T.off + q];
-				pv[T.off + q] = inner(T, q) ? jacq(T, pu, q, c) : c;
-			}
+			pts(T.rows, T.cols, [&](int r, int c, int q) {
+				const float v = pu[T.off + q];
+				pv[T.off + q] = inner(T, r, c) ? jacq(T, pu, q, v) : v;
+			});
 			__syncthreads();
 			last = pv;
 		}
 		float* dst = l == 0 ? A.out0 : pu + T.off;
 		if (NPOST == 1 && l > 0) dst = pv + T.off;
-		for (int q = t; q < n; q += TAIL_BLOCK) {
-			const float c = last[T.off + q];
-			dst[q] = inner(T, q) ? jacq(T, last, q, c) : c;
-		}
+		pts(T.rows, T.cols, [&](int r, int c, int q) {
+			const float v = last[T.off + q];
+			dst[q] = inner(T, r, c) ? jacq(T, last, q, v) : v;
+		});
 		__syncthreads();
 		if (NPOST == 1 && l > 0) {
-			for (int q = t; q < n; q += TAIL_BLOCK) pu[T.off + q] = pv[T.off + q];
+			pts(T.rows, T.cols, [&](int, int, int q) { pu[T.off + q] = pv[T.off + q]; });
 			__syncthreads();
 		}
 	}

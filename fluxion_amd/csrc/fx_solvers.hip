@@ -420,6 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_fxr32v(const SState* st, double* __re
 	const double al = st->alpha;
 	Acc acc[1] = {Acc{0.0, 0.0}};
 	const size_t stride = (size_t)gridDim.x * BLOCK;
+#pragma unroll 4
 	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n2; j += stride) {
 		const double2 xv = ((const double2*)x)[j], pv = ((const double2*)p)[j];
 		const double2 rv = ((const double2*)r)[j], av = ((const double2*)ap)[j];
@@ -1422,8 +1423,11 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 				double* pold = pp[i & 1];
 				double* pnew = pp[(i + 1) & 1];
 				if (even) {
-					// Two columns per thread: half as many threads per row.
-					int rpb = stream_rpb(c.rows, c.cols / 2);
+					// Two columns per thread: half as many threads per row.  Taller row blocks (fewer
+					// halo rows) while the grid keeps at least 1024 workgroups.
+					const int64_t cb = (c.cols / 2 + BLOCK - 1) / BLOCK;
+					int rpb = 64;
+					while (rpb > 4 && cb * ((c.rows + rpb - 1) / rpb) < 1024) rpb /= 2;
 					dim3 grid = fx::stream_grid(c.rows, c.cols / 2, rpb);
 					FX_CHECK(fx::ensure_partials(c.be, (size_t)grid.x * grid.y));
 					const double* kp = vk ? (const double*)c.op->k->ptr : nullptr;
@@ -1442,7 +1446,9 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 					                   grid.x * grid.y, FinAlpha{c.d}, c.done);
 					FX_CHECK(check_launch("fcg fused step"));
 					const size_t n2 = c.n / 2;
-					const unsigned nb = fx::red_blocks(c.n);
+					// About four pairs per thread, unrolled, so each thread keeps several loads in flight.
+					const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>(16384, (n2 + 4 * BLOCK - 1) / (4 * BLOCK)));
+					FX_CHECK(fx::ensure_partials(c.be, nb));
 					fx::probe_begin(c.be, FX_PROBE_FCG_UPDATE);
 					hipLaunchKernelGGL(k_fxr32v, dim3(nb), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, r, r32,
 					                   (const double*)pnew, (const double*)ap, n2, fx::red_scratch(c.be), c.skip);
