@@ -88,16 +88,12 @@ __device__ __forceinline__ float jac(const LWf& L, float c, float n, float s, fl
 	float res = b - ax;
 	return fmaf(L.omega, L.inv_diag * res, c);
 }
-// First sweep from u = 0 (interior); boundary values stay 0.
+// First sweep from u = 0 (interior); boundary values stay 0.  With finite
+// weights the operator of u = 0 sums to +0, so b − A·0 is b (a zero of either
+// sign gives +0 after the final fma either way) and the sweep is exactly
+// ω·(d⁻¹·b) + 0.
 __device__ __forceinline__ float jac0(const LWf& L, float b) {
-	float ax = 0.0f;
-	ax = fmaf(L.w[0], 0.0f, ax);
-	ax = fmaf(L.w[1], 0.0f, ax);
-	ax = fmaf(L.w[2], 0.0f, ax);
-	ax = fmaf(L.w[3], 0.0f, ax);
-	ax = fmaf(L.w[4], 0.0f, ax);
-	float res = b - ax;
-	return fmaf(L.omega, L.inv_diag * res, 0.0f);
+	return fmaf(L.omega, L.inv_diag * b, 0.0f);
 }
 // S(u) of the level operator (fx_mg.hip k_lstencil interior).
 __device__ __forceinline__ float sop(const LWf& L, float c, float n, float s, float w, float e) {
@@ -986,31 +982,44 @@ __device__ __forceinline__ float lane_east(float v) {
 
 // One row of a level's inputs at lane column gc: b and, for variable
 // coefficients, the faces west (kx[gc − 1]), east (kx[gc]), north (ky one row
-// up) and south (ky[gc]) of the point, and ω/d.  wload reads them
-// unconditionally from clamped addresses (a uniform row offset plus the lane's
-// 32-bit column); wfix, applied when the row is consumed (not when it is
-// loaded, so that the compiler cannot tie the load to a branch and wait on it
-// at once), puts 0 and 1 outside the grid, as the LDS kernels stage them.
+// up) and south (ky[gc]) of the point, and ω/d.  wload reads b, the east and
+// south faces and ω/d unconditionally from clamped addresses (a uniform row
+// offset plus the lane's 32-bit column): four loads per row.  wtake completes
+// the row when it is consumed: the west face is the east face of lane x − 1
+// and the north face the south face of the row above (the previous step's
+// row, still in the ring).  Lane 0's west face (0) and the north face of a
+// wave's first row (1) are therefore not the grid's, but every value that
+// depends on them already depends on a west neighbour or a row above that
+// the wave does not hold: halo lanes and lag rows, which the wave discards.
+// wfix, applied when the row is consumed (not when it is loaded, so that the
+// compiler cannot tie the load to a branch and wait on it at once), puts 0
+// and 1 outside the grid, as the LDS kernels stage them.
 struct WRow {
 	float b, kw, ke, kn, ks, wd;
 };
 template <bool VK>
 __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols,
-                                      unsigned c, unsigned cw) {
+                                      unsigned c) {
 	const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
 	const size_t o = (size_t)rc * (size_t)cols;
 	WRow v;
 	v.b = (b + o)[c];
 	if constexpr (VK) {
-		const size_t on = (size_t)(rc > 0 ? rc - 1 : 0) * (size_t)cols;
-		v.kw = (L.kx + o)[cw];
 		v.ke = (L.kx + o)[c];
-		v.kn = (L.ky + on)[c];
 		v.ks = (L.ky + o)[c];
 		v.wd = (L.wd + o)[c];
+		v.kw = v.kn = 1.0f;
 	} else {
 		v.kw = v.ke = v.kn = v.ks = 1.0f;
 		v.wd = 0.0f;
+	}
+	return v;
+}
+template <bool VK>
+__device__ __forceinline__ WRow wtake(WRow v, const WRow& above) {
+	if constexpr (VK) {
+		v.kw = lane_west(v.ke);
+		v.kn = above.ks;
 	}
 	return v;
 }
@@ -1051,9 +1060,12 @@ __device__ __forceinline__ float wjac(const LWf& L, const WRow& v, float c, floa
 	if constexpr (VK) return jackw(L, v.kw, v.ke, v.kn, v.ks, v.wd, c, n, s, w, e, v.b);
 	else return jac(L, c, n, s, w, e, v.b);
 }
+// First sweep from u = 0: with finite faces the operator of u = 0 is a zero
+// and (b − A·0) is b up to the sign of a zero, which the final fma with +0
+// removes, so jackw(…, 0, 0, 0, 0, 0, b) is exactly (ω/d)·b + 0.
 template <bool VK>
 __device__ __forceinline__ float wjac0(const LWf& L, const WRow& v) {
-	if constexpr (VK) return jackw(L, v.kw, v.ke, v.kn, v.ks, v.wd, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, v.b);
+	if constexpr (VK) return fmaf(v.wd, v.b, 0.0f);
 	else return jac0(L, v.b);
 }
 template <bool VK>
@@ -1096,26 +1108,30 @@ inline int64_t wave_strips_restrict(int64_t cc) { return (cc - 2 + RCX - 1) / RC
 inline int64_t wave_segs_restrict(int64_t cr, int scr) { return (cr - 2 + scr - 1) / scr; }   // interior coarse rows
 inline unsigned wave_blocks(int64_t nwaves) { return (unsigned)((nwaves + NWAVES - 1) / NWAVES); }
 
-// Runs step(integral_constant<k>, t) for t = t0 … t1 − 1, k = (t − t0) mod WD:
-// whole blocks of WD steps without conditions (so the compiler's memory-wait
-// counts carry across the loop), then the remainder.
-template <class F>
-__device__ __forceinline__ void run_steps(F&& step, int t0, int t1) {
-	auto all = [&](auto kc, int b) { step(kc, b + decltype(kc)::value); };
+// Runs step(integral_constant<k>, t, cl) for t = t0 … t1 − 1, k = (t − t0) mod
+// WD: whole blocks of WD steps without conditions (so the compiler's
+// memory-wait counts carry across the loop), then the remainder.  cl
+// (true_type or false_type) is passed through: a wave whose rows t0 … t1 − 1
+// and columns are all interior runs its steps without the grid-edge fixes.
+// The choice is made once per wave, so that each variant is a loop nest of
+// its own (one loop holding both would need the registers of both).
+template <class F, class CL>
+__device__ __forceinline__ void run_steps(F&& step, int t0, int t1, CL cl) {
+	auto all = [&](int b, auto cl) {
+		step(std::integral_constant<int, 0>{}, b + 0, cl);
+		step(std::integral_constant<int, 1>{}, b + 1, cl);
+		step(std::integral_constant<int, 2>{}, b + 2, cl);
+		step(std::integral_constant<int, 3>{}, b + 3, cl);
+		step(std::integral_constant<int, 4>{}, b + 4, cl);
+		step(std::integral_constant<int, 5>{}, b + 5, cl);
+		step(std::integral_constant<int, 6>{}, b + 6, cl);
+		step(std::integral_constant<int, 7>{}, b + 7, cl);
+	};
 	int b = t0;
-	for (; b + WD <= t1; b += WD) {
-		all(std::integral_constant<int, 0>{}, b);
-		all(std::integral_constant<int, 1>{}, b);
-		all(std::integral_constant<int, 2>{}, b);
-		all(std::integral_constant<int, 3>{}, b);
-		all(std::integral_constant<int, 4>{}, b);
-		all(std::integral_constant<int, 5>{}, b);
-		all(std::integral_constant<int, 6>{}, b);
-		all(std::integral_constant<int, 7>{}, b);
-	}
+	for (; b + WD <= t1; b += WD) all(b, cl);
 	auto one = [&](auto kc) {
 		constexpr int k = decltype(kc)::value;
-		if (b + k < t1) step(kc, b + k);
+		if (b + k < t1) step(kc, b + k, cl);
 	};
 	one(std::integral_constant<int, 0>{});
 	one(std::integral_constant<int, 1>{});
@@ -1153,11 +1169,11 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 		const int strip = gw % nstrips, seg = gw / nstrips;
 		const int gc = strip * SSW - SHX + x;
 		const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < cols ? gc : cols - 1));
-		const unsigned ucw = uc > 0 ? uc - 1 : 0;
 		const bool colin = gc >= 0 && gc < cols;
 		const bool colint = gc >= 1 && gc <= cols - 2;
 		const bool kwok = gc >= 1 && gc <= cols - 1, keok = gc >= 0 && gc <= cols - 2;
 		const bool outlane = colin && x >= SHX && x < SRW - SHX;
+		const bool wclean = strip * SSW - SHX >= 1 && strip * SSW - SHX + SRW - 1 <= cols - 2;  // all lanes interior
 		const int S0 = seg * sh, S1 = S0 + sh < rows ? S0 + sh : rows;
 		// Column interpolation data of this lane.
 		unsigned b0 = 0, b1 = 0;
@@ -1190,63 +1206,74 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 			return epi.load_at((size_t)rc * (size_t)cols, uc);
 		};
 		const int T0 = S0 - LZ, T1 = S1 + LZ;  // steps: rows loaded
-		le.init(T0 - LU, rows, cr);
-		// Input ring: row T0 + j in slot j mod WD; the first WD − LZ rows now, the rest as slots free up.
-		WRow rb[WD];
-		EV evb[WD];
-		typename Epi::Pf pfb[WD];
+		// The wave's whole walk, instantiated once with and once without the
+		// grid-edge fixes (cl), so that each variant gets its own registers.
+		auto walk = [&](auto cl) {
+			le.init(T0 - LU, rows, cr);
+			// Input ring: row T0 + j in slot j mod WD; the first WD − LZ rows now, the rest as slots free up.
+			WRow rb[WD];
+			EV evb[WD];
+			typename Epi::Pf pfb[WD];
 #pragma unroll
-		for (int k = 0; k < WD; ++k) {
-			rb[k] = k < WD - LZ ? wload<VK>(b, L, T0 + k, rows, cols, uc, ucw) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
-			evb[k] = eload();
-			pfb[k] = pload(T0 + k - LZ);
-		}
-		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, U[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, S[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
-		auto step = [&](auto kc, int t) {
-			constexpr int k = decltype(kc)::value, q = k % WR;
-			rb[k] = wfix<VK>(rb[k], t, rows, colin, kwok, keok);
-			// Inputs of row t − i: rb[Slot<WD, k, i>].
-			float u;  // iterate at row t − LU
-			if constexpr (NPRE == 2) {
-				const float u1 = wjac0<VK>(L, rb[k]);
-				U1[q] = inter(t) ? u1 : 0.0f;
-				const float c = U1[Slot<WR, q, 1>::v];
+			for (int k = 0; k < WD; ++k) {
+				rb[k] = k < WD - LZ ? wload<VK>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+				evb[k] = eload();
+				pfb[k] = pload(T0 + k - LZ);
+			}
+			float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, U[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, S[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
+			auto step = [&](auto kc, int t, auto cl) {
+				constexpr int k = decltype(kc)::value, q = k % WR;
+				constexpr bool CL = decltype(cl)::value;
+				auto in = [&](int r) { return CL || inter(r); };
+				{
+					const WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+					rb[k] = CL ? v : wfix<VK>(v, t, rows, colin, kwok, keok);
+				}
+				// Inputs of row t − i: rb[Slot<WD, k, i>].
+				float u;  // iterate at row t − LU
+				if constexpr (NPRE == 2) {
+					const float u1 = wjac0<VK>(L, rb[k]);
+					U1[q] = in(t) ? u1 : 0.0f;
+					const float c = U1[Slot<WR, q, 1>::v];
+					const float wv = lane_west(c), evv = lane_east(c);
+					u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+				} else {
+					u = wjac0<VK>(L, rb[k]);
+				}
+				{
+					const EV& ev = evb[k];
+					const float wy = ev.wy, vy = 1.0f - wy;
+					const float top = vx * ev.e00 + wx * ev.e01;
+					const float bot = vx * ev.e10 + wx * ev.e11;
+					u += vy * top + wy * bot;
+				}
+				U[q] = in(t - LU) ? u : 0.0f;
+				const float* C = U;
+				if constexpr (NPOST == 2) {
+					const float c = U[Slot<WR, q, 1>::v];
+					const float wv = lane_west(c), evv = lane_east(c);
+					const float sv = wjac<VK>(L, rb[Slot<WD, k, LS>::v], c, U[Slot<WR, q, 2>::v], U[q], wv, evv);
+					S[q] = in(t - LS) ? sv : c;
+					C = S;
+				}
+				const float c = C[Slot<WR, q, 1>::v];
 				const float wv = lane_west(c), evv = lane_east(c);
-				u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
-			} else {
-				u = wjac0<VK>(L, rb[k]);
-			}
-			{
-				const EV& ev = evb[k];
-				const float wy = ev.wy, vy = 1.0f - wy;
-				const float top = vx * ev.e00 + wx * ev.e01;
-				const float bot = vx * ev.e10 + wx * ev.e11;
-				u += vy * top + wy * bot;
-			}
-			U[q] = inter(t - LU) ? u : 0.0f;
-			const float* C = U;
-			if constexpr (NPOST == 2) {
-				const float c = U[Slot<WR, q, 1>::v];
-				const float wv = lane_west(c), evv = lane_east(c);
-				const float sv = wjac<VK>(L, rb[Slot<WD, k, LS>::v], c, U[Slot<WR, q, 2>::v], U[q], wv, evv);
-				S[q] = inter(t - LS) ? sv : c;
-				C = S;
-			}
-			const float c = C[Slot<WR, q, 1>::v];
-			const float wv = lane_west(c), evv = lane_east(c);
-			const float zv = wjac<VK>(L, rb[Slot<WD, k, LZ>::v], c, C[Slot<WR, q, 2>::v], C[q], wv, evv);
-			const int rz = t - LZ;
-			const float z = inter(rz) ? zv : c;
-			if (outlane && rz >= S0 && rz < S1) {
-				(out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
-				epi.use_lane(pfb[k], z, lsum);
-			}
-			// Refill: the slot of row t − LZ (no longer read) with row t + WD − LZ, and this step's operands.
-			rb[Slot<WD, k, LZ>::v] = wload<VK>(b, L, t + WD - LZ, rows, cols, uc, ucw);
-			evb[k] = eload();
-			pfb[k] = pload(t + WD - LZ);
+				const float zv = wjac<VK>(L, rb[Slot<WD, k, LZ>::v], c, C[Slot<WR, q, 2>::v], C[q], wv, evv);
+				const int rz = t - LZ;
+				const float z = in(rz) ? zv : c;
+				if (outlane && rz >= S0 && rz < S1) {
+					(out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
+					epi.use_lane(pfb[k], z, lsum);
+				}
+				// Refill: the slot of row t − LZ (no longer read) with row t + WD − LZ, and this step's operands.
+				rb[Slot<WD, k, LZ>::v] = wload<VK>(b, L, t + WD - LZ, rows, cols, uc);
+				evb[k] = eload();
+				pfb[k] = pload(t + WD - LZ);
 		};
-		run_steps(step, T0, T1);
+		run_steps(step, T0, T1, cl);
+		};
+		if (wclean && T0 >= 1 && T1 <= rows - 1) walk(std::true_type{});
+		else walk(std::false_type{});
 	}
 	if constexpr (Epi::NS > 0) {
 		Acc acc[NA];
@@ -1307,8 +1334,8 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	const int C0 = G0 - 2;  // fine column of lane 0
 	const int gc = C0 + x;
 	const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < fc ? gc : fc - 1));
-	const unsigned ucw = uc > 0 ? uc - 1 : 0;
 	const bool colin = gc >= 0 && gc < fc;
+	const bool wclean = C0 >= 1 && C0 + SRW - 1 <= fc - 2;  // all lanes interior
 	const bool colint = gc >= 1 && gc <= fc - 2;
 	const bool kwok = gc >= 1 && gc <= fc - 1, keok = gc >= 0 && gc <= fc - 2;
 	auto inter = [&](int r) { return colint && r >= 1 && r <= fr - 2; };
@@ -1328,6 +1355,12 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 			wk[k] = ac == J ? float(1.0 - wv) : (ac + 1 == J ? float(wv) : 0.0f);
 			sk[k] = v ? jj - C0 : 0;
 		}
+		// Unused terms read the first gathered column with weight 0: a finite
+		// value times 0 is a zero, and adding a zero to the row sum (which
+		// starts at +0 and so is never −0) leaves it unchanged.
+#pragma unroll
+		for (int k = 1; k < WNMAX; ++k)
+			if (k >= cnt) sk[k] = sk[0];
 	}
 	const float scale = X.sy * X.sx;
 	int Icur = X.i0r[F0];  // coarse row accumulated in alo (ahi: Icur + 1)
@@ -1337,61 +1370,69 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	};
 	const int T0 = F0 - LR, T1 = F1 + LR + 1;  // steps: rows loaded
 	const double rinv = 1.0 / double(fr - 1);
-	LinRow lr;  // fine row whose residual the step produces
-	lr.init(T0 - LR, fr, cr);
-	WRow rb[WD];
+	// The wave's whole walk, instantiated once with and once without the
+	// grid-edge fixes (cl), so that each variant gets its own registers.
+	auto walk = [&](auto cl) {
+		LinRow lr;  // fine row whose residual the step produces
+		lr.init(T0 - LR, fr, cr);
+		WRow rb[WD];
 #pragma unroll
-	for (int k = 0; k < WD; ++k)
-		rb[k] = k < WD - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc, ucw) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
-	float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
-	auto step = [&](auto kc, int t) {
-		constexpr int k = decltype(kc)::value, q = k % WR;
-		rb[k] = wfix<VK>(rb[k], t, fr, colin, kwok, keok);
-		float u;  // pre-smoothed iterate at row t − LU
-		if constexpr (NPRE == 2) {
-			const float u1 = wjac0<VK>(L, rb[k]);
-			U1[q] = inter(t) ? u1 : 0.0f;
-			const float c = U1[Slot<WR, q, 1>::v];
-			const float wv = lane_west(c), evv = lane_east(c);
-			u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
-		} else {
-			u = wjac0<VK>(L, rb[k]);
-		}
-		Uv[q] = inter(t - LU) ? u : 0.0f;
-		// Residual at fine row i = t − LR.
-		const int i = t - LR;
-		const float c = Uv[Slot<WR, q, 1>::v];
-		const float wv = lane_west(c), evv = lane_east(c);
-		const WRow& ri = rb[Slot<WD, k, LR>::v];
-		const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
-		const float res = inter(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
-		// Refill the slot of row t − LR (no longer read) with row t + WD − LR.
-		rb[Slot<WD, k, LR>::v] = wload<VK>(b, L, t + WD - LR, fr, fc, uc, ucw);
-		// Row sum over the owned coarse column's fine columns.
-		float row = 0.0f;
-#pragma unroll
-		for (int k2 = 0; k2 < WNMAX; ++k2) {
-			const float v = __shfl(res, sk[k2]);
-			if (k2 < cnt) row += wk[k2] * v;
-		}
-		const int a = lr.i0;
-		const double wrv = lr.w(rinv);
-		lr.next();
-		if (i >= F0 && i <= F1) {
-			// Coarse rows advance by at most one per fine row (nc ≤ nf).
-			if (Icur < a) {
-				emit(Icur, alo);
-				alo = ahi;
-				ahi = 0.0f;
-				++Icur;
+		for (int k = 0; k < WD; ++k)
+			rb[k] = k < WD - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
+		auto step = [&](auto kc, int t, auto cl) {
+			constexpr int k = decltype(kc)::value, q = k % WR;
+			constexpr bool CL = decltype(cl)::value;
+			auto in = [&](int r) { return CL || inter(r); };
+			{
+				const WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+				rb[k] = CL ? v : wfix<VK>(v, t, fr, colin, kwok, keok);
 			}
-			alo += float(1.0 - wrv) * row;
-			ahi += float(wrv) * row;
-		}
+			float u;  // pre-smoothed iterate at row t − LU
+			if constexpr (NPRE == 2) {
+				const float u1 = wjac0<VK>(L, rb[k]);
+				U1[q] = in(t) ? u1 : 0.0f;
+				const float c = U1[Slot<WR, q, 1>::v];
+				const float wv = lane_west(c), evv = lane_east(c);
+				u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+			} else {
+				u = wjac0<VK>(L, rb[k]);
+			}
+			Uv[q] = in(t - LU) ? u : 0.0f;
+			// Residual at fine row i = t − LR.
+			const int i = t - LR;
+			const float c = Uv[Slot<WR, q, 1>::v];
+			const float wv = lane_west(c), evv = lane_east(c);
+			const WRow& ri = rb[Slot<WD, k, LR>::v];
+			const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
+			const float res = in(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
+			// Refill the slot of row t − LR (no longer read) with row t + WD − LR.
+			rb[Slot<WD, k, LR>::v] = wload<VK>(b, L, t + WD - LR, fr, fc, uc);
+			// Row sum over the owned coarse column's fine columns.
+			float row = 0.0f;
+#pragma unroll
+			for (int k2 = 0; k2 < WNMAX; ++k2) row += wk[k2] * __shfl(res, sk[k2]);
+			const int a = lr.i0;
+			const double wrv = lr.w(rinv);
+			lr.next();
+			if (i >= F0 && i <= F1) {
+				// Coarse rows advance by at most one per fine row (nc ≤ nf).
+				if (Icur < a) {
+					emit(Icur, alo);
+					alo = ahi;
+					ahi = 0.0f;
+					++Icur;
+				}
+				alo += float(1.0 - wrv) * row;
+				ahi += float(wrv) * row;
+			}
 	};
-	run_steps(step, T0, T1);
+	run_steps(step, T0, T1, cl);
 	emit(Icur, alo);
 	emit(Icur + 1, ahi);
+	};
+	if (wclean && T0 >= 1 && T1 <= fr - 1) walk(std::true_type{});
+	else walk(std::false_type{});
 }
 
 // ── k_ftail: the V-cycle below a small level, in one workgroup ─────────
