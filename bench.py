@@ -47,13 +47,15 @@ def log(msg):
 def kernel_bytes(vk: bool) -> dict:
     """Algorithmic HBM bytes per grid point of each probed kernel (each array once at its dtype)."""
     return {
-        # x, p, r, ap read (FP64); x, r written (FP64); FP32 copy of r written.
-        "fcg_update": 4 * 8 + 2 * 8 + 4,
-        # z (FP32) and p_old read, p_new and A·p written (FP64); k read for the var-k operator.
-        "fcg_step": 4 + 8 + 8 + 8 + (8 if vk else 0),
-        # level-0 rhs (FP32), coarse correction (¼ point, FP32) and z (FP32); r, A·p (FP64) for the dots;
-        # the level's FP32 face coefficients kx, ky and ω/d for the var-k operator.
-        "vc_post0": 4 + 1 + 4 + 8 + 8 + (12 if vk else 0),
+        # r, ap read (FP64); r written (FP64); FP32 copy of r written (the x update is deferred
+        # to the next step kernel).
+        "fcg_update": 2 * 8 + 8 + 4,
+        # z (FP32), p_old and x read; p_new, A·p and x (the previous iteration's deferred
+        # x += α·p_old) written (FP64); k read for the var-k operator.
+        "fcg_step": 4 + 8 + 8 + 8 + 8 + 8 + (8 if vk else 0),
+        # coarse correction (¼ point, FP32) read and z (FP32) written; r, A·p (FP64) read for the
+        # dots (the rhs is float(r)); the level's FP32 face coefficients kx, ky and ω/d for var-k.
+        "vc_post0": 1 + 4 + 8 + 8 + (12 if vk else 0),
         # level-0 rhs (FP32), face coefficients and ω/d read, coarse rhs (¼ point, FP32) written.
         "vc_restrict0": 4 + 1 + (12 if vk else 0),
         "stencil_axpy": 24,

@@ -344,6 +344,7 @@ static_assert(PTX == WAVE, "one tile column per lane in the final sweep");
 
 struct NoEpi {
 	static constexpr int NS = 0;
+	static constexpr bool RHS_FROM_R = false;  // the wave kernel reads the level-0 rhs itself
 	struct Pf {};
 	__device__ void term(int64_t, float, Acc*) const {}
 	__device__ Pf load(int64_t) const { return Pf{}; }
@@ -997,13 +998,13 @@ __device__ __forceinline__ float lane_east(float v) {
 struct WRow {
 	float b, kw, ke, kn, ks, wd;
 };
-template <bool VK>
+template <bool VK, bool LOADB = true>
 __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols,
                                       unsigned c) {
 	const int rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
 	const size_t o = (size_t)rc * (size_t)cols;
 	WRow v;
-	v.b = (b + o)[c];
+	v.b = LOADB ? (b + o)[c] : 0.0f;
 	if constexpr (VK) {
 		v.ke = (L.kx + o)[c];
 		v.ks = (L.ky + o)[c];
@@ -1159,6 +1160,9 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 	constexpr int LZ = LU + NPOST;   // the level's result
 	static_assert(LZ <= SHX && LZ < WR && LZ < WD, "halo or rings too small");
 	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
+	// With Epi::RHS_FROM_R the rhs is float(r) of the epilogue's FP64 residual
+	// (which it loads anyway, for the same row and column), not a load of b.
+	constexpr bool LOADB = !Epi::RHS_FROM_R;
 	double lsum[NA];
 #pragma unroll
 	for (int k = 0; k < NA; ++k) lsum[k] = 0.0;
@@ -1216,7 +1220,7 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 			typename Epi::Pf pfb[WD];
 #pragma unroll
 			for (int k = 0; k < WD; ++k) {
-				rb[k] = k < WD - LZ ? wload<VK>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+				rb[k] = k < WD - LZ ? wload<VK, LOADB>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
 				evb[k] = eload();
 				pfb[k] = pload(T0 + k - LZ);
 			}
@@ -1226,7 +1230,9 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				constexpr bool CL = decltype(cl)::value;
 				auto in = [&](int r) { return CL || inter(r); };
 				{
-					const WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+					WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+					// Row t's residual was loaded with the row: it is in the epilogue slot LZ steps on.
+					if constexpr (!LOADB) v.b = epi.rhs(pfb[Slot<WD, k, -LZ>::v]);
 					rb[k] = CL ? v : wfix<VK>(v, t, rows, colin, kwok, keok);
 				}
 				// Inputs of row t − i: rb[Slot<WD, k, i>].
@@ -1266,7 +1272,7 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 					epi.use_lane(pfb[k], z, lsum);
 				}
 				// Refill: the slot of row t − LZ (no longer read) with row t + WD − LZ, and this step's operands.
-				rb[Slot<WD, k, LZ>::v] = wload<VK>(b, L, t + WD - LZ, rows, cols, uc);
+				rb[Slot<WD, k, LZ>::v] = wload<VK, LOADB>(b, L, t + WD - LZ, rows, cols, uc);
 				evb[k] = eload();
 				pfb[k] = pload(t + WD - LZ);
 		};

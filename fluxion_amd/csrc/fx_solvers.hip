@@ -59,6 +59,10 @@ struct SState {
 	// Host-visible copy of `done` (coherent pinned memory), set together with it so that the host's
 	// enqueue loop can stop without a per-iteration device-to-host copy.
 	int* hdone;
+	// Mixed FCG with the x update deferred to the next step kernel: xpend = 1 while x += alpha·xp
+	// (the last x/r update's direction) is still to be applied.
+	int32_t xpend, pad4;
+	const double* xp;
 };
 
 __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
@@ -316,7 +320,9 @@ __device__ __forceinline__ double dlane(double v) {
 
 // FCG step of the fused loop on grids with an even column count, two columns
 // per thread (16-byte loads and stores): p = double(z) + β·p_old, ap = A·p
-// (identity boundary rows), p·ap.  A is the 5-point stencil operator
+// (identity boundary rows), p·ap; with st->xpend it also applies the previous
+// iteration's deferred x += α·p_old (the x/r update then reads neither x nor
+// p), the same fma on the same operands, only later.  A is the 5-point stencil operator
 // p + coeff·S(p) (chain order of fxd::chain5) or, with VK, the
 // variable-coefficient operator of k_stream_vk.  The west and east neighbours
 // of a thread's pair come from the adjacent lanes (DPP); only the first and
@@ -329,9 +335,11 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
                                                      double* __restrict__ pnew, double* __restrict__ ap, W9 w,
                                                      double coeff, const double* __restrict__ k, double sigma,
                                                      double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
-                                                     RedScratch rs, const SState* st) {
+                                                     RedScratch rs, const SState* st, double* __restrict__ x) {
 	if (st->done) return;
 	const double be = st->beta;
+	const bool xu = st->xpend != 0;
+	const double al = st->alpha;
 	int bx, by;
 	xcd_tile(&bx, &by);
 	const int lane = (int)(threadIdx.x & (WAVE - 1));
@@ -345,12 +353,15 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 	struct Row {
 		double2 p, k;
 		double pw, pe, kw, ke;   // p and k west of the pair (first lane) and east of it (last lane)
+		double2 x, xn;           // owned rows with a deferred x update: x and x + α·p_old
 	};
-	auto ld = [&](int64_t r) {
+	auto ld = [&](int64_t r, bool own) {
 		Row v;
 		const int64_t i = r * cols + cl;
 		const float2 zz = *(const float2*)(z + i);
 		const double2 pp = *(const double2*)(pold + i);
+		v.x = xu && own ? *(const double2*)(x + i) : double2{0.0, 0.0};
+		v.xn = double2{fma(al, pp.x, v.x.x), fma(al, pp.y, v.x.y)};
 		v.p = double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
 		v.k = VK ? *(const double2*)(k + i) : double2{1.0, 1.0};
 		v.pw = v.pe = 0.0;
@@ -365,13 +376,13 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 		}
 		return v;
 	};
-	const Row none{double2{0.0, 0.0}, double2{1.0, 1.0}, 0.0, 0.0, 1.0, 1.0};
+	const Row none{double2{0.0, 0.0}, double2{1.0, 1.0}, 0.0, 0.0, 1.0, 1.0, double2{0.0, 0.0}, double2{0.0, 0.0}};
 	Acc acc[1] = {Acc{0.0, 0.0}};
-	Row up = r0 > 0 ? ld(r0 - 1) : none;
-	Row cur = ld(r0 < rows ? r0 : rows - 1);
+	Row up = r0 > 0 ? ld(r0 - 1, false) : none;
+	Row cur = ld(r0 < rows ? r0 : rows - 1, true);
 	for (int64_t r = r0; r < r1; ++r) {
 		const int64_t i = r * cols + c0;
-		const Row dn = r + 1 < rows ? ld(r + 1) : none;
+		const Row dn = r + 1 < rows ? ld(r + 1, r + 1 < r1) : none;
 		double pw = dlane<0x138>(cur.p.y), pe = dlane<0x130>(cur.p.x);
 		double kw = dlane<0x138>(cur.k.y), ke = dlane<0x130>(cur.k.x);
 		if (lane == 0) pw = cur.pw, kw = cur.kw;
@@ -401,6 +412,7 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 		if (ok) {
 			*(double2*)(pnew + i) = cur.p;
 			*(double2*)(ap + i) = av;
+			if (xu) *(double2*)(x + i) = cur.xn;
 			acc_add(acc[0], cur.p.x * av.x);
 			acc_add(acc[0], cur.p.y * av.y);
 		}
@@ -410,23 +422,20 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 	block_partial<1>(acc, rs);
 }
 
-// x += α·p; r −= α·ap with r·r, and the FP32 copy of r; two points per thread
-// per step (n even).  Per-point arithmetic equals FXR32.
-__global__ __launch_bounds__(BLOCK) void k_fxr32v(const SState* st, double* __restrict__ x, double* __restrict__ r,
-                                                  float* __restrict__ r32, const double* __restrict__ p,
-                                                  const double* __restrict__ ap, size_t n2, RedScratch rs,
-                                                  const int* skip) {
+// r −= α·ap with r·r, and the FP32 copy of r, two points per thread per step
+// (n even); the x update of the same iteration is deferred to the next FCG
+// step kernel (or k_xflush).  Per-point arithmetic equals FXR32's r part.
+__global__ __launch_bounds__(BLOCK) void k_fr32v(const SState* st, double* __restrict__ r, float* __restrict__ r32,
+                                                 const double* __restrict__ ap, size_t n2, RedScratch rs,
+                                                 const int* skip) {
 	if (skip && *skip) return;
 	const double al = st->alpha;
 	Acc acc[1] = {Acc{0.0, 0.0}};
 	const size_t stride = (size_t)gridDim.x * BLOCK;
 #pragma unroll 4
 	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n2; j += stride) {
-		const double2 xv = ((const double2*)x)[j], pv = ((const double2*)p)[j];
 		const double2 rv = ((const double2*)r)[j], av = ((const double2*)ap)[j];
-		const double2 xn{fma(al, pv.x, xv.x), fma(al, pv.y, xv.y)};
 		const double2 rn{fma(-al, av.x, rv.x), fma(-al, av.y, rv.y)};
-		((double2*)x)[j] = xn;
 		((double2*)r)[j] = rn;
 		((float2*)r32)[j] = float2{float(rn.x), float(rn.y)};
 		acc_add(acc[0], rn.x * rn.x);
@@ -492,6 +501,33 @@ struct FinPcgRR {
 		else if (sqrt(rr) < st->tol) finish(st, it, sqrt(rr), FX_CONVERGED);
 	}
 };
+// FinPcgRR after an r-only update whose x update (x += α·p) is deferred: the
+// next step kernel applies it, or k_xflush once the loop has stopped.
+struct FinPcgRRDefer {
+	SState* st;
+	const double* p;
+	__device__ void operator()(Acc* o) const {
+		FinPcgRR{st}(o);
+		st->xpend = 1;
+		st->xp = p;
+	}
+};
+// FinAlpha after a step kernel that applied the deferred x update.
+struct FinAlphaDefer {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		st->xpend = 0;
+		FinAlpha{st}(o);
+	}
+};
+// The deferred x update left when the loop stopped: x += α·xp (if any).
+__global__ __launch_bounds__(BLOCK) void k_xflush(const SState* st, double* __restrict__ x, size_t n) {
+	if (!st->xpend) return;
+	const double al = st->alpha;
+	const double* p = st->xp;
+	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n; j += (size_t)gridDim.x * BLOCK)
+		x[j] = fma(al, p[j], x[j]);
+}
 // PCG init: r = b − Ax with r·r (no check yet: cg.rs:188-202 computes z first).
 struct FinPcgInitRR {
 	SState* st;
@@ -752,8 +788,12 @@ struct FXR32 {
 	}
 };
 // Level-0 epilogues of the fused V-cycle: r·z (start) and (r·z, z·Ap) (iterations).
+// RHS_FROM_R: the level-0 rhs of the V-cycle these epilogues end is float(r)
+// (written by the residual and x/r-update kernels), so the wave post kernel
+// takes it from the residual it loads for the reduction.
 struct EpiRZ {
 	static constexpr int NS = 1;
+	static constexpr bool RHS_FROM_R = true;
 	const double* r;
 	SState* st;
 	__device__ void term(int64_t j, float z, Acc* a) const { acc_add(a[0], r[j] * double(z)); }
@@ -765,11 +805,13 @@ struct EpiRZ {
 	// Row-offset form (uniform row offset o, lane column c) and plain per-lane sums (wave kernels).
 	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c]}; }
 	__device__ void use_lane(const Pf& p, float z, double* s) const { s[0] += p.r * double(z); }
+	__device__ float rhs(const Pf& p) const { return float(p.r); }
 	using Fin = FinPcgInitRZ;
 	Fin fin() const { return Fin{st}; }
 };
 struct EpiFcg {
 	static constexpr int NS = 2;
+	static constexpr bool RHS_FROM_R = true;
 	const double *r, *ap;
 	SState* st;
 	__device__ void term(int64_t j, float z, Acc* a) const {
@@ -792,6 +834,7 @@ struct EpiFcg {
 		s[0] += p.r * zd;
 		s[1] += zd * p.ap;
 	}
+	__device__ float rhs(const Pf& p) const { return float(p.r); }
 	using Fin = FinFcgBeta;
 	Fin fin() const { return Fin{st}; }
 };
@@ -1436,25 +1479,25 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 					if (vk)
 						hipLaunchKernelGGL(k_fcg_step2<true>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
 						                   c.op->coeff, kp, c.op->sigma, ix2, iy2, c.rows, c.cols, rpb,
-						                   fx::red_scratch(c.be), (const SState*)c.d);
+						                   fx::red_scratch(c.be), (const SState*)c.d, x);
 					else
 						hipLaunchKernelGGL(k_fcg_step2<false>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
 						                   c.op->coeff, kp, 0.0, ix2, iy2, c.rows, c.cols, rpb, fx::red_scratch(c.be),
-						                   (const SState*)c.d);
+						                   (const SState*)c.d, x);
 					fx::probe_end(c.be, FX_PROBE_FCG_STEP);
-					hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
-					                   grid.x * grid.y, FinAlpha{c.d}, c.done);
+					hipLaunchKernelGGL((k_finish<1, FinAlphaDefer>), dim3(1), dim3(FIN_BLOCK), 0, c.st,
+					                   fx::red_scratch(c.be), grid.x * grid.y, FinAlphaDefer{c.d}, c.done);
 					FX_CHECK(check_launch("fcg fused step"));
 					const size_t n2 = c.n / 2;
 					// About four pairs per thread, unrolled, so each thread keeps several loads in flight.
 					const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>(16384, (n2 + 4 * BLOCK - 1) / (4 * BLOCK)));
 					FX_CHECK(fx::ensure_partials(c.be, nb));
 					fx::probe_begin(c.be, FX_PROBE_FCG_UPDATE);
-					hipLaunchKernelGGL(k_fxr32v, dim3(nb), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, r, r32,
-					                   (const double*)pnew, (const double*)ap, n2, fx::red_scratch(c.be), c.skip);
+					hipLaunchKernelGGL(k_fr32v, dim3(nb), dim3(BLOCK), 0, c.st, (const SState*)c.d, r, r32,
+					                   (const double*)ap, n2, fx::red_scratch(c.be), c.skip);
 					fx::probe_end(c.be, FX_PROBE_FCG_UPDATE);
-					hipLaunchKernelGGL((k_finish<1, FinPcgRR>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
-					                   nb, FinPcgRR{c.d}, c.skip);
+					hipLaunchKernelGGL((k_finish<1, FinPcgRRDefer>), dim3(1), dim3(FIN_BLOCK), 0, c.st,
+					                   fx::red_scratch(c.be), nb, FinPcgRRDefer{c.d, pnew}, c.skip);
 					FX_CHECK(check_launch("fcg x/r update"));
 					return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
 				}
@@ -1476,6 +1519,8 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 				FX_CHECK(c.ered<1>(FXR32{c.d, x, r, r32, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
 				return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
 			}, (int64_t)(s->max_iter - used)));
+			hipLaunchKernelGGL(k_xflush, dim3(fx::red_blocks(c.n)), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, c.n);
+			FX_CHECK(check_launch("fcg deferred x update"));
 		} else {
 			FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "fcg init", nullptr));
 			FX_CHECK(c.precond(z, r));
