@@ -63,6 +63,9 @@ struct SState {
 	// (the last x/r update's direction) is still to be applied.
 	int32_t xpend, pad4;
 	const double* xp;
+	// Host-visible iteration counter (next to hdone): FinPcgRR stores the iteration count once the
+	// iteration's x/r update is done, so the enqueue loop can follow the device without events.
+	int* hprog;
 };
 
 __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
@@ -499,6 +502,8 @@ struct FinPcgRR {
 		record(st, it, sqrt(rr));
 		if (it >= st->max_iter) finish(st, it, sqrt(rr), FX_MAX_ITERATIONS);
 		else if (sqrt(rr) < st->tol) finish(st, it, sqrt(rr), FX_CONVERGED);
+		// Release: a host that sees the count also sees the done flag stored above.
+		if (st->hprog) __hip_atomic_store(st->hprog, (int)it, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 	}
 };
 // FinPcgRR after an r-only update whose x update (x += α·p) is deferred: the
@@ -1145,7 +1150,7 @@ struct fx_solver {
 	SState* hst = nullptr;       // pinned host copy
 	double* dhist = nullptr;
 	double* dgm = nullptr;       // GMRES H, cc, ss, grs, y
-	int* hdone = nullptr;        // coherent pinned copy of the device `done` flag
+	int* hdone = nullptr;        // coherent pinned host ints: [0] copy of the device `done` flag, [1] iteration count
 	hipEvent_t ev[8] = {};
 	std::vector<double> hist;
 	double* vec(int i) {
@@ -1328,9 +1333,11 @@ fx_status reset_state(fx_solver* s, int64_t max_iter, const fx_precond* pc = nul
 	h.tol = s->tol;
 	h.max_iter = max_iter;
 	__atomic_store_n(s->hdone, 0, __ATOMIC_RELEASE);
+	__atomic_store_n(s->hdone + 1, 0, __ATOMIC_RELEASE);
 	void* hd = nullptr;
 	FX_HIP(hipHostGetDevicePointer(&hd, s->hdone, 0));
 	h.hdone = (int*)hd;
+	h.hprog = (int*)hd + 1;
 	h.hist = s->dhist;
 	h.hist_len = HIST_MAX;
 	h.m = (int32_t)s->restart;
@@ -1362,6 +1369,35 @@ fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
 			FX_HIP(hipEventSynchronize(s->ev[(i - look) % RING]));
 			if (__atomic_load_n(s->hdone, __ATOMIC_ACQUIRE)) break;
 		}
+	}
+	FX_HIP(hipStreamSynchronize(st));
+	return FX_OK;
+}
+
+// drive() for loops whose iteration finisher is FinPcgRR (or FinPcgRRDefer):
+// after enqueuing step i it waits, without stream events (each of which
+// stalls the queue for a few microseconds), until step i's x/r update has
+// stored the iteration count i + 1 in host memory.  The rest of step i (the
+// preconditioner) is still queued then, so the device does not run dry while
+// step i + 1 is enqueued, and once the solve is done no further step has been
+// enqueued.  An emptied stream (an error, or a step that ends early) also ends
+// the wait.
+template <class Step>
+fx_status drive_counted(fx_solver* s, Step step, int64_t max_steps) {
+	hipStream_t st = s->be->stream;
+	const volatile int* done = s->hdone;
+	const volatile int* prog = s->hdone + 1;
+	for (int64_t i = 0; i <= max_steps + 2; ++i) {
+		FX_CHECK(step(i));
+		for (unsigned spin = 0;; ++spin) {
+			if (__atomic_load_n(prog, __ATOMIC_ACQUIRE) >= i + 1 || __atomic_load_n(done, __ATOMIC_ACQUIRE)) break;
+			if ((spin & 63) == 63) {
+				const hipError_t q = hipStreamQuery(st);
+				if (q == hipSuccess) break;
+				FX_REQUIRE(q == hipErrorNotReady, FX_ERR_HIP, "solver stream failed");
+			}
+		}
+		if (__atomic_load_n(done, __ATOMIC_ACQUIRE)) break;
 	}
 	FX_HIP(hipStreamSynchronize(st));
 	return FX_OK;
@@ -1462,7 +1498,7 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 			FX_CHECK(c.ered<1>(FResid32{b, ap, r, r32}, FinPcgInitRR{c.d}, "fcg init", nullptr));
 			FX_CHECK(fmg::fast_cycle(fm, c.skip, EpiRZ{r, c.d}));
 			const bool even = c.cols % 2 == 0;
-			FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+			FX_CHECK(drive_counted(s, [&](int64_t i) -> fx_status {
 				double* pold = pp[i & 1];
 				double* pnew = pp[(i + 1) & 1];
 				if (even) {
@@ -2062,7 +2098,7 @@ fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, ui
 	if ((st = dev_alloc(be, sizeof(double) * HIST_MAX, &p)) != FX_OK) return fail(st);
 	s->dhist = (double*)p;
 	if (hipHostMalloc((void**)&s->hst, sizeof(SState), hipHostMallocDefault) != hipSuccess) return fail(FX_ERR_ALLOC);
-	if (hipHostMalloc((void**)&s->hdone, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+	if (hipHostMalloc((void**)&s->hdone, 2 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
 		return fail(FX_ERR_ALLOC);
 	for (auto& e : s->ev)
 		if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(FX_ERR_HIP);
