@@ -296,6 +296,52 @@ __global__ __launch_bounds__(BLOCK) void k_stream_vk(V v, E e, const double* __r
 	stream_vk_body<NS>(bx, by, v, e, k, sigma, alpha, ix2, iy2, rows, cols, rpb, acc);
 	block_partial<NS>(acc, rs);
 }
+// r = b − A·x for the variable-coefficient operator in one pass over x, k and
+// b (the arithmetic of fx_ops.hip k_vark, identity boundary), with r·r; E
+// receives each point's b − A·x.  Replaces the operator kernel plus the
+// residual reduction, and the temporary A·x they pass between them.
+struct VX {
+	const double* x;
+	double be;
+	__device__ double load(int64_t i) const { return x[i]; }
+};
+template <class E>
+__global__ __launch_bounds__(BLOCK) void k_resid_vk(VX v, E e, const double* __restrict__ k, double sigma,
+                                                    double alpha, double ix2, double iy2, int64_t rows, int64_t cols,
+                                                    int rpb, RedScratch rs) {
+	int bx, by;
+	xcd_tile(&bx, &by);
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	stream_vk_body<1>(bx, by, v, e, k, sigma, alpha, ix2, iy2, rows, cols, rpb, acc);
+	block_partial<1>(acc, rs);
+}
+// Writes r and its FP32 copy (FResid32's per-point work).
+struct EpiRes32 {
+	const double* b;
+	double* r;
+	float* r32;
+	__device__ void put(int64_t i, double av, Acc* a) const {
+		const double v = b[i] - av;
+		r[i] = v;
+		r32[i] = float(v);
+		acc_add(a[0], v * v);
+	}
+	__device__ void interior(int64_t i, double av, double, Acc* a) const { put(i, av, a); }
+	__device__ void boundary(int64_t i, double c, Acc* a) const { put(i, c, a); }
+};
+// Only ‖b − A·x‖² (the true-residual check).
+struct EpiResNorm {
+	const double* b;
+	__device__ void interior(int64_t i, double av, double, Acc* a) const {
+		const double v = b[i] - av;
+		acc_add(a[0], v * v);
+	}
+	__device__ void boundary(int64_t i, double c, Acc* a) const {
+		const double v = b[i] - c;
+		acc_add(a[0], v * v);
+	}
+};
+
 // ap = A·p inside (operator value given), ap = p on the boundary; writes p_new; p·ap.
 struct EpiCGv {
 	double *pout, *ap;
@@ -1238,6 +1284,19 @@ struct Ctx {
 		return fx::k_apply_operator(be, op, x, y, &g, sk);
 	}
 	fx_status apply_op(const double* x, double* y) { return apply_op(x, y, skip); }
+	// r = b − A·x in one pass (variable-coefficient operator), then fin on r·r.
+	template <class E, class Fin>
+	fx_status resid_vk(const double* x, E e, Fin fin, const char* what) {
+		const int rpb = stream_rpb(rows, cols);
+		const dim3 grid = fx::stream_grid(rows, cols, rpb);
+		FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
+		hipLaunchKernelGGL((k_resid_vk<E>), grid, dim3(BLOCK), 0, st, VX{x, 0.0}, e, (const double*)op->k->ptr,
+		                   op->sigma, op->coeff, 1.0 / (s->g.dx * s->g.dx), 1.0 / (s->g.dy * s->g.dy), rows, cols, rpb,
+		                   fx::red_scratch(be));
+		hipLaunchKernelGGL((k_finish<1, Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be),
+		                   grid.x * grid.y, fin, nullptr);
+		return check_launch(what);
+	}
 	// z = M·r, z zeroed first (cg.rs:193-194); identity copies (preconditioner.rs:29-37).
 	fx_status precond(double* z, const double* r, const int* sk) {
 		if (!pc || pc->kind == FX_PC_IDENTITY) return eupd(FCopy{r, z}, "pc identity", sk);
@@ -1424,8 +1483,12 @@ fx_status fetch_stats(fx_solver* s, fx_solve_stats* out) {
 
 // ‖b − A·x‖₂ in FP64 (overwrites work vector `tmp`).
 fx_status true_residual(Ctx& c, const double* x, const double* b, double* tmp, double* out) {
-	FX_CHECK(c.apply_op(x, tmp, nullptr));
-	FX_CHECK(c.ered<1>(FResid{b, tmp, tmp}, FinPcgInitRR{c.d}, "true residual", nullptr));
+	if (c.op->kind == FX_OP_VARK) {
+		FX_CHECK(c.resid_vk(x, EpiResNorm{b}, FinPcgInitRR{c.d}, "true residual"));
+	} else {
+		FX_CHECK(c.apply_op(x, tmp, nullptr));
+		FX_CHECK(c.ered<1>(FResid{b, tmp, tmp}, FinPcgInitRR{c.d}, "true residual", nullptr));
+	}
 	FX_HIP(hipMemcpyAsync(c.s->hst, c.d, sizeof(SState), hipMemcpyDeviceToHost, c.st));
 	FX_HIP(hipStreamSynchronize(c.st));
 	*out = std::sqrt(c.s->hst->rr);
@@ -1490,12 +1553,16 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	double prev_tr = INFINITY;
 	for (int pass = 0; pass < 8; ++pass) {
 		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used), c.pc));
-		FX_CHECK(c.apply_op(x, ap, nullptr));
 		FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
 		if (fm) {
 			float* r32 = (float*)fm->lv[0].rhs;
 			const float* z32 = (const float*)fm->lv[0].tmp;
-			FX_CHECK(c.ered<1>(FResid32{b, ap, r, r32}, FinPcgInitRR{c.d}, "fcg init", nullptr));
+			if (vk) {
+				FX_CHECK(c.resid_vk(x, EpiRes32{b, r, r32}, FinPcgInitRR{c.d}, "fcg init"));
+			} else {
+				FX_CHECK(c.apply_op(x, ap, nullptr));
+				FX_CHECK(c.ered<1>(FResid32{b, ap, r, r32}, FinPcgInitRR{c.d}, "fcg init", nullptr));
+			}
 			FX_CHECK(fmg::fast_cycle(fm, c.skip, EpiRZ{r, c.d}));
 			const bool even = c.cols % 2 == 0;
 			FX_CHECK(drive_counted(s, [&](int64_t i) -> fx_status {
@@ -1558,6 +1625,7 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 			hipLaunchKernelGGL(k_xflush, dim3(fx::red_blocks(c.n)), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, c.n);
 			FX_CHECK(check_launch("fcg deferred x update"));
 		} else {
+			FX_CHECK(c.apply_op(x, ap, nullptr));
 			FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "fcg init", nullptr));
 			FX_CHECK(c.precond(z, r));
 			FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "fcg r.z"));
