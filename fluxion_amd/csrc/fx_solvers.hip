@@ -66,6 +66,7 @@ struct SState {
 	// Host-visible iteration counter (next to hdone): FinPcgRR stores the iteration count once the
 	// iteration's x/r update is done, so the enqueue loop can follow the device without events.
 	int* hprog;
+	double tres;   // ‖b − A·x‖ of the last true-residual check (FinTrueRes)
 };
 
 __device__ __forceinline__ void record(SState* s, int64_t k, double v) {
@@ -328,6 +329,14 @@ struct EpiRes32 {
 	}
 	__device__ void interior(int64_t i, double av, double, Acc* a) const { put(i, av, a); }
 	__device__ void boundary(int64_t i, double c, Acc* a) const { put(i, c, a); }
+	__device__ void put2(int64_t i, double2 av, Acc* a) const {
+		const double2 bv = *(const double2*)(b + i);
+		const double2 v{bv.x - av.x, bv.y - av.y};
+		*(double2*)(r + i) = v;
+		*(float2*)(r32 + i) = float2{float(v.x), float(v.y)};
+		acc_add(a[0], v.x * v.x);
+		acc_add(a[0], v.y * v.y);
+	}
 };
 // Only ‖b − A·x‖² (the true-residual check).
 struct EpiResNorm {
@@ -340,6 +349,18 @@ struct EpiResNorm {
 		const double v = b[i] - c;
 		acc_add(a[0], v * v);
 	}
+	__device__ void put2(int64_t i, double2 av, Acc* a) const {
+		const double2 bv = *(const double2*)(b + i);
+		const double vx = bv.x - av.x, vy = bv.y - av.y;
+		acc_add(a[0], vx * vx);
+		acc_add(a[0], vy * vy);
+	}
+};
+
+// ‖b − A·x‖ into st->tres, leaving the iteration state alone.
+struct FinTrueRes {
+	SState* st;
+	__device__ void operator()(Acc* o) const { st->tres = sqrt(acc_value(o[0])); }
 };
 
 // ap = A·p inside (operator value given), ap = p on the boundary; writes p_new; p·ap.
@@ -367,6 +388,71 @@ __device__ __forceinline__ double dlane(double v) {
 	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// r = b − A·x for the variable-coefficient operator on grids with an even
+// column count: two columns per thread and the DPP neighbour exchange of
+// k_fcg_step2 below, the operator arithmetic of k_vark (identity boundary);
+// E::put2 receives each pair's A·x.
+template <class E>
+__global__ __launch_bounds__(BLOCK) void k_resid2(const double* __restrict__ x, const double* __restrict__ k, E e,
+                                                  double sigma, double alpha, double ix2, double iy2, int64_t rows,
+                                                  int64_t cols, int rpb, RedScratch rs) {
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int lane = (int)(threadIdx.x & (WAVE - 1));
+	const int64_t c0 = 2 * ((int64_t)bx * BLOCK + threadIdx.x);
+	const int64_t r0 = (int64_t)by * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool ok = c0 < cols;
+	const int64_t cl = ok ? c0 : cols - 2;
+	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;
+	const bool edgeW = lane == 0 && ok && c0 > 0, edgeE = lane == WAVE - 1 && ok && c0 + 2 < cols;
+	struct Row {
+		double2 x, k;
+		double xw, xe, kw, ke;
+	};
+	auto ld = [&](int64_t r) {
+		Row v;
+		const int64_t i = r * cols + cl;
+		v.x = *(const double2*)(x + i);
+		v.k = *(const double2*)(k + i);
+		v.xw = v.xe = 0.0;
+		v.kw = v.ke = 1.0;
+		if (edgeW) v.xw = x[i - 1], v.kw = k[i - 1];
+		if (edgeE) v.xe = x[i + 2], v.ke = k[i + 2];
+		return v;
+	};
+	const Row none{double2{0.0, 0.0}, double2{1.0, 1.0}, 0.0, 0.0, 1.0, 1.0};
+	Acc acc[1] = {Acc{0.0, 0.0}};
+	Row up = r0 > 0 ? ld(r0 - 1) : none;
+	Row cur = ld(r0 < rows ? r0 : rows - 1);
+	auto op = [&](double kc, double kn, double ks, double kwv, double kev, double c, double n, double s_, double w_,
+	              double e_) {
+		const double hw = hmean_d(kc, kwv), he = hmean_d(kc, kev);
+		const double hn = hmean_d(kc, kn), hs = hmean_d(kc, ks);
+		const double fx = he * (e_ - c) - hw * (c - w_);
+		const double fy = hs * (s_ - c) - hn * (c - n);
+		const double L = -(fx * ix2 + fy * iy2);
+		return fma(alpha, L, sigma * c);
+	};
+	for (int64_t r = r0; r < r1; ++r) {
+		const int64_t i = r * cols + c0;
+		const Row dn = r + 1 < rows ? ld(r + 1) : none;
+		double xw = dlane<0x138>(cur.x.y), xe = dlane<0x130>(cur.x.x);
+		double kw = dlane<0x138>(cur.k.y), ke = dlane<0x130>(cur.k.x);
+		if (lane == 0) xw = cur.xw, kw = cur.kw;
+		if (lane == WAVE - 1) xe = cur.xe, ke = cur.ke;
+		double2 av = cur.x;   // boundary rows and columns: identity
+		if (r > 0 && r < rows - 1) {
+			if (intA) av.x = op(cur.k.x, up.k.x, dn.k.x, kw, cur.k.y, cur.x.x, up.x.x, dn.x.x, xw, cur.x.y);
+			if (intB) av.y = op(cur.k.y, up.k.y, dn.k.y, cur.k.x, ke, cur.x.y, up.x.y, dn.x.y, cur.x.x, xe);
+		}
+		if (ok) e.put2(i, av, acc);
+		up = cur;
+		cur = dn;
+	}
+	block_partial<1>(acc, rs);
+}
+
 // FCG step of the fused loop on grids with an even column count, two columns
 // per thread (16-byte loads and stores): p = double(z) + β·p_old, ap = A·p
 // (identity boundary rows), p·ap; with st->xpend it also applies the previous
@@ -384,7 +470,8 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
                                                      double* __restrict__ pnew, double* __restrict__ ap, W9 w,
                                                      double coeff, const double* __restrict__ k, double sigma,
                                                      double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
-                                                     RedScratch rs, const SState* st, double* __restrict__ x) {
+                                                     RedScratch rs, const SState* st, double* __restrict__ x,
+                                                     int first) {
 	if (st->done) return;
 	const double be = st->beta;
 	const bool xu = st->xpend != 0;
@@ -408,7 +495,7 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 		Row v;
 		const int64_t i = r * cols + cl;
 		const float2 zz = *(const float2*)(z + i);
-		const double2 pp = *(const double2*)(pold + i);
+		const double2 pp = first ? double2{0.0, 0.0} : *(const double2*)(pold + i);   // first: p_old = 0
 		v.x = xu && own ? *(const double2*)(x + i) : double2{0.0, 0.0};
 		v.xn = double2{fma(al, pp.x, v.x.x), fma(al, pp.y, v.x.y)};
 		v.p = double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
@@ -416,11 +503,11 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 		v.pw = v.pe = 0.0;
 		v.kw = v.ke = 1.0;
 		if (edgeW) {
-			v.pw = double(z[i - 1]) + be * pold[i - 1];
+			v.pw = double(z[i - 1]) + be * (first ? 0.0 : pold[i - 1]);
 			if constexpr (VK) v.kw = k[i - 1];
 		}
 		if (edgeE) {
-			v.pe = double(z[i + 2]) + be * pold[i + 2];
+			v.pe = double(z[i + 2]) + be * (first ? 0.0 : pold[i + 2]);
 			if constexpr (VK) v.ke = k[i + 2];
 		}
 		return v;
@@ -1287,12 +1374,25 @@ struct Ctx {
 	// r = b − A·x in one pass (variable-coefficient operator), then fin on r·r.
 	template <class E, class Fin>
 	fx_status resid_vk(const double* x, E e, Fin fin, const char* what) {
-		const int rpb = stream_rpb(rows, cols);
-		const dim3 grid = fx::stream_grid(rows, cols, rpb);
-		FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
-		hipLaunchKernelGGL((k_resid_vk<E>), grid, dim3(BLOCK), 0, st, VX{x, 0.0}, e, (const double*)op->k->ptr,
-		                   op->sigma, op->coeff, 1.0 / (s->g.dx * s->g.dx), 1.0 / (s->g.dy * s->g.dy), rows, cols, rpb,
-		                   fx::red_scratch(be));
+		const double ix2 = 1.0 / (s->g.dx * s->g.dx), iy2 = 1.0 / (s->g.dy * s->g.dy);
+		const double* kp = (const double*)op->k->ptr;
+		dim3 grid;
+		if (cols % 2 == 0) {
+			// Two columns per thread; row blocks as tall as 1024 workgroups allow.
+			const int64_t cb = (cols / 2 + BLOCK - 1) / BLOCK;
+			int rpb = 64;
+			while (rpb > 4 && cb * ((rows + rpb - 1) / rpb) < 1024) rpb /= 2;
+			grid = fx::stream_grid(rows, cols / 2, rpb);
+			FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
+			hipLaunchKernelGGL((k_resid2<E>), grid, dim3(BLOCK), 0, st, x, kp, e, op->sigma, op->coeff, ix2, iy2, rows,
+			                   cols, rpb, fx::red_scratch(be));
+		} else {
+			const int rpb = stream_rpb(rows, cols);
+			grid = fx::stream_grid(rows, cols, rpb);
+			FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
+			hipLaunchKernelGGL((k_resid_vk<E>), grid, dim3(BLOCK), 0, st, VX{x, 0.0}, e, kp, op->sigma, op->coeff,
+			                   ix2, iy2, rows, cols, rpb, fx::red_scratch(be));
+		}
 		hipLaunchKernelGGL((k_finish<1, Fin>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be),
 		                   grid.x * grid.y, fin, nullptr);
 		return check_launch(what);
@@ -1409,8 +1509,9 @@ fx_status reset_state(fx_solver* s, int64_t max_iter, const fx_precond* pc = nul
 		h.y = h.grs + m + 1;
 	}
 	*s->hst = h;
+	// hst is pinned and the host next writes it only through fetch_stats (a copy ordered after this
+	// one on the stream), so the copy needs no synchronisation here.
 	FX_HIP(hipMemcpyAsync(s->dst, s->hst, sizeof(SState), hipMemcpyHostToDevice, s->be->stream));
-	FX_HIP(hipStreamSynchronize(s->be->stream));
 	return FX_OK;
 }
 
@@ -1553,7 +1654,8 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	double prev_tr = INFINITY;
 	for (int pass = 0; pass < 8; ++pass) {
 		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used), c.pc));
-		FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
+		// The fused even-column step takes p_old = 0 in its first iteration itself.
+		if (!fm || c.cols % 2 != 0) FX_CHECK(fx::k_fill(c.be, p0, 0.0, c.n));
 		if (fm) {
 			float* r32 = (float*)fm->lv[0].rhs;
 			const float* z32 = (const float*)fm->lv[0].tmp;
@@ -1582,11 +1684,11 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 					if (vk)
 						hipLaunchKernelGGL(k_fcg_step2<true>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
 						                   c.op->coeff, kp, c.op->sigma, ix2, iy2, c.rows, c.cols, rpb,
-						                   fx::red_scratch(c.be), (const SState*)c.d, x);
+						                   fx::red_scratch(c.be), (const SState*)c.d, x, (int)(i == 0));
 					else
 						hipLaunchKernelGGL(k_fcg_step2<false>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
 						                   c.op->coeff, kp, 0.0, ix2, iy2, c.rows, c.cols, rpb, fx::red_scratch(c.be),
-						                   (const SState*)c.d, x);
+						                   (const SState*)c.d, x, (int)(i == 0));
 					fx::probe_end(c.be, FX_PROBE_FCG_STEP);
 					hipLaunchKernelGGL((k_finish<1, FinAlphaDefer>), dim3(1), dim3(FIN_BLOCK), 0, c.st,
 					                   fx::red_scratch(c.be), grid.x * grid.y, FinAlphaDefer{c.d}, c.done);
@@ -1638,11 +1740,19 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 				return c.ered<2>(FDot2{r, z, z, ap}, FinFcgBeta{c.d}, "fcg r.z z.ap");
 			}, (int64_t)(s->max_iter - used)));
 		}
-		FX_CHECK(fetch_stats(s, &st));
+		double tr = 0;
+		if (fm && vk) {
+			// The true residual is reduced into the device state before the stats are copied: one
+			// host synchronisation for both.
+			FX_CHECK(c.resid_vk(x, EpiResNorm{b}, FinTrueRes{c.d}, "true residual"));
+			FX_CHECK(fetch_stats(s, &st));
+			tr = s->hst->tres;
+		} else {
+			FX_CHECK(fetch_stats(s, &st));
+			FX_CHECK(true_residual(c, x, b, ap, &tr));
+		}
 		if (c.pc && c.pc->kind == FX_PC_LEARNED) ((fx_learned*)c.pc->handle)->fell_back |= s->hst->lplain;
 		used += st.iterations;
-		double tr = 0;
-		FX_CHECK(true_residual(c, x, b, ap, &tr));
 		st.true_residual = tr;
 		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
 		// A restart that did not halve the true residual has reached the FP64
