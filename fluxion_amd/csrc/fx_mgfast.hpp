@@ -968,7 +968,7 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 // post-sweeps, residual) runs a fixed number of rows behind the row being
 // loaded and keeps its last three rows in registers; west and east neighbours
 // come from the adjacent lanes through DPP wavefront shifts.  Rows are loaded
-// WD steps ahead from clamped (always in-bounds) addresses, so the loads are
+// several steps ahead from clamped (always in-bounds) addresses, so the loads are
 // unconditional and the steps straight-line code.  Per-point arithmetic, and
 // the restriction's summation order, are those of the unfused kernels.
 constexpr int WNMAX = 5;  // most fine columns (or rows) a coarse point gathers from (2·(h/H)⁻¹ + 1 for H ≈ 2h)
@@ -1042,13 +1042,14 @@ __device__ __forceinline__ WRow wfix(const WRow& v, int r, int rows, bool colin,
 	return o;
 }
 
-// Row rings.  A wave's steps are unrolled by WD = 8, and row t of unrolled step
-// k sits in slot (k − i) mod W of a W-slot ring when the current row is t + i,
-// a compile-time index, so advancing a row moves no data.  The inputs' ring has
-// W = WD slots: the stages read the last few rows in place and each slot is
-// refilled, WD − (rows kept) steps ahead, once its row is no longer read.  The
-// stages' own rings have W = 4 slots.
-constexpr int WD = 8;
+// Row rings.  A wave's steps are unrolled by the input ring's depth D (8), and
+// row t of unrolled step k sits in slot (k − i) mod W of a W-slot ring when the
+// current row is t + i, a compile-time index, so advancing a row moves no data.
+// The inputs' ring has W = D slots: the stages read the last few rows in place
+// and each slot is refilled, D − (rows kept) steps ahead, once its row is no
+// longer read.  The stages' own rings have W = 4 slots (D is a multiple of 4).
+constexpr int WD_POST = 8;      // input ring depth of k_wpost
+constexpr int WD_RESTRICT = 8;  // and of k_wrestrict
 constexpr int WR = 4;
 template <int W, int K, int I>
 struct Slot {
@@ -1110,38 +1111,25 @@ inline int64_t wave_segs_restrict(int64_t cr, int scr) { return (cr - 2 + scr - 
 inline unsigned wave_blocks(int64_t nwaves) { return (unsigned)((nwaves + NWAVES - 1) / NWAVES); }
 
 // Runs step(integral_constant<k>, t, cl) for t = t0 … t1 − 1, k = (t − t0) mod
-// WD: whole blocks of WD steps without conditions (so the compiler's
-// memory-wait counts carry across the loop), then the remainder.  cl
-// (true_type or false_type) is passed through: a wave whose rows t0 … t1 − 1
-// and columns are all interior runs its steps without the grid-edge fixes.
-// The choice is made once per wave, so that each variant is a loop nest of
-// its own (one loop holding both would need the registers of both).
-template <class F, class CL>
+// W: whole blocks of W steps without conditions (so the compiler's memory-wait
+// counts carry across the loop), then the remainder.  cl (true_type or
+// false_type) is passed through: a wave whose rows t0 … t1 − 1 and columns are
+// all interior runs its steps without the grid-edge fixes.  The choice is made
+// once per wave, so that each variant is a loop nest of its own (one loop
+// holding both would need the registers of both).
+template <class F, class CL, int... K>
+__device__ __forceinline__ void run_block(F& step, int b, CL cl, std::integer_sequence<int, K...>) {
+	(step(std::integral_constant<int, K>{}, b + K, cl), ...);
+}
+template <class F, class CL, int... K>
+__device__ __forceinline__ void run_rest(F& step, int b, int t1, CL cl, std::integer_sequence<int, K...>) {
+	((b + K < t1 ? step(std::integral_constant<int, K>{}, b + K, cl) : void()), ...);
+}
+template <int W, class F, class CL>
 __device__ __forceinline__ void run_steps(F&& step, int t0, int t1, CL cl) {
-	auto all = [&](int b, auto cl) {
-		step(std::integral_constant<int, 0>{}, b + 0, cl);
-		step(std::integral_constant<int, 1>{}, b + 1, cl);
-		step(std::integral_constant<int, 2>{}, b + 2, cl);
-		step(std::integral_constant<int, 3>{}, b + 3, cl);
-		step(std::integral_constant<int, 4>{}, b + 4, cl);
-		step(std::integral_constant<int, 5>{}, b + 5, cl);
-		step(std::integral_constant<int, 6>{}, b + 6, cl);
-		step(std::integral_constant<int, 7>{}, b + 7, cl);
-	};
 	int b = t0;
-	for (; b + WD <= t1; b += WD) all(b, cl);
-	auto one = [&](auto kc) {
-		constexpr int k = decltype(kc)::value;
-		if (b + k < t1) step(kc, b + k, cl);
-	};
-	one(std::integral_constant<int, 0>{});
-	one(std::integral_constant<int, 1>{});
-	one(std::integral_constant<int, 2>{});
-	one(std::integral_constant<int, 3>{});
-	one(std::integral_constant<int, 4>{});
-	one(std::integral_constant<int, 5>{});
-	one(std::integral_constant<int, 6>{});
-	one(std::integral_constant<int, 7>{});
+	for (; b + W <= t1; b += W) run_block(step, b, cl, std::make_integer_sequence<int, W>{});
+	run_rest(step, b, t1, cl, std::make_integer_sequence<int, W>{});
 }
 
 // out = J^NPOST(J^NPRE(0; b) + P·e) over the level (k_fpost's result), with Epi's
@@ -1158,7 +1146,8 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 	constexpr int LU = NPRE - 1;     // rows the iterate u = J^npre(0; b) + P·e lags the loaded row
 	constexpr int LS = LU + 1;       // first post-sweep (NPOST == 2)
 	constexpr int LZ = LU + NPOST;   // the level's result
-	static_assert(LZ <= SHX && LZ < WR && LZ < WD, "halo or rings too small");
+	constexpr int WP = WD_POST;
+	static_assert(LZ <= SHX && LZ < WR && LZ < WP && WP % WR == 0, "halo or rings too small");
 	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
 	// With Epi::RHS_FROM_R the rhs is float(r) of the epilogue's FP64 residual
 	// (which it loads anyway, for the same row and column), not a load of b.
@@ -1190,7 +1179,7 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 			vx = 1.0f - wx;
 		}
 		auto inter = [&](int r) { return colint && r >= 1 && r <= rows - 2; };
-		// Coarse-correction operands of a row (read WD steps before the row's iterate is formed).
+		// Coarse-correction operands of a row (read WP steps before the row's iterate is formed).
 		struct EV {
 			float e00, e01, e10, e11, wy;
 		};
@@ -1214,13 +1203,13 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 		// grid-edge fixes (cl), so that each variant gets its own registers.
 		auto walk = [&](auto cl) {
 			le.init(T0 - LU, rows, cr);
-			// Input ring: row T0 + j in slot j mod WD; the first WD − LZ rows now, the rest as slots free up.
-			WRow rb[WD];
-			EV evb[WD];
-			typename Epi::Pf pfb[WD];
+			// Input ring: row T0 + j in slot j mod WP; the first WP − LZ rows now, the rest as slots free up.
+			WRow rb[WP];
+			EV evb[WP];
+			typename Epi::Pf pfb[WP];
 #pragma unroll
-			for (int k = 0; k < WD; ++k) {
-				rb[k] = k < WD - LZ ? wload<VK, LOADB>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+			for (int k = 0; k < WP; ++k) {
+				rb[k] = k < WP - LZ ? wload<VK, LOADB>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
 				evb[k] = eload();
 				pfb[k] = pload(T0 + k - LZ);
 			}
@@ -1230,19 +1219,19 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				constexpr bool CL = decltype(cl)::value;
 				auto in = [&](int r) { return CL || inter(r); };
 				{
-					WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+					WRow v = wtake<VK>(rb[k], rb[Slot<WP, k, 1>::v]);
 					// Row t's residual was loaded with the row: it is in the epilogue slot LZ steps on.
-					if constexpr (!LOADB) v.b = epi.rhs(pfb[Slot<WD, k, -LZ>::v]);
+					if constexpr (!LOADB) v.b = epi.rhs(pfb[Slot<WP, k, -LZ>::v]);
 					rb[k] = CL ? v : wfix<VK>(v, t, rows, colin, kwok, keok);
 				}
-				// Inputs of row t − i: rb[Slot<WD, k, i>].
+				// Inputs of row t − i: rb[Slot<WP, k, i>].
 				float u;  // iterate at row t − LU
 				if constexpr (NPRE == 2) {
 					const float u1 = wjac0<VK>(L, rb[k]);
 					U1[q] = in(t) ? u1 : 0.0f;
 					const float c = U1[Slot<WR, q, 1>::v];
 					const float wv = lane_west(c), evv = lane_east(c);
-					u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+					u = wjac<VK>(L, rb[Slot<WP, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
 				} else {
 					u = wjac0<VK>(L, rb[k]);
 				}
@@ -1258,25 +1247,25 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				if constexpr (NPOST == 2) {
 					const float c = U[Slot<WR, q, 1>::v];
 					const float wv = lane_west(c), evv = lane_east(c);
-					const float sv = wjac<VK>(L, rb[Slot<WD, k, LS>::v], c, U[Slot<WR, q, 2>::v], U[q], wv, evv);
+					const float sv = wjac<VK>(L, rb[Slot<WP, k, LS>::v], c, U[Slot<WR, q, 2>::v], U[q], wv, evv);
 					S[q] = in(t - LS) ? sv : c;
 					C = S;
 				}
 				const float c = C[Slot<WR, q, 1>::v];
 				const float wv = lane_west(c), evv = lane_east(c);
-				const float zv = wjac<VK>(L, rb[Slot<WD, k, LZ>::v], c, C[Slot<WR, q, 2>::v], C[q], wv, evv);
+				const float zv = wjac<VK>(L, rb[Slot<WP, k, LZ>::v], c, C[Slot<WR, q, 2>::v], C[q], wv, evv);
 				const int rz = t - LZ;
 				const float z = in(rz) ? zv : c;
 				if (outlane && rz >= S0 && rz < S1) {
 					(out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
 					epi.use_lane(pfb[k], z, lsum);
 				}
-				// Refill: the slot of row t − LZ (no longer read) with row t + WD − LZ, and this step's operands.
-				rb[Slot<WD, k, LZ>::v] = wload<VK, LOADB>(b, L, t + WD - LZ, rows, cols, uc);
+				// Refill: the slot of row t − LZ (no longer read) with row t + WP − LZ, and this step's operands.
+				rb[Slot<WP, k, LZ>::v] = wload<VK, LOADB>(b, L, t + WP - LZ, rows, cols, uc);
 				evb[k] = eload();
-				pfb[k] = pload(t + WD - LZ);
+				pfb[k] = pload(t + WP - LZ);
 		};
-		run_steps(step, T0, T1, cl);
+		run_steps<WP>(step, T0, T1, cl);
 		};
 		if (wclean && T0 >= 1 && T1 <= rows - 1) walk(std::true_type{});
 		else walk(std::false_type{});
@@ -1304,7 +1293,8 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
                                                      int scr, const int* skip) {
 	if (skip && *skip) return;
 	constexpr int LU = NPRE - 1, LR = NPRE;  // rows the pre-smoothed iterate and the residual lag the loaded row
-	static_assert(LR < WR && LR < WD, "rings too small");
+	constexpr int WQ = WD_RESTRICT;
+	static_assert(LR < WR && LR < WQ && WQ % WR == 0, "rings too small");
 	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
 	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
@@ -1381,17 +1371,17 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	auto walk = [&](auto cl) {
 		LinRow lr;  // fine row whose residual the step produces
 		lr.init(T0 - LR, fr, cr);
-		WRow rb[WD];
+		WRow rb[WQ];
 #pragma unroll
-		for (int k = 0; k < WD; ++k)
-			rb[k] = k < WD - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+		for (int k = 0; k < WQ; ++k)
+			rb[k] = k < WQ - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
 		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
 		auto step = [&](auto kc, int t, auto cl) {
 			constexpr int k = decltype(kc)::value, q = k % WR;
 			constexpr bool CL = decltype(cl)::value;
 			auto in = [&](int r) { return CL || inter(r); };
 			{
-				const WRow v = wtake<VK>(rb[k], rb[Slot<WD, k, 1>::v]);
+				const WRow v = wtake<VK>(rb[k], rb[Slot<WQ, k, 1>::v]);
 				rb[k] = CL ? v : wfix<VK>(v, t, fr, colin, kwok, keok);
 			}
 			float u;  // pre-smoothed iterate at row t − LU
@@ -1400,7 +1390,7 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 				U1[q] = in(t) ? u1 : 0.0f;
 				const float c = U1[Slot<WR, q, 1>::v];
 				const float wv = lane_west(c), evv = lane_east(c);
-				u = wjac<VK>(L, rb[Slot<WD, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
+				u = wjac<VK>(L, rb[Slot<WQ, k, 1>::v], c, U1[Slot<WR, q, 2>::v], U1[q], wv, evv);
 			} else {
 				u = wjac0<VK>(L, rb[k]);
 			}
@@ -1409,11 +1399,11 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 			const int i = t - LR;
 			const float c = Uv[Slot<WR, q, 1>::v];
 			const float wv = lane_west(c), evv = lane_east(c);
-			const WRow& ri = rb[Slot<WD, k, LR>::v];
+			const WRow& ri = rb[Slot<WQ, k, LR>::v];
 			const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
 			const float res = in(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
-			// Refill the slot of row t − LR (no longer read) with row t + WD − LR.
-			rb[Slot<WD, k, LR>::v] = wload<VK>(b, L, t + WD - LR, fr, fc, uc);
+			// Refill the slot of row t − LR (no longer read) with row t + WQ − LR.
+			rb[Slot<WQ, k, LR>::v] = wload<VK>(b, L, t + WQ - LR, fr, fc, uc);
 			// Row sum over the owned coarse column's fine columns.
 			float row = 0.0f;
 #pragma unroll
@@ -1433,7 +1423,7 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 				ahi += float(wrv) * row;
 			}
 	};
-	run_steps(step, T0, T1, cl);
+	run_steps<WQ>(step, T0, T1, cl);
 	emit(Icur, alo);
 	emit(Icur + 1, ahi);
 	};

@@ -658,13 +658,18 @@ struct FinAlphaDefer {
 		FinAlpha{st}(o);
 	}
 };
-// The deferred x update left when the loop stopped: x += α·xp (if any).
-__global__ __launch_bounds__(BLOCK) void k_xflush(const SState* st, double* __restrict__ x, size_t n) {
+// The deferred x update left when the loop stopped: x += α·xp (if any), two
+// points per thread per step (n even), unrolled like k_fr32v.
+__global__ __launch_bounds__(BLOCK) void k_xflush(const SState* st, double* __restrict__ x, size_t n2) {
 	if (!st->xpend) return;
 	const double al = st->alpha;
-	const double* p = st->xp;
-	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n; j += (size_t)gridDim.x * BLOCK)
-		x[j] = fma(al, p[j], x[j]);
+	const double2* p = (const double2*)st->xp;
+	const size_t stride = (size_t)gridDim.x * BLOCK;
+#pragma unroll 4
+	for (size_t j = (size_t)blockIdx.x * BLOCK + threadIdx.x; j < n2; j += stride) {
+		const double2 xv = ((const double2*)x)[j], pv = p[j];
+		((double2*)x)[j] = double2{fma(al, pv.x, xv.x), fma(al, pv.y, xv.y)};
+	}
 }
 // PCG init: r = b − Ax with r·r (no check yet: cg.rs:188-202 computes z first).
 struct FinPcgInitRR {
@@ -1724,7 +1729,11 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 				FX_CHECK(c.ered<1>(FXR32{c.d, x, r, r32, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
 				return fmg::fast_cycle(fm, c.skip, EpiFcg{r, ap, c.d});
 			}, (int64_t)(s->max_iter - used)));
-			hipLaunchKernelGGL(k_xflush, dim3(fx::red_blocks(c.n)), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, c.n);
+			if (c.n % 2 == 0) {   // only the even-column loop defers its x update
+				const size_t n2 = c.n / 2;
+				const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>(16384, (n2 + 4 * BLOCK - 1) / (4 * BLOCK)));
+				hipLaunchKernelGGL(k_xflush, dim3(nb), dim3(BLOCK), 0, c.st, (const SState*)c.d, x, n2);
+			}
 			FX_CHECK(check_launch("fcg deferred x update"));
 		} else {
 			FX_CHECK(c.apply_op(x, ap, nullptr));
