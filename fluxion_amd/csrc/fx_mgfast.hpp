@@ -1443,6 +1443,9 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 // arithmetic is that of the other fused kernels, so the result is
 // bit-identical.
 constexpr int TAIL_MAXL = 6;
+// Plane points per thread staged by k_ftail (≥ ⌈plane length / TAIL_BLOCK⌉; the
+// LDS limit alone keeps the plane length below 7 (VK) or 14 blocks).
+constexpr int tail_su(bool vk) { return vk ? 7 : 14; }
 constexpr int TAIL_BLOCK = 1024;
 constexpr size_t TAIL_LDS_MAX = 160 * 1024;
 struct TailLevel {
@@ -1488,35 +1491,49 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 	int32_t* rlo = (int32_t*)(twh + A.ntab);   // per coarse row / column: fine range lo..hi
 	int32_t* rhi = rlo + A.nrng;
 	const int t = threadIdx.x;
-	// Stage the planes and tables.
+	// Stage the planes and tables, level by level (a uniform loop, so that the
+	// level's fields are scalar loads); within a level each thread issues all of
+	// its global loads before its first LDS store, so that their latencies overlap.
 	for (int l = 0; l < A.nl; ++l) {
 		const TailLevel& T = A.lv[l];
 		const int n = T.rows * T.cols;
-		for (int q = t; q < n; q += TAIL_BLOCK) {
-			if constexpr (VK) {
-				kxp[T.off + q] = T.L.kx[q];
-				kyp[T.off + q] = T.L.ky[q];
-				wdp[T.off + q] = T.L.wd[q];
+		float sv[tail_su(VK)][VK ? 4 : 1];
+#pragma unroll
+		for (int j = 0; j < tail_su(VK); ++j) {
+			const int q = t + TAIL_BLOCK * j;
+			if (q < n) {
+				sv[j][0] = l == 0 ? A.rhs0[q] : 0.0f;   // b of the other levels comes from the restriction
+				if constexpr (VK) {
+					sv[j][1] = T.L.kx[q];
+					sv[j][2] = T.L.ky[q];
+					sv[j][3] = T.L.wd[q];
+				}
 			}
-			if (l == 0) pb[q] = A.rhs0[q];
 		}
-		if (l + 1 < A.nl) {
-			const TailLevel& C = A.lv[l + 1];
-			for (int q = t; q < T.rows; q += TAIL_BLOCK) {
-				const double w = T.X.wr[q];
-				ti0[T.tr + q] = T.X.i0r[q];
-				twl[T.tr + q] = float(1.0 - w);
-				twh[T.tr + q] = float(w);
+		const bool tab = l + 1 < A.nl;
+		const int cr = tab ? A.lv[l + 1].rows : 0, cc = tab ? A.lv[l + 1].cols : 0;
+		int ir = 0, ic = 0, lr = 0, hr = 0, lc = 0, hc = 0;
+		double wrv = 0.0, wcv = 0.0;
+		if (tab && t < T.rows) ir = T.X.i0r[t], wrv = T.X.wr[t];
+		if (tab && t < T.cols) ic = T.X.i0c[t], wcv = T.X.wc[t];
+		if (tab && t < cr) lr = T.X.lor[t], hr = T.X.hir[t];
+		if (tab && t < cc) lc = T.X.loc[t], hc = T.X.hic[t];
+#pragma unroll
+		for (int j = 0; j < tail_su(VK); ++j) {
+			const int q = t + TAIL_BLOCK * j;
+			if (q < n) {
+				pb[T.off + q] = sv[j][0];
+				if constexpr (VK) {
+					kxp[T.off + q] = sv[j][1];
+					kyp[T.off + q] = sv[j][2];
+					wdp[T.off + q] = sv[j][3];
+				}
 			}
-			for (int q = t; q < T.cols; q += TAIL_BLOCK) {
-				const double w = T.X.wc[q];
-				ti0[T.tc + q] = T.X.i0c[q];
-				twl[T.tc + q] = float(1.0 - w);
-				twh[T.tc + q] = float(w);
-			}
-			for (int q = t; q < C.rows; q += TAIL_BLOCK) rlo[T.tcr + q] = T.X.lor[q], rhi[T.tcr + q] = T.X.hir[q];
-			for (int q = t; q < C.cols; q += TAIL_BLOCK) rlo[T.tcc + q] = T.X.loc[q], rhi[T.tcc + q] = T.X.hic[q];
 		}
+		if (tab && t < T.rows) ti0[T.tr + t] = ir, twl[T.tr + t] = float(1.0 - wrv), twh[T.tr + t] = float(wrv);
+		if (tab && t < T.cols) ti0[T.tc + t] = ic, twl[T.tc + t] = float(1.0 - wcv), twh[T.tc + t] = float(wcv);
+		if (tab && t < cr) rlo[T.tcr + t] = lr, rhi[T.tcr + t] = hr;
+		if (tab && t < cc) rlo[T.tcc + t] = lc, rhi[T.tcc + t] = hc;
 	}
 	__syncthreads();
 	// One Jacobi point update of level T at q from plane `in` (interior points only).
@@ -1698,7 +1715,11 @@ inline int tail_plan(const fx_multigrid* mg, bool vk, TailArgs* A, size_t* lds) 
 			}
 		}
 		const size_t b = tail_lds(vk, npts, nrs, ntab, nrng);
-		if (b > TAIL_LDS_MAX) continue;
+		bool fits = b <= TAIL_LDS_MAX;
+		for (int l = lt; l < nl; ++l)   // staging: a level's points in tail_su blocks, its table rows and columns in one
+			fits = fits && lv[l].g.rows * lv[l].g.cols <= (size_t)tail_su(vk) * TAIL_BLOCK && lv[l].g.rows <= TAIL_BLOCK &&
+			       lv[l].g.cols <= TAIL_BLOCK;
+		if (!fits) continue;
 		A->nl = nl - lt;
 		A->sweeps = mg->opt.n_pre + mg->opt.n_post + mg->opt.extra_coarse;
 		A->npts = npts, A->nrs = nrs, A->ntab = ntab, A->nrng = nrng;
