@@ -44,8 +44,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(vk: bool) -> dict:
-    """Algorithmic HBM bytes per grid point of each probed kernel (each array once at its dtype)."""
+def kernel_bytes(vk: bool, fused_update: bool) -> dict:
+    """Algorithmic HBM bytes per grid point of each probed kernel (each array once at its dtype).
+    fused_update: the FCG residual update rides in the level-0 wave restriction (even columns)."""
     return {
         # r, ap read (FP64); r written (FP64); FP32 copy of r written (the x update is deferred
         # to the next step kernel).
@@ -56,8 +57,9 @@ def kernel_bytes(vk: bool) -> dict:
         # coarse correction (¼ point, FP32) read and z (FP32) written; r, A·p (FP64) read for the
         # dots (the rhs is float(r)); the level's FP32 face coefficients kx, ky and ω/d for var-k.
         "vc_post0": 1 + 4 + 8 + 8 + (12 if vk else 0),
-        # level-0 rhs (FP32), face coefficients and ω/d read, coarse rhs (¼ point, FP32) written.
-        "vc_restrict0": 4 + 1 + (12 if vk else 0),
+        # level-0 rhs (FP32) -- or, with the fused update, r and A·p read and r_new written (FP64) --,
+        # face coefficients and ω/d read, coarse rhs (¼ point, FP32) written.
+        "vc_restrict0": (8 + 8 + 8 if fused_update else 4) + 1 + (12 if vk else 0),
         "stencil_axpy": 24,
     }
 
@@ -192,7 +194,7 @@ def main():
         t_value, s_value, label = t_l, s_l, "learned+mixed"
 
     # Per-kernel probes of the mixed solve (the x/r update was probed over the timed region).
-    kb = kernel_bytes(True)
+    kb = kernel_bytes(True, mgk == MgKernels.WAVE and n % 2 == 0)
     probes = {"fcg_update": pr_update}
     for name, kind in (("fcg_step", be.PROBE_FCG_STEP), ("vc_post0", be.PROBE_VC_POST0),
                        ("vc_restrict0", be.PROBE_VC_RESTRICT0)):

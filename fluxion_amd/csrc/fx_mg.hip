@@ -499,6 +499,9 @@ fx_status build_xtab(fx_backend* be, fx_multigrid::Level& L, const fx_grid& c) {
 		if (hic[Jb] >= loc[Ja]) wspan = std::max<int64_t>(wspan, hic[Jb] - loc[Ja] + 1);
 	}
 	X.wx = (int)wspan + 4;
+	X.adv = 0;
+	for (int64_t Ja = 1; Ja + fmg::RCX + 1 < cc; Ja += fmg::RCX)
+		X.adv = std::max<int>(X.adv, loc[Ja + fmg::RCX] - loc[Ja]);
 	return FX_OK;
 }
 

@@ -19,6 +19,7 @@ struct fx_xtab {
 	int ry = 0, rx = 0;             // LDS region (halo included) of the fused restriction's widest tile
 	int gmax = 0;                   // most fine columns any coarse column gathers from
 	int wx = 0;                     // fine window (halo included) of the wave restriction's widest strip
+	int adv = 0;                    // widest fine-column advance between the wave restriction's consecutive strips
 };
 
 struct fx_multigrid {

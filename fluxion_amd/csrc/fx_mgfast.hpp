@@ -342,6 +342,13 @@ constexpr int PTX = 64, PTY = 16, PH = 3;
 constexpr int PRY = PTY + 2 * PH, PRX = PTX + 2 * PH;
 static_assert(PTX == WAVE, "one tile column per lane in the final sweep");
 
+// No residual update in the wave restriction (its rhs is read as it is).
+struct NoUpd {
+	static constexpr bool ON = false;
+	struct Pf {};
+	__device__ void begin() {}
+	__device__ Pf load_at(size_t, unsigned) const { return Pf{}; }
+};
 struct NoEpi {
 	static constexpr int NS = 0;
 	static constexpr bool RHS_FROM_R = false;  // the wave kernel reads the level-0 rhs itself
@@ -1287,18 +1294,26 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 // the coarse column's fine columns in ascending order (cross-lane permutes),
 // then added, in ascending fine-row order, to the (at most two) coarse rows the
 // fine row contributes to: the order of k_restrict's loops.
-template <bool VK, int NPRE>
-__global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
-                                                     fx_xtab X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
-                                                     int scr, const int* skip) {
-	if (skip && *skip) return;
+//
+// With Upd::ON the rhs is not read: the wave forms it, row by row, as
+// float(r_new) with r_new = r − α·A·p of the FCG x/r update (Upd loads r and
+// A·p), stores r_new at the points it owns, and sums r_new² over them into
+// lsum.  Ownership partitions the grid: strip s owns fine columns from
+// ⌈C0(s)⌉₈ (its lane-0 column rounded up to a multiple of 8, so that owners'
+// stores cover whole 64-byte pieces) to the next strip's, which lie in its
+// window while consecutive strips' first columns are at most 57 apart;
+// segment g owns fine rows from its first gathered row to the next segment's.
+template <bool VK, int NPRE, class Upd>
+__device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, float* __restrict__ bco, const LWf& L,
+                                               const fx_xtab& X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
+                                               int scr, const Upd& upd, double& lsum) {
 	constexpr int LU = NPRE - 1, LR = NPRE;  // rows the pre-smoothed iterate and the residual lag the loaded row
 	constexpr int WQ = WD_RESTRICT;
 	static_assert(LR < WR && LR < WQ && WQ % WR == 0, "rings too small");
 	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
 	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
-	if (gw >= nstrips * nseg) return;  // no workgroup-wide operation follows
+	if (gw >= nstrips * nseg) return;
 	const int strip = gw % nstrips, seg = gw / nstrips;
 	const bool sfirst = strip == 0, slast = strip == nstrips - 1, gfirst = seg == 0, glast = seg == nseg - 1;
 	const int Ja = 1 + strip * RCX, Jb = Ja + RCX - 1 < cc - 2 ? Ja + RCX - 1 : cc - 2;
@@ -1330,6 +1345,11 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	const int C0 = G0 - 2;  // fine column of lane 0
 	const int gc = C0 + x;
 	const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < fc ? gc : fc - 1));
+	// Points whose r_new this wave stores (Upd::ON).
+	const int ocl = sfirst ? 0 : (C0 + 7) & ~7;
+	const int och = slast ? fc : (X.loc[Ja + RCX] - 2 + 7) & ~7;
+	const bool ownc = gc >= ocl && gc < och;
+	const int orl = gfirst ? 0 : F0, orh = glast ? fr : X.lor[Ia + scr];
 	const bool colin = gc >= 0 && gc < fc;
 	const bool wclean = C0 >= 1 && C0 + SRW - 1 <= fc - 2;  // all lanes interior
 	const bool colint = gc >= 1 && gc <= fc - 2;
@@ -1371,17 +1391,34 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	auto walk = [&](auto cl) {
 		LinRow lr;  // fine row whose residual the step produces
 		lr.init(T0 - LR, fr, cr);
+		constexpr bool LOADB = !Upd::ON;
+		auto uload = [&](int r) {
+			const int rc = r < 0 ? 0 : (r < fr ? r : fr - 1);
+			return upd.load_at((size_t)rc * (size_t)fc, uc);
+		};
 		WRow rb[WQ];
+		typename Upd::Pf ub[WQ];  // Upd::ON: row t's r and A·p in slot (k + LR) mod WQ
 #pragma unroll
-		for (int k = 0; k < WQ; ++k)
-			rb[k] = k < WQ - LR ? wload<VK>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+		for (int k = 0; k < WQ; ++k) {
+			rb[k] = k < WQ - LR ? wload<VK, LOADB>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+			if constexpr (Upd::ON) ub[k] = uload(T0 + k - LR);
+		}
 		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
 		auto step = [&](auto kc, int t, auto cl) {
 			constexpr int k = decltype(kc)::value, q = k % WR;
 			constexpr bool CL = decltype(cl)::value;
 			auto in = [&](int r) { return CL || inter(r); };
 			{
-				const WRow v = wtake<VK>(rb[k], rb[Slot<WQ, k, 1>::v]);
+				WRow v = wtake<VK>(rb[k], rb[Slot<WQ, k, 1>::v]);
+				if constexpr (Upd::ON) {
+					const double rn = upd.rnew(ub[Slot<WQ, k, -LR>::v]);
+					v.b = float(rn);
+					if (ownc && t >= orl && t < orh) {
+						upd.store((size_t)t * (size_t)fc + (unsigned)gc, rn);
+						lsum += rn * rn;
+					}
+					ub[k] = uload(t + WQ - LR);
+				}
 				rb[k] = CL ? v : wfix<VK>(v, t, fr, colin, kwok, keok);
 			}
 			float u;  // pre-smoothed iterate at row t − LU
@@ -1403,7 +1440,7 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 			const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
 			const float res = in(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
 			// Refill the slot of row t − LR (no longer read) with row t + WQ − LR.
-			rb[Slot<WQ, k, LR>::v] = wload<VK>(b, L, t + WQ - LR, fr, fc, uc);
+			rb[Slot<WQ, k, LR>::v] = wload<VK, LOADB>(b, L, t + WQ - LR, fr, fc, uc);
 			// Row sum over the owned coarse column's fine columns.
 			float row = 0.0f;
 #pragma unroll
@@ -1422,13 +1459,26 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 				alo += float(1.0 - wrv) * row;
 				ahi += float(wrv) * row;
 			}
-	};
-	run_steps<WQ>(step, T0, T1, cl);
-	emit(Icur, alo);
-	emit(Icur + 1, ahi);
+		};
+		run_steps<WQ>(step, T0, T1, cl);
+		emit(Icur, alo);
+		emit(Icur + 1, ahi);
 	};
 	if (wclean && T0 >= 1 && T1 <= fr - 1) walk(std::true_type{});
 	else walk(std::false_type{});
+}
+template <bool VK, int NPRE, class Upd>
+__global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b, float* __restrict__ bco, LWf L,
+                                                     fx_xtab X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
+                                                     int scr, Upd upd, RedScratch rs, const int* skip) {
+	if (skip && *skip) return;
+	double lsum = 0.0;
+	upd.begin();
+	wrestrict_wave<VK, NPRE>(b, bco, L, X, fr, fc, cr, cc, nstrips, nseg, scr, upd, lsum);
+	if constexpr (Upd::ON) {
+		Acc acc[1] = {Acc{lsum, 0.0}};
+		block_partial<1>(acc, rs);
+	}
 }
 
 // ── k_ftail: the V-cycle below a small level, in one workgroup ─────────
@@ -1755,11 +1805,12 @@ void launch_wpost(hipStream_t st, unsigned blocks, const float* b, const float* 
 	hipLaunchKernelGGL((k_wpost<VK, NPRE, NPOST, Epi>), dim3(blocks), dim3(BLOCK), 0, st, b, ec, out, L, X, rows, cols,
 	                   cr, cc, ns, nseg, sh, epi, rs, skip);
 }
-template <bool VK, int NPRE>
+template <bool VK, int NPRE, class Upd = NoUpd>
 void launch_wrestrict(hipStream_t st, unsigned blocks, const float* b, float* bco, const LWf& L, const fx_xtab& X,
-                      int fr, int fc, int cr, int cc, int ns, int nseg, int scr, const int* skip) {
-	hipLaunchKernelGGL((k_wrestrict<VK, NPRE>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr, cc, ns, nseg,
-	                   scr, skip);
+                      int fr, int fc, int cr, int cc, int ns, int nseg, int scr, const int* skip, Upd upd = Upd{},
+                      RedScratch rs = RedScratch{nullptr}) {
+	hipLaunchKernelGGL((k_wrestrict<VK, NPRE, Upd>), dim3(blocks), dim3(BLOCK), 0, st, b, bco, L, X, fr, fc, cr, cc, ns,
+	                   nseg, scr, upd, rs, skip);
 }
 
 // Segment heights of the wave-streaming kernels for a level: the tallest of
@@ -1794,8 +1845,18 @@ inline bool fused_ok(const fx_multigrid* mg) {
 	return g.rows * g.cols <= 4096;
 }
 
-template <bool VK, int NPRE, int NPOST, class Epi>
-fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
+// Whether level 0's wave restriction can also carry the FCG residual update
+// (Upd::ON): wave kernels, a level large enough that every strip and segment
+// gathers, and consecutive strips at most 57 fine columns apart (column
+// ownership in 64-byte pieces).
+inline bool upd_ok(const fx_multigrid* mg) {
+	if (mg->kernels != FX_MGK_WAVE || mg->lv.size() < 2) return false;
+	const auto& L = mg->lv[0];
+	return wrestrict_ok(L.xt) && L.xt.adv <= 57 && L.g.rows >= 17 && L.g.cols >= 17;
+}
+
+template <bool VK, int NPRE, int NPOST, class Epi, class Upd>
+fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 	fx_backend* be = mg->be;
 	hipStream_t st = be->stream;
 	auto& lv = mg->lv;
@@ -1810,7 +1871,25 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 		const fx_grid& g = lv[l].g;
 		const fx_grid& cg = lv[l + 1].g;
 		if (l == 0) fx::probe_begin(be, FX_PROBE_VC_RESTRICT0);
-		if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt)) {
+		bool fused0 = false;
+		unsigned fin_blocks = 0;   // the fused update's partial sums, merged after the probe ends
+		if constexpr (Upd::ON) {
+			if (l == 0) {
+				// Level 0 forms its rhs from the FCG residual update (the caller checked upd_ok).
+				int scr = tuned(mg->wrows_restrict, l);
+				if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
+				const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
+				const unsigned blocks = wave_blocks(ns * nseg);
+				FX_CHECK(fx::ensure_partials(be, blocks));
+				launch_wrestrict<VK, NPRE>(st, blocks, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lwf(mg, lv[l]),
+				                           lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns,
+				                           (int)nseg, scr, skip, upd0, fx::red_scratch(be));
+				fin_blocks = blocks;
+				fused0 = true;
+			}
+		}
+		if (fused0) {
+		} else if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt)) {
 			int scr = tuned(mg->wrows_restrict, l);
 			if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
 			const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
@@ -1831,6 +1910,11 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0) {
 				                   (int64_t)cg.rows, (int64_t)cg.cols, skip);
 		}
 		if (l == 0) fx::probe_end(be, FX_PROBE_VC_RESTRICT0);
+		if constexpr (Upd::ON) {
+			if (fin_blocks)
+				hipLaunchKernelGGL((k_finish<1, typename Upd::Fin>), dim3(1), dim3(FIN_BLOCK), 0, st,
+				                   fx::red_scratch(be), fin_blocks, upd0.fin(), skip);
+		}
 	}
 	if (lt < nl - 1) {
 		ta.rhs0 = (const float*)lv[lt].rhs;
@@ -1984,22 +2068,24 @@ inline fx_status autotune(fx_multigrid* mg, int reps, double* ms_out) {
 	return FX_ERR_UNSUPPORTED;
 }
 
-template <bool VK, class Epi>
-fx_status fast_cycle_v(fx_multigrid* mg, const int* skip, Epi epi0) {
+template <bool VK, class Epi, class Upd>
+fx_status fast_cycle_v(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 	const int a = mg->opt.n_pre, b = mg->opt.n_post;
-	if (a == 2 && b == 2) return fast_cycle_t<VK, 2, 2>(mg, skip, epi0);
-	if (a == 1 && b == 1) return fast_cycle_t<VK, 1, 1>(mg, skip, epi0);
-	if (a == 1 && b == 2) return fast_cycle_t<VK, 1, 2>(mg, skip, epi0);
-	return fast_cycle_t<VK, 2, 1>(mg, skip, epi0);
+	if (a == 2 && b == 2) return fast_cycle_t<VK, 2, 2>(mg, skip, epi0, upd0);
+	if (a == 1 && b == 1) return fast_cycle_t<VK, 1, 1>(mg, skip, epi0, upd0);
+	if (a == 1 && b == 2) return fast_cycle_t<VK, 1, 2>(mg, skip, epi0, upd0);
+	return fast_cycle_t<VK, 2, 1>(mg, skip, epi0, upd0);
 }
 
 // One fused V-cycle with a zero initial guess.  The level-0 rhs must already be
-// in lv[0].rhs (FP32); the result is written to lv[0].tmp, with epi0's
+// in lv[0].rhs (FP32), unless upd0 (Upd::ON, allowed when upd_ok) forms it
+// from the FCG residual update in the level-0 restriction, followed by
+// k_finish on upd0.fin(); the result is written to lv[0].tmp, with epi0's
 // per-point terms reduced over level 0 (k_finish runs epi0.fin() on the sums).
-template <class Epi>
-fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0) {
-	if (mg->lv[0].k) return fast_cycle_v<true>(mg, skip, epi0);
-	return fast_cycle_v<false>(mg, skip, epi0);
+template <class Epi, class Upd = NoUpd>
+fx_status fast_cycle(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0 = Upd{}) {
+	if (mg->lv[0].k) return fast_cycle_v<true>(mg, skip, epi0, upd0);
+	return fast_cycle_v<false>(mg, skip, epi0, upd0);
 }
 
 }  // namespace fmg

@@ -931,6 +931,29 @@ struct FXR32 {
 	}
 };
 // Level-0 epilogues of the fused V-cycle: r·z (start) and (r·z, z·Ap) (iterations).
+// The FCG x/r update's residual part, carried by the level-0 wave restriction
+// of the fused V-cycle (fmg::upd_ok): r_new = r − α·A·p with k_fr32v's
+// arithmetic at every point, read from r and stored to a second vector (other
+// waves still read r around the points a wave owns), and r_new·r_new reduced
+// into FinPcgRRDefer, whose deferred x update is along p.
+struct RUpd {
+	static constexpr bool ON = true;
+	const double *r, *ap;
+	double* rn;
+	SState* st;
+	const double* p;
+	double al;
+	struct Pf {
+		double r, ap;
+	};
+	__device__ void begin() { al = st->alpha; }
+	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c], (ap + o)[c]}; }
+	__device__ double rnew(const Pf& v) const { return fma(-al, v.ap, v.r); }
+	__device__ void store(size_t i, double v) const { rn[i] = v; }
+	using Fin = FinPcgRRDefer;
+	Fin fin() const { return Fin{st, p}; }
+};
+
 // RHS_FROM_R: the level-0 rhs of the V-cycle these epilogues end is float(r)
 // (written by the residual and x/r-update kernels), so the wave post kernel
 // takes it from the residual it loads for the reduction.
@@ -1672,6 +1695,11 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 			}
 			FX_CHECK(fmg::fast_cycle(fm, c.skip, EpiRZ{r, c.d}));
 			const bool even = c.cols % 2 == 0;
+			// With the residual update carried by the V-cycle's level-0 restriction, r alternates
+			// between two vectors (the restriction reads one while it writes the other).
+			const bool rupd = even && fmg::upd_ok(fm);
+			double* r2 = rupd ? s->vec(5) : nullptr;
+			FX_REQUIRE(!rupd || r2, FX_ERR_ALLOC, "work vector allocation failed");
 			FX_CHECK(drive_counted(s, [&](int64_t i) -> fx_status {
 				double* pold = pp[i & 1];
 				double* pnew = pp[(i + 1) & 1];
@@ -1698,6 +1726,11 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 					hipLaunchKernelGGL((k_finish<1, FinAlphaDefer>), dim3(1), dim3(FIN_BLOCK), 0, c.st,
 					                   fx::red_scratch(c.be), grid.x * grid.y, FinAlphaDefer{c.d}, c.done);
 					FX_CHECK(check_launch("fcg fused step"));
+					if (rupd) {
+						double* rold = (i & 1) ? r2 : r;
+						double* rnew = (i & 1) ? r : r2;
+						return fmg::fast_cycle(fm, c.skip, EpiFcg{rnew, ap, c.d}, RUpd{rold, ap, rnew, c.d, pnew, 0.0});
+					}
 					const size_t n2 = c.n / 2;
 					// About four pairs per thread, unrolled, so each thread keeps several loads in flight.
 					const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>(16384, (n2 + 4 * BLOCK - 1) / (4 * BLOCK)));

@@ -417,6 +417,45 @@ def test_fused_vcycle_kernel_variants(be, shape, vk):
     assert np.array_equal(run(), ref)
 
 
+@pytest.mark.parametrize("shape,vk", [((256, 256), True), ((130, 200), True), ((301, 128), False), ((1024, 1000), True)])
+def test_fcg_residual_update_in_restriction(be, shape, vk):
+    # With the wave kernels the FCG residual update rides in the level-0 restriction (each point stored by
+    # the wave owning it, r alternating between two vectors); with the LDS-ring kernels it is a separate
+    # kernel.  The V-cycles are bitwise equal and the update's arithmetic is the same; only the order of
+    # the block sums in the reductions differs (last-bit differences in α and β), so the solves agree to
+    # rounding.  A point the update missed would leave a stale residual and miss this by far.
+    hip = H()
+    from fluxion_amd.core import MgKernels
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    y, xx = np.linspace(0, 1, rows)[:, None], np.linspace(0, 1, cols)[None, :]
+    f = (np.sin(3 * xx + 1) * np.cos(2 * y) + 0.3 * np.exp(-40 * ((xx - 0.4) ** 2 + (y - 0.6) ** 2))).ravel()
+    f = O.zero_boundaries(f, rows, cols)
+    tol = 1e-9 * np.linalg.norm(f)
+    B = hip.Field.from_f64(g, f, be)
+    out = []
+    for kern in (MgKernels.RING, MgKernels.WAVE):
+        if vk:
+            k = np.exp(1.2 * np.sin(7 * xx + 1) * np.cos(5 * y) + 0.5 * np.sin(23 * xx * y)).ravel()
+            K = hip.Field.from_f64(g, k, be)
+            mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
+                               precision=DType.F32, fused=True, scale=1.0, k=K, be=be)
+            mg.set_kernels(kern)
+            x = hip.Field.zeros(g, be=be)
+            s = hip.FcgSolver(g, tol, 300, be=be).solve_op(x, B, hip.Operator.var_k(K), mg)
+        else:
+            st = Stencil.poisson_minus_identity(1.0 / (cols - 1), 1.0 / (rows - 1))
+            mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
+                               precision=DType.F32, fused=True, scale=-1.0, be=be)
+            mg.set_kernels(kern)
+            x = hip.Field.zeros(g, be=be)
+            s = hip.FcgSolver(g, tol, 300, be=be).solve_preconditioned(x, B, 1.0, st, ZD, mg)
+        assert s.converged() and s.true_residual < tol, kern.name
+        out.append((x.to_vec_f64(), s.iterations))
+    assert out[0][1] == out[1][1]
+    assert rel_l2(out[1][0], out[0][0]) < 1e-11
+
+
 @pytest.mark.parametrize("n", [256, 255])
 def test_vark_fcg_fused_mg(be, n):
     # −∇·(k∇u) = f with log-normal k: FCG + fused FP32 coefficient-aware V-cycle against the oracle.
