@@ -69,10 +69,9 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
 # profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels.
 MEASURED_TRAFFIC_4096 = {
-    "vc_restrict0": (159304 * 2 + 16400) * 1024,
-    "vc_post0": (331539 * 2 + 67647) * 1024,
-    "fcg_step": (186508 * 2 + 262208) * 1024,
-    "fcg_update": (262167 * 2 + 327744) * 1024,
+    "vc_restrict0": (263700 * 2 + 151241) * 1024,
+    "vc_post0": (300292 * 2 + 68802) * 1024,
+    "fcg_step": (246339 * 2 + 393248) * 1024,
 }
 
 
