@@ -47,6 +47,7 @@ struct fx_multigrid {
 	// streaming) or FX_MGK_TILED; levels a variant cannot run fall back to the tiled kernels.
 	int kernels = 2;
 	std::vector<int> wrows_post, wrows_restrict;   // tuned wave segment heights per level (0: heuristic)
+	int wrows_restrict_upd = 0;   // level 0's restriction carrying the FCG residual update (0: as wrows_restrict)
 	std::vector<Level> lv;
 };
 

@@ -342,6 +342,20 @@ constexpr int PTX = 64, PTY = 16, PH = 3;
 constexpr int PRY = PTY + 2 * PH, PRX = PTX + 2 * PH;
 static_assert(PTX == WAVE, "one tile column per lane in the final sweep");
 
+// The fused residual update's memory traffic on scratch vectors, for timing it
+// (the autotuner): r and A·p read, r_new stored, α = 0.
+struct TuneUpd {
+	static constexpr bool ON = true;
+	const double *r, *ap;
+	double* rn;
+	struct Pf {
+		double r, ap;
+	};
+	__device__ void begin() {}
+	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c], (ap + o)[c]}; }
+	__device__ double rnew(const Pf& v) const { return fma(-0.0, v.ap, v.r); }
+	__device__ void store(size_t i, double v) const { rn[i] = v; }
+};
 // No residual update in the wave restriction (its rhs is read as it is).
 struct NoUpd {
 	static constexpr bool ON = false;
@@ -1876,7 +1890,7 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 		if constexpr (Upd::ON) {
 			if (l == 0) {
 				// Level 0 forms its rhs from the FCG residual update (the caller checked upd_ok).
-				int scr = tuned(mg->wrows_restrict, l);
+				int scr = mg->wrows_restrict_upd > 0 ? mg->wrows_restrict_upd : tuned(mg->wrows_restrict, l);
 				if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
 				const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
 				const unsigned blocks = wave_blocks(ns * nseg);
@@ -1997,6 +2011,7 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 	const int nl = (int)lv.size();
 	mg->wrows_post.assign(nl, 0);
 	mg->wrows_restrict.assign(nl, 0);
+	mg->wrows_restrict_upd = 0;
 	hipEvent_t e0, e1;
 	FX_HIP(hipEventCreate(&e0));
 	FX_HIP(hipEventCreate(&e1));
@@ -2048,6 +2063,32 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 		}
 		mg->wrows_restrict[l] = pick;
 		total += best;
+		if (l == 0 && upd_ok(mg)) {
+			// Level 0's restriction as the FCG loop runs it, with the residual update (three scratch
+			// vectors of the level's size; the bytes per row change, and with them the best height).
+			const size_t n0 = (size_t)g.rows * g.cols;
+			void* sc = nullptr;
+			status = fx::dev_alloc(be, 3 * n0 * sizeof(double), &sc);
+			if (status != FX_OK) break;
+			const TuneUpd tu{(const double*)sc, (const double*)sc + n0, (double*)sc + 2 * n0};
+			float bu = INFINITY;
+			for (int scr = 1; scr <= 256 && status == FX_OK; scr *= 2) {
+				if (scr > 1 && scr / 2 >= cg.rows - 2) break;
+				const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
+				const unsigned blocks = wave_blocks(ns * nseg);
+				status = fx::ensure_partials(be, blocks);
+				if (status != FX_OK) break;
+				float ms = 0.0f;
+				status = time_it([&] {
+					launch_wrestrict<VK, NPRE>(st, blocks, (const float*)lv[l].rhs, (float*)lv[l + 1].rhs, lw, lv[l].xt,
+					                           (int)g.rows, (int)g.cols, (int)cg.rows, (int)cg.cols, (int)ns, (int)nseg,
+					                           scr, nullptr, tu, fx::red_scratch(be));
+				}, &ms);
+				if (ms < bu) bu = ms, mg->wrows_restrict_upd = scr;
+			}
+			(void)hipStreamSynchronize(st);
+			fx::dev_free(sc);
+		}
 	}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
