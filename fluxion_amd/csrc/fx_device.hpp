@@ -127,6 +127,17 @@ struct RedScratch {
 	Acc* partials;        // [nblocks * NS]
 };
 
+// Non-temporal stores, for results the next kernels do not read back from
+// the caches (a later kernel streams them from HBM): they leave the L2 and
+// the MALL to the data that is reused, which measurably speeds up the kernel
+// that follows a large writer.
+typedef double fx_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(double* p, double2 v) {
+	__builtin_nontemporal_store(fx_d2v{v.x, v.y}, (fx_d2v*)p);
+}
+__device__ __forceinline__ void store_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 // Writes this block's partial sums to partials[block * NS + k].  The global
 // sums are formed by k_finish in the next launch on the same stream: the
 // kernel boundary orders the partial stores before their loads, so no

@@ -1278,7 +1278,9 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				const int rz = t - LZ;
 				const float z = in(rz) ? zv : c;
 				if (outlane && rz >= S0 && rz < S1) {
-					(out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
+					// The solver's level 0 (an epilogue): z is read back only by the next FCG step.
+					if constexpr (Epi::NS > 0) store_nt((out + (size_t)rz * (size_t)cols) + (unsigned)gc, z);
+					else (out + (size_t)rz * (size_t)cols)[(unsigned)gc] = z;
 					epi.use_lane(pfb[k], z, lsum);
 				}
 				// Refill: the slot of row t − LZ (no longer read) with row t + WP − LZ, and this step's operands.

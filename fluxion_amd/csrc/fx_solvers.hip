@@ -546,9 +546,9 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 			}
 		}
 		if (ok) {
-			*(double2*)(pnew + i) = cur.p;
-			*(double2*)(ap + i) = av;
-			if (xu) *(double2*)(x + i) = cur.xn;
+			store_nt(pnew + i, cur.p);
+			store_nt(ap + i, av);
+			if (xu) store_nt(x + i, cur.xn);
 			acc_add(acc[0], cur.p.x * av.x);
 			acc_add(acc[0], cur.p.y * av.y);
 		}
@@ -949,7 +949,7 @@ struct RUpd {
 	__device__ void begin() { al = st->alpha; }
 	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c], (ap + o)[c]}; }
 	__device__ double rnew(const Pf& v) const { return fma(-al, v.ap, v.r); }
-	__device__ void store(size_t i, double v) const { rn[i] = v; }
+	__device__ void store(size_t i, double v) const { store_nt(rn + i, v); }
 	using Fin = FinPcgRRDefer;
 	Fin fin() const { return Fin{st, p}; }
 };
