@@ -455,26 +455,28 @@ __global__ __launch_bounds__(BLOCK) void k_resid2(const double* __restrict__ x, 
 
 // FCG step of the fused loop on grids with an even column count, two columns
 // per thread (16-byte loads and stores): p = double(z) + β·p_old, ap = A·p
-// (identity boundary rows), p·ap; with st->xpend it also applies the previous
-// iteration's deferred x += α·p_old (the x/r update then reads neither x nor
-// p), the same fma on the same operands, only later.  A is the 5-point stencil operator
-// p + coeff·S(p) (chain order of fxd::chain5) or, with VK, the
-// variable-coefficient operator of k_stream_vk.  The west and east neighbours
-// of a thread's pair come from the adjacent lanes (DPP); only the first and
-// last lane of a wave read the column beyond the wave, and do so together with
-// the row's other loads.  Every lane walks the rows (addresses clamped into
-// the grid), so the cross-lane moves always see all 64 lanes.  Per-point
+// (identity boundary rows), p·ap; after the first iteration (!FIRST) it also
+// applies the previous iteration's deferred x += α·p_old (the x/r update then
+// reads neither x nor p), the same fma on the same operands, only later.  A is
+// the 5-point stencil operator p + coeff·S(p) (chain order of fxd::chain5) or,
+// with VK, the variable-coefficient operator of k_stream_vk.  The west and
+// east neighbours of a thread's pair come from the adjacent lanes (DPP); the
+// first and last lane of a wave read the column beyond the wave with the
+// row's other loads (the other lanes read their own column there, unused).
+// Rows are loaded four at a time, unconditionally from clamped addresses, and
+// combined (p, x + α·p_old) only when they are used, so that four rows' loads
+// are in flight together.  Every lane
+// walks the rows, so the cross-lane moves always see all 64 lanes.  Per-point
 // arithmetic equals k_stream_st / k_stream_vk.
-template <bool VK>
+template <bool VK, bool FIRST>
 __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z, const double* __restrict__ pold,
                                                      double* __restrict__ pnew, double* __restrict__ ap, W9 w,
                                                      double coeff, const double* __restrict__ k, double sigma,
                                                      double ix2, double iy2, int64_t rows, int64_t cols, int rpb,
-                                                     RedScratch rs, const SState* st, double* __restrict__ x,
-                                                     int first) {
+                                                     RedScratch rs, const SState* st, double* __restrict__ x) {
 	if (st->done) return;
+	constexpr bool XU = !FIRST;   // a deferred x update is pending after every iteration but the first
 	const double be = st->beta;
-	const bool xu = st->xpend != 0;
 	const double al = st->alpha;
 	int bx, by;
 	xcd_tile(&bx, &by);
@@ -485,44 +487,62 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 	const bool ok = c0 < cols;
 	const int64_t cl = ok ? c0 : cols - 2;   // clamped pair column
 	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;   // column interior tests (c0 + 1 > 0 always)
-	const bool edgeW = lane == 0 && ok && c0 > 0, edgeE = lane == WAVE - 1 && ok && c0 + 2 < cols;
-	struct Row {
-		double2 p, k;
-		double pw, pe, kw, ke;   // p and k west of the pair (first lane) and east of it (last lane)
-		double2 x, xn;           // owned rows with a deferred x update: x and x + α·p_old
+	// Column beside the pair: west of it for the first lane, east for the last (clamped; a column a
+	// lane does not use, where it would leave the grid).
+	const int64_t cb = lane == 0 ? (cl > 0 ? cl - 1 : cl) : (lane == WAVE - 1 ? (cl + 2 < cols ? cl + 2 : cl) : cl);
+	struct Raw {
+		float2 z;
+		double2 po, k, x;
+		float zb;
+		double pob, kb;
 	};
-	auto ld = [&](int64_t r, bool own) {
-		Row v;
-		const int64_t i = r * cols + cl;
-		const float2 zz = *(const float2*)(z + i);
-		const double2 pp = first ? double2{0.0, 0.0} : *(const double2*)(pold + i);   // first: p_old = 0
-		v.x = xu && own ? *(const double2*)(x + i) : double2{0.0, 0.0};
-		v.xn = double2{fma(al, pp.x, v.x.x), fma(al, pp.y, v.x.y)};
-		v.p = double2{double(zz.x) + be * pp.x, double(zz.y) + be * pp.y};
-		v.k = VK ? *(const double2*)(k + i) : double2{1.0, 1.0};
-		v.pw = v.pe = 0.0;
-		v.kw = v.ke = 1.0;
-		if (edgeW) {
-			v.pw = double(z[i - 1]) + be * (first ? 0.0 : pold[i - 1]);
-			if constexpr (VK) v.kw = k[i - 1];
+	auto ld = [&](int64_t r) {
+		const int64_t rc = r < 0 ? 0 : (r < rows ? r : rows - 1);
+		const int64_t i = rc * cols + cl, ib = rc * cols + cb;
+		Raw v;
+		v.z = *(const float2*)(z + i);
+		v.zb = z[ib];
+		if constexpr (FIRST) {
+			v.po = double2{0.0, 0.0};   // p_old = 0
+			v.pob = 0.0;
+		} else {
+			v.po = *(const double2*)(pold + i);
+			v.pob = pold[ib];
 		}
-		if (edgeE) {
-			v.pe = double(z[i + 2]) + be * (first ? 0.0 : pold[i + 2]);
-			if constexpr (VK) v.ke = k[i + 2];
+		if constexpr (VK) {
+			v.k = *(const double2*)(k + i);
+			v.kb = k[ib];
+		} else {
+			v.k = double2{1.0, 1.0};
+			v.kb = 1.0;
 		}
+		v.x = XU ? *(const double2*)(x + i) : double2{0.0, 0.0};
 		return v;
 	};
-	const Row none{double2{0.0, 0.0}, double2{1.0, 1.0}, 0.0, 0.0, 1.0, 1.0, double2{0.0, 0.0}, double2{0.0, 0.0}};
+	struct Row {
+		double2 p, k, xn;   // p, k and (XU) x + α·p_old of the pair
+		double pb, kb;      // p and k beside the pair
+	};
+	auto use = [&](const Raw& v) {
+		Row o;
+		o.p = double2{double(v.z.x) + be * v.po.x, double(v.z.y) + be * v.po.y};
+		o.k = v.k;
+		o.xn = double2{fma(al, v.po.x, v.x.x), fma(al, v.po.y, v.x.y)};
+		o.pb = double(v.zb) + be * v.pob;
+		o.kb = v.kb;
+		return o;
+	};
 	Acc acc[1] = {Acc{0.0, 0.0}};
-	Row up = r0 > 0 ? ld(r0 - 1, false) : none;
-	Row cur = ld(r0 < rows ? r0 : rows - 1, true);
-	for (int64_t r = r0; r < r1; ++r) {
+	Row up = use(ld(r0 - 1));   // rows outside the grid are only read by boundary rows, which do not use them
+	Row cur = use(ld(r0));
+	// Row r from up, cur and dn (row r + 1, formed from its loads).
+	auto row = [&](int64_t r, const Raw& nx) {
 		const int64_t i = r * cols + c0;
-		const Row dn = r + 1 < rows ? ld(r + 1, r + 1 < r1) : none;
+		const Row dn = use(nx);
 		double pw = dlane<0x138>(cur.p.y), pe = dlane<0x130>(cur.p.x);
 		double kw = dlane<0x138>(cur.k.y), ke = dlane<0x130>(cur.k.x);
-		if (lane == 0) pw = cur.pw, kw = cur.kw;
-		if (lane == WAVE - 1) pe = cur.pe, ke = cur.ke;
+		if (lane == 0) pw = cur.pb, kw = cur.kb;
+		if (lane == WAVE - 1) pe = cur.pb, ke = cur.kb;
 		const bool rint = r > 0 && r < rows - 1;
 		double2 av = cur.p;   // boundary rows and columns: identity
 		if (rint) {
@@ -548,12 +568,24 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_step2(const float* __restrict__ z
 		if (ok) {
 			store_nt(pnew + i, cur.p);
 			store_nt(ap + i, av);
-			if (xu) store_nt(x + i, cur.xn);
+			if (XU) store_nt(x + i, cur.xn);
 			acc_add(acc[0], cur.p.x * av.x);
 			acc_add(acc[0], cur.p.y * av.y);
 		}
 		up = cur;
 		cur = dn;
+	};
+	// SR rows per loop trip: the loads of the trip's SR next rows are issued together at its start and
+	// used within it, so no loaded value is carried round the loop (a carried one is copied between
+	// registers at the loop's end, which waits for every load in flight).
+	constexpr int SR = 4;
+	for (int64_t r = r0; r < r1; r += SR) {
+		Raw q[SR];
+#pragma unroll
+		for (int j = 0; j < SR; ++j) q[j] = ld(r + 1 + j);
+#pragma unroll
+		for (int j = 0; j < SR; ++j)
+			if (r + j < r1) row(r + j, q[j]);
 	}
 	block_partial<1>(acc, rs);
 }
@@ -1714,14 +1746,13 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 					const double* kp = vk ? (const double*)c.op->k->ptr : nullptr;
 					const double ix2 = 1.0 / (s->g.dx * s->g.dx), iy2 = 1.0 / (s->g.dy * s->g.dy);
 					fx::probe_begin(c.be, FX_PROBE_FCG_STEP);
-					if (vk)
-						hipLaunchKernelGGL(k_fcg_step2<true>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
-						                   c.op->coeff, kp, c.op->sigma, ix2, iy2, c.rows, c.cols, rpb,
-						                   fx::red_scratch(c.be), (const SState*)c.d, x, (int)(i == 0));
-					else
-						hipLaunchKernelGGL(k_fcg_step2<false>, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9,
-						                   c.op->coeff, kp, 0.0, ix2, iy2, c.rows, c.cols, rpb, fx::red_scratch(c.be),
-						                   (const SState*)c.d, x, (int)(i == 0));
+					auto step = [&](auto kern) {
+						hipLaunchKernelGGL(kern, grid, dim3(BLOCK), 0, c.st, z32, pold, pnew, ap, c.w9, c.op->coeff, kp,
+						                   vk ? c.op->sigma : 0.0, ix2, iy2, c.rows, c.cols, rpb, fx::red_scratch(c.be),
+						                   (const SState*)c.d, x);
+					};
+					if (vk) i == 0 ? step(k_fcg_step2<true, true>) : step(k_fcg_step2<true, false>);
+					else i == 0 ? step(k_fcg_step2<false, true>) : step(k_fcg_step2<false, false>);
 					fx::probe_end(c.be, FX_PROBE_FCG_STEP);
 					hipLaunchKernelGGL((k_finish<1, FinAlphaDefer>), dim3(1), dim3(FIN_BLOCK), 0, c.st,
 					                   fx::red_scratch(c.be), grid.x * grid.y, FinAlphaDefer{c.d}, c.done);
