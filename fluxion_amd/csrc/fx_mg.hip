@@ -707,7 +707,7 @@ fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t p
 	mg->wrows_restrict.resize(mg->lv.size(), 0);
 	mg->wrows_post[level] = post_rows;
 	mg->wrows_restrict[level] = restrict_rows;
-	if (level == 0) mg->wrows_restrict_upd = restrict_rows;
+	if (level == 0) mg->wrows_restrict_upd = restrict_rows, mg->wrows_post_epi = post_rows;
 	return FX_OK;
 }
 

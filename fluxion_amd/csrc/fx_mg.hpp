@@ -48,6 +48,7 @@ struct fx_multigrid {
 	int kernels = 2;
 	std::vector<int> wrows_post, wrows_restrict;   // tuned wave segment heights per level (0: heuristic)
 	int wrows_restrict_upd = 0;   // level 0's restriction carrying the FCG residual update (0: as wrows_restrict)
+	int wrows_post_epi = 0;       // level 0's post kernel with a solver epilogue (0: as wrows_post)
 	std::vector<Level> lv;
 };
 

@@ -356,6 +356,22 @@ struct TuneUpd {
 	__device__ double rnew(const Pf& v) const { return fma(-0.0, v.ap, v.r); }
 	__device__ void store(size_t i, double v) const { rn[i] = v; }
 };
+// A solver epilogue's memory traffic and reductions on scratch vectors (r and
+// A·p read, the rhs taken from r), for timing level 0's post kernel.
+struct TuneEpi {
+	static constexpr int NS = 2;
+	static constexpr bool RHS_FROM_R = true;
+	const double *r, *ap;
+	struct Pf {
+		double r, ap;
+	};
+	__device__ Pf load_at(size_t o, unsigned c) const { return Pf{(r + o)[c], (ap + o)[c]}; }
+	__device__ void use_lane(const Pf& p, float z, double* s) const {
+		s[0] += p.r * double(z);
+		s[1] += double(z) * p.ap;
+	}
+	__device__ float rhs(const Pf& p) const { return float(p.r); }
+};
 // No residual update in the wave restriction (its rhs is read as it is).
 struct NoUpd {
 	static constexpr bool ON = false;
@@ -1957,7 +1973,7 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 	auto post_shape = [&](int l) {
 		const fx_grid& g = lv[l].g;
 		if (mg->kernels == FX_MGK_WAVE) {
-			int sh = tuned(mg->wrows_post, l);
+			int sh = l == 0 && Epi::NS > 0 && mg->wrows_post_epi > 0 ? mg->wrows_post_epi : tuned(mg->wrows_post, l);
 			if (sh <= 0) sh = wpost_rows(g.rows, g.cols);
 			const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
 			return PostShape{FX_MGK_WAVE, wave_blocks(ns * nseg), (int)ns, (int)nseg, sh};
@@ -2014,6 +2030,7 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 	mg->wrows_post.assign(nl, 0);
 	mg->wrows_restrict.assign(nl, 0);
 	mg->wrows_restrict_upd = 0;
+	mg->wrows_post_epi = 0;
 	hipEvent_t e0, e1;
 	FX_HIP(hipEventCreate(&e0));
 	FX_HIP(hipEventCreate(&e1));
@@ -2087,6 +2104,23 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 					                           scr, nullptr, tu, fx::red_scratch(be));
 				}, &ms);
 				if (ms < bu) bu = ms, mg->wrows_restrict_upd = scr;
+			}
+			// Level 0's post kernel as the FCG loop runs it, with the solver's epilogue traffic.
+			const TuneEpi te{(const double*)sc, (const double*)sc + n0};
+			bu = INFINITY;
+			for (int sh = 2; sh <= 512 && status == FX_OK; sh *= 2) {
+				if (sh > 2 && sh / 2 >= g.rows) break;
+				const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
+				const unsigned blocks = wave_blocks(ns * nseg);
+				status = fx::ensure_partials(be, (size_t)blocks * TuneEpi::NS);
+				if (status != FX_OK) break;
+				float ms = 0.0f;
+				status = time_it([&] {
+					launch_wpost<VK, NPRE, NPOST>(st, blocks, (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp,
+					                              (float*)lv[l].tmp, lw, lv[l].xt, (int)g.rows, (int)g.cols, (int)cg.rows,
+					                              (int)cg.cols, (int)ns, (int)nseg, sh, te, fx::red_scratch(be), nullptr);
+				}, &ms);
+				if (ms < bu) bu = ms, mg->wrows_post_epi = sh;
 			}
 			(void)hipStreamSynchronize(st);
 			fx::dev_free(sc);
