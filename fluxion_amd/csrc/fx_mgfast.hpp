@@ -1216,18 +1216,21 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 			vx = 1.0f - wx;
 		}
 		auto inter = [&](int r) { return colint && r >= 1 && r <= rows - 2; };
-		// Coarse-correction operands of a row (read WP steps before the row's iterate is formed).
+		// Coarse-correction operands of a row (read WP steps before the row's iterate is formed): the
+		// coarse row a0 below the fine row, and the values of the row a0 + 1 above it.  The values of
+		// row a0 are those of row a0 + 1 of an earlier fine row (or of the wave's first), since a0
+		// advances by at most one per fine row, so a row costs two loads, not four.
 		struct EV {
-			float e00, e01, e10, e11, wy;
+			float e10, e11, wy;
+			int a0;
 		};
 		const double rinv = 1.0 / double(rows - 1);
 		LinRow le;  // row of the next coarse-correction load
 		auto eload = [&]() {
 			const int a0 = le.i0;
 			const int a1 = a0 + 1 < cr ? a0 + 1 : a0;
-			const float* e0 = ec + (size_t)a0 * (size_t)cc;
 			const float* e1 = ec + (size_t)a1 * (size_t)cc;
-			const EV v{e0[b0], e0[b1], e1[b0], e1[b1], float(le.w(rinv))};
+			const EV v{e1[b0], e1[b1], float(le.w(rinv)), a0};
 			le.next();
 			return v;
 		};
@@ -1240,6 +1243,10 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 		// grid-edge fixes (cl), so that each variant gets its own registers.
 		auto walk = [&](auto cl) {
 			le.init(T0 - LU, rows, cr);
+			// Row a0 of the current fine row (ca), its values (e00, e01) and row a0 + 1's of the last one.
+			int ca = le.i0;
+			float e00 = (ec + (size_t)ca * (size_t)cc)[b0], e01 = (ec + (size_t)ca * (size_t)cc)[b1];
+			float p10 = 0.0f, p11 = 0.0f;
 			// Input ring: row T0 + j in slot j mod WP; the first WP − LZ rows now, the rest as slots free up.
 			WRow rb[WP];
 			EV evb[WP];
@@ -1274,8 +1281,13 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 				}
 				{
 					const EV& ev = evb[k];
+					const bool adv = ev.a0 != ca;   // the coarse row below moved up by one
+					e00 = adv ? p10 : e00;
+					e01 = adv ? p11 : e01;
+					ca = ev.a0;
+					p10 = ev.e10, p11 = ev.e11;
 					const float wy = ev.wy, vy = 1.0f - wy;
-					const float top = vx * ev.e00 + wx * ev.e01;
+					const float top = vx * e00 + wx * e01;
 					const float bot = vx * ev.e10 + wx * ev.e11;
 					u += vy * top + wy * bot;
 				}
