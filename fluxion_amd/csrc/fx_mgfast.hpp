@@ -1312,9 +1312,12 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 					epi.use_lane(pfb[k], z, lsum);
 				}
 				// Refill: the slot of row t − LZ (no longer read) with row t + WP − LZ, and this step's operands.
-				rb[Slot<WP, k, LZ>::v] = wload<VK, LOADB>(b, L, t + WP - LZ, rows, cols, uc);
+				// Rows past the walk's last are never used: the last one is read again instead (a cache
+				// hit), so that a segment does not fetch the next segment's rows from memory.
+				const int tn = t + WP - LZ < T1 ? t + WP - LZ : T1 - 1;
+				rb[Slot<WP, k, LZ>::v] = wload<VK, LOADB>(b, L, tn, rows, cols, uc);
 				evb[k] = eload();
-				pfb[k] = pload(t + WP - LZ);
+				pfb[k] = pload(tn);
 		};
 		run_steps<WP>(step, T0, T1, cl);
 		};
@@ -1461,7 +1464,7 @@ __device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, floa
 						upd.store((size_t)t * (size_t)fc + (unsigned)gc, rn);
 						lsum += rn * rn;
 					}
-					ub[k] = uload(t + WQ - LR);
+					ub[k] = uload(t + WQ - LR < T1 ? t + WQ - LR : T1 - 1);   // rows past the walk: never used
 				}
 				rb[k] = CL ? v : wfix<VK>(v, t, fr, colin, kwok, keok);
 			}
@@ -1483,8 +1486,9 @@ __device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, floa
 			const WRow& ri = rb[Slot<WQ, k, LR>::v];
 			const float sv = wop<VK>(L, ri, c, Uv[Slot<WR, q, 2>::v], Uv[q], wv, evv);
 			const float res = in(i) ? fmaf(1.0f, ri.b, -sv) : 0.0f;
-			// Refill the slot of row t − LR (no longer read) with row t + WQ − LR.
-			rb[Slot<WQ, k, LR>::v] = wload<VK, LOADB>(b, L, t + WQ - LR, fr, fc, uc);
+			// Refill the slot of row t − LR (no longer read) with row t + WQ − LR (rows past the walk's
+			// last are never used: the last is read again, a cache hit, instead of the next segment's).
+			rb[Slot<WQ, k, LR>::v] = wload<VK, LOADB>(b, L, t + WQ - LR < T1 ? t + WQ - LR : T1 - 1, fr, fc, uc);
 			// Row sum over the owned coarse column's fine columns.
 			float row = 0.0f;
 #pragma unroll
