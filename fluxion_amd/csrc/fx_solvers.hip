@@ -1345,7 +1345,10 @@ struct fx_solver {
 	double* dgm = nullptr;       // GMRES H, cc, ss, grs, y
 	int* hdone = nullptr;        // coherent pinned host ints: [0] copy of the device `done` flag, [1] iteration count
 	hipEvent_t ev[8] = {};
-	std::vector<double> hist;
+	// Residual history of the last pass: hist_n entries in dhist, copied to hist when first asked for.
+	mutable std::vector<double> hist;
+	mutable bool hist_stale = false;
+	int64_t hist_n = 0;
 	double* vec(int i) {
 		while ((int)v.size() <= i) {
 			void* p = nullptr;
@@ -1619,19 +1622,16 @@ fx_status drive_counted(fx_solver* s, Step step, int64_t max_steps) {
 		}
 		if (__atomic_load_n(done, __ATOMIC_ACQUIRE)) break;
 	}
-	FX_HIP(hipStreamSynchronize(st));
+	// No synchronisation: the caller's next work (x flush, true residual, statistics copy) is ordered
+	// after the loop's kernels on the stream, and the statistics copy synchronises.
 	return FX_OK;
 }
 
 fx_status fetch_stats(fx_solver* s, fx_solve_stats* out) {
 	FX_HIP(hipMemcpyAsync(s->hst, s->dst, sizeof(SState), hipMemcpyDeviceToHost, s->be->stream));
 	FX_HIP(hipStreamSynchronize(s->be->stream));
-	int64_t nh = std::min<int64_t>(s->hst->iter + 1, HIST_MAX);
-	s->hist.assign((size_t)std::max<int64_t>(nh, 0), 0.0);
-	if (nh > 0) {
-		FX_HIP(hipMemcpyAsync(s->hist.data(), s->dhist, sizeof(double) * nh, hipMemcpyDeviceToHost, s->be->stream));
-		FX_HIP(hipStreamSynchronize(s->be->stream));
-	}
+	s->hist_n = std::max<int64_t>(std::min<int64_t>(s->hst->iter + 1, HIST_MAX), 0);
+	s->hist_stale = true;   // copied by fx_solver_history, when asked for
 	if (out) {
 		out->iterations = (uint64_t)s->hst->iter;
 		out->residual = s->hst->resid;
@@ -2393,6 +2393,16 @@ const char* fx_solver_name(const fx_solver* s) { return s ? solver_name(s->kind)
 
 size_t fx_solver_history(const fx_solver* s, double* out, size_t len) {
 	if (!s) return 0;
+	if (s->hist_stale) {
+		s->hist.assign((size_t)s->hist_n, 0.0);
+		if (s->hist_n > 0 && (hipMemcpyAsync(s->hist.data(), s->dhist, sizeof(double) * s->hist_n, hipMemcpyDeviceToHost,
+		                                     s->be->stream) != hipSuccess ||
+		                      hipStreamSynchronize(s->be->stream) != hipSuccess)) {
+			s->hist.clear();
+			return 0;
+		}
+		s->hist_stale = false;
+	}
 	size_t n = std::min(len, s->hist.size());
 	if (out && n) std::memcpy(out, s->hist.data(), n * sizeof(double));
 	return s->hist.size();
