@@ -456,6 +456,40 @@ def test_fcg_residual_update_in_restriction(be, shape, vk):
     assert rel_l2(out[1][0], out[0][0]) < 1e-11
 
 
+def test_fused_fcg_limits_restarts_and_guess(be):
+    # The fused FCG loop's deferred x update and alternating residual vectors across the ways a solve
+    # ends: the iteration limit (the pending x update flushed), refinement passes to a tolerance near
+    # FP64's attainable accuracy (each pass restarting both), and a nonzero initial guess.  The wave
+    # kernels (update in the restriction) must agree with the LDS-ring kernels (separate update).
+    hip = H()
+    from fluxion_amd.core import MgKernels
+    n = 256
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    f = gaussian_bumps(n).ravel()
+    K = hip.Field.from_f64(g, k, be)
+    B = hip.Field.from_f64(g, f, be)
+    x0 = O.zero_boundaries(np.random.default_rng(5).standard_normal(n * n) * 1e-3, n, n)
+    fn = np.linalg.norm(f)
+    for max_iter, tol, guess in ((5, 1e-10 * fn, False), (200, 1e-14 * fn, False), (200, 1e-9 * fn, True)):
+        out = []
+        for kern in (MgKernels.RING, MgKernels.WAVE):
+            mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
+                               precision=DType.F32, fused=True, scale=1.0, k=K, be=be)
+            mg.set_kernels(kern)
+            x = hip.Field.from_f64(g, x0, be) if guess else hip.Field.zeros(g, be=be)
+            s = hip.FcgSolver(g, tol, max_iter, be=be).solve_op(x, B, hip.Operator.var_k(K), mg)
+            out.append((x.to_vec_f64(), s))
+        (xa, sa), (xb, sb) = out
+        assert sa.iterations == sb.iterations and sa.reason == sb.reason, (max_iter, tol)
+        assert rel_l2(xb, xa) < 1e-10, (max_iter, tol)
+        if max_iter == 5:
+            assert sb.reason == ConvergenceReason.MAX_ITERATIONS and sb.iterations == 5
+        # The reported true residual is that of the returned x (the flushed x update included).
+        tr = np.linalg.norm(f - O.apply_op(O.Op.var_k(k), xb, g))
+        assert abs(tr - sb.true_residual) <= 1e-6 * tr, (max_iter, tol, tr, sb.true_residual)
+
+
 @pytest.mark.parametrize("n", [256, 255])
 def test_vark_fcg_fused_mg(be, n):
     # −∇·(k∇u) = f with log-normal k: FCG + fused FP32 coefficient-aware V-cycle against the oracle.
