@@ -232,7 +232,10 @@ fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels);
 fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows);
 /* Autotuner (extension): times every level's wave kernels over candidate segment heights with HIP
  * events (reps launches each, on the hierarchy's own buffers, whose contents it overwrites) and keeps
- * the fastest per level; *ms_out (may be NULL) gets the summed per-launch time of the chosen shapes. */
+ * the fastest per level; *ms_out (may be NULL) gets the summed per-launch time of the chosen shapes.
+ * Level 0 is also timed as the fused FCG loop runs it (restriction with the residual update, post
+ * kernel with the solver's reductions), on scratch vectors it allocates and frees; those heights are
+ * kept apart.  fx_multigrid_set_wave_rows on level 0 sets both. */
 fx_status fx_multigrid_autotune(fx_multigrid* mg, int32_t reps, double* ms_out);
 
 /* Preconditioner handle (preconditioner.rs:10-37). */
