@@ -1646,12 +1646,16 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_ftail(TailArgs A, const int* ski
 			return jac(T.L, c, u[q - cols], u[q + cols], u[q - 1], u[q + 1], pb[T.off + q]);
 		}
 	};
-	// Point loops: lanes take columns, waves take rows (no index division).
-	const int tx = t & (WAVE - 1), ty = t / WAVE;
-	constexpr int NYW = TAIL_BLOCK / WAVE;
+	// Point loops: thread t takes points q = t + TAIL_BLOCK·j of the row-major plane, so that every
+	// thread has work on levels narrower than a wave too.  The row is ⌊(q + ½)/cols⌋ in FP32: exact
+	// while rows·cols < 2²¹ (the product's rounding error stays below the ½/cols margin).
 	auto pts = [&](int rows, int cols, auto f) {
-		for (int r = ty; r < rows; r += NYW)
-			for (int c = tx; c < cols; c += WAVE) f(r, c, r * cols + c);
+		const int n = rows * cols;
+		const float inv = 1.0f / float(cols);
+		for (int q = t; q < n; q += TAIL_BLOCK) {
+			const int r = int((float(q) + 0.5f) * inv);
+			f(r, q - r * cols, q);
+		}
 	};
 	auto inner = [](const TailLevel& T, int r, int c) { return r > 0 && c > 0 && r < T.rows - 1 && c < T.cols - 1; };
 	// ── down: pre-smoothing, residual, restriction ──
