@@ -59,6 +59,10 @@ class FxPrecond(C.Structure):
     _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("handle", C.c_void_p)]
 
 
+class FxCnOptions(C.Structure):
+    _fields_ = [("op", C.c_int32), ("solver", C.c_int32), ("k", c_buf_p), ("mg", C.POINTER(FxMgOptions))]
+
+
 def grid_c(g: core.Grid) -> FxGrid:
     return FxGrid(g.rows, g.cols, g.dx, g.dy)
 
@@ -171,6 +175,20 @@ _SIGS = {
                                     _P(C.c_void_p)]),
     "fx_learned_destroy": (None, [C.c_void_p]),
     "fx_learned_fell_back": (C.c_int32, [C.c_void_p]),
+    "fx_cn_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, _P(FxBoundaries), C.c_double,
+                               C.c_uint64, _P(FxCnOptions), _P(C.c_void_p)]),
+    "fx_cn_destroy": (None, [C.c_void_p]),
+    "fx_cn_step": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, _P(FxSolveStats)]),
+    "fx_cn_step_n": (C.c_int, [C.c_void_p, c_buf_p, C.c_uint64]),
+    "fx_cn_dt": (C.c_double, [C.c_void_p]),
+    "fx_cn_steps_done": (C.c_uint64, [C.c_void_p]),
+    "fx_cn_sim_time": (C.c_double, [C.c_void_p]),
+    "fx_cn_solver": (C.c_void_p, [C.c_void_p]),
+    "fx_cn_multigrid": (C.c_void_p, [C.c_void_p]),
+    "fx_write_raw": (C.c_int, [c_be_p, C.c_char_p, c_buf_p, _P(FxGrid)]),
+    "fx_read_raw": (C.c_int, [c_be_p, C.c_char_p, _P(FxGrid), _P(c_buf_p)]),
+    "fx_write_raw_host": (C.c_int, [C.c_char_p, C.c_void_p, _P(FxGrid)]),
+    "fx_read_raw_host": (C.c_int, [C.c_char_p, _P(FxGrid), C.c_void_p, C.c_size_t]),
 }
 
 _lib = None
@@ -214,6 +232,7 @@ _ERRORS = {
     2: core.DimensionMismatch,
     3: core.InvalidGrid,
     4: core.InvalidStencil,
+    10: core.IoError,
 }
 
 

@@ -41,6 +41,10 @@ class BackendError(CoreError):
     pass
 
 
+class IoError(CoreError, OSError):
+    """std::io::Error of the raw field format (crates/io/src/raw.rs:13-77)."""
+
+
 class DType(enum.IntEnum):
     F32 = 0
     F64 = 1

@@ -1,0 +1,271 @@
+// fx_implicit.hip — callers either side of the solve: the Crank–Nicolson
+// stepper (crates/runtime/src/implicit.rs:1-238) and the raw field format
+// (crates/io/src/raw.rs:1-77).
+//
+// The stepper is host code over the library's own entry points: one operator
+// pass assembles the right-hand side (the reference's copy + copy +
+// stencil_axpy, implicit.rs:187-194, in one read of u and one write), an axpy
+// adds the source, and the solver of the step runs from u^n as its initial
+// guess, so every step is warm-started as in the reference.
+#include <cerrno>
+#include <cmath>
+#include <cstring>
+
+#include "fx_internal.hpp"
+
+struct fx_cn {
+	fx_backend* be = nullptr;
+	fx_grid g{};
+	double alpha = 0, dt = 0;
+	uint64_t steps = 0;
+	fx_boundaries bc{};
+	fx_stencil_entry ent[5]{};
+	fx_stencil st{};
+	int32_t op_kind = FX_OP_STENCIL;
+	int32_t solver_kind = FX_SOLVER_CG;
+	const fx_buf* k = nullptr;
+	fx_solver* solver = nullptr;
+	fx_multigrid* mg = nullptr;
+	fx_buf* rhs = nullptr;
+};
+
+namespace {
+
+// Stencil::laplacian_2d_5pt (core/src/stencil.rs:56-72), entry order kept.
+void laplacian_5pt(double dx, double dy, fx_stencil_entry* e) {
+	const double inv_dx2 = 1.0 / (dx * dx);
+	const double inv_dy2 = 1.0 / (dy * dy);
+	e[0] = {0, 0, -2.0 * (inv_dx2 + inv_dy2)};
+	e[1] = {-1, 0, inv_dy2};
+	e[2] = {1, 0, inv_dy2};
+	e[3] = {0, -1, inv_dx2};
+	e[4] = {0, 1, inv_dx2};
+}
+
+// The step's two operators.  With h = (0.5·dt)·α (implicit.rs:183, evaluated
+// left to right as the reference writes it):
+//   stencil form: explicit u + h·S(u), implicit u − h·S(u)  (A·v = v + coeff·S(v),
+//                 identity rows on the boundary: cg.rs:104-111)
+//   variable k:   explicit u − h·(−∇·(k∇u)), implicit u + h·(−∇·(k∇u))
+fx_operator cn_operator(const fx_cn* cn, bool implicit_side) {
+	const double h = 0.5 * cn->dt * cn->alpha;
+	fx_operator op{};
+	op.kind = cn->op_kind;
+	op.sigma = 1.0;
+	if (cn->op_kind == FX_OP_STENCIL) {
+		op.coeff = implicit_side ? -h : h;
+		op.stencil = &cn->st;
+		op.bc = &cn->bc;
+	} else {
+		op.coeff = implicit_side ? h : -h;
+		op.k = cn->k;
+	}
+	return op;
+}
+
+void cn_free(fx_cn* cn) {
+	if (!cn) return;
+	if (cn->solver) fx_solver_destroy(cn->solver);
+	if (cn->mg) fx_multigrid_destroy(cn->mg);
+	if (cn->rhs) fx_buffer_free(cn->rhs);
+	delete cn;
+}
+
+// ── raw field format (io/src/raw.rs:7-77) ────────────────────────────
+constexpr size_t RAW_HEADER = 32;
+
+void put_le(unsigned char* p, uint64_t v) {
+	for (int i = 0; i < 8; ++i) p[i] = (unsigned char)(v >> (8 * i));
+}
+uint64_t get_le(const unsigned char* p) {
+	uint64_t v = 0;
+	for (int i = 0; i < 8; ++i) v |= (uint64_t)p[i] << (8 * i);
+	return v;
+}
+uint64_t f64_bits(double d) {
+	uint64_t u;
+	std::memcpy(&u, &d, 8);
+	return u;
+}
+double bits_f64(uint64_t u) {
+	double d;
+	std::memcpy(&d, &u, 8);
+	return d;
+}
+
+struct File {
+	FILE* f = nullptr;
+	~File() {
+		if (f) std::fclose(f);
+	}
+};
+
+fx_status raw_write(const char* path, const double* data, const fx_grid* g) {
+	FX_REQUIRE(path && data && g, FX_ERR_ARGUMENT, "null argument");
+	File f;
+	f.f = std::fopen(path, "wb");
+	FX_REQUIRE(f.f, FX_ERR_IO, "cannot create %s: %s", path, std::strerror(errno));
+	unsigned char h[RAW_HEADER];
+	put_le(h, g->rows);
+	put_le(h + 8, g->cols);
+	put_le(h + 16, f64_bits(g->dx));
+	put_le(h + 24, f64_bits(g->dy));
+	FX_REQUIRE(std::fwrite(h, 1, RAW_HEADER, f.f) == RAW_HEADER, FX_ERR_IO, "write to %s failed", path);
+	const size_t n = (size_t)(g->rows * g->cols);
+	std::vector<unsigned char> buf(8 * (size_t)65536);
+	for (size_t i0 = 0; i0 < n; i0 += 65536) {
+		const size_t m = std::min<size_t>(65536, n - i0);
+		for (size_t i = 0; i < m; ++i) put_le(buf.data() + 8 * i, f64_bits(data[i0 + i]));
+		FX_REQUIRE(std::fwrite(buf.data(), 8, m, f.f) == m, FX_ERR_IO, "write to %s failed", path);
+	}
+	FX_REQUIRE(std::fclose(f.f) == 0, FX_ERR_IO, "closing %s failed", path);
+	f.f = nullptr;
+	return FX_OK;
+}
+
+// Header, validated as Grid::new validates it (grid.rs:15-36); then, if data is
+// not null, exactly rows·cols values (a short file is an error, as read_exact's is).
+fx_status raw_read(const char* path, fx_grid* g, double* data, size_t len) {
+	FX_REQUIRE(path && g, FX_ERR_ARGUMENT, "null argument");
+	File f;
+	f.f = std::fopen(path, "rb");
+	FX_REQUIRE(f.f, FX_ERR_IO, "cannot open %s: %s", path, std::strerror(errno));
+	unsigned char h[RAW_HEADER];
+	FX_REQUIRE(std::fread(h, 1, RAW_HEADER, f.f) == RAW_HEADER, FX_ERR_IO, "%s: file shorter than its header",
+	           path);
+	fx_grid r{get_le(h), get_le(h + 8), bits_f64(get_le(h + 16)), bits_f64(get_le(h + 24))};
+	FX_CHECK(fx::check_grid(&r));
+	FX_REQUIRE(r.rows <= (uint64_t)1 << 31 && r.cols <= (uint64_t)1 << 31, FX_ERR_GRID,
+	           "%s: grid %llux%llu is too large", path, (unsigned long long)r.rows, (unsigned long long)r.cols);
+	*g = r;
+	if (!data) return FX_OK;
+	const size_t n = (size_t)(r.rows * r.cols);
+	FX_REQUIRE(len == n, FX_ERR_DIMENSION, "dimension mismatch: expected %zu, got %zu (data)", n, len);
+	std::vector<unsigned char> buf(8 * (size_t)65536);
+	for (size_t i0 = 0; i0 < n; i0 += 65536) {
+		const size_t m = std::min<size_t>(65536, n - i0);
+		FX_REQUIRE(std::fread(buf.data(), 8, m, f.f) == m, FX_ERR_IO, "%s: file holds fewer than %zu values", path,
+		           n);
+		for (size_t i = 0; i < m; ++i) data[i0 + i] = bits_f64(get_le(buf.data() + 8 * i));
+	}
+	return FX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fx_status fx_cn_create(fx_backend* be, const fx_grid* grid, double alpha, double dt, const fx_boundaries* bc,
+                       double cg_tol, uint64_t cg_max_iter, const fx_cn_options* opt, fx_cn** out) {
+	FX_REQUIRE(be && bc && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(fx::check_grid(grid));
+	FX_CHECK(fx::check_bc(bc));
+	FX_REQUIRE(std::isfinite(alpha) && std::isfinite(dt), FX_ERR_ARGUMENT, "alpha and dt must be finite");
+	fx_cn_options o{};
+	o.op = FX_OP_STENCIL;
+	o.solver = FX_SOLVER_CG;
+	if (opt) o = *opt;
+	FX_REQUIRE(o.op == FX_OP_STENCIL || o.op == FX_OP_VARK, FX_ERR_ARGUMENT,
+	           "Crank-Nicolson operator must be FX_OP_STENCIL or FX_OP_VARK, got %d", (int)o.op);
+	FX_REQUIRE(o.solver == FX_SOLVER_CG || o.solver == FX_SOLVER_FCG, FX_ERR_ARGUMENT,
+	           "Crank-Nicolson solver must be FX_SOLVER_CG or FX_SOLVER_FCG, got %d", (int)o.solver);
+	FX_REQUIRE(o.solver != FX_SOLVER_FCG || o.mg, FX_ERR_ARGUMENT, "flexible CG needs a multigrid preconditioner");
+	const size_t n = (size_t)(grid->rows * grid->cols);
+	if (o.op == FX_OP_VARK) FX_CHECK(fx::check_buf(o.k, n, FX_F64, "k"));
+
+	fx_cn* cn = new fx_cn;
+	cn->be = be;
+	cn->g = *grid;
+	cn->alpha = alpha;
+	cn->dt = dt;
+	cn->bc = *bc;
+	laplacian_5pt(grid->dx, grid->dy, cn->ent);
+	cn->st = {5, cn->ent};
+	cn->op_kind = o.op;
+	cn->solver_kind = o.solver;
+	cn->k = o.op == FX_OP_VARK ? o.k : nullptr;
+	fx_status s = fx_allocate(be, n, FX_F64, &cn->rhs);
+	if (s == FX_OK) s = fx_solver_create(be, o.solver, grid, 0, cg_tol, cg_max_iter, &cn->solver);
+	if (s == FX_OK && o.mg) s = fx_multigrid_create_ex(be, grid, bc, o.mg, &cn->mg);
+	if (s != FX_OK) {
+		cn_free(cn);
+		return s;
+	}
+	*out = cn;
+	return FX_OK;
+}
+
+void fx_cn_destroy(fx_cn* cn) { cn_free(cn); }
+
+fx_status fx_cn_step(fx_cn* cn, fx_buf* u, const fx_buf* source, fx_solve_stats* stats) {
+	FX_REQUIRE(cn, FX_ERR_ARGUMENT, "solver is null");
+	const size_t n = (size_t)(cn->g.rows * cn->g.cols);
+	FX_CHECK(fx::check_buf(u, n, FX_F64, "u"));
+	if (source) FX_CHECK(fx::check_buf(source, n, FX_F64, "source"));
+	// RHS = (I + (dt/2)·α·Δ) u^n + dt·source   (implicit.rs:185-197)
+	const fx_operator ex = cn_operator(cn, false);
+	FX_CHECK(fx_apply_operator(cn->be, &ex, u, cn->rhs, &cn->g));
+	if (source) FX_CHECK(fx_axpy(cn->be, cn->dt, source, cn->rhs));
+	// (I − (dt/2)·α·Δ) u^{n+1} = RHS from u^n   (implicit.rs:199-218)
+	const fx_operator im = cn_operator(cn, true);
+	fx_precond pc{FX_PC_MULTIGRID, 0, cn->mg};
+	fx_solve_stats st{};
+	FX_CHECK(fx_solver_solve_op(cn->solver, u, cn->rhs, &im, cn->mg ? &pc : nullptr, &st));
+	cn->steps += 1;
+	if (stats) *stats = st;
+	return FX_OK;
+}
+
+fx_status fx_cn_step_n(fx_cn* cn, fx_buf* u, uint64_t n) {
+	for (uint64_t i = 0; i < n; ++i) FX_CHECK(fx_cn_step(cn, u, nullptr, nullptr));
+	return FX_OK;
+}
+
+double fx_cn_dt(const fx_cn* cn) { return cn ? cn->dt : 0.0; }
+uint64_t fx_cn_steps_done(const fx_cn* cn) { return cn ? cn->steps : 0; }
+double fx_cn_sim_time(const fx_cn* cn) { return cn ? (double)cn->steps * cn->dt : 0.0; }
+
+fx_solver* fx_cn_solver(fx_cn* cn) { return cn ? cn->solver : nullptr; }
+fx_multigrid* fx_cn_multigrid(fx_cn* cn) { return cn ? cn->mg : nullptr; }
+
+fx_status fx_write_raw_host(const char* path, const double* data, const fx_grid* grid) {
+	FX_CHECK(fx::check_grid(grid));
+	return raw_write(path, data, grid);
+}
+
+fx_status fx_read_raw_host(const char* path, fx_grid* grid, double* data, size_t len) {
+	return raw_read(path, grid, data, len);
+}
+
+fx_status fx_write_raw(fx_backend* be, const char* path, const fx_buf* field, const fx_grid* grid) {
+	FX_REQUIRE(be && field, FX_ERR_ARGUMENT, "null argument");
+	FX_CHECK(fx::check_grid(grid));
+	const size_t n = (size_t)(grid->rows * grid->cols);
+	FX_REQUIRE(field->len == n, FX_ERR_DIMENSION, "dimension mismatch: expected %zu, got %zu (field)", n, field->len);
+	std::vector<double> host(n);
+	if (field->dtype == FX_F64) {
+		FX_CHECK(fx_copy_to_host_f64(be, field, host.data(), n));
+	} else {
+		// Field::to_vec_f64 (runtime/src/field.rs:134-138) returns an F32 field widened to f64.
+		std::vector<float> h32(n);
+		FX_CHECK(fx_copy_to_host_f32(be, field, h32.data(), n));
+		for (size_t i = 0; i < n; ++i) host[i] = (double)h32[i];
+	}
+	return raw_write(path, host.data(), grid);
+}
+
+fx_status fx_read_raw(fx_backend* be, const char* path, fx_grid* grid, fx_buf** out) {
+	FX_REQUIRE(be && grid && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	fx_grid g{};
+	FX_CHECK(raw_read(path, &g, nullptr, 0));
+	const size_t n = (size_t)(g.rows * g.cols);
+	std::vector<double> host(n);
+	FX_CHECK(raw_read(path, &g, host.data(), n));
+	FX_CHECK(fx_upload_f64(be, host.data(), n, out));
+	*grid = g;
+	return FX_OK;
+}
+
+}  // extern "C"

@@ -6,6 +6,10 @@ CgSolver, PipelinedCgSolver, BiCgStabSolver, GmresSolver
               ~ crates/runtime/src/{cg,pipelined_cg,bicgstab,gmres}.rs
 Multigrid, Identity
               ~ crates/runtime/src/{multigrid,preconditioner}.rs
+CrankNicolsonSolver
+              ~ crates/runtime/src/implicit.rs
+write_raw, read_raw
+              ~ crates/io/src/raw.rs
 
 Every method calls one C entry point of libfluxion_hip.so; nothing here computes
 on the CPU.
@@ -555,3 +559,141 @@ class BatchSolver:
         L.check(self._lib.fx_batch_solve(self.h, x.h, b.h, t.ctypes.data, int(max_iter), C.cast(st, C.c_void_p)),
                 "batch_solve")
         return [SolveStats(s.iterations, s.residual, ConvergenceReason(s.reason), s.true_residual) for s in st]
+
+
+# ── implicit time stepping (fx_cn_*) ──────────────────────────────────
+def _mg_opts(n_pre, n_post, omega, smoother=SmootherKind.JACOBI, transfer=Transfer.REFERENCE, coarsest=4,
+             extra_coarse=10, precision=DType.F64, fused=False, sigma=0.0, scale=1.0, k=None) -> L.FxMgOptions:
+    return L.FxMgOptions(n_pre, n_post, omega, int(smoother), int(transfer), coarsest, extra_coarse, int(precision),
+                         int(bool(fused)), sigma, scale, k.h if k is not None else None)
+
+
+class CrankNicolsonSolver:
+    """CrankNicolsonSolver (crates/runtime/src/implicit.rs:1-238), stepped on the device by fx_cn_*.
+
+    new(grid, alpha, dt, bc, cg_tol, cg_max_iter)                          implicit.rs:41-75
+    with_multigrid(..., mg_pre, mg_post, mg_omega)                         implicit.rs:77-107
+    with_multigrid_smoother(..., mg_pre, mg_post, mg_omega, smoother)      implicit.rs:109-154
+    var_k(grid, alpha, dt, k, cg_tol, cg_max_iter, mixed=True)             extension: ∂u/∂t = α∇·(k∇u)
+
+    step(u, source=None) -> SolveStats, step_n(u, n), dt(), steps_done(), sim_time() as in implicit.rs:156-237.
+    """
+
+    def __init__(self, grid: Grid, alpha: float, dt: float, bc: Boundaries, cg_tol: float, cg_max_iter: int, *,
+                 mg: Optional[L.FxMgOptions] = None, op: int = 0, solver: int = 0, k: Optional[Field] = None,
+                 be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        self._k = k
+        self._mg = mg
+        g, b = L.grid_c(grid), L.bcs_c(bc)
+        opt = L.FxCnOptions(op, solver, k.h if k is not None else None,
+                            C.pointer(mg) if mg is not None else None)
+        h = C.c_void_p()
+        L.check(self._lib.fx_cn_create(self.be.h, C.byref(g), alpha, dt, C.byref(b), cg_tol, cg_max_iter,
+                                       C.byref(opt), C.byref(h)), "cn_create")
+        self.h = h
+
+    @staticmethod
+    def new(grid, alpha, dt, bc, cg_tol, cg_max_iter, be=None) -> "CrankNicolsonSolver":
+        return CrankNicolsonSolver(grid, alpha, dt, bc, cg_tol, cg_max_iter, be=be)
+
+    @staticmethod
+    def with_multigrid(grid, alpha, dt, bc, cg_tol, cg_max_iter, mg_pre, mg_post, mg_omega,
+                       be=None) -> "CrankNicolsonSolver":
+        return CrankNicolsonSolver.with_multigrid_smoother(grid, alpha, dt, bc, cg_tol, cg_max_iter, mg_pre, mg_post,
+                                                           mg_omega, SmootherKind.JACOBI, be=be)
+
+    @staticmethod
+    def with_multigrid_smoother(grid, alpha, dt, bc, cg_tol, cg_max_iter, mg_pre, mg_post, mg_omega, smoother,
+                                be=None) -> "CrankNicolsonSolver":
+        return CrankNicolsonSolver(grid, alpha, dt, bc, cg_tol, cg_max_iter,
+                                   mg=_mg_opts(mg_pre, mg_post, mg_omega, smoother), be=be)
+
+    @staticmethod
+    def var_k(grid, alpha, dt, k: Field, cg_tol, cg_max_iter, mixed: bool = True, n_pre=2, n_post=2,
+              omega=2.0 / 3.0, coarsest=16, be=None) -> "CrankNicolsonSolver":
+        """Extension: ∂u/∂t = α·∇·(k∇u) (zero Dirichlet) by Crank–Nicolson.  mixed: flexible CG around the
+        fused FP32 V-cycle of the step operator I + (dt/2)·α·(−∇·k∇); otherwise CG around its FP64 V-cycle."""
+        scale = 0.5 * dt * alpha
+        mg = _mg_opts(n_pre, n_post, omega, SmootherKind.JACOBI, Transfer.FW_BILINEAR, coarsest, 10,
+                      DType.F32 if mixed else DType.F64, mixed, 1.0, scale, k)
+        return CrankNicolsonSolver(grid, alpha, dt, Boundaries.zero_dirichlet(), cg_tol, cg_max_iter, mg=mg,
+                                   op=2, solver=4 if mixed else 0, k=k, be=be)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and L.live():
+            self._lib.fx_cn_destroy(h)
+            self.h = None
+
+    def step(self, u: Field, source: Optional[Field] = None) -> SolveStats:
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_cn_step(self.h, u.h, source.h if source is not None else None, C.byref(out)),
+                "cn_step")
+        return SolveStats(int(out.iterations), out.residual, ConvergenceReason(out.reason), out.true_residual)
+
+    def step_n(self, u: Field, n: int) -> None:
+        L.check(self._lib.fx_cn_step_n(self.h, u.h, int(n)), "cn_step_n")
+
+    def dt(self) -> float:
+        return float(self._lib.fx_cn_dt(self.h))
+
+    def steps_done(self) -> int:
+        return int(self._lib.fx_cn_steps_done(self.h))
+
+    def sim_time(self) -> float:
+        return float(self._lib.fx_cn_sim_time(self.h))
+
+    def history(self) -> list:
+        """Residual history of the last step's solve."""
+        s = self._lib.fx_cn_solver(self.h)
+        n = self._lib.fx_solver_history(s, None, 0)
+        hist = (C.c_double * max(n, 1))()
+        self._lib.fx_solver_history(s, hist, n)
+        return list(hist[:n])
+
+    def autotune(self, reps: int = 5) -> float:
+        """Wave segment heights of the step's multigrid (fx_multigrid_autotune); ms per cycle of level kernels."""
+        m = self._lib.fx_cn_multigrid(self.h)
+        if not m:
+            raise ValueError("the stepper has no multigrid")
+        ms = C.c_double()
+        L.check(self._lib.fx_multigrid_autotune(m, reps, C.byref(ms)), "multigrid_autotune")
+        return ms.value
+
+
+# ── raw field format (fx_*_raw; crates/io/src/raw.rs) ──────────────────
+def write_raw(path, field: Field) -> None:
+    """write_raw (raw.rs:13-32): 32-byte little-endian header rows, cols (u64), dx, dy (f64), then f64 data."""
+    g = L.grid_c(field.grid)
+    L.check(field._lib.fx_write_raw(field.be.h, str(path).encode(), field.h, C.byref(g)), "write_raw")
+
+
+def read_raw(path, be: Optional[HipBackend] = None) -> Field:
+    """read_raw (raw.rs:38-77)."""
+    be = _backend(be)
+    g = L.FxGrid()
+    h = C.c_void_p()
+    L.check(be._lib.fx_read_raw(be.h, str(path).encode(), C.byref(g), C.byref(h)), "read_raw")
+    return Field(Grid(int(g.rows), int(g.cols), g.dx, g.dy), Buffer(be, h))
+
+
+def write_raw_host(path, data, grid: Grid) -> None:
+    """The raw format from host memory (no device)."""
+    a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+    if a.size != len(grid):
+        raise DimensionMismatch(f"expected {len(grid)}, got {a.size}")
+    g = L.grid_c(grid)
+    L.check(L.lib().fx_write_raw_host(str(path).encode(), a.ctypes.data, C.byref(g)), "write_raw_host")
+
+
+def read_raw_host(path):
+    """(grid, data) of a raw file, read into host memory (no device)."""
+    g = L.FxGrid()
+    L.check(L.lib().fx_read_raw_host(str(path).encode(), C.byref(g), None, 0), "read_raw_host")
+    grid = Grid(int(g.rows), int(g.cols), g.dx, g.dy)
+    out = np.empty(len(grid), dtype=np.float64)
+    L.check(L.lib().fx_read_raw_host(str(path).encode(), C.byref(g), out.ctypes.data, out.size), "read_raw_host")
+    return grid, out

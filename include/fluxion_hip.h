@@ -43,7 +43,8 @@ typedef enum {
 	FX_ERR_HIP = 6,         /* CoreError::BackendError from the HIP runtime */
 	FX_ERR_ALLOC = 7,       /* device allocation failed */
 	FX_ERR_NO_DEVICE = 8,   /* no HIP device / not gfx950 */
-	FX_ERR_UNSUPPORTED = 9
+	FX_ERR_UNSUPPORTED = 9,
+	FX_ERR_IO = 10          /* std::io::Error of the raw field format (io/src/raw.rs:13-77) */
 } fx_status;
 
 typedef enum { FX_F32 = 0, FX_F64 = 1 } fx_dtype;            /* core/src/dtype.rs:1-16 */
@@ -329,6 +330,52 @@ fx_status fx_batch_solve(fx_batch* bt, fx_buf* x, const fx_buf* b, const double*
 void fx_batch_destroy(fx_batch* bt);
 fx_status fx_batch_info(const fx_batch* bt, int32_t* rows_per_block, int32_t* points_per_block, uint64_t* points,
                         int32_t* graph_iterations);
+
+/* ── Crank–Nicolson stepper (CrankNicolsonSolver, runtime/src/implicit.rs:1-238) ──────────
+ * One step solves (I − (dt/2)·α·Δ) u^{n+1} = (I + (dt/2)·α·Δ) u^n + dt·source from u^n as the initial
+ * guess (implicit.rs:178-222), with Δ = Stencil::laplacian_2d_5pt(dx, dy) and the step's boundary
+ * conditions.  opt == NULL is CrankNicolsonSolver::new (unpreconditioned CG, implicit.rs:41-75).
+ * With opt->mg the solver is preconditioned by the multigrid those options build: the reference
+ * options of fx_multigrid_create are CrankNicolsonSolver::with_multigrid / with_multigrid_smoother
+ * (implicit.rs:77-154).  Extensions: opt->op = FX_OP_VARK steps ∂u/∂t = α·∇·(k∇u) + source with
+ * the variable-coefficient operator (nodal k, harmonic face means, zero-Dirichlet identity rows;
+ * opt->k must stay allocated), and opt->solver = FX_SOLVER_FCG runs flexible CG (needs opt->mg;
+ * for the fused FP32 cycle on this operator give the level operator sigma = 1, scale = (dt/2)·α,
+ * k = opt->k). */
+typedef struct fx_cn fx_cn;
+typedef struct {
+	int32_t op;                 /* FX_OP_STENCIL (reference) or FX_OP_VARK */
+	int32_t solver;             /* FX_SOLVER_CG (reference) or FX_SOLVER_FCG */
+	const fx_buf* k;            /* FX_OP_VARK: nodal coefficient, FP64, rows·cols values */
+	const fx_mg_options* mg;    /* NULL: unpreconditioned */
+} fx_cn_options;
+fx_status fx_cn_create(fx_backend* be, const fx_grid* grid, double alpha, double dt, const fx_boundaries* bc,
+                       double cg_tol, uint64_t cg_max_iter, const fx_cn_options* opt, fx_cn** out);
+void fx_cn_destroy(fx_cn* cn);
+/* CrankNicolsonSolver::step(u, source) (implicit.rs:178-222); source may be NULL. */
+fx_status fx_cn_step(fx_cn* cn, fx_buf* u, const fx_buf* source, fx_solve_stats* stats);
+/* CrankNicolsonSolver::step_n(u, n) (implicit.rs:224-237): n steps without a source (stats dropped). */
+fx_status fx_cn_step_n(fx_cn* cn, fx_buf* u, uint64_t n);
+double fx_cn_dt(const fx_cn* cn);                /* implicit.rs:156-160 */
+uint64_t fx_cn_steps_done(const fx_cn* cn);      /* implicit.rs:162-166 */
+double fx_cn_sim_time(const fx_cn* cn);          /* implicit.rs:168-172: steps_done · dt */
+/* Extension: the step's solver (residual history of the last step: fx_solver_history) and
+ * multigrid (NULL when unpreconditioned; fx_multigrid_autotune applies), owned by the stepper. */
+fx_solver* fx_cn_solver(fx_cn* cn);
+fx_multigrid* fx_cn_multigrid(fx_cn* cn);
+
+/* ── raw field format (io/src/raw.rs:1-77) ───────────────────────────────────────────────
+ * [rows:u64][cols:u64][dx:f64][dy:f64][data:f64 × rows·cols], all little-endian, row-major.
+ * Reading validates the header as Grid::new does (FX_ERR_GRID) and fails on a short file
+ * (FX_ERR_IO). */
+/* write_raw(path, field) (raw.rs:13-32); an FX_F32 field is written widened to f64. */
+fx_status fx_write_raw(fx_backend* be, const char* path, const fx_buf* field, const fx_grid* grid);
+/* read_raw(path, device) (raw.rs:38-77): *out is a new FX_F64 buffer, *grid its grid. */
+fx_status fx_read_raw(fx_backend* be, const char* path, fx_grid* grid, fx_buf** out);
+/* Host-memory forms (no device): data has rows·cols values; fx_read_raw_host with data == NULL
+ * reads and validates the header only. */
+fx_status fx_write_raw_host(const char* path, const double* data, const fx_grid* grid);
+fx_status fx_read_raw_host(const char* path, fx_grid* grid, double* data, size_t len);
 
 #ifdef __cplusplus
 }

@@ -355,6 +355,36 @@ def gmres(x, b, grid, op, restart, tol, max_iter, mg: OMg = None, identity=False
     return _solve(lib().orc_gmres, x, b, grid, op, tol, max_iter, (C.c_int64(restart),) + tuple(pc))
 
 
+# ── CrankNicolsonSolver::step (runtime/src/implicit.rs:178-222) ────────
+def cn_step(u, grid, alpha, dt, bc=None, source=None, tol=1e-8, max_iter=200, mg: OMg = None, k=None,
+            flexible=False):
+    """One Crank–Nicolson step from u; returns (u_new, stats).
+
+    h = 0.5·dt·α evaluated left to right (implicit.rs:183).  RHS = u + h·S(u) on the interior and u on the
+    boundary (copy, copy, stencil_axpy: implicit.rs:187-194), then RHS += dt·source (axpy, 195-197); the
+    system u − h·S(u) = RHS (identity boundary rows, coeff = −h) is solved from u by CG, or by PCG with the
+    multigrid `mg` (implicit.rs:199-218), S = Stencil::laplacian_2d_5pt(dx, dy).  k (extension): the same
+    recurrence with the variable-coefficient operator −∇·(k∇·) in place of −S (explicit side
+    u − h·(−∇·k∇u), system u + h·(−∇·k∇u)); flexible selects FCG (needs mg)."""
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    bc = bc or Boundaries.zero_dirichlet()
+    h = 0.5 * dt * alpha
+    if k is None:
+        st = Stencil.laplacian_2d_5pt(grid.dx, grid.dy)
+        rhs = stencil_axpy(h, u, u, grid, st, bc)
+        system = Op.stencil(-h, st, bc)
+    else:
+        rhs = apply_op(Op.var_k(k, coeff=-h, sigma=1.0), u, grid)
+        system = Op.var_k(k, coeff=h, sigma=1.0)
+    if source is not None:
+        rhs = axpy(dt, np.ascontiguousarray(source, dtype=np.float64).ravel(), rhs)
+    if flexible:
+        return fcg(u, rhs, grid, system, tol, max_iter, mg=mg)
+    if mg is None:
+        return cg(u, rhs, grid, system, tol, max_iter)
+    return pcg(u, rhs, grid, system, tol, max_iter, mg=mg)
+
+
 def num_threads() -> int:
     return lib().orc_num_threads()
 
