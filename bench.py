@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
+    ap.add_argument("--no-cn", action="store_true", help="skip the Crank-Nicolson record")
+    ap.add_argument("--cn-steps", type=int, default=100)
     ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
@@ -260,6 +262,44 @@ def main():
                    "speedup_mixed_vs_plain": tp / tm}
         del mp64, mp32
 
+    # ── Crank–Nicolson on variable k: 1024², warm-started steps (implicit.rs:178-222 recurrence) ──
+    cn = None
+    if not args.no_cn:
+        nc, cn_dt = 1024, 1e-4
+        gc = Grid.square(nc, 1.0 / (nc - 1))
+        kc = hip.Field.from_f64(gc, grf_log_k(nc).ravel(), be)
+        u0c = gaussian_bumps(nc).ravel()
+        cn_tol = args.tol * float(np.linalg.norm(u0c))
+        cn = {"grid": nc, "dt": cn_dt, "alpha": 1.0, "steps": args.cn_steps,
+              "tol_abs": cn_tol, "tol_note": f"absolute ||r|| < {args.tol:g}*||u0|| per step"}
+        finals = {}
+        for name, mixed in (("mixed_fcg", True), ("plain_fp64_mg_pcg", False)):
+            stp = hip.CrankNicolsonSolver.var_k(gc, 1.0, cn_dt, kc, cn_tol, max_iter, mixed=mixed, be=be)
+            if mixed and mgk == MgKernels.WAVE and not args.no_autotune:
+                stp.autotune(5)
+            U = hip.Field.from_f64(gc, u0c, be)
+            for _ in range(2):
+                stp.step(U)          # warm-up: first-use allocations, then reset
+            U.write(u0c)
+            be.synchronize()
+            its = 0
+            t0 = time.perf_counter()
+            for _ in range(args.cn_steps):
+                its += stp.step(U).iterations
+            be.synchronize()
+            t = (time.perf_counter() - t0) / args.cn_steps
+            finals[name] = U.to_vec_f64()
+            cn[name] = {"steps_per_s": 1.0 / t, "ms_per_step": 1e3 * t, "avg_iterations": its / args.cn_steps}
+            del stp
+        cn["speedup_mixed_vs_plain"] = cn["plain_fp64_mg_pcg"]["ms_per_step"] / cn["mixed_fcg"]["ms_per_step"]
+        cn["final_state_rel_diff"] = float(np.linalg.norm(finals["mixed_fcg"] - finals["plain_fp64_mg_pcg"])
+                                           / np.linalg.norm(finals["plain_fp64_mg_pcg"]))
+        log(f"bench: CN var-k {nc}^2, {args.cn_steps} steps dt={cn_dt:g}: mixed "
+            f"{cn['mixed_fcg']['ms_per_step']:.2f} ms/step ({cn['mixed_fcg']['avg_iterations']:.1f} its), plain "
+            f"{cn['plain_fp64_mg_pcg']['ms_per_step']:.2f} ms/step ({cn['plain_fp64_mg_pcg']['avg_iterations']:.1f} its), "
+            f"final states differ by {cn['final_state_rel_diff']:.1e}")
+        del kc
+
     # ── CPU baseline: the oracle's plain FP64 MG-PCG on the same var-k problem ──
     cpu = None
     if not args.no_cpu:
@@ -309,6 +349,7 @@ def main():
             "speedup_mixed_vs_plain": t_plain / t_mixed,
         },
         "poisson_4096": poisson,
+        "cn_vark_1024": cn,
         "kernels": kernels,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
