@@ -185,6 +185,17 @@ _SIGS = {
     "fx_cn_sim_time": (C.c_double, [C.c_void_p]),
     "fx_cn_solver": (C.c_void_p, [C.c_void_p]),
     "fx_cn_multigrid": (C.c_void_p, [C.c_void_p]),
+    "fx_diffusion_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, _P(C.c_double), _P(FxBoundaries),
+                                      C.c_int32, _P(C.c_void_p)]),
+    "fx_diffusion_destroy": (None, [C.c_void_p]),
+    "fx_diffusion_enable_graph": (C.c_int, [C.c_void_p]),
+    "fx_diffusion_step": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p]),
+    "fx_diffusion_step_n": (C.c_int, [C.c_void_p, c_buf_p, C.c_uint64]),
+    "fx_diffusion_step_n_with_source": (C.c_int, [C.c_void_p, c_buf_p, C.c_uint64, c_buf_p]),
+    "fx_diffusion_dt": (C.c_double, [C.c_void_p]),
+    "fx_diffusion_steps_done": (C.c_uint64, [C.c_void_p]),
+    "fx_diffusion_sim_time": (C.c_double, [C.c_void_p]),
+    "fx_diffusion_scheme": (C.c_int32, [C.c_void_p]),
     "fx_write_raw": (C.c_int, [c_be_p, C.c_char_p, c_buf_p, _P(FxGrid)]),
     "fx_read_raw": (C.c_int, [c_be_p, C.c_char_p, _P(FxGrid), _P(c_buf_p)]),
     "fx_write_raw_host": (C.c_int, [C.c_char_p, C.c_void_p, _P(FxGrid)]),

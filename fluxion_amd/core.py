@@ -237,6 +237,14 @@ class SmootherKind(enum.IntEnum):
     CHEBYSHEV = 1
 
 
+class TimeScheme(enum.IntEnum):
+    """solver.rs:14-23."""
+
+    EULER = 0
+    RK2 = 1
+    RK4 = 2
+
+
 class MgKernels(enum.IntEnum):
     """Level kernels of the fused FP32 V-cycle (fx_mg_kernels); all give bit-identical results."""
 

@@ -1,5 +1,6 @@
 // fx_implicit.hip — callers either side of the solve: the Crank–Nicolson
-// stepper (crates/runtime/src/implicit.rs:1-238) and the raw field format
+// stepper (crates/runtime/src/implicit.rs:1-238), the explicit diffusion
+// stepper (crates/runtime/src/solver.rs:1-494) and the raw field format
 // (crates/io/src/raw.rs:1-77).
 //
 // The stepper is host code over the library's own entry points: one operator
@@ -7,6 +8,7 @@
 // stencil_axpy, implicit.rs:187-194, in one read of u and one write), an axpy
 // adds the source, and the solver of the step runs from u^n as its initial
 // guess, so every step is warm-started as in the reference.
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstring>
@@ -27,6 +29,27 @@ struct fx_cn {
 	fx_solver* solver = nullptr;
 	fx_multigrid* mg = nullptr;
 	fx_buf* rhs = nullptr;
+};
+
+// DiffusionSolver (solver.rs:25-48).  Explicit steps write a second buffer
+// instead of copying u first: Euler and the first Heun stage go u → s, the
+// second Heun stage s → u, and step_n pairs Euler steps u → s → u, so no step
+// copies data except a lone Euler step, which copies s back to u.
+struct fx_diffusion {
+	fx_backend* be = nullptr;
+	fx_grid g{};
+	double alpha = 0, dt = 0;
+	int32_t scheme = FX_SCHEME_EULER;
+	uint64_t steps = 0;
+	fx_boundaries bc{};
+	fx_stencil_entry ent[5]{};
+	fx_stencil st{};
+	std::vector<fx_buf*> scratch;
+	bool use_graph = false;
+	// Captured HIP graph of step_n's unit (two Euler steps, or one RK step), for the u it was captured on.
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	void* graph_u = nullptr;
 };
 
 namespace {
@@ -69,6 +92,123 @@ void cn_free(fx_cn* cn) {
 	if (cn->mg) fx_multigrid_destroy(cn->mg);
 	if (cn->rhs) fx_buffer_free(cn->rhs);
 	delete cn;
+}
+
+// ── DiffusionSolver steps (solver.rs:339-494) ────────────────────────
+fx_status euler_stage(fx_diffusion* d, const fx_buf* x, fx_buf* y, const fx_buf* src) {
+	// u += α·dt·Δ(u) + dt·source (solver.rs:344-367), written to y.
+	return fx::explicit_stage(d->be, (const double*)x->ptr, (double*)y->ptr, d->alpha * d->dt,
+	                          src ? (const double*)src->ptr : nullptr, d->dt, nullptr, &d->st, (int64_t)d->g.rows,
+	                          (int64_t)d->g.cols);
+}
+
+// Heun (solver.rs:371-418): u* = u + dt·α·Δu (+ dt·s); u** = u* + dt·α·Δu* (+ dt·s); u = (u** + u)·0.5.
+fx_status rk2_step(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
+	const double coeff = d->dt * d->alpha;
+	const double* sp = src ? (const double*)src->ptr : nullptr;
+	fx_buf* s = d->scratch[0];
+	const int64_t rows = (int64_t)d->g.rows, cols = (int64_t)d->g.cols;
+	FX_CHECK(fx::explicit_stage(d->be, (const double*)u->ptr, (double*)s->ptr, coeff, sp, d->dt, nullptr, &d->st,
+	                            rows, cols));
+	return fx::explicit_stage(d->be, (const double*)s->ptr, (double*)u->ptr, coeff, sp, d->dt,
+	                          (const double*)u->ptr, &d->st, rows, cols);
+}
+
+// Classical RK4 (solver.rs:421-494), the reference's operations in its order;
+// scratch = [k1, k2, k3, k4, u_old].  The stage u = u_old + ½dt·α·Δ(u_old) is
+// one pass instead of a copy and a stencil_axpy.
+fx_status rk4_step(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
+	fx_backend* be = d->be;
+	const double dt = d->dt, alpha = d->alpha;
+	fx_buf** k = d->scratch.data();
+	fx_buf* uold = k[4];
+	const int64_t rows = (int64_t)d->g.rows, cols = (int64_t)d->g.cols;
+	FX_CHECK(fx_copy(be, u, uold));
+	FX_CHECK(fx_apply_stencil(be, u, k[0], &d->g, &d->st, &d->bc));
+	FX_CHECK(fx::explicit_stage(be, (const double*)uold->ptr, (double*)u->ptr, 0.5 * dt * alpha, nullptr, 0.0, nullptr,
+	                            &d->st, rows, cols));
+	FX_CHECK(fx_apply_stencil(be, u, k[1], &d->g, &d->st, &d->bc));
+	FX_CHECK(fx_copy(be, uold, u));
+	FX_CHECK(fx_axpy(be, 0.5 * dt * alpha, k[1], u));
+	FX_CHECK(fx_zero_boundaries(be, u, d->g.rows, d->g.cols));
+	FX_CHECK(fx_apply_stencil(be, u, k[2], &d->g, &d->st, &d->bc));
+	FX_CHECK(fx_copy(be, uold, u));
+	FX_CHECK(fx_axpy(be, dt * alpha, k[2], u));
+	FX_CHECK(fx_zero_boundaries(be, u, d->g.rows, d->g.cols));
+	FX_CHECK(fx_apply_stencil(be, u, k[3], &d->g, &d->st, &d->bc));
+	FX_CHECK(fx_copy(be, uold, u));
+	const double c = dt * alpha / 6.0;
+	FX_CHECK(fx_axpy(be, c, k[0], u));
+	FX_CHECK(fx_axpy(be, 2.0 * c, k[1], u));
+	FX_CHECK(fx_axpy(be, 2.0 * c, k[2], u));
+	FX_CHECK(fx_axpy(be, c, k[3], u));
+	if (src) FX_CHECK(fx_axpy(be, dt, src, u));
+	return FX_OK;
+}
+
+// One step from u back into u.
+fx_status diff_step(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
+	switch (d->scheme) {
+	case FX_SCHEME_EULER:
+		FX_CHECK(euler_stage(d, u, d->scratch[0], src));
+		return fx_copy(d->be, d->scratch[0], u);
+	case FX_SCHEME_RK2: return rk2_step(d, u, src);
+	default: return rk4_step(d, u, src);
+	}
+}
+
+// step_n's unit: two Euler steps u → s → u, or one RK step.
+fx_status diff_unit(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
+	if (d->scheme == FX_SCHEME_EULER) {
+		FX_CHECK(euler_stage(d, u, d->scratch[0], src));
+		return euler_stage(d, d->scratch[0], u, src);
+	}
+	return diff_step(d, u, src);
+}
+int unit_steps(const fx_diffusion* d) { return d->scheme == FX_SCHEME_EULER ? 2 : 1; }
+
+void drop_graph(fx_diffusion* d) {
+	if (d->exec) (void)hipGraphExecDestroy(d->exec);
+	if (d->graph) (void)hipGraphDestroy(d->graph);
+	d->exec = nullptr;
+	d->graph = nullptr;
+	d->graph_u = nullptr;
+}
+
+// Captures step_n's unit on the backend's stream (the reference captures one
+// step: solver.rs:259-303) for this u; a different u is captured anew.
+fx_status capture_unit(fx_diffusion* d, fx_buf* u) {
+	if (d->exec && d->graph_u == u->ptr) return FX_OK;
+	drop_graph(d);
+	hipStream_t st = d->be->stream;
+	FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+	fx_status s = diff_unit(d, u, nullptr);
+	hipGraph_t g = nullptr;
+	hipError_t e = hipStreamEndCapture(st, &g);
+	if (s != FX_OK) {
+		if (g) (void)hipGraphDestroy(g);
+		return s;
+	}
+	if (e != hipSuccess) {
+		fx::set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+		return FX_ERR_HIP;
+	}
+	d->graph = g;
+	e = hipGraphInstantiate(&d->exec, g, nullptr, nullptr, 0);
+	if (e != hipSuccess) {
+		drop_graph(d);
+		fx::set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+		return FX_ERR_HIP;
+	}
+	d->graph_u = u->ptr;
+	return FX_OK;
+}
+
+void diff_free(fx_diffusion* d) {
+	if (!d) return;
+	drop_graph(d);
+	for (fx_buf* b : d->scratch) fx_buffer_free(b);
+	delete d;
 }
 
 // ── raw field format (io/src/raw.rs:7-77) ────────────────────────────
@@ -228,6 +368,99 @@ double fx_cn_sim_time(const fx_cn* cn) { return cn ? (double)cn->steps * cn->dt 
 
 fx_solver* fx_cn_solver(fx_cn* cn) { return cn ? cn->solver : nullptr; }
 fx_multigrid* fx_cn_multigrid(fx_cn* cn) { return cn ? cn->mg : nullptr; }
+
+fx_status fx_diffusion_create(fx_backend* be, const fx_grid* grid, double alpha, const double* dt,
+                              const fx_boundaries* bc, int32_t scheme, fx_diffusion** out) {
+	FX_REQUIRE(be && bc && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(fx::check_grid(grid));
+	FX_CHECK(fx::check_bc(bc));
+	FX_REQUIRE(scheme == FX_SCHEME_EULER || scheme == FX_SCHEME_RK2 || scheme == FX_SCHEME_RK4, FX_ERR_ARGUMENT,
+	           "unknown time scheme %d", (int)scheme);
+	FX_REQUIRE(std::isfinite(alpha) && (!dt || std::isfinite(*dt)), FX_ERR_ARGUMENT, "alpha and dt must be finite");
+	fx_diffusion* d = new fx_diffusion;
+	d->be = be;
+	d->g = *grid;
+	d->alpha = alpha;
+	d->scheme = scheme;
+	d->bc = *bc;
+	// dt = safety·h_min²/(4α) unless given; safety 0.4, 0.8, 1.4 (solver.rs:103-113).
+	const double h_min = std::min(grid->dx, grid->dy);
+	const double safety = scheme == FX_SCHEME_EULER ? 0.4 : scheme == FX_SCHEME_RK2 ? 0.8 : 1.4;
+	d->dt = dt ? *dt : safety * h_min * h_min / (4.0 * alpha);
+	laplacian_5pt(grid->dx, grid->dy, d->ent);
+	d->st = {5, d->ent};
+	const int nscr = scheme == FX_SCHEME_RK4 ? 5 : 1;
+	for (int i = 0; i < nscr; ++i) {
+		fx_buf* b = nullptr;
+		fx_status s = fx_allocate(be, (size_t)(grid->rows * grid->cols), FX_F64, &b);
+		if (s != FX_OK) {
+			diff_free(d);
+			return s;
+		}
+		d->scratch.push_back(b);
+	}
+	*out = d;
+	return FX_OK;
+}
+
+void fx_diffusion_destroy(fx_diffusion* d) { diff_free(d); }
+
+fx_status fx_diffusion_enable_graph(fx_diffusion* d) {
+	FX_REQUIRE(d, FX_ERR_ARGUMENT, "solver is null");
+	d->use_graph = true;
+	return FX_OK;
+}
+
+fx_status fx_diffusion_step(fx_diffusion* d, fx_buf* u, const fx_buf* source) {
+	FX_REQUIRE(d, FX_ERR_ARGUMENT, "solver is null");
+	const size_t n = (size_t)(d->g.rows * d->g.cols);
+	FX_CHECK(fx::check_buf(u, n, FX_F64, "u"));
+	if (source) FX_CHECK(fx::check_buf(source, n, FX_F64, "source"));
+	FX_CHECK(diff_step(d, u, source));
+	d->steps += 1;
+	return FX_OK;
+}
+
+static fx_status step_many(fx_diffusion* d, fx_buf* u, uint64_t n, const fx_buf* source, bool graph) {
+	FX_REQUIRE(d, FX_ERR_ARGUMENT, "solver is null");
+	const size_t len = (size_t)(d->g.rows * d->g.cols);
+	FX_CHECK(fx::check_buf(u, len, FX_F64, "u"));
+	if (source) FX_CHECK(fx::check_buf(source, len, FX_F64, "source"));
+	const uint64_t per = (uint64_t)unit_steps(d);
+	const uint64_t units = n / per;
+	if (graph && units > 0) {
+		FX_CHECK(capture_unit(d, u));
+		for (uint64_t i = 0; i < units; ++i) {
+			FX_HIP(hipGraphLaunch(d->exec, d->be->stream));
+			d->steps += per;
+		}
+	} else {
+		for (uint64_t i = 0; i < units; ++i) {
+			FX_CHECK(diff_unit(d, u, source));
+			d->steps += per;
+		}
+	}
+	for (uint64_t i = units * per; i < n; ++i) {
+		FX_CHECK(diff_step(d, u, source));
+		d->steps += 1;
+	}
+	return FX_OK;
+}
+
+fx_status fx_diffusion_step_n(fx_diffusion* d, fx_buf* u, uint64_t n) {
+	return step_many(d, u, n, nullptr, d && d->use_graph);
+}
+
+fx_status fx_diffusion_step_n_with_source(fx_diffusion* d, fx_buf* u, uint64_t n, const fx_buf* source) {
+	FX_REQUIRE(source, FX_ERR_ARGUMENT, "source is null");
+	return step_many(d, u, n, source, false);
+}
+
+double fx_diffusion_dt(const fx_diffusion* d) { return d ? d->dt : 0.0; }
+uint64_t fx_diffusion_steps_done(const fx_diffusion* d) { return d ? d->steps : 0; }
+double fx_diffusion_sim_time(const fx_diffusion* d) { return d ? (double)d->steps * d->dt : 0.0; }
+int32_t fx_diffusion_scheme(const fx_diffusion* d) { return d ? d->scheme : -1; }
 
 fx_status fx_write_raw_host(const char* path, const double* data, const fx_grid* grid) {
 	FX_CHECK(fx::check_grid(grid));

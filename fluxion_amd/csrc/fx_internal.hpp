@@ -79,6 +79,12 @@ inline fxd::RedScratch red_scratch(fx_backend* be) { return fxd::RedScratch{be->
 fx_status dev_alloc(fx_backend* be, size_t bytes, void** out);
 void dev_free(void* p);
 
+// One explicit diffusion stage y = x + coeff·S(x) inside, x on the boundary, then
+// y += dt·src (src may be null) and y = (y + avg)·0.5 (avg may be null), in one
+// pass of the 5-point streaming kernel (or the generic one for other stencils).
+fx_status explicit_stage(fx_backend* be, const double* x, double* y, double coeff, const double* src, double dt,
+                         const double* avg, const fx_stencil* st, int64_t rows, int64_t cols);
+
 // Validation helpers.
 fx_status check_grid(const fx_grid* g);
 fx_status check_buf(const fx_buf* b, size_t len, fx_dtype dt, const char* name);

@@ -167,6 +167,23 @@ struct EpiOp {
 	template <class A> __device__ void boundary(int64_t i, double c, A*) const { y[i] = c; }
 };
 
+// One explicit diffusion stage into a second buffer: t = x + coeff·lap inside
+// (stencil_axpy's fma on a copy of x), x on the boundary; then t += dt·src
+// (axpy) and, for the second stage of Heun's method, t = (t + avg)·0.5 (axpy
+// with 1 then scale by 0.5).  Each step is the reference's own operation in its
+// order (runtime/src/solver.rs:344-418), so the values are identical; only the
+// copies between them are gone.
+struct EpiStage {
+	double coeff; double* y; const double* src; double dt; const double* avg;
+	__device__ void put(int64_t i, double t) const {
+		if (src) t = fma(dt, src[i], t);
+		if (avg) t = fma(1.0, avg[i], t) * 0.5;
+		y[i] = t;
+	}
+	template <class A> __device__ void interior(int64_t i, double lap, double c, A*) const { put(i, fma(coeff, lap, c)); }
+	template <class A> __device__ void boundary(int64_t i, double c, A*) const { put(i, c); }
+};
+
 // Per-point radius-1 kernel for 9-entry and generic stencils.
 template <class E>
 __global__ __launch_bounds__(BLOCK) void k_point3(const double* x, E e, W9 w, Ent ent, int mode, int64_t rows, int64_t cols,
@@ -638,6 +655,17 @@ fx_status fx_pipelined_cg_fused(fx_backend* be, double alpha, double beta, fx_bu
 	FX_CHECK(len_eq(z, q, "z", "q"));
 	return launch_elem(be, PcgFusedF{alpha, beta, D(z), D(t), D(p), D(x), D(r), D(w), CD(q)}, z->len, "pipelined_cg_fused");
 }
+
+}  // extern "C"
+
+namespace fx {
+fx_status explicit_stage(fx_backend* be, const double* x, double* y, double coeff, const double* src, double dt,
+                         const double* avg, const fx_stencil* st, int64_t rows, int64_t cols) {
+	return launch_stencil(be, x, EpiStage{coeff, y, src, dt, avg}, st, rows, cols, "explicit_stage");
+}
+}  // namespace fx
+
+extern "C" {
 
 fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y, const fx_grid* g) {
 	FX_REQUIRE(be, FX_ERR_ARGUMENT, "backend is null");

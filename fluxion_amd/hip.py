@@ -8,6 +8,8 @@ Multigrid, Identity
               ~ crates/runtime/src/{multigrid,preconditioner}.rs
 CrankNicolsonSolver
               ~ crates/runtime/src/implicit.rs
+DiffusionSolver
+              ~ crates/runtime/src/solver.rs
 write_raw, read_raw
               ~ crates/io/src/raw.rs
 
@@ -23,7 +25,7 @@ import numpy as np
 
 from . import _lib as L
 from .core import (Boundaries, ConvergenceReason, DType, DimensionMismatch, Grid, SmootherKind, SolveStats,
-                   Stencil, Transfer)
+                   Stencil, TimeScheme, Transfer)
 
 
 class HipBackend:
@@ -662,6 +664,74 @@ class CrankNicolsonSolver:
         ms = C.c_double()
         L.check(self._lib.fx_multigrid_autotune(m, reps, C.byref(ms)), "multigrid_autotune")
         return ms.value
+
+
+# ── explicit time stepping (fx_diffusion_*) ───────────────────────────
+class DiffusionSolver:
+    """DiffusionSolver (crates/runtime/src/solver.rs:25-494): ∂u/∂t = α·Δu + source, explicit.
+
+    new(grid, alpha, dt)                                  solver.rs:51-69 (Euler, zero Dirichlet)
+    with_boundaries(grid, alpha, dt, boundaries)          solver.rs:71-93
+    build(grid, alpha, dt, boundaries, scheme)            solver.rs:96-148
+    dt=None takes the scheme's CFL-safe step.  step, step_n, step_n_with_source, enable_hip_graph,
+    dt(), steps_done(), sim_time(), scheme() as in solver.rs:149-337.
+    """
+
+    def __init__(self, grid: Grid, alpha: float, dt: Optional[float], boundaries: Boundaries,
+                 scheme: TimeScheme = TimeScheme.EULER, be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        g, b = L.grid_c(grid), L.bcs_c(boundaries)
+        d = C.c_double(dt) if dt is not None else None
+        h = C.c_void_p()
+        L.check(self._lib.fx_diffusion_create(self.be.h, C.byref(g), alpha, C.byref(d) if d is not None else None,
+                                              C.byref(b), int(scheme), C.byref(h)), "diffusion_create")
+        self.h = h
+
+    @staticmethod
+    def new(grid, alpha, dt=None, be=None) -> "DiffusionSolver":
+        return DiffusionSolver(grid, alpha, dt, Boundaries.zero_dirichlet(), TimeScheme.EULER, be)
+
+    @staticmethod
+    def with_boundaries(grid, alpha, dt, boundaries, be=None) -> "DiffusionSolver":
+        return DiffusionSolver(grid, alpha, dt, boundaries, TimeScheme.EULER, be)
+
+    @staticmethod
+    def build(grid, alpha, dt, boundaries, scheme, be=None) -> "DiffusionSolver":
+        return DiffusionSolver(grid, alpha, dt, boundaries, scheme, be)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and L.live():
+            self._lib.fx_diffusion_destroy(h)
+            self.h = None
+
+    def enable_hip_graph(self) -> None:
+        L.check(self._lib.fx_diffusion_enable_graph(self.h), "diffusion_enable_graph")
+
+    def step(self, u: Field, source: Optional[Field] = None) -> None:
+        L.check(self._lib.fx_diffusion_step(self.h, u.h, source.h if source is not None else None),
+                "diffusion_step")
+
+    def step_n(self, u: Field, n: int) -> None:
+        L.check(self._lib.fx_diffusion_step_n(self.h, u.h, int(n)), "diffusion_step_n")
+
+    def step_n_with_source(self, u: Field, n: int, source: Field) -> None:
+        L.check(self._lib.fx_diffusion_step_n_with_source(self.h, u.h, int(n), source.h),
+                "diffusion_step_n_with_source")
+
+    def dt(self) -> float:
+        return float(self._lib.fx_diffusion_dt(self.h))
+
+    def steps_done(self) -> int:
+        return int(self._lib.fx_diffusion_steps_done(self.h))
+
+    def sim_time(self) -> float:
+        return float(self._lib.fx_diffusion_sim_time(self.h))
+
+    def scheme(self) -> TimeScheme:
+        return TimeScheme(self._lib.fx_diffusion_scheme(self.h))
 
 
 # ── raw field format (fx_*_raw; crates/io/src/raw.rs) ──────────────────

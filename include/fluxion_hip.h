@@ -364,6 +364,31 @@ double fx_cn_sim_time(const fx_cn* cn);          /* implicit.rs:168-172: steps_d
 fx_solver* fx_cn_solver(fx_cn* cn);
 fx_multigrid* fx_cn_multigrid(fx_cn* cn);
 
+/* ── explicit diffusion stepper (DiffusionSolver, runtime/src/solver.rs:1-494) ─────────────
+ * ∂u/∂t = α·Δu + source with Δ = Stencil::laplacian_2d_5pt(dx, dy) and the given boundary conditions.
+ * Schemes (TimeScheme, solver.rs:14-23): forward Euler, Heun (RK2, Shu–Osher form), classical RK4.
+ * dt == NULL takes safety·min(dx, dy)²/(4α) with safety 0.4 / 0.8 / 1.4 (solver.rs:96-113).  Each step
+ * gives the reference's values (solver.rs:344-494) while writing a second buffer instead of copying
+ * u first.  fx_diffusion_enable_graph (enable_hip_graph, solver.rs:173-184) makes step_n replay a
+ * captured HIP graph of its unit (two Euler steps, or one RK step) on the backend's stream; the graph
+ * is recaptured when step_n is given a different u. */
+typedef struct fx_diffusion fx_diffusion;
+typedef enum { FX_SCHEME_EULER = 0, FX_SCHEME_RK2 = 1, FX_SCHEME_RK4 = 2 } fx_time_scheme;
+/* DiffusionSolver::build(grid, alpha, dt, boundaries, scheme, device) (solver.rs:96-148);
+ * new / with_boundaries are the Euler forms (solver.rs:51-93). */
+fx_status fx_diffusion_create(fx_backend* be, const fx_grid* grid, double alpha, const double* dt,
+                              const fx_boundaries* bc, int32_t scheme, fx_diffusion** out);
+void fx_diffusion_destroy(fx_diffusion* d);
+fx_status fx_diffusion_enable_graph(fx_diffusion* d);                                  /* solver.rs:182-184 */
+fx_status fx_diffusion_step(fx_diffusion* d, fx_buf* u, const fx_buf* source);        /* solver.rs:187-222 */
+fx_status fx_diffusion_step_n(fx_diffusion* d, fx_buf* u, uint64_t n);                /* solver.rs:224-320 */
+fx_status fx_diffusion_step_n_with_source(fx_diffusion* d, fx_buf* u, uint64_t n,
+                                          const fx_buf* source);                      /* solver.rs:322-337 */
+double fx_diffusion_dt(const fx_diffusion* d);                                         /* solver.rs:149-153 */
+uint64_t fx_diffusion_steps_done(const fx_diffusion* d);                               /* solver.rs:155-159 */
+double fx_diffusion_sim_time(const fx_diffusion* d);                                   /* solver.rs:161-165 */
+int32_t fx_diffusion_scheme(const fx_diffusion* d);                                    /* solver.rs:167-171 */
+
 /* ── raw field format (io/src/raw.rs:1-77) ───────────────────────────────────────────────
  * [rows:u64][cols:u64][dx:f64][dy:f64][data:f64 × rows·cols], all little-endian, row-major.
  * Reading validates the header as Grid::new does (FX_ERR_GRID) and fails on a short file

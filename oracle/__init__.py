@@ -385,6 +385,55 @@ def cn_step(u, grid, alpha, dt, bc=None, source=None, tol=1e-8, max_iter=200, mg
     return pcg(u, rhs, grid, system, tol, max_iter, mg=mg)
 
 
+# ── DiffusionSolver (runtime/src/solver.rs:96-494) ─────────────────────
+def diffusion_dt(grid, alpha, scheme):
+    """Default step safety·min(dx, dy)²/(4α), safety 0.4 / 0.8 / 1.4 for Euler / RK2 / RK4 (solver.rs:103-113)."""
+    h_min = min(grid.dx, grid.dy)
+    safety = (0.4, 0.8, 1.4)[int(scheme)]
+    return safety * h_min * h_min / (4.0 * alpha)
+
+
+def diffusion_step(u, grid, alpha, dt, bc=None, scheme=0, source=None):
+    """One explicit step of the oracle, the reference's operations in its order:
+    Euler (solver.rs:344-367), Heun (371-418), RK4 (421-494).  Returns the new u."""
+    bc = bc or Boundaries.zero_dirichlet()
+    st = Stencil.laplacian_2d_5pt(grid.dx, grid.dy)
+    u = np.array(u, dtype=np.float64, copy=True).ravel()
+    src = None if source is None else np.ascontiguousarray(source, dtype=np.float64).ravel()
+    rows, cols = grid.rows, grid.cols
+    if int(scheme) == 0:
+        u = stencil_axpy(alpha * dt, u.copy(), u, grid, st, bc)
+        if src is not None:
+            u = axpy(dt, src, u)
+        return u
+    if int(scheme) == 1:
+        coeff = dt * alpha
+        u_old = u.copy()
+        for _ in range(2):
+            u = stencil_axpy(coeff, u.copy(), u, grid, st, bc)
+            if src is not None:
+                u = axpy(dt, src, u)
+        u = axpy(1.0, u_old, u)
+        return scale(u, 0.5)
+    u_old = u.copy()
+    k1 = apply_stencil(u, grid, st, bc)
+    u = stencil_axpy(0.5 * dt * alpha, u_old, u_old.copy(), grid, st, bc)
+    k2 = apply_stencil(u, grid, st, bc)
+    u = zero_boundaries(axpy(0.5 * dt * alpha, k2, u_old.copy()), rows, cols)
+    k3 = apply_stencil(u, grid, st, bc)
+    u = zero_boundaries(axpy(dt * alpha, k3, u_old.copy()), rows, cols)
+    k4 = apply_stencil(u, grid, st, bc)
+    u = u_old.copy()
+    c = dt * alpha / 6.0
+    u = axpy(c, k1, u)
+    u = axpy(2.0 * c, k2, u)
+    u = axpy(2.0 * c, k3, u)
+    u = axpy(c, k4, u)
+    if src is not None:
+        u = axpy(dt, src, u)
+    return u
+
+
 def num_threads() -> int:
     return lib().orc_num_threads()
 

@@ -240,3 +240,131 @@ def test_raw_device_round_trip(be, tmp_path):
     hip.write_raw(tmp_path / "s.raw", F32)
     _, d3 = hip.read_raw_host(tmp_path / "s.raw")
     assert np.array_equal(d3, data.astype(np.float32).astype(np.float64))
+
+
+# ── explicit DiffusionSolver (runtime/src/solver.rs) ──────────────────
+def _schemes():
+    from fluxion_amd.core import TimeScheme
+    return [TimeScheme.EULER, TimeScheme.RK2, TimeScheme.RK4]
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+@pytest.mark.parametrize("bcs", ["zero", "mixed"])
+def test_diffusion_steps_bitwise_vs_oracle(be, scheme, bcs):
+    # Every step is the reference's operations in its order, so the state matches the oracle's bit for
+    # bit: seven steps (Euler's odd count ends on a lone step), then three with a source.
+    hip = H()
+    from fluxion_amd.core import TimeScheme
+    g = Grid(37, 53, 1.0 / 52, 1.0 / 36)
+    bc = ZD if bcs == "zero" else Boundaries(BoundaryCondition.dirichlet(0.5), BoundaryCondition.neumann(0.2),
+                                             BoundaryCondition.robin(1.0, 0.5, 0.3), BoundaryCondition.dirichlet(0.0))
+    rng = np.random.default_rng(11)
+    u0 = rng.standard_normal(len(g))
+    src = rng.standard_normal(len(g))
+    d = hip.DiffusionSolver.build(g, 0.8, None, bc, TimeScheme(scheme), be=be)
+    assert d.dt() == O.diffusion_dt(g, 0.8, scheme)
+    assert d.scheme() == TimeScheme(scheme)
+    u = hip.Field.from_f64(g, u0, be)
+    d.step_n(u, 7)
+    ur = u0
+    for _ in range(7):
+        ur = O.diffusion_step(ur, g, 0.8, d.dt(), bc, scheme)
+    assert np.array_equal(u.to_vec_f64(), ur)
+    S = hip.Field.from_f64(g, src, be)
+    d.step(u, S)
+    d.step_n_with_source(u, 2, S)
+    for _ in range(3):
+        ur = O.diffusion_step(ur, g, 0.8, d.dt(), bc, scheme, source=src)
+    assert np.array_equal(u.to_vec_f64(), ur)
+    assert d.steps_done() == 10
+    assert d.sim_time() == 10 * d.dt()
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_diffusion_graph_matches_regular(be, scheme):
+    # hip_graph.rs:31-67 (graph_euler_matches_regular, there to 1e-10): bitwise here, for every scheme,
+    # odd step counts, repeated calls and a second field (recaptured).
+    hip = H()
+    from fluxion_amd.core import TimeScheme
+    n = 64
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    data = gaussian_ic(n, h)
+    a = hip.DiffusionSolver.build(g, 0.01, None, ZD, TimeScheme(scheme), be=be)
+    b = hip.DiffusionSolver.build(g, 0.01, None, ZD, TimeScheme(scheme), be=be)
+    b.enable_hip_graph()
+    ua, ub = hip.Field.from_f64(g, data, be), hip.Field.from_f64(g, data, be)
+    for n_steps in (200, 3, 1, 0):
+        a.step_n(ua, n_steps)
+        b.step_n(ub, n_steps)
+        assert np.array_equal(ua.to_vec_f64(), ub.to_vec_f64()), n_steps
+    assert a.steps_done() == b.steps_done() == 204
+    uc = hip.Field.from_f64(g, data, be)
+    b.step_n(uc, 5)
+    a2 = hip.DiffusionSolver.build(g, 0.01, None, ZD, TimeScheme(scheme), be=be)
+    ud = hip.Field.from_f64(g, data, be)
+    a2.step_n(ud, 5)
+    assert np.array_equal(uc.to_vec_f64(), ud.to_vec_f64())
+    b.step_n(ub, 2)          # back to the first field
+    a.step_n(ua, 2)
+    assert np.array_equal(ua.to_vec_f64(), ub.to_vec_f64())
+
+
+def test_diffusion_reference_behaviour(be):
+    # solver.rs:676-800 (decreasing peak, incremental step_n, source heats, RK4 stable),
+    # solvers.rs:544-582 (RK2 diffuses), 735-805 (Neumann mass, Dirichlet decay),
+    # stability.rs:15-90 (auto dt stable for 1000 steps, tiny and large α), hip_graph.rs:69-100.
+    hip = H()
+    from fluxion_amd.core import TimeScheme
+    n = 64
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    data = gaussian_ic(n, h)
+    u = hip.Field.from_f64(g, data, be)
+    hip.DiffusionSolver.new(g, 0.01, be=be).step_n(u, 100)
+    assert 0.0 < u.max() < data.max()
+    gr = hip.DiffusionSolver.new(g, 0.01, be=be)
+    gr.enable_hip_graph()
+    u = hip.Field.from_f64(g, data, be)
+    gr.step_n(u, 500)
+    assert u.max() < data.max()
+    u = hip.Field.zeros(g, be=be)
+    u.fill(1.0)
+    hip.DiffusionSolver.new(g, 1.0, be=be).step_n(u, 1000)
+    assert np.isfinite(u.norm_l2()) and u.norm_l2() > 0.0
+
+    n = 32
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    data = gaussian_ic(n, h)
+    d = hip.DiffusionSolver.new(g, 0.01, be=be)
+    u = hip.Field.from_f64(g, np.ones(n * n), be)
+    d.step_n(u, 10)
+    assert d.steps_done() == 10
+    d.step_n(u, 5)
+    assert d.steps_done() == 15
+    u = hip.Field.zeros(g, be=be)
+    d = hip.DiffusionSolver.new(g, 0.01, be=be)
+    d.step_n_with_source(u, 50, hip.Field.from_f64(g, np.full(n * n, 100.0), be))
+    assert u.max() > 0.0
+    x = np.arange(n) * h
+    u = hip.Field.from_f64(g, np.outer(np.sin(np.pi * x), np.sin(np.pi * x)).ravel(), be)
+    hip.DiffusionSolver.build(g, 0.01, None, ZD, TimeScheme.RK4, be=be).step_n(u, 200)
+    assert np.isfinite(u.norm_l2()) and u.norm_l2() > 0.0
+    u = hip.Field.from_f64(g, data, be)
+    hip.DiffusionSolver.build(g, 1.0, None, ZD, TimeScheme.RK2, be=be).step_n(u, 100)
+    assert np.isfinite(u.max()) and u.max() < data.max()
+    neu = Boundaries.uniform(BoundaryCondition.neumann(0.0))
+    u = hip.Field.from_f64(g, data, be)
+    s0 = u.sum()
+    hip.DiffusionSolver.with_boundaries(g, 1.0, None, neu, be=be).step_n(u, 200)
+    s1 = u.sum()
+    assert np.isfinite(s1) and s1 > 0.0 and abs(s1 - s0) / abs(s0) < 0.5
+    u = hip.Field.from_f64(g, data, be)
+    n0 = u.norm_l2()
+    hip.DiffusionSolver.with_boundaries(g, 1.0, None, ZD, be=be).step_n(u, 2000)
+    assert u.norm_l2() < 0.1 * n0
+    for alpha in (1e-8, 100.0):
+        u = hip.Field.from_f64(g, np.ones(n * n), be)
+        hip.DiffusionSolver.new(g, alpha, be=be).step_n(u, 100 if alpha < 1 else 50)
+        assert np.isfinite(u.norm_l2())

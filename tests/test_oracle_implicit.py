@@ -137,6 +137,38 @@ def test_cn_step_multigrid_and_flexible_agree_with_cg():
     assert rel_l2(u_b, u_a) < 1e-10
 
 
+# ── explicit DiffusionSolver steps (solver.rs:344-494) ─────────────────
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_diffusion_step_eigenmodes_closed_form(scheme):
+    # For an eigenmode m of Δ_h (λ, zero on the boundary) and z = dt·α·λ, one step multiplies m by the
+    # scheme's stability polynomial: Euler 1 + z, Heun 1 + z + z²/2, RK4 1 + z + z²/2 + z³/6 + z⁴/24.
+    # A source s·m enters as dt·s·m (Euler and RK4, which adds it once at the end: solver.rs:488-490) or
+    # dt·s·(2 + z)/2·m (Heun adds it in both stages, then averages).
+    g = rect_grid()
+    alpha = 0.9
+    dt = O.diffusion_dt(g, alpha, scheme)
+    m1, m2 = mode(g, 1, 2), mode(g, 5, 3)
+    out = np.zeros(len(g))
+    u0 = m1 + 0.25 * m2
+    for m, a, lam in ((m1, 1.0, eig(g, 1, 2)), (m2, 0.25, eig(g, 5, 3))):
+        z = dt * alpha * lam
+        R = (1 + z, 1 + z + z * z / 2, 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24)[scheme]
+        out += a * R * m
+    src = 3.0 * m1
+    z1 = dt * alpha * eig(g, 1, 2)
+    out += (dt * 3.0 * (2 + z1) / 2 if scheme == 1 else dt * 3.0) * m1
+    u1 = O.diffusion_step(u0, g, alpha, dt, scheme=scheme, source=src)
+    assert rel_l2(u1, out) < 1e-13
+
+
+def test_diffusion_default_dt_is_cfl_safe():
+    # dt = safety·h_min²/(4α) (solver.rs:103-113): Euler's 0.4 keeps |1 + z| < 1 for the stiffest mode.
+    g = Grid(17, 33, 1.0 / 32, 1.0 / 16)
+    assert O.diffusion_dt(g, 2.0, 0) == 0.4 * (1.0 / 32) * (1.0 / 32) / 8.0
+    zmax = O.diffusion_dt(g, 2.0, 0) * 2.0 * eig(g, 31, 15)
+    assert -2.0 < zmax < 0.0
+
+
 # ── raw field format ──────────────────────────────────────────────────
 def hip():
     from fluxion_amd import hip as H
