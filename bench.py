@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cn", action="store_true", help="skip the Crank-Nicolson record")
     ap.add_argument("--cn-steps", type=int, default=100)
+    ap.add_argument("--no-explicit", action="store_true", help="skip the explicit diffusion record")
     ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
@@ -300,6 +301,70 @@ def main():
             f"final states differ by {cn['final_state_rel_diff']:.1e}")
         del kc
 
+    # ── explicit diffusion (DiffusionSolver) on the 4096² grid, step_n from a captured graph ──
+    explicit = None
+    if not args.no_explicit:
+        from fluxion_amd.core import TimeScheme
+        ue0 = gaussian_bumps(n).ravel()
+        UE = hip.Field.from_f64(g, ue0, be)
+        explicit = {"grid": n, "alpha": 1.0, "dt": "scheme default (CFL-safe)", "steps": 200}
+        for name, sch, bpp in (("euler", TimeScheme.EULER, 16), ("rk2", TimeScheme.RK2, 40),
+                               ("rk4", TimeScheme.RK4, None)):
+            dsol = hip.DiffusionSolver.build(g, 1.0, None, bc, sch, be=be)
+            dsol.step_n(UE, 4)
+            be.synchronize()
+            t0 = time.perf_counter()
+            dsol.step_n(UE, 200)
+            be.synchronize()
+            t = (time.perf_counter() - t0) / 200
+            rec = {"us_per_step": 1e6 * t, "steps_per_s": 1.0 / t}
+            if bpp is not None:
+                rec["bytes_per_point"] = bpp
+                rec["GBs_algorithmic"] = bpp * n * n / t / 1e9
+                rec["frac"] = rec["GBs_algorithmic"] / HBM_PEAK_GBS
+            explicit[name] = rec
+            del dsol
+        # Euler from the captured graph against direct launches of the same kernels, here and on a small
+        # grid, where launch latency rather than bandwidth sets the step time.
+        for gn, tag in ((n, ""), (256, "_256")):
+            gg = g if gn == n else Grid.square(gn, 1.0 / (gn - 1))
+            UU = UE if gn == n else hip.Field.from_f64(gg, gaussian_bumps(gn).ravel(), be)
+            for use_graph in (True, False):
+                dsol = hip.DiffusionSolver.new(gg, 1.0, be=be)
+                if use_graph:
+                    dsol.enable_hip_graph()
+                dsol.step_n(UU, 4)
+                be.synchronize()
+                t0 = time.perf_counter()
+                dsol.step_n(UU, 200)
+                be.synchronize()
+                t = (time.perf_counter() - t0) / 200
+                explicit["euler" + tag + ("_graph" if use_graph else "_no_graph")] = {"us_per_step": 1e6 * t}
+                del dsol
+        # The reference's Euler step as its own op sequence (copy u → s, then stencil_axpy s → u).
+        SE = hip.Field.zeros(g, be=be)
+        lap5 = Stencil.laplacian_2d_5pt(h, h)
+        dt_e = 0.4 * h * h / 4.0
+        for _ in range(4):
+            SE.copy_from(UE)
+            UE.stencil_axpy(dt_e, SE, lap5, bc)
+        be.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(200):
+            SE.copy_from(UE)
+            UE.stencil_axpy(dt_e, SE, lap5, bc)
+        be.synchronize()
+        t = (time.perf_counter() - t0) / 200
+        explicit["euler_reference_op_sequence"] = {"us_per_step": 1e6 * t, "steps_per_s": 1.0 / t,
+                                                   "bytes_per_point": 40}
+        explicit["euler_speedup_vs_op_sequence"] = t / (1e-6 * explicit["euler"]["us_per_step"])
+        log(f"bench: explicit diffusion {n}^2: Euler {explicit['euler']['us_per_step']:.1f} us/step "
+            f"({explicit['euler']['GBs_algorithmic']:.0f} GB/s at 16 B/pt), reference op sequence "
+            f"{1e6 * t:.1f} us/step, graph {explicit['euler_graph']['us_per_step']:.1f}; 256^2 graph "
+            f"{explicit['euler_256_graph']['us_per_step']:.1f} vs {explicit['euler_256_no_graph']['us_per_step']:.1f} us/step; "
+            f"RK2 {explicit['rk2']['us_per_step']:.1f}, RK4 {explicit['rk4']['us_per_step']:.1f} us/step")
+        del UE, SE
+
     # ── CPU baseline: the oracle's plain FP64 MG-PCG on the same var-k problem ──
     cpu = None
     if not args.no_cpu:
@@ -350,6 +415,7 @@ def main():
         },
         "poisson_4096": poisson,
         "cn_vark_1024": cn,
+        "explicit_diffusion_4096": explicit,
         "kernels": kernels,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -34,7 +34,8 @@ struct fx_cn {
 // DiffusionSolver (solver.rs:25-48).  Explicit steps write a second buffer
 // instead of copying u first: Euler and the first Heun stage go u → s, the
 // second Heun stage s → u, and step_n pairs Euler steps u → s → u, so no step
-// copies data except a lone Euler step, which copies s back to u.
+// copies data except a lone Euler step, which copies s back to u.  The graph
+// of step_n holds many units (graph_units), replayed while whole graphs fit.
 struct fx_diffusion {
 	fx_backend* be = nullptr;
 	fx_grid g{};
@@ -46,7 +47,7 @@ struct fx_diffusion {
 	fx_stencil st{};
 	std::vector<fx_buf*> scratch;
 	bool use_graph = false;
-	// Captured HIP graph of step_n's unit (two Euler steps, or one RK step), for the u it was captured on.
+	// Captured HIP graph of graph_units() step_n units, for the u it was captured on.
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	void* graph_u = nullptr;
@@ -166,6 +167,11 @@ fx_status diff_unit(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
 	return diff_step(d, u, src);
 }
 int unit_steps(const fx_diffusion* d) { return d->scheme == FX_SCHEME_EULER ? 2 : 1; }
+// Units per captured graph: about 64 kernels, so that one graph launch (which
+// costs more than a kernel launch on this runtime) is spread over many steps.
+int graph_units(const fx_diffusion* d) {
+	return d->scheme == FX_SCHEME_EULER ? 32 : d->scheme == FX_SCHEME_RK2 ? 32 : 4;
+}
 
 void drop_graph(fx_diffusion* d) {
 	if (d->exec) (void)hipGraphExecDestroy(d->exec);
@@ -182,7 +188,8 @@ fx_status capture_unit(fx_diffusion* d, fx_buf* u) {
 	drop_graph(d);
 	hipStream_t st = d->be->stream;
 	FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-	fx_status s = diff_unit(d, u, nullptr);
+	fx_status s = FX_OK;
+	for (int i = 0; i < graph_units(d) && s == FX_OK; ++i) s = diff_unit(d, u, nullptr);
 	hipGraph_t g = nullptr;
 	hipError_t e = hipStreamEndCapture(st, &g);
 	if (s != FX_OK) {
@@ -428,20 +435,20 @@ static fx_status step_many(fx_diffusion* d, fx_buf* u, uint64_t n, const fx_buf*
 	FX_CHECK(fx::check_buf(u, len, FX_F64, "u"));
 	if (source) FX_CHECK(fx::check_buf(source, len, FX_F64, "source"));
 	const uint64_t per = (uint64_t)unit_steps(d);
-	const uint64_t units = n / per;
-	if (graph && units > 0) {
+	uint64_t done = 0;
+	const uint64_t gsteps = per * (uint64_t)graph_units(d);
+	if (graph && n >= gsteps) {
 		FX_CHECK(capture_unit(d, u));
-		for (uint64_t i = 0; i < units; ++i) {
+		for (; done + gsteps <= n; done += gsteps) {
 			FX_HIP(hipGraphLaunch(d->exec, d->be->stream));
-			d->steps += per;
-		}
-	} else {
-		for (uint64_t i = 0; i < units; ++i) {
-			FX_CHECK(diff_unit(d, u, source));
-			d->steps += per;
+			d->steps += gsteps;
 		}
 	}
-	for (uint64_t i = units * per; i < n; ++i) {
+	for (; done + per <= n; done += per) {
+		FX_CHECK(diff_unit(d, u, source));
+		d->steps += per;
+	}
+	for (uint64_t i = done; i < n; ++i) {
 		FX_CHECK(diff_step(d, u, source));
 		d->steps += 1;
 	}
