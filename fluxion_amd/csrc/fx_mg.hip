@@ -272,6 +272,16 @@ __global__ void k_faces(const float* k, float* kx, float* ky, int64_t rows, int6
 	ky[i] = r + 1 < rows ? fmg::hmf(k[i], k[i + cols]) : 0.0f;
 }
 
+// {kx, ky, ω/d} of each point side by side, so that a wave reads a row's three
+// coefficients with one 12-byte load per lane instead of three 4-byte loads.
+__global__ void k_pack3(const float* kx, const float* ky, const float* wd, float* kp, int64_t n) {
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+		kp[3 * i] = kx[i];
+		kp[3 * i + 1] = ky[i];
+		kp[3 * i + 2] = wd[i];
+	}
+}
+
 // ω/diagonal of the variable-coefficient level operator on interior points, from the
 // face coefficients (the d of fmg::vkff, so wd equals the ω/d of fmg::jackf).
 __global__ void k_wdiag(const float* kx, const float* ky, float* wd, fmg::LWf L, int64_t rows, int64_t cols) {
@@ -417,6 +427,7 @@ void free_levels(fx_multigrid* mg) {
 		fx::dev_free(L.kx);
 		fx::dev_free(L.ky);
 		fx::dev_free(L.wd);
+		fx::dev_free(L.kp);
 		fx::dev_free(L.xbuf);
 	}
 	mg->lv.clear();
@@ -632,6 +643,14 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 				                   (const float*)L.ky, (float*)L.wd, fmg::lwf(mg, L), (int64_t)L.g.rows,
 				                   (int64_t)L.g.cols);
 				s = check_launch("multigrid omega/diagonal");
+			}
+			if (s == FX_OK) s = dev_alloc(be, 3 * bytes, &L.kp);
+			if (s == FX_OK) {
+				const int64_t n = (int64_t)L.g.rows * (int64_t)L.g.cols;
+				hipLaunchKernelGGL(k_pack3, dim3((unsigned)std::min<int64_t>((n + BLOCK - 1) / BLOCK, 65536)),
+				                   dim3(BLOCK), 0, be->stream, (const float*)L.kx, (const float*)L.ky,
+				                   (const float*)L.wd, (float*)L.kp, n);
+				s = check_launch("multigrid packed coefficients");
 			}
 			if (s != FX_OK) { free_levels(mg); delete mg; return s; }
 		}

@@ -34,6 +34,7 @@ struct fx_multigrid {
 		void* kx = nullptr;    // FP32 fused hierarchies: harmonic-mean face coefficient east of each point
 		void* ky = nullptr;    //   and south of each point (float[n]; 0 past the last column / row)
 		void* wd = nullptr;    //   and ω/diagonal of the level operator (float[n]; 0 on the boundary)
+		void* kp = nullptr;    //   the same three packed per point, {kx, ky, ω/d} (float[3n]): one load per row
 		double w[5];           // sigma·I + scale·Δ_h weights, laplacian_2d_5pt order
 		double inv_diag;
 		fx_xtab xt;            // transfer to the next coarser level (linear transfers only)

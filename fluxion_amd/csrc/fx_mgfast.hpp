@@ -33,6 +33,7 @@ struct LWf {
 	const float* kx;
 	const float* ky;
 	const float* wd;
+	const float* kp;   // {kx, ky, wd} packed per point (the wave kernels' row loads)
 	float ix2, iy2, sigma, scale;
 };
 
@@ -990,6 +991,7 @@ inline LWf lwf(const fx_multigrid* mg, const fx_multigrid::Level& L) {
 	o.kx = (const float*)L.kx;
 	o.ky = (const float*)L.ky;
 	o.wd = (const float*)L.wd;
+	o.kp = (const float*)L.kp;
 	o.ix2 = float(1.0 / (L.g.dx * L.g.dx));
 	o.iy2 = float(1.0 / (L.g.dy * L.g.dy));
 	o.sigma = float(mg->opt.sigma);
@@ -1020,9 +1022,9 @@ __device__ __forceinline__ float lane_east(float v) {
 
 // One row of a level's inputs at lane column gc: b and, for variable
 // coefficients, the faces west (kx[gc − 1]), east (kx[gc]), north (ky one row
-// up) and south (ky[gc]) of the point, and ω/d.  wload reads b, the east and
-// south faces and ω/d unconditionally from clamped addresses (a uniform row
-// offset plus the lane's 32-bit column): four loads per row.  wtake completes
+// up) and south (ky[gc]) of the point, and ω/d.  wload reads b and the packed
+// {east face, south face, ω/d} unconditionally from clamped addresses (a
+// uniform row offset plus the lane's 32-bit column): two loads per row.  wtake completes
 // the row when it is consumed: the west face is the east face of lane x − 1
 // and the north face the south face of the row above (the previous step's
 // row, still in the ring).  Lane 0's west face (0) and the north face of a
@@ -1035,6 +1037,9 @@ __device__ __forceinline__ float lane_east(float v) {
 struct WRow {
 	float b, kw, ke, kn, ks, wd;
 };
+struct __attribute__((aligned(4))) KP3 {
+	float kx, ky, wd;
+};
 template <bool VK, bool LOADB = true>
 __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L, int r, int rows, int cols,
                                       unsigned c) {
@@ -1043,9 +1048,10 @@ __device__ __forceinline__ WRow wload(const float* __restrict__ b, const LWf& L,
 	WRow v;
 	v.b = LOADB ? (b + o)[c] : 0.0f;
 	if constexpr (VK) {
-		v.ke = (L.kx + o)[c];
-		v.ks = (L.ky + o)[c];
-		v.wd = (L.wd + o)[c];
+		const KP3 q = ((const KP3*)(L.kp + 3 * o))[c];
+		v.ke = q.kx;
+		v.ks = q.ky;
+		v.wd = q.wd;
 		v.kw = v.kn = 1.0f;
 	} else {
 		v.kw = v.ke = v.kn = v.ks = 1.0f;
