@@ -69,9 +69,9 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
 # profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels.
 MEASURED_TRAFFIC_4096 = {
-    "vc_restrict0": (278458 * 2 + 152321) * 1024,
-    "vc_post0": (289649 * 2 + 74963) * 1024,
-    "fcg_step": (244650 * 2 + 393248) * 1024,
+    "vc_restrict0": (277170 * 2 + 151884) * 1024,
+    "vc_post0": (289092 * 2 + 74922) * 1024,
+    "fcg_step": (244617 * 2 + 393248) * 1024,
 }
 
 
