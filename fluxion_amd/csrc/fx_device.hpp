@@ -249,6 +249,67 @@ __global__ __launch_bounds__(BLOCK) void k_stream5(V v, E e, F fin, W9 w, int64_
 	if constexpr (NS > 0) block_partial<NS>(acc, rs);
 }
 
+// v of lane x − 1 (CTRL 0x138, wave_shr:1) or x + 1 (0x130, wave_shl:1) for a
+// double, as two 32-bit DPP moves; 0 past the wave.
+template <int CTRL>
+__device__ __forceinline__ double dlane64(double v) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// k_stream5 for an even column count, two columns per thread: one 16-byte load
+// of the pair per row instead of a load per point and its west and east
+// neighbours; the neighbours come from the adjacent lanes by DPP (the wave's
+// first and last lanes read the column beyond the wave).  The epilogue gets
+// both points at once (E::pair) and moves its own vectors 16 bytes at a time.  Rows past the grid
+// are read from the last row (clamped) and never used.  Every lane walks the
+// rows, so the cross-lane moves see all 64 lanes.  chain5 is evaluated on the
+// same operands per point, so the results equal k_stream5's.
+template <class E>
+__global__ __launch_bounds__(BLOCK) void k_stream5d(const double* __restrict__ x, E e, W9 w, int64_t rows,
+                                                    int64_t cols, int rpb, const int* done) {
+	if (done && *done) return;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int lane = (int)(threadIdx.x & (WAVE - 1));
+	const int64_t c0 = 2 * ((int64_t)bx * BLOCK + threadIdx.x);
+	const int64_t r0 = (int64_t)by * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool ok = c0 < cols;
+	const int64_t cl = ok ? c0 : cols - 2;
+	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;
+	const bool edgeW = lane == 0 && ok && c0 > 0, edgeE = lane == WAVE - 1 && ok && c0 + 2 < cols;
+	struct Row {
+		double2 v;
+		double w, e;
+	};
+	auto ld = [&](int64_t r) {
+		Row o;
+		const int64_t i = r * cols + cl;
+		o.v = *(const double2*)(x + i);
+		o.w = edgeW ? x[i - 1] : 0.0;
+		o.e = edgeE ? x[i + 2] : 0.0;
+		return o;
+	};
+	Row up = ld(r0 > 0 ? r0 - 1 : r0);
+	Row cur = ld(r0);
+	for (int64_t r = r0; r < r1; ++r) {
+		const Row dn = ld(r + 1 < rows ? r + 1 : r);
+		double wv = dlane64<0x138>(cur.v.y), ev = dlane64<0x130>(cur.v.x);
+		if (lane == 0) wv = cur.w;
+		if (lane == WAVE - 1) ev = cur.e;
+		if (ok) {
+			const bool rin = r > 0 && r < rows - 1;
+			e.pair(r * cols + c0, rin && intA, chain5(w, cur.v.x, up.v.x, dn.v.x, wv, cur.v.y), rin && intB,
+			       chain5(w, cur.v.y, up.v.y, dn.v.y, cur.v.x, ev), cur.v);
+		}
+		up = cur;
+		cur = dn;
+	}
+}
+
 // Rows per block for the streaming kernels: enough blocks to fill 256 CUs
 // several times over, long enough columns to amortise the two halo rows.
 inline int stream_rpb(int64_t rows, int64_t cols) {

@@ -250,12 +250,15 @@ def _schemes():
 
 @pytest.mark.parametrize("scheme", [0, 1, 2])
 @pytest.mark.parametrize("bcs", ["zero", "mixed"])
-def test_diffusion_steps_bitwise_vs_oracle(be, scheme, bcs):
+@pytest.mark.parametrize("cols", [53, 258])
+def test_diffusion_steps_bitwise_vs_oracle(be, scheme, bcs, cols):
     # Every step is the reference's operations in its order, so the state matches the oracle's bit for
-    # bit: seven steps (Euler's odd count ends on a lone step), then three with a source.
+    # bit: seven steps (Euler's odd count ends on a lone step), then three with a source.  An odd column
+    # count runs the one-column stencil kernel, an even one the two-column kernel (a wave boundary inside
+    # the row at 258 columns).
     hip = H()
     from fluxion_amd.core import TimeScheme
-    g = Grid(37, 53, 1.0 / 52, 1.0 / 36)
+    g = Grid(37, cols, 1.0 / (cols - 1), 1.0 / 36)
     bc = ZD if bcs == "zero" else Boundaries(BoundaryCondition.dirichlet(0.5), BoundaryCondition.neumann(0.2),
                                              BoundaryCondition.robin(1.0, 0.5, 0.3), BoundaryCondition.dirichlet(0.0))
     rng = np.random.default_rng(11)
