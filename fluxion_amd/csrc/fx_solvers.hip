@@ -2235,7 +2235,13 @@ template <class T>
 fx_status upload_vec(fx_backend* be, const std::vector<T>& v, T** out) {
 	void* p = nullptr;
 	FX_CHECK(fx::dev_alloc(be, std::max<size_t>(1, v.size()) * sizeof(T), &p));
-	if (!v.empty()) FX_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	// The copy goes on the backend's stream, behind dev_alloc's zero fill: a copy on the null stream is
+	// not ordered after work on this non-blocking stream, and the fill could land on the table afterwards.
+	// It is finished before returning, as v may be a temporary.
+	if (!v.empty()) {
+		FX_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, be->stream));
+		FX_HIP(hipStreamSynchronize(be->stream));
+	}
 	*out = (T*)p;
 	return FX_OK;
 }
