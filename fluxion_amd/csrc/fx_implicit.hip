@@ -34,7 +34,9 @@ struct fx_cn {
 // DiffusionSolver (solver.rs:25-48).  Explicit steps write a second buffer
 // instead of copying u first: Euler and the first Heun stage go u → s, the
 // second Heun stage s → u, and step_n pairs Euler steps u → s → u, so no step
-// copies data except a lone Euler step, which copies s back to u.  The graph
+// copies data except a lone Euler step, which copies s back to u.  RK4 keeps
+// u as u_old until its last pass and its stages in two of its five scratch
+// vectors (the other three hold k1 … k3).  The graph
 // of step_n holds many units (graph_units), replayed while whole graphs fit.
 struct fx_diffusion {
 	fx_backend* be = nullptr;
@@ -115,36 +117,13 @@ fx_status rk2_step(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
 	                          (const double*)u->ptr, &d->st, rows, cols);
 }
 
-// Classical RK4 (solver.rs:421-494), the reference's operations in its order;
-// scratch = [k1, k2, k3, k4, u_old].  The stage u = u_old + ½dt·α·Δ(u_old) is
-// one pass instead of a copy and a stencil_axpy.
+// Classical RK4 (solver.rs:421-494): four fused stencil passes (fx::rk4_passes),
+// each point computed by the reference's operations in its order.
 fx_status rk4_step(fx_diffusion* d, fx_buf* u, const fx_buf* src) {
-	fx_backend* be = d->be;
-	const double dt = d->dt, alpha = d->alpha;
-	fx_buf** k = d->scratch.data();
-	fx_buf* uold = k[4];
-	const int64_t rows = (int64_t)d->g.rows, cols = (int64_t)d->g.cols;
-	FX_CHECK(fx_copy(be, u, uold));
-	FX_CHECK(fx_apply_stencil(be, u, k[0], &d->g, &d->st, &d->bc));
-	FX_CHECK(fx::explicit_stage(be, (const double*)uold->ptr, (double*)u->ptr, 0.5 * dt * alpha, nullptr, 0.0, nullptr,
-	                            &d->st, rows, cols));
-	FX_CHECK(fx_apply_stencil(be, u, k[1], &d->g, &d->st, &d->bc));
-	FX_CHECK(fx_copy(be, uold, u));
-	FX_CHECK(fx_axpy(be, 0.5 * dt * alpha, k[1], u));
-	FX_CHECK(fx_zero_boundaries(be, u, d->g.rows, d->g.cols));
-	FX_CHECK(fx_apply_stencil(be, u, k[2], &d->g, &d->st, &d->bc));
-	FX_CHECK(fx_copy(be, uold, u));
-	FX_CHECK(fx_axpy(be, dt * alpha, k[2], u));
-	FX_CHECK(fx_zero_boundaries(be, u, d->g.rows, d->g.cols));
-	FX_CHECK(fx_apply_stencil(be, u, k[3], &d->g, &d->st, &d->bc));
-	FX_CHECK(fx_copy(be, uold, u));
-	const double c = dt * alpha / 6.0;
-	FX_CHECK(fx_axpy(be, c, k[0], u));
-	FX_CHECK(fx_axpy(be, 2.0 * c, k[1], u));
-	FX_CHECK(fx_axpy(be, 2.0 * c, k[2], u));
-	FX_CHECK(fx_axpy(be, c, k[3], u));
-	if (src) FX_CHECK(fx_axpy(be, dt, src, u));
-	return FX_OK;
+	double* k[5];
+	for (int i = 0; i < 5; ++i) k[i] = (double*)d->scratch[i]->ptr;
+	return fx::rk4_passes(d->be, (double*)u->ptr, k, src ? (const double*)src->ptr : nullptr, d->dt, d->alpha, &d->st,
+	                      &d->bc, &d->g);
 }
 
 // One step from u back into u.
@@ -170,7 +149,7 @@ int unit_steps(const fx_diffusion* d) { return d->scheme == FX_SCHEME_EULER ? 2 
 // Units per captured graph: about 64 kernels, so that one graph launch (which
 // costs more than a kernel launch on this runtime) is spread over many steps.
 int graph_units(const fx_diffusion* d) {
-	return d->scheme == FX_SCHEME_EULER ? 32 : d->scheme == FX_SCHEME_RK2 ? 32 : 4;
+	return d->scheme == FX_SCHEME_EULER ? 32 : d->scheme == FX_SCHEME_RK2 ? 32 : 8;
 }
 
 void drop_graph(fx_diffusion* d) {

@@ -85,6 +85,11 @@ void dev_free(void* p);
 fx_status explicit_stage(fx_backend* be, const double* x, double* y, double coeff, const double* src, double dt,
                          const double* avg, const fx_stencil* st, int64_t rows, int64_t cols);
 
+// One classical RK4 step of ∂u/∂t = α·S(u) (+ source) in four stencil passes and
+// three BC-perimeter passes, u in place; k holds five scratch vectors of u's size.
+fx_status rk4_passes(fx_backend* be, double* u, double* const k[5], const double* src, double dt, double alpha,
+                     const fx_stencil* st, const fx_boundaries* bc, const fx_grid* g);
+
 // Validation helpers.
 fx_status check_grid(const fx_grid* g);
 fx_status check_buf(const fx_buf* b, size_t len, fx_dtype dt, const char* name);

@@ -221,6 +221,86 @@ struct EpiStage {
 	}
 };
 
+// Classical RK4 (solver.rs:421-494) in four stencil passes.  The reference
+// keeps u_old, forms each stage in u and copies u_old back; here u stays u_old
+// until the last pass and the stages go to two scratch buffers.  Per point the
+// operations are the reference's, in its order: k = S(stage) inside (the
+// boundary values of k come from the BC perimeter kernel, as apply_stencil's);
+// stage 2 = u_old + ½dtα·S(u_old) inside and u_old on the boundary (copy +
+// stencil_axpy); stages 3, 4 = u_old + coeff·k inside and 0 on the boundary
+// (copy, axpy, zero_boundaries); u = u_old + c·k1 + 2c·k2 + 2c·k3 + c·k4
+// (+ dt·source), one fma per axpy.
+struct EpiRk1 {
+	static constexpr bool PAIR = true;
+	double h; double* k; double* st;
+	template <class A> __device__ void interior(int64_t i, double lap, double c, A*) const {
+		k[i] = lap;
+		st[i] = fma(h, lap, c);
+	}
+	template <class A> __device__ void boundary(int64_t i, double c, A*) const { st[i] = c; }
+	__device__ void pair(int64_t i, bool ia, double la, bool ib, double lb, double2 c) const {
+		*(double2*)(st + i) = double2{ia ? fma(h, la, c.x) : c.x, ib ? fma(h, lb, c.y) : c.y};
+		EpiStore{k}.pair(i, ia, la, ib, lb, c);
+	}
+};
+struct EpiRkMid {
+	static constexpr bool PAIR = true;
+	double h; double* k; double* st; const double* uold;
+	template <class A> __device__ void interior(int64_t i, double lap, double, A*) const {
+		k[i] = lap;
+		st[i] = fma(h, lap, uold[i]);
+	}
+	template <class A> __device__ void boundary(int64_t i, double, A*) const { st[i] = 0.0; }
+	__device__ void pair(int64_t i, bool ia, double la, bool ib, double lb, double2 c) const {
+		const double2 u = *(const double2*)(uold + i);
+		*(double2*)(st + i) = double2{ia ? fma(h, la, u.x) : 0.0, ib ? fma(h, lb, u.y) : 0.0};
+		EpiStore{k}.pair(i, ia, la, ib, lb, c);
+	}
+};
+__device__ __forceinline__ double rk4_comb(double c, double u, double k1, double k2, double k3, double k4) {
+	return fma(c, k4, fma(2.0 * c, k3, fma(2.0 * c, k2, fma(c, k1, u))));
+}
+struct EpiRkLast {
+	static constexpr bool PAIR = true;
+	double c; double* u; const double* k1; const double* k2; const double* k3; const double* src; double dt;
+	template <class A> __device__ void interior(int64_t i, double lap, double, A*) const {
+		double t = rk4_comb(c, u[i], k1[i], k2[i], k3[i], lap);
+		if (src) t = fma(dt, src[i], t);
+		u[i] = t;
+	}
+	template <class A> __device__ void boundary(int64_t, double, A*) const {}
+	__device__ void pair(int64_t i, bool ia, double la, bool ib, double lb, double2) const {
+		if (!ia && !ib) return;
+		double2 t = *(const double2*)(u + i);
+		const double2 a = *(const double2*)(k1 + i), b = *(const double2*)(k2 + i), d = *(const double2*)(k3 + i);
+		if (ia) t.x = rk4_comb(c, t.x, a.x, b.x, d.x, la);
+		if (ib) t.y = rk4_comb(c, t.y, a.y, b.y, d.y, lb);
+		if (src) {
+			const double2 s2 = *(const double2*)(src + i);
+			if (ia) t.x = fma(dt, s2.x, t.x);
+			if (ib) t.y = fma(dt, s2.y, t.y);
+		}
+		*(double2*)(u + i) = t;
+	}
+};
+// The last pass on the boundary: u = u_old + c·k1 + … + c·k4 (+ dt·source) with the k's perimeter values.
+__global__ void k_rk4_edge(double* u, const double* k1, const double* k2, const double* k3, const double* k4,
+                           const double* src, double c, double dt, int64_t rows, int64_t cols) {
+	const int64_t per = 2 * cols + 2 * (rows - 2);
+	for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < per; q += (int64_t)gridDim.x * blockDim.x) {
+		int64_t i;
+		if (q < cols) i = q;                                         // top row
+		else if (q < 2 * cols) i = (rows - 1) * cols + (q - cols);   // bottom row
+		else {
+			const int64_t j = q - 2 * cols, r = 1 + j / 2;           // left and right columns, rows 1 … rows − 2
+			i = r * cols + (j % 2 ? cols - 1 : 0);
+		}
+		double t = rk4_comb(c, u[i], k1[i], k2[i], k3[i], k4[i]);
+		if (src) t = fma(dt, src[i], t);
+		u[i] = t;
+	}
+}
+
 // Per-point radius-1 kernel for 9-entry and generic stencils.
 template <class E>
 __global__ __launch_bounds__(BLOCK) void k_point3(const double* x, E e, W9 w, Ent ent, int mode, int64_t rows, int64_t cols,
@@ -705,6 +785,32 @@ fx_status fx_pipelined_cg_fused(fx_backend* be, double alpha, double beta, fx_bu
 }  // extern "C"
 
 namespace fx {
+fx_status rk4_passes(fx_backend* be, double* u, double* const k[5], const double* src, double dt, double alpha,
+                     const fx_stencil* st, const fx_boundaries* bc, const fx_grid* g) {
+	const int64_t rows = (int64_t)g->rows, cols = (int64_t)g->cols;
+	const int64_t per = 2 * cols + 2 * (rows - 2);
+	const dim3 pg((unsigned)((per + 255) / 256));
+	double *k1 = k[0], *k2 = k[1], *k3 = k[2], *sb = k[3], *sa = k[4];
+	auto perim = [&](const double* in, double* out) {
+		hipLaunchKernelGGL(k_bc_perimeter, pg, dim3(256), 0, be->stream, in, out, (const double*)nullptr, *bc, rows,
+		                   cols, g->dx, g->dy);
+		return check_launch("rk4 perimeter");
+	};
+	const double h2 = 0.5 * dt * alpha;
+	FX_CHECK(launch_stencil(be, u, EpiRk1{h2, k1, sa}, st, rows, cols, "rk4 pass 1"));
+	FX_CHECK(perim(u, k1));
+	FX_CHECK(launch_stencil(be, sa, EpiRkMid{h2, k2, sb, u}, st, rows, cols, "rk4 pass 2"));
+	FX_CHECK(perim(sa, k2));
+	FX_CHECK(launch_stencil(be, sb, EpiRkMid{dt * alpha, k3, sa, u}, st, rows, cols, "rk4 pass 3"));
+	FX_CHECK(perim(sb, k3));
+	const double c = dt * alpha / 6.0;
+	FX_CHECK(launch_stencil(be, sa, EpiRkLast{c, u, k1, k2, k3, src, dt}, st, rows, cols, "rk4 pass 4"));
+	FX_CHECK(perim(sa, sb));
+	hipLaunchKernelGGL(k_rk4_edge, pg, dim3(256), 0, be->stream, u, (const double*)k1, (const double*)k2,
+	                   (const double*)k3, (const double*)sb, src, c, dt, rows, cols);
+	return check_launch("rk4 edge");
+}
+
 fx_status explicit_stage(fx_backend* be, const double* x, double* y, double coeff, const double* src, double dt,
                          const double* avg, const fx_stencil* st, int64_t rows, int64_t cols) {
 	return launch_stencil(be, x, EpiStage{coeff, y, src, dt, avg}, st, rows, cols, "explicit_stage");
