@@ -369,9 +369,10 @@ fx_multigrid* fx_cn_multigrid(fx_cn* cn);
  * Schemes (TimeScheme, solver.rs:14-23): forward Euler, Heun (RK2, Shu–Osher form), classical RK4.
  * dt == NULL takes safety·min(dx, dy)²/(4α) with safety 0.4 / 0.8 / 1.4 (solver.rs:96-113).  Each step
  * gives the reference's values (solver.rs:344-494) while writing a second buffer instead of copying
- * u first (RK4: four fused stencil passes instead of 17 operations).  fx_diffusion_enable_graph (enable_hip_graph, solver.rs:173-184) makes step_n replay a
- * captured HIP graph of about 64 kernels (64 Euler, 32 Heun or 8 RK4 steps) on the backend's stream,
- * the remaining steps launched directly; the graph is recaptured when step_n is given a different u. */
+ * u first (RK4: four fused stencil passes instead of 17 operations).  fx_diffusion_enable_graph
+ * (enable_hip_graph, solver.rs:173-184) makes step_n replay a captured HIP graph of about 64 kernels
+ * (64 Euler, 32 Heun or 8 RK4 steps) on the backend's stream, the remaining steps launched directly;
+ * the graph is recaptured when step_n is given a different u. */
 typedef struct fx_diffusion fx_diffusion;
 typedef enum { FX_SCHEME_EULER = 0, FX_SCHEME_RK2 = 1, FX_SCHEME_RK4 = 2 } fx_time_scheme;
 /* DiffusionSolver::build(grid, alpha, dt, boundaries, scheme, device) (solver.rs:96-148);
