@@ -185,6 +185,15 @@ _SIGS = {
     "fx_cn_sim_time": (C.c_double, [C.c_void_p]),
     "fx_cn_solver": (C.c_void_p, [C.c_void_p]),
     "fx_cn_multigrid": (C.c_void_p, [C.c_void_p]),
+    "fx_bdf_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, C.c_uint32, _P(FxBoundaries),
+                                C.c_double, C.c_uint64, _P(FxMgOptions), _P(C.c_void_p)]),
+    "fx_bdf_destroy": (None, [C.c_void_p]),
+    "fx_bdf_step": (C.c_int, [C.c_void_p, c_buf_p, _P(FxSolveStats)]),
+    "fx_bdf_step_n": (C.c_int, [C.c_void_p, c_buf_p, C.c_uint64]),
+    "fx_bdf_dt": (C.c_double, [C.c_void_p]),
+    "fx_bdf_steps_done": (C.c_uint64, [C.c_void_p]),
+    "fx_bdf_sim_time": (C.c_double, [C.c_void_p]),
+    "fx_bdf_order": (C.c_uint32, [C.c_void_p]),
     "fx_diffusion_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, _P(C.c_double), _P(FxBoundaries),
                                       C.c_int32, _P(C.c_void_p)]),
     "fx_diffusion_destroy": (None, [C.c_void_p]),

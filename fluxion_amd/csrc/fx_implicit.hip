@@ -1,5 +1,5 @@
 // fx_implicit.hip — callers either side of the solve: the Crank–Nicolson
-// stepper (crates/runtime/src/implicit.rs:1-238), the explicit diffusion
+// stepper (crates/runtime/src/implicit.rs:1-238), BDF1–4 (bdf.rs:1-256), the explicit diffusion
 // stepper (crates/runtime/src/solver.rs:1-494) and the raw field format
 // (crates/io/src/raw.rs:1-77).
 //
@@ -55,7 +55,58 @@ struct fx_diffusion {
 	void* graph_u = nullptr;
 };
 
+// BdfSolver (bdf.rs:9-37): BDF1–4 with CG or MG-PCG per step.  The history is a
+// ring of buffers: the reference's shift (history[i] = history[i − 1], then
+// history[0] = u) is a rotation of the ring plus one copy of u.
+struct fx_bdf {
+	fx_backend* be = nullptr;
+	fx_grid g{};
+	double alpha = 0, dt = 0;
+	int order = 1;
+	uint64_t steps = 0;
+	fx_boundaries bc{};
+	fx_stencil_entry ent[5]{};
+	fx_stencil st{};
+	fx_solver* solver = nullptr;
+	fx_multigrid* mg = nullptr;
+	fx_buf* rhs = nullptr;
+	std::vector<fx_buf*> hist;   // hist[0] = u^n, hist[1] = u^{n−1}, …
+};
+
 namespace {
+
+// BDF coefficients (bdf.rs:41-56): history weights of u^n, u^{n−1}, … and the lhs weight.
+void bdf_coeffs(int order, double* h, double* lhs) {
+	switch (order) {
+	case 1: h[0] = -1.0; *lhs = 1.0; break;
+	case 2: h[0] = -2.0; h[1] = 0.5; *lhs = 1.5; break;
+	case 3: h[0] = -3.0; h[1] = 1.5; h[2] = -1.0 / 3.0; *lhs = 11.0 / 6.0; break;
+	default: h[0] = -4.0; h[1] = 3.0; h[2] = -4.0 / 3.0; h[3] = 0.25; *lhs = 25.0 / 12.0; break;
+	}
+}
+
+// rhs = (u·(−h0) − h1·u^{n−1} − h2·u^{n−2} − h3·u^{n−3})·(1/lhs): the reference's copy, scale, axpys and
+// scale (bdf.rs:189-206) per point, in their order, in one pass.
+__global__ void k_bdf_rhs(const double* __restrict__ u, const double* __restrict__ h1, const double* __restrict__ h2,
+                          const double* __restrict__ h3, int nh, double m0, double m1, double m2, double m3, double inv,
+                          double* __restrict__ rhs, int64_t n) {
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+		double t = u[i] * m0;
+		if (nh > 0) t = fma(m1, h1[i], t);
+		if (nh > 1) t = fma(m2, h2[i], t);
+		if (nh > 2) t = fma(m3, h3[i], t);
+		rhs[i] = t * inv;
+	}
+}
+
+void bdf_free(fx_bdf* b) {
+	if (!b) return;
+	if (b->solver) fx_solver_destroy(b->solver);
+	if (b->mg) fx_multigrid_destroy(b->mg);
+	if (b->rhs) fx_buffer_free(b->rhs);
+	for (fx_buf* h : b->hist) fx_buffer_free(h);
+	delete b;
+}
 
 // Stencil::laplacian_2d_5pt (core/src/stencil.rs:56-72), entry order kept.
 void laplacian_5pt(double dx, double dy, fx_stencil_entry* e) {
@@ -447,6 +498,85 @@ double fx_diffusion_dt(const fx_diffusion* d) { return d ? d->dt : 0.0; }
 uint64_t fx_diffusion_steps_done(const fx_diffusion* d) { return d ? d->steps : 0; }
 double fx_diffusion_sim_time(const fx_diffusion* d) { return d ? (double)d->steps * d->dt : 0.0; }
 int32_t fx_diffusion_scheme(const fx_diffusion* d) { return d ? d->scheme : -1; }
+
+fx_status fx_bdf_create(fx_backend* be, const fx_grid* grid, double alpha, double dt, uint32_t order,
+                        const fx_boundaries* bc, double cg_tol, uint64_t cg_max_iter, const fx_mg_options* mg,
+                        fx_bdf** out) {
+	FX_REQUIRE(be && bc && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(fx::check_grid(grid));
+	FX_CHECK(fx::check_bc(bc));
+	FX_REQUIRE(order >= 1 && order <= 4, FX_ERR_ARGUMENT, "BDF order must be 1-4, got %u", order);
+	FX_REQUIRE(std::isfinite(alpha) && std::isfinite(dt), FX_ERR_ARGUMENT, "alpha and dt must be finite");
+	fx_bdf* b = new fx_bdf;
+	b->be = be;
+	b->g = *grid;
+	b->alpha = alpha;
+	b->dt = dt;
+	b->order = (int)order;
+	b->bc = *bc;
+	laplacian_5pt(grid->dx, grid->dy, b->ent);
+	b->st = {5, b->ent};
+	const size_t n = (size_t)(grid->rows * grid->cols);
+	fx_status s = fx_allocate(be, n, FX_F64, &b->rhs);
+	for (uint32_t i = 0; i < order && s == FX_OK; ++i) {
+		fx_buf* h = nullptr;
+		s = fx_allocate(be, n, FX_F64, &h);
+		if (s == FX_OK) b->hist.push_back(h);
+	}
+	if (s == FX_OK) s = fx_solver_create(be, FX_SOLVER_CG, grid, 0, cg_tol, cg_max_iter, &b->solver);
+	if (s == FX_OK && mg) s = fx_multigrid_create_ex(be, grid, bc, mg, &b->mg);
+	if (s != FX_OK) {
+		bdf_free(b);
+		return s;
+	}
+	*out = b;
+	return FX_OK;
+}
+
+void fx_bdf_destroy(fx_bdf* b) { bdf_free(b); }
+
+fx_status fx_bdf_step(fx_bdf* b, fx_buf* u, fx_solve_stats* stats) {
+	FX_REQUIRE(b, FX_ERR_ARGUMENT, "solver is null");
+	const size_t n = (size_t)(b->g.rows * b->g.cols);
+	FX_CHECK(fx::check_buf(u, n, FX_F64, "u"));
+	// Effective order min(order, steps_done + 1) bootstraps the history (bdf.rs:176-179).
+	const int eff = (int)std::min<uint64_t>((uint64_t)b->order, b->steps + 1);
+	double h[4] = {0, 0, 0, 0}, lhs = 1.0;
+	bdf_coeffs(eff, h, &lhs);
+	const double* hp[3] = {nullptr, nullptr, nullptr};
+	for (int i = 1; i < eff; ++i) hp[i - 1] = (const double*)b->hist[i - 1]->ptr;
+	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_bdf_rhs, dim3(blocks), dim3(256), 0, b->be->stream, (const double*)u->ptr, hp[0], hp[1], hp[2],
+	                   eff - 1, -h[0], -h[1], -h[2], -h[3], 1.0 / lhs, (double*)b->rhs->ptr, (int64_t)n);
+	FX_CHECK(fx::check_launch("bdf rhs"));
+	// (I + coeff·Δ) u = rhs with coeff = −dt·α/lhs (bdf.rs:196-205), after the history shift (207-217).
+	const double coeff = -b->dt * b->alpha / lhs;
+	std::rotate(b->hist.rbegin(), b->hist.rbegin() + 1, b->hist.rend());
+	FX_CHECK(fx_copy(b->be, u, b->hist[0]));
+	fx_operator op{};
+	op.kind = FX_OP_STENCIL;
+	op.coeff = coeff;
+	op.sigma = 1.0;
+	op.stencil = &b->st;
+	op.bc = &b->bc;
+	fx_precond pc{FX_PC_MULTIGRID, 0, b->mg};
+	fx_solve_stats st{};
+	FX_CHECK(fx_solver_solve_op(b->solver, u, b->rhs, &op, b->mg ? &pc : nullptr, &st));
+	b->steps += 1;
+	if (stats) *stats = st;
+	return FX_OK;
+}
+
+fx_status fx_bdf_step_n(fx_bdf* b, fx_buf* u, uint64_t n) {
+	for (uint64_t i = 0; i < n; ++i) FX_CHECK(fx_bdf_step(b, u, nullptr));
+	return FX_OK;
+}
+
+double fx_bdf_dt(const fx_bdf* b) { return b ? b->dt : 0.0; }
+uint64_t fx_bdf_steps_done(const fx_bdf* b) { return b ? b->steps : 0; }
+double fx_bdf_sim_time(const fx_bdf* b) { return b ? (double)b->steps * b->dt : 0.0; }
+uint32_t fx_bdf_order(const fx_bdf* b) { return b ? (uint32_t)b->order : 0; }
 
 fx_status fx_write_raw_host(const char* path, const double* data, const fx_grid* grid) {
 	FX_CHECK(fx::check_grid(grid));

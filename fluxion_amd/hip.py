@@ -6,8 +6,8 @@ CgSolver, PipelinedCgSolver, BiCgStabSolver, GmresSolver
               ~ crates/runtime/src/{cg,pipelined_cg,bicgstab,gmres}.rs
 Multigrid, Identity
               ~ crates/runtime/src/{multigrid,preconditioner}.rs
-CrankNicolsonSolver
-              ~ crates/runtime/src/implicit.rs
+CrankNicolsonSolver, BdfSolver
+              ~ crates/runtime/src/{implicit,bdf}.rs
 DiffusionSolver
               ~ crates/runtime/src/solver.rs
 write_raw, read_raw
@@ -664,6 +664,64 @@ class CrankNicolsonSolver:
         ms = C.c_double()
         L.check(self._lib.fx_multigrid_autotune(m, reps, C.byref(ms)), "multigrid_autotune")
         return ms.value
+
+
+class BdfSolver:
+    """BdfSolver (crates/runtime/src/bdf.rs:1-256), stepped on the device by fx_bdf_*.
+
+    new(grid, alpha, dt, order, boundaries, cg_tol, cg_max_iter)                      bdf.rs:58-108
+    with_multigrid(..., mg_pre, mg_post, mg_omega)                                   bdf.rs:110-137
+    step(u) -> SolveStats, step_n(u, n), dt(), steps_done(), sim_time(), order().
+    """
+
+    def __init__(self, grid: Grid, alpha: float, dt: float, order: int, boundaries: Boundaries, cg_tol: float,
+                 cg_max_iter: int, mg: Optional[L.FxMgOptions] = None, be: Optional[HipBackend] = None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        self._mg = mg
+        g, b = L.grid_c(grid), L.bcs_c(boundaries)
+        h = C.c_void_p()
+        L.check(self._lib.fx_bdf_create(self.be.h, C.byref(g), alpha, dt, int(order), C.byref(b), cg_tol,
+                                        int(cg_max_iter), C.byref(mg) if mg is not None else None, C.byref(h)),
+                "bdf_create")
+        self.h = h
+
+    @staticmethod
+    def new(grid, alpha, dt, order, boundaries, cg_tol, cg_max_iter, be=None) -> "BdfSolver":
+        return BdfSolver(grid, alpha, dt, order, boundaries, cg_tol, cg_max_iter, be=be)
+
+    @staticmethod
+    def with_multigrid(grid, alpha, dt, order, boundaries, cg_tol, cg_max_iter, mg_pre, mg_post, mg_omega,
+                       be=None) -> "BdfSolver":
+        return BdfSolver(grid, alpha, dt, order, boundaries, cg_tol, cg_max_iter,
+                         mg=_mg_opts(mg_pre, mg_post, mg_omega), be=be)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and L.live():
+            self._lib.fx_bdf_destroy(h)
+            self.h = None
+
+    def step(self, u: Field) -> SolveStats:
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_bdf_step(self.h, u.h, C.byref(out)), "bdf_step")
+        return SolveStats(int(out.iterations), out.residual, ConvergenceReason(out.reason), out.true_residual)
+
+    def step_n(self, u: Field, n: int) -> None:
+        L.check(self._lib.fx_bdf_step_n(self.h, u.h, int(n)), "bdf_step_n")
+
+    def dt(self) -> float:
+        return float(self._lib.fx_bdf_dt(self.h))
+
+    def steps_done(self) -> int:
+        return int(self._lib.fx_bdf_steps_done(self.h))
+
+    def sim_time(self) -> float:
+        return float(self._lib.fx_bdf_sim_time(self.h))
+
+    def order(self) -> int:
+        return int(self._lib.fx_bdf_order(self.h))
 
 
 # ── explicit time stepping (fx_diffusion_*) ───────────────────────────

@@ -364,6 +364,23 @@ double fx_cn_sim_time(const fx_cn* cn);          /* implicit.rs:168-172: steps_d
 fx_solver* fx_cn_solver(fx_cn* cn);
 fx_multigrid* fx_cn_multigrid(fx_cn* cn);
 
+/* ── BDF stepper (BdfSolver, runtime/src/bdf.rs:1-256) ─────────────────────────────────────
+ * ∂u/∂t = α·Δu by BDF of order 1–4, the first steps at order steps_done + 1 (bdf.rs:163-240):
+ * (lhs·I − dt·α·Δ) u^{n+1} = −Σ a_i·u^{n+1−i}, solved as (I + coeff·Δ) u = rhs/lhs with coeff = −dt·α/lhs
+ * by CG from u^n, or by PCG with the multigrid of `mg` (Multigrid::new options for with_multigrid,
+ * bdf.rs:110-137; NULL: BdfSolver::new, bdf.rs:58-108). */
+typedef struct fx_bdf fx_bdf;
+fx_status fx_bdf_create(fx_backend* be, const fx_grid* grid, double alpha, double dt, uint32_t order,
+                        const fx_boundaries* bc, double cg_tol, uint64_t cg_max_iter, const fx_mg_options* mg,
+                        fx_bdf** out);
+void fx_bdf_destroy(fx_bdf* b);
+fx_status fx_bdf_step(fx_bdf* b, fx_buf* u, fx_solve_stats* stats);                  /* bdf.rs:163-240 */
+fx_status fx_bdf_step_n(fx_bdf* b, fx_buf* u, uint64_t n);                            /* bdf.rs:242-255 */
+double fx_bdf_dt(const fx_bdf* b);
+uint64_t fx_bdf_steps_done(const fx_bdf* b);
+double fx_bdf_sim_time(const fx_bdf* b);
+uint32_t fx_bdf_order(const fx_bdf* b);
+
 /* ── explicit diffusion stepper (DiffusionSolver, runtime/src/solver.rs:1-494) ─────────────
  * ∂u/∂t = α·Δu + source with Δ = Stencil::laplacian_2d_5pt(dx, dy) and the given boundary conditions.
  * Schemes (TimeScheme, solver.rs:14-23): forward Euler, Heun (RK2, Shu–Osher form), classical RK4.

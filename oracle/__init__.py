@@ -385,6 +385,33 @@ def cn_step(u, grid, alpha, dt, bc=None, source=None, tol=1e-8, max_iter=200, mg
     return pcg(u, rhs, grid, system, tol, max_iter, mg=mg)
 
 
+# ── BdfSolver::step (runtime/src/bdf.rs:163-240) ───────────────────────
+BDF_COEFFS = {1: ([-1.0], 1.0), 2: ([-2.0, 0.5], 1.5), 3: ([-3.0, 1.5, -1.0 / 3.0], 11.0 / 6.0),
+              4: ([-4.0, 3.0, -4.0 / 3.0, 0.25], 25.0 / 12.0)}   # bdf.rs:41-56
+
+
+def bdf_step(u, history, steps_done, order, grid, alpha, dt, bc=None, tol=1e-8, max_iter=200, mg: OMg = None):
+    """One BDF step of the oracle at order min(order, steps_done + 1): rhs = copy(u)·(−a0), rhs −= a_i·h_{i−1},
+    rhs ·= 1/lhs, history shifted (history[0] = u^n), then CG / PCG on u + coeff·S(u), coeff = −dt·α/lhs,
+    from u.  Returns (u_new, history_new, stats); history is a list of arrays, newest first."""
+    bc = bc or Boundaries.zero_dirichlet()
+    st = Stencil.laplacian_2d_5pt(grid.dx, grid.dy)
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    eff = min(order, steps_done + 1)
+    a, lhs = BDF_COEFFS[eff]
+    rhs = scale(u.copy(), -a[0])
+    for i in range(1, eff):
+        rhs = axpy(-a[i], history[i - 1], rhs)
+    coeff = -dt * alpha / lhs
+    rhs = scale(rhs, 1.0 / lhs)
+    history = [u.copy()] + list(history[:order - 1])
+    if mg is None:
+        x, s = cg(u, rhs, grid, Op.stencil(coeff, st, bc), tol, max_iter)
+    else:
+        x, s = pcg(u, rhs, grid, Op.stencil(coeff, st, bc), tol, max_iter, mg=mg)
+    return x, history, s
+
+
 # ── DiffusionSolver (runtime/src/solver.rs:96-494) ─────────────────────
 def diffusion_dt(grid, alpha, scheme):
     """Default step safety·min(dx, dy)²/(4α), safety 0.4 / 0.8 / 1.4 for Euler / RK2 / RK4 (solver.rs:103-113)."""

@@ -371,3 +371,48 @@ def test_diffusion_reference_behaviour(be):
         u = hip.Field.from_f64(g, np.ones(n * n), be)
         hip.DiffusionSolver.new(g, alpha, be=be).step_n(u, 100 if alpha < 1 else 50)
         assert np.isfinite(u.norm_l2())
+
+
+# ── BdfSolver (runtime/src/bdf.rs) ────────────────────────────────────
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("with_mg", [False, True])
+def test_bdf_steps_parity(be, order, with_mg):
+    # Five steps (the first ones bootstrap the order) against the oracle's, step by step from the same state.
+    hip = H()
+    n = 64
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    u0 = gaussian_ic(n, h)
+    alpha, dt = 1.0, 0.01
+    if with_mg:
+        b = hip.BdfSolver.with_multigrid(g, alpha, dt, order, ZD, 1e-9, 200, 2, 2, 2.0 / 3.0, be=be)
+        cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI)
+    else:
+        b = hip.BdfSolver.new(g, alpha, dt, order, ZD, 1e-9, 1000, be=be)
+        cfg = None
+    u = hip.Field.from_f64(g, u0, be)
+    ur, hist = u0, [np.zeros(len(g))] * order
+    for i in range(5):
+        s = b.step(u)
+        ur, hist, sr = O.bdf_step(ur, hist, i, order, g, alpha, dt, tol=1e-9, max_iter=200 if with_mg else 1000,
+                                  mg=cfg)
+        check(s, sr, u.to_vec_f64(), ur)
+        ur = u.to_vec_f64()   # next step from the same state (per-step parity); hist[0] is this step's input
+    assert b.steps_done() == 5 and b.order() == order and abs(b.sim_time() - 0.05) < 1e-15
+
+
+def test_bdf_reference_behaviour(be):
+    # solvers.rs:1800-1900: BDF1 and BDF2 lower the peak over 50 steps; BDF4 + multigrid converges.
+    hip = H()
+    n = 32
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    data = gaussian_ic(n, h)
+    for order in (1, 2):
+        b = hip.BdfSolver.new(g, 0.01, 0.01, order, ZD, 1e-8, 200, be=be)
+        u = hip.Field.from_f64(g, data, be)
+        b.step_n(u, 50)
+        assert np.isfinite(u.max()) and u.max() < data.max()
+        assert b.steps_done() == 50
+    b = hip.BdfSolver.with_multigrid(g, 0.01, 0.01, 4, ZD, 1e-8, 200, 2, 2, 2.0 / 3.0, be=be)
+    assert b.step(hip.Field.from_f64(g, data, be)).converged()

@@ -137,6 +137,31 @@ def test_cn_step_multigrid_and_flexible_agree_with_cg():
     assert rel_l2(u_b, u_a) < 1e-10
 
 
+# ── BdfSolver steps (bdf.rs:163-240) ──────────────────────────────────
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_bdf_steps_eigenmodes_closed_form(order):
+    # Per eigenmode m (λ, z = dt·α·λ) the amplitudes obey the scalar BDF recurrence
+    # Σ_j a_j·y_{n+1−j} = z·y_{n+1} (lhs weight a_0), at order min(order, n + 1) while the history fills.
+    g = rect_grid()
+    alpha, dt = 0.8, 2e-3
+    modes = [(mode(g, 1, 2), 1.0, eig(g, 1, 2)), (mode(g, 4, 3), 0.5, eig(g, 4, 3))]
+    u = sum(a * m for m, a, _ in modes)
+    hist = [np.zeros(len(g))] * order
+    amps = [[a] for _, a, _ in modes]
+    coef = {1: [1.0, -1.0], 2: [1.5, -2.0, 0.5], 3: [11 / 6, -3.0, 1.5, -1 / 3], 4: [25 / 12, -4.0, 3.0, -4 / 3, 0.25]}
+    for n in range(6):
+        u, hist, st = O.bdf_step(u, hist, n, order, g, alpha, dt, tol=1e-13, max_iter=200)
+        assert st.converged()
+        eff = min(order, n + 1)
+        c = coef[eff]
+        for k, (_, _, lam) in enumerate(modes):
+            y = amps[k]
+            rhs = -sum(c[j] * y[-j] for j in range(1, eff + 1))
+            y.append(rhs / (c[0] - dt * alpha * lam))
+        want = sum(amps[k][-1] * modes[k][0] for k in range(len(modes)))
+        assert rel_l2(u, want) < 1e-11, (order, n)
+
+
 # ── explicit DiffusionSolver steps (solver.rs:344-494) ─────────────────
 @pytest.mark.parametrize("scheme", [0, 1, 2])
 def test_diffusion_step_eigenmodes_closed_form(scheme):
