@@ -185,6 +185,14 @@ _SIGS = {
     "fx_cn_sim_time": (C.c_double, [C.c_void_p]),
     "fx_cn_solver": (C.c_void_p, [C.c_void_p]),
     "fx_cn_multigrid": (C.c_void_p, [C.c_void_p]),
+    "fx_imex_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, C.c_double, C.c_double,
+                                 _P(FxBoundaries), C.c_double, C.c_uint64, _P(FxMgOptions), _P(C.c_void_p)]),
+    "fx_imex_destroy": (None, [C.c_void_p]),
+    "fx_imex_step": (C.c_int, [C.c_void_p, c_buf_p, _P(FxSolveStats)]),
+    "fx_imex_step_n": (C.c_int, [C.c_void_p, c_buf_p, C.c_uint64]),
+    "fx_imex_dt": (C.c_double, [C.c_void_p]),
+    "fx_imex_steps_done": (C.c_uint64, [C.c_void_p]),
+    "fx_imex_sim_time": (C.c_double, [C.c_void_p]),
     "fx_bdf_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, C.c_uint32, _P(FxBoundaries),
                                 C.c_double, C.c_uint64, _P(FxMgOptions), _P(C.c_void_p)]),
     "fx_bdf_destroy": (None, [C.c_void_p]),

@@ -1,5 +1,5 @@
 // fx_implicit.hip — callers either side of the solve: the Crank–Nicolson
-// stepper (crates/runtime/src/implicit.rs:1-238), BDF1–4 (bdf.rs:1-256), the explicit diffusion
+// stepper (crates/runtime/src/implicit.rs:1-238), IMEX (imex.rs:1-274), BDF1–4 (bdf.rs:1-256), the explicit diffusion
 // stepper (crates/runtime/src/solver.rs:1-494) and the raw field format
 // (crates/io/src/raw.rs:1-77).
 //
@@ -53,6 +53,18 @@ struct fx_diffusion {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	void* graph_u = nullptr;
+};
+
+// ImexSolver (imex.rs:10-45): a Crank–Nicolson step in κ whose source is the
+// explicit upwind convection v·∇u of the current state (apply_conv_diff with a
+// zero κ field: imex.rs:215-274), i.e. the reference's step sequence
+// (imex.rs:147-197) as conv_diff, then fx_cn_step(u, conv).
+struct fx_imex {
+	fx_backend* be = nullptr;
+	fx_grid g{};
+	fx_boundaries bc{};
+	fx_cn* cn = nullptr;
+	fx_buf *conv = nullptr, *kzero = nullptr, *vx = nullptr, *vy = nullptr;
 };
 
 // BdfSolver (bdf.rs:9-37): BDF1–4 with CG or MG-PCG per step.  The history is a
@@ -498,6 +510,60 @@ double fx_diffusion_dt(const fx_diffusion* d) { return d ? d->dt : 0.0; }
 uint64_t fx_diffusion_steps_done(const fx_diffusion* d) { return d ? d->steps : 0; }
 double fx_diffusion_sim_time(const fx_diffusion* d) { return d ? (double)d->steps * d->dt : 0.0; }
 int32_t fx_diffusion_scheme(const fx_diffusion* d) { return d ? d->scheme : -1; }
+
+fx_status fx_imex_create(fx_backend* be, const fx_grid* grid, double kappa, double vx, double vy, double dt,
+                         const fx_boundaries* bc, double cg_tol, uint64_t cg_max_iter, const fx_mg_options* mg,
+                         fx_imex** out) {
+	FX_REQUIRE(be && bc && out, FX_ERR_ARGUMENT, "null argument");
+	*out = nullptr;
+	FX_CHECK(fx::check_grid(grid));
+	FX_REQUIRE(std::isfinite(vx) && std::isfinite(vy), FX_ERR_ARGUMENT, "velocity must be finite");
+	fx_imex* m = new fx_imex;
+	m->be = be;
+	m->g = *grid;
+	m->bc = *bc;
+	auto done = [&](fx_status s) {
+		if (s == FX_OK) {
+			*out = m;
+			return s;
+		}
+		fx_imex_destroy(m);
+		return s;
+	};
+	fx_cn_options o{FX_OP_STENCIL, FX_SOLVER_CG, nullptr, mg};
+	fx_status s = fx_cn_create(be, grid, kappa, dt, bc, cg_tol, cg_max_iter, &o, &m->cn);
+	const size_t n = (size_t)(grid->rows * grid->cols);
+	if (s == FX_OK) s = fx_allocate(be, n, FX_F64, &m->conv);
+	if (s == FX_OK) s = fx_allocate(be, n, FX_F64, &m->kzero);
+	if (s == FX_OK) s = fx_allocate(be, n, FX_F64, &m->vx);
+	if (s == FX_OK) s = fx_allocate(be, n, FX_F64, &m->vy);
+	if (s == FX_OK) s = fx_fill(be, m->vx, vx);
+	if (s == FX_OK) s = fx_fill(be, m->vy, vy);
+	return done(s);
+}
+
+void fx_imex_destroy(fx_imex* m) {
+	if (!m) return;
+	if (m->cn) fx_cn_destroy(m->cn);
+	for (fx_buf* b : {m->conv, m->kzero, m->vx, m->vy})
+		if (b) fx_buffer_free(b);
+	delete m;
+}
+
+fx_status fx_imex_step(fx_imex* m, fx_buf* u, fx_solve_stats* stats) {
+	FX_REQUIRE(m, FX_ERR_ARGUMENT, "solver is null");
+	FX_CHECK(fx_apply_conv_diff(m->be, u, m->conv, m->kzero, m->vx, m->vy, &m->g, &m->bc));
+	return fx_cn_step(m->cn, u, m->conv, stats);
+}
+
+fx_status fx_imex_step_n(fx_imex* m, fx_buf* u, uint64_t n) {
+	for (uint64_t i = 0; i < n; ++i) FX_CHECK(fx_imex_step(m, u, nullptr));
+	return FX_OK;
+}
+
+double fx_imex_dt(const fx_imex* m) { return m ? fx_cn_dt(m->cn) : 0.0; }
+uint64_t fx_imex_steps_done(const fx_imex* m) { return m ? fx_cn_steps_done(m->cn) : 0; }
+double fx_imex_sim_time(const fx_imex* m) { return m ? fx_cn_sim_time(m->cn) : 0.0; }
 
 fx_status fx_bdf_create(fx_backend* be, const fx_grid* grid, double alpha, double dt, uint32_t order,
                         const fx_boundaries* bc, double cg_tol, uint64_t cg_max_iter, const fx_mg_options* mg,

@@ -6,8 +6,8 @@ CgSolver, PipelinedCgSolver, BiCgStabSolver, GmresSolver
               ~ crates/runtime/src/{cg,pipelined_cg,bicgstab,gmres}.rs
 Multigrid, Identity
               ~ crates/runtime/src/{multigrid,preconditioner}.rs
-CrankNicolsonSolver, BdfSolver
-              ~ crates/runtime/src/{implicit,bdf}.rs
+CrankNicolsonSolver, ImexSolver, BdfSolver
+              ~ crates/runtime/src/{implicit,imex,bdf}.rs
 DiffusionSolver
               ~ crates/runtime/src/solver.rs
 write_raw, read_raw
@@ -664,6 +664,60 @@ class CrankNicolsonSolver:
         ms = C.c_double()
         L.check(self._lib.fx_multigrid_autotune(m, reps, C.byref(ms)), "multigrid_autotune")
         return ms.value
+
+
+class ImexSolver:
+    """ImexSolver (crates/runtime/src/imex.rs:1-274), stepped on the device by fx_imex_*.
+
+    new(grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter)                       imex.rs:46-93
+    with_multigrid(..., mg_pre, mg_post, mg_omega)                                      imex.rs:95-137
+    step(u) -> SolveStats, step_n(u, n), dt(), steps_done(), sim_time().
+    """
+
+    def __init__(self, grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter, mg=None, be=None):
+        self.be = _backend(be)
+        self._lib = self.be._lib
+        self.grid = grid
+        self._mg = mg
+        g, b = L.grid_c(grid), L.bcs_c(boundaries)
+        h = C.c_void_p()
+        L.check(self._lib.fx_imex_create(self.be.h, C.byref(g), kappa, vx, vy, dt, C.byref(b), cg_tol,
+                                         int(cg_max_iter), C.byref(mg) if mg is not None else None, C.byref(h)),
+                "imex_create")
+        self.h = h
+
+    @staticmethod
+    def new(grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter, be=None) -> "ImexSolver":
+        return ImexSolver(grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter, be=be)
+
+    @staticmethod
+    def with_multigrid(grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter, mg_pre, mg_post, mg_omega,
+                       be=None) -> "ImexSolver":
+        return ImexSolver(grid, kappa, vx, vy, dt, boundaries, cg_tol, cg_max_iter,
+                          mg=_mg_opts(mg_pre, mg_post, mg_omega), be=be)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and L.live():
+            self._lib.fx_imex_destroy(h)
+            self.h = None
+
+    def step(self, u: Field) -> SolveStats:
+        out = L.FxSolveStats()
+        L.check(self._lib.fx_imex_step(self.h, u.h, C.byref(out)), "imex_step")
+        return SolveStats(int(out.iterations), out.residual, ConvergenceReason(out.reason), out.true_residual)
+
+    def step_n(self, u: Field, n: int) -> None:
+        L.check(self._lib.fx_imex_step_n(self.h, u.h, int(n)), "imex_step_n")
+
+    def dt(self) -> float:
+        return float(self._lib.fx_imex_dt(self.h))
+
+    def steps_done(self) -> int:
+        return int(self._lib.fx_imex_steps_done(self.h))
+
+    def sim_time(self) -> float:
+        return float(self._lib.fx_imex_sim_time(self.h))
 
 
 class BdfSolver:

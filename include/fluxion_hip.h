@@ -364,6 +364,21 @@ double fx_cn_sim_time(const fx_cn* cn);          /* implicit.rs:168-172: steps_d
 fx_solver* fx_cn_solver(fx_cn* cn);
 fx_multigrid* fx_cn_multigrid(fx_cn* cn);
 
+/* ── IMEX convection–diffusion stepper (ImexSolver, runtime/src/imex.rs:1-274) ──────────────
+ * Uniform velocity (vx, vy) and diffusivity κ: conv = v·∇u (upwind, apply_conv_diff with κ = 0), then
+ * (I − (dt/2)·κ·Δ) u^{n+1} = u^n + (dt/2)·κ·Δu^n + dt·conv by CG, or PCG with the multigrid of `mg`
+ * (ImexSolver::new / with_multigrid, imex.rs:46-137; step imex.rs:147-197). */
+typedef struct fx_imex fx_imex;
+fx_status fx_imex_create(fx_backend* be, const fx_grid* grid, double kappa, double vx, double vy, double dt,
+                         const fx_boundaries* bc, double cg_tol, uint64_t cg_max_iter, const fx_mg_options* mg,
+                         fx_imex** out);
+void fx_imex_destroy(fx_imex* m);
+fx_status fx_imex_step(fx_imex* m, fx_buf* u, fx_solve_stats* stats);
+fx_status fx_imex_step_n(fx_imex* m, fx_buf* u, uint64_t n);
+double fx_imex_dt(const fx_imex* m);
+uint64_t fx_imex_steps_done(const fx_imex* m);
+double fx_imex_sim_time(const fx_imex* m);
+
 /* ── BDF stepper (BdfSolver, runtime/src/bdf.rs:1-256) ─────────────────────────────────────
  * ∂u/∂t = α·Δu by BDF of order 1–4, the first steps at order steps_done + 1 (bdf.rs:163-240):
  * (lhs·I − dt·α·Δ) u^{n+1} = −Σ a_i·u^{n+1−i}, solved as (I + coeff·Δ) u = rhs/lhs with coeff = −dt·α/lhs

@@ -385,6 +385,16 @@ def cn_step(u, grid, alpha, dt, bc=None, source=None, tol=1e-8, max_iter=200, mg
     return pcg(u, rhs, grid, system, tol, max_iter, mg=mg)
 
 
+# ── ImexSolver::step (runtime/src/imex.rs:147-197) ─────────────────────
+def imex_step(u, grid, kappa, vx, vy, dt, bc=None, tol=1e-8, max_iter=200, mg: OMg = None):
+    """conv = v·∇u by apply_conv_diff with a zero κ field and uniform velocity fields (imex.rs:215-274), then the
+    Crank–Nicolson step in κ with conv as its source (rhs = u + h·S(u) + dt·conv, system u − h·S(u))."""
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    n = u.size
+    conv = conv_diff(0, 0.0, u, np.zeros(n), np.full(n, float(vx)), np.full(n, float(vy)), grid)
+    return cn_step(u, grid, kappa, dt, bc=bc, source=conv, tol=tol, max_iter=max_iter, mg=mg)
+
+
 # ── BdfSolver::step (runtime/src/bdf.rs:163-240) ───────────────────────
 BDF_COEFFS = {1: ([-1.0], 1.0), 2: ([-2.0, 0.5], 1.5), 3: ([-3.0, 1.5, -1.0 / 3.0], 11.0 / 6.0),
               4: ([-4.0, 3.0, -4.0 / 3.0, 0.25], 25.0 / 12.0)}   # bdf.rs:41-56

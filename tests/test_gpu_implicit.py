@@ -416,3 +416,30 @@ def test_bdf_reference_behaviour(be):
         assert b.steps_done() == 50
     b = hip.BdfSolver.with_multigrid(g, 0.01, 0.01, 4, ZD, 1e-8, 200, 2, 2, 2.0 / 3.0, be=be)
     assert b.step(hip.Field.from_f64(g, data, be)).converged()
+
+
+# ── ImexSolver (runtime/src/imex.rs) ──────────────────────────────────
+@pytest.mark.parametrize("with_mg", [False, True])
+def test_imex_steps_parity(be, with_mg):
+    hip = H()
+    n = 64
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    u0 = gaussian_ic(n, h)
+    kappa, vx, vy, dt = 0.05, 0.8, -0.3, 2e-3
+    if with_mg:
+        m = hip.ImexSolver.with_multigrid(g, kappa, vx, vy, dt, ZD, 1e-9, 200, 2, 2, 2.0 / 3.0, be=be)
+        cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI)
+    else:
+        m = hip.ImexSolver.new(g, kappa, vx, vy, dt, ZD, 1e-9, 1000, be=be)
+        cfg = None
+    u = hip.Field.from_f64(g, u0, be)
+    ur = u0
+    for _ in range(4):
+        s = m.step(u)
+        ur, sr = O.imex_step(ur, g, kappa, vx, vy, dt, tol=1e-9, max_iter=200 if with_mg else 1000, mg=cfg)
+        check(s, sr, u.to_vec_f64(), ur)
+        ur = u.to_vec_f64()
+    m.step_n(u, 3)
+    assert m.steps_done() == 7 and abs(m.sim_time() - 7 * dt) < 1e-15 and m.dt() == dt
+    assert np.isfinite(u.max())

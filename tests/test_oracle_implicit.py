@@ -137,6 +137,25 @@ def test_cn_step_multigrid_and_flexible_agree_with_cg():
     assert rel_l2(u_b, u_a) < 1e-10
 
 
+# ── ImexSolver steps (imex.rs:147-197) ────────────────────────────────
+def test_imex_step_limits():
+    # v = 0: the Crank–Nicolson step in κ, bit for bit.  κ = 0: the system is the identity, so the step is
+    # u + dt·(v·∇⁻u) with the upwind convection of apply_conv_diff.
+    g = rect_grid()
+    rng = np.random.default_rng(8)
+    u0 = rng.standard_normal(len(g)).reshape(g.rows, g.cols)
+    u0[[0, -1], :] = 0.0
+    u0[:, [0, -1]] = 0.0
+    u0 = u0.ravel()
+    a, sa = O.imex_step(u0, g, 0.3, 0.0, 0.0, 1e-3, tol=1e-12, max_iter=500)
+    b, sb = O.cn_step(u0, g, 0.3, 1e-3, tol=1e-12, max_iter=500)
+    assert np.array_equal(a, b) and sa.iterations == sb.iterations
+    n = len(g)
+    c, sc = O.imex_step(u0, g, 0.0, 0.7, -0.4, 1e-3, tol=1e-12, max_iter=50)
+    conv = O.conv_diff(0, 0.0, u0, np.zeros(n), np.full(n, 0.7), np.full(n, -0.4), g)
+    assert sc.converged() and rel_l2(c, u0 + 1e-3 * conv) < 1e-14
+
+
 # ── BdfSolver steps (bdf.rs:163-240) ──────────────────────────────────
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_bdf_steps_eigenmodes_closed_form(order):
