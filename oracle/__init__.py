@@ -491,8 +491,9 @@ def unet_layout(cin=2):
     return list(woff), list(boff), list(lcin), list(lcout)
 
 
-def unet_forward(params, x):
-    """x: (B, H, W, CIN) float64 -> (B, H, W)."""
+def unet_forward(params, x, bf16=False):
+    """x: (B, H, W, CIN) float64 -> (B, H, W).  bf16=True rounds input, weights, activations and
+    pooled maps to BF16 (biases to FP32) as the GPU stores them; sums stay FP64."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     b, h, w, cin = x.shape
     if h % 8 or w % 8:
@@ -500,13 +501,15 @@ def unet_forward(params, x):
     y = np.zeros((b, h, w))
     p = np.ascontiguousarray(params, dtype=np.float64)
     lib().orc_unet_forward(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(y)), C.c_int(b), C.c_int(h),
-                           C.c_int(w), C.c_int(cin))
+                           C.c_int(w), C.c_int(cin), C.c_int(int(bool(bf16))))
     return y
 
 
-def unet_loss_grad(params, x, t, energy=False):
+def unet_loss_grad(params, x, t, energy=False, bf16=False):
     """Relative squared error (or, with energy=True, relative energy norm under the unit-spacing
-    −∇·(k∇·) with k = exp(x[..., 1])) summed over the batch, and its gradient w.r.t. the parameters."""
+    −∇·(k∇·) with k = exp(x[..., 1])) summed over the batch, and its gradient w.r.t. the parameters.
+    bf16=True emulates the GPU's BF16 storage (see unet_forward) and rounds the ReLU-masked
+    gradient to BF16 where it multiplies weights and activations."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     t = np.ascontiguousarray(t, dtype=np.float64)
     b, h, w, cin = x.shape
@@ -518,5 +521,5 @@ def unet_loss_grad(params, x, t, energy=False):
     fn = L.orc_unet_loss_grad_energy if energy else L.orc_unet_loss_grad
     fn.restype = C.c_double
     loss = fn(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(t)), C.c_void_p(_p(g)),
-              C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin))
+              C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin), C.c_int(int(bool(bf16))))
     return loss, g

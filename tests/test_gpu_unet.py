@@ -1,4 +1,8 @@
-"""The BF16/MFMA U-Net kernels against the FP64 oracle network (GPU)."""
+"""The BF16/MFMA U-Net kernels against the oracle network (GPU).
+
+The oracle runs in its BF16 storage mode (input, weights, activations and pooled maps rounded to BF16
+as the GPU stores them, FP64 sums), so the remaining difference is the GPU's FP32 accumulation order
+and the rare BF16 rounding it flips: the forward is held to 1e-2 relative L2 and the gradient to 1e-2."""
 import numpy as np
 import pytest
 
@@ -35,22 +39,26 @@ def test_unet_forward_and_gradient(be):
     net = UNet(be, cin)
     net.set_params(p)
     y = net.forward(x)
-    yr = O.unet_forward(p, x)
+    yr = O.unet_forward(p, x, bf16=True)
     rel = np.linalg.norm(y - yr) / np.linalg.norm(yr)
-    print(f"forward rel diff {rel:.3e}")
-    assert rel < 0.05
+    rel64 = np.linalg.norm(y - O.unet_forward(p, x)) / np.linalg.norm(yr)
+    print(f"forward rel diff {rel:.3e} against the BF16-storage oracle ({rel64:.3e} against FP64 storage)")
+    assert rel < 1e-2
     loss, g = net.loss_grad(x, t)
-    lr, gr = O.unet_loss_grad(p, x, t)
+    lr, gr = O.unet_loss_grad(p, x, t, bf16=True)
     relg = np.linalg.norm(g - gr) / np.linalg.norm(gr)
     cos = np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr))
-    print(f"loss {loss:.6f} vs {lr:.6f}; grad rel diff {relg:.3e}, cosine {cos:.6f}")
-    assert abs(loss - lr) < 0.05 * lr
-    assert cos > 0.995 and relg < 0.1
+    print(f"loss {loss:.6f} vs {lr:.6f}; grad rel diff {relg:.3e}, cosine {cos:.8f}")
+    assert abs(loss - lr) < 1e-2 * lr
+    assert cos > 0.9999 and relg < 1e-2
     woff, boff, lcin, lcout = O.unet_layout(cin)
-    for i in range(15):   # every layer's weight gradient, not only the large ones
-        sl = slice(woff[i], woff[i] + 9 * lcin[i] * lcout[i])
-        c = np.dot(g[sl], gr[sl]) / (np.linalg.norm(g[sl]) * np.linalg.norm(gr[sl]) + 1e-300)
-        assert c > 0.98, (i, c)
+    worst = 1.0
+    for i in range(15):   # every layer's weight and bias gradient, not only the large ones
+        for sl in (slice(woff[i], woff[i] + 9 * lcin[i] * lcout[i]), slice(boff[i], boff[i] + lcout[i])):
+            d = np.linalg.norm(g[sl] - gr[sl]) / (np.linalg.norm(gr[sl]) + 1e-300)
+            worst = min(worst, 1.0 - d)
+            assert d < 3e-2, (i, sl, d)
+    print(f"worst per-layer gradient rel diff {1.0 - worst:.3e}")
 
 
 def test_unet_adam_decreases_loss(be):
@@ -72,7 +80,7 @@ def test_unet_adam_decreases_loss(be):
 
 def test_unet_energy_gradient(be):
     # The energy-norm loss and its gradient on the GPU (FP32 loss kernel, BF16 network) against the
-    # FP64 oracle.
+    # oracle in BF16 storage mode.
     from fluxion_amd.learned import UNet
     cin, B, H, W = 2, 2, 32, 40
     p = bf16_round(init_params(cin, 7))
@@ -82,8 +90,11 @@ def test_unet_energy_gradient(be):
     net = UNet(be, cin)
     net.set_params(p)
     loss, g = net.loss_grad(x, t, energy=True)
-    lr, gr = O.unet_loss_grad(p, x, t, energy=True)
+    lr, gr = O.unet_loss_grad(p, x, t, energy=True, bf16=True)
     cos = np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr))
-    print(f"energy loss {loss:.6f} vs {lr:.6f}; grad cosine {cos:.6f}")
-    assert abs(loss - lr) < 0.05 * lr
-    assert cos > 0.995
+    relg = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"energy loss {loss:.6f} vs {lr:.6f}; grad rel diff {relg:.3e}, cosine {cos:.8f}")
+    assert abs(loss - lr) < 1e-2 * lr
+    # dLoss/dy = 2·A(y − t)/tᵀAt: the 5-point operator A amplifies the pointwise (rough) part of the
+    # forward's few-1e-3 difference, so this gradient is held to 5e-2 instead of 1e-2.
+    assert cos > 0.999 and relg < 5e-2

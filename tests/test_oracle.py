@@ -126,7 +126,7 @@ def test_blas_fma_sites():
     exact = a * x[0] + y[0]  # not fused in Python: 2 roundings
     fused = float(np.longdouble(a) * np.longdouble(x[0]) + np.longdouble(y[0]))
     assert O.axpy(a, x, y)[0] == fused
-    assert fused != exact or True
+    assert fused != exact   # the inputs are chosen so that one rounding and two differ
     assert O.aypx(a, y, x)[0] == float(np.longdouble(a) * np.longdouble(x[0]) + np.longdouble(y[0]))
 
 
@@ -364,6 +364,33 @@ def test_pcg_multigrid_fewer_iterations():
     for sm in (O.SMOOTH_REF_GS, O.SMOOTH_JACOBI):
         _, s2 = O.pcg(u, b, g, op, 1e-8, 100, mg=O.mg_cfg(2, 2, 2.0 / 3.0, smoother=sm))
         assert s2.converged() and s2.iterations <= s1.iterations
+
+
+def cheb_T(k, z):
+    return np.polynomial.chebyshev.chebval(z, [0.0] * k + [1.0])
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 7])
+def test_chebyshev_mode_factor_closed_form(sweeps):
+    # multigrid.rs:317-473: with zero rhs, each discrete sine mode u_pq is an eigenvector of D⁻¹Δ_h
+    # (D = −(2/dx² + 2/dy²)) with eigenvalue λ = (4/dy²·sin²(pπ/2(R−1)) + 4/dx²·sin²(qπ/2(C−1)))/(2/dx² + 2/dy²),
+    # and n sweeps of the first-kind Chebyshev smoother with bounds [emin, emax] = [0.2, 2] scale it by
+    # T_n((emax + emin − 2λ)/(emax − emin)) / T_n((emax + emin)/(emax − emin)).  A grid no larger than
+    # the coarsest level is a one-level hierarchy whose V-cycle is n_pre + n_post + extra_coarse sweeps.
+    R, Cc = 13, 9
+    dx, dy = 1.0 / (Cc - 1), 0.7 / (R - 1)
+    g = Grid(R, Cc, dx, dy)
+    cfg = O.mg_cfg(0, 0, 2.0 / 3.0, smoother=O.SMOOTH_CHEBYSHEV, coarsest=64, extra_coarse=sweeps)
+    assert O.mg_depth(g, cfg) == 1
+    emax, emin = 2.0, 0.2
+    rr, cc = np.meshgrid(np.arange(R), np.arange(Cc), indexing="ij")
+    for p_, q_ in ((1, 1), (3, 2), (R - 2, Cc - 2), (6, 1), (2, 7)):
+        u = (np.sin(p_ * np.pi * rr / (R - 1)) * np.sin(q_ * np.pi * cc / (Cc - 1))).ravel()
+        lam = (4 / dy ** 2 * np.sin(p_ * np.pi / (2 * (R - 1))) ** 2 + 4 / dx ** 2 * np.sin(q_ * np.pi / (2 * (Cc - 1))) ** 2) \
+            / (2 / dx ** 2 + 2 / dy ** 2)
+        factor = cheb_T(sweeps, (emax + emin - 2 * lam) / (emax - emin)) / cheb_T(sweeps, (emax + emin) / (emax - emin))
+        out = O.vcycle(u, np.zeros_like(u), g, cfg)
+        assert np.max(np.abs(out - factor * u)) < 1e-13, (p_, q_, sweeps, factor)
 
 
 def test_chebyshev_not_slower_than_jacobi():
