@@ -126,6 +126,7 @@ _SIGS = {
     "fx_reduce_max": (C.c_int, [c_be_p, c_buf_p, _P(C.c_double)]),
     "fx_reduce_min": (C.c_int, [c_be_p, c_buf_p, _P(C.c_double)]),
     "fx_dot": (C.c_int, [c_be_p, c_buf_p, c_buf_p, _P(C.c_double)]),
+    "fx_dot_noalloc": (C.c_int, [c_be_p, c_buf_p, c_buf_p, c_buf_p]),
     "fx_dot2": (C.c_int, [c_be_p, c_buf_p, c_buf_p, c_buf_p, c_buf_p, _P(C.c_double)]),
     "fx_restrict": (C.c_int, [c_be_p, c_buf_p, c_buf_p, _P(FxGrid), _P(FxGrid)]),
     "fx_prolong": (C.c_int, [c_be_p, c_buf_p, c_buf_p, _P(FxGrid), _P(FxGrid)]),

@@ -26,6 +26,8 @@ struct fx_backend {
 	std::vector<hipEvent_t> probe_a, probe_b;
 	size_t probe_used = 0;
 	std::vector<float> probe_ms;
+	// This device's address of the static scratch behind fx_dot_noalloc (set up at creation).
+	void* noalloc_scratch = nullptr;
 };
 
 // HipBuffer analogue (backend-hip/src/buffer.rs:16-123).
@@ -122,6 +124,9 @@ fx_status k_fill_f32(fx_backend* be, float* dst, float v, size_t n);
 // y = A·x for an operator (fx_operator semantics); identity rows on the boundary.
 fx_status k_apply_operator(fx_backend* be, const fx_operator* op, const double* x, double* y, const fx_grid* g,
                            const int* skip = nullptr);
+// Finds this device's address of fx_dot_noalloc's static scratch (once per device, with the
+// backend's device current) and stores it in be->noalloc_scratch.
+fx_status noalloc_setup(fx_backend* be);
 // Host-returning dot of device arrays (synchronises).
 fx_status k_dot_host(fx_backend* be, const double* x, const double* y, size_t n, double* out);
 

@@ -8,6 +8,7 @@
 #include "fx_internal.hpp"
 
 #include <cmath>
+#include <mutex>
 
 using namespace fxd;
 
@@ -496,6 +497,19 @@ fx_status len_eq(const fx_buf* a, const fx_buf* b, const char* na, const char* n
 #define D(b) ((double*)(b)->ptr)
 #define CD(b) ((const double*)(b)->ptr)
 
+// ── static scratch of fx_dot_noalloc ──────────────────────────────────
+// The block partials of the dot live in this file-scope device array instead
+// of the backend's allocated partials buffer.  Each GPU holds its own copy for
+// the life of the process.  In host code `scratch` names a host-side shadow,
+// not device memory: kernels take the per-device address found by
+// noalloc_setup() instead.  red_blocks() never launches more than 2048
+// workgroups, so every input length fits in its 2048 · 16 bytes = 32 KiB.
+constexpr unsigned NOALLOC_PARTIALS = 2048;
+static __device__ __align__(256) Acc scratch[NOALLOC_PARTIALS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
 fx_status jacobi_scratch(fx_backend* be, size_t n, double** out) {
 	static thread_local fx_backend* owner = nullptr;
 	static thread_local double* buf = nullptr;
@@ -523,6 +537,24 @@ fx_status k_fill(fx_backend* be, double* d, double v, size_t n) { return launch_
 fx_status k_fill_f32(fx_backend* be, float* d, float v, size_t n) { return launch_elem(be, FillF32{d, v}, n, "fill32"); }
 fx_status k_dot_host(fx_backend* be, const double* x, const double* y, size_t n, double* out) {
 	return reduce_host<1>(be, DotT{x, y}, n, out, "dot");
+}
+
+// Looks up this device's address of `scratch` once (the backend's device is
+// current: fx_backend_create has just selected it) and hands it to the backend.
+fx_status noalloc_setup(fx_backend* be) {
+	FX_REQUIRE(be->device >= 0 && be->device < NOALLOC_MAX_DEVICES, FX_ERR_NO_DEVICE,
+	           "device %d is beyond the %d devices the no-allocation scratch table covers", be->device,
+	           NOALLOC_MAX_DEVICES);
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[be->device]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		FX_REQUIRE(e == hipSuccess && p, FX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", be->device,
+		           hipGetErrorString(e));
+		g_scratch_ptr[be->device] = p;
+	}
+	be->noalloc_scratch = g_scratch_ptr[be->device];
+	return FX_OK;
 }
 
 fx_status k_apply_operator(fx_backend* be, const fx_operator* op, const double* x, double* y, const fx_grid* g,
@@ -613,6 +645,24 @@ fx_status fx_dot(fx_backend* be, const fx_buf* x, const fx_buf* y, double* out) 
 	FX_REQUIRE(be && out, FX_ERR_ARGUMENT, "null argument");
 	FX_CHECK(len_eq(x, y, "x", "y"));
 	return reduce_host<1>(be, DotT{CD(x), CD(y)}, x->len, out, "dot");
+}
+// fx_dot's kernels with the block partials in the static scratch and the sum
+// stored into out[0] on the device: no allocation, no host synchronisation.
+fx_status fx_dot_noalloc(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* out) {
+	FX_REQUIRE(be && out && out->ptr, FX_ERR_ARGUMENT, "null argument");
+	FX_REQUIRE(out->dtype == FX_F64, FX_ERR_ARGUMENT, "out must be an fp64 buffer");
+	FX_REQUIRE(out->len >= 1, FX_ERR_DIMENSION, "out holds %zu values; the dot needs 1", out->len);
+	FX_CHECK(len_eq(x, y, "x", "y"));
+	FX_REQUIRE(be->noalloc_scratch, FX_ERR_HIP, "the no-allocation scratch is not set up for device %d", be->device);
+	const unsigned nb = red_blocks(x->len);
+	FX_REQUIRE(nb <= NOALLOC_PARTIALS, FX_ERR_DIMENSION,
+	           "a dot of %zu values needs %u block partials; the static scratch holds %u", x->len, nb,
+	           NOALLOC_PARTIALS);
+	const RedScratch rs{(Acc*)be->noalloc_scratch};
+	hipLaunchKernelGGL((k_reduce<1, DotT>), dim3(nb), dim3(BLOCK), 0, be->stream, DotT{CD(x), CD(y)}, x->len, rs);
+	hipLaunchKernelGGL((k_finish<1, FinStore<1>>), dim3(1), dim3(FIN_BLOCK), 0, be->stream, rs, nb,
+	                   FinStore<1>{D(out)}, (const int*)nullptr);
+	return check_launch("dot_noalloc");
 }
 fx_status fx_dot2(fx_backend* be, const fx_buf* a, const fx_buf* b, const fx_buf* c, const fx_buf* d, double* out) {
 	FX_REQUIRE(be && out, FX_ERR_ARGUMENT, "null argument");

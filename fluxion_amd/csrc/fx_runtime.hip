@@ -195,6 +195,7 @@ fx_status fx_backend_create(int device, fx_backend** out) {
 		return FX_ERR_ALLOC;
 	}
 	if ((s = ensure_partials(be, 8192)) != FX_OK) { fx_backend_destroy(be); return s; }
+	if ((s = noalloc_setup(be)) != FX_OK) { fx_backend_destroy(be); return s; }
 	if (hipStreamSynchronize(be->stream) != hipSuccess) { fx_backend_destroy(be); set_error("sync failed"); return FX_ERR_HIP; }
 	*out = be;
 	return FX_OK;

@@ -131,6 +131,14 @@ fx_status fx_reduce_sum(fx_backend* be, const fx_buf* buf, double* out);        
 fx_status fx_reduce_max(fx_backend* be, const fx_buf* buf, double* out);             /* backend.rs:194 */
 fx_status fx_reduce_min(fx_backend* be, const fx_buf* buf, double* out);             /* backend.rs:200 */
 fx_status fx_dot(fx_backend* be, const fx_buf* x, const fx_buf* y, double* out);     /* backend.rs:206-210 */
+/* No counterpart in the reference: fx_dot for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It allocates nothing and
+ * does not synchronise: the block partials live in a static device array of 2048 partials (32 KiB)
+ * that each GPU holds until the process ends, and x·y is stored into out[0] (FP64, len >= 1) on the
+ * backend's stream — the same value as fx_dot, bit for bit.  Every input length fits (the reduction
+ * never launches more than 2048 workgroups).  The array is shared by every call on one GPU: calls
+ * that overlap on one GPU (from two backends or threads) race. */
+fx_status fx_dot_noalloc(fx_backend* be, const fx_buf* x, const fx_buf* y, fx_buf* out);
 /* out[0] = a·b, out[1] = c·d in one pass (backend.rs:219-230; HIP dot2 backend-hip/src/backend.rs:1143-1210) */
 fx_status fx_dot2(fx_backend* be, const fx_buf* a, const fx_buf* b, const fx_buf* c, const fx_buf* d,
                   double* out);
