@@ -52,7 +52,8 @@ class FxMgOptions(C.Structure):
     _fields_ = [("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", C.c_double),
                 ("smoother", C.c_int32), ("transfer", C.c_int32), ("coarsest", C.c_int32),
                 ("extra_coarse", C.c_int32), ("precision", C.c_int32), ("fused", C.c_int32),
-                ("sigma", C.c_double), ("scale", C.c_double), ("k", c_buf_p)]
+                ("sigma", C.c_double), ("scale", C.c_double), ("k", c_buf_p),
+                ("vx", c_buf_p), ("vy", c_buf_p), ("kappa", C.c_double)]
 
 
 class FxPrecond(C.Structure):

@@ -32,6 +32,10 @@ template <class T> struct LW {
 	// Variable-coefficient level operator sigma·u + scale·(−∇·(k∇u)) when k != nullptr.
 	const T* k;
 	T ix2, iy2, sigma, scale;
+	// Convection-diffusion level operator sigma·u + scale·(κΔu + v·∇⁻u) when vx != nullptr:
+	// 1/dx, 1/dy and their squares formed as (1/dx)·(1/dx), as ops.rs:686-689 forms them.
+	const T *vx, *vy;
+	T kap, ix, iy, cix2, ciy2;
 };
 
 template <class T> __device__ __forceinline__ T hmT(T a, T b) { return T(2) * a * b / (a + b); }
@@ -47,6 +51,22 @@ __device__ __forceinline__ T vk_apply(const T* u, const LW<T>& L, int64_t i, int
 	T Lv = -(fx * L.ix2 + fy * L.iy2);
 	*diag = fmaT<T>(L.scale, (kw + ke) * L.ix2 + (kn + ks) * L.iy2, L.sigma);
 	return fmaT<T>(L.scale, Lv, L.sigma * c);
+}
+
+// sigma·u + scale·(κΔu + v·∇⁻u) at interior point i, the arithmetic of k_conv_diff (ops.rs:707-731) in T;
+// *diag gets sigma + scale·(|vx|/dx + |vy|/dy − 2κ(1/dx² + 1/dy²)).
+template <class T>
+__device__ __forceinline__ T cd_apply(const T* u, const LW<T>& L, int64_t i, int64_t cols, T* diag) {
+	const T c = u[i];
+	const T lap = fmaT<T>(u[i - 1] + u[i + 1], L.cix2, (u[i - cols] + u[i + cols]) * L.ciy2) - T(2) * (L.cix2 + L.ciy2) * c;
+	const T diff = L.kap * lap;
+	const T vxi = L.vx[i], vyi = L.vy[i];
+	const T dudx = vxi >= T(0) ? (c - u[i - 1]) * L.ix : (u[i + 1] - c) * L.ix;
+	const T dudy = vyi >= T(0) ? (c - u[i - cols]) * L.iy : (u[i + cols] - c) * L.iy;
+	const T conv = fmaT<T>(vxi, dudx, vyi * dudy);
+	const T ax = vxi >= T(0) ? vxi : -vxi, ay = vyi >= T(0) ? vyi : -vyi;
+	*diag = fmaT<T>(L.scale, fmaT<T>(ax, L.ix, ay * L.iy) - T(2) * L.kap * (L.cix2 + L.ciy2), L.sigma);
+	return fmaT<T>(L.scale, diff + conv, L.sigma * c);
 }
 
 #define SKIP_RETURN if (skip && *skip) return
@@ -108,6 +128,12 @@ __global__ __launch_bounds__(BLOCK) void k_jac(const T* xo, T* xn, const T* b, L
 	SKIP_RETURN;
 	PT_INDEX;
 	if (bnd) { xn[i] = xo[i]; return; }
+	if (L.vx) {
+		T d;
+		T au = cd_apply<T>(xo, L, i, cols, &d);
+		xn[i] = fmaT<T>(L.omega / d, b[i] - au, xo[i]);
+		return;
+	}
 	if (L.k) {
 		T d;
 		T au = vk_apply<T>(xo, L, i, cols, &d);
@@ -149,6 +175,9 @@ __global__ __launch_bounds__(BLOCK) void k_lstencil(const T* u, const T* rhs, T*
 		else if (r == rows - 1) s = T(bcv(bc.bottom, double(u[i]), double(u[i - cols]), dy));
 		else if (c == 0) s = T(bcv(bc.left, double(u[i]), double(u[i + 1]), dx));
 		else s = T(bcv(bc.right, double(u[i]), double(u[i - 1]), dx));
+	} else if (L.vx) {
+		T d;
+		s = cd_apply<T>(u, L, i, cols, &d);
 	} else if (L.k) {
 		T d;
 		s = vk_apply<T>(u, L, i, cols, &d);
@@ -314,6 +343,13 @@ struct Driver {
 		o.iy2 = T(1.0 / (L.g.dy * L.g.dy));
 		o.sigma = T(mg->opt.sigma);
 		o.scale = T(mg->opt.scale);
+		o.vx = (const T*)L.vx;
+		o.vy = (const T*)L.vy;
+		o.kap = T(mg->opt.kappa);
+		o.ix = T(1.0 / L.g.dx);
+		o.iy = T(1.0 / L.g.dy);
+		o.cix2 = T((1.0 / L.g.dx) * (1.0 / L.g.dx));
+		o.ciy2 = T((1.0 / L.g.dy) * (1.0 / L.g.dy));
 		return o;
 	}
 	size_t n(const fx_multigrid::Level& L) const { return (size_t)L.g.rows * L.g.cols; }
@@ -424,6 +460,8 @@ void free_levels(fx_multigrid* mg) {
 	for (auto& L : mg->lv) {
 		fx::dev_free(L.u); fx::dev_free(L.rhs); fx::dev_free(L.tmp); fx::dev_free(L.w0); fx::dev_free(L.w1);
 		fx::dev_free(L.k);
+		fx::dev_free(L.vx);
+		fx::dev_free(L.vy);
 		fx::dev_free(L.kx);
 		fx::dev_free(L.ky);
 		fx::dev_free(L.wd);
@@ -560,6 +598,14 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 		FX_REQUIRE(opt->smoother == FX_SMOOTH_JACOBI, FX_ERR_UNSUPPORTED,
 		           "variable-coefficient levels use the Jacobi smoother");
 	}
+	FX_REQUIRE(!opt->vx == !opt->vy, FX_ERR_ARGUMENT, "give both velocity components or neither");
+	if (opt->vx) {
+		FX_CHECK(check_buf(opt->vx, grid->rows * grid->cols, FX_F64, "vx"));
+		FX_CHECK(check_buf(opt->vy, grid->rows * grid->cols, FX_F64, "vy"));
+		FX_REQUIRE(!opt->k && !opt->fused && opt->smoother == FX_SMOOTH_JACOBI, FX_ERR_UNSUPPORTED,
+		           "convection-diffusion levels use the unfused Jacobi cycle without a coefficient field");
+		FX_REQUIRE(std::isfinite(opt->kappa), FX_ERR_ARGUMENT, "kappa must be finite");
+	}
 	fx_multigrid* mg = new fx_multigrid();
 	mg->be = be;
 	mg->opt = *opt;
@@ -603,6 +649,32 @@ fx_status fx_multigrid_create_ex(fx_backend* be, const fx_grid* grid, const fx_b
 					                        (int64_t)g.rows, (int64_t)g.cols, lin);
 				}
 				s = check_launch("multigrid coefficient");
+			}
+		}
+		if (s == FX_OK && opt->vx) {
+			const size_t n = (size_t)g.rows * g.cols;
+			void** vb[2] = {&L.vx, &L.vy};
+			const fx_buf* vsrc[2] = {opt->vx, opt->vy};
+			for (int q = 0; q < 2 && s == FX_OK; ++q) {
+				s = dev_alloc(be, bytes, vb[q]);
+				if (s != FX_OK) break;
+				if (mg->lv.empty()) {
+					if (mg->f32) hipLaunchKernelGGL(k_cvt_k<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+					                                be->stream, (const double*)vsrc[q]->ptr, (float*)*vb[q], n);
+					else hipLaunchKernelGGL(k_cvt_k<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+					                        be->stream, (const double*)vsrc[q]->ptr, (double*)*vb[q], n);
+				} else {
+					const fx_multigrid::Level& F = mg->lv.back();
+					const void* fv = q == 0 ? F.vx : F.vy;
+					int lin = opt->transfer == FX_TRANSFER_FW_BILINEAR;
+					if (mg->f32) hipLaunchKernelGGL(k_coarsen_k<float>, pgrid(g.rows, g.cols), dim3(BLOCK), 0, be->stream,
+					                                (const float*)fv, (float*)*vb[q], (int64_t)F.g.rows, (int64_t)F.g.cols,
+					                                (int64_t)g.rows, (int64_t)g.cols, lin);
+					else hipLaunchKernelGGL(k_coarsen_k<double>, pgrid(g.rows, g.cols), dim3(BLOCK), 0, be->stream,
+					                        (const double*)fv, (double*)*vb[q], (int64_t)F.g.rows, (int64_t)F.g.cols,
+					                        (int64_t)g.rows, (int64_t)g.cols, lin);
+				}
+				s = check_launch("multigrid velocity");
 			}
 		}
 		mg->lv.push_back(L);

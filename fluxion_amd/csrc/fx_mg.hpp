@@ -31,6 +31,8 @@ struct fx_multigrid {
 		void* w0 = nullptr;    // Chebyshev work (T[n])
 		void* w1 = nullptr;
 		void* k = nullptr;     // nodal coefficient (T[n]) of a variable-coefficient hierarchy
+		void* vx = nullptr;    // nodal velocity (T[n] each) of a convection-diffusion hierarchy
+		void* vy = nullptr;
 		void* kx = nullptr;    // FP32 fused hierarchies: harmonic-mean face coefficient east of each point
 		void* ky = nullptr;    //   and south of each point (float[n]; 0 past the last column / row)
 		void* wd = nullptr;    //   and ω/diagonal of the level operator (float[n]; 0 on the boundary)

@@ -1339,6 +1339,7 @@ struct fx_solver {
 	uint64_t max_iter = 0;
 	std::vector<double*> v;      // work vectors
 	std::vector<double*> basis;  // GMRES Krylov basis (m+1)
+	std::vector<double*> zbasis; // FGMRES preconditioned directions z_k = M⁻¹v_k (m)
 	SState* dst = nullptr;       // device state
 	SState* hst = nullptr;       // pinned host copy
 	double* dhist = nullptr;
@@ -1386,6 +1387,7 @@ const char* solver_name(int kind) {
 	case FX_SOLVER_BICGSTAB: return "BiCGSTAB";
 	case FX_SOLVER_GMRES: return "GMRES";
 	case FX_SOLVER_FCG: return "FCG";
+	case FX_SOLVER_FGMRES: return "FGMRES";
 	default: return "?";
 	}
 }
@@ -1922,13 +1924,17 @@ fx_status solve_bicgstab(Ctx& c, double* x, const double* b, fx_solve_stats* sta
 fx_status solve_gmres(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	fx_solver* s = c.s;
 	const int m = (int)s->restart;
-	const bool pre = c.pc != nullptr;
+	// Flexible GMRES preconditions on the right (z_k = M⁻¹v_k); GMRES on the left (gmres.rs:294-454).
+	const bool flex = s->kind == FX_SOLVER_FGMRES && c.pc != nullptr;
+	const bool pre = c.pc != nullptr && !flex;
 	double* w = s->vec(0);
 	FX_REQUIRE(w && (int)s->basis.size() == m + 1, FX_ERR_ALLOC, "work vector allocation failed");
+	FX_REQUIRE(!flex || (int)s->zbasis.size() == m, FX_ERR_ALLOC, "work vector allocation failed");
 	auto& vv = s->basis;
+	auto& zz = s->zbasis;
 	FX_CHECK(reset_state(s, (int64_t)s->max_iter));
 	VecList vl{};
-	for (int i = 0; i <= m; ++i) vl.v[i] = vv[i];
+	for (int i = 0; i < m + (flex ? 0 : 1); ++i) vl.v[i] = flex ? zz[i] : vv[i];
 	const SState* dd = c.d;
 	FX_CHECK(drive(s, [&](int64_t) -> fx_status {
 		// Cycle start (gmres.rs:124-157 / 305-337).
@@ -1943,7 +1949,12 @@ fx_status solve_gmres(Ctx& c, double* x, const double* b, fx_solve_stats* stats)
 		}
 		FX_CHECK(c.eupd(FGmScale{&dd->gbeta, vv[0], vv[0], 0.0}, "gmres v0 scale"));
 		for (int k = 0; k < m; ++k) {
-			FX_CHECK(c.apply_op(vv[k], w));
+			if (flex) {
+				FX_CHECK(c.precond(zz[k], vv[k]));
+				FX_CHECK(c.apply_op(zz[k], w));
+			} else {
+				FX_CHECK(c.apply_op(vv[k], w));
+			}
 			if (pre) {
 				FX_CHECK(c.precond(vv[k + 1], w));
 				FX_CHECK(c.eupd(FCopy{vv[k + 1], w}, "gmres copy"));
@@ -2333,9 +2344,10 @@ fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, ui
 	FX_REQUIRE(be && out, FX_ERR_ARGUMENT, "null argument");
 	*out = nullptr;
 	FX_CHECK(check_grid(grid));
-	FX_REQUIRE(kind >= FX_SOLVER_CG && kind <= FX_SOLVER_FCG, FX_ERR_ARGUMENT, "unknown solver kind %d", kind);
-	FX_REQUIRE(kind != FX_SOLVER_GMRES || (restart >= 1 && restart <= GM_MAX), FX_ERR_ARGUMENT,
-	           "GMRES restart must be in 1..%d", GM_MAX);
+	FX_REQUIRE(kind >= FX_SOLVER_CG && kind <= FX_SOLVER_FGMRES, FX_ERR_ARGUMENT, "unknown solver kind %d", kind);
+	const bool gm = kind == FX_SOLVER_GMRES || kind == FX_SOLVER_FGMRES;
+	FX_REQUIRE(!gm || (restart >= 1 && restart <= GM_MAX), FX_ERR_ARGUMENT, "GMRES restart must be in 1..%d",
+	           GM_MAX);
 	fx_solver* s = new fx_solver();
 	s->be = be;
 	s->kind = kind;
@@ -2366,10 +2378,14 @@ fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, ui
 	                                                         : 1;
 	for (int i = 0; i < nvec; ++i)
 		if (!s->vec(i)) return fail(FX_ERR_ALLOC);
-	if (kind == FX_SOLVER_GMRES) {
+	if (gm) {
 		for (uint32_t i = 0; i <= restart; ++i) {
 			if (dev_alloc(be, s->n * sizeof(double), &p) != FX_OK) return fail(FX_ERR_ALLOC);
 			s->basis.push_back((double*)p);
+		}
+		for (uint32_t i = 0; kind == FX_SOLVER_FGMRES && i < restart; ++i) {
+			if (dev_alloc(be, s->n * sizeof(double), &p) != FX_OK) return fail(FX_ERR_ALLOC);
+			s->zbasis.push_back((double*)p);
 		}
 		size_t m = restart;
 		if (dev_alloc(be, sizeof(double) * ((m + 1) * m + 3 * m + 1 + m), &p) != FX_OK) return fail(FX_ERR_ALLOC);
@@ -2385,6 +2401,7 @@ void fx_solver_destroy(fx_solver* s) {
 	if (s->be) (void)hipStreamSynchronize(s->be->stream);
 	for (double* p : s->v) dev_free(p);
 	for (double* p : s->basis) dev_free(p);
+	for (double* p : s->zbasis) dev_free(p);
 	dev_free(s->dst);
 	dev_free(s->dhist);
 	dev_free(s->dgm);
@@ -2428,7 +2445,8 @@ fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_
 	case FX_SOLVER_FCG: return solve_fcg(c, xp, bp, stats);
 	case FX_SOLVER_PIPELINED_CG: return pc ? solve_ppcg(c, xp, bp, stats) : solve_pipelined(c, xp, bp, stats);
 	case FX_SOLVER_BICGSTAB: return solve_bicgstab(c, xp, bp, stats);
-	case FX_SOLVER_GMRES: return solve_gmres(c, xp, bp, stats);
+	case FX_SOLVER_GMRES:
+	case FX_SOLVER_FGMRES: return solve_gmres(c, xp, bp, stats);
 	default: break;
 	}
 	set_error("solver kind %d not implemented", s->kind);

@@ -359,19 +359,22 @@ class Multigrid:
 
     Multigrid(grid, bc, n_pre, n_post, omega) is Multigrid::new; the keyword
     options select the extensions (transfer operators, coarsest size, FP32
-    levels, level operator sigma·I + scale·Δ_h)."""
+    levels, level operator sigma·I + scale·Δ_h, or with k sigma·I + scale·(−∇·k∇),
+    or with vx, vy the upwind convection-diffusion sigma·I + scale·(κΔ + v·∇⁻))."""
 
     def __init__(self, grid: Grid, bc: Boundaries, n_pre: int = 2, n_post: int = 2, omega: float = 2.0 / 3.0,
                  smoother: SmootherKind = SmootherKind.JACOBI, *, transfer: Transfer = Transfer.REFERENCE,
                  coarsest: int = 4, extra_coarse: int = 10, precision: DType = DType.F64, fused: bool = False,
                  sigma: float = 0.0, scale: float = 1.0, k: Optional["Field"] = None,
+                 vx: Optional["Field"] = None, vy: Optional["Field"] = None, kappa: float = 0.0,
                  be: Optional[HipBackend] = None):
         self.be = _backend(be)
         self._lib = self.be._lib
         self.grid = grid
-        self._k = k
+        self._k = (k, vx, vy)
         opt = L.FxMgOptions(n_pre, n_post, omega, int(smoother), int(transfer), coarsest, extra_coarse,
-                            int(precision), int(bool(fused)), sigma, scale, k.h if k is not None else None)
+                            int(precision), int(bool(fused)), sigma, scale, k.h if k is not None else None,
+                            vx.h if vx is not None else None, vy.h if vy is not None else None, kappa)
         g, b = L.grid_c(grid), L.bcs_c(bc)
         h = C.c_void_p()
         L.check(self._lib.fx_multigrid_create_ex(self.be.h, C.byref(g), C.byref(b), C.byref(opt), C.byref(h)),
@@ -503,6 +506,15 @@ class BiCgStabSolver(_Solver):
 class GmresSolver(_Solver):
     """gmres.rs:24-490: GmresSolver::new(grid, device, restart, tol, max_iter)."""
     KIND = 3
+
+    def __init__(self, grid, restart, tol, max_iter, be=None):
+        super().__init__(grid, tol, max_iter, restart, be)
+
+
+class FgmresSolver(_Solver):
+    """Flexible right-preconditioned GMRES(m) (extension; Saad 1993): the residual estimate is
+    ‖b − A·x‖ and the preconditioner may vary; unpreconditioned it is GmresSolver."""
+    KIND = 5
 
     def __init__(self, grid, restart, tol, max_iter, be=None):
         super().__init__(grid, tol, max_iter, restart, be)

@@ -219,6 +219,12 @@ typedef struct {
 	int32_t fused;           /* 1: fused FP32 kernels (smoother+residual+restriction); needs FX_F32, Jacobi, FW */
 	double sigma, scale;     /* level operator sigma·I + scale·Δ_h (reference: 0, 1 — multigrid.rs:105-106) */
 	const fx_buf* k;         /* optional nodal coefficient: level operator sigma·I + scale·(−∇·k∇) */
+	/* Optional nodal velocity (FP64, both or neither; not with k, Jacobi only, unfused): level operator
+	 * sigma·I + scale·(κΔ + v·∇⁻v), the upwind convection-diffusion of ops.rs:707-731 rediscretised on
+	 * every level with the velocity interpolated like k, κ = kappa. */
+	const fx_buf* vx;
+	const fx_buf* vy;
+	double kappa;
 } fx_mg_options;
 
 /* Multigrid::new(grid, bc, device, n_pre, n_post, omega) / Multigrid::build(.., smoother)
@@ -257,15 +263,20 @@ typedef enum {
 	FX_SOLVER_PIPELINED_CG = 1,  /* PipelinedCgSolver (pipelined_cg.rs:25-238) */
 	FX_SOLVER_BICGSTAB = 2,      /* BiCgStabSolver (bicgstab.rs:19-346) */
 	FX_SOLVER_GMRES = 3,         /* GmresSolver (gmres.rs:24-490) */
-	FX_SOLVER_FCG = 4            /* extension: flexible CG with FP64 true-residual refinement */
+	FX_SOLVER_FCG = 4,           /* extension: flexible CG with FP64 true-residual refinement */
+	FX_SOLVER_FGMRES = 5         /* extension: flexible right-preconditioned GMRES(m) (Saad 1993): each step
+	                              * stores z_k = M⁻¹v_k and applies A to it, x += Σ y_i·z_i, so the Givens
+	                              * estimate is ‖b − A·x‖ and the preconditioner may vary (FP32 V-cycle);
+	                              * without a preconditioner it is GmresSolver (gmres.rs:116-283) */
 } fx_solver_kind;
 
 /* XxxSolver::new(grid, device, [restart,] tol, max_iter): all work vectors allocated once
- * (cg.rs:72-86, pipelined_cg.rs:50-70, bicgstab.rs:61-82, gmres.rs:56-81). restart is used by GMRES only. */
+ * (cg.rs:72-86, pipelined_cg.rs:50-70, bicgstab.rs:61-82, gmres.rs:56-81). restart is used by GMRES and
+ * FGMRES only. */
 fx_status fx_solver_create(fx_backend* be, int32_t kind, const fx_grid* grid, uint32_t restart, double tol,
                            uint64_t max_iter, fx_solver** out);
 void fx_solver_destroy(fx_solver* s);
-/* LinearSolver::name (linear_solver.rs:30): "CG", "PipelinedCG", "BiCGSTAB", "GMRES", "FCG". */
+/* LinearSolver::name (linear_solver.rs:30): "CG", "PipelinedCG", "BiCGSTAB", "GMRES", "FCG", "FGMRES". */
 const char* fx_solver_name(const fx_solver* s);
 /* LinearSolver::solve(x, b, coeff, stencil, bc) (linear_solver.rs:13-27). */
 fx_status fx_solver_solve(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff, const fx_stencil* st,

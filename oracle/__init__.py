@@ -42,7 +42,7 @@ class OMg(C.Structure):
     _fields_ = [("n_pre", C.c_int32), ("n_post", C.c_int32), ("omega", C.c_double), ("smoother", C.c_int32),
                 ("transfer", C.c_int32), ("coarsest", C.c_int32), ("extra_coarse", C.c_int32),
                 ("precision", C.c_int32), ("pad", C.c_int32), ("sigma", C.c_double), ("scale", C.c_double),
-                ("bc", OBcs), ("k", C.c_void_p)]
+                ("bc", OBcs), ("k", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("kappa", C.c_double)]
 
 
 class OStats(C.Structure):
@@ -107,13 +107,19 @@ def obcs(bc: Boundaries) -> OBcs:
 
 
 def mg_cfg(n_pre=2, n_post=2, omega=2.0 / 3.0, smoother=SMOOTH_JACOBI, transfer=0, coarsest=4, extra_coarse=10,
-           precision=0, sigma=0.0, scale=1.0, bc: Boundaries = None, k=None) -> OMg:
+           precision=0, sigma=0.0, scale=1.0, bc: Boundaries = None, k=None, vx=None, vy=None, kappa=0.0) -> OMg:
     """Multigrid options; `k` (nodal coefficient, kept alive by the returned object) selects the
-    level operator sigma·I + scale·(−∇·k∇)."""
+    level operator sigma·I + scale·(−∇·k∇); `vx`, `vy` (nodal velocity) select the upwind
+    convection-diffusion level operator sigma·I + scale·(κΔ + v·∇⁻) with κ = kappa."""
     c = OMg()
     if k is not None:
         c._k = np.ascontiguousarray(k, dtype=np.float64)
         c.k = c._k.ctypes.data
+    if vx is not None:
+        c._vx = np.ascontiguousarray(vx, dtype=np.float64)
+        c._vy = np.ascontiguousarray(vy, dtype=np.float64)
+        c.vx, c.vy = c._vx.ctypes.data, c._vy.ctypes.data
+    c.kappa = kappa
     c.n_pre, c.n_post, c.omega, c.smoother = n_pre, n_post, omega, smoother
     c.transfer, c.coarsest, c.extra_coarse, c.precision = transfer, coarsest, extra_coarse, precision
     c.sigma, c.scale = sigma, scale
@@ -353,6 +359,11 @@ def gmres(x, b, grid, op, restart, tol, max_iter, mg: OMg = None, identity=False
     else:
         pc = _pc(mg)
     return _solve(lib().orc_gmres, x, b, grid, op, tol, max_iter, (C.c_int64(restart),) + tuple(pc))
+
+
+def fgmres(x, b, grid, op, restart, tol, max_iter, mg: OMg = None):
+    """Flexible right-preconditioned GMRES(m) (extension): z_k = M⁻¹v_k stored, x += Z·y."""
+    return _solve(lib().orc_fgmres, x, b, grid, op, tol, max_iter, (C.c_int64(restart),) + tuple(_pc(mg)))
 
 
 # ── CrankNicolsonSolver::step (runtime/src/implicit.rs:178-222) ────────

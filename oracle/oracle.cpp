@@ -483,6 +483,12 @@ struct orc_mg_cfg {
 	double sigma, scale;   // level operator sigma·I + scale·Δ_h (reference: 0, 1)
 	orc_bcs bc;
 	const double* k;       // EXTENSION: nodal coefficient -> level operator sigma·I + scale·(−∇·k∇)
+	// EXTENSION: nodal velocity -> level operator sigma·I + scale·(κΔ + v·∇⁻), the upwind
+	// convection-diffusion of ops.rs:707-731 rediscretised on every level (velocity interpolated
+	// like k); kappa is the scalar κ.
+	const double* vx;
+	const double* vy;
+	double kappa;
 };
 
 struct orc_stats { int64_t iterations; double residual; int32_t reason; int32_t pad; };
@@ -531,6 +537,7 @@ struct MgLevel {
 	Lvl5<T> L;
 	std::vector<T> u, rhs, tmp, w0, w1;
 	std::vector<T> k;   // variable-coefficient levels only
+	std::vector<T> vx, vy;   // convection-diffusion levels only
 };
 
 // EXTENSION: variable-coefficient level operator sigma·u + scale·(−∇·(k∇u)) at an interior
@@ -546,6 +553,24 @@ inline T vk_apply(const T* u, const T* k, int64_t i, int64_t cols, T ix2, T iy2,
 	T L = -(fx * ix2 + fy * iy2);
 	*diag = fmaT<T>(scale, (kw + ke) * ix2 + (kn + ks) * iy2, sigma);
 	return fmaT<T>(scale, L, sigma * c);
+}
+
+// EXTENSION: convection-diffusion level operator sigma·u + scale·(κΔu + v·∇⁻u) at an interior point,
+// the arithmetic of conv_diff_core (ops.rs:707-731) in T; `diag` receives its diagonal
+// sigma + scale·(|vx|/dx + |vy|/dy − 2κ(1/dx² + 1/dy²)).
+template <class T>
+inline T cd_apply(const T* u, const T* vx, const T* vy, int64_t i, int64_t cols, T ix, T iy, T ix2, T iy2, T kap,
+                  T sigma, T scale, T* diag) {
+	T c = u[i];
+	T lap = fmaT<T>(u[i - 1] + u[i + 1], ix2, (u[i - cols] + u[i + cols]) * iy2) - T(2) * (ix2 + iy2) * c;
+	T diff = kap * lap;
+	T vxi = vx[i], vyi = vy[i];
+	T dudx = vxi >= T(0) ? (c - u[i - 1]) * ix : (u[i + 1] - c) * ix;
+	T dudy = vyi >= T(0) ? (c - u[i - cols]) * iy : (u[i + cols] - c) * iy;
+	T conv = fmaT<T>(vxi, dudx, vyi * dudy);
+	T ax = vxi >= T(0) ? vxi : -vxi, ay = vyi >= T(0) ? vyi : -vyi;
+	*diag = fmaT<T>(scale, fmaT<T>(ax, ix, ay * iy) - T(2) * kap * (ix2 + iy2), sigma);
+	return fmaT<T>(scale, diff + conv, sigma * c);
 }
 
 // Coefficient on a coarse level: injection for halving levels, linear
@@ -598,6 +623,17 @@ struct Mg {
 					coarsen_k(F.k.data(), F.g, L.k.data(), g, c.transfer);
 				}
 			}
+			if (c.vx && c.vy) {
+				L.vx.assign(n, T(0));
+				L.vy.assign(n, T(0));
+				if (lv.empty()) {
+					for (size_t i = 0; i < n; ++i) { L.vx[i] = T(c.vx[i]); L.vy[i] = T(c.vy[i]); }
+				} else {
+					const MgLevel<T>& F = lv.back();
+					coarsen_k(F.vx.data(), F.g, L.vx.data(), g, c.transfer);
+					coarsen_k(F.vy.data(), F.g, L.vy.data(), g, c.transfer);
+				}
+			}
 			lv.push_back(std::move(L));
 			if (g.rows <= lim || g.cols <= lim) break;
 			if (c.transfer == 0) {
@@ -619,7 +655,7 @@ struct Mg {
 		T om = T(cfg.omega);
 		if (cfg.smoother == 2) { smooth_cheby(l, n); return; }
 		for (int it = 0; it < n; ++it) {
-			if (!L.k.empty()) {
+			if (!L.k.empty() || !L.vx.empty()) {
 				smooth_jacobi_k(l, L.u.data(), L.tmp.data());
 				std::swap(L.u, L.tmp);
 			} else if (cfg.smoother == 0) smooth_ref_gs<T>(L.u.data(), L.rhs.data(), om, L.g, L.L);
@@ -630,20 +666,23 @@ struct Mg {
 		}
 	}
 
-	// EXTENSION: two-buffer Jacobi with the variable-coefficient level operator:
-	// x_new = fma(ω, (b − A·x)/diag, x).
+	// EXTENSION: two-buffer Jacobi with the variable-coefficient or convection-diffusion level
+	// operator: x_new = fma(ω/diag, b − A·x, x).
 	void smooth_jacobi_k(int l, const T* xo, T* xn) {
 		MgLevel<T>& L = lv[l];
 		const int64_t rows = L.g.rows, cols = L.g.cols;
 		const T ix2 = T(1.0 / (L.g.dx * L.g.dx)), iy2 = T(1.0 / (L.g.dy * L.g.dy));
 		const T sg = T(cfg.sigma), sc = T(cfg.scale), om = T(cfg.omega);
+		const T ix = T(1.0 / L.g.dx), iy = T(1.0 / L.g.dy), kap = T(cfg.kappa);
+		const T cix2 = T((1.0 / L.g.dx) * (1.0 / L.g.dx)), ciy2 = T((1.0 / L.g.dy) * (1.0 / L.g.dy));
 #pragma omp parallel for schedule(static)
 		for (int64_t r = 0; r < rows; ++r)
 			for (int64_t c = 0; c < cols; ++c) {
 				int64_t i = r * cols + c;
 				if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) { xn[i] = xo[i]; continue; }
 				T d;
-				T au = vk_apply<T>(xo, L.k.data(), i, cols, ix2, iy2, sg, sc, &d);
+				T au = L.vx.empty() ? vk_apply<T>(xo, L.k.data(), i, cols, ix2, iy2, sg, sc, &d)
+				                    : cd_apply<T>(xo, L.vx.data(), L.vy.data(), i, cols, ix, iy, cix2, ciy2, kap, sg, sc, &d);
 				xn[i] = fmaT<T>(om / d, L.rhs[i] - au, xo[i]);
 			}
 	}
@@ -668,10 +707,17 @@ struct Mg {
 		const T* w = L.L.w;
 		const T ix2 = T(1.0 / (L.g.dx * L.g.dx)), iy2 = T(1.0 / (L.g.dy * L.g.dy));
 		const T sg = T(cfg.sigma), sc = T(cfg.scale);
+		const T ix = T(1.0 / L.g.dx), iy = T(1.0 / L.g.dy), kap = T(cfg.kappa);
+		const T cix2 = T((1.0 / L.g.dx) * (1.0 / L.g.dx)), ciy2 = T((1.0 / L.g.dy) * (1.0 / L.g.dy));
 #pragma omp parallel for schedule(static)
 		for (int64_t r = 1; r < lr; ++r)
 			for (int64_t c = 1; c < lc; ++c) {
 				int64_t i = r * cols + c;
+				if (!L.vx.empty()) {
+					T d;
+					out[i] = cd_apply<T>(in, L.vx.data(), L.vy.data(), i, cols, ix, iy, cix2, ciy2, kap, sg, sc, &d);
+					continue;
+				}
 				if (!L.k.empty()) {
 					T d;
 					out[i] = vk_apply<T>(in, L.k.data(), i, cols, ix2, iy2, sg, sc, &d);
@@ -1166,15 +1212,23 @@ int orc_bicgstab(double* x, const double* b, int64_t rows, int64_t cols, double 
 }
 
 // ── GmresSolver — gmres.rs:116-490 (left preconditioning when pc_kind != 0)
-int orc_gmres(double* x, const double* b, int64_t rows, int64_t cols, double dx, double dy, const orc_op* op,
-              int64_t restart, int pc_kind, const orc_mg_cfg* cfg, double tol, int64_t max_iter, orc_stats* st,
-              double* hist) {
+}  // extern "C"
+
+namespace {
+// GmresSolver::solve / solve_preconditioned (gmres.rs:116-490; left preconditioning), or with
+// `flex` the EXTENSION flexible right-preconditioned GMRES(m) (Saad 1993): the cycle starts from the
+// unpreconditioned residual, every Arnoldi step applies z_k = M⁻¹v_k (stored) and then A, and the
+// correction is x += Σ y_i·z_i, so |g_{k+1}| estimates ‖b − A·x_k‖ whatever M does.
+int gmres_impl(double* x, const double* b, int64_t rows, int64_t cols, double dx, double dy, const orc_op* op,
+               int64_t restart, int pc_kind, const orc_mg_cfg* cfg, double tol, int64_t max_iter, orc_stats* st,
+               double* hist, bool flex) {
 	Grid g = mkgrid(rows, cols, dx, dy);
 	if (int e = valid_grid(g)) return e;
 	int64_t n = g.len(), m = restart;
-	bool pre = pc_kind != 0;
+	bool pre = pc_kind != 0 && !flex;
 	Pc pc(pc_kind, g, cfg);
 	std::vector<std::vector<double>> vv(m + 1, std::vector<double>(n));
+	std::vector<std::vector<double>> zz(flex ? m : 0, std::vector<double>(n));
 	std::vector<double> w(n), hh((m + 1) * m), cc(m), ss(m), grs(m + 1);
 	auto H = [&](int64_t i, int64_t j) -> double& { return hh[j * (m + 1) + i]; };
 	// gmres.rs:461-490.
@@ -1188,7 +1242,7 @@ int orc_gmres(double* x, const double* b, int64_t rows, int64_t cols, double dx,
 			if (std::fabs(diag) < 1e-30) break;
 			y[i] = s / diag;
 		}
-		for (int64_t i = 0; i < k; ++i) axpy(y[i], vv[i].data(), x, n);
+		for (int64_t i = 0; i < k; ++i) axpy(y[i], flex ? zz[i].data() : vv[i].data(), x, n);
 	};
 	int64_t total = 0;
 	for (;;) {
@@ -1215,7 +1269,13 @@ int orc_gmres(double* x, const double* b, int64_t rows, int64_t cols, double dx,
 		while (k < m) {
 			if (total >= max_iter) { build(k); set_stats(st, total, cur, MAX_ITERATIONS); return OK; }
 			if (hist) hist[total] = cur;
-			apply_op(*op, vv[k].data(), w.data(), g);
+			if (flex) {
+				fill(zz[k].data(), 0.0, n);
+				pc.apply(zz[k].data(), vv[k].data(), n);
+				apply_op(*op, zz[k].data(), w.data(), g);
+			} else {
+				apply_op(*op, vv[k].data(), w.data(), g);
+			}
 			if (pre) {
 				fill(vv[k + 1].data(), 0.0, n);
 				pc.apply(vv[k + 1].data(), w.data(), n);
@@ -1254,6 +1314,21 @@ int orc_gmres(double* x, const double* b, int64_t rows, int64_t cols, double dx,
 		if (cur < tol) { set_stats(st, total, cur, CONVERGED); return OK; }
 		if (total >= max_iter) { set_stats(st, total, cur, MAX_ITERATIONS); return OK; }
 	}
+}
+}  // namespace
+
+extern "C" {
+
+int orc_gmres(double* x, const double* b, int64_t rows, int64_t cols, double dx, double dy, const orc_op* op,
+              int64_t restart, int pc_kind, const orc_mg_cfg* cfg, double tol, int64_t max_iter, orc_stats* st,
+              double* hist) {
+	return gmres_impl(x, b, rows, cols, dx, dy, op, restart, pc_kind, cfg, tol, max_iter, st, hist, false);
+}
+
+int orc_fgmres(double* x, const double* b, int64_t rows, int64_t cols, double dx, double dy, const orc_op* op,
+               int64_t restart, int pc_kind, const orc_mg_cfg* cfg, double tol, int64_t max_iter, orc_stats* st,
+               double* hist) {
+	return gmres_impl(x, b, rows, cols, dx, dy, op, restart, pc_kind, cfg, tol, max_iter, st, hist, true);
 }
 
 int orc_num_threads() { return omp_get_max_threads(); }
