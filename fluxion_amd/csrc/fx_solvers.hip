@@ -842,6 +842,82 @@ struct FinPP {
 	}
 };
 
+// ── pipelined flexible CG around an FP32 multigrid (extension) ─────────
+// One fused reduction per iteration, as in pipelined_cg.rs:130-159, but with the preconditioner applied
+// to the current residual every iteration (u = M(r), w = A·u) instead of carried by recurrences, so an
+// inexact (FP32) M does not corrupt the recurrence: from (r·u, w·u, u·s_old, r·r) the flexible
+// β = −(u·s_old)/(p_old·s_old), p·s = w·u + β·(u·s_old) (A symmetric) and α = (r·u)/(p·s);
+// p = u + β·p, s = w + β·s (= A·p), x += α·p, r −= α·s.
+// w = A·u (identity boundary rows) with the four dots.
+struct EpiPF {
+	const double *r, *s;
+	double* w;
+	__device__ void interior(int64_t i, double av, double c, Acc* a) const {
+		w[i] = av;
+		const double ri = r[i];
+		acc_add(a[0], ri * c);
+		acc_add(a[1], av * c);
+		acc_add(a[2], c * s[i]);
+		acc_add(a[3], ri * ri);
+	}
+	__device__ void boundary(int64_t i, double c, Acc* a) const { interior(i, c, c, a); }
+};
+// EpiPF for the 5-point stencil traversal, which hands over S(u): A·u = fma(coeff, S(u), u).
+struct EpiPFs {
+	double coeff;
+	EpiPF e;
+	__device__ void interior(int64_t i, double lap, double c, Acc* a) const { e.interior(i, fma(coeff, lap, c), c, a); }
+	__device__ void boundary(int64_t i, double c, Acc* a) const { e.boundary(i, c, a); }
+};
+// The same reductions over a stored w (operators without a fused traversal).
+struct FDotPF {
+	const double *r, *u, *w, *s;
+	__device__ void load() {}
+	__device__ void operator()(size_t i, Acc* a) const {
+		const double ri = r[i], ui = u[i];
+		acc_add(a[0], ri * ui);
+		acc_add(a[1], w[i] * ui);
+		acc_add(a[2], ui * s[i]);
+		acc_add(a[3], ri * ri);
+	}
+};
+// Top of iteration k: ‖r‖ check (pipelined_cg.rs:134-141 order: budget, then tolerance), then α, β.
+struct FinPF {
+	SState* st;
+	bool first;
+	__device__ void operator()(Acc* o) const {
+		const double gamma = acc_value(o[0]), delta = acc_value(o[1]), eps = acc_value(o[2]), rr = acc_value(o[3]);
+		const int64_t k = first ? 0 : st->iter + 1;
+		st->iter = k;
+		st->rr = rr;
+		record(st, k, sqrt(rr));
+		if (k >= st->max_iter) { finish(st, k, sqrt(rr), FX_MAX_ITERATIONS); return; }
+		if (sqrt(rr) < st->tol) { finish(st, k, sqrt(rr), FX_CONVERGED); return; }
+		const double beta = first ? 0.0 : -eps / st->pap;
+		const double eta = first ? delta : delta + beta * eps;
+		if (!(fabs(eta) >= 1e-30)) { finish(st, k, sqrt(rr), FX_BREAKDOWN); return; }
+		st->beta = beta;
+		st->alpha = gamma / eta;
+		st->pap = eta;
+	}
+};
+// p = u + β·p; s = w + β·s; x += α·p; r −= α·s.
+struct FPFUpd {
+	const SState* st;
+	double *p, *s, *x, *r;
+	const double *u, *w;
+	double a, b;
+	__device__ void load() { a = st->alpha; b = st->beta; }
+	__device__ void operator()(size_t i) const {
+		const double pi = u[i] + b * p[i];
+		const double si = w[i] + b * s[i];
+		x[i] = fma(a, pi, x[i]);
+		r[i] = fma(-a, si, r[i]);
+		p[i] = pi;
+		s[i] = si;
+	}
+};
+
 // ── learned preconditioner pieces ─────────────────────────────────────
 // r1 = r − A·z (in place over the A·z buffer), with Σ r1².
 struct FLr1 {
@@ -1887,6 +1963,63 @@ fx_status solve_ppcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) 
 	return FX_OK;
 }
 
+// ── pipelined flexible CG with FP64 true-residual refinement (extension) ─
+// For an FP32 multigrid, whose rounding breaks the Ghysels–Vanroose recurrences (u = M·r carried by
+// u −= α·M·s) after a few digits.  Passes restart from x while the FP64 true residual is above the
+// tolerance and keeps halving, as solve_fcg does.
+fx_status solve_pfcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
+	fx_solver* s = c.s;
+	double *r = s->vec(0), *u = s->vec(1), *w = s->vec(2), *p = s->vec(3), *sv = s->vec(4);
+	FX_REQUIRE(r && u && w && p && sv, FX_ERR_ALLOC, "work vector allocation failed");
+	const bool vk = c.op->kind == FX_OP_VARK;
+	uint64_t used = 0;
+	fx_solve_stats st{};
+	double prev_tr = INFINITY;
+	for (int pass = 0; pass < 8; ++pass) {
+		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used)));
+		FX_CHECK(c.apply_op(x, w, nullptr));
+		FX_CHECK(c.ered<1>(FResid{b, w, r}, FinPcgInitRR{c.d}, "pfcg init", nullptr));
+		FX_CHECK(fx::k_fill(c.be, p, 0.0, c.n));
+		FX_CHECK(fx::k_fill(c.be, sv, 0.0, c.n));
+		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+			FX_CHECK(c.precond(u, r));
+			if (c.fused5 || vk) {
+				const int rpb = stream_rpb(c.rows, c.cols);
+				dim3 grid = fx::stream_grid(c.rows, c.cols, rpb);
+				FX_CHECK(fx::ensure_partials(c.be, (size_t)grid.x * grid.y * 4));
+				if (vk)
+					hipLaunchKernelGGL((k_stream_vk<4, VX, EpiPF>), grid, dim3(BLOCK), 0, c.st, VX{u, 0.0},
+					                   EpiPF{r, sv, w}, (const double*)c.op->k->ptr, c.op->sigma, c.op->coeff,
+					                   1.0 / (s->g.dx * s->g.dx), 1.0 / (s->g.dy * s->g.dy), c.rows, c.cols, rpb,
+					                   fx::red_scratch(c.be), (const SState*)c.d);
+				else
+					hipLaunchKernelGGL((k_stream_st<4, VX, EpiPFs>), grid, dim3(BLOCK), 0, c.st, VX{u, 0.0},
+					                   EpiPFs{c.op->coeff, EpiPF{r, sv, w}}, c.w9, c.rows, c.cols, rpb,
+					                   fx::red_scratch(c.be), (const SState*)c.d);
+				hipLaunchKernelGGL((k_finish<4, FinPF>), dim3(1), dim3(FIN_BLOCK), 0, c.st, fx::red_scratch(c.be),
+				                   grid.x * grid.y, FinPF{c.d, i == 0}, c.done);
+				FX_CHECK(check_launch("pfcg operator"));
+			} else {
+				FX_CHECK(c.apply_op(u, w));
+				FX_CHECK(c.ered<4>(FDotPF{r, u, w, sv}, FinPF{c.d, i == 0}, "pfcg dots"));
+			}
+			return c.eupd(FPFUpd{c.d, p, sv, x, r, u, w, 0.0, 0.0}, "pfcg update");
+		}, (int64_t)(s->max_iter - used)));
+		double tr = 0;
+		FX_CHECK(fetch_stats(s, &st));
+		FX_CHECK(true_residual(c, x, b, w, &tr));
+		used += st.iterations;
+		st.true_residual = tr;
+		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
+		if (!(tr < 0.5 * prev_tr)) break;
+		prev_tr = tr;
+	}
+	st.iterations = used;
+	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol)) st.reason = FX_MAX_ITERATIONS;
+	if (stats) *stats = st;
+	return FX_OK;
+}
+
 // ── BiCgStabSolver (bicgstab.rs:90-346) ───────────────────────────────
 fx_status solve_bicgstab(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	fx_solver* s = c.s;
@@ -2443,7 +2576,11 @@ fx_status fx_solver_solve_op(fx_solver* s, fx_buf* x, const fx_buf* b, const fx_
 	switch (s->kind) {
 	case FX_SOLVER_CG: return solve_cg(c, xp, bp, stats);
 	case FX_SOLVER_FCG: return solve_fcg(c, xp, bp, stats);
-	case FX_SOLVER_PIPELINED_CG: return pc ? solve_ppcg(c, xp, bp, stats) : solve_pipelined(c, xp, bp, stats);
+	case FX_SOLVER_PIPELINED_CG:
+		if (!pc) return solve_pipelined(c, xp, bp, stats);
+		// An FP32 multigrid gets the flexible pipelined form; exact preconditioners Ghysels–Vanroose.
+		if (pc->kind == FX_PC_MULTIGRID && ((fx_multigrid*)pc->handle)->f32) return solve_pfcg(c, xp, bp, stats);
+		return solve_ppcg(c, xp, bp, stats);
 	case FX_SOLVER_BICGSTAB: return solve_bicgstab(c, xp, bp, stats);
 	case FX_SOLVER_GMRES:
 	case FX_SOLVER_FGMRES: return solve_gmres(c, xp, bp, stats);

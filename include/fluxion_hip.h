@@ -281,7 +281,10 @@ const char* fx_solver_name(const fx_solver* s);
 /* LinearSolver::solve(x, b, coeff, stencil, bc) (linear_solver.rs:13-27). */
 fx_status fx_solver_solve(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff, const fx_stencil* st,
                           const fx_boundaries* bc, fx_solve_stats* stats);
-/* solve_preconditioned (cg.rs:178-256, bicgstab.rs:218-346, gmres.rs:294-454). */
+/* solve_preconditioned (cg.rs:178-256, bicgstab.rs:218-346, gmres.rs:294-454).  PipelinedCgSolver has no
+ * preconditioned form in the reference (pipelined_cg.rs:96-167); here an exact preconditioner runs the
+ * Ghysels–Vanroose recurrence and an FP32 multigrid runs the flexible single-reduction pipelined form
+ * (M applied to the current residual each iteration) inside FP64 true-residual refinement passes. */
 fx_status fx_solver_solve_preconditioned(fx_solver* s, fx_buf* x, const fx_buf* b, double coeff,
                                          const fx_stencil* st, const fx_boundaries* bc, const fx_precond* pc,
                                          fx_solve_stats* stats);

@@ -1114,12 +1114,81 @@ int orc_pipelined_cg(double* x, const double* b, int64_t rows, int64_t cols, dou
 // ── EXTENSION: preconditioned pipelined CG (Ghysels & Vanroose 2014,
 // Alg. 3).  u = M r, w = A u; per iteration one fused (γ,δ,‖r‖²) reduction,
 // m = M w, n = A m.
+}  // extern "C"
+
+namespace {
+// EXTENSION: pipelined flexible CG for an FP32 multigrid, inside FP64 true-residual refinement passes.
+// One reduction per iteration, the pipelined loop of pipelined_cg.rs:130-159 with the preconditioner
+// applied to the current residual (u = M(r), w = A·u): from (r·u, w·u, u·s_old, r·r) the flexible
+// β = −(u·s_old)/(p_old·s_old), p·s = w·u + β·(u·s_old) and α = (r·u)/(p·s); then p = u + β·p,
+// s = w + β·s (= A·p), x += α·p, r −= α·s.  A pass restarts from x while the true residual is above
+// tol and keeps halving (the refinement rule of the flexible CG).
+int pfcg(double* x, const double* b, const Grid& g, const orc_op* op, Pc& pc, double tol, int64_t max_iter,
+         orc_stats* st, double* hist) {
+	const int64_t N = g.len();
+	std::vector<double> r(N), u(N), w(N), p(N), s(N);
+	int64_t used = 0;
+	double prev_tr = INFINITY, rn = 0.0, tr = 0.0;
+	int reason = CONVERGED;
+	for (int pass = 0; pass < 8; ++pass) {
+		apply_op(*op, x, w.data(), g);
+		copy(b, r.data(), N);
+		axpy(-1.0, w.data(), r.data(), N);
+		fill(p.data(), 0.0, N);
+		fill(s.data(), 0.0, N);
+		double pap = 0.0;
+		int64_t k = 0;
+		reason = CONVERGED;
+		for (;; ++k) {
+			fill(u.data(), 0.0, N);
+			pc.apply(u.data(), r.data(), N);
+			apply_op(*op, u.data(), w.data(), g);
+			const double gamma = dot(r.data(), u.data(), N), delta = dot(w.data(), u.data(), N);
+			const double eps = dot(u.data(), s.data(), N);
+			rn = std::sqrt(dot(r.data(), r.data(), N));
+			if (hist) hist[used + k] = rn;
+			if (used + k >= max_iter) { reason = MAX_ITERATIONS; break; }
+			if (rn < tol) break;
+			const double beta = k == 0 ? 0.0 : -eps / pap;
+			const double eta = k == 0 ? delta : delta + beta * eps;
+			if (!(std::fabs(eta) >= 1e-30)) { reason = BREAKDOWN; break; }
+			const double alpha = gamma / eta;
+			for (int64_t j = 0; j < N; ++j) {
+				const double pj = u[j] + beta * p[j];
+				const double sj = w[j] + beta * s[j];
+				x[j] = std::fma(alpha, pj, x[j]);
+				r[j] = std::fma(-alpha, sj, r[j]);
+				p[j] = pj;
+				s[j] = sj;
+			}
+			pap = eta;
+		}
+		used += k;
+		apply_op(*op, x, w.data(), g);
+		std::vector<double> tres(b, b + N);
+		axpy(-1.0, w.data(), tres.data(), N);
+		tr = std::sqrt(dot(tres.data(), tres.data(), N));
+		if (reason != CONVERGED || tr < tol || used >= max_iter) break;
+		if (!(tr < 0.5 * prev_tr)) break;
+		prev_tr = tr;
+	}
+	if (reason == CONVERGED && !(tr < tol)) reason = MAX_ITERATIONS;
+	set_stats(st, used, rn, reason);
+	return OK;
+}
+}  // namespace
+
+extern "C" {
+
+// Preconditioned pipelined CG (EXTENSION; PipelinedCgSolver has no preconditioned form): Ghysels &
+// Vanroose 2014 Alg. 3 for an exact preconditioner; an FP32 multigrid gets pfcg above.
 int orc_ppcg(double* x, const double* b, int64_t rows, int64_t cols, double dx, double dy, const orc_op* op, int pc_kind,
              const orc_mg_cfg* cfg, double tol, int64_t max_iter, orc_stats* st, double* hist) {
 	Grid g = mkgrid(rows, cols, dx, dy);
 	if (int e = valid_grid(g)) return e;
 	int64_t N = g.len();
 	Pc pc(pc_kind, g, cfg);
+	if (pc.m32) return pfcg(x, b, g, op, pc, tol, max_iter, st, hist);
 	std::vector<double> r(N), u(N), w(N), m(N), nn(N), z(N), q(N), s(N), p(N);
 	apply_op(*op, x, s.data(), g);
 	copy(b, r.data(), N);

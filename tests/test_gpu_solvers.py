@@ -508,3 +508,48 @@ def test_vark_fcg_fused_mg(be, n):
     assert s.converged() and s.true_residual < tol
     assert abs(s.iterations - sr.iterations) <= IT_TOL
     assert rel_l2(x.to_vec_f64(), xr) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["poisson", "vark"])
+def test_pipelined_cg_fp32_multigrid_to_1e10(be, case):
+    # PipelinedCgSolver preconditioned by the fused FP32 V-cycle (the C2 configuration at 256²): the
+    # flexible single-reduction pipelined form inside FP64 true-residual refinement, to 1e-10 relative
+    # FP64 true residual, against the oracle's same algorithm and a tightly solved FP64 reference.
+    hip = H()
+    n = 256
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    f = gaussian_bumps(n).ravel()
+    fn = np.linalg.norm(f)
+    tol = 1e-10 * fn
+    if case == "poisson":
+        st = Stencil.poisson_minus_identity(h, h)
+        op_r = O.Op.stencil(1.0, st)
+        cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=-1.0)
+        mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                           fused=True, scale=-1.0, be=be)
+        op = hip.Operator.stencil(1.0, st, ZD)
+        keep = None
+    else:
+        k = grf_log_k(n).ravel()
+        op_r = O.Op.var_k(k)
+        cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=1.0, k=k)
+        keep = hip.Field.from_f64(g, k, be)
+        mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                           fused=True, scale=1.0, k=keep, be=be)
+        op = hip.Operator.var_k(keep)
+    xr, sr = O.ppcg(np.zeros_like(f), f, g, op_r, tol, 300, mg=cfg)
+    xt, _ = O.cg(np.zeros_like(f), f, g, op_r, 1e-14 * fn, 50000) if case == "poisson" else \
+        O.pcg(np.zeros_like(f), f, g, op_r, 1e-13 * fn, 500, mg=O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16,
+                                                                          scale=1.0, k=k))
+    x = hip.Field.zeros(g, be=be)
+    s = hip.PipelinedCgSolver(g, tol, 300, be=be).solve_op(x, hip.Field.from_f64(g, f, be), op, mg)
+    xg = x.to_vec_f64()
+    print(f"pipelined CG + FP32 MG ({case} 256²): gpu {s.iterations} its, oracle {sr.iterations} its, "
+          f"true rel res {s.true_residual / fn:.2e}, diff vs oracle {rel_l2(xg, xr):.2e}, "
+          f"err vs FP64 reference {rel_l2(xg, xt):.2e}")
+    assert s.converged() and sr.converged()
+    assert s.true_residual < tol
+    assert abs(s.iterations - sr.iterations) <= IT_TOL
+    assert rel_l2(xg, xr) < 1e-8
+    assert rel_l2(xg, xt) < 1e-7

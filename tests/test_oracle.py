@@ -619,3 +619,26 @@ def test_fgmres_estimate_is_true_residual(prec):
     true = np.linalg.norm(f - A @ x)
     assert true < 1.05 * tol and abs(true - s.residual) < 1e-3 * tol + 1e-6 * true
     assert rel_l2(x, xd) < 1e-8
+
+
+def test_pipelined_flexible_cg_matches_fcg_and_dense():
+    # With an FP32 multigrid the preconditioned pipelined CG runs its flexible single-reduction form;
+    # in exact arithmetic it is flexible CG(1) step for step, so its iteration counts equal the
+    # oracle's FCG with the same cycle (±1 for rounding), and a 16² solve is the dense solution.
+    for n in (16, 64, 128):
+        h = 1.0 / (n - 1)
+        g = Grid.square(n, h)
+        st = Stencil.poisson_minus_identity(h, h)
+        from fluxion_amd.inputs import gaussian_bumps
+        f = gaussian_bumps(n).ravel()
+        op = O.Op.stencil(1.0, st)
+        cfg = O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=4, precision=1, scale=-1.0)
+        tol = 1e-11 * np.linalg.norm(f)
+        xp, sp = O.ppcg(np.zeros_like(f), f, g, op, tol, 300, mg=cfg)
+        xf, sf = O.fcg(np.zeros_like(f), f, g, op, tol, 300, mg=cfg)
+        assert sp.converged() and sf.converged()
+        assert abs(sp.iterations - sf.iterations) <= 1, (n, sp.iterations, sf.iterations)
+        assert np.linalg.norm(f - O.apply_op(op, xp, g)) < tol
+        if n == 16:
+            A = np.stack([O.apply_op(op, e, g) for e in np.eye(n * n)], axis=1)
+            assert rel_l2(xp, np.linalg.solve(A, f)) < 1e-10
