@@ -67,12 +67,24 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # HBM bytes per launch of the probed level-0 kernels on the 4096² var-k workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by the same counters' reading of a copy of
 # 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
-# profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels.
+# profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels, and
+# only for the kernel sources the counters were read from: "sources_sha16" is the hash of
+# TRAFFIC_SOURCES at that run, and a bench whose sources differ reports traffic null.
+TRAFFIC_SOURCES = ("fluxion_amd/csrc/fx_device.hpp", "fluxion_amd/csrc/fx_mgfast.hpp", "fluxion_amd/csrc/fx_solvers.hip")
 MEASURED_TRAFFIC_4096 = {
+    "sources_sha16": None,
     "vc_restrict0": (277170 * 2 + 151884) * 1024,
     "vc_post0": (289092 * 2 + 74922) * 1024,
     "fcg_step": (244617 * 2 + 393248) * 1024,
 }
+
+
+def sources_sha16() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def main():
@@ -82,6 +94,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--c2-tol", type=float, default=1e-10, help="relative tolerance of the Poisson 4096^2 record")
+    ap.add_argument("--no-convdiff", action="store_true", help="skip the convection-diffusion 2048^2 record")
+    ap.add_argument("--no-reductions", action="store_true", help="skip the dot / dot2 / norm bandwidth record")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU baseline solves (after one warm-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cn", action="store_true", help="skip the Crank-Nicolson record")
@@ -243,8 +259,10 @@ def main():
         mp32.set_kernels(mgk)
         if mgk == MgKernels.WAVE and not args.no_autotune:
             mp32.autotune(5)
-        ppcg = hip.CgSolver(g, args.tol * fpn, max_iter, be=be)
-        pfcg = hip.FcgSolver(g, args.tol * fpn, max_iter, be=be)
+        ptol = args.c2_tol * fpn
+        ppcg = hip.CgSolver(g, ptol, max_iter, be=be)
+        pfcg = hip.FcgSolver(g, ptol, max_iter, be=be)
+        ppipe = hip.PipelinedCgSolver(g, ptol, max_iter, be=be)
 
         def p_plain():
             X.fill(0.0)
@@ -253,15 +271,97 @@ def main():
         def p_mixed():
             X.fill(0.0)
             return pfcg.solve_preconditioned(X, B, 1.0, st, bc, mp32)
+
+        def p_pipe():
+            X.fill(0.0)
+            return ppipe.solve_preconditioned(X, B, 1.0, st, bc, mp32)
         tp, sp, _ = timed(p_plain, max(1, args.steps // 2), 1)
         tm, sm, _ = timed(p_mixed, max(1, args.steps // 2), 1)
-        log(f"bench: poisson plain {sp.iterations} its {1e3 * tp:.2f} ms, mixed {sm.iterations} its {1e3 * tm:.2f} ms")
-        poisson = {"plain_fp64_mg_pcg": {"solves_per_s": 1.0 / tp, "iterations": sp.iterations,
-                                         "true_rel_residual": sp.true_residual / fpn},
-                   "mixed_fcg": {"solves_per_s": 1.0 / tm, "iterations": sm.iterations,
-                                 "true_rel_residual": sm.true_residual / fpn},
-                   "speedup_mixed_vs_plain": tp / tm}
+        tq, sq, _ = timed(p_pipe, max(1, args.steps // 2), 1)
+        log(f"bench: poisson to {args.c2_tol:g}: plain {sp.iterations} its {1e3 * tp:.2f} ms, mixed FCG {sm.iterations} its "
+            f"{1e3 * tm:.2f} ms, pipelined CG + FP32 MG {sq.iterations} its {1e3 * tq:.2f} ms")
+
+        def rec(t, s_):
+            return {"solves_per_s": 1.0 / t, "ms_per_solve": 1e3 * t, "iterations": s_.iterations,
+                    "reason": s_.reason.name, "true_rel_residual": s_.true_residual / fpn}
+        poisson = {"tol_rel": args.c2_tol, "operator": "reference form coeff=1, S=-Lap_h-I (zero Dirichlet)",
+                   "plain_fp64_mg_pcg": rec(tp, sp), "mixed_fcg": rec(tm, sm), "pipelined_cg_fp32_mg": rec(tq, sq),
+                   "speedup_mixed_vs_plain": tp / tm, "speedup_pipelined_vs_plain": tp / tq}
         del mp64, mp32
+
+    # ── upwind convection-diffusion 2048² (κ = 1e-3, streamfunction velocity, cell Péclet ≤ 50) ──
+    convdiff = None
+    if not args.no_convdiff:
+        from fluxion_amd.inputs import streamfunction_velocity
+        nd, kap = 2048, 1e-3
+        gd = Grid.square(nd, 1.0 / (nd - 1))
+        vxd, vyd, pe = streamfunction_velocity(nd, kap)
+        fd = gaussian_bumps(nd).ravel()
+        fdn = float(np.linalg.norm(fd))
+        VX, VY = hip.Field.from_f64(gd, vxd.ravel(), be), hip.Field.from_f64(gd, vyd.ravel(), be)
+        FD, XD = hip.Field.from_f64(gd, fd, be), hip.Field.zeros(gd, be=be)
+        opd = hip.Operator.conv_diff(VX, VY, kappa_scalar=-kap, coeff=1.0, sigma=0.0)
+        mgd = hip.Multigrid(gd, bc, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F32,
+                            fused=False, scale=1.0, vx=VX, vy=VY, kappa=-kap, be=be)
+        dtol = args.tol * fdn
+        convdiff = {"grid": nd, "kappa": kap, "tol_rel": args.tol,
+                    "operator": "-kappa*Lap_h u + v.grad^- u (reference upwind, ops.rs:707-731), zero Dirichlet",
+                    "mg": "FP32 V(2,2) Jacobi w=2/3 on the same upwind operator per level, linear transfers, coarsest 16",
+                    "peclet": {"min": float(pe.min()), "median": float(np.median(pe)), "max": float(pe.max()),
+                               "p10": float(np.percentile(pe, 10)), "p90": float(np.percentile(pe, 90))}}
+        for name, solver in (("bicgstab", hip.BiCgStabSolver(gd, dtol, 3000, be=be)),
+                             ("fgmres30", hip.FgmresSolver(gd, 30, dtol, 3000, be=be)),
+                             ("gmres30_left", hip.GmresSolver(gd, 30, dtol, 3000, be=be))):
+            def run(sv=solver):
+                XD.fill(0.0)
+                return sv.solve_op(XD, FD, opd, mgd)
+            t, s_, _ = timed(run, 2, 1)
+            convdiff[name] = {"solves_per_s": 1.0 / t, "ms_per_solve": 1e3 * t, "iterations": s_.iterations,
+                              "reason": s_.reason.name, "true_rel_residual": s_.true_residual / fdn}
+            log(f"bench: conv-diff 2048^2 {name}: {s_.iterations} its, {1e3 * t:.1f} ms/solve, "
+                f"true rel res {s_.true_residual / fdn:.2e}")
+            del solver
+        convdiff["gmres30_left"]["note"] = ("reference left preconditioning: stops on ||M^-1 r||, so the FP64 "
+                                            "true residual is not held to the tolerance")
+        del mgd, VX, VY, FD, XD
+
+    # ── Backend reductions at 2^24 points: dot, dot2, norm_l2 (block-reduction kernel, HIP events) ──
+    reductions = None
+    if not args.no_reductions:
+        reductions = {"points": n * n}
+        Y2 = hip.Field.from_f64(g, np.linspace(0.0, 1.0, n * n), be)
+        for name, fn_, bpp in (("dot", lambda: X.dot(F), 16), ("dot2", lambda: X.dot2(F, K, Y2), 32),
+                               ("norm_l2", lambda: X.norm_l2(), 8)):
+            for _ in range(3):
+                fn_()
+            be.probe(be.PROBE_REDUCE)
+            for _ in range(20):
+                fn_()
+            ms, cnt = be.probe_read()
+            be.probe(be.PROBE_OFF)
+            us = 1e3 * ms / cnt
+            gbs = bpp * n * n / (us * 1e-6) / 1e9
+            reductions[name] = {"us_per_launch": us, "bytes_per_point": bpp, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+            log(f"bench: {name} {us:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.0f}%)")
+        del Y2
+
+    # ── config 1: Poisson 256², unpreconditioned CgSolver to 1e-8 (the reference's CPU-runnable case) ──
+    c1 = None
+    n1 = 256
+    g1 = Grid.square(n1, 1.0 / (n1 - 1))
+    st1 = Stencil.poisson_minus_identity(1.0 / (n1 - 1), 1.0 / (n1 - 1))
+    f1 = gaussian_bumps(n1).ravel()
+    tol1 = args.tol * float(np.linalg.norm(f1))
+    B1, X1 = hip.Field.from_f64(g1, f1, be), hip.Field.zeros(g1, be=be)
+    cg1 = hip.CgSolver(g1, tol1, 5000, be=be)
+
+    def c1_solve():
+        X1.fill(0.0)
+        return cg1.solve(X1, B1, 1.0, st1, bc)
+    t1, s1, _ = timed(c1_solve, 20, 3)
+    c1 = {"grid": n1, "tol_rel": args.tol, "gpu_cg": {"solves_per_s": 1.0 / t1, "ms_per_solve": 1e3 * t1,
+                                                      "iterations": s1.iterations}}
+    log(f"bench: C1 Poisson 256^2 CG {s1.iterations} its, {1e3 * t1:.2f} ms/solve on the GPU")
 
     # ── Crank–Nicolson on variable k: 1024², warm-started steps (implicit.rs:178-222 recurrence) ──
     cn = None
@@ -371,21 +471,60 @@ def main():
         import oracle as O
         kk = k.ravel()
         cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, scale=1.0, k=kk)
-        t0 = time.perf_counter()
-        xr, sr = O.pcg(np.zeros_like(f), f, g, O.Op.var_k(kk), tol, max_iter, mg=cfg)
-        t_cpu = time.perf_counter() - t0
+
+        def cpu_solve():
+            return O.pcg(np.zeros_like(f), f, g, O.Op.var_k(kk), tol, max_iter, mg=cfg)
+        xr, sr = cpu_solve()   # warm-up
+        times = []
+        for _ in range(max(1, args.cpu_reps)):
+            t0 = time.perf_counter()
+            xr, sr = cpu_solve()
+            times.append(time.perf_counter() - t0)
+        t_cpu = float(np.median(times))
         cores = O.num_threads()
+        model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         nr = float(np.linalg.norm(xr))
         dp = float(np.linalg.norm(x_plain - xr) / nr)
         dm = float(np.linalg.norm(x_mixed - xr) / nr)
-        log(f"bench: CPU oracle plain MG-PCG {sr.iterations} its, {t_cpu:.1f} s on {cores} threads; "
-            f"GPU plain vs CPU rel diff {dp:.2e} ({s_plain.iterations} vs {sr.iterations} its), mixed {dm:.2e}")
-        cpu = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": cores, "kind": "port",
-               "sample": f"one full plain FP64 MG-PCG solve of the same {n}x{n} variable-coefficient problem "
-                         f"({sr.iterations} its)",
-               "iterations": sr.iterations, "gpu_plain_rel_diff": dp, "gpu_mixed_rel_diff": dm}
+        log(f"bench: CPU oracle plain MG-PCG {sr.iterations} its, median {t_cpu:.2f} s of {len(times)} on {cores} "
+            f"threads ({model}); GPU plain vs CPU rel diff {dp:.2e} ({s_plain.iterations} vs {sr.iterations} its), "
+            f"mixed {dm:.2e}")
+        # Config 1 on the CPU: the reference's own CPU-runnable case.
+        cfn = lambda: O.cg(np.zeros_like(f1), f1, g1, O.Op.stencil(1.0, st1), tol1, 5000)
+        xr1, sr1 = cfn()
+        t1s = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            xr1, sr1 = cfn()
+            t1s.append(time.perf_counter() - t0)
+        t1c = float(np.median(t1s))
+        c1["cpu_cg"] = {"solves_per_s": 1.0 / t1c, "ms_per_solve": 1e3 * t1c, "iterations": sr1.iterations,
+                        "cores": cores, "kind": "port", "sample": "median of 5 solves after one warm-up"}
+        c1["gpu_vs_cpu_rel_diff"] = float(np.linalg.norm(X1.to_vec_f64() - xr1) / np.linalg.norm(xr1))
+        log(f"bench: C1 CPU oracle CG {sr1.iterations} its, {1e3 * t1c:.1f} ms/solve; GPU vs CPU rel diff "
+            f"{c1['gpu_vs_cpu_rel_diff']:.2e}")
+        cpu = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": cores, "kind": "port", "cpu_model": model,
+               "sample": f"plain FP64 MG-PCG of the same {n}x{n} variable-coefficient problem ({sr.iterations} its): "
+                         f"median of {len(times)} solves after one warm-up",
+               "seconds": times, "iterations": sr.iterations, "gpu_plain_rel_diff": dp, "gpu_mixed_rel_diff": dm}
 
     kd = kernels[dom]
+    sha = sources_sha16()
+    traffic, traffic_note = None, None
+    if n != 4096 or args.mg_kernels != "wave":
+        traffic_note = "counter figures exist for the 4096^2 wave-kernel workload only"
+    elif MEASURED_TRAFFIC_4096["sources_sha16"] != sha:
+        traffic_note = f"kernel sources ({sha}) differ from those of the counter run: traffic not restated"
+    else:
+        traffic = MEASURED_TRAFFIC_4096.get(dom)
+        traffic_note = f"rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated, sources {sha}"
     out = {
         "metric": METRIC,
         "value": 1.0 / t_value,
@@ -414,13 +553,17 @@ def main():
             "speedup_mixed_vs_plain": t_plain / t_mixed,
         },
         "poisson_4096": poisson,
+        "convdiff_2048": convdiff,
+        "reductions_4096": reductions,
+        "c1_poisson_256": c1,
         "cn_vark_1024": cn,
         "explicit_diffusion_4096": explicit,
         "kernels": kernels,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["frac"],
-                     "traffic": MEASURED_TRAFFIC_4096.get(dom) if n == 4096 and args.mg_kernels == "wave" else None,
+                     "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "bytes_per_point": kd["bytes_per_point"],
                      "points_per_launch": n * n, "us_per_launch": kd["us_per_launch"]},
         "cpu_baseline": cpu,

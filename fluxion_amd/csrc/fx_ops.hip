@@ -136,7 +136,9 @@ template <int NS, class T>
 fx_status reduce_host(fx_backend* be, T term, size_t n, double* out, const char* what) {
 	unsigned nb = fx::red_blocks(n);
 	FX_CHECK(fx::ensure_partials(be, (size_t)nb * NS));
+	fx::probe_begin(be, FX_PROBE_REDUCE);
 	hipLaunchKernelGGL((k_reduce<NS, T>), dim3(nb), dim3(BLOCK), 0, be->stream, term, n, fx::red_scratch(be));
+	fx::probe_end(be, FX_PROBE_REDUCE);
 	hipLaunchKernelGGL((k_finish<NS, FinStore<NS>>), dim3(1), dim3(FIN_BLOCK), 0, be->stream, fx::red_scratch(be), nb,
 	                   FinStore<NS>{be->dres}, (const int*)nullptr);
 	FX_CHECK(fx::check_launch(what));

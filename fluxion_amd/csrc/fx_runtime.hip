@@ -218,7 +218,7 @@ void* fx_backend_stream(fx_backend* be) { return be ? (void*)be->stream : nullpt
 
 fx_status fx_probe_set(fx_backend* be, int32_t kind) {
 	FX_REQUIRE(be, FX_ERR_ARGUMENT, "backend is null");
-	FX_REQUIRE(kind >= FX_PROBE_OFF && kind <= FX_PROBE_STENCIL_AXPY, FX_ERR_ARGUMENT, "bad probe kind %d", kind);
+	FX_REQUIRE(kind >= FX_PROBE_OFF && kind <= FX_PROBE_REDUCE, FX_ERR_ARGUMENT, "bad probe kind %d", kind);
 	FX_HIP(hipStreamSynchronize(be->stream));
 	if (kind != FX_PROBE_OFF && be->probe_a.empty()) {
 		be->probe_a.resize(256);

@@ -55,7 +55,8 @@ class HipBackend:
         return self._lib.fx_backend_stream(self.h) or 0
 
     # ── kernel probe (fx_probe_set / fx_probe_read) ───────────────────
-    PROBE_OFF, PROBE_FCG_UPDATE, PROBE_FCG_STEP, PROBE_VC_POST0, PROBE_VC_RESTRICT0, PROBE_STENCIL_AXPY = range(6)
+    PROBE_OFF, PROBE_FCG_UPDATE, PROBE_FCG_STEP, PROBE_VC_POST0, PROBE_VC_RESTRICT0, PROBE_STENCIL_AXPY, \
+        PROBE_REDUCE = range(7)
 
     def probe(self, kind: int) -> None:
         """Time every launch of one kernel family with HIP events on this stream (0 = off)."""

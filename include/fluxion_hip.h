@@ -83,7 +83,8 @@ typedef enum {
 	FX_PROBE_FCG_STEP = 2,        /* FCG p update + A·p + p·Ap                        */
 	FX_PROBE_VC_POST0 = 3,        /* fused V-cycle, finest-level post kernel          */
 	FX_PROBE_VC_RESTRICT0 = 4,    /* fused V-cycle, finest-level residual+restriction */
-	FX_PROBE_STENCIL_AXPY = 5     /* Backend::stencil_axpy, 5-point streaming kernel  */
+	FX_PROBE_STENCIL_AXPY = 5,    /* Backend::stencil_axpy, 5-point streaming kernel  */
+	FX_PROBE_REDUCE = 6           /* dot / dot2 / norm / sum block-reduction kernel   */
 } fx_probe_kind;
 fx_status fx_probe_set(fx_backend* be, int32_t kind);
 fx_status fx_probe_read(fx_backend* be, double* total_ms, uint64_t* launches);
