@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--no-convdiff", action="store_true", help="skip the convection-diffusion 2048^2 record")
     ap.add_argument("--no-reductions", action="store_true", help="skip the dot / dot2 / norm bandwidth record")
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU baseline solves (after one warm-up)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-problem batched record")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cn", action="store_true", help="skip the Crank-Nicolson record")
@@ -115,6 +116,7 @@ def main():
               "(the library has no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
+    from fluxion_amd import _lib as L
     from fluxion_amd import hip
     from fluxion_amd.core import Boundaries, DType, Grid, Stencil, Transfer
     from fluxion_amd.inputs import gaussian_bumps, grf_log_k
@@ -345,6 +347,42 @@ def main():
             log(f"bench: {name} {us:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.0f}%)")
         del Y2
 
+    # ── batched variable-coefficient diffusion: 512 problems, sides log-uniform in [64, 1024] ──
+    batch = None
+    if not args.no_batch:
+        from fluxion_amd.inputs import SEED0, batch_sizes
+        sizes = batch_sizes(512)
+        bgrids = [Grid.square(m, 1.0 / (m - 1)) for m in sizes]
+        bks = [grf_log_k(m, seed=SEED0 + i).ravel() for i, m in enumerate(sizes)]
+        bfs = [gaussian_bumps(m, seed=SEED0 + 1000 + i).ravel() for i, m in enumerate(sizes)]
+        npts = int(sum(m * m for m in sizes))
+        BK, BB = be.upload(np.concatenate(bks)), be.upload(np.concatenate(bfs))
+        BX = be.upload(np.zeros(npts))
+        btols = np.array([args.tol * np.linalg.norm(v) for v in bfs])
+        t0 = time.perf_counter()
+        bsol = hip.BatchSolver(bgrids, BK, be=be)
+        t_create = time.perf_counter() - t0
+
+        # Every solve starts from x = 0, cleared on the device (a copy from a zero buffer).
+        zero = be.upload(np.zeros(npts))
+
+        def bsolve():
+            L.check(be._lib.fx_copy(be.h, zero.h, BX.h), "copy")
+            return bsol.solve(BX, BB, btols, max_iter)
+        tb, sb, _ = timed(bsolve, 3, 1)
+        bits = np.array([q.iterations for q in sb])
+        btr = np.array([q.true_residual for q in sb])
+        batch = {"problems": len(sizes), "points": npts, "side_min": int(min(sizes)), "side_max": int(max(sizes)),
+                 "tol_rel": args.tol, "create_s": t_create, "ms_per_batch_solve": 1e3 * tb,
+                 "batch_solves_per_s": 1.0 / tb, "problem_solves_per_s": len(sizes) / tb,
+                 "iterations": {"min": int(bits.min()), "mean": float(bits.mean()), "max": int(bits.max())},
+                 "all_converged": bool(all(q.converged() for q in sb)), "worst_true_res_over_tol": float(np.max(btr / btols)),
+                 "point_iterations_per_s": float(np.sum(np.array(sizes, dtype=np.float64) ** 2 * bits)) / tb,
+                 "info": bsol.info()}
+        log(f"bench: batch of {len(sizes)} ({npts / 1e6:.1f}M points): {1e3 * tb:.1f} ms per batch solve, "
+            f"its {bits.min()}..{bits.max()}, all converged {batch['all_converged']}")
+        del bsol, BK, BB, BX, zero
+
     # ── config 1: Poisson 256², unpreconditioned CgSolver to 1e-8 (the reference's CPU-runnable case) ──
     c1 = None
     n1 = 256
@@ -556,6 +594,7 @@ def main():
         "convdiff_2048": convdiff,
         "reductions_4096": reductions,
         "c1_poisson_256": c1,
+        "batch_512": batch,
         "cn_vark_1024": cn,
         "explicit_diffusion_4096": explicit,
         "kernels": kernels,
