@@ -131,7 +131,7 @@ fx_status noalloc_setup(fx_backend* be);
 fx_status k_dot_host(fx_backend* be, const double* x, const double* y, size_t n, double* out);
 
 // Network inference on a device FP32 input [B][H][W][cin]; returns the FP32 output
-// (channel 0 of a [B][H][W][16] array) in *y.  All kernels return early while *skip != 0.
+// ([B][H][W], channel 0 of the output layer) in *y.  All kernels return early while *skip != 0.
 fx_status unet_infer(fx_unet* u, const float* x, int B, int H, int W, const int* skip, const float** y);
 
 }  // namespace fx

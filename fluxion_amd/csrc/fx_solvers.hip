@@ -952,7 +952,7 @@ struct FPackIn {
 		x[2 * i + 1] = logk ? float(logk[i]) : 0.0f;
 	}
 };
-// c = s·h²·y on interior points, 0 on the boundary (y = channel 0 of the [p][16] network output).
+// c = s·h²·y on interior points, 0 on the boundary (y = the network output, one value per point).
 struct FNetC {
 	double* c;
 	const float* y;
@@ -963,7 +963,7 @@ struct FNetC {
 	__device__ void load() { sc = *s * h2; }
 	__device__ void operator()(size_t i) const {
 		int64_t r = (int64_t)i / cols, col = (int64_t)i % cols;
-		c[i] = (r == 0 || col == 0 || r == rows - 1 || col == cols - 1) ? 0.0 : sc * double(y[16 * i]);
+		c[i] = (r == 0 || col == 0 || r == rows - 1 || col == cols - 1) ? 0.0 : sc * double(y[i]);
 	}
 };
 // α* = (c·r1)/(c·Ac), the step along c that minimises the A-norm of the error left after the V-cycle
@@ -984,7 +984,7 @@ struct FAddStep {
 	__device__ void load() { al = *a; }
 	__device__ void operator()(size_t i) const { z[i] = fma(al, c[i], z[i]); }
 };
-// z += s·h²·y on interior points (y = channel 0 of the [p][16] network output).
+// z += s·h²·y on interior points (y = the network output, one value per point).
 struct FAddNet {
 	double* z;
 	const float* y;
@@ -996,7 +996,7 @@ struct FAddNet {
 	__device__ void operator()(size_t i) const {
 		int64_t r = (int64_t)i / cols, c = (int64_t)i % cols;
 		if (r == 0 || c == 0 || r == rows - 1 || c == cols - 1) return;
-		z[i] += sc * double(y[16 * i]);
+		z[i] += sc * double(y[i]);
 	}
 };
 

@@ -1,21 +1,29 @@
 // fx_unet.hip — learned preconditioner network on gfx950 (no reference counterpart).
 //
 // A 4-level U-Net (3×3 convolutions, 16/32/48/64 channels) with BF16 weights
-// and activations and FP32 accumulation on the matrix cores
-// (v_mfma_f32_16x16x16_bf16).  Layer order, parameter layout and loss are
-// those of oracle/unet.cpp (the FP64 checker):
+// and activations and FP32 accumulation on the matrix cores.  Layer order,
+// parameter layout and loss are those of oracle/unet.cpp (the FP64 checker):
 //   enc l=0..3: relu(conv3) ×2, skip, avgpool2 (l<3)
 //   dec l=2..0: up2(nearest) ‖ skip_l, relu(conv3) ×2;   out: conv3 → 1 channel
 // Activations are NHWC with the channel count padded to a multiple of 16.
 //
-// 3×3 convolution as an implicit GEMM: one wave computes 16 pixels of one
-// image row × all output channels; K runs over (tap, 16-channel block).
-// MFMA 16x16x16 operand layout (lane l, element j): A[m = l%16][k = 4(l/16)+j],
-// B[k = 4(l/16)+j][n = l%16], D[m = 4(l/16)+j][n = l%16].
+// 3×3 convolution (k_conv3m) as an implicit GEMM on gfx950's
+// v_mfma_f32_16x16x32_bf16: M = pixels, N = output channels, K = (tap, input
+// channel) flattened, k = tap·cin_p + c.  A workgroup stages the input tile of
+// its 4 output rows × 64 columns (plus the one-pixel halo, zero outside the
+// image) in LDS once, with 16-byte loads; each wave then computes one row of
+// 64 pixels × all output channels, so every A fragment feeds all N tiles and
+// every B fragment (weights, read through the cache) all four M tiles.
+// MFMA 16x16x32 operand layout (lane l, element j): A[m = l%16][k = 8(l/16)+j],
+// B[k = 8(l/16)+j][n = l%16], D[m = 4(l/16)+j][n = l%16].
 // The same kernel computes the data gradient with the transposed, flipped
-// weights.  Weight gradients reduce over pixels with the pixel index as K.
+// weights (its staging masks the FP32 gradient by the forward ReLU and rounds
+// it to BF16).  Inference fuses the decoder's upsample + concatenation into
+// the staging and writes only output channel 0.  Weight gradients reduce over
+// pixels with the pixel index as K.
 #include "fx_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -41,74 +49,227 @@ constexpr int CH[4] = {16, 32, 48, 64};
 
 inline int pad16(int c) { return (c + 15) / 16 * 16; }
 
-// Input kind of the conv kernel: BF16 activations, or FP32 gradients masked by
-// the ReLU of the forward output (mask != nullptr) and rounded to BF16.
+// Source of a convolution's staged input tile, and the form of its output.
+enum { SRC_BF16 = 0, SRC_UPCAT = 1, SRC_F32 = 2 };
+enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_F32_CH0 = 2 };
+
 struct ConvArgs {
-	const void* in;        // [B][H][W][cin_p]
-	const bf16* mask;      // [B][H][W][cin_p] or null
+	const void* in;        // SRC_BF16 bf16 [B][H][W][cin_p]; SRC_F32 f32 [B][H][W][cin_p];
+	                       // SRC_UPCAT bf16 [B][H/2][W/2][ca] (the coarser level, upsampled by r/2, c/2)
+	const bf16* in2;       // SRC_UPCAT: the skip, bf16 [B][H][W][cin_p − ca], channels ca.. of the tile
+	const bf16* mask;      // SRC_F32: forward output whose ReLU masks the gradient (bf16 [B][H][W][cin_p]), or null
 	const bf16* w;         // [9][cout_p][cin_p]
 	const float* bias;     // [cout] or null
-	void* out;             // [B][H][W][cout_p] (bf16 or f32)
-	int B, H, W, cin_p, cout_p, cout;
-	int in_f32, out_f32, relu, flip;
+	void* out;             // OUT_BF16 bf16 / OUT_F32 f32 [B][H][W][cout_p]; OUT_F32_CH0 f32 [B][H][W]
+	int B, H, W, cin_p, cout_p, cout, ca;
+	int relu, flip;
 	const int* skip;       // kernels return early while *skip != 0 (may be null)
 };
 
-__global__ __launch_bounds__(BLOCK) void k_conv3(ConvArgs a) {
+constexpr int CTW = 64, CTH = 4;   // output tile of a workgroup: CTH rows (one per wave) × CTW columns
+// LDS of a workgroup: the staged input tile, reused afterwards for the BF16 output transpose
+// (CTH waves × CTW pixels × cout_p).
+inline size_t conv_lds_bytes(int cin_p, int cout_p) {
+	const size_t in = (size_t)(CTH + 2) * (CTW + 2) * (cin_p + 8) * sizeof(bf16);
+	const size_t out = (size_t)CTH * CTW * cout_p * sizeof(bf16);
+	return in > out ? in : out;
+}
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+template <int SRC, int OUTK, int NT>
+__global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	if (a.skip && *a.skip) return;
-	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-	const int b = blockIdx.z, row = blockIdx.y;
-	const int col0 = blockIdx.x * 64 + wv * 16;
-	const int m = lane & 15, kq = lane >> 4;
-	const int nt = a.cout_p / 16, cb = a.cin_p / 16;   // nt <= 8 (cout_p <= 128, checked on the host)
-	f4 acc[8];
-	for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-	for (int tap = 0; tap < 9; ++tap) {
-		const int rr = row + tap / 3 - 1, cc = col0 + m + tap % 3 - 1;
-		const bool ok = rr >= 0 && rr < a.H && cc >= 0 && cc < a.W;
-		const size_t pix = ((size_t)b * a.H + (ok ? rr : 0)) * a.W + (ok ? cc : 0);
-		const int wt = a.flip ? 8 - tap : tap;
-		for (int c = 0; c < cb; ++c) {
-			const int ch = c * 16 + kq * 4;
-			s4 av = {0, 0, 0, 0};
-			if (ok) {
-				const size_t q = pix * a.cin_p + ch;
-				if (a.in_f32) {
-					const float* src = (const float*)a.in + q;
-					float v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
-					if (a.mask) {
-						const bf16* mk = a.mask + q;
-						v0 = bf2f(mk[0]) > 0.f ? v0 : 0.f;
-						v1 = bf2f(mk[1]) > 0.f ? v1 : 0.f;
-						v2 = bf2f(mk[2]) > 0.f ? v2 : 0.f;
-						v3 = bf2f(mk[3]) > 0.f ? v3 : 0.f;
-					}
-					av = s4{(short)f2bf(v0), (short)f2bf(v1), (short)f2bf(v2), (short)f2bf(v3)};
+	extern __shared__ __align__(16) bf16 tile[];
+	const int b = blockIdx.z, row0 = blockIdx.y * CTH, col0 = blockIdx.x * CTW;
+	// Pixel pitch cin_p + 8 keeps the 16 lanes of an A fragment (16 neighbouring pixels) off a
+	// power-of-two stride in LDS.
+	const int pitch = a.cin_p + 8;
+	constexpr int TPX = (CTH + 2) * (CTW + 2);
+	// One 16-byte item per (channel chunk, tile pixel); a thread loads SU items before storing any, so
+	// that SU global loads are in flight together.
+	constexpr int SU = 4;
+	auto fetch = [&](int item) -> u4v {
+		u4v v = {0u, 0u, 0u, 0u};
+		const int ch = (item / TPX) << 3, p = item - (item / TPX) * TPX;
+		const int pr = p / (CTW + 2), pc = p - pr * (CTW + 2);
+		const int rr = row0 - 1 + pr, cc = col0 - 1 + pc;
+		if (ch < a.cin_p && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W) {
+			const size_t pix = ((size_t)b * a.H + rr) * a.W + cc;
+			if constexpr (SRC == SRC_BF16) {
+				v = *(const u4v*)((const bf16*)a.in + pix * a.cin_p + ch);
+			} else if constexpr (SRC == SRC_UPCAT) {
+				if (ch < a.ca) {
+					const size_t pa = ((size_t)b * (a.H >> 1) + (rr >> 1)) * (a.W >> 1) + (cc >> 1);
+					v = *(const u4v*)((const bf16*)a.in + pa * a.ca + ch);
 				} else {
-					av = *(const s4*)((const bf16*)a.in + q);
+					v = *(const u4v*)(a.in2 + pix * (a.cin_p - a.ca) + (ch - a.ca));
 				}
+			} else {
+				const float* src = (const float*)a.in + pix * a.cin_p + ch;
+				const float4 lo = *(const float4*)src, hi = *(const float4*)(src + 4);
+				float f[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+				if (a.mask) {
+					const u4v mk = *(const u4v*)(a.mask + pix * a.cin_p + ch);
+#pragma unroll
+					for (int e = 0; e < 8; ++e) {
+						const bf16 mb = (bf16)(mk[e >> 1] >> ((e & 1) * 16));
+						if (!(bf2f(mb) > 0.f)) f[e] = 0.f;
+					}
+				}
+#pragma unroll
+				for (int e = 0; e < 4; ++e) v[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
 			}
-			for (int t = 0; t < 8; ++t) {
-				if (t >= nt) break;
-				const s4 bv = *(const s4*)(a.w + ((size_t)wt * a.cout_p + t * 16 + m) * a.cin_p + ch);
-				acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+		}
+		return v;
+	};
+	const int nitems = (a.cin_p >> 3) * TPX;
+	for (int base = threadIdx.x; base < nitems; base += SU * BLOCK) {
+		u4v v[SU];
+#pragma unroll
+		for (int u = 0; u < SU; ++u) v[u] = fetch(base + u * BLOCK);
+#pragma unroll
+		for (int u = 0; u < SU; ++u) {
+			const int item = base + u * BLOCK;
+			if (item < nitems) {
+				const int ch = (item / TPX) << 3, p = item - (item / TPX) * TPX;
+				*(u4v*)(tile + (size_t)p * pitch + ch) = v[u];
 			}
 		}
 	}
-	for (int t = 0; t < 8; ++t) {
-		if (t >= nt) break;
-		const int co = t * 16 + m;
-		for (int j = 0; j < 4; ++j) {
-			const int col = col0 + kq * 4 + j;
-			if (col >= a.W) continue;
-			float v = acc[t][j];
-			if (co < a.cout && a.bias) v += a.bias[co];
-			if (co >= a.cout) v = 0.f;
-			if (a.relu && v < 0.f) v = 0.f;
-			const size_t q = (((size_t)b * a.H + row) * a.W + col) * a.cout_p + co;
-			if (a.out_f32) ((float*)a.out)[q] = v;
-			else ((bf16*)a.out)[q] = f2bf(v);
+	__syncthreads();
+
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int m = lane & 15, g = lane >> 4;
+	f4 acc[4][NT];
+#pragma unroll
+	for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+		for (int t = 0; t < NT; ++t) acc[mt][t] = f4{0.f, 0.f, 0.f, 0.f};
+	// This lane's k = 8g + j (j = 0..7) within each 32-wide K step, as (tap, first channel c0).
+	int tap = 0, c0 = 8 * g;
+	while (c0 >= a.cin_p) { c0 -= a.cin_p; ++tap; }
+	const int K = 9 * a.cin_p;
+	const u4v zero = {0u, 0u, 0u, 0u};
+	// Weight fragments of the K step at (tap, c0); a step past 9·cin_p (the tail) contributes zero.
+	auto wload = [&](int tp, int c, u4v* bw) {
+		const bool ok = tp < 9;
+		const int wt = a.flip ? 8 - (ok ? tp : 0) : (ok ? tp : 0);
+#pragma unroll
+		for (int t = 0; t < NT; ++t) {
+			const u4v w = *(const u4v*)(a.w + ((size_t)(wt * a.cout_p + 16 * t + m)) * a.cin_p + c);
+			bw[t] = ok ? w : zero;
 		}
+	};
+	u4v bcur[NT];
+	wload(tap, c0, bcur);
+	for (int kk = 0; kk < K; kk += 32) {
+		const bool ok = tap < 9;
+		const int t9 = ok ? tap : 0;
+		const int dr = t9 / 3, dc = t9 - 3 * dr;
+		// The next step's (tap, c0), and its weights fetched while this step computes.
+		int ntap = tap, nc0 = c0 + 32;
+		while (nc0 >= a.cin_p) { nc0 -= a.cin_p; ++ntap; }
+		u4v bnext[NT];
+		wload(ntap, nc0, bnext);
+		u4v x[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(tile + ((size_t)(wv + dr) * (CTW + 2) + 16 * mt + m + dc) * pitch + c0);
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) {
+			const bf8 av = __builtin_bit_cast(bf8, ok ? x[mt] : zero);
+#pragma unroll
+			for (int t = 0; t < NT; ++t)
+				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, bcur[t]), acc[mt][t], 0, 0, 0);
+		}
+#pragma unroll
+		for (int t = 0; t < NT; ++t) bcur[t] = bnext[t];
+		tap = ntap;
+		c0 = nc0;
+	}
+
+	const int orow = row0 + wv;
+	auto value = [&](int mt, int t, int j) {
+		const int co = 16 * t + m;
+		float v = acc[mt][t][j];
+		if (co < a.cout && a.bias) v += a.bias[co];
+		if (co >= a.cout) v = 0.f;
+		if (a.relu && v < 0.f) v = 0.f;
+		return v;
+	};
+	if constexpr (OUTK == OUT_F32) {
+		// Training's data gradients: FP32 stores straight from the fragments (16 lanes write 64
+		// contiguous bytes).
+		if (orow >= a.H) return;
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+			for (int t = 0; t < NT; ++t)
+#pragma unroll
+				for (int j = 0; j < 4; ++j) {
+					const int col = col0 + 16 * mt + 4 * g + j;
+					if (col < a.W)
+						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * a.cout_p + 16 * t + m] = value(mt, t, j);
+				}
+	} else {
+		// The wave's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS
+		// (the input tile is no longer read once every wave has passed the barrier) and store 16 bytes
+		// per lane.
+		__syncthreads();
+		if constexpr (OUTK == OUT_BF16) {
+			bf16* st = tile + (size_t)wv * CTW * a.cout_p;
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+				for (int t = 0; t < NT; ++t)
+#pragma unroll
+					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * a.cout_p + 16 * t + m] = f2bf(value(mt, t, j));
+			__syncthreads();
+			if (orow >= a.H) return;
+			const int nvalid = min(CTW, a.W - col0);
+			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * a.cout_p;
+			const int nchunk = nvalid * a.cout_p / 8;
+			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
+		} else {
+			float* st = (float*)tile + (size_t)wv * CTW;
+			if (m == 0)
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+					for (int j = 0; j < 4; ++j) st[16 * mt + 4 * g + j] = value(mt, 0, j);
+			__syncthreads();
+			if (orow >= a.H) return;
+			const int col = col0 + lane;
+			if (col < a.W) ((float*)a.out)[((size_t)b * a.H + orow) * a.W + col] = st[lane];
+		}
+	}
+}
+
+template <int SRC, int OUTK, int NT>
+fx_status launch_conv_nt(const ConvArgs& a, hipStream_t st) {
+	static bool attr = false;   // raise the dynamic LDS limit once per instantiation
+	if (!attr) {
+		FX_HIP(hipFuncSetAttribute((const void*)k_conv3m<SRC, OUTK, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                           (int)conv_lds_bytes(128, 128)));
+		attr = true;
+	}
+	dim3 grid((unsigned)((a.W + CTW - 1) / CTW), (unsigned)((a.H + CTH - 1) / CTH), (unsigned)a.B);
+	hipLaunchKernelGGL((k_conv3m<SRC, OUTK, NT>), grid, dim3(BLOCK), conv_lds_bytes(a.cin_p, a.cout_p), st, a);
+	return fx::check_launch("unet conv");
+}
+template <int SRC, int OUTK>
+fx_status launch_conv(const ConvArgs& a, hipStream_t st) {
+	switch (a.cout_p / 16) {
+	case 1: return launch_conv_nt<SRC, OUTK, 1>(a, st);
+	case 2: return launch_conv_nt<SRC, OUTK, 2>(a, st);
+	case 3: return launch_conv_nt<SRC, OUTK, 3>(a, st);
+	case 4: return launch_conv_nt<SRC, OUTK, 4>(a, st);
+	case 5: return launch_conv_nt<SRC, OUTK, 5>(a, st);
+	case 6: return launch_conv_nt<SRC, OUTK, 6>(a, st);
+	case 7: return launch_conv_nt<SRC, OUTK, 7>(a, st);
+	case 8: return launch_conv_nt<SRC, OUTK, 8>(a, st);
+	default: fx::set_error("conv with %d output channels is not supported", a.cout_p); return FX_ERR_ARGUMENT;
 	}
 }
 
@@ -169,18 +330,37 @@ __global__ void k_bgrad(const float* dy, const bf16* y, int relu, float* db, lon
 	atomicAdd(&db[co], s);
 }
 
+// y[b][r][c][ch] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] + x[2r+1][2c+1])), 8 channels per
+// thread (16-byte loads and store); C is a multiple of 8.
 __global__ void k_pool2(const bf16* x, bf16* y, int B, int H, int W, int C, const int* skip) {
 	if (skip && *skip) return;
-	long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	int h = H / 2, w = W / 2;
-	long n = (long)B * h * w * C;
-	if (q >= n) return;
-	int c = q % C;
-	long p = q / C;
-	int cc = p % w, rr = (p / w) % h;
-	long b = p / ((long)w * h);
-	auto X = [&](int r, int s) { return bf2f(x[(((size_t)b * H + r) * W + s) * C + c]); };
-	y[q] = f2bf(0.25f * (X(2 * rr, 2 * cc) + X(2 * rr, 2 * cc + 1) + X(2 * rr + 1, 2 * cc) + X(2 * rr + 1, 2 * cc + 1)));
+	const int h = H / 2, w = W / 2, nch = C >> 3;
+	const long n = (long)B * h * w * nch;
+	for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+		const long p = q / nch;
+		const int ch = (int)(q - p * nch) << 3;
+		const int cc = (int)(p % w);
+		const long br = p / w;                 // b·h + rr
+		const int rr = (int)(br % h);
+		const long b = br / h;
+		const bf16* x0 = x + (((size_t)b * H + 2 * rr) * W + 2 * cc) * C + ch;
+		const u4v a0 = *(const u4v*)x0, a1 = *(const u4v*)(x0 + C);
+		const u4v a2 = *(const u4v*)(x0 + (size_t)W * C), a3 = *(const u4v*)(x0 + (size_t)W * C + C);
+		u4v o;
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			uint32_t r2 = 0;
+#pragma unroll
+			for (int half = 0; half < 2; ++half) {
+				const int sh = 16 * half;
+				const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) + bf2f((bf16)(a2[e] >> sh)) +
+				                         bf2f((bf16)(a3[e] >> sh)));
+				r2 |= (uint32_t)f2bf(v) << sh;
+			}
+			o[e] = r2;
+		}
+		*(u4v*)(y + (size_t)p * C + ch) = o;
+	}
 }
 // dx (f32, [B][H][W][C]) += 0.25·dy[B][H/2][W/2][C]
 __global__ void k_pool2_bwd(const float* dy, float* dx, int B, int H, int W, int C) {
@@ -323,20 +503,22 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, long n, flo
 	v[q] = vv;
 	p[q] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
-// NHWC f32 input [B][H][W][cin] -> BF16 [B][H][W][cin_p].
+// NHWC f32 input [B][H][W][cin] -> BF16 [B][H][W][cin_p], one pixel per thread (16-byte stores).
 __global__ void k_pack_in(const float* x, bf16* out, long npix, int cin, int cin_p, const int* skip) {
 	if (skip && *skip) return;
-	long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= npix * cin_p) return;
-	int c = q % cin_p;
-	long p = q / cin_p;
-	out[q] = c < cin ? f2bf(x[p * cin + c]) : (bf16)0;
-}
-
-// y[p] = src[p·stride] (channel 0 of the padded network output).
-__global__ void k_extract(const float* src, int stride, float* y, long n) {
-	long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (q < n) y[q] = src[q * stride];
+	for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (long)gridDim.x * blockDim.x) {
+		for (int c8 = 0; c8 < cin_p; c8 += 8) {
+			u4v v;
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				const int c = c8 + 2 * e;
+				const bf16 lo = c < cin ? f2bf(x[p * cin + c]) : (bf16)0;
+				const bf16 hi = c + 1 < cin ? f2bf(x[p * cin + c + 1]) : (bf16)0;
+				v[e] = (uint32_t)lo | ((uint32_t)hi << 16);
+			}
+			*(u4v*)(out + p * cin_p + c8) = v;
+		}
+	}
 }
 
 unsigned nb(long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -357,7 +539,8 @@ struct fx_unet {
 	const int* skip = nullptr;   // set for inference inside a solver
 	std::vector<void*> ws;
 	bf16 *xin = nullptr, *act[NLAYER] = {}, *cat[3] = {}, *pool[3] = {};
-	float* yout = nullptr;   // layer-14 output, FP32 [B][H][W][16]
+	float* yout = nullptr;   // layer-14 output while training, FP32 [B][H][W][16]
+	float* ych0 = nullptr;   // channel 0 of the layer-14 output in inference, FP32 [B][H][W]
 	float *g1 = nullptr, *g2 = nullptr;   // gradient ping-pong, sized for the largest activation
 	float *gskip[3] = {};
 	float* sums = nullptr;
@@ -400,6 +583,7 @@ fx_status ensure_ws(fx_unet* u, int B, int H, int W) {
 		FX_CHECK(alloc(u, npix(l) * CH[l] * 4, &p)); u->gskip[l] = (float*)p;
 	}
 	FX_CHECK(alloc(u, npix(0) * 16 * 4, &p)); u->yout = (float*)p;
+	FX_CHECK(alloc(u, npix(0) * 4, &p)); u->ych0 = (float*)p;
 	size_t gmax = npix(0) * 128;
 	FX_CHECK(alloc(u, gmax * 4, &p)); u->g1 = (float*)p;
 	FX_CHECK(alloc(u, gmax * 4, &p)); u->g2 = (float*)p;
@@ -417,10 +601,14 @@ fx_status pack_weights(fx_unet* u) {
 	return fx::check_launch("unet pack weights");
 }
 
-fx_status conv(fx_unet* u, int li, const void* in, int in_f32, const bf16* mask, void* out, int out_f32, int l,
-               bool data_grad) {
+// One convolution of layer li at level l: the forward (weights Wp, bias, ReLU on all but the output
+// layer) or, with data_grad, the data gradient (the transposed, flipped weights Wt).
+fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int ca, const bf16* mask, void* out,
+               int outk, int l, bool data_grad) {
 	ConvArgs a;
 	a.in = in;
+	a.in2 = in2;
+	a.ca = ca;
 	a.mask = mask;
 	a.B = u->B;
 	a.H = u->H >> l;
@@ -445,42 +633,55 @@ fx_status conv(fx_unet* u, int li, const void* in, int in_f32, const bf16* mask,
 	FX_REQUIRE(a.cout_p <= 128 && a.cin_p <= 128, FX_ERR_ARGUMENT, "conv channel count out of range");
 	a.out = out;
 	a.skip = data_grad ? nullptr : u->skip;
-	a.in_f32 = in_f32;
-	a.out_f32 = out_f32;
-	dim3 grid((unsigned)((a.W + 63) / 64), (unsigned)a.H, (unsigned)a.B);
-	hipLaunchKernelGGL(k_conv3, grid, dim3(BLOCK), 0, u->be->stream, a);
-	return fx::check_launch("unet conv");
+	hipStream_t st = u->be->stream;
+	if (src == SRC_BF16 && outk == OUT_BF16) return launch_conv<SRC_BF16, OUT_BF16>(a, st);
+	if (src == SRC_UPCAT && outk == OUT_BF16) return launch_conv<SRC_UPCAT, OUT_BF16>(a, st);
+	if (src == SRC_BF16 && outk == OUT_F32) return launch_conv<SRC_BF16, OUT_F32>(a, st);
+	if (src == SRC_BF16 && outk == OUT_F32_CH0) return launch_conv<SRC_BF16, OUT_F32_CH0>(a, st);
+	if (src == SRC_F32 && outk == OUT_F32) return launch_conv<SRC_F32, OUT_F32>(a, st);
+	fx::set_error("unsupported conv form (source %d, output %d)", src, outk);
+	return FX_ERR_UNSUPPORTED;
 }
 
-// Forward over the packed input u->xin; result in u->yout (channel 0 of 16).
-fx_status forward(fx_unet* u) {
+// Forward over the packed input u->xin.  Training (infer false) keeps what the backward needs: the
+// decoder's concatenations materialised in u->cat and the output layer's 16 FP32 channels in u->yout.
+// Inference upsamples and concatenates inside the decoder convs' staging and writes output channel 0
+// only, to y ([B][H][W] FP32).
+fx_status forward(fx_unet* u, bool infer, float* y) {
 	hipStream_t st = u->be->stream;
 	const bf16* h = u->xin;
 	int li = 0;
 	for (int l = 0; l < 4; ++l) {
 		for (int k = 0; k < 2; ++k, ++li) {
-			FX_CHECK(conv(u, li, h, 0, nullptr, u->act[li], 0, l, false));
+			FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
 			h = u->act[li];
 		}
 		if (l < 3) {
-			long n = (long)u->B * (u->H >> (l + 1)) * (u->W >> (l + 1)) * CH[l];
-			hipLaunchKernelGGL(k_pool2, dim3(nb(n)), dim3(256), 0, st, h, u->pool[l], u->B, u->H >> l, u->W >> l, CH[l],
-			                   u->skip);
+			long n = (long)u->B * (u->H >> (l + 1)) * (u->W >> (l + 1)) * (CH[l] / 8);
+			hipLaunchKernelGGL(k_pool2, dim3((unsigned)std::min<long>(nb(n), 65536)), dim3(256), 0, st, h, u->pool[l],
+			                   u->B, u->H >> l, u->W >> l, CH[l], u->skip);
 			h = u->pool[l];
 		}
 	}
 	for (int l = 2; l >= 0; --l) {
 		int ca = CH[l + 1], cs = CH[l], cop = pad16(ca + cs);
-		long n = (long)u->B * (u->H >> l) * (u->W >> l) * cop;
-		hipLaunchKernelGGL(k_upcat, dim3(nb(n)), dim3(256), 0, st, h, ca, (const bf16*)u->act[2 * l + 1], cs, u->cat[l],
-		                   cop, u->B, u->H >> l, u->W >> l, u->skip);
-		h = u->cat[l];
-		for (int k = 0; k < 2; ++k, ++li) {
-			FX_CHECK(conv(u, li, h, 0, nullptr, u->act[li], 0, l, false));
-			h = u->act[li];
+		const bf16* skipv = u->act[2 * l + 1];
+		if (infer) {
+			FX_CHECK(conv(u, li, SRC_UPCAT, h, skipv, ca, nullptr, u->act[li], OUT_BF16, l, false));
+		} else {
+			long n = (long)u->B * (u->H >> l) * (u->W >> l) * cop;
+			hipLaunchKernelGGL(k_upcat, dim3(nb(n)), dim3(256), 0, st, h, ca, skipv, cs, u->cat[l], cop, u->B, u->H >> l,
+			                   u->W >> l, u->skip);
+			FX_CHECK(conv(u, li, SRC_BF16, u->cat[l], nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
 		}
+		h = u->act[li];
+		++li;
+		FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
+		h = u->act[li];
+		++li;
 	}
-	FX_CHECK(conv(u, li, h, 0, nullptr, u->yout, 1, 0, false));
+	if (infer) FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, y, OUT_F32_CH0, 0, false));
+	else FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->yout, OUT_F32, 0, false));
 	return fx::check_launch("unet forward");
 }
 
@@ -522,12 +723,12 @@ fx_status backward(fx_unet* u) {
 	FX_CHECK(wgrad(li, u->dy, nullptr, 0, lin[li], 0));
 	float* g = u->g1;
 	float* gn = u->g2;
-	FX_CHECK(conv(u, li, u->dy, 1, nullptr, g, 1, 0, true));   // d(act13) (16 ch)
+	FX_CHECK(conv(u, li, SRC_F32, u->dy, nullptr, 0, nullptr, g, OUT_F32, 0, true));   // d(act13) (16 ch)
 	--li;
 	for (int l = 0; l <= 2; ++l) {
 		for (int k = 0; k < 2; ++k, --li) {
 			FX_CHECK(wgrad(li, g, u->act[li], 1, lin[li], l));
-			FX_CHECK(conv(u, li, g, 1, u->act[li], gn, 1, l, true));
+			FX_CHECK(conv(u, li, SRC_F32, g, nullptr, 0, u->act[li], gn, OUT_F32, l, true));
 			std::swap(g, gn);
 		}
 		// g = d(cat_l) [B][H_l][W_l][pad16(ca+cs)] -> d(a) at level l+1, d(skip_l) accumulated.
@@ -549,7 +750,7 @@ fx_status backward(fx_unet* u) {
 		for (int k = 0; k < 2; ++k, --li) {
 			FX_CHECK(wgrad(li, g, u->act[li], 1, lin[li], l));
 			if (li > 0) {
-				FX_CHECK(conv(u, li, g, 1, u->act[li], gn, 1, l, true));
+				FX_CHECK(conv(u, li, SRC_F32, g, nullptr, 0, u->act[li], gn, OUT_F32, l, true));
 				std::swap(g, gn);
 			}
 		}
@@ -641,14 +842,15 @@ fx_status fx_unet_get_grad(fx_unet* u, float* host, size_t n) {
 	return FX_OK;
 }
 
-static fx_status run_forward(fx_unet* u, const float* x, int32_t B, int32_t H, int32_t W) {
+// Packs x (FP32 [B][H][W][cin]) and runs the forward; inference writes channel 0 to y.
+static fx_status run_forward(fx_unet* u, const float* x, int32_t B, int32_t H, int32_t W, bool infer, float* y) {
 	FX_REQUIRE(u && x && B > 0 && H % 8 == 0 && W % 8 == 0 && H > 0 && W > 0, FX_ERR_ARGUMENT,
 	           "batch > 0 and H, W divisible by 8 required");
 	FX_CHECK(ensure_ws(u, B, H, W));
 	long npix = (long)B * H * W;
-	hipLaunchKernelGGL(k_pack_in, dim3(nb(npix * pad16(u->cin))), dim3(256), 0, u->be->stream, x, u->xin, npix, u->cin,
-	                   pad16(u->cin), u->skip);
-	return forward(u);
+	hipLaunchKernelGGL(k_pack_in, dim3((unsigned)std::min<long>(nb(npix), 65536)), dim3(256), 0, u->be->stream, x, u->xin,
+	                   npix, u->cin, pad16(u->cin), u->skip);
+	return forward(u, infer, y ? y : u->ych0);
 }
 
 
@@ -665,11 +867,7 @@ static fx_status check_tensors(fx_unet* u, const fx_buf* x, const fx_buf* y, int
 // x: FP32 [B][H][W][cin]; y: FP32 [B][H][W].
 fx_status fx_unet_forward(fx_unet* u, const fx_buf* xb, fx_buf* yb, int32_t B, int32_t H, int32_t W) {
 	FX_CHECK(check_tensors(u, xb, yb, B, H, W));
-	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W));
-	long npix = (long)B * H * W;
-	hipLaunchKernelGGL(k_extract, dim3(nb(npix)), dim3(256), 0, u->be->stream, (const float*)u->yout, 16,
-	                   (float*)yb->ptr, npix);
-	return check_launch("unet forward");
+	return run_forward(u, (const float*)xb->ptr, B, H, W, true, (float*)yb->ptr);
 }
 
 // Loss Σ_b ‖y_b − t_b‖²/‖t_b‖² and its gradient (kept on the device; fx_unet_get_grad reads it).
@@ -677,7 +875,7 @@ fx_status fx_unet_loss_grad(fx_unet* u, const fx_buf* xb, const fx_buf* tb, int3
                             double* loss) {
 	FX_CHECK(check_tensors(u, xb, tb, B, H, W));
 	const float* t = (const float*)tb->ptr;
-	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W));
+	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W, false, nullptr));
 	hipStream_t st = u->be->stream;
 	long hw = (long)H * W;
 	FX_HIP(hipMemsetAsync(u->sums, 0, (size_t)B * 2 * 4, st));
@@ -701,7 +899,7 @@ fx_status fx_unet_loss_grad_energy(fx_unet* u, const fx_buf* xb, const fx_buf* t
 	FX_CHECK(check_tensors(u, xb, tb, B, H, W));
 	FX_REQUIRE(u->cin >= 2, FX_ERR_ARGUMENT, "the energy loss reads log k from input channel 1");
 	const float* t = (const float*)tb->ptr;
-	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W));
+	FX_CHECK(run_forward(u, (const float*)xb->ptr, B, H, W, false, nullptr));
 	hipStream_t st = u->be->stream;
 	long hw = (long)H * W;
 	FX_HIP(hipMemsetAsync(u->sums, 0, (size_t)B * 2 * 4, st));
@@ -737,9 +935,9 @@ namespace fx {
 fx_status unet_infer(fx_unet* u, const float* x, int B, int H, int W, const int* skip, const float** y) {
 	FX_REQUIRE(H % 8 == 0 && W % 8 == 0, FX_ERR_DIMENSION, "U-Net image sides must be multiples of 8, got %dx%d", H, W);
 	u->skip = skip;
-	fx_status s = run_forward(u, x, B, H, W);
+	fx_status s = run_forward(u, x, B, H, W, true, nullptr);
 	u->skip = nullptr;
-	*y = u->yout;
+	*y = u->ych0;
 	return s;
 }
 }  // namespace fx
