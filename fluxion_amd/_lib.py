@@ -177,6 +177,7 @@ _SIGS = {
                                     _P(C.c_void_p)]),
     "fx_learned_destroy": (None, [C.c_void_p]),
     "fx_learned_fell_back": (C.c_int32, [C.c_void_p]),
+    "fx_learned_set_schedule": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fx_cn_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, _P(FxBoundaries), C.c_double,
                                C.c_uint64, _P(FxCnOptions), _P(C.c_void_p)]),
     "fx_cn_destroy": (None, [C.c_void_p]),
@@ -187,6 +188,7 @@ _SIGS = {
     "fx_cn_sim_time": (C.c_double, [C.c_void_p]),
     "fx_cn_solver": (C.c_void_p, [C.c_void_p]),
     "fx_cn_multigrid": (C.c_void_p, [C.c_void_p]),
+    "fx_cn_set_preconditioner": (C.c_int, [C.c_void_p, C.POINTER(FxPrecond)]),
     "fx_imex_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, C.c_double, C.c_double,
                                  _P(FxBoundaries), C.c_double, C.c_uint64, _P(FxMgOptions), _P(C.c_void_p)]),
     "fx_imex_destroy": (None, [C.c_void_p]),

@@ -29,6 +29,8 @@ struct fx_cn {
 	fx_solver* solver = nullptr;
 	fx_multigrid* mg = nullptr;
 	fx_buf* rhs = nullptr;
+	fx_precond user_pc{};      // fx_cn_set_preconditioner: used instead of the stepper's multigrid
+	bool has_user_pc = false;
 };
 
 // DiffusionSolver (solver.rs:25-48).  Explicit steps write a second buffer
@@ -399,8 +401,9 @@ fx_status fx_cn_step(fx_cn* cn, fx_buf* u, const fx_buf* source, fx_solve_stats*
 	// (I − (dt/2)·α·Δ) u^{n+1} = RHS from u^n   (implicit.rs:199-218)
 	const fx_operator im = cn_operator(cn, true);
 	fx_precond pc{FX_PC_MULTIGRID, 0, cn->mg};
+	const fx_precond* use = cn->has_user_pc ? &cn->user_pc : cn->mg ? &pc : nullptr;
 	fx_solve_stats st{};
-	FX_CHECK(fx_solver_solve_op(cn->solver, u, cn->rhs, &im, cn->mg ? &pc : nullptr, &st));
+	FX_CHECK(fx_solver_solve_op(cn->solver, u, cn->rhs, &im, use, &st));
 	cn->steps += 1;
 	if (stats) *stats = st;
 	return FX_OK;
@@ -414,6 +417,19 @@ fx_status fx_cn_step_n(fx_cn* cn, fx_buf* u, uint64_t n) {
 double fx_cn_dt(const fx_cn* cn) { return cn ? cn->dt : 0.0; }
 uint64_t fx_cn_steps_done(const fx_cn* cn) { return cn ? cn->steps : 0; }
 double fx_cn_sim_time(const fx_cn* cn) { return cn ? (double)cn->steps * cn->dt : 0.0; }
+
+fx_status fx_cn_set_preconditioner(fx_cn* cn, const fx_precond* pc) {
+	FX_REQUIRE(cn, FX_ERR_ARGUMENT, "solver is null");
+	if (!pc) {
+		cn->has_user_pc = false;
+		return FX_OK;
+	}
+	FX_REQUIRE(pc->kind != FX_PC_LEARNED || cn->solver_kind == FX_SOLVER_FCG, FX_ERR_ARGUMENT,
+	           "the learned preconditioner varies from step to step: it needs the flexible CG stepper");
+	cn->user_pc = *pc;
+	cn->has_user_pc = true;
+	return FX_OK;
+}
 
 fx_solver* fx_cn_solver(fx_cn* cn) { return cn ? cn->solver : nullptr; }
 fx_multigrid* fx_cn_multigrid(fx_cn* cn) { return cn ? cn->mg : nullptr; }

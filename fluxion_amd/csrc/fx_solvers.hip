@@ -52,10 +52,14 @@ struct SState {
 	int32_t pad2;
 	double gbeta, gcur, ghk;
 	double *H, *cc, *ss, *grs, *y;
-	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once
-	// ‖r‖² fails to drop below stall2·(previous ‖r‖²).
-	int32_t lmode, lplain, lskip, pad3;
+	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once the
+	// FP64 true residual ‖b − A·x‖² (ltres, evaluated after every x/r update while the network is in
+	// use) fails to drop below stall2·(its previous value).  lgate = 1 skips the network at this
+	// application (schedule: every `every`-th iteration, at most `max_apps` times; lapps counts them).
+	int32_t lmode, lplain, lskip, lgate;
 	double stall2;
+	double ltres;
+	int32_t lapps, pad3;
 	// Host-visible copy of `done` (coherent pinned memory), set together with it so that the host's
 	// enqueue loop can stop without a per-iteration device-to-host copy.
 	int* hdone;
@@ -657,10 +661,6 @@ struct FinPcgRR {
 	SState* st;
 	__device__ void operator()(Acc* o) const {
 		double rr = acc_value(o[0]);
-		if (st->lmode && !st->lplain && rr > st->stall2 * st->rr) {
-			st->lplain = 1;
-			st->lskip = 1;
-		}
 		st->rr = rr;
 		int64_t it = st->iter + 1;
 		st->iter = it;
@@ -703,13 +703,37 @@ __global__ __launch_bounds__(BLOCK) void k_xflush(const SState* st, double* __re
 		((double2*)x)[j] = double2{fma(al, pv.x, xv.x), fma(al, pv.y, xv.y)};
 	}
 }
-// PCG init: r = b − Ax with r·r (no check yet: cg.rs:188-202 computes z first).
+// PCG init: r = b − Ax with r·r (no check yet: cg.rs:188-202 computes z first).  r is the true
+// residual here: it also starts the learned preconditioner's true-residual record.
 struct FinPcgInitRR {
 	SState* st;
 	__device__ void operator()(Acc* o) const {
 		st->rr = acc_value(o[0]);
+		st->ltres = st->rr;
 		st->beta = 0.0;
 		st->iter = 0;
+	}
+};
+// Learned preconditioner's fallback test on the FP64 true residual ‖b − A·x‖² after an x/r update.
+struct FinLearnJudge {
+	SState* st;
+	__device__ void operator()(Acc* o) const {
+		const double tr = acc_value(o[0]);
+		if (!(tr < st->stall2 * st->ltres)) {
+			st->lplain = 1;
+			st->lskip = 1;
+		}
+		st->ltres = tr;
+	}
+};
+// Whether the network runs at this preconditioner application (lgate = 0) or only the V-cycle.
+struct FinLearnGate {
+	SState* st;
+	int every, max_apps;
+	__device__ void operator()() const {
+		const bool on = !st->done && !st->lplain && st->iter % every == 0 && (max_apps <= 0 || st->lapps < max_apps);
+		st->lgate = on ? 0 : 1;
+		if (on) st->lapps += 1;
 	}
 };
 // PCG: rz_old = r·z at start, check ‖r‖ (cg.rs:202-211).
@@ -1449,6 +1473,8 @@ struct fx_learned {
 	double* c = nullptr;       // the network correction
 	double* ac = nullptr;      // A·c
 	float* xin = nullptr;
+	double* tr = nullptr;      // b − A·x of the fallback test
+	int every = 1, max_apps = 0;
 	int fell_back = 0;
 };
 
@@ -1552,9 +1578,12 @@ struct Ctx {
 	fx_status precond(double* z, const double* r) { return precond(z, r, skip); }
 
 	// z = V(r), then the network correction unless *lsk is set.
-	fx_status learned(fx_learned* L, double* z, const double* r, const int* sk, const int* lsk) {
+	fx_status learned(fx_learned* L, double* z, const double* r, const int* sk, const int* lskip) {
+		(void)lskip;
 		FX_CHECK(eupd(FZero{z}, "pc zero", sk));
 		FX_CHECK(fx::mg_apply(L->mg, z, r, sk));
+		FX_CHECK(scalar(FinLearnGate{d, L->every, L->max_apps}, "learned gate", nullptr));
+		const int* lsk = &d->lgate;
 		FX_CHECK(apply_op(z, L->r1, lsk));
 		FX_CHECK(ered<1>(FLr1{r, L->r1}, FinScale{L->s_dev, (double)n}, "learned r1", lsk));
 		hipLaunchKernelGGL(k_eupd<FPackIn>, dim3(eb()), dim3(BLOCK), 0, st,
@@ -1887,6 +1916,13 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 				double* pnew = pp[(i + 1) & 1];
 				FX_CHECK(c.cg_step1(z, pold, pnew, ap));
 				FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "fcg x/r update"));
+				if (c.pc && c.pc->kind == FX_PC_LEARNED) {
+					// While the network is in use, judge it by the FP64 true residual of this iterate.
+					double* tr = ((fx_learned*)c.pc->handle)->tr;
+					const int* lsk = &c.d->lskip;
+					FX_CHECK(c.apply_op(x, tr, lsk));
+					FX_CHECK(c.ered<1>(FResid{b, tr, tr}, FinLearnJudge{c.d}, "learned true residual", lsk));
+				}
 				FX_CHECK(c.precond(z, r));
 				return c.ered<2>(FDot2{r, z, z, ap}, FinFcgBeta{c.d}, "fcg r.z z.ap");
 			}, (int64_t)(s->max_iter - used)));
@@ -2620,6 +2656,8 @@ fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, cons
 	L->ac = (double*)p;
 	if ((st = dev_alloc(be, n * 2 * sizeof(float), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
 	L->xin = (float*)p;
+	if ((st = dev_alloc(be, n * sizeof(double), &p)) != FX_OK) { fx_learned_destroy(L); return st; }
+	L->tr = (double*)p;
 	*out = L;
 	return FX_OK;
 }
@@ -2632,7 +2670,16 @@ void fx_learned_destroy(fx_learned* L) {
 	dev_free(L->c);
 	dev_free(L->ac);
 	dev_free(L->xin);
+	dev_free(L->tr);
 	delete L;
+}
+
+fx_status fx_learned_set_schedule(fx_learned* L, int32_t every, int32_t max_apps) {
+	FX_REQUIRE(L, FX_ERR_ARGUMENT, "learned preconditioner is null");
+	FX_REQUIRE(every >= 1 && max_apps >= 0, FX_ERR_ARGUMENT, "every must be >= 1 and max_apps >= 0");
+	L->every = every;
+	L->max_apps = max_apps;
+	return FX_OK;
 }
 
 int32_t fx_learned_fell_back(fx_learned* L) {

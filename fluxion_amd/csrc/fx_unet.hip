@@ -32,7 +32,6 @@ using namespace fxd;
 
 namespace {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;
 
@@ -274,46 +273,58 @@ fx_status launch_conv(const ConvArgs& a, hipStream_t st) {
 }
 
 // dW[tap][co][ci] += Σ_p g[p][co]·x[p+off][ci], g = dy masked by the ReLU of y.
-// One wave: a 16(co) × 16(ci) tile of one tap over a chunk of pixels.
+// One wave: a 16(co) × 16(ci) tile of one tap over `rpw` image rows (rows counted over the batch),
+// 32 pixels of a row per v_mfma_f32_16x16x32_bf16 (K = pixels): lane (m, q) feeds pixels 8q..8q+7 of
+// the chunk, g rounded to BF16 as the A row co = m and x as the B column ci = m.
 __global__ __launch_bounds__(BLOCK) void k_conv3_wgrad(const float* dy, const bf16* y, int relu, const bf16* x,
                                                        float* dW, int B, int H, int W, int cin_p, int cout_p,
-                                                       int cin, int cout, int chunk) {
+                                                       int cin, int cout, int rpw) {
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int tap = blockIdx.y, tile = blockIdx.z;
 	const int nco = cout_p / 16;
 	const int cot = tile % nco, cit = tile / nco;
-	const int m = lane & 15, kq = lane >> 4;
+	const int m = lane & 15, q = lane >> 4;
 	const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
-	const long total = (long)B * H * W;
-	const long p0 = ((long)blockIdx.x * 4 + wv) * chunk;
+	const int co = cot * 16 + m, ci = cit * 16 + m;
+	const long rows = (long)B * H;
+	const long rw0 = ((long)blockIdx.x * NWAVES + wv) * rpw;
 	f4 acc = {0.f, 0.f, 0.f, 0.f};
-	for (long pb = p0; pb < p0 + chunk && pb < total; pb += 16) {
-		s4 av, bv;
-		short ae[4], be_[4];
-		for (int j = 0; j < 4; ++j) {
-			long p = pb + kq * 4 + j;
-			float g = 0.f;
-			float xv = 0.f;
-			if (p < total) {
-				const int co = cot * 16 + m;
-				g = dy[p * cout_p + co];
-				if (relu && !(bf2f(y[p * cout_p + co]) > 0.f)) g = 0.f;
-				long bb = p / ((long)H * W);
-				int rem = (int)(p - bb * H * W);
-				int rr = rem / W + dyo, cc = rem % W + dxo;
-				if (rr >= 0 && rr < H && cc >= 0 && cc < W)
-					xv = bf2f(x[(((size_t)bb * H + rr) * W + cc) * cin_p + cit * 16 + m]);
+	for (long rw = rw0; rw < rw0 + rpw && rw < rows; ++rw) {
+		const long b = rw / H;
+		const int r = (int)(rw - b * H);
+		const int rs = r + dyo;
+		const bool row_ok = rs >= 0 && rs < H;
+		const size_t prow = (size_t)rw * W;                  // first pixel of this output row
+		const size_t srow = ((size_t)b * H + (row_ok ? rs : 0)) * W;   // first pixel of the shifted input row
+		for (int cb = 0; cb < W; cb += 32) {
+			uint16_t ga[8], xb[8];
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				const int c = cb + 8 * q + j;
+				float gv = 0.f;
+				uint16_t xv = 0;
+				if (c < W) {
+					gv = dy[(prow + c) * cout_p + co];
+					if (relu && !(bf2f(y[(prow + c) * cout_p + co]) > 0.f)) gv = 0.f;
+					const int cs = c + dxo;
+					if (row_ok && cs >= 0 && cs < W) xv = x[(srow + cs) * cin_p + ci];
+				}
+				ga[j] = f2bf(gv);
+				xb[j] = xv;
 			}
-			ae[j] = (short)f2bf(g);
-			be_[j] = (short)f2bf(xv);
+			u4v av, bv;
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				av[e] = (uint32_t)ga[2 * e] | ((uint32_t)ga[2 * e + 1] << 16);
+				bv[e] = (uint32_t)xb[2 * e] | ((uint32_t)xb[2 * e + 1] << 16);
+			}
+			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv), acc, 0,
+			                                                0, 0);
 		}
-		av = s4{ae[0], ae[1], ae[2], ae[3]};
-		bv = s4{be_[0], be_[1], be_[2], be_[3]};
-		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
 	}
 	for (int j = 0; j < 4; ++j) {
-		const int co = cot * 16 + kq * 4 + j, ci = cit * 16 + m;
-		if (co < cout && ci < cin) atomicAdd(&dW[((size_t)tap * cout + co) * cin + ci], acc[j]);
+		const int o = cot * 16 + 4 * q + j;
+		if (o < cout && ci < cin) atomicAdd(&dW[((size_t)tap * cout + o) * cin + ci], acc[j]);
 	}
 }
 
@@ -541,6 +552,7 @@ struct fx_unet {
 	bf16 *xin = nullptr, *act[NLAYER] = {}, *cat[3] = {}, *pool[3] = {};
 	float* yout = nullptr;   // layer-14 output while training, FP32 [B][H][W][16]
 	float* ych0 = nullptr;   // channel 0 of the layer-14 output in inference, FP32 [B][H][W]
+	bool train_ws = false;   // the backward's buffers are allocated for (B, H, W)
 	float *g1 = nullptr, *g2 = nullptr;   // gradient ping-pong, sized for the largest activation
 	float *gskip[3] = {};
 	float* sums = nullptr;
@@ -560,35 +572,47 @@ void free_ws(fx_unet* u) {
 	for (void* p : u->ws) fx::dev_free(p);
 	u->ws.clear();
 	u->B = u->H = u->W = 0;
+	u->train_ws = false;
 }
 
 int level_of(int li) { return li < 8 ? li / 2 : (li < 14 ? 2 - (li - 8) / 2 : 0); }
 
 // Buffers for batch B of H×W images.
-fx_status ensure_ws(fx_unet* u, int B, int H, int W) {
-	if (u->B == B && u->H == H && u->W == W) return FX_OK;
-	free_ws(u);
-	u->B = B; u->H = H; u->W = W;
-	auto npix = [&](int l) { return (size_t)B * (H >> l) * (W >> l); };
-	void* p;
-	FX_CHECK(alloc(u, npix(0) * pad16(u->cin) * 2, &p)); u->xin = (bf16*)p;
-	for (int li = 0; li < NLAYER; ++li) {
-		int l = level_of(li);
-		FX_CHECK(alloc(u, npix(l) * pad16(u->lcout[li]) * 2, &p));
-		u->act[li] = (bf16*)p;
+// Buffers for batch B of H×W images: the forward's always, the backward's (concatenations, FP32
+// gradients, the output layer's 16 channels) only once a training call asks for them.
+fx_status ensure_ws(fx_unet* u, int B, int H, int W, bool train) {
+	if (!(u->B == B && u->H == H && u->W == W)) {
+		free_ws(u);
+		u->B = B; u->H = H; u->W = W;
+		auto npix = [&](int l) { return (size_t)B * (H >> l) * (W >> l); };
+		void* p;
+		FX_CHECK(alloc(u, npix(0) * pad16(u->cin) * 2, &p)); u->xin = (bf16*)p;
+		for (int li = 0; li < NLAYER; ++li) {
+			int l = level_of(li);
+			FX_CHECK(alloc(u, npix(l) * pad16(u->lcout[li]) * 2, &p));
+			u->act[li] = (bf16*)p;
+		}
+		for (int l = 0; l < 3; ++l) {
+			FX_CHECK(alloc(u, npix(l + 1) * CH[l] * 2, &p)); u->pool[l] = (bf16*)p;
+		}
+		FX_CHECK(alloc(u, npix(0) * 4, &p)); u->ych0 = (float*)p;
+		u->train_ws = false;
 	}
-	for (int l = 0; l < 3; ++l) {
-		FX_CHECK(alloc(u, npix(l) * pad16(CH[l + 1] + CH[l]) * 2, &p)); u->cat[l] = (bf16*)p;
-		FX_CHECK(alloc(u, npix(l + 1) * CH[l] * 2, &p)); u->pool[l] = (bf16*)p;
-		FX_CHECK(alloc(u, npix(l) * CH[l] * 4, &p)); u->gskip[l] = (float*)p;
+	if (train && !u->train_ws) {
+		auto npix = [&](int l) { return (size_t)B * (H >> l) * (W >> l); };
+		void* p;
+		for (int l = 0; l < 3; ++l) {
+			FX_CHECK(alloc(u, npix(l) * pad16(CH[l + 1] + CH[l]) * 2, &p)); u->cat[l] = (bf16*)p;
+			FX_CHECK(alloc(u, npix(l) * CH[l] * 4, &p)); u->gskip[l] = (float*)p;
+		}
+		FX_CHECK(alloc(u, npix(0) * 16 * 4, &p)); u->yout = (float*)p;
+		size_t gmax = npix(0) * 128;
+		FX_CHECK(alloc(u, gmax * 4, &p)); u->g1 = (float*)p;
+		FX_CHECK(alloc(u, gmax * 4, &p)); u->g2 = (float*)p;
+		FX_CHECK(alloc(u, (size_t)B * 2 * 4, &p)); u->sums = (float*)p;
+		FX_CHECK(alloc(u, npix(0) * 16 * 4, &p)); u->dy = (float*)p;
+		u->train_ws = true;
 	}
-	FX_CHECK(alloc(u, npix(0) * 16 * 4, &p)); u->yout = (float*)p;
-	FX_CHECK(alloc(u, npix(0) * 4, &p)); u->ych0 = (float*)p;
-	size_t gmax = npix(0) * 128;
-	FX_CHECK(alloc(u, gmax * 4, &p)); u->g1 = (float*)p;
-	FX_CHECK(alloc(u, gmax * 4, &p)); u->g2 = (float*)p;
-	FX_CHECK(alloc(u, (size_t)B * 2 * 4, &p)); u->sums = (float*)p;
-	FX_CHECK(alloc(u, npix(0) * 16 * 4, &p)); u->dy = (float*)p;
 	return FX_OK;
 }
 
@@ -694,11 +718,13 @@ fx_status backward(fx_unet* u) {
 	auto wgrad = [&](int li, const float* g, const bf16* yact, int relu, const bf16* xin, int l) -> fx_status {
 		int Hl = u->H >> l, Wl = u->W >> l;
 		long total = (long)u->B * Hl * Wl;
-		int chunk = 256;
 		int cip = pad16(u->lcin[li]), cop = pad16(u->lcout[li]);
-		dim3 grid((unsigned)((total + 4L * chunk - 1) / (4L * chunk)), 9, (unsigned)((cop / 16) * (cip / 16)));
+		// Rows per wave: about 2048 pixels of work per wave, at least one row.
+		const long rows = (long)u->B * Hl;
+		const int rpw = (int)std::max<long>(1, 2048 / Wl);
+		dim3 grid((unsigned)((rows + (long)NWAVES * rpw - 1) / ((long)NWAVES * rpw)), 9, (unsigned)((cop / 16) * (cip / 16)));
 		hipLaunchKernelGGL(k_conv3_wgrad, grid, dim3(BLOCK), 0, st, g, yact, relu, xin, u->G + u->woff[li], u->B, Hl, Wl,
-		                   cip, cop, u->lcin[li], u->lcout[li], chunk);
+		                   cip, cop, u->lcin[li], u->lcout[li], rpw);
 		hipLaunchKernelGGL(k_bgrad, dim3(256), dim3(256), 0, st, g, yact, relu, u->G + u->boff[li], total, cop,
 		                   u->lcout[li]);
 		return fx::check_launch("unet wgrad");
@@ -846,7 +872,7 @@ fx_status fx_unet_get_grad(fx_unet* u, float* host, size_t n) {
 static fx_status run_forward(fx_unet* u, const float* x, int32_t B, int32_t H, int32_t W, bool infer, float* y) {
 	FX_REQUIRE(u && x && B > 0 && H % 8 == 0 && W % 8 == 0 && H > 0 && W > 0, FX_ERR_ARGUMENT,
 	           "batch > 0 and H, W divisible by 8 required");
-	FX_CHECK(ensure_ws(u, B, H, W));
+	FX_CHECK(ensure_ws(u, B, H, W, !infer));
 	long npix = (long)B * H * W;
 	hipLaunchKernelGGL(k_pack_in, dim3((unsigned)std::min<long>(nb(npix), 65536)), dim3(256), 0, u->be->stream, x, u->xin,
 	                   npix, u->cin, pad16(u->cin), u->skip);

@@ -678,6 +678,34 @@ class CrankNicolsonSolver:
         L.check(self._lib.fx_multigrid_autotune(m, reps, C.byref(ms)), "multigrid_autotune")
         return ms.value
 
+    def multigrid(self) -> "_MgView":
+        """The stepper's own multigrid (owned by the stepper), e.g. to build a learned preconditioner on."""
+        m = self._lib.fx_cn_multigrid(self.h)
+        if not m:
+            raise ValueError("the stepper has no multigrid")
+        return _MgView(m, self)
+
+    def set_preconditioner(self, pc) -> None:
+        """Solve every step with `pc` (kept alive by the stepper object) instead of its multigrid; None
+        restores the multigrid (fx_cn_set_preconditioner)."""
+        self._pc_keep = pc
+        if pc is None:
+            L.check(self._lib.fx_cn_set_preconditioner(self.h, None), "cn_set_preconditioner")
+        else:
+            c = pc._pc()
+            L.check(self._lib.fx_cn_set_preconditioner(self.h, C.byref(c)), "cn_set_preconditioner")
+
+
+class _MgView:
+    """A multigrid owned by another object (no destructor call); `owner` keeps that object alive."""
+
+    def __init__(self, h, owner):
+        self.h = h
+        self._owner = owner
+
+    def _pc(self):
+        return L.FxPrecond(1, 0, self.h)
+
 
 class ImexSolver:
     """ImexSolver (crates/runtime/src/imex.rs:1-274), stepped on the device by fx_imex_*.

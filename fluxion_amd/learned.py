@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
+from pathlib import Path
 
 import numpy as np
 
@@ -108,7 +109,8 @@ def he_init(cin: int = 2, seed: int = 0) -> np.ndarray:
 
 
 class LearnedPreconditioner:
-    """FX_PC_LEARNED: z = V(r) + s·h²·N([r1/s, log k]) with r1 = r − A·V(r), s = rms(r1).
+    """FX_PC_LEARNED: z = V(r) + α*·s·h²·N([r1/s, log k]) with r1 = r − A·V(r), s = rms(r1), α* the
+    A-norm-optimal step; falls back to V(r) once the FP64 true residual stalls.
 
     `mg` is the V-cycle (also the fallback), `logk` an FP64 field of log k (None for k ≡ 1)."""
 
@@ -130,6 +132,10 @@ class LearnedPreconditioner:
 
     def fell_back(self) -> bool:
         return bool(self._lib.fx_learned_fell_back(self.h))
+
+    def set_schedule(self, every: int = 1, max_apps: int = 0) -> None:
+        """Network at every `every`-th iteration, at most `max_apps` times per solve (0: no limit)."""
+        L.check(self._lib.fx_learned_set_schedule(self.h, every, max_apps), "learned_set_schedule")
 
     def _pc(self):
         return L.FxPrecond(2, 0, self.h)
@@ -204,3 +210,57 @@ def train(net: UNet, sets, steps: int = 400, batch: int = 8, lr: float = 1e-3, s
         if log and (step % 50 == 0 or step == steps - 1):
             log(f"train step {step}: loss {losses[-1]:.4f}")
     return losses
+
+
+# Weights trained by train_vark() on the log-normal-k family (written by `python -m fluxion_amd.learned`).
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_vark.npz"
+
+
+def train_vark(be, steps: int = 1500, sizes=((256, 16), (512, 6)), max_its: int = 10, batch: int = 8,
+               lr: float = 1e-3, seed0: int = 31000, log=None):
+    """Trains the network on residual→error pairs that the mixed FCG solver generates on the log-normal-k
+    diffusion family: for each (n, count) in `sizes`, `count` problems at n² with seeds seed0 + 1000·n + i,
+    pairs from the first `max_its` FCG iterates (solver_pairs); relative energy-norm loss, Adam with a
+    cosine schedule.  Returns (net, losses)."""
+    sets = []
+    for n, count in sizes:
+        ps = [solver_pairs(be, n, seed=seed0 + 1000 * n + i, max_its=max_its) for i in range(count)]
+        sets.append(PairSet(np.concatenate([p.x for p in ps]), np.concatenate([p.t for p in ps])))
+        if log:
+            log(f"pairs at {n}^2: {len(sets[-1].x)}")
+    net = UNet(be, 2)
+    net.set_params(he_init(2, 0))
+    losses = train(net, sets, steps=steps, batch=batch, lr=lr, energy=True, log=log)
+    return net, losses
+
+
+def save_weights(path, params, meta: dict) -> None:
+    p = Path(path)
+    p.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(p, params=np.asarray(params, dtype=np.float32),
+                        meta=np.array(repr(sorted(meta.items()))))
+
+
+def load_weights(path=DEFAULT_WEIGHTS) -> np.ndarray:
+    with np.load(path, allow_pickle=False) as z:
+        return np.asarray(z["params"], dtype=np.float32)
+
+
+if __name__ == "__main__":
+    import argparse
+    import sys
+    import time
+
+    from .hip import HipBackend
+
+    ap = argparse.ArgumentParser(description="train the learned preconditioner's network on the GPU")
+    ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    ap.add_argument("--steps", type=int, default=1500)
+    a = ap.parse_args()
+    be = HipBackend(0)
+    t0 = time.time()
+    net, losses = train_vark(be, steps=a.steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+    meta = {"steps": a.steps, "sizes": "256^2 x16, 512^2 x6", "loss": "relative energy norm",
+            "final_loss": float(np.mean(losses[-50:])), "seconds": time.time() - t0}
+    save_weights(a.out, net.params(), meta)
+    print(f"saved {a.out}: {meta}", flush=True)

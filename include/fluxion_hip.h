@@ -320,16 +320,22 @@ fx_status fx_unet_loss_grad_energy(fx_unet* u, const fx_buf* x, const fx_buf* t,
                                    double* loss);
 fx_status fx_unet_adam_step(fx_unet* u, float lr, float beta1, float beta2, float eps);
 
-/* Learned preconditioner (FX_PC_LEARNED): z = V(r) + s·h²·N([r1/s, log k]) with r1 = r − A·V(r),
- * s = rms(r1), V the multigrid V-cycle, N the network (h² = dx·dy, logk may be NULL for k ≡ 1).
- * Inside a solve it falls back to z = V(r) for the remaining iterations once ‖r‖ fails to drop by the
- * factor `stall` in one iteration (flexible CG judges by its FP64 residual). */
+/* Learned preconditioner (FX_PC_LEARNED): z = V(r) + α*·c with c = s·h²·N([r1/s, log k]), r1 = r − A·V(r),
+ * s = rms(r1), V the multigrid V-cycle, N the network (h² = dx·dy, logk may be NULL for k ≡ 1) and
+ * α* = (c·r1)/(c·A·c) clipped at 0, the step that minimises the A-norm of the remaining error.  Used by
+ * flexible CG.  While the network is in use the solve recomputes the FP64 true residual ‖b − A·x‖ after
+ * every iteration, and falls back to z = V(r) for the rest of the solve once it fails to drop by the
+ * factor `stall`. */
 typedef struct fx_learned fx_learned;
 fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, const fx_buf* logk, const fx_grid* grid,
                             double stall, fx_learned** out);
 void fx_learned_destroy(fx_learned* l);
 /* 1 if a solve since the previous call switched to the plain V-cycle (and clears the flag). */
 int32_t fx_learned_fell_back(fx_learned* l);
+/* Schedule of the network inside a solve: applied at the preconditioner applications made after
+ * 0, every, 2·every, … iterations, at most max_apps times per solve pass (0: no limit); the other
+ * applications are the V-cycle alone.  Default every = 1, max_apps = 0. */
+fx_status fx_learned_set_schedule(fx_learned* l, int32_t every, int32_t max_apps);
 
 /* Extension: A·x for an operator (same arithmetic as the solvers use). */
 fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y,
@@ -386,6 +392,10 @@ double fx_cn_sim_time(const fx_cn* cn);          /* implicit.rs:168-172: steps_d
  * multigrid (NULL when unpreconditioned; fx_multigrid_autotune applies), owned by the stepper. */
 fx_solver* fx_cn_solver(fx_cn* cn);
 fx_multigrid* fx_cn_multigrid(fx_cn* cn);
+/* Extension: solve every step with the caller's preconditioner instead of the stepper's multigrid (the
+ * caller keeps it alive; NULL restores the multigrid).  FX_PC_LEARNED needs opt->solver = FX_SOLVER_FCG;
+ * build it over fx_cn_multigrid(cn), whose level operator is the step's. */
+fx_status fx_cn_set_preconditioner(fx_cn* cn, const fx_precond* pc);
 
 /* ── IMEX convection–diffusion stepper (ImexSolver, runtime/src/imex.rs:1-274) ──────────────
  * Uniform velocity (vx, vy) and diffusivity κ: conv = v·∇u (upwind, apply_conv_diff with κ = 0), then

@@ -534,3 +534,70 @@ def unet_loss_grad(params, x, t, energy=False, bf16=False):
     loss = fn(C.c_void_p(_p(p)), C.c_void_p(_p(x)), C.c_void_p(_p(t)), C.c_void_p(_p(g)),
               C.c_int(b), C.c_int(h), C.c_int(w), C.c_int(cin), C.c_int(int(bool(bf16))))
     return loss, g
+
+
+# ── learned preconditioner inside flexible CG (composition of the pieces above) ──
+def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9, every=1, max_apps=0):
+    """Flexible CG (Notay's β = −z·Ap_old / p_old·Ap_old) with the learned preconditioner
+    z = V(r) + α*·c, c = s·h²·N([r1/s, log k]) on interior points, r1 = r − A·V(r), s = rms(r1),
+    α* = (c·r1)/(c·A·c) clipped at 0, the network applied after 0, every, 2·every, … iterations (at most
+    max_apps times, 0 = no limit) until the FP64 true residual ‖b − A·x‖ fails to drop by `stall` in
+    one iteration.  The operation order of the device solver; network in BF16 storage mode.
+    Returns (x, iterations, fell_back)."""
+    n = grid.rows * grid.cols
+    h2 = grid.dx * grid.dy
+    inner = np.zeros((grid.rows, grid.cols), bool)
+    inner[1:-1, 1:-1] = True
+    inner = inner.ravel()
+    A = lambda v: apply_op(op, v, grid)
+    st = {"iter": 0, "plain": False, "apps": 0}
+
+    def pc(r):
+        z = vcycle(np.zeros(n), r, grid, mg)
+        on = (not st["plain"]) and st["iter"] % every == 0 and (max_apps <= 0 or st["apps"] < max_apps)
+        if not on:
+            return z
+        st["apps"] += 1
+        r1 = r - A(z)
+        s = float(np.sqrt(np.dot(r1, r1) / n))
+        if s == 0.0:
+            s = 1.0
+        x = np.stack([(r1 / s).astype(np.float32).astype(np.float64).reshape(grid.rows, grid.cols),
+                      logk.astype(np.float32).astype(np.float64).reshape(grid.rows, grid.cols)], axis=-1)[None]
+        y = unet_forward(params, x, bf16=True)[0].astype(np.float32).astype(np.float64).ravel()
+        c = np.where(inner, s * h2 * y, 0.0)
+        ac = A(c)
+        cr, cac = float(c @ r1), float(c @ ac)
+        a = cr / cac if (cac > 0.0 and cr > 0.0) else 0.0
+        return z + a * c
+
+    x = np.zeros(n)
+    r = f - A(x)
+    ltres = float(r @ r)
+    z = pc(r)
+    rs = float(r @ z)
+    if np.sqrt(r @ r) < tol:
+        return x, 0, False
+    p = np.zeros(n)
+    ap = np.zeros(n)
+    beta, pap = 0.0, 1.0
+    for it in range(max_iter):
+        p = z + beta * p
+        ap = A(p)
+        pap = float(p @ ap)
+        alpha = rs / pap
+        x = x + alpha * p
+        r = r - alpha * ap
+        st["iter"] = it + 1
+        if np.sqrt(r @ r) < tol:
+            return x, it + 1, st["plain"]
+        if not st["plain"]:
+            tr = f - A(x)
+            t2 = float(tr @ tr)
+            if not t2 < stall * stall * ltres:
+                st["plain"] = True
+            ltres = t2
+        z = pc(r)
+        rs = float(r @ z)
+        beta = -float(z @ ap) / pap
+    return x, max_iter, st["plain"]

@@ -92,3 +92,100 @@ def test_learned_network_gain(be):
           f"(fell back to the V-cycle: {lp.fell_back()})")
     assert s_mg.converged() and s_l.converged()
     assert s_l.true_residual < tol
+
+
+def _vark_problem(be, n, kseed=424242, fseed=434343):
+    from fluxion_amd import hip
+    from fluxion_amd.core import Boundaries, DType, Grid, Transfer
+    from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n, seed=kseed)
+    f = gaussian_bumps(n, seed=fseed).ravel()
+    K = hip.Field.from_f64(g, k.ravel(), be)
+    LK = hip.Field.from_f64(g, np.log(k).ravel(), be)
+    mg = hip.Multigrid(g, Boundaries.zero_dirichlet(), 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16,
+                       precision=DType.F32, fused=True, scale=1.0, k=K, be=be)
+    return hip, g, k, f, K, LK, mg
+
+
+def test_learned_fcg_matches_oracle_composition(be):
+    # The device's learned flexible CG (committed weights) against the oracle's composition of the same
+    # pieces (FP32 V-cycle, BF16-storage U-Net, α* step, true-residual fallback) at 64².  The networks
+    # differ by FP32 accumulation order, so the trajectories agree to the iteration count ±2, the same
+    # fallback decision and solutions within the tolerance's reach.
+    import oracle as O
+    from fluxion_amd.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, UNet, load_weights
+    hip, g, k, f, K, LK, mg = _vark_problem(be, 64)
+    params = load_weights(DEFAULT_WEIGHTS)
+    net = UNet(be, 2)
+    net.set_params(params)
+    tol = 1e-8 * np.linalg.norm(f)
+    lp = LearnedPreconditioner(net, mg, g, LK, stall=0.9)
+    x = hip.Field.zeros(g, be=be)
+    s = hip.FcgSolver(g, tol, 200, be=be).solve_op(x, hip.Field.from_f64(g, f, be), hip.Operator.var_k(K), lp)
+    fb = lp.fell_back()
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=1, scale=1.0,
+                   k=k.ravel())
+    xr, itr, fbr = O.learned_fcg(f, g, O.Op.var_k(k.ravel()), cfg, params.astype(np.float64), np.log(k).ravel(),
+                                 tol, 200)
+    d = np.linalg.norm(x.to_vec_f64() - xr) / np.linalg.norm(xr)
+    print(f"learned FCG 64²: gpu {s.iterations} its (fell back {fb}), oracle {itr} its (fell back {fbr}), "
+          f"rel diff {d:.2e}")
+    assert s.converged() and s.true_residual < tol
+    assert abs(s.iterations - itr) <= 2 and fb == fbr
+    assert d < 1e-5
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_learned_fcg_large(be, n):
+    # The C5 evaluation sizes: the learned preconditioner inside FCG reaches 1e-8 in FP64 true residual;
+    # iterations and time against the V-cycle alone are printed (the network's cost and gain are
+    # weighed in the design notes).
+    import time
+    from fluxion_amd.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, UNet, load_weights
+    hip, g, k, f, K, LK, mg = _vark_problem(be, n)
+    net = UNet(be, 2)
+    net.set_params(load_weights(DEFAULT_WEIGHTS))
+    tol = 1e-8 * np.linalg.norm(f)
+    op = hip.Operator.var_k(K)
+    F = hip.Field.from_f64(g, f, be)
+    solver = hip.FcgSolver(g, tol, 300, be=be)
+    out = {}
+    for name, pc in (("V-cycle", mg), ("learned", LearnedPreconditioner(net, mg, g, LK, stall=0.9))):
+        x = hip.Field.zeros(g, be=be)
+        solver.solve_op(x, F, op, pc)
+        x.fill(0.0)
+        be.synchronize()
+        t0 = time.perf_counter()
+        s = solver.solve_op(x, F, op, pc)
+        be.synchronize()
+        out[name] = (s, time.perf_counter() - t0)
+        assert s.converged() and s.true_residual < tol, (name, s)
+    print(f"FCG {n}²: V-cycle {out['V-cycle'][0].iterations} its {1e3 * out['V-cycle'][1]:.2f} ms; learned "
+          f"{out['learned'][0].iterations} its {1e3 * out['learned'][1]:.2f} ms")
+
+
+def test_learned_crank_nicolson_steps(be):
+    # Warm-started Crank–Nicolson steps of the 1024² variable-k diffusion with the learned preconditioner
+    # (over the stepper's own multigrid) agree with the V-cycle stepper's states at every step's tolerance.
+    from fluxion_amd.inputs import gaussian_bumps
+    from fluxion_amd.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, UNet, load_weights
+    hip, g, k, f, K, LK, mg = _vark_problem(be, 1024)
+    net = UNet(be, 2)
+    net.set_params(load_weights(DEFAULT_WEIGHTS))
+    u0 = gaussian_bumps(1024).ravel()
+    tol = 1e-8 * np.linalg.norm(u0)
+    finals, its = {}, {}
+    for name in ("V-cycle", "learned"):
+        stp = hip.CrankNicolsonSolver.var_k(g, 1.0, 1e-4, K, tol, 200, mixed=True, be=be)
+        lp = None
+        if name == "learned":
+            lp = LearnedPreconditioner(net, stp.multigrid(), g, LK, stall=0.9)
+            stp.set_preconditioner(lp)
+        U = hip.Field.from_f64(g, u0, be)
+        its[name] = [stp.step(U).iterations for _ in range(20)]
+        finals[name] = U.to_vec_f64()
+    d = np.linalg.norm(finals["learned"] - finals["V-cycle"]) / np.linalg.norm(finals["V-cycle"])
+    print(f"CN 1024² x20: iterations V-cycle {sum(its['V-cycle'])}, learned {sum(its['learned'])}; final states differ "
+          f"by {d:.1e}")
+    assert d < 1e-7
