@@ -37,6 +37,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_FLOPS = 2.5e15   # MI355X dense BF16 MFMA
 METRIC = "solves/s to 1e-8 rel residual: learned+mixed vs plain MG-PCG; stencil HBM GB/s"
 
 
@@ -104,7 +105,9 @@ def main():
     ap.add_argument("--no-cn", action="store_true", help="skip the Crank-Nicolson record")
     ap.add_argument("--cn-steps", type=int, default=100)
     ap.add_argument("--no-explicit", action="store_true", help="skip the explicit diffusion record")
-    ap.add_argument("--learned", type=str, default=None, help="trained U-Net weights (.npy, fx_unet layout)")
+    ap.add_argument("--learned", type=str, default=None,
+                    help="trained U-Net weights (.npz from fluxion_amd.learned; default: the committed weights)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner records")
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in wave segment heights")
@@ -198,20 +201,44 @@ def main():
     log(f"bench: var-k mixed FCG {s_mixed.iterations} its, {1e3 * t_mixed:.2f} ms/solve, "
         f"true rel res {s_mixed.true_residual / fnorm:.2e}")
 
-    learned = None
+    # ── learned + mixed: the same FCG with the U-Net correction (committed weights) ──
+    learned, unet_rec, net, LK = None, None, None, None
     t_value, s_value, label = t_mixed, s_mixed, "mixed"
-    if args.learned:
-        from fluxion_amd.learned import LearnedPreconditioner, UNet
+    if not args.no_learned:
+        from fluxion_amd.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, UNet, load_weights
+        wpath = args.learned or DEFAULT_WEIGHTS
         net = UNet(be, 2)
-        net.set_params(np.load(args.learned))
+        net.set_params(load_weights(wpath))
         LK = hip.Field.from_f64(g, np.log(k).ravel(), be)
-        lp = LearnedPreconditioner(net, mgv32, g, LK)
-        t_l, s_l, _ = timed(v_mixed(lp), args.steps, args.warmup)
+        lp = LearnedPreconditioner(net, mgv32, g, LK, stall=0.9)
+        t_l, s_l, _ = timed(v_mixed(lp), max(1, args.steps // 4), 1)
+        fb = lp.fell_back()
         log(f"bench: var-k learned+mixed FCG {s_l.iterations} its, {1e3 * t_l:.2f} ms/solve, "
-            f"true rel res {s_l.true_residual / fnorm:.2e}, fell back {lp.fell_back()}")
-        learned = {"solves_per_s": 1.0 / t_l, "iterations": s_l.iterations,
-                   "true_rel_residual": s_l.true_residual / fnorm, "fell_back": lp.fell_back()}
-        t_value, s_value, label = t_l, s_l, "learned+mixed"
+            f"true rel res {s_l.true_residual / fnorm:.2e}, fell back to the V-cycle {fb}")
+        learned = {"solves_per_s": 1.0 / t_l, "ms_per_solve": 1e3 * t_l, "iterations": s_l.iterations,
+                   "true_rel_residual": s_l.true_residual / fnorm, "fell_back": fb,
+                   "speedup_vs_plain": t_plain / t_l, "speedup_vs_mixed": t_mixed / t_l,
+                   "weights": Path(wpath).name,
+                   "note": "the network correction costs more iterations and time than it saves; the headline "
+                           "value is the mixed path"}
+        # The network alone: one forward at this size (15 convolutions, 61,920 FLOP per pixel).
+        XIN = be.allocate(n * n * 2, DType.F32)
+        YOUT = be.allocate(n * n, DType.F32)
+        for _ in range(2):
+            net.forward_device(XIN, YOUT, 1, n, n)
+        be.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            net.forward_device(XIN, YOUT, 1, n, n)
+        be.synchronize()
+        t_f = (time.perf_counter() - t0) / 10
+        flop = 61920.0 * n * n
+        unet_rec = {"grid": n, "ms_per_forward": 1e3 * t_f, "flop_per_forward": flop, "TFLOPs": flop / t_f / 1e12,
+                    "frac_bf16_dense_peak": flop / t_f / BF16_PEAK_FLOPS,
+                    "mixed_fcg_iterations_per_forward": t_f / (t_mixed / max(1, s_mixed.iterations))}
+        log(f"bench: U-Net forward {n}^2 {1e3 * t_f:.2f} ms, {flop / t_f / 1e12:.0f} TFLOP/s "
+            f"({100 * flop / t_f / BF16_PEAK_FLOPS:.1f}% of BF16 dense peak)")
+        del XIN, YOUT
 
     # Per-kernel probes of the mixed solve (the x/r update was probed over the timed region).
     kb = kernel_bytes(True, mgk == MgKernels.WAVE and n % 2 == 0)
@@ -412,10 +439,18 @@ def main():
         cn = {"grid": nc, "dt": cn_dt, "alpha": 1.0, "steps": args.cn_steps,
               "tol_abs": cn_tol, "tol_note": f"absolute ||r|| < {args.tol:g}*||u0|| per step"}
         finals = {}
-        for name, mixed in (("mixed_fcg", True), ("plain_fp64_mg_pcg", False)):
+        legs = [("mixed_fcg", True), ("plain_fp64_mg_pcg", False)]
+        if net is not None:
+            legs.append(("learned_mixed_fcg", True))
+            LKc = hip.Field.from_f64(gc, np.log(grf_log_k(nc)).ravel(), be)
+        for name, mixed in legs:
             stp = hip.CrankNicolsonSolver.var_k(gc, 1.0, cn_dt, kc, cn_tol, max_iter, mixed=mixed, be=be)
             if mixed and mgk == MgKernels.WAVE and not args.no_autotune:
                 stp.autotune(5)
+            if name == "learned_mixed_fcg":
+                from fluxion_amd.learned import LearnedPreconditioner
+                lpc = LearnedPreconditioner(net, stp.multigrid(), gc, LKc, stall=0.9)
+                stp.set_preconditioner(lpc)
             U = hip.Field.from_f64(gc, u0c, be)
             for _ in range(2):
                 stp.step(U)          # warm-up: first-use allocations, then reset
@@ -431,6 +466,11 @@ def main():
             cn[name] = {"steps_per_s": 1.0 / t, "ms_per_step": 1e3 * t, "avg_iterations": its / args.cn_steps}
             del stp
         cn["speedup_mixed_vs_plain"] = cn["plain_fp64_mg_pcg"]["ms_per_step"] / cn["mixed_fcg"]["ms_per_step"]
+        if "learned_mixed_fcg" in cn:
+            cn["learned_final_state_rel_diff"] = float(np.linalg.norm(finals["learned_mixed_fcg"] - finals["mixed_fcg"])
+                                                       / np.linalg.norm(finals["mixed_fcg"]))
+            log(f"bench: CN learned+mixed {cn['learned_mixed_fcg']['ms_per_step']:.2f} ms/step "
+                f"({cn['learned_mixed_fcg']['avg_iterations']:.1f} its)")
         cn["final_state_rel_diff"] = float(np.linalg.norm(finals["mixed_fcg"] - finals["plain_fp64_mg_pcg"])
                                            / np.linalg.norm(finals["plain_fp64_mg_pcg"]))
         log(f"bench: CN var-k {nc}^2, {args.cn_steps} steps dt={cn_dt:g}: mixed "
@@ -574,7 +614,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 outer / f32 V-cycle" + (" / bf16 network" if learned else ""),
+        "dtype": "f64 outer / f32 V-cycle",
         "data": "synthetic (seeded log-normal k, 8 Gaussian bumps)",
         "config": {"workload": f"variable-coefficient diffusion -div(k grad u)=f, {n}x{n}, log-normal k "
                                f"(corr 0.1, log-var 1), FCG to {args.tol:g} FP64 true relative residual",
@@ -597,6 +637,7 @@ def main():
         "batch_512": batch,
         "cn_vark_1024": cn,
         "explicit_diffusion_4096": explicit,
+        "unet_forward": unet_rec,
         "kernels": kernels,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

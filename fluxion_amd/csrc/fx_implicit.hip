@@ -12,6 +12,8 @@
 #include <cerrno>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <sys/stat.h>
 
 #include "fx_internal.hpp"
 
@@ -328,9 +330,15 @@ fx_status raw_read(const char* path, fx_grid* g, double* data, size_t len) {
 	FX_CHECK(fx::check_grid(&r));
 	FX_REQUIRE(r.rows <= (uint64_t)1 << 31 && r.cols <= (uint64_t)1 << 31, FX_ERR_GRID,
 	           "%s: grid %llux%llu is too large", path, (unsigned long long)r.rows, (unsigned long long)r.cols);
+	// The file must hold all rows·cols values before anything is sized from the header, so that a
+	// truncated or corrupt header is an I/O error rather than a huge allocation.
+	const size_t n = (size_t)(r.rows * r.cols);
+	struct stat sb;
+	FX_REQUIRE(fstat(fileno(f.f), &sb) == 0, FX_ERR_IO, "cannot stat %s: %s", path, std::strerror(errno));
+	const uint64_t held = (uint64_t)sb.st_size >= RAW_HEADER ? ((uint64_t)sb.st_size - RAW_HEADER) / 8 : 0;
+	FX_REQUIRE(held >= (uint64_t)n, FX_ERR_IO, "%s: file holds fewer than %zu values", path, n);
 	*g = r;
 	if (!data) return FX_OK;
-	const size_t n = (size_t)(r.rows * r.cols);
 	FX_REQUIRE(len == n, FX_ERR_DIMENSION, "dimension mismatch: expected %zu, got %zu (data)", n, len);
 	std::vector<unsigned char> buf(8 * (size_t)65536);
 	for (size_t i0 = 0; i0 < n; i0 += 65536) {
@@ -674,16 +682,21 @@ fx_status fx_write_raw(fx_backend* be, const char* path, const fx_buf* field, co
 	FX_CHECK(fx::check_grid(grid));
 	const size_t n = (size_t)(grid->rows * grid->cols);
 	FX_REQUIRE(field->len == n, FX_ERR_DIMENSION, "dimension mismatch: expected %zu, got %zu (field)", n, field->len);
-	std::vector<double> host(n);
-	if (field->dtype == FX_F64) {
-		FX_CHECK(fx_copy_to_host_f64(be, field, host.data(), n));
-	} else {
-		// Field::to_vec_f64 (runtime/src/field.rs:134-138) returns an F32 field widened to f64.
-		std::vector<float> h32(n);
-		FX_CHECK(fx_copy_to_host_f32(be, field, h32.data(), n));
-		for (size_t i = 0; i < n; ++i) host[i] = (double)h32[i];
+	try {
+		std::vector<double> host(n);
+		if (field->dtype == FX_F64) {
+			FX_CHECK(fx_copy_to_host_f64(be, field, host.data(), n));
+		} else {
+			// Field::to_vec_f64 (runtime/src/field.rs:134-138) returns an F32 field widened to f64.
+			std::vector<float> h32(n);
+			FX_CHECK(fx_copy_to_host_f32(be, field, h32.data(), n));
+			for (size_t i = 0; i < n; ++i) host[i] = (double)h32[i];
+		}
+		return raw_write(path, host.data(), grid);
+	} catch (const std::bad_alloc&) {
+		fx::set_error("%s: no host memory for %zu values", path, n);
+		return FX_ERR_ALLOC;
 	}
-	return raw_write(path, host.data(), grid);
 }
 
 fx_status fx_read_raw(fx_backend* be, const char* path, fx_grid* grid, fx_buf** out) {
@@ -692,9 +705,14 @@ fx_status fx_read_raw(fx_backend* be, const char* path, fx_grid* grid, fx_buf** 
 	fx_grid g{};
 	FX_CHECK(raw_read(path, &g, nullptr, 0));
 	const size_t n = (size_t)(g.rows * g.cols);
-	std::vector<double> host(n);
-	FX_CHECK(raw_read(path, &g, host.data(), n));
-	FX_CHECK(fx_upload_f64(be, host.data(), n, out));
+	try {
+		std::vector<double> host(n);
+		FX_CHECK(raw_read(path, &g, host.data(), n));
+		FX_CHECK(fx_upload_f64(be, host.data(), n, out));
+	} catch (const std::bad_alloc&) {
+		fx::set_error("%s: no host memory for %zu values", path, n);
+		return FX_ERR_ALLOC;
+	}
 	*grid = g;
 	return FX_OK;
 }

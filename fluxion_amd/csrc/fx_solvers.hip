@@ -1685,6 +1685,13 @@ fx_status reset_state(fx_solver* s, int64_t max_iter, const fx_precond* pc = nul
 	return FX_OK;
 }
 
+// A step failed part-way: let the kernels already queued finish before returning, so that none of
+// them stores to the solver's host-visible flags after the next solve has reset them.
+fx_status drain_on_error(hipStream_t st, fx_status e) {
+	(void)hipStreamSynchronize(st);
+	return e;
+}
+
 // Enqueues `step(i)` until the device reports done, polling `look` steps behind.
 template <class Step>
 fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
@@ -1692,7 +1699,7 @@ fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
 	hipStream_t st = s->be->stream;
 	for (int64_t i = 0;; ++i) {
 		if (i > max_steps + look + 2) break;
-		FX_CHECK(step(i));
+		if (const fx_status e = step(i); e != FX_OK) return drain_on_error(st, e);
 		FX_HIP(hipEventRecord(s->ev[i % RING], st));
 		if (i >= look) {
 			// Once step i − look has completed, its kernels' stores to *hdone are visible here.
@@ -1718,7 +1725,7 @@ fx_status drive_counted(fx_solver* s, Step step, int64_t max_steps) {
 	const volatile int* done = s->hdone;
 	const volatile int* prog = s->hdone + 1;
 	for (int64_t i = 0; i <= max_steps + 2; ++i) {
-		FX_CHECK(step(i));
+		if (const fx_status e = step(i); e != FX_OK) return drain_on_error(st, e);
 		for (unsigned spin = 0;; ++spin) {
 			if (__atomic_load_n(prog, __ATOMIC_ACQUIRE) >= i + 1 || __atomic_load_n(done, __ATOMIC_ACQUIRE)) break;
 			if ((spin & 63) == 63) {

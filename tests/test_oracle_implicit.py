@@ -272,6 +272,12 @@ def test_raw_errors(tmp_path):
     p.write_bytes(struct.pack("<QQdd", 3, 3, 1.0, 1.0) + np.zeros(8).tobytes())
     with pytest.raises(IoError):
         H.read_raw_host(p)
+    # A header announcing 2^62 values over a few bytes of data fails on the file size, before any
+    # buffer is sized from it.
+    p = tmp_path / "huge_header.raw"
+    p.write_bytes(struct.pack("<QQdd", 1 << 31, 1 << 31, 1.0, 1.0) + np.zeros(4).tobytes())
+    with pytest.raises(IoError):
+        H.read_raw_host(p)
     for hdr in (struct.pack("<QQdd", 2, 3, 1.0, 1.0), struct.pack("<QQdd", 3, 3, 0.0, 1.0),
                 struct.pack("<QQdd", 3, 3, 1.0, float("nan"))):
         p = tmp_path / "bad.raw"
