@@ -310,6 +310,105 @@ __global__ __launch_bounds__(BLOCK) void k_stream5d(const double* __restrict__ x
 	}
 }
 
+// A point's five-point neighbourhood: centre, north (row above), south, west, east.
+struct Nb {
+	double c, n, s, w, e;
+};
+__device__ __forceinline__ double hmean2(double a, double b) { return 2.0 * a * b / (a + b); }
+// −∇·(k∇u) at a point with harmonic face means, as ops-level kernels form it:
+// L = −(fx·ix2 + fy·iy2) with fx = k_e·(u_e − u) − k_w·(u − u_w), fy likewise;
+// *fsum gets (k_w + k_e)·ix2 + (k_n + k_s)·iy2 (the diagonal before scale and sigma).
+__device__ __forceinline__ double vk_lap(const Nb& u, const Nb& k, double ix2, double iy2, double* fsum) {
+	const double kw = hmean2(k.c, k.w), ke = hmean2(k.c, k.e);
+	const double kn = hmean2(k.c, k.n), ks = hmean2(k.c, k.s);
+	const double fx = ke * (u.e - u.c) - kw * (u.c - u.w);
+	const double fy = ks * (u.s - u.c) - kn * (u.c - u.n);
+	*fsum = (kw + ke) * ix2 + (kn + ks) * iy2;
+	return -(fx * ix2 + fy * iy2);
+}
+
+// Two-column streaming traversal for FP64 grids with an even column count,
+// the shape of k_stream5d carrying an optional coefficient field k (VK) beside
+// the values: per row one 16-byte load of the value pair (V::pair) and, with
+// VK, of the k pair; west and east neighbours by DPP from the adjacent lanes
+// (the first and last lane of a wave read the column beyond the wave).  The
+// epilogue receives both points' neighbourhoods (Nb of values and of k) with
+// their row and column flags and does all stores, 16 bytes at a time:
+//   V::prep()                    once, before any load (reads device scalars)
+//   V::pair(i) -> double2        values at i, i + 1 (i even)
+//   V::one(i)  -> double         value at i
+//   E::pair(i, r, c0, rin, ia, ib, uA, uB, kA, kB, acc)
+// Rows past the grid are read from the last row (clamped) and never used.
+// Every lane walks the rows, so the cross-lane moves always see all 64 lanes.
+// With NS > 0 the epilogue's sums go through block_partial.
+template <int NS, bool VK, class V, class E>
+__global__ __launch_bounds__(BLOCK) void k_pair5(V v, const double* __restrict__ k, E e, int64_t rows, int64_t cols,
+                                                 int rpb, RedScratch rs, const int* skip) {
+	if (skip && *skip) return;
+	v.prep();
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int lane = (int)(threadIdx.x & (WAVE - 1));
+	const int64_t c0 = 2 * ((int64_t)bx * BLOCK + threadIdx.x);
+	const int64_t r0 = (int64_t)by * rpb;
+	const int64_t r1 = r0 + rpb < rows ? r0 + rpb : rows;
+	const bool ok = c0 < cols;
+	const int64_t cl = ok ? c0 : cols - 2;
+	const bool intA = c0 > 0, intB = c0 + 1 < cols - 1;
+	const bool edgeW = lane == 0 && ok && c0 > 0, edgeE = lane == WAVE - 1 && ok && c0 + 2 < cols;
+	struct Row {
+		double2 x, k;
+		double xw, xe, kw, ke;
+	};
+	auto ld = [&](int64_t r) {
+		const int64_t i = (r < rows ? r : rows - 1) * cols + cl;
+		Row o;
+		o.x = v.pair(i);
+		o.xw = edgeW ? v.one(i - 1) : 0.0;
+		o.xe = edgeE ? v.one(i + 2) : 0.0;
+		if constexpr (VK) {
+			o.k = *(const double2*)(k + i);
+			o.kw = edgeW ? k[i - 1] : 1.0;
+			o.ke = edgeE ? k[i + 2] : 1.0;
+		} else {
+			o.k = double2{1.0, 1.0};
+			o.kw = o.ke = 1.0;
+		}
+		return o;
+	};
+	Acc acc[NS > 0 ? NS : 1];
+#pragma unroll
+	for (int q = 0; q < (NS > 0 ? NS : 1); ++q) acc[q] = Acc{0.0, 0.0};
+	Row up = ld(r0 > 0 ? r0 - 1 : r0);
+	Row cur = ld(r0);
+	for (int64_t r = r0; r < r1; ++r) {
+		const Row dn = ld(r + 1);
+		double xw = dlane64<0x138>(cur.x.y), xe = dlane64<0x130>(cur.x.x);
+		double kw = 1.0, ke = 1.0;
+		if constexpr (VK) {
+			kw = dlane64<0x138>(cur.k.y);
+			ke = dlane64<0x130>(cur.k.x);
+		}
+		if (lane == 0) xw = cur.xw, kw = cur.kw;
+		if (lane == WAVE - 1) xe = cur.xe, ke = cur.ke;
+		if (ok) {
+			const Nb uA{cur.x.x, up.x.x, dn.x.x, xw, cur.x.y}, uB{cur.x.y, up.x.y, dn.x.y, cur.x.x, xe};
+			const Nb kA{cur.k.x, up.k.x, dn.k.x, kw, cur.k.y}, kB{cur.k.y, up.k.y, dn.k.y, cur.k.x, ke};
+			e.pair(r * cols + c0, r, c0, r > 0 && r < rows - 1, intA, intB, uA, uB, kA, kB, acc);
+		}
+		up = cur;
+		cur = dn;
+	}
+	if constexpr (NS > 0) block_partial<NS>(acc, rs);
+}
+// Value sources for k_pair5.
+struct PLoad {
+	const double* x;
+	__device__ void prep() {}
+	__device__ double2 pair(int64_t i) const { return *(const double2*)(x + i); }
+	__device__ double one(int64_t i) const { return x[i]; }
+};
+
 // Rows per block for the streaming kernels: enough blocks to fill 256 CUs
 // several times over, long enough columns to amortise the two halo rows.
 inline int stream_rpb(int64_t rows, int64_t cols) {

@@ -14,6 +14,7 @@
 #include "fx_mgfast.hpp"
 
 #include <cmath>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -260,6 +261,232 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(const T* co, T* f, int64_t cr
 	f[r * fc + col] += vy * top + wy * bot;
 }
 
+// ── streaming FP64 level kernels (even column counts) ────────────────
+// The per-point arithmetic of k_jac / k_lstencil on the two-column traversal of
+// fxd::k_pair5: the level's constant 5-point weights or, with VK, its
+// coefficient field with harmonic face means.  Results equal the per-point
+// kernels' bit for bit.
+
+// Weighted Jacobi sweep: xn = xo + (ω/d)·(b − A·xo) inside, xn = xo on the boundary.
+template <bool VK>
+struct EpiJacD {
+	const double* b;
+	double* xn;
+	LW<double> L;
+	__device__ double pt(const Nb& u, const Nb& k, double bv) const {
+		if constexpr (VK) {
+			double fs;
+			const double lv = vk_lap(u, k, L.ix2, L.iy2, &fs);
+			const double d = fma(L.scale, fs, L.sigma);
+			const double au = fma(L.scale, lv, L.sigma * u.c);
+			return fma(L.omega / d, bv - au, u.c);
+		} else {
+			double ax = 0.0;
+			ax = fma(L.w[0], u.c, ax);
+			ax = fma(L.w[1], u.n, ax);
+			ax = fma(L.w[2], u.s, ax);
+			ax = fma(L.w[3], u.w, ax);
+			ax = fma(L.w[4], u.e, ax);
+			return fma(L.omega, L.inv_diag * (bv - ax), u.c);
+		}
+	}
+	template <class A>
+	__device__ void pair(int64_t i, int64_t, int64_t, bool rin, bool ia, bool ib, const Nb& uA, const Nb& uB,
+	                     const Nb& kA, const Nb& kB, A*) const {
+		const double2 bv = *(const double2*)(b + i);
+		double2 o{uA.c, uB.c};
+		if (rin && ia) o.x = pt(uA, kA, bv.x);
+		if (rin && ib) o.y = pt(uB, kB, bv.y);
+		*(double2*)(xn + i) = o;
+	}
+};
+
+// Level stencil with bc_value boundaries: out = S(u) (mode 0) or rhs − S(u) (mode 1).
+template <bool VK>
+struct EpiLStD {
+	const double* rhs;
+	double* out;
+	LW<double> L;
+	fx_boundaries bc;
+	double dx, dy;
+	int64_t rows, cols;
+	int mode;
+	__device__ double in(const Nb& u, const Nb& k) const {
+		if constexpr (VK) {
+			double fs;
+			return fma(L.scale, vk_lap(u, k, L.ix2, L.iy2, &fs), L.sigma * u.c);
+		} else {
+			return fma(L.w[0], u.c, fma(L.w[1], u.n, fma(L.w[2], u.s, fma(L.w[3], u.w, L.w[4] * u.e))));
+		}
+	}
+	__device__ double bnd(int64_t r, int64_t c, const Nb& u) const {
+		if (r == 0) return bcv(bc.top, u.c, u.s, dy);
+		if (r == rows - 1) return bcv(bc.bottom, u.c, u.n, dy);
+		if (c == 0) return bcv(bc.left, u.c, u.e, dx);
+		return bcv(bc.right, u.c, u.w, dx);
+	}
+	template <class A>
+	__device__ void pair(int64_t i, int64_t r, int64_t c0, bool rin, bool ia, bool ib, const Nb& uA, const Nb& uB,
+	                     const Nb& kA, const Nb& kB, A*) const {
+		double2 s{rin && ia ? in(uA, kA) : bnd(r, c0, uA), rin && ib ? in(uB, kB) : bnd(r, c0 + 1, uB)};
+		if (mode == 1) {
+			const double2 rv = *(const double2*)(rhs + i);
+			s = double2{fma(1.0, rv.x, -s.x), fma(1.0, rv.y, -s.y)};
+		}
+		*(double2*)(out + i) = s;
+	}
+};
+
+// Rows per block of a two-column traversal: as tall as 1024 workgroups allow.
+int pair_rpb(int64_t rows, int64_t cols) {
+	const int64_t cb = (cols / 2 + BLOCK - 1) / BLOCK;
+	int rpb = 64;
+	while (rpb > 4 && cb * ((rows + rpb - 1) / rpb) < 1024) rpb /= 2;
+	return rpb;
+}
+
+// Linear-transfer restriction streamed down the fine rows.  A thread owns one
+// coarse column J of its block's 256-column strip; the block walks the fine
+// rows of a band of `rb` coarse rows once.  Each fine row's column sum is the
+// j-ordered sum of T(lin_weight)·f that k_restrict forms, and it is added, with
+// its row weight, to every coarse row whose window [lor, hir] holds that fine
+// row: at most three, the lowest open one (Ic) and the two above it, since a
+// fine row at coarse coordinate t lies in the windows of the rows in [t−1, t+1].
+// A coarse row is written once the fine rows pass its window.  Every coarse
+// value therefore gets exactly k_restrict's terms in k_restrict's order.
+template <class T, int MAXW>
+__global__ __launch_bounds__(BLOCK) void k_restrict_s(const T* __restrict__ f, T* __restrict__ co, int64_t fr,
+                                                      int64_t fc, int64_t cr, int64_t cc, int rb, fx_xtab X,
+                                                      const int* skip) {
+	SKIP_RETURN;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t J = (int64_t)bx * BLOCK + threadIdx.x;
+	const int64_t I0 = (int64_t)by * rb, I1 = I0 + rb < cr ? I0 + rb : cr;
+	if (J >= cc) return;   // no workgroup barrier below
+	if (J == 0 || J == cc - 1) {
+		for (int64_t I = I0; I < I1; ++I) co[I * cc + J] = T(0);
+		return;
+	}
+	if (I0 == 0) co[J] = T(0);
+	if (I1 == cr) co[(cr - 1) * cc + J] = T(0);
+	const int64_t Ia = I0 > 1 ? I0 : 1, Ib = I1 < cr - 1 ? I1 : cr - 1;   // interior coarse rows [Ia, Ib)
+	if (Ia >= Ib) return;
+	const T sy = T(double(cr - 1) / double(fr - 1)), sx = T(double(cc - 1) / double(fc - 1));
+	const T sc = sy * sx;
+	const int64_t clo = X.loc[J];
+	const int nw = (int)(X.hic[J] - clo + 1);
+	T cw[MAXW];
+#pragma unroll
+	for (int q = 0; q < MAXW; ++q) cw[q] = q < nw ? T(lin_weight(X.i0c, X.wc, clo + q, J)) : T(0);
+	int64_t Ic = Ia;
+	T a0 = T(0), a1 = T(0), a2 = T(0);   // coarse rows Ic, Ic + 1, Ic + 2
+	const int64_t ilo = X.lor[Ia], ihi = X.hir[Ib - 1];
+	for (int64_t i = ilo; i <= ihi; ++i) {
+		while (Ic < Ib && X.hir[Ic] < i) {
+			co[Ic * cc + J] = a0 * sc;
+			a0 = a1;
+			a1 = a2;
+			a2 = T(0);
+			++Ic;
+		}
+		const T* fi = f + i * fc + clo;
+		T row = T(0);
+#pragma unroll
+		for (int q = 0; q < MAXW; ++q)
+			if (q < nw) row += cw[q] * fi[q];
+		if (X.lor[Ic] <= i && i <= X.hir[Ic]) a0 += T(lin_weight(X.i0r, X.wr, i, Ic)) * row;
+		if (Ic + 1 < Ib && X.lor[Ic + 1] <= i && i <= X.hir[Ic + 1]) a1 += T(lin_weight(X.i0r, X.wr, i, Ic + 1)) * row;
+		if (Ic + 2 < Ib && X.lor[Ic + 2] <= i && i <= X.hir[Ic + 2]) a2 += T(lin_weight(X.i0r, X.wr, i, Ic + 2)) * row;
+	}
+	while (Ic < Ib) {
+		co[Ic * cc + J] = a0 * sc;
+		a0 = a1;
+		a1 = a2;
+		a2 = T(0);
+		++Ic;
+	}
+}
+constexpr int RESTRICT_MAXW = 8;
+
+// Linear-transfer restriction, one thread per coarse point with its whole
+// window's loads issued together (at most MAXW fine rows and columns): for
+// levels too small to give the streaming kernel enough workgroups, whose
+// sequential walk would then expose each fine row's latency.  The same terms
+// in the same order as k_restrict.
+template <class T, int MAXW>
+__global__ __launch_bounds__(BLOCK) void k_restrict_p(const T* __restrict__ f, T* __restrict__ co, int64_t fr,
+                                                      int64_t fc, int64_t cr, int64_t cc, fx_xtab X, const int* skip) {
+	SKIP_RETURN;
+	const int64_t col = (int64_t)blockIdx.x * WAVE + (threadIdx.x & (WAVE - 1));
+	const int64_t r = (int64_t)blockIdx.y * NWAVES + (threadIdx.x / WAVE);
+	if (r >= cr || col >= cc) return;
+	if (r == 0 || col == 0 || r == cr - 1 || col == cc - 1) {
+		co[r * cc + col] = T(0);
+		return;
+	}
+	const T sy = T(double(cr - 1) / double(fr - 1)), sx = T(double(cc - 1) / double(fc - 1));
+	const int64_t rlo = X.lor[r], clo = X.loc[col];
+	const int nh = (int)(X.hir[r] - rlo + 1), nw = (int)(X.hic[col] - clo + 1);
+	T cw[MAXW], rw[MAXW], v[MAXW][MAXW];
+#pragma unroll
+	for (int p = 0; p < MAXW; ++p) cw[p] = p < nw ? T(lin_weight(X.i0c, X.wc, clo + p, col)) : T(0);
+#pragma unroll
+	for (int q = 0; q < MAXW; ++q) rw[q] = q < nh ? T(lin_weight(X.i0r, X.wr, rlo + q, r)) : T(0);
+#pragma unroll
+	for (int q = 0; q < MAXW; ++q)
+#pragma unroll
+		for (int p = 0; p < MAXW; ++p) v[q][p] = q < nh && p < nw ? f[(rlo + q) * fc + clo + p] : T(0);
+	T acc = T(0);
+#pragma unroll
+	for (int q = 0; q < MAXW; ++q) {
+		if (q < nh) {
+			T row = T(0);
+#pragma unroll
+			for (int p = 0; p < MAXW; ++p)
+				if (p < nw) row += cw[p] * v[q][p];
+			acc += rw[q] * row;
+		}
+	}
+	co[r * cc + col] = acc * (sy * sx);
+}
+constexpr int RESTRICT_PMAXW = 5;
+
+// Linear prolongation (additive) streamed down the fine rows: a thread owns one
+// fine column; the four coarse values it interpolates between are reloaded only
+// when the coarse row pair (i0r[r], i0r[r] + 1) changes.  Per-point arithmetic
+// equals k_prolong's.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_prolong_s(const T* __restrict__ co, T* __restrict__ f, int64_t cr,
+                                                     int64_t cc, int64_t fr, int64_t fc, int rpb, fx_xtab X,
+                                                     const int* skip) {
+	SKIP_RETURN;
+	int bx, by;
+	xcd_tile(&bx, &by);
+	const int64_t c = (int64_t)bx * BLOCK + threadIdx.x;
+	if (c <= 0 || c >= fc - 1) return;
+	const int64_t r0 = (int64_t)by * rpb > 1 ? (int64_t)by * rpb : 1;
+	const int64_t r1 = (int64_t)by * rpb + rpb < fr - 1 ? (int64_t)by * rpb + rpb : fr - 1;
+	const T wx = T(X.wc[c]), vx = T(1) - wx;
+	const int64_t b0 = X.i0c[c], b1 = b0 + 1 < cc ? b0 + 1 : b0;
+	int64_t ra = -1;
+	T t0 = T(0), t1 = T(0), u0 = T(0), u1 = T(0);   // rows a0 (t) and a1 (u) at columns b0, b1
+	for (int64_t r = r0; r < r1; ++r) {
+		const int64_t a0 = X.i0r[r], a1 = a0 + 1 < cr ? a0 + 1 : a0;
+		if (a0 != ra) {
+			t0 = co[a0 * cc + b0];
+			t1 = co[a0 * cc + b1];
+			u0 = co[a1 * cc + b0];
+			u1 = co[a1 * cc + b1];
+			ra = a0;
+		}
+		const T wy = T(X.wr[r]), vy = T(1) - wy;
+		const T top = vx * t0 + wx * t1;
+		const T bot = vx * u0 + wx * u1;
+		f[r * fc + c] += vy * top + wy * bot;
+	}
+}
+
 dim3 pgrid(int64_t rows, int64_t cols) {
 	return dim3((unsigned)((cols + WAVE - 1) / WAVE), (unsigned)((rows + NWAVES - 1) / NWAVES));
 }
@@ -360,6 +587,25 @@ struct Driver {
 		if (mg->opt.smoother == FX_SMOOTH_CHEBYSHEV) return cheby(l, count);
 		LW<T> w = lw(L);
 		for (int it = 0; it < count; ++it) {
+			if constexpr (std::is_same_v<T, double>) {
+				if (pair_ok(L)) {
+					const int64_t rows = (int64_t)L.g.rows, cols = (int64_t)L.g.cols;
+					const int rpb = pair_rpb(rows, cols);
+					const dim3 grid = fx::stream_grid(rows, cols / 2, rpb);
+					const PLoad v{(const double*)L.u};
+					if (L.k)
+						hipLaunchKernelGGL((k_pair5<0, true, PLoad, EpiJacD<true>>), grid, dim3(BLOCK), 0, st, v,
+						                   (const double*)L.k, EpiJacD<true>{(const double*)L.rhs, (double*)L.tmp, w},
+						                   rows, cols, rpb, RedScratch{}, skip);
+					else
+						hipLaunchKernelGGL((k_pair5<0, false, PLoad, EpiJacD<false>>), grid, dim3(BLOCK), 0, st, v,
+						                   (const double*)nullptr,
+						                   EpiJacD<false>{(const double*)L.rhs, (double*)L.tmp, w}, rows, cols, rpb,
+						                   RedScratch{}, skip);
+					std::swap(L.u, L.tmp);
+					continue;
+				}
+			}
 			hipLaunchKernelGGL(k_jac<T>, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, st, (const T*)L.u, (T*)L.tmp,
 			                   (const T*)L.rhs, w, (int64_t)L.g.rows, (int64_t)L.g.cols, skip);
 			std::swap(L.u, L.tmp);
@@ -367,8 +613,32 @@ struct Driver {
 		return fx::check_launch("mg jacobi");
 	}
 
+	// FP64 levels without a velocity field and with an even column count take the
+	// two-column streaming kernels.
+	static bool pair_ok(const fx_multigrid::Level& L) { return !L.vx && L.g.cols % 2 == 0 && L.g.cols >= 2; }
+
 	fx_status lstencil(int l, const T* u, T* out, int mode) {
 		fx_multigrid::Level& L = mg->lv[l];
+		if constexpr (std::is_same_v<T, double>) {
+			if (pair_ok(L)) {
+				const int64_t rows = (int64_t)L.g.rows, cols = (int64_t)L.g.cols;
+				const int rpb = pair_rpb(rows, cols);
+				const dim3 grid = fx::stream_grid(rows, cols / 2, rpb);
+				if (L.k)
+					hipLaunchKernelGGL((k_pair5<0, true, PLoad, EpiLStD<true>>), grid, dim3(BLOCK), 0, st, PLoad{u},
+					                   (const double*)L.k,
+					                   EpiLStD<true>{(const double*)L.rhs, out, lw(L), mg->bc, L.g.dx, L.g.dy, rows, cols,
+					                                 mode},
+					                   rows, cols, rpb, RedScratch{}, skip);
+				else
+					hipLaunchKernelGGL((k_pair5<0, false, PLoad, EpiLStD<false>>), grid, dim3(BLOCK), 0, st, PLoad{u},
+					                   (const double*)nullptr,
+					                   EpiLStD<false>{(const double*)L.rhs, out, lw(L), mg->bc, L.g.dx, L.g.dy, rows,
+					                                  cols, mode},
+					                   rows, cols, rpb, RedScratch{}, skip);
+				return fx::check_launch("mg level stencil");
+			}
+		}
 		hipLaunchKernelGGL(k_lstencil<T>, pgrid(L.g.rows, L.g.cols), dim3(BLOCK), 0, st, u, (const T*)L.rhs, out, lw(L),
 		                   mg->bc, L.g.dx, L.g.dy, mode, (int64_t)L.g.rows, (int64_t)L.g.cols, skip);
 		return fx::check_launch("mg level stencil");
@@ -424,34 +694,105 @@ struct Driver {
 		return fx::check_launch("mg chebyshev");
 	}
 
+	fx_status restrict_level(int l) {
+		auto& lv = mg->lv;
+		const fx_grid& fg = lv[l].g;
+		const fx_grid& cg = lv[l + 1].g;
+		const int fw = mg->opt.transfer == FX_TRANSFER_FW_BILINEAR;
+		const int rb = 8;
+		// The streaming kernel where its bands give at least 1024 workgroups (measured at
+		// 4096² -> 2048² FP64: 61 µs against 83 µs per point; the per-point kernel is the
+		// faster one on every smaller level).
+		const bool stream = (int64_t)((cg.cols + BLOCK - 1) / BLOCK) * (int64_t)((cg.rows + rb - 1) / rb) >= 1024;
+		if (fw && !stream && lv[l].xt.gmax <= RESTRICT_PMAXW && lv[l].xt.gmaxr <= RESTRICT_PMAXW) {
+			hipLaunchKernelGGL((k_restrict_p<T, RESTRICT_PMAXW>), pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st,
+			                   (const T*)lv[l].tmp, (T*)lv[l + 1].rhs, (int64_t)fg.rows, (int64_t)fg.cols,
+			                   (int64_t)cg.rows, (int64_t)cg.cols, lv[l].xt, skip);
+		} else if (fw && lv[l].xt.gmax <= RESTRICT_MAXW) {
+			hipLaunchKernelGGL((k_restrict_s<T, RESTRICT_MAXW>),
+			                   dim3((unsigned)((cg.cols + BLOCK - 1) / BLOCK), (unsigned)((cg.rows + rb - 1) / rb)),
+			                   dim3(BLOCK), 0, st, (const T*)lv[l].tmp, (T*)lv[l + 1].rhs, (int64_t)fg.rows,
+			                   (int64_t)fg.cols, (int64_t)cg.rows, (int64_t)cg.cols, rb, lv[l].xt, skip);
+		} else {
+			hipLaunchKernelGGL(k_restrict<T>, pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st, (const T*)lv[l].tmp,
+			                   (T*)lv[l + 1].rhs, (int64_t)fg.rows, (int64_t)fg.cols, (int64_t)cg.rows, (int64_t)cg.cols,
+			                   fw, lv[l].xt, skip);
+		}
+		return FX_OK;
+	}
+
+	fx_status prolong_level(int l) {
+		auto& lv = mg->lv;
+		const fx_grid& fg = lv[l].g;
+		const fx_grid& cg = lv[l + 1].g;
+		const int fw = mg->opt.transfer == FX_TRANSFER_FW_BILINEAR;
+		if (fw) {
+			// Rows per block: the walk is sequential, so small levels take short walks
+			// and more workgroups.
+			const int64_t cb = (int64_t)((fg.cols + BLOCK - 1) / BLOCK);
+			int rpb = 16;
+			while (rpb > 1 && cb * (((int64_t)fg.rows + rpb - 1) / rpb) < 2048) rpb /= 2;
+			hipLaunchKernelGGL(k_prolong_s<T>,
+			                   dim3((unsigned)((fg.cols + BLOCK - 1) / BLOCK), (unsigned)((fg.rows + rpb - 1) / rpb)),
+			                   dim3(BLOCK), 0, st, (const T*)lv[l + 1].u, (T*)lv[l].u, (int64_t)cg.rows, (int64_t)cg.cols,
+			                   (int64_t)fg.rows, (int64_t)fg.cols, rpb, lv[l].xt, skip);
+		} else {
+			hipLaunchKernelGGL(k_prolong<T>, pgrid(fg.rows, fg.cols), dim3(BLOCK), 0, st, (const T*)lv[l + 1].u,
+			                   (T*)lv[l].u, (int64_t)cg.rows, (int64_t)cg.cols, (int64_t)fg.rows, (int64_t)fg.cols, fw,
+			                   lv[l].xt, skip);
+		}
+		return FX_OK;
+	}
+
 	fx_status run(double* z, const double* r) {
 		auto& lv = mg->lv;
 		const int nl = (int)lv.size();
 		const auto& o = mg->opt;
-		const int fw = o.transfer == FX_TRANSFER_FW_BILINEAR;  // linear same-domain transfer
 		size_t n0 = n(lv[0]);
-		hipLaunchKernelGGL(k_cvt_in<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, r, (const double*)z, (T*)lv[0].rhs, (T*)lv[0].u,
-		                   n0, skip);
-		for (int l = 0; l < nl - 1; ++l) {
-			FX_CHECK(smooth(l, o.n_pre));
-			FX_CHECK(lstencil(l, (const T*)lv[l].u, (T*)lv[l].tmp, 1));
-			const fx_grid& fg = lv[l].g;
-			const fx_grid& cg = lv[l + 1].g;
-			hipLaunchKernelGGL(k_restrict<T>, pgrid(cg.rows, cg.cols), dim3(BLOCK), 0, st, (const T*)lv[l].tmp,
-			                   (T*)lv[l + 1].rhs, (int64_t)fg.rows, (int64_t)fg.cols, (int64_t)cg.rows, (int64_t)cg.cols,
-			                   fw, lv[l].xt, skip);
+		// FP64 levels work on the caller's vectors in place of the copies
+		// rhs ← r, u ← z, z ← u (multigrid.rs:194-195, 228): level 0 reads r as its
+		// right-hand side and iterates in z, the level's own u buffer standing aside.
+		// The values are those of the copies.  Not when z and r share memory.
+		fx_multigrid::Level& L0 = lv[0];
+		void* const own_u = L0.u;
+		void* const own_rhs = L0.rhs;
+		void* const own_tmp = L0.tmp;
+		const bool alias = std::is_same_v<T, double> && (const void*)z != (const void*)r;
+		if (alias) {
+			L0.u = z;
+			L0.rhs = const_cast<double*>(r);
+		} else {
+			hipLaunchKernelGGL(k_cvt_in<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, r, (const double*)z, (T*)lv[0].rhs,
+			                   (T*)lv[0].u, n0, skip);
+		}
+		fx_status s = FX_OK;
+		for (int l = 0; l < nl - 1 && s == FX_OK; ++l) {
+			s = smooth(l, o.n_pre);
+			if (s == FX_OK) s = lstencil(l, (const T*)lv[l].u, (T*)lv[l].tmp, 1);
+			if (s == FX_OK) s = restrict_level(l);
 			size_t nc = n(lv[l + 1]);
 			hipLaunchKernelGGL(k_fillT<T>, dim3(eb(nc)), dim3(BLOCK), 0, st, (T*)lv[l + 1].u, T(0), nc, skip);
 		}
-		FX_CHECK(smooth(nl - 1, o.n_pre + o.n_post + o.extra_coarse));
-		for (int l = nl - 2; l >= 0; --l) {
-			const fx_grid& fg = lv[l].g;
-			const fx_grid& cg = lv[l + 1].g;
-			hipLaunchKernelGGL(k_prolong<T>, pgrid(fg.rows, fg.cols), dim3(BLOCK), 0, st, (const T*)lv[l + 1].u, (T*)lv[l].u,
-			                   (int64_t)cg.rows, (int64_t)cg.cols, (int64_t)fg.rows, (int64_t)fg.cols, fw, lv[l].xt, skip);
-			FX_CHECK(smooth(l, o.n_post));
+		if (s == FX_OK) s = smooth(nl - 1, o.n_pre + o.n_post + o.extra_coarse);
+		for (int l = nl - 2; l >= 0 && s == FX_OK; --l) {
+			s = prolong_level(l);
+			if (s == FX_OK) s = smooth(l, o.n_post);
 		}
-		hipLaunchKernelGGL(k_cvt_out<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, (const T*)lv[0].u, z, n0, skip);
+		if (s == FX_OK) {
+			if (alias) {
+				if (L0.u != (void*)z)
+					hipLaunchKernelGGL(k_copyT<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, (const T*)L0.u, (T*)z, n0, skip);
+			} else {
+				hipLaunchKernelGGL(k_cvt_out<T>, dim3(eb(n0)), dim3(BLOCK), 0, st, (const T*)lv[0].u, z, n0, skip);
+			}
+		}
+		if (alias) {
+			// The smoother swapped u and tmp between z and the level's tmp buffer.
+			L0.u = own_u;
+			L0.tmp = own_tmp;
+			L0.rhs = own_rhs;
+		}
+		FX_CHECK(s);
 		return fx::check_launch("mg vcycle");
 	}
 };
@@ -540,6 +881,8 @@ fx_status build_xtab(fx_backend* be, fx_multigrid::Level& L, const fx_grid& c) {
 	X.rx = (int)span(cc, fmg::RCX, loc, hic) + 4;
 	X.gmax = 0;
 	for (int64_t J = 1; J + 1 < cc; ++J) X.gmax = std::max<int>(X.gmax, hic[J] - loc[J] + 1);
+	X.gmaxr = 0;
+	for (int64_t I = 1; I + 1 < cr; ++I) X.gmaxr = std::max<int>(X.gmaxr, hir[I] - lor[I] + 1);
 	// Fine window (two columns of halo each side) of the wave restriction's strips of RCX interior
 	// coarse columns.
 	int64_t wspan = 1;

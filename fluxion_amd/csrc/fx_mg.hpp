@@ -18,6 +18,7 @@ struct fx_xtab {
 	float sy = 0.0f, sx = 0.0f;     // float((nc−1)/(nf−1)) per axis: the restriction scale (h/H)
 	int ry = 0, rx = 0;             // LDS region (halo included) of the fused restriction's widest tile
 	int gmax = 0;                   // most fine columns any coarse column gathers from
+	int gmaxr = 0;                  // most fine rows any coarse row gathers from
 	int wx = 0;                     // fine window (halo included) of the wave restriction's widest strip
 	int adv = 0;                    // widest fine-column advance between the wave restriction's consecutive strips
 };

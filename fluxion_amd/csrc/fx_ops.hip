@@ -445,6 +445,22 @@ __global__ __launch_bounds__(BLOCK) void k_vark(double sigma, double alpha, cons
 	out[i] = fma(alpha, L, sigma * cv);
 }
 
+// k_vark on the two-column traversal (even column counts): the same per-point
+// arithmetic, identity boundary.
+struct EpiOpVk {
+	double sigma, alpha, ix2, iy2;
+	double* out;
+	template <class A>
+	__device__ void pair(int64_t i, int64_t, int64_t, bool rin, bool ia, bool ib, const Nb& uA, const Nb& uB,
+	                     const Nb& kA, const Nb& kB, A*) const {
+		double2 o{uA.c, uB.c};
+		double fs;
+		if (rin && ia) o.x = fma(alpha, vk_lap(uA, kA, ix2, iy2, &fs), sigma * uA.c);
+		if (rin && ib) o.y = fma(alpha, vk_lap(uB, kB, ix2, iy2, &fs), sigma * uB.c);
+		*(double2*)(out + i) = o;
+	}
+};
+
 // Multigrid transfer (ops.rs:369-423); prolong clamps the coarse index for
 // odd fine sizes where the reference would index past the coarse array.
 __global__ void k_restrict_ref(const double* f, double* c, int64_t fr, int64_t fc, int64_t cr, int64_t cc) {
@@ -581,6 +597,16 @@ fx_status k_apply_operator(fx_backend* be, const fx_operator* op, const double* 
 	if (op->kind == FX_OP_VARK) {
 		FX_REQUIRE(op->k, FX_ERR_ARGUMENT, "variable-coefficient operator needs k");
 		FX_CHECK(check_buf(op->k, n, FX_F64, "k"));
+		if (cols % 2 == 0) {
+			const int64_t cb = (cols / 2 + BLOCK - 1) / BLOCK;
+			int rpb = 64;
+			while (rpb > 4 && cb * ((rows + rpb - 1) / rpb) < 1024) rpb /= 2;
+			hipLaunchKernelGGL((k_pair5<0, true, PLoad, EpiOpVk>), fx::stream_grid(rows, cols / 2, rpb), dim3(BLOCK), 0,
+			                   be->stream, PLoad{x}, CD(op->k),
+			                   EpiOpVk{op->sigma, op->coeff, 1.0 / (g->dx * g->dx), 1.0 / (g->dy * g->dy), y}, rows, cols,
+			                   rpb, RedScratch{}, skip);
+			return check_launch("vark operator");
+		}
 		hipLaunchKernelGGL(k_vark, pgrid(rows, cols), dim3(BLOCK), 0, be->stream, op->sigma, op->coeff, CD(op->k), x, y,
 		                   rows, cols, 1.0 / (g->dx * g->dx), 1.0 / (g->dy * g->dy), skip);
 		return check_launch("vark operator");

@@ -367,6 +367,34 @@ struct FinTrueRes {
 	__device__ void operator()(Acc* o) const { st->tres = sqrt(acc_value(o[0])); }
 };
 
+// k_pair5 value source p = src + β·p_old (VPNew's arithmetic), β read from the state.
+struct VPNew2 {
+	const double *src, *pold;
+	const SState* st;
+	double be;
+	__device__ void prep() { be = st->beta; }
+	__device__ double2 pair(int64_t i) const {
+		const double2 s = *(const double2*)(src + i), p = *(const double2*)(pold + i);
+		return double2{s.x + be * p.x, s.y + be * p.y};
+	}
+	__device__ double one(int64_t i) const { return src[i] + be * pold[i]; }
+};
+// EpiCGv on k_pair5's pairs: ap = A·p inside (k_vark's arithmetic), ap = p on the boundary; writes p; p·ap.
+struct EpiCGvk {
+	double sigma, alpha, ix2, iy2;
+	double *pout, *ap;
+	__device__ void pair(int64_t i, int64_t, int64_t, bool rin, bool ia, bool ib, const Nb& uA, const Nb& uB,
+	                     const Nb& kA, const Nb& kB, Acc* a) const {
+		double2 o{uA.c, uB.c};
+		double fs;
+		if (rin && ia) o.x = fma(alpha, vk_lap(uA, kA, ix2, iy2, &fs), sigma * uA.c);
+		if (rin && ib) o.y = fma(alpha, vk_lap(uB, kB, ix2, iy2, &fs), sigma * uB.c);
+		*(double2*)(pout + i) = double2{uA.c, uB.c};
+		*(double2*)(ap + i) = o;
+		acc_add(a[0], uA.c * o.x);
+		acc_add(a[0], uB.c * o.y);
+	}
+};
 // ap = A·p inside (operator value given), ap = p on the boundary; writes p_new; p·ap.
 struct EpiCGv {
 	double *pout, *ap;
@@ -1610,6 +1638,21 @@ struct Ctx {
 			hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be),
 			                   grid.x * grid.y, FinAlpha{d}, done);
 			return check_launch("cg fused step");
+		}
+		if (op->kind == FX_OP_VARK && cols % 2 == 0) {
+			// p = src + β·p_old, ap = A·p and p·ap in one two-column pass over src, p_old and k.
+			const int64_t cb = (cols / 2 + BLOCK - 1) / BLOCK;
+			int rpb = 64;
+			while (rpb > 4 && cb * ((rows + rpb - 1) / rpb) < 1024) rpb /= 2;
+			const dim3 grid = fx::stream_grid(rows, cols / 2, rpb);
+			FX_CHECK(fx::ensure_partials(be, (size_t)grid.x * grid.y));
+			hipLaunchKernelGGL((k_pair5<1, true, VPNew2, EpiCGvk>), grid, dim3(BLOCK), 0, st, VPNew2{src, pold, d, 0.0},
+			                   (const double*)op->k->ptr,
+			                   EpiCGvk{op->sigma, op->coeff, 1.0 / (s->g.dx * s->g.dx), 1.0 / (s->g.dy * s->g.dy), pout, ap},
+			                   rows, cols, rpb, fx::red_scratch(be), done);
+			hipLaunchKernelGGL((k_finish<1, FinAlpha>), dim3(1), dim3(FIN_BLOCK), 0, st, fx::red_scratch(be),
+			                   grid.x * grid.y, FinAlpha{d}, done);
+			return check_launch("cg fused step (variable coefficient)");
 		}
 		FX_CHECK(eupd(FPUpd{d, src, pold, pout, 0.0}, "p update"));
 		FX_CHECK(apply_op(pout, ap));

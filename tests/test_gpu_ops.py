@@ -214,6 +214,18 @@ def test_conv_diff_bitwise(be):
     assert np.array_equal(Fo.to_vec_f64(), O.conv_diff(1, 0.37, u, k, vx, vy, g, out=y))
 
 
+@pytest.mark.parametrize("shape", [(200, 214), (5, 4), (257, 130), (3, 6)])
+def test_vark_operator_pair_bitwise(be, shape):
+    # Even column counts take the two-column streaming form of the variable-coefficient operator.
+    hip = H()
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    u, k = rnd(rows * cols, 40), np.exp(rnd(rows * cols, 41))
+    out = hip.Field.zeros(g, be=be)
+    hip.Operator.var_k(field(be, g, k), coeff=0.5, sigma=1.0).apply(field(be, g, u), out)
+    assert np.array_equal(out.to_vec_f64(), O.apply_op(O.Op.var_k(k, coeff=0.5, sigma=1.0), u, g))
+
+
 def test_operators_bitwise(be):
     hip = H()
     n = 200

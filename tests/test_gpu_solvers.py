@@ -360,6 +360,64 @@ def test_fused_fcg_matches_unfused(be, n):
     assert rel_l2(out[0][0], out[1][0]) < 1e-9
 
 
+@pytest.mark.parametrize("shape", [(128, 128), (96, 130), (201, 256), (40, 2050)])
+@pytest.mark.parametrize("vk,transfer", [(True, 1), (False, 1), (False, 0)])
+def test_vcycle_fp64_streaming_bitwise(be, shape, vk, transfer):
+    # FP64 levels with an even column count take the two-column streaming Jacobi and residual
+    # kernels, the streaming or per-point linear restriction and the streaming prolongation, and
+    # iterate in the caller's vector: bit for bit the oracle's V-cycle from a nonzero initial guess.
+    hip = H()
+    rows, cols = shape
+    g = Grid(rows, cols, 1.0 / (cols - 1), 1.0 / (rows - 1))
+    rng = np.random.default_rng(rows + cols)
+    r = O.zero_boundaries(rng.standard_normal(rows * cols), rows, cols)
+    x0 = rng.standard_normal(rows * cols)
+    k = np.exp(0.5 * rng.standard_normal(rows * cols)) if vk else None
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer(transfer), coarsest=16, precision=DType.F64,
+                       scale=1.0, k=hip.Field.from_f64(g, k, be) if vk else None, be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=transfer, coarsest=16, precision=0,
+                   scale=1.0, k=k)
+    X = hip.Field.from_f64(g, x0, be)
+    B = hip.Field.from_f64(g, r, be)
+    mg.v_cycle(X, B)
+    ref = O.vcycle(x0, r, g, cfg)
+    assert np.array_equal(X.to_vec_f64(), ref)
+    assert np.array_equal(B.to_vec_f64(), r)   # the right-hand side is read, never written
+    mg.v_cycle(X, B)   # the level's own buffers are intact after the first cycle
+    assert np.array_equal(X.to_vec_f64(), O.vcycle(ref, r, g, cfg))
+
+
+def test_vcycle_same_buffer(be):
+    # x and b in one buffer: the cycle copies the right-hand side first (multigrid.rs:194-195).
+    hip = H()
+    n = 96
+    g = Grid.square(n, 1.0 / (n - 1))
+    r = O.zero_boundaries(np.random.default_rng(3).standard_normal(n * n), n, n)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F64,
+                       scale=1.0, be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=0, scale=1.0)
+    X = hip.Field.from_f64(g, r, be)
+    mg.v_cycle(X, X)
+    assert np.array_equal(X.to_vec_f64(), O.vcycle(r, r, g, cfg))
+
+
+def test_vark_vcycle_fp64_4096_bitwise(be):
+    # The plain path's FP64 cycle at the headline size, where the fine restriction streams.
+    hip = H()
+    n = 4096
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    r = O.zero_boundaries(np.random.default_rng(5).standard_normal(n * n), n, n)
+    mg = hip.Multigrid(g, ZD, 2, 2, 2.0 / 3.0, transfer=Transfer.FW_BILINEAR, coarsest=16, precision=DType.F64,
+                       scale=1.0, k=hip.Field.from_f64(g, k, be), be=be)
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, smoother=O.SMOOTH_JACOBI, transfer=1, coarsest=16, precision=0, scale=1.0, k=k)
+    X = hip.Field.zeros(g, be=be)
+    mg.v_cycle(X, hip.Field.from_f64(g, r, be))
+    got, ref = X.to_vec_f64(), O.vcycle(np.zeros(n * n), r, g, cfg)
+    print(f"FP64 var-k V-cycle 4096^2: max |diff| {np.max(np.abs(got - ref)):.3e}")
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("precision,fused,transfer", [(0, False, 1), (1, False, 1), (1, True, 1), (0, False, 0)])
 def test_vark_vcycle_bitwise(be, precision, fused, transfer):
     # Coefficient-aware levels (level operator −∇·(k∇·), coarse k interpolated) match the oracle bit for bit.
