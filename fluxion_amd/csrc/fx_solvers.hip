@@ -1869,6 +1869,7 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	fx_solve_stats st{};
 	double* pp[2] = {p0, p1};
 	double prev_tr = INFINITY;
+	bool at_floor = false;
 	for (int pass = 0; pass < 8; ++pass) {
 		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used), c.pc));
 		// The fused even-column step takes p_old = 0 in its first iteration itself.
@@ -1994,11 +1995,13 @@ fx_status solve_fcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
 		// A restart that did not halve the true residual has reached the FP64
 		// attainable accuracy (about eps·‖A‖·‖x‖); further passes only repeat it.
-		if (!(tr < 0.5 * prev_tr)) break;
+		if (!(tr < 0.5 * prev_tr)) { at_floor = true; break; }
 		prev_tr = tr;
 	}
 	st.iterations = used;
-	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol)) st.reason = FX_MAX_ITERATIONS;
+	// Converged means the recurrence met the tolerance (CgSolver's criterion, cg.rs:118-124) and the
+	// true residual is below it or at the FP64 floor; the true residual is reported either way.
+	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol) && !at_floor) st.reason = FX_MAX_ITERATIONS;
 	if (stats) *stats = st;
 	return FX_OK;
 }
@@ -2061,6 +2064,7 @@ fx_status solve_pfcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) 
 	uint64_t used = 0;
 	fx_solve_stats st{};
 	double prev_tr = INFINITY;
+	bool at_floor = false;
 	for (int pass = 0; pass < 8; ++pass) {
 		FX_CHECK(reset_state(s, (int64_t)(s->max_iter - used)));
 		FX_CHECK(c.apply_op(x, w, nullptr));
@@ -2097,11 +2101,11 @@ fx_status solve_pfcg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) 
 		used += st.iterations;
 		st.true_residual = tr;
 		if (st.reason != FX_CONVERGED || tr < s->tol || used >= s->max_iter) break;
-		if (!(tr < 0.5 * prev_tr)) break;
+		if (!(tr < 0.5 * prev_tr)) { at_floor = true; break; }
 		prev_tr = tr;
 	}
 	st.iterations = used;
-	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol)) st.reason = FX_MAX_ITERATIONS;
+	if (st.reason == FX_CONVERGED && !(st.true_residual < s->tol) && !at_floor) st.reason = FX_MAX_ITERATIONS;
 	if (stats) *stats = st;
 	return FX_OK;
 }

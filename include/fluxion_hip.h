@@ -264,7 +264,10 @@ typedef enum {
 	FX_SOLVER_PIPELINED_CG = 1,  /* PipelinedCgSolver (pipelined_cg.rs:25-238) */
 	FX_SOLVER_BICGSTAB = 2,      /* BiCgStabSolver (bicgstab.rs:19-346) */
 	FX_SOLVER_GMRES = 3,         /* GmresSolver (gmres.rs:24-490) */
-	FX_SOLVER_FCG = 4,           /* extension: flexible CG with FP64 true-residual refinement */
+	FX_SOLVER_FCG = 4,           /* extension: flexible CG with FP64 true-residual refinement: after the
+	                              * recurrence meets tol, restarts from x while ‖b − A·x‖ ≥ tol and each
+	                              * restart halves it; FX_CONVERGED if the true residual ends below tol or
+	                              * at that FP64 floor (stats.true_residual says which) */
 	FX_SOLVER_FGMRES = 5         /* extension: flexible right-preconditioned GMRES(m) (Saad 1993): each step
 	                              * stores z_k = M⁻¹v_k and applies A to it, x += Σ y_i·z_i, so the Givens
 	                              * estimate is ‖b − A·x‖ and the preconditioner may vary (FP32 V-cycle);

@@ -1034,6 +1034,7 @@ int orc_fcg(double* x, const double* b, int64_t rows, int64_t cols, double dx, d
 	std::vector<double> r(n), z(n), p(n), ap(n);
 	int64_t it = 0;
 	double r_norm = 0.0, prev_tr = INFINITY;
+	bool at_floor = false;
 	for (int restart = 0; restart < 8; ++restart) {
 		apply_op(*op, x, ap.data(), g);
 		copy(b, r.data(), n);
@@ -1041,8 +1042,9 @@ int orc_fcg(double* x, const double* b, int64_t rows, int64_t cols, double dx, d
 		r_norm = std::sqrt(dot(r.data(), r.data(), n));
 		if (r_norm < tol) { set_stats(st, it, r_norm, CONVERGED); return OK; }
 		// A restart that did not halve the true residual has reached the FP64
-		// attainable accuracy; further passes only repeat it.
-		if (restart > 0 && !(r_norm < 0.5 * prev_tr)) break;
+		// attainable accuracy; further passes only repeat it.  The recurrence of the
+		// pass before met the tolerance, so the solve counts as converged.
+		if (restart > 0 && !(r_norm < 0.5 * prev_tr)) { at_floor = true; break; }
 		prev_tr = r_norm;
 		fill(z.data(), 0.0, n);
 		pc.apply(z.data(), r.data(), n);
@@ -1074,7 +1076,7 @@ int orc_fcg(double* x, const double* b, int64_t rows, int64_t cols, double dx, d
 	copy(b, r.data(), n);
 	axpy(-1.0, ap.data(), r.data(), n);
 	r_norm = std::sqrt(dot(r.data(), r.data(), n));
-	set_stats(st, it, r_norm, r_norm < tol ? CONVERGED : MAX_ITERATIONS);
+	set_stats(st, it, r_norm, r_norm < tol || at_floor ? CONVERGED : MAX_ITERATIONS);
 	return OK;
 }
 
@@ -1129,6 +1131,7 @@ int pfcg(double* x, const double* b, const Grid& g, const orc_op* op, Pc& pc, do
 	std::vector<double> r(N), u(N), w(N), p(N), s(N);
 	int64_t used = 0;
 	double prev_tr = INFINITY, rn = 0.0, tr = 0.0;
+	bool at_floor = false;
 	int reason = CONVERGED;
 	for (int pass = 0; pass < 8; ++pass) {
 		apply_op(*op, x, w.data(), g);
@@ -1169,10 +1172,10 @@ int pfcg(double* x, const double* b, const Grid& g, const orc_op* op, Pc& pc, do
 		axpy(-1.0, w.data(), tres.data(), N);
 		tr = std::sqrt(dot(tres.data(), tres.data(), N));
 		if (reason != CONVERGED || tr < tol || used >= max_iter) break;
-		if (!(tr < 0.5 * prev_tr)) break;
+		if (!(tr < 0.5 * prev_tr)) { at_floor = true; break; }
 		prev_tr = tr;
 	}
-	if (reason == CONVERGED && !(tr < tol)) reason = MAX_ITERATIONS;
+	if (reason == CONVERGED && !(tr < tol) && !at_floor) reason = MAX_ITERATIONS;
 	set_stats(st, used, rn, reason);
 	return OK;
 }

@@ -466,6 +466,28 @@ def test_fcg_fp32_reaches_fp64_true_residual():
     assert st.iterations <= 16
 
 
+def test_refinement_stops_converged_at_the_fp64_floor():
+    # A tolerance below FP64's attainable true residual (1e-14·‖f‖; the floor at 64² is about
+    # 8e-14·‖f‖, growing as 1/h²): the recurrence still meets it, the refinement passes stop once a
+    # restart no longer halves ‖f − A·x‖, and the solve counts as converged with that floor reported.
+    # A budget the first pass cannot finish in stays MAX_ITERATIONS.
+    n = 64
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    from fluxion_amd.inputs import gaussian_bumps
+    f = gaussian_bumps(n).ravel()
+    op = O.Op.stencil(1.0, Stencil.poisson_minus_identity(h, h))
+    cfg = O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16, precision=1, scale=-1.0)
+    tol = 1e-14 * np.linalg.norm(f)
+    for solve in (O.fcg, O.ppcg):
+        x, st = solve(np.zeros_like(f), f, g, op, tol, 400, mg=cfg)
+        true = np.linalg.norm(f - O.apply_op(op, x, g))
+        assert st.reason == ConvergenceReason.CONVERGED and st.iterations < 400, (solve, st.iterations)
+        assert 2 * tol < true < 2e-13 * np.linalg.norm(f), (solve, true)
+        _, st2 = solve(np.zeros_like(f), f, g, op, tol, 5, mg=cfg)
+        assert st2.reason == ConvergenceReason.MAX_ITERATIONS and st2.iterations == 5
+
+
 def test_planted_mistake_is_caught():
     # A transposed spacing (dx <-> dy) in the stencil on a rectangular grid fails the quadratic
     # closed form: the checks above discriminate such errors.
