@@ -98,6 +98,26 @@ def poisson_residual_noise(x: np.ndarray, f: np.ndarray, n: int, h: float) -> tu
             float(np.linalg.norm((out[0] - out[1]).astype(np.float64))) / fn)
 
 
+def unet_forward_record(be, net, n: int, reps: int = 10) -> dict:
+    """One network forward on an n×n two-channel input (15 convolutions, 61,920 FLOP per pixel), timed
+    over `reps` forwards after two warm-ups; inputs and outputs stay on the device."""
+    from fluxion_amd.core import DType
+    XIN = be.allocate(n * n * 2, DType.F32)
+    YOUT = be.allocate(n * n, DType.F32)
+    for _ in range(2):
+        net.forward_device(XIN, YOUT, 1, n, n)
+    be.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward_device(XIN, YOUT, 1, n, n)
+    be.synchronize()
+    t_f = (time.perf_counter() - t0) / reps
+    flop = 61920.0 * n * n
+    del XIN, YOUT
+    return {"grid": n, "ms_per_forward": 1e3 * t_f, "flop_per_forward": flop, "TFLOPs": flop / t_f / 1e12,
+            "frac_bf16_dense_peak": flop / t_f / BF16_PEAK_FLOPS}
+
+
 def sources_sha16() -> str:
     import hashlib
     h = hashlib.sha256()
@@ -129,6 +149,8 @@ def main():
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in wave segment heights")
+    ap.add_argument("--unet-only", action="store_true",
+                    help="only time the U-Net forward at --size (committed weights) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run Backend::copy of 2^24 doubles 20 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -141,6 +163,17 @@ def main():
     from fluxion_amd import hip
     from fluxion_amd.core import Boundaries, DType, Grid, Stencil, Transfer
     from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+
+    if args.unet_only:
+        # The network forward alone (committed weights), for kernel traces of the convolutions.
+        from fluxion_amd.learned import DEFAULT_WEIGHTS, UNet, load_weights
+        be = hip.HipBackend(0)
+        net = UNet(be, 2)
+        net.set_params(load_weights(args.learned or DEFAULT_WEIGHTS))
+        rec = unet_forward_record(be, net, args.size)
+        log(f"bench: U-Net forward {args.size}^2 {rec['ms_per_forward']:.2f} ms, {rec['TFLOPs']:.0f} TFLOP/s")
+        print(json.dumps({"unet_forward": rec}), flush=True)
+        return
 
     if args.copy_calibration:
         # A copy of known size: 2^24 doubles read and written per launch (128 MiB + 128 MiB).
@@ -239,24 +272,12 @@ def main():
                    "weights": Path(wpath).name,
                    "note": "the network correction costs more iterations and time than it saves; the headline "
                            "value is the mixed path"}
-        # The network alone: one forward at this size (15 convolutions, 61,920 FLOP per pixel).
-        XIN = be.allocate(n * n * 2, DType.F32)
-        YOUT = be.allocate(n * n, DType.F32)
-        for _ in range(2):
-            net.forward_device(XIN, YOUT, 1, n, n)
-        be.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            net.forward_device(XIN, YOUT, 1, n, n)
-        be.synchronize()
-        t_f = (time.perf_counter() - t0) / 10
-        flop = 61920.0 * n * n
-        unet_rec = {"grid": n, "ms_per_forward": 1e3 * t_f, "flop_per_forward": flop, "TFLOPs": flop / t_f / 1e12,
-                    "frac_bf16_dense_peak": flop / t_f / BF16_PEAK_FLOPS,
-                    "mixed_fcg_iterations_per_forward": t_f / (t_mixed / max(1, s_mixed.iterations))}
-        log(f"bench: U-Net forward {n}^2 {1e3 * t_f:.2f} ms, {flop / t_f / 1e12:.0f} TFLOP/s "
-            f"({100 * flop / t_f / BF16_PEAK_FLOPS:.1f}% of BF16 dense peak)")
-        del XIN, YOUT
+        # The network alone: one forward at this size.
+        unet_rec = unet_forward_record(be, net, n)
+        t_f = 1e-3 * unet_rec["ms_per_forward"]
+        unet_rec["mixed_fcg_iterations_per_forward"] = t_f / (t_mixed / max(1, s_mixed.iterations))
+        log(f"bench: U-Net forward {n}^2 {1e3 * t_f:.2f} ms, {unet_rec['TFLOPs']:.0f} TFLOP/s "
+            f"({100 * unet_rec['frac_bf16_dense_peak']:.1f}% of BF16 dense peak)")
 
     # Per-kernel probes of the mixed solve (the x/r update was probed over the timed region).
     kb = kernel_bytes(True, mgk == MgKernels.WAVE and n % 2 == 0)
