@@ -48,8 +48,10 @@ constexpr int CH[4] = {16, 32, 48, 64};
 
 inline int pad16(int c) { return (c + 15) / 16 * 16; }
 
-// Source of a convolution's staged input tile, and the form of its output.
-enum { SRC_BF16 = 0, SRC_UPCAT = 1, SRC_F32 = 2 };
+// Source of a convolution's staged input tile, and the form of its output.  SRC_F32IN is the
+// network's raw FP32 input [B][H][W][ca], rounded to BF16 and zero-padded to cin_p as it is staged
+// (the bits k_pack_in would store), so inference needs no packed copy of it.
+enum { SRC_BF16 = 0, SRC_UPCAT = 1, SRC_F32 = 2, SRC_F32IN = 3 };
 enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_F32_CH0 = 2 };
 
 struct ConvArgs {
@@ -104,6 +106,15 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 					v = *(const u4v*)((const bf16*)a.in + pa * a.ca + ch);
 				} else {
 					v = *(const u4v*)(a.in2 + pix * (a.cin_p - a.ca) + (ch - a.ca));
+				}
+			} else if constexpr (SRC == SRC_F32IN) {
+				const float* src = (const float*)a.in + pix * a.ca;
+#pragma unroll
+				for (int e = 0; e < 4; ++e) {
+					const int c = ch + 2 * e;
+					const bf16 lo = c < a.ca ? f2bf(src[c]) : (bf16)0;
+					const bf16 hi = c + 1 < a.ca ? f2bf(src[c + 1]) : (bf16)0;
+					v[e] = (uint32_t)lo | ((uint32_t)hi << 16);
 				}
 			} else {
 				const float* src = (const float*)a.in + pix * a.cin_p + ch;
@@ -659,6 +670,7 @@ fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int
 	a.skip = data_grad ? nullptr : u->skip;
 	hipStream_t st = u->be->stream;
 	if (src == SRC_BF16 && outk == OUT_BF16) return launch_conv<SRC_BF16, OUT_BF16>(a, st);
+	if (src == SRC_F32IN && outk == OUT_BF16) return launch_conv<SRC_F32IN, OUT_BF16>(a, st);
 	if (src == SRC_UPCAT && outk == OUT_BF16) return launch_conv<SRC_UPCAT, OUT_BF16>(a, st);
 	if (src == SRC_BF16 && outk == OUT_F32) return launch_conv<SRC_BF16, OUT_F32>(a, st);
 	if (src == SRC_BF16 && outk == OUT_F32_CH0) return launch_conv<SRC_BF16, OUT_F32_CH0>(a, st);
@@ -671,13 +683,16 @@ fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int
 // decoder's concatenations materialised in u->cat and the output layer's 16 FP32 channels in u->yout.
 // Inference upsamples and concatenates inside the decoder convs' staging and writes output channel 0
 // only, to y ([B][H][W] FP32).
-fx_status forward(fx_unet* u, bool infer, float* y) {
+fx_status forward(fx_unet* u, bool infer, float* y, const float* xraw = nullptr) {
 	hipStream_t st = u->be->stream;
 	const bf16* h = u->xin;
 	int li = 0;
 	for (int l = 0; l < 4; ++l) {
 		for (int k = 0; k < 2; ++k, ++li) {
-			FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
+			if (li == 0 && xraw)   // the raw FP32 input, rounded and padded while staged
+				FX_CHECK(conv(u, li, SRC_F32IN, xraw, nullptr, u->cin, nullptr, u->act[li], OUT_BF16, l, false));
+			else
+				FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
 			h = u->act[li];
 		}
 		if (l < 3) {
@@ -868,11 +883,14 @@ fx_status fx_unet_get_grad(fx_unet* u, float* host, size_t n) {
 	return FX_OK;
 }
 
-// Packs x (FP32 [B][H][W][cin]) and runs the forward; inference writes channel 0 to y.
+// Runs the forward on x (FP32 [B][H][W][cin]): inference reads x in its first convolution and writes
+// channel 0 to y; training first packs x to BF16 (the backward reads the packed copy).
 static fx_status run_forward(fx_unet* u, const float* x, int32_t B, int32_t H, int32_t W, bool infer, float* y) {
 	FX_REQUIRE(u && x && B > 0 && H % 8 == 0 && W % 8 == 0 && H > 0 && W > 0, FX_ERR_ARGUMENT,
 	           "batch > 0 and H, W divisible by 8 required");
 	FX_CHECK(ensure_ws(u, B, H, W, !infer));
+	// Inference reads x in the first convolution; training keeps the packed copy for the backward.
+	if (infer) return forward(u, infer, y ? y : u->ych0, x);
 	long npix = (long)B * H * W;
 	hipLaunchKernelGGL(k_pack_in, dim3((unsigned)std::min<long>(nb(npix), 65536)), dim3(256), 0, u->be->stream, x, u->xin,
 	                   npix, u->cin, pad16(u->cin), u->skip);

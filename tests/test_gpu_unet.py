@@ -98,3 +98,21 @@ def test_unet_energy_gradient(be):
     # dLoss/dy = 2·A(y − t)/tᵀAt: the 5-point operator A amplifies the pointwise (rough) part of the
     # forward's few-1e-3 difference, so this gradient is held to 5e-2 instead of 1e-2.
     assert cos > 0.999 and relg < 5e-2
+
+
+@pytest.mark.parametrize("shape", [(1, 136, 200), (2, 72, 64)])
+def test_unet_forward_many_tiles(be, shape):
+    # Inference over several tile columns and rows per level, a partial last tile column, and the first
+    # convolution reading the raw FP32 input (rounded to BF16 as it is staged), against the BF16-storage
+    # oracle.
+    from fluxion_amd.learned import UNet
+    B, H, W = shape
+    p = bf16_round(init_params(2, 11))
+    x = np.random.default_rng(12).standard_normal((B, H, W, 2))
+    net = UNet(be, 2)
+    net.set_params(p)
+    y = net.forward(x)
+    yr = O.unet_forward(p, bf16_round(x), bf16=True)
+    rel = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f"{shape}: forward vs oracle {rel:.3e}")
+    assert rel < 1e-2
