@@ -73,10 +73,10 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # TRAFFIC_SOURCES at that run, and a bench whose sources differ reports traffic null.
 TRAFFIC_SOURCES = ("fluxion_amd/csrc/fx_device.hpp", "fluxion_amd/csrc/fx_mgfast.hpp", "fluxion_amd/csrc/fx_solvers.hip")
 MEASURED_TRAFFIC_4096 = {
-    "sources_sha16": None,
-    "vc_restrict0": (277170 * 2 + 151884) * 1024,
-    "vc_post0": (289092 * 2 + 74922) * 1024,
-    "fcg_step": (244617 * 2 + 393248) * 1024,
+    "sources_sha16": "a5f3e8a6b33aab5f",
+    "vc_restrict0": (277197 * 2 + 152147) * 1024,
+    "vc_post0": (284921 * 2 + 74934) * 1024,
+    "fcg_step": (244735 * 2 + 393248) * 1024,
 }
 
 
@@ -118,6 +118,22 @@ def unet_forward_record(be, net, n: int, reps: int = 10) -> dict:
             "frac_bf16_dense_peak": flop / t_f / BF16_PEAK_FLOPS}
 
 
+# Which attainable ceiling each probed kernel is read against: the step kernel has the byte mix of the
+# 52 B/pt streaming kernel; the level-0 V-cycle kernels read 28-29 B/pt and write 4-9 B/pt, so the
+# read-only ceiling bounds them from above.
+ATTAINABLE_KIND = {"fcg_step": "mix52", "vc_post0": "read", "vc_restrict0": "read", "fcg_update": "copy",
+                   "stencil_axpy": "copy"}
+
+
+def hbm_ceilings_record(be) -> dict:
+    """Attainable HBM bandwidth of plain streaming kernels over 2^24 points (4 rotating copies of the
+    arrays, past the 256 MB last-level cache): read, write, copy and the FCG step kernel's 52 B/pt mix."""
+    rec = {k: {"us_per_launch": us, "GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "bytes_per_point": hip_bytes}
+           for (k, (us, gbs)), hip_bytes in zip(be.hbm_ceilings(1 << 24, 20).items(), be.HBM_KINDS.values())}
+    log("bench: attainable HBM " + ", ".join(f"{k} {v['GBs']:.0f} GB/s ({100 * v['frac']:.0f}%)" for k, v in rec.items()))
+    return rec
+
+
 def sources_sha16() -> str:
     import hashlib
     h = hashlib.sha256()
@@ -149,6 +165,8 @@ def main():
     ap.add_argument("--mg-kernels", choices=("wave", "ring", "tiled"), default="wave",
                     help="level kernels of the fused FP32 V-cycle (all bit-identical)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in wave segment heights")
+    ap.add_argument("--hbm-only", action="store_true",
+                    help="only measure the attainable HBM bandwidth of plain streaming kernels and exit")
     ap.add_argument("--unet-only", action="store_true",
                     help="only time the U-Net forward at --size (committed weights) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -163,6 +181,11 @@ def main():
     from fluxion_amd import hip
     from fluxion_amd.core import Boundaries, DType, Grid, Stencil, Transfer
     from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+
+    if args.hbm_only:
+        rec = hbm_ceilings_record(hip.HipBackend(0))
+        print(json.dumps({"hbm_ceilings": rec}), flush=True)
+        return
 
     if args.unet_only:
         # The network forward alone (committed weights), for kernel traces of the convolutions.
@@ -296,6 +319,11 @@ def main():
                          "frac": gbs / HBM_PEAK_GBS, "ms_per_solve": ms / (args.steps if name == "fcg_update" else 2)}
         log(f"bench: kernel {name}: {us:.1f} us/launch, {gbs:.0f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.0f}%)")
     dom = max(kernels, key=lambda m: kernels[m]["ms_per_solve"])
+    ceilings = hbm_ceilings_record(be)
+    for name, kd_ in kernels.items():
+        c = ceilings[ATTAINABLE_KIND[name]]
+        kd_["attainable_kind"] = ATTAINABLE_KIND[name]
+        kd_["frac_of_attainable"] = kd_["GBs"] / c["GBs"]
 
     # Stencil HBM GB/s (Backend::stencil_axpy, 5-point) with the same probe.
     Y = hip.Field.zeros(g, be=be)
@@ -686,9 +714,13 @@ def main():
         "explicit_diffusion_4096": explicit,
         "unet_forward": unet_rec,
         "kernels": kernels,
+        "hbm_ceilings": ceilings,
         "stencil_axpy_GBs": st_gbs,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kd["frac"],
+                     "attainable": ceilings[ATTAINABLE_KIND[dom]]["GBs"],
+                     "attainable_note": f"measured {ATTAINABLE_KIND[dom]} streaming kernel (fx_hbm_ceilings), same byte mix",
+                     "frac_of_attainable": kd["frac_of_attainable"],
                      "traffic": traffic,
                      "traffic_note": traffic_note,
                      "bytes_per_point": kd["bytes_per_point"],

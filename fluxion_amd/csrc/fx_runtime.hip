@@ -150,6 +150,39 @@ fx_status stencil_args(const fx_stencil* st, fxd::W9* w9, fxd::Ent* ent) {
 
 using namespace fx;
 
+// ── attainable HBM bandwidth (fx_hbm_ceilings) ────────────────────────
+namespace {
+typedef double hbm_d2v __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(fxd::BLOCK) void k_hbm_read(const double2* __restrict__ a, double* out, size_t n2) {
+	double s = 0.0;
+	for (size_t i = (size_t)blockIdx.x * fxd::BLOCK + threadIdx.x; i < n2; i += (size_t)gridDim.x * fxd::BLOCK) {
+		const double2 v = a[i];
+		s += v.x + v.y;
+	}
+	if (s == -1.0) out[0] = s;   // keeps the loads; never true for the zero-filled arrays
+}
+__global__ __launch_bounds__(fxd::BLOCK) void k_hbm_write(double2* __restrict__ b, size_t n2) {
+	for (size_t i = (size_t)blockIdx.x * fxd::BLOCK + threadIdx.x; i < n2; i += (size_t)gridDim.x * fxd::BLOCK)
+		b[i] = double2{1.0, 2.0};
+}
+__global__ __launch_bounds__(fxd::BLOCK) void k_hbm_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                         size_t n2) {
+	for (size_t i = (size_t)blockIdx.x * fxd::BLOCK + threadIdx.x; i < n2; i += (size_t)gridDim.x * fxd::BLOCK) b[i] = a[i];
+}
+__global__ __launch_bounds__(fxd::BLOCK) void k_hbm_mix52(const float2* __restrict__ z, const double2* __restrict__ po,
+                                                          const double2* __restrict__ k, double2* __restrict__ x,
+                                                          double2* __restrict__ p, double2* __restrict__ ap, size_t n2) {
+	for (size_t i = (size_t)blockIdx.x * fxd::BLOCK + threadIdx.x; i < n2; i += (size_t)gridDim.x * fxd::BLOCK) {
+		const float2 zz = z[i];
+		const double2 pp = po[i], kk = k[i], xx = x[i];
+		const double2 pn{zz.x + 0.5 * pp.x, zz.y + 0.5 * pp.y};
+		__builtin_nontemporal_store(hbm_d2v{pn.x, pn.y}, (hbm_d2v*)&p[i]);
+		__builtin_nontemporal_store(hbm_d2v{pn.x * kk.x, pn.y * kk.y}, (hbm_d2v*)&ap[i]);
+		__builtin_nontemporal_store(hbm_d2v{xx.x + 0.25 * pp.x, xx.y + 0.25 * pp.y}, (hbm_d2v*)&x[i]);
+	}
+}
+}  // namespace
+
 extern "C" {
 
 size_t fx_last_error(char* buf, size_t len) {
@@ -344,6 +377,67 @@ fx_status fx_copy_to_host_f32(fx_backend* be, const fx_buf* buf, float* dst, siz
 		for (size_t i = 0; i < len; ++i) dst[i] = (float)tmp[i];
 	}
 	return FX_OK;
+}
+
+
+fx_status fx_hbm_ceilings(fx_backend* be, size_t n, int32_t reps, double* out_us) {
+	FX_REQUIRE(be && out_us && reps > 0 && n >= 2 && n % 2 == 0, FX_ERR_ARGUMENT,
+	           "backend, output, reps > 0 and an even n >= 2 required");
+	constexpr int NC = 4;   // rotating copies, so that consecutive launches do not hit in the last-level cache
+	// Per copy: z (FP32) and six FP64 arrays; the other kinds reuse them.
+	const size_t per = n * 4 + 6 * n * 8;
+	void* mem = nullptr;
+	FX_CHECK(dev_alloc(be, NC * per, &mem));
+	hipStream_t st = be->stream;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	fx_status status = FX_OK;
+	auto run = [&]() -> fx_status {
+		FX_HIP(hipMemsetAsync(mem, 0, NC * per, st));
+		FX_HIP(hipEventCreate(&e0));
+		FX_HIP(hipEventCreate(&e1));
+		const size_t n2 = n / 2;
+		// Enough workgroups to fill the chip several times over, each thread looping over a few pairs.
+		const unsigned blocks = (unsigned)std::min<size_t>(16384, (n2 + fxd::BLOCK - 1) / fxd::BLOCK);
+		auto arr = [&](int c, int j) { return (char*)mem + c * per + n * 4 + (size_t)j * n * 8; };
+		auto zf = [&](int c) { return (const float2*)((char*)mem + c * per); };
+		double* sink = (double*)arr(0, 5);
+		for (int kind = 0; kind < FX_HBM_NKINDS; ++kind) {
+			auto launch = [&](int c) {
+				switch (kind) {
+				case FX_HBM_READ:
+					hipLaunchKernelGGL(k_hbm_read, dim3(blocks), dim3(fxd::BLOCK), 0, st, (const double2*)arr(c, 0), sink, n2);
+					break;
+				case FX_HBM_WRITE:
+					hipLaunchKernelGGL(k_hbm_write, dim3(blocks), dim3(fxd::BLOCK), 0, st, (double2*)arr(c, 1), n2);
+					break;
+				case FX_HBM_COPY:
+					hipLaunchKernelGGL(k_hbm_copy, dim3(blocks), dim3(fxd::BLOCK), 0, st, (const double2*)arr(c, 0),
+					                   (double2*)arr(c, 1), n2);
+					break;
+				default:
+					hipLaunchKernelGGL(k_hbm_mix52, dim3(blocks), dim3(fxd::BLOCK), 0, st, zf(c), (const double2*)arr(c, 0),
+					                   (const double2*)arr(c, 1), (double2*)arr(c, 2), (double2*)arr(c, 3),
+					                   (double2*)arr(c, 4), n2);
+				}
+			};
+			for (int w = 0; w < 3; ++w) launch(w % NC);
+			FX_HIP(hipEventRecord(e0, st));
+			for (int r = 0; r < reps; ++r) launch(r % NC);
+			FX_HIP(hipEventRecord(e1, st));
+			FX_HIP(hipEventSynchronize(e1));
+			FX_CHECK(check_launch("hbm ceilings"));
+			float ms = 0.0f;
+			FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+			out_us[kind] = 1e3 * (double)ms / reps;
+		}
+		return FX_OK;
+	};
+	status = run();
+	(void)hipStreamSynchronize(st);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	dev_free(mem);
+	return status;
 }
 
 }  // extern "C"

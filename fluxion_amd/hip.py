@@ -68,6 +68,14 @@ class HipBackend:
         L.check(self._lib.fx_probe_read(self.h, C.byref(ms), C.byref(cnt)), "probe_read")
         return ms.value, cnt.value
 
+    HBM_KINDS = {"read": 8, "write": 8, "copy": 16, "mix52": 52}   # kind -> bytes per point
+
+    def hbm_ceilings(self, n: int = 1 << 24, reps: int = 20) -> dict:
+        """Attainable bandwidth of plain streaming kernels (fx_hbm_ceilings): {kind: (us, GB/s)}."""
+        out = (C.c_double * 4)()
+        L.check(self._lib.fx_hbm_ceilings(self.h, int(n), int(reps), out), "hbm_ceilings")
+        return {k: (out[i], b * n / (out[i] * 1e-6) / 1e9) for i, (k, b) in enumerate(self.HBM_KINDS.items())}
+
     # ── memory ────────────────────────────────────────────────────────
     def allocate(self, n: int, dtype: DType = DType.F64) -> "Buffer":
         h = C.c_void_p()

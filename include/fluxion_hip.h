@@ -88,6 +88,17 @@ typedef enum {
 } fx_probe_kind;
 fx_status fx_probe_set(fx_backend* be, int32_t kind);
 fx_status fx_probe_read(fx_backend* be, double* total_ms, uint64_t* launches);
+/* Attainable HBM bandwidth of plain streaming kernels (measurement extension; no reference
+ * counterpart), the ceilings the solver kernels are read against.  Each kind streams n points once
+ * per launch, 16-byte accesses, grid-stride, over four rotating copies of its arrays (4·n·bytes
+ * beyond the 256 MB last-level cache for n = 2^24) allocated and freed by the call; out_us[kind]
+ * receives the mean microseconds per launch over `reps` launches (HIP events, after 3 warm-ups):
+ *   FX_HBM_READ   8 B/pt read (sum of an FP64 array)       FX_HBM_WRITE 8 B/pt written (fill)
+ *   FX_HBM_COPY  16 B/pt (FP64 copy)
+ *   FX_HBM_MIX52 52 B/pt: FP32 + 3 FP64 arrays read, 3 FP64 arrays written (the byte mix of the
+ *                FCG step kernel), non-temporal stores. */
+enum { FX_HBM_READ = 0, FX_HBM_WRITE = 1, FX_HBM_COPY = 2, FX_HBM_MIX52 = 3, FX_HBM_NKINDS = 4 };
+fx_status fx_hbm_ceilings(fx_backend* be, size_t n, int32_t reps, double* out_us);
 
 /* ── memory (backend.rs:50-79, BackendBuffer 13-36) ──────────────────── */
 fx_status fx_allocate(fx_backend* be, size_t len, fx_dtype dtype, fx_buf** out);   /* zero-filled */
