@@ -94,6 +94,7 @@ _SIGS = {
     "fx_batch_solve": (C.c_int, [C.c_void_p, c_buf_p, c_buf_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fx_batch_destroy": (None, [C.c_void_p]),
     "fx_batch_info": (C.c_int, [C.c_void_p, _P(C.c_int32), _P(C.c_int32), _P(C.c_uint64), _P(C.c_int32)]),
+    "fx_batch_level_kernels": (C.c_int, [C.c_void_p, _P(C.c_int32), _P(C.c_int32), _P(C.c_int32)]),
     "fx_probe_read": (C.c_int, [c_be_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fx_hbm_ceilings": (C.c_int, [c_be_p, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]),
     "fx_allocate": (C.c_int, [c_be_p, C.c_size_t, C.c_int, _P(c_buf_p)]),

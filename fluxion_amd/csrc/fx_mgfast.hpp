@@ -1180,29 +1180,22 @@ __device__ __forceinline__ void run_steps(F&& step, int t0, int t1, CL cl) {
 // compensation across lanes and workgroups).  Wave gw takes strip gw % nstrips
 // and row segment gw / nstrips of sh rows.  Stage values are computed on every
 // lane and row and then selected, so a step is branch-free up to the store.
+// One wave of k_wpost: strip `strip` of SSW owned columns and row segment `seg` of sh rows of the level;
+// per-lane reduction terms of Epi accumulate into lsum.
 template <bool VK, int NPRE, int NPOST, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, const float* __restrict__ ec,
-                                                 float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
-                                                 int cc, int nstrips, int nseg, int sh, Epi epi, RedScratch rs,
-                                                 const int* skip) {
-	if (skip && *skip) return;
+__device__ __forceinline__ void wpost_wave(const float* __restrict__ b, const float* __restrict__ ec,
+                                           float* __restrict__ out, const LWf& L, const fx_xtab& X, int rows, int cols,
+                                           int cr, int cc, int strip, int seg, int sh, Epi& epi, double* lsum) {
 	constexpr int LU = NPRE - 1;     // rows the iterate u = J^npre(0; b) + P·e lags the loaded row
 	constexpr int LS = LU + 1;       // first post-sweep (NPOST == 2)
 	constexpr int LZ = LU + NPOST;   // the level's result
 	constexpr int WP = WD_POST;
 	static_assert(LZ <= SHX && LZ < WR && LZ < WP && WP % WR == 0, "halo or rings too small");
-	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
 	// With Epi::RHS_FROM_R the rhs is float(r) of the epilogue's FP64 residual
 	// (which it loads anyway, for the same row and column), not a load of b.
 	constexpr bool LOADB = !Epi::RHS_FROM_R;
-	double lsum[NA];
-#pragma unroll
-	for (int k = 0; k < NA; ++k) lsum[k] = 0.0;
-	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
-	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
-	if (gw < nstrips * nseg) {
-		const int strip = gw % nstrips, seg = gw / nstrips;
+	{
 		const int gc = strip * SSW - SHX + x;
 		const unsigned uc = (unsigned)(gc < 0 ? 0 : (gc < cols ? gc : cols - 1));
 		const bool colin = gc >= 0 && gc < cols;
@@ -1330,6 +1323,21 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 		if (wclean && T0 >= 1 && T1 <= rows - 1) walk(std::true_type{});
 		else walk(std::false_type{});
 	}
+}
+template <bool VK, int NPRE, int NPOST, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, const float* __restrict__ ec,
+                                                 float* __restrict__ out, LWf L, fx_xtab X, int rows, int cols, int cr,
+                                                 int cc, int nstrips, int nseg, int sh, Epi epi, RedScratch rs,
+                                                 const int* skip) {
+	if (skip && *skip) return;
+	constexpr int NA = Epi::NS > 0 ? Epi::NS : 1;
+	double lsum[NA];
+#pragma unroll
+	for (int k = 0; k < NA; ++k) lsum[k] = 0.0;
+	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
+	if (gw < nstrips * nseg)
+		wpost_wave<VK, NPRE, NPOST>(b, ec, out, L, X, rows, cols, cr, cc, gw % nstrips, gw / nstrips, sh, epi, lsum);
 	if constexpr (Epi::NS > 0) {
 		Acc acc[NA];
 #pragma unroll
@@ -1359,15 +1367,11 @@ __global__ __launch_bounds__(BLOCK) void k_wpost(const float* __restrict__ b, co
 template <bool VK, int NPRE, class Upd>
 __device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, float* __restrict__ bco, const LWf& L,
                                                const fx_xtab& X, int fr, int fc, int cr, int cc, int nstrips, int nseg,
-                                               int scr, const Upd& upd, double& lsum) {
+                                               int strip, int seg, int scr, const Upd& upd, double& lsum) {
 	constexpr int LU = NPRE - 1, LR = NPRE;  // rows the pre-smoothed iterate and the residual lag the loaded row
 	constexpr int WQ = WD_RESTRICT;
 	static_assert(LR < WR && LR < WQ && WQ % WR == 0, "rings too small");
-	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
 	const int x = (int)(threadIdx.x & (WAVE - 1));
-	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
-	if (gw >= nstrips * nseg) return;
-	const int strip = gw % nstrips, seg = gw / nstrips;
 	const bool sfirst = strip == 0, slast = strip == nstrips - 1, gfirst = seg == 0, glast = seg == nseg - 1;
 	const int Ja = 1 + strip * RCX, Jb = Ja + RCX - 1 < cc - 2 ? Ja + RCX - 1 : cc - 2;
 	const int Ia = 1 + seg * scr, Ib = Ia + scr - 1 < cr - 2 ? Ia + scr - 1 : cr - 2;
@@ -1528,7 +1532,10 @@ __global__ __launch_bounds__(BLOCK) void k_wrestrict(const float* __restrict__ b
 	if (skip && *skip) return;
 	double lsum = 0.0;
 	upd.begin();
-	wrestrict_wave<VK, NPRE>(b, bco, L, X, fr, fc, cr, cc, nstrips, nseg, scr, upd, lsum);
+	const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	const int gw = xcd_remap(blockIdx.x, gridDim.x) * NWAVES + w;
+	if (gw < nstrips * nseg)
+		wrestrict_wave<VK, NPRE>(b, bco, L, X, fr, fc, cr, cc, nstrips, nseg, gw % nstrips, gw / nstrips, scr, upd, lsum);
 	if constexpr (Upd::ON) {
 		Acc acc[1] = {Acc{lsum, 0.0}};
 		block_partial<1>(acc, rs);

@@ -2233,6 +2233,9 @@ struct BLev {
 	fmg::LWf L;
 	fx_xtab X;
 	int32_t rows, cols, cr, cc;   // this level and the next coarser one
+	// Wave-streaming launch shapes (wave work lists): post strips × segments of psh rows, restriction
+	// strips × segments of rscr coarse rows.
+	int32_t pns, pnseg, psh, rns, rnseg, rscr;
 };
 struct BItem {
 	int32_t p, l, bx, by;         // problem, level-table index, tile
@@ -2371,6 +2374,64 @@ __global__ __launch_bounds__(BLOCK) void k_b_post0(const BItem* __restrict__ ite
 	block_partial<NS>(acc, rs);
 }
 
+// The same V-cycle pieces on the wave-streaming level kernels: an item is a block of NWAVES consecutive
+// waves (strip-major within the segment order of fmg::k_wpost / k_wrestrict) of one problem's level;
+// waves past the level's last return at once (those of a level-0 post block still join its partial sums).
+template <bool VK, int NPRE>
+__global__ __launch_bounds__(BLOCK) void k_b_wrestrict(const BItem* __restrict__ items, const BLev* __restrict__ lv,
+                                                       const SState* __restrict__ st) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BLev& L = lv[it.l];
+	const int gw = it.bx + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	if (gw >= L.rns * L.rnseg) return;
+	double lsum = 0.0;
+	fmg::wrestrict_wave<VK, NPRE>(L.rhs, lv[it.l + 1].rhs, L.L, L.X, L.rows, L.cols, L.cr, L.cc, L.rns, L.rnseg,
+	                              gw % L.rns, gw / L.rns, L.rscr, fmg::NoUpd{}, lsum);
+}
+template <bool VK, int NPRE, int NPOST>
+__global__ __launch_bounds__(BLOCK) void k_b_wpost(const BItem* __restrict__ items, const BLev* __restrict__ lv,
+                                                   const SState* __restrict__ st) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;
+	const BLev& L = lv[it.l];
+	const int gw = it.bx + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	if (gw >= L.pns * L.pnseg) return;
+	double lsum[1] = {0.0};
+	fmg::NoEpi e;
+	fmg::wpost_wave<VK, NPRE, NPOST>(L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols, L.cr, L.cc, gw % L.pns,
+	                                 gw / L.pns, L.psh, e, lsum);
+}
+template <bool VK, int NPRE, int NPOST, int NS>
+__global__ __launch_bounds__(BLOCK) void k_b_wpost0(const BItem* __restrict__ items, const BProb* __restrict__ pr,
+                                                    const BLev* __restrict__ lv, SState* __restrict__ st,
+                                                    const double* __restrict__ r, const double* __restrict__ ap,
+                                                    RedScratch rs) {
+	const BItem it = items[blockIdx.x];
+	if (st[it.p].skip) return;   // the whole block belongs to one problem
+	const BLev& L = lv[it.l];
+	const BProb& P = pr[it.p];
+	const int gw = it.bx + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+	double lsum[NS];
+#pragma unroll
+	for (int q = 0; q < NS; ++q) lsum[q] = 0.0;
+	if (gw < L.pns * L.pnseg) {
+		if constexpr (NS == 2) {
+			EpiFcg e{r + P.off, ap + P.off, st + it.p};
+			fmg::wpost_wave<VK, NPRE, NPOST>(L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols, L.cr, L.cc,
+			                                 gw % L.pns, gw / L.pns, L.psh, e, lsum);
+		} else {
+			EpiRZ e{r + P.off, st + it.p};
+			fmg::wpost_wave<VK, NPRE, NPOST>(L.rhs, lv[it.l + 1].tmp, L.tmp, L.L, L.X, L.rows, L.cols, L.cr, L.cc,
+			                                 gw % L.pns, gw / L.pns, L.psh, e, lsum);
+		}
+	}
+	Acc acc[NS];
+#pragma unroll
+	for (int q = 0; q < NS; ++q) acc[q] = Acc{lsum[q], 0.0};
+	block_partial<NS>(acc, rs);
+}
+
 // Per-problem finish: workgroup p merges its blocks' partials (fixed order) and runs Fin on its state.
 template <int NS, class Fin>
 __global__ __launch_bounds__(BLOCK) void k_b_finish(const BProb* __restrict__ pr, SState* __restrict__ st,
@@ -2426,6 +2487,14 @@ struct fx_batch {
 	std::vector<BItem*> restr, post;         // per level depth d (restrict d → d+1; post at d ≥ 1)
 	std::vector<int> nrestr, npost;
 	std::vector<size_t> restr_lds;
+	// Wave-streaming level kernels (when every level's restriction fits a wave): work lists of
+	// NWAVES-wave blocks per level depth, the level-0 post blocks, and the segment heights in use.
+	bool wave = false;
+	std::vector<BLev> hl;                   // host copy of the level table
+	std::vector<BItem*> wrestr, wpost;
+	std::vector<int> nwrestr, nwpost;
+	BItem* wpost0 = nullptr;
+	int nwpost0 = 0, wsh = 16, wscr = 8;
 	int nstep = 0, nupd = 0, npost0 = 0, maxdepth = 0;
 	int rpb = 16, chunk = BLOCK * BUPD_THREAD;
 	// work vectors
@@ -2454,6 +2523,9 @@ void batch_free(fx_batch* B) {
 	fx::dev_free(B->post0);
 	for (auto* q : B->restr) fx::dev_free(q);
 	for (auto* q : B->post) fx::dev_free(q);
+	for (auto* q : B->wrestr) fx::dev_free(q);
+	for (auto* q : B->wpost) fx::dev_free(q);
+	fx::dev_free(B->wpost0);
 	fx::dev_free(B->x);
 	fx::dev_free(B->r);
 	fx::dev_free(B->p0);
@@ -2505,8 +2577,89 @@ std::vector<BItem> upd_items(fx_batch* B, int chunk) {
 	return v;
 }
 
+// Wave work lists for post segments of `sh` fine rows and restriction segments of `scr` coarse rows:
+// sets the levels' launch shapes (B->hl, uploaded to B->dl), the per-depth lists and the level-0 post
+// blocks' ranges per problem (hp[p].pst0 / npst).  Frees the previous lists.
+fx_status wave_lists(fx_batch* B, int sh, int scr) {
+	fx_backend* be = B->be;
+	for (auto* q : B->wrestr) fx::dev_free(q);
+	for (auto* q : B->wpost) fx::dev_free(q);
+	fx::dev_free(B->wpost0);
+	B->wrestr.assign(B->maxdepth, nullptr);
+	B->wpost.assign(B->maxdepth, nullptr);
+	B->wpost0 = nullptr;
+	B->nwrestr.assign(B->maxdepth, 0);
+	B->nwpost.assign(B->maxdepth, 0);
+	for (int p = 0; p < B->nprob; ++p) {
+		const BProb& P = B->hp[p];
+		for (int l = 0; l + 1 < P.nlev; ++l) {
+			BLev& L = B->hl[P.lev0 + l];
+			L.psh = sh;
+			L.pns = (int32_t)fmg::wave_strips_post(L.cols);
+			L.pnseg = (int32_t)((L.rows + sh - 1) / sh);
+			L.rscr = scr;
+			L.rns = (int32_t)fmg::wave_strips_restrict(L.cc);
+			L.rnseg = (int32_t)fmg::wave_segs_restrict(L.cr, scr);
+		}
+	}
+	std::vector<BItem> p0;
+	for (int d = 0; d < B->maxdepth; ++d) {
+		std::vector<BItem> ri, pi;
+		for (int p = 0; p < B->nprob; ++p) {
+			const BProb& P = B->hp[p];
+			if (d + 1 >= P.nlev) continue;
+			const BLev& L = B->hl[P.lev0 + d];
+			const int nr = L.rns * L.rnseg, np = L.pns * L.pnseg;
+			for (int w0 = 0; w0 < nr; w0 += NWAVES) ri.push_back(BItem{p, P.lev0 + d, w0, 0});
+			std::vector<BItem>& dst = d == 0 ? p0 : pi;
+			if (d == 0) B->hp[p].pst0 = (int32_t)p0.size();
+			for (int w0 = 0; w0 < np; w0 += NWAVES) dst.push_back(BItem{p, P.lev0 + d, w0, 0});
+			if (d == 0) B->hp[p].npst = (int32_t)p0.size() - B->hp[p].pst0;
+		}
+		B->nwrestr[d] = (int)ri.size();
+		B->nwpost[d] = (int)pi.size();
+		FX_CHECK(upload_vec(be, ri, &B->wrestr[d]));
+		FX_CHECK(upload_vec(be, pi, &B->wpost[d]));
+	}
+	B->nwpost0 = (int)p0.size();
+	FX_CHECK(upload_vec(be, p0, &B->wpost0));
+	FX_HIP(hipMemcpyAsync(B->dl, B->hl.data(), B->hl.size() * sizeof(BLev), hipMemcpyHostToDevice, be->stream));
+	FX_HIP(hipStreamSynchronize(be->stream));
+	B->wsh = sh;
+	B->wscr = scr;
+	return FX_OK;
+}
+
+template <bool VK, int NPRE, int NPOST>
+fx_status batch_vcycle_wave(fx_batch* B, bool start) {
+	hipStream_t st = B->be->stream;
+	for (int d = 0; d + 1 <= B->maxdepth - 1; ++d) {
+		if (B->nwrestr[d] == 0) continue;
+		hipLaunchKernelGGL((k_b_wrestrict<VK, NPRE>), dim3(B->nwrestr[d]), dim3(BLOCK), 0, st, B->wrestr[d], B->dl,
+		                   B->dst);
+	}
+	hipLaunchKernelGGL(k_b_coarse, dim3(B->nprob), dim3(1024), 0, st, B->dp, B->dl, B->dst);
+	for (int d = B->maxdepth - 2; d >= 1; --d) {
+		if (B->nwpost[d] == 0) continue;
+		hipLaunchKernelGGL((k_b_wpost<VK, NPRE, NPOST>), dim3(B->nwpost[d]), dim3(BLOCK), 0, st, B->wpost[d], B->dl,
+		                   B->dst);
+	}
+	RedScratch rs{B->partials};
+	if (start) {
+		hipLaunchKernelGGL((k_b_wpost0<VK, NPRE, NPOST, 1>), dim3(B->nwpost0), dim3(BLOCK), 0, st, B->wpost0, B->dp,
+		                   B->dl, B->dst, (const double*)B->r, (const double*)B->ap, rs);
+		hipLaunchKernelGGL((k_b_finish<1, FinPcgInitRZ>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 2, 0);
+	} else {
+		hipLaunchKernelGGL((k_b_wpost0<VK, NPRE, NPOST, 2>), dim3(B->nwpost0), dim3(BLOCK), 0, st, B->wpost0, B->dp,
+		                   B->dl, B->dst, (const double*)B->r, (const double*)B->ap, rs);
+		hipLaunchKernelGGL((k_b_finish<2, FinFcgBeta>), dim3(B->nprob), dim3(BLOCK), 0, st, B->dp, B->dst, rs, 2, 0);
+	}
+	return fx::check_launch("batched V-cycle (wave kernels)");
+}
+
 template <bool VK, int NPRE, int NPOST>
 fx_status batch_vcycle(fx_batch* B, bool start) {
+	if (B->wave) return batch_vcycle_wave<VK, NPRE, NPOST>(B, start);
 	hipStream_t st = B->be->stream;
 	for (int d = 0; d + 1 <= B->maxdepth - 1; ++d) {
 		if (B->nrestr[d] == 0) continue;
@@ -2876,6 +3029,12 @@ fx_status fx_batch_create(fx_backend* be, uint32_t nprob, const fx_grid* grids, 
 	B->npost0 = (int)p0items.size();
 	if ((s = upload_vec(be, p0items, &B->post0)) != FX_OK) return fail(s);
 	if ((s = upload_vec(be, lev, &B->dl)) != FX_OK) return fail(s);
+	// The wave-streaming level kernels serve the batch when every level's restriction fits a wave.
+	B->hl = lev;
+	B->wave = true;
+	for (int p = 0; p < B->nprob && B->wave; ++p)
+		for (int l = 0; l + 1 < B->hp[p].nlev; ++l)
+			if (!fmg::wrestrict_ok(lev[B->hp[p].lev0 + l].X)) B->wave = false;
 	// Work vectors and state.
 	void* q = nullptr;
 	double** vecs[5] = {&B->x, &B->r, &B->p0, &B->p1, &B->ap};
@@ -2895,6 +3054,15 @@ fx_status fx_batch_create(fx_backend* be, uint32_t nprob, const fx_grid* grids, 
 	// each timed on this batch with zero states (β = α = 0, nothing skipped; only scratch vectors move).
 	FX_HIP(hipMemsetAsync(B->dst, 0, (size_t)B->nprob * sizeof(SState), be->stream));
 	size_t maxparts = (size_t)B->npost0 * 2;
+	static constexpr int WSH[4] = {8, 16, 32, 64}, WSCR[4] = {4, 8, 16, 32};   // wave segment heights tried
+	if (B->wave) {
+		size_t nb = 0;   // level-0 post blocks at the shortest segments
+		for (int p = 0; p < B->nprob; ++p) {
+			const fx_grid& g = B->grids[p];
+			nb += (size_t)((fmg::wave_strips_post(g.cols) * ((g.rows + WSH[0] - 1) / WSH[0]) + NWAVES - 1) / NWAVES);
+		}
+		maxparts = std::max(maxparts, nb * 2);
+	}
 	for (int rpb : {4, 8, 16, 32}) maxparts = std::max(maxparts, step_items(B, rpb).size());
 	for (int ch : {BLOCK * 4, BLOCK * 8, BLOCK * 16}) maxparts = std::max(maxparts, upd_items(B, ch).size());
 	if ((s = fx::dev_alloc(be, maxparts * sizeof(fxd::Acc), &q)) != FX_OK) return fail(s);
@@ -2950,6 +3118,21 @@ fx_status fx_batch_create(fx_backend* be, uint32_t nprob, const fx_grid* grids, 
 			best = ms;
 			B->chunk = ch;
 		}
+	}
+	if (B->wave) {
+		// Segment heights of the wave level kernels, one pair for the whole batch: the V-cycle timed with
+		// zero states for each candidate (taller segments re-read fewer lag rows, shorter ones give more
+		// waves).
+		best = 1e30f;
+		int pick = 1;
+		for (int c = 0; c < 4; ++c) {
+			if ((s = wave_lists(B, WSH[c], WSCR[c])) != FX_OK) return fail(s);
+			float ms = 0.0f;
+			s = timed([&] { (void)batch_vcycle_any(B, false); }, &ms);
+			if (s != FX_OK) return fail(s);
+			if (ms < best) best = ms, pick = c;
+		}
+		if ((s = wave_lists(B, WSH[pick], WSCR[pick])) != FX_OK) return fail(s);
 	}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
@@ -3043,6 +3226,14 @@ fx_status fx_batch_info(const fx_batch* B, int32_t* rows_per_block, int32_t* poi
 	if (points_per_block) *points_per_block = B->chunk;
 	if (points) *points = (uint64_t)B->ntot;
 	if (graph_iterations) *graph_iterations = B->graph_iters;
+	return FX_OK;
+}
+
+fx_status fx_batch_level_kernels(const fx_batch* B, int32_t* wave, int32_t* post_rows, int32_t* restrict_rows) {
+	FX_REQUIRE(B, FX_ERR_ARGUMENT, "null batch");
+	if (wave) *wave = B->wave ? 1 : 0;
+	if (post_rows) *post_rows = B->wave ? B->wsh : 0;
+	if (restrict_rows) *restrict_rows = B->wave ? B->wscr : 0;
 	return FX_OK;
 }
 

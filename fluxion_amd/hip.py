@@ -572,8 +572,11 @@ class BatchSolver:
     def info(self) -> dict:
         rpb, ppb, pts, gi = C.c_int32(), C.c_int32(), C.c_uint64(), C.c_int32()
         L.check(self._lib.fx_batch_info(self.h, C.byref(rpb), C.byref(ppb), C.byref(pts), C.byref(gi)), "batch_info")
+        wv, ph, rh = C.c_int32(), C.c_int32(), C.c_int32()
+        L.check(self._lib.fx_batch_level_kernels(self.h, C.byref(wv), C.byref(ph), C.byref(rh)), "batch_level_kernels")
         return {"rows_per_block": rpb.value, "points_per_block": ppb.value, "points": pts.value,
-                "graph_iterations": gi.value}
+                "graph_iterations": gi.value, "wave_level_kernels": bool(wv.value), "wave_post_rows": ph.value,
+                "wave_restrict_rows": rh.value}
 
     def solve(self, x: "Buffer", b: "Buffer", tol, max_iter: int) -> list:
         n = len(self.grids)

@@ -373,6 +373,10 @@ fx_status fx_batch_solve(fx_batch* bt, fx_buf* x, const fx_buf* b, const double*
 void fx_batch_destroy(fx_batch* bt);
 fx_status fx_batch_info(const fx_batch* bt, int32_t* rows_per_block, int32_t* points_per_block, uint64_t* points,
                         int32_t* graph_iterations);
+/* Level kernels of the batch's V-cycles: wave = 1 when every problem's levels run on the
+ * wave-streaming kernels (one wave per strip and row segment; post_rows fine rows and restrict_rows
+ * coarse rows per segment, tuned at creation), 0 for the LDS-tiled kernels (both heights 0). */
+fx_status fx_batch_level_kernels(const fx_batch* bt, int32_t* wave, int32_t* post_rows, int32_t* restrict_rows);
 
 /* ── Crank–Nicolson stepper (CrankNicolsonSolver, runtime/src/implicit.rs:1-238) ──────────
  * One step solves (I − (dt/2)·α·Δ) u^{n+1} = (I + (dt/2)·α·Δ) u^n + dt·source from u^n as the initial
