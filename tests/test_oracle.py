@@ -664,3 +664,92 @@ def test_pipelined_flexible_cg_matches_fcg_and_dense():
         if n == 16:
             A = np.stack([O.apply_op(op, e, g) for e in np.eye(n * n)], axis=1)
             assert rel_l2(xp, np.linalg.solve(A, f)) < 1e-10
+
+
+# ── elementwise ops, extra reductions and the variable-coefficient stencil (ops.rs:129-154, 318-331,
+#    476-585, 650-672) against known values ───────────────────────────────────────────────────
+def test_scale_waxpy_pointwise_known_values():
+    x = np.array([1.0, 2.0, 3.0])
+    y = np.array([4.0, 5.0, 6.0])
+    assert np.array_equal(O.scale(x, 2.5), [2.5, 5.0, 7.5])
+    assert np.array_equal(O.waxpy(2.0, x, 3.0, y), [14.0, 19.0, 24.0])
+    # waxpy is alpha.mul_add(x, beta·y): one rounding of α·x + (β·y).  With α = x = 1 + 2⁻³⁰ and
+    # β·y = −(1 + 2⁻²⁹) the exact result is 2⁻⁶⁰; rounding α·x first would give 0.
+    a = 1.0 + 2.0 ** -30
+    w = O.waxpy(a, np.array([a]), 1.0, np.array([-(1.0 + 2.0 ** -29)]))
+    assert w[0] == 2.0 ** -60
+    assert np.array_equal(O.pointwise(0, np.array([2.0, 3.0]), np.array([4.0, 5.0])), [8.0, 15.0])
+    assert np.array_equal(O.pointwise(1, np.array([1.0, 3.0]), np.array([4.0, 6.0])), [0.25, 0.5])
+    # f64::max / f64::min ignore a NaN operand (ops.rs:650-672).
+    nan = float("nan")
+    assert np.array_equal(O.pointwise(2, np.array([nan, 1.0, -2.0]), np.array([1.0, nan, 3.0])), [1.0, 1.0, 3.0])
+    assert np.array_equal(O.pointwise(3, np.array([nan, 1.0, -2.0]), np.array([1.0, nan, 3.0])), [1.0, 1.0, -2.0])
+
+
+def test_reduce_max_min_known_values():
+    v = np.array([3.0, -1.0, 7.0, 2.0])
+    assert O.reduce_max(v) == 7.0 and O.reduce_min(v) == -1.0
+    w = np.array([float("nan"), 2.0, -5.0])
+    assert O.reduce_max(w) == 2.0 and O.reduce_min(w) == -5.0       # NaN ignored, as f64::max/min do
+    e = np.zeros(0)
+    assert O.reduce_max(e) == -np.inf and O.reduce_min(e) == np.inf  # identities of the folds
+
+
+def test_apply_stencil_var_quadratic():
+    # coeff_i·S(u)_i (ops.rs:129-154): Δ(x² + y²) = 4 exactly on the interior of a uniform grid, so the
+    # result is 4·coeff there; with zero Dirichlet data the boundary values are coeff·0 = 0.
+    n = 33
+    h = 1.0 / (n - 1)
+    g = Grid.square(n, h)
+    xs = np.arange(n) * h
+    X, Y = np.meshgrid(xs, xs)
+    u = (X ** 2 + Y ** 2).ravel()
+    coeff = (1.0 + X + 2.0 * Y * Y).ravel()
+    out = O.apply_stencil_var(u, coeff, g, lap(h), ZD).reshape(n, n)
+    assert np.max(np.abs(out[1:-1, 1:-1] - 4.0 * coeff.reshape(n, n)[1:-1, 1:-1])) < 1e-9
+    assert not out[0].any() and not out[-1].any() and not out[:, 0].any() and not out[:, -1].any()
+    # A constant coefficient scales apply_stencil.
+    assert np.array_equal(O.apply_stencil_var(u, np.full(n * n, 2.0), g, lap(h), ZD),
+                          2.0 * O.apply_stencil(u, g, lap(h), ZD))
+
+
+def _learned_problem(n=32):
+    from fluxion_amd.inputs import gaussian_bumps, grf_log_k
+    g = Grid.square(n, 1.0 / (n - 1))
+    k = grf_log_k(n).ravel()
+    f = gaussian_bumps(n).ravel()
+    mg = O.mg_cfg(2, 2, 2.0 / 3.0, transfer=1, coarsest=16, precision=1, scale=1.0, k=k)
+    return g, k, f, O.Op.var_k(k), mg
+
+
+def test_learned_fcg_zero_network_is_fcg():
+    # With all network parameters zero the correction c is 0, so α* = 0 and the learned preconditioner is
+    # the V-cycle alone: the composition must then be flexible CG around the same V-cycle.
+    g, k, f, op, mg = _learned_problem()
+    tol = 1e-10 * np.linalg.norm(f)
+    params = np.zeros(O.unet_nparams(2))
+    xl, itl, fb = O.learned_fcg(f, g, op, mg, params, np.log(k), tol, 200)
+    xf, sf = O.fcg(np.zeros_like(f), f, g, op, tol, 200, mg=mg)
+    assert sf.converged() and abs(itl - sf.iterations) <= 1
+    assert rel_l2(xl, xf) < 1e-8
+
+
+def test_learned_fcg_random_network_still_converges():
+    # α* = (c·r1)/(c·A·c), clipped at 0, minimises the A-norm of the error left after the V-cycle along c,
+    # so each application is at least as good as the V-cycle alone, whatever the network.  With an
+    # untrained network (random weights) and the stall fallback disabled the solve therefore still
+    # converges to the tolerance on the true residual.  It takes more iterations than the V-cycle alone
+    # (30 against 15 here): the varying preconditioner costs flexible CG more than the per-application gain.
+    g, k, f, op, mg = _learned_problem()
+    tol = 1e-9 * np.linalg.norm(f)
+    woff, boff, lcin, lcout = O.unet_layout(2)
+    rng = np.random.default_rng(3)
+    params = np.zeros(O.unet_nparams(2))
+    for i in range(15):
+        m = 9 * lcin[i] * lcout[i]
+        params[woff[i]:woff[i] + m] = rng.standard_normal(m) * np.sqrt(2.0 / (9 * lcin[i]))
+    xl, itl, fb = O.learned_fcg(f, g, op, mg, params, np.log(k), tol, 200, stall=1e9)
+    _, sf = O.fcg(np.zeros_like(f), f, g, op, tol, 200, mg=mg)
+    assert not fb and itl < 200
+    assert np.linalg.norm(f - O.apply_op(op, xl, g)) < 2 * tol
+    assert sf.iterations <= itl <= 3 * sf.iterations
