@@ -1135,8 +1135,9 @@ fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t p
 	FX_REQUIRE(mg, FX_ERR_ARGUMENT, "multigrid is null");
 	FX_REQUIRE(level + 1 < mg->lv.size(), FX_ERR_ARGUMENT, "level %u has no coarser level (depth %zu)", level,
 	           mg->lv.size());
-	FX_REQUIRE(post_rows >= 0 && post_rows <= 4096 && restrict_rows >= 0 && restrict_rows <= 4096, FX_ERR_ARGUMENT,
-	           "segment heights must lie in [0, 4096]");
+	const int lo = level == 0 ? 0 : -1;
+	FX_REQUIRE(post_rows >= lo && post_rows <= 4096 && restrict_rows >= lo && restrict_rows <= 4096,
+	           FX_ERR_ARGUMENT, "segment heights must lie in [%d, 4096] on level %u", lo, level);
 	mg->wrows_post.resize(mg->lv.size(), 0);
 	mg->wrows_restrict.resize(mg->lv.size(), 0);
 	mg->wrows_post[level] = post_rows;

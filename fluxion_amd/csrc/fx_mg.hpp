@@ -50,7 +50,8 @@ struct fx_multigrid {
 	// Fused-cycle level kernels: FX_MGK_WAVE (wave-streaming, default), FX_MGK_RING (LDS-ring
 	// streaming) or FX_MGK_TILED; levels a variant cannot run fall back to the tiled kernels.
 	int kernels = 2;
-	std::vector<int> wrows_post, wrows_restrict;   // tuned wave segment heights per level (0: heuristic)
+	std::vector<int> wrows_post, wrows_restrict;   // tuned wave segment heights per level (0: heuristic;
+	                                               //   −1, levels ≥ 1: the LDS-tile kernel instead)
 	int wrows_restrict_upd = 0;   // level 0's restriction carrying the FCG residual update (0: as wrows_restrict)
 	int wrows_post_epi = 0;       // level 0's post kernel with a solver epilogue (0: as wrows_post)
 	std::vector<Level> lv;

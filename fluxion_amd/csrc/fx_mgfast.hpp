@@ -1882,6 +1882,8 @@ void launch_wrestrict(hipStream_t st, unsigned blocks, const float* b, float* bc
 // 256 … 8 fine rows (post) or 128 … 4 coarse rows (restrict) that still gives
 // WAVE_TARGET waves (eight per CU), unless a tuned height is set.
 constexpr int64_t WAVE_TARGET = 2048;
+// A tuned height of WROWS_TILED (levels ≥ 1 only) runs the level on the LDS-tile kernels instead.
+constexpr int WROWS_TILED = -1;
 inline int wpost_rows(int64_t rows, int64_t cols) {
 	const int64_t ns = wave_strips_post(cols);
 	int sh = 256;
@@ -1954,7 +1956,7 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 			}
 		}
 		if (fused0) {
-		} else if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt)) {
+		} else if (mg->kernels == FX_MGK_WAVE && wrestrict_ok(lv[l].xt) && tuned(mg->wrows_restrict, l) >= 0) {
 			int scr = tuned(mg->wrows_restrict, l);
 			if (scr <= 0) scr = wrestrict_rows(cg.rows, cg.cols);
 			const int64_t ns = wave_strips_restrict(cg.cols), nseg = wave_segs_restrict(cg.rows, scr);
@@ -2005,7 +2007,7 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 	};
 	auto post_shape = [&](int l) {
 		const fx_grid& g = lv[l].g;
-		if (mg->kernels == FX_MGK_WAVE) {
+		if (mg->kernels == FX_MGK_WAVE && tuned(mg->wrows_post, l) >= 0) {
 			int sh = l == 0 && Epi::NS > 0 && mg->wrows_post_epi > 0 ? mg->wrows_post_epi : tuned(mg->wrows_post, l);
 			if (sh <= 0) sh = wpost_rows(g.rows, g.cols);
 			const int64_t ns = wave_strips_post(g.cols), nseg = (g.rows + sh - 1) / sh;
@@ -2052,7 +2054,8 @@ fx_status fast_cycle_t(fx_multigrid* mg, const int* skip, Epi epi0, Upd upd0) {
 // Autotuner of the wave kernels' segment heights: for every level, each
 // candidate height is timed with HIP events over `reps` launches of the level's
 // kernel on the level's own buffers (the kernels' speed does not depend on the
-// values), and the fastest is kept in wrows_post / wrows_restrict.  *ms_out
+// values), and the fastest is kept in wrows_post / wrows_restrict; on levels
+// ≥ 1 the LDS-tile kernels compete too (WROWS_TILED when they win).  *ms_out
 // (if given) receives the sum over levels of the chosen per-launch times.
 template <bool VK, int NPRE, int NPOST>
 fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
@@ -2097,6 +2100,18 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 			}, &ms);
 			if (ms < best) best = ms, pick = sh;
 		}
+		if (l > 0 && status == FX_OK) {
+			// Below level 0 a level's data sits in the caches and a wave's walk down its segment is
+			// latency-bound; the LDS-tile kernel issues all its loads at once and may win there.
+			float ms = 0.0f;
+			status = time_it([&] {
+				hipLaunchKernelGGL((k_fpost<VK, NPRE, NPOST, NoEpi>), tgrid(g.rows, g.cols), dim3(BLOCK), 0, st,
+				                   (const float*)lv[l].rhs, (const float*)lv[l + 1].tmp, (float*)lv[l].tmp, lw,
+				                   lv[l].xt, (int64_t)g.rows, (int64_t)g.cols, (int64_t)cg.rows, (int64_t)cg.cols,
+				                   NoEpi{}, RedScratch{nullptr}, nullptr);
+			}, &ms);
+			if (ms < best) best = ms, pick = WROWS_TILED;
+		}
 		mg->wrows_post[l] = pick;
 		total += best;
 		if (!wrestrict_ok(lv[l].xt)) continue;
@@ -2112,6 +2127,16 @@ fx_status autotune_t(fx_multigrid* mg, int reps, double* ms_out) {
 				                           (int)nseg, scr, nullptr);
 			}, &ms);
 			if (ms < best) best = ms, pick = scr;
+		}
+		if (l > 0 && status == FX_OK) {
+			float ms = 0.0f;
+			status = time_it([&] {
+				hipLaunchKernelGGL((k_frestrict<VK, NPRE>), rgrid(cg.rows, cg.cols), dim3(BLOCK),
+				                   (unsigned)frestrict_lds(lv[l].xt, VK), st, (const float*)lv[l].rhs,
+				                   (float*)lv[l + 1].rhs, lw, lv[l].xt, (int64_t)g.rows, (int64_t)g.cols,
+				                   (int64_t)cg.rows, (int64_t)cg.cols, nullptr);
+			}, &ms);
+			if (ms < best) best = ms, pick = WROWS_TILED;
 		}
 		mg->wrows_restrict[l] = pick;
 		total += best;

@@ -19,6 +19,7 @@
 #include "fx_mgfast.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -682,6 +683,8 @@ struct FinCGBeta {
 		record(st, it, sqrt(rs_new));
 		if (it >= st->max_iter) finish(st, it, sqrt(rs_new), FX_MAX_ITERATIONS);
 		else if (sqrt(rs_new) < st->tol) finish(st, it, sqrt(rs_new), FX_CONVERGED);
+		// Release: a host that sees the count also sees the done flag stored above.
+		if (st->hprog) __hip_atomic_store(st->hprog, (int)it, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 	}
 };
 // PCG: r·r after the x/r update (cg.rs:205-211 of the next iteration).
@@ -1754,24 +1757,33 @@ fx_status drive(fx_solver* s, Step step, int64_t max_steps, int look = 2) {
 	return FX_OK;
 }
 
-// drive() for loops whose iteration finisher is FinPcgRR (or FinPcgRRDefer):
-// after enqueuing step i it waits, without stream events (each of which
-// stalls the queue for a few microseconds), until step i's x/r update has
-// stored the iteration count i + 1 in host memory.  The rest of step i (the
-// preconditioner) is still queued then, so the device does not run dry while
-// step i + 1 is enqueued, and once the solve is done no further step has been
-// enqueued.  An emptied stream (an error, or a step that ends early) also ends
-// the wait.
+// drive() for loops whose step i stores the iteration count i + 1 in host
+// memory (FinPcgRR, FinPcgRRDefer, FinCGBeta): after enqueuing step i it waits,
+// without stream events (each of which stalls the queue for a few
+// microseconds), until step i − lag has stored its count.  With lag 0 (the
+// count stored before the step's preconditioner) the rest of step i is still
+// queued while step i + 1 is enqueued, so the device does not run dry and once
+// the solve is done no further step has been enqueued; a step that stores its
+// count at its end takes lag 2, and the steps enqueued past the last return
+// early on the done flag.  An emptied stream (an error, or a step that ends early) also ends
+// the wait; the stream is queried only once the count has not moved for
+// QUIET_US, since each query leaves a marker in the queue that holds the next
+// kernel back by a few microseconds.
 template <class Step>
-fx_status drive_counted(fx_solver* s, Step step, int64_t max_steps) {
+fx_status drive_counted(fx_solver* s, Step step, int64_t max_steps, int lag = 0) {
+	using clk = std::chrono::steady_clock;
+	constexpr int64_t QUIET_US = 2000;
 	hipStream_t st = s->be->stream;
 	const volatile int* done = s->hdone;
 	const volatile int* prog = s->hdone + 1;
-	for (int64_t i = 0; i <= max_steps + 2; ++i) {
+	for (int64_t i = 0; i <= max_steps + 2 + lag; ++i) {
 		if (const fx_status e = step(i); e != FX_OK) return drain_on_error(st, e);
+		const clk::time_point t0 = clk::now();
 		for (unsigned spin = 0;; ++spin) {
-			if (__atomic_load_n(prog, __ATOMIC_ACQUIRE) >= i + 1 || __atomic_load_n(done, __ATOMIC_ACQUIRE)) break;
-			if ((spin & 63) == 63) {
+			if (__atomic_load_n(prog, __ATOMIC_ACQUIRE) >= i + 1 - lag || __atomic_load_n(done, __ATOMIC_ACQUIRE))
+				break;
+			if ((spin & 63) == 63 &&
+			    std::chrono::duration_cast<std::chrono::microseconds>(clk::now() - t0).count() >= QUIET_US) {
 				const hipError_t q = hipStreamQuery(st);
 				if (q == hipSuccess) break;
 				FX_REQUIRE(q == hipErrorNotReady, FX_ERR_HIP, "solver stream failed");
@@ -1824,26 +1836,26 @@ fx_status solve_cg(Ctx& c, double* x, const double* b, fx_solve_stats* stats) {
 	double* pp[2] = {p0, p1};
 	if (!c.pc) {
 		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinCGInit{c.d}, "cg init", nullptr));
-		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+		FX_CHECK(drive_counted(s, [&](int64_t i) -> fx_status {
 			double* pold = pp[i & 1];
 			double* pnew = pp[(i + 1) & 1];
 			FX_CHECK(c.cg_step1(r, pold, pnew, ap));
 			return c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinCGBeta{c.d}, "cg x/r update");
-		}, (int64_t)s->max_iter));
+		}, (int64_t)s->max_iter, 2));
 	} else {
 		double* z = s->vec(4);
 		FX_REQUIRE(z, FX_ERR_ALLOC, "work vector allocation failed");
 		FX_CHECK(c.ered<1>(FResid{b, ap, r}, FinPcgInitRR{c.d}, "pcg init", nullptr));
 		FX_CHECK(c.precond(z, r));
 		FX_CHECK(c.ered<1>(FDot{r, z}, FinPcgInitRZ{c.d}, "pcg r.z"));
-		FX_CHECK(drive(s, [&](int64_t i) -> fx_status {
+		FX_CHECK(drive_counted(s, [&](int64_t i) -> fx_status {
 			double* pold = pp[i & 1];
 			double* pnew = pp[(i + 1) & 1];
 			FX_CHECK(c.cg_step1(z, pold, pnew, ap));
 			FX_CHECK(c.ered<1>(FXR{c.d, x, r, pnew, ap, 0.0}, FinPcgRR{c.d}, "pcg x/r update"));
 			FX_CHECK(c.precond(z, r));
 			return c.ered<1>(FDot{r, z}, FinPcgBeta{c.d}, "pcg r.z");
-		}, (int64_t)s->max_iter));
+		}, (int64_t)s->max_iter, 1));
 	}
 	FX_CHECK(fetch_stats(s, stats));
 	if (stats) FX_CHECK(true_residual(c, x, b, ap, &stats->true_residual));

@@ -418,7 +418,7 @@ class Multigrid:
         return ms.value
 
     def set_wave_rows(self, level: int, post_rows: int, restrict_rows: int):
-        """Wave segment heights of one level (0: built-in choice)."""
+        """Wave segment heights of one level (0: built-in choice; -1 on levels >= 1: LDS-tile kernel)."""
         L.check(self._lib.fx_multigrid_set_wave_rows(self.h, level, post_rows, restrict_rows),
                 "multigrid_set_wave_rows")
 

@@ -253,13 +253,15 @@ fx_status fx_multigrid_vcycle(fx_multigrid* mg, fx_buf* x, const fx_buf* b);
  * default) runs each level as independent wave-wide row streams, FX_MGK_RING as LDS-ring row streams and
  * FX_MGK_TILED as LDS tiles; all three give bit-identical results.  fx_multigrid_set_wave_rows sets
  * level `level`'s wave segment heights (fine rows per wave of its post kernel, coarse rows per wave of
- * its restriction; 0 restores the built-in choice). */
+ * its restriction; 0 restores the built-in choice; −1 on levels ≥ 1 runs that kernel of the level as
+ * LDS tiles instead). */
 typedef enum { FX_MGK_TILED = 0, FX_MGK_RING = 1, FX_MGK_WAVE = 2 } fx_mg_kernels;
 fx_status fx_multigrid_set_kernels(fx_multigrid* mg, int32_t kernels);
 fx_status fx_multigrid_set_wave_rows(fx_multigrid* mg, uint32_t level, int32_t post_rows, int32_t restrict_rows);
 /* Autotuner (extension): times every level's wave kernels over candidate segment heights with HIP
  * events (reps launches each, on the hierarchy's own buffers, whose contents it overwrites) and keeps
- * the fastest per level; *ms_out (may be NULL) gets the summed per-launch time of the chosen shapes.
+ * the fastest per level, the LDS-tile kernels competing on levels ≥ 1; *ms_out (may be NULL) gets the
+ * summed per-launch time of the chosen shapes.
  * Level 0 is also timed as the fused FCG loop runs it (restriction with the residual update, post
  * kernel with the solver's reductions), on scratch vectors it allocates and frees; those heights are
  * kept apart.  fx_multigrid_set_wave_rows on level 0 sets both. */

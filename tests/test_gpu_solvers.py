@@ -469,6 +469,12 @@ def test_fused_vcycle_kernel_variants(be, shape, vk):
         for lev in range(mg.depth() - 1):
             mg.set_wave_rows(lev, post_rows, restrict_rows)
         assert np.array_equal(run(), ref), (post_rows, restrict_rows)
+    # Levels below 0 mixed: -1 runs that kernel as LDS tiles, alternating with wave kernels.
+    for lev in range(mg.depth() - 1):
+        mg.set_wave_rows(lev, -1 if lev % 2 else 16, 8 if lev % 2 or lev == 0 else -1)
+    assert np.array_equal(run(), ref), "mixed wave and tile levels"
+    with pytest.raises(Exception):
+        mg.set_wave_rows(0, -1, 8)               # level 0 runs the wave kernels
     with pytest.raises(Exception):
         mg.set_wave_rows(mg.depth() - 1, 8, 4)   # the coarsest level has no level kernels
     assert mg.autotune(2) > 0.0                  # the autotuner's picks give the same bits
