@@ -1248,12 +1248,12 @@ __device__ __forceinline__ void wpost_wave(const float* __restrict__ b, const fl
 			float p10 = 0.0f, p11 = 0.0f;
 			// Input ring: row T0 + j in slot j mod WP; the first WP − LZ rows now, the rest as slots free up.
 			WRow rb[WP];
-			EV evb[WP];
+			EV evb[WR];   // coarse-correction operands, WR rows ahead (cache hits mostly: a short lead suffices)
 			typename Epi::Pf pfb[WP];
 #pragma unroll
 			for (int k = 0; k < WP; ++k) {
 				rb[k] = k < WP - LZ ? wload<VK, LOADB>(b, L, T0 + k, rows, cols, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
-				evb[k] = eload();
+				if (k < WR) evb[k] = eload();
 				pfb[k] = pload(T0 + k - LZ);
 			}
 			float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, U[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, S[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -1279,7 +1279,7 @@ __device__ __forceinline__ void wpost_wave(const float* __restrict__ b, const fl
 					u = wjac0<VK>(L, rb[k]);
 				}
 				{
-					const EV& ev = evb[k];
+					const EV& ev = evb[q];
 					const bool adv = ev.a0 != ca;   // the coarse row below moved up by one
 					e00 = adv ? p10 : e00;
 					e01 = adv ? p11 : e01;
@@ -1315,7 +1315,7 @@ __device__ __forceinline__ void wpost_wave(const float* __restrict__ b, const fl
 				// hit), so that a segment does not fetch the next segment's rows from memory.
 				const int tn = t + WP - LZ < T1 ? t + WP - LZ : T1 - 1;
 				rb[Slot<WP, k, LZ>::v] = wload<VK, LOADB>(b, L, tn, rows, cols, uc);
-				evb[k] = eload();
+				evb[q] = eload();
 				pfb[k] = pload(tn);
 		};
 		run_steps<WP>(step, T0, T1, cl);
@@ -1454,11 +1454,12 @@ __device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, floa
 			return upd.load_at((size_t)rc * (size_t)fc, uc);
 		};
 		WRow rb[WQ];
-		typename Upd::Pf ub[WQ];  // Upd::ON: row t's r and A·p in slot (k + LR) mod WQ
+		typename Upd::Pf ub[WR];  // Upd::ON: row t's r and A·p in slot k mod WR, loaded WR steps ahead
 #pragma unroll
 		for (int k = 0; k < WQ; ++k) {
 			rb[k] = k < WQ - LR ? wload<VK, LOADB>(b, L, T0 + k, fr, fc, uc) : WRow{0.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
-			if constexpr (Upd::ON) ub[k] = uload(T0 + k - LR);
+			if constexpr (Upd::ON)
+				if (k < WR) ub[k] = uload(T0 + k);
 		}
 		float U1[WR] = {0.0f, 0.0f, 0.0f, 0.0f}, Uv[WR] = {0.0f, 0.0f, 0.0f, 0.0f};
 		auto step = [&](auto kc, int t, auto cl) {
@@ -1468,13 +1469,13 @@ __device__ __forceinline__ void wrestrict_wave(const float* __restrict__ b, floa
 			{
 				WRow v = wtake<VK>(rb[k], rb[Slot<WQ, k, 1>::v]);
 				if constexpr (Upd::ON) {
-					const double rn = upd.rnew(ub[Slot<WQ, k, -LR>::v]);
+					const double rn = upd.rnew(ub[q]);
 					v.b = float(rn);
 					if (ownc && t >= orl && t < orh) {
 						upd.store((size_t)t * (size_t)fc + (unsigned)gc, rn);
 						lsum += rn * rn;
 					}
-					ub[k] = uload(t + WQ - LR < T1 ? t + WQ - LR : T1 - 1);   // rows past the walk: never used
+					ub[q] = uload(t + WR < T1 ? t + WR : T1 - 1);   // rows past the walk: never used
 				}
 				rb[k] = CL ? v : wfix<VK>(v, t, fr, colin, kwok, keok);
 			}
