@@ -70,13 +70,15 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # 2^24 doubles (fetch ×2.000, write ×1.000); summary and commands in
 # profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels, and
 # only for the kernel sources the counters were read from: "sources_sha16" is the hash of
-# TRAFFIC_SOURCES at that run, and a bench whose sources differ reports traffic null.
+# TRAFFIC_SOURCES at that run, and a bench whose sources differ reports traffic null.  The level-0
+# V-cycle kernels' fetch depends on the autotuned segment heights (lag rows): the restriction's
+# figure is at 16 coarse rows per wave, the post kernel's at 16 fine rows (its writes at 32).
 TRAFFIC_SOURCES = ("fluxion_amd/csrc/fx_device.hpp", "fluxion_amd/csrc/fx_mgfast.hpp", "fluxion_amd/csrc/fx_solvers.hip")
 MEASURED_TRAFFIC_4096 = {
-    "sources_sha16": "a5f3e8a6b33aab5f",
-    "vc_restrict0": (277197 * 2 + 152147) * 1024,
-    "vc_post0": (284921 * 2 + 74934) * 1024,
-    "fcg_step": (244735 * 2 + 393248) * 1024,
+    "sources_sha16": "64ac4c3bd22c999c",
+    "vc_restrict0": (272809 * 2 + 151979) * 1024,
+    "vc_post0": (307830 * 2 + 74923) * 1024,
+    "fcg_step": (244676 * 2 + 393248) * 1024,
 }
 
 
