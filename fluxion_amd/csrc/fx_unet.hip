@@ -87,66 +87,86 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	// Pixel pitch cin_p + 8 keeps the 16 lanes of an A fragment (16 neighbouring pixels) off a
 	// power-of-two stride in LDS.
 	const int pitch = a.cin_p + 8;
-	constexpr int TPX = (CTH + 2) * (CTW + 2);
-	// One 16-byte item per (channel chunk, tile pixel); a thread loads SU items before storing any, so
-	// that SU global loads are in flight together.
-	constexpr int SU = 4;
-	auto fetch = [&](int item) -> u4v {
-		u4v v = {0u, 0u, 0u, 0u};
-		const int ch = (item / TPX) << 3, p = item - (item / TPX) * TPX;
-		const int pr = p / (CTW + 2), pc = p - pr * (CTW + 2);
+	constexpr int TW = CTW + 2, TPX = (CTH + 2) * TW;
+	// Staging, chunk-major: a thread owns tile pixels threadIdx.x + k·BLOCK (k < PPT), whose image
+	// position, bounds test and source pointers it forms once; the loop over 8-channel chunks is then
+	// wave-uniform (the up+cat source switch included), and CG chunks × PPT pixels of 16-byte loads
+	// are in flight together.  Pixels outside the image stage zeros.
+	constexpr int PPT = (TPX + BLOCK - 1) / BLOCK, CG = 2;
+	const int nch = a.cin_p >> 3;
+	const char* src0[PPT];   // the pixel's first byte in `in` (the coarse pixel for SRC_UPCAT)
+	const char* src1[PPT];   // SRC_UPCAT: the pixel in the skip `in2`; SRC_F32: its mask row
+	bool pin[PPT], pst[PPT];
+	int lofs[PPT];
+#pragma unroll
+	for (int k = 0; k < PPT; ++k) {
+		const int p = threadIdx.x + k * BLOCK;
+		const int pr = p / TW, pc = p - pr * TW;
 		const int rr = row0 - 1 + pr, cc = col0 - 1 + pc;
-		if (ch < a.cin_p && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W) {
-			const size_t pix = ((size_t)b * a.H + rr) * a.W + cc;
-			if constexpr (SRC == SRC_BF16) {
-				v = *(const u4v*)((const bf16*)a.in + pix * a.cin_p + ch);
-			} else if constexpr (SRC == SRC_UPCAT) {
-				if (ch < a.ca) {
-					const size_t pa = ((size_t)b * (a.H >> 1) + (rr >> 1)) * (a.W >> 1) + (cc >> 1);
-					v = *(const u4v*)((const bf16*)a.in + pa * a.ca + ch);
-				} else {
-					v = *(const u4v*)(a.in2 + pix * (a.cin_p - a.ca) + (ch - a.ca));
-				}
-			} else if constexpr (SRC == SRC_F32IN) {
-				const float* src = (const float*)a.in + pix * a.ca;
+		pst[k] = p < TPX;
+		pin[k] = pst[k] && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W;
+		const size_t pix = pin[k] ? ((size_t)b * a.H + rr) * a.W + cc : 0;
+		lofs[k] = p * pitch;
+		if constexpr (SRC == SRC_BF16) {
+			src0[k] = (const char*)((const bf16*)a.in + pix * a.cin_p);
+			src1[k] = nullptr;
+		} else if constexpr (SRC == SRC_UPCAT) {
+			const size_t pa = pin[k] ? ((size_t)b * (a.H >> 1) + (rr >> 1)) * (a.W >> 1) + (cc >> 1) : 0;
+			src0[k] = (const char*)((const bf16*)a.in + pa * a.ca);
+			src1[k] = (const char*)(a.in2 + pix * (a.cin_p - a.ca));
+		} else if constexpr (SRC == SRC_F32IN) {
+			src0[k] = (const char*)((const float*)a.in + pix * a.ca);
+			src1[k] = nullptr;
+		} else {
+			src0[k] = (const char*)((const float*)a.in + pix * a.cin_p);
+			src1[k] = a.mask ? (const char*)(a.mask + pix * a.cin_p) : nullptr;
+		}
+	}
+	auto fetch = [&](int k, int ch) -> u4v {
+		u4v v = {0u, 0u, 0u, 0u};
+		if (!pin[k]) return v;
+		if constexpr (SRC == SRC_BF16) {
+			v = *(const u4v*)((const bf16*)src0[k] + ch);
+		} else if constexpr (SRC == SRC_UPCAT) {
+			if (ch < a.ca) v = *(const u4v*)((const bf16*)src0[k] + ch);
+			else v = *(const u4v*)((const bf16*)src1[k] + (ch - a.ca));
+		} else if constexpr (SRC == SRC_F32IN) {
+			const float* src = (const float*)src0[k];
 #pragma unroll
-				for (int e = 0; e < 4; ++e) {
-					const int c = ch + 2 * e;
-					const bf16 lo = c < a.ca ? f2bf(src[c]) : (bf16)0;
-					const bf16 hi = c + 1 < a.ca ? f2bf(src[c + 1]) : (bf16)0;
-					v[e] = (uint32_t)lo | ((uint32_t)hi << 16);
-				}
-			} else {
-				const float* src = (const float*)a.in + pix * a.cin_p + ch;
-				const float4 lo = *(const float4*)src, hi = *(const float4*)(src + 4);
-				float f[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-				if (a.mask) {
-					const u4v mk = *(const u4v*)(a.mask + pix * a.cin_p + ch);
-#pragma unroll
-					for (int e = 0; e < 8; ++e) {
-						const bf16 mb = (bf16)(mk[e >> 1] >> ((e & 1) * 16));
-						if (!(bf2f(mb) > 0.f)) f[e] = 0.f;
-					}
-				}
-#pragma unroll
-				for (int e = 0; e < 4; ++e) v[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+			for (int e = 0; e < 4; ++e) {
+				const int c = ch + 2 * e;
+				const bf16 lo = c < a.ca ? f2bf(src[c]) : (bf16)0;
+				const bf16 hi = c + 1 < a.ca ? f2bf(src[c + 1]) : (bf16)0;
+				v[e] = (uint32_t)lo | ((uint32_t)hi << 16);
 			}
+		} else {
+			const float* src = (const float*)src0[k] + ch;
+			const float4 lo = *(const float4*)src, hi = *(const float4*)(src + 4);
+			float f[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+			if (src1[k]) {
+				const u4v mk = *(const u4v*)((const bf16*)src1[k] + ch);
+#pragma unroll
+				for (int e = 0; e < 8; ++e) {
+					const bf16 mb = (bf16)(mk[e >> 1] >> ((e & 1) * 16));
+					if (!(bf2f(mb) > 0.f)) f[e] = 0.f;
+				}
+			}
+#pragma unroll
+			for (int e = 0; e < 4; ++e) v[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
 		}
 		return v;
 	};
-	const int nitems = (a.cin_p >> 3) * TPX;
-	for (int base = threadIdx.x; base < nitems; base += SU * BLOCK) {
-		u4v v[SU];
+	for (int c8 = 0; c8 < nch; c8 += CG) {
+		u4v v[CG][PPT];
 #pragma unroll
-		for (int u = 0; u < SU; ++u) v[u] = fetch(base + u * BLOCK);
+		for (int q = 0; q < CG; ++q)
 #pragma unroll
-		for (int u = 0; u < SU; ++u) {
-			const int item = base + u * BLOCK;
-			if (item < nitems) {
-				const int ch = (item / TPX) << 3, p = item - (item / TPX) * TPX;
-				*(u4v*)(tile + (size_t)p * pitch + ch) = v[u];
-			}
-		}
+			for (int k = 0; k < PPT; ++k) v[q][k] = fetch(k, (c8 + q < nch ? c8 + q : c8) << 3);
+#pragma unroll
+		for (int q = 0; q < CG; ++q)
+#pragma unroll
+			for (int k = 0; k < PPT; ++k)
+				if (c8 + q < nch && pst[k]) *(u4v*)(tile + lofs[k] + ((c8 + q) << 3)) = v[q][k];
 	}
 	__syncthreads();
 
@@ -160,44 +180,62 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	// This lane's k = 8g + j (j = 0..7) within each 32-wide K step, as (tap, first channel c0).
 	int tap = 0, c0 = 8 * g;
 	while (c0 >= a.cin_p) { c0 -= a.cin_p; ++tap; }
-	const int K = 9 * a.cin_p;
+	const int nsteps = (9 * a.cin_p + 31) >> 5;
 	const u4v zero = {0u, 0u, 0u, 0u};
-	// Weight fragments of the K step at (tap, c0); a step past 9·cin_p (the tail) contributes zero.
-	auto wload = [&](int tp, int c, u4v* bw) {
-		const bool ok = tp < 9;
-		const int wt = a.flip ? 8 - (ok ? tp : 0) : (ok ? tp : 0);
+	// A fragment rows: tile pixel (wv + dr, 16·mt + m + dc) at LDS element lbase + (dr·TW + dc)·pitch
+	// + 16·mt·pitch; B fragment rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.
+	const int lbase = (wv * TW + m) * pitch;
+	const bf16* wrow[NT];
 #pragma unroll
-		for (int t = 0; t < NT; ++t) {
-			const u4v w = *(const u4v*)(a.w + ((size_t)(wt * a.cout_p + 16 * t + m)) * a.cin_p + c);
-			bw[t] = ok ? w : zero;
+	for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(16 * t + m) * a.cin_p;
+	const size_t wtap = (size_t)a.cout_p * a.cin_p;
+	// Weight fragments of the K step at (tap, c0); a step past 9·cin_p (only in the last step) reads
+	// tap 0, in bounds, and its fragments are replaced by zeros where they are used.
+	auto wload = [&](int tp, int c, u4v* bw) {
+		const int t9 = tp < 9 ? tp : 0;
+		const int wt = a.flip ? 8 - t9 : t9;
+#pragma unroll
+		for (int t = 0; t < NT; ++t) bw[t] = *(const u4v*)(wrow[t] + wt * wtap + c);
+	};
+	auto adv = [&](int& tp, int& c) {
+		c += 32;
+		while (c >= a.cin_p) { c -= a.cin_p; ++tp; }
+	};
+	// One K step; LAST (the final step, the only one whose k can pass 9·cin_p) masks the lanes past it.
+	auto kstep = [&](auto last, const u4v* bc) {
+		constexpr bool LAST = decltype(last)::value;
+		const bool ok = !LAST || tap < 9;
+		const int t9 = LAST ? (ok ? tap : 0) : tap;
+		const int dr = (t9 * 11) >> 5, dc = t9 - 3 * dr;   // t9 / 3 and t9 % 3 for t9 < 9
+		const bf16* ap = tile + lbase + (dr * TW + dc) * pitch + c0;
+		u4v x[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(ap + 16 * mt * pitch);
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) {
+			const bf8 av = __builtin_bit_cast(bf8, LAST && !ok ? zero : x[mt]);
+#pragma unroll
+			for (int t = 0; t < NT; ++t) {
+				const u4v bv = LAST && !ok ? zero : bc[t];
+				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, bv), acc[mt][t], 0, 0, 0);
+			}
 		}
 	};
 	u4v bcur[NT];
 	wload(tap, c0, bcur);
-	for (int kk = 0; kk < K; kk += 32) {
-		const bool ok = tap < 9;
-		const int t9 = ok ? tap : 0;
-		const int dr = t9 / 3, dc = t9 - 3 * dr;
-		// The next step's (tap, c0), and its weights fetched while this step computes.
-		int ntap = tap, nc0 = c0 + 32;
-		while (nc0 >= a.cin_p) { nc0 -= a.cin_p; ++ntap; }
+	for (int s1 = 1; s1 < nsteps; ++s1) {
+		// The next step's weights are fetched while this step computes.
+		int ntap = tap, nc0 = c0;
+		adv(ntap, nc0);
 		u4v bnext[NT];
 		wload(ntap, nc0, bnext);
-		u4v x[4];
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(tile + ((size_t)(wv + dr) * (CTW + 2) + 16 * mt + m + dc) * pitch + c0);
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) {
-			const bf8 av = __builtin_bit_cast(bf8, ok ? x[mt] : zero);
-#pragma unroll
-			for (int t = 0; t < NT; ++t)
-				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, bcur[t]), acc[mt][t], 0, 0, 0);
-		}
+		kstep(std::false_type{}, bcur);
 #pragma unroll
 		for (int t = 0; t < NT; ++t) bcur[t] = bnext[t];
 		tap = ntap;
 		c0 = nc0;
 	}
+	kstep(std::true_type{}, bcur);
 
 	const int orow = row0 + wv;
 	auto value = [&](int mt, int t, int j) {
