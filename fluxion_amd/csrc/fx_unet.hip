@@ -68,10 +68,17 @@ struct ConvArgs {
 };
 
 constexpr int CTW = 64, CTH = 4;   // output tile of a workgroup: CTH rows (one per wave) × CTW columns
+// SRC_UPCAT stages its upsampled operand at the coarse resolution: the coarse pixels under the
+// (CTH + 2) × (CTW + 2) fine input pixels, CUR rows × CUW columns (CTH and CTW even, tiles start on
+// even rows and columns).
+constexpr int CUW = CTW / 2 + 2, CUR = CTH / 2 + 2, CUPX = CUR * CUW;
 // LDS of a workgroup: the staged input tile, reused afterwards for the BF16 output transpose
-// (CTH waves × CTW pixels × cout_p).
-inline size_t conv_lds_bytes(int cin_p, int cout_p) {
-	const size_t in = (size_t)(CTH + 2) * (CTW + 2) * (cin_p + 8) * sizeof(bf16);
+// (CTH waves × CTW pixels × cout_p).  ca > 0: the up+cat tile (coarse operand with pitch ca + 8,
+// then the fine skip with pitch cin_p − ca + 8).
+inline size_t conv_lds_bytes(int cin_p, int cout_p, int ca = 0) {
+	const size_t tpx = (size_t)(CTH + 2) * (CTW + 2);
+	const size_t in = ca > 0 ? ((size_t)CUPX * (ca + 8) + tpx * (cin_p - ca + 8)) * sizeof(bf16)
+	                         : tpx * (cin_p + 8) * sizeof(bf16);
 	const size_t out = (size_t)CTH * CTW * cout_p * sizeof(bf16);
 	return in > out ? in : out;
 }
@@ -90,12 +97,16 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	constexpr int TW = CTW + 2, TPX = (CTH + 2) * TW;
 	// Staging, chunk-major: a thread owns tile pixels threadIdx.x + k·BLOCK (k < PPT), whose image
 	// position, bounds test and source pointers it forms once; the loop over 8-channel chunks is then
-	// wave-uniform (the up+cat source switch included), and CG chunks × PPT pixels of 16-byte loads
-	// are in flight together.  Pixels outside the image stage zeros.
+	// wave-uniform, and CG chunks × PPT pixels of 16-byte loads are in flight together.  Pixels
+	// outside the image stage zeros.
 	constexpr int PPT = (TPX + BLOCK - 1) / BLOCK, CG = 2;
-	const int nch = a.cin_p >> 3;
-	const char* src0[PPT];   // the pixel's first byte in `in` (the coarse pixel for SRC_UPCAT)
-	const char* src1[PPT];   // SRC_UPCAT: the pixel in the skip `in2`; SRC_F32: its mask row
+	// SRC_UPCAT: the upsampled operand's CUR × CUW coarse pixels at element 0 with pitch cpitch, the
+	// skip's fine pixels after them (element soff) with pitch spitch.  A fine pixel outside the image
+	// is exactly a coarse pixel outside the coarse image (H and W are even), so both stage zeros there.
+	const int cpitch = a.ca + 8, spitch = a.cin_p - a.ca + 8, soff = CUPX * cpitch;
+	const int nch = SRC == SRC_UPCAT ? (a.cin_p - a.ca) >> 3 : a.cin_p >> 3;
+	const char* src0[PPT];   // the pixel's first byte in `in` (SRC_UPCAT: in the skip `in2`)
+	const char* src1[PPT];   // SRC_F32: the pixel's mask row
 	bool pin[PPT], pst[PPT];
 	int lofs[PPT];
 #pragma unroll
@@ -106,14 +117,13 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 		pst[k] = p < TPX;
 		pin[k] = pst[k] && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W;
 		const size_t pix = pin[k] ? ((size_t)b * a.H + rr) * a.W + cc : 0;
-		lofs[k] = p * pitch;
+		lofs[k] = SRC == SRC_UPCAT ? soff + p * spitch : p * pitch;
 		if constexpr (SRC == SRC_BF16) {
 			src0[k] = (const char*)((const bf16*)a.in + pix * a.cin_p);
 			src1[k] = nullptr;
 		} else if constexpr (SRC == SRC_UPCAT) {
-			const size_t pa = pin[k] ? ((size_t)b * (a.H >> 1) + (rr >> 1)) * (a.W >> 1) + (cc >> 1) : 0;
-			src0[k] = (const char*)((const bf16*)a.in + pa * a.ca);
-			src1[k] = (const char*)(a.in2 + pix * (a.cin_p - a.ca));
+			src0[k] = (const char*)(a.in2 + pix * (a.cin_p - a.ca));
+			src1[k] = nullptr;
 		} else if constexpr (SRC == SRC_F32IN) {
 			src0[k] = (const char*)((const float*)a.in + pix * a.ca);
 			src1[k] = nullptr;
@@ -122,14 +132,36 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 			src1[k] = a.mask ? (const char*)(a.mask + pix * a.cin_p) : nullptr;
 		}
 	}
+	if constexpr (SRC == SRC_UPCAT) {
+		// The coarse operand: thread t < CUPX stages coarse tile pixel t, four 16-byte loads in flight.
+		static_assert(CUPX <= BLOCK, "one coarse pixel per thread");
+		const int t = threadIdx.x;
+		if (t < CUPX) {
+			const int pr = t / CUW, pc = t - pr * CUW;
+			const int Hc = a.H >> 1, Wc = a.W >> 1;
+			const int rr = (row0 >> 1) - 1 + pr, cc = (col0 >> 1) - 1 + pc;
+			const bool in = rr >= 0 && rr < Hc && cc >= 0 && cc < Wc;
+			const bf16* src = (const bf16*)a.in + (in ? (((size_t)b * Hc + rr) * Wc + cc) * a.ca : 0);
+			bf16* dst = tile + t * cpitch;
+			const int nc = a.ca >> 3;
+			for (int c8 = 0; c8 < nc; c8 += 4) {
+				u4v v[4];
+#pragma unroll
+				for (int q = 0; q < 4; ++q) {
+					v[q] = u4v{0u, 0u, 0u, 0u};
+					if (in && c8 + q < nc) v[q] = *(const u4v*)(src + ((c8 + q) << 3));
+				}
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					if (c8 + q < nc) *(u4v*)(dst + ((c8 + q) << 3)) = v[q];
+			}
+		}
+	}
 	auto fetch = [&](int k, int ch) -> u4v {
 		u4v v = {0u, 0u, 0u, 0u};
 		if (!pin[k]) return v;
-		if constexpr (SRC == SRC_BF16) {
+		if constexpr (SRC == SRC_BF16 || SRC == SRC_UPCAT) {
 			v = *(const u4v*)((const bf16*)src0[k] + ch);
-		} else if constexpr (SRC == SRC_UPCAT) {
-			if (ch < a.ca) v = *(const u4v*)((const bf16*)src0[k] + ch);
-			else v = *(const u4v*)((const bf16*)src1[k] + (ch - a.ca));
 		} else if constexpr (SRC == SRC_F32IN) {
 			const float* src = (const float*)src0[k];
 #pragma unroll
@@ -184,7 +216,10 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	const u4v zero = {0u, 0u, 0u, 0u};
 	// A fragment rows: tile pixel (wv + dr, 16·mt + m + dc) at LDS element lbase + (dr·TW + dc)·pitch
 	// + 16·mt·pitch; B fragment rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.
-	const int lbase = (wv * TW + m) * pitch;
+	// SRC_UPCAT: a lane whose 8 channels lie in the upsampled operand (c0 < ca) reads the coarse pixel
+	// under that fine pixel, coarse tile row (wv + dr + 1)/2 and column 8·mt + (m + dc + 1)/2;
+	// otherwise the skip tile at channel c0 − ca.
+	const int lbase = SRC == SRC_UPCAT ? soff + (wv * TW + m) * spitch : (wv * TW + m) * pitch;
 	const bf16* wrow[NT];
 #pragma unroll
 	for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(16 * t + m) * a.cin_p;
@@ -207,10 +242,23 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 		const bool ok = !LAST || tap < 9;
 		const int t9 = LAST ? (ok ? tap : 0) : tap;
 		const int dr = (t9 * 11) >> 5, dc = t9 - 3 * dr;   // t9 / 3 and t9 % 3 for t9 < 9
-		const bf16* ap = tile + lbase + (dr * TW + dc) * pitch + c0;
+		const bf16* ap;
+		int mstride;
+		if constexpr (SRC == SRC_UPCAT) {
+			if (c0 < a.ca) {
+				ap = tile + (((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1)) * cpitch + c0;
+				mstride = 8 * cpitch;
+			} else {
+				ap = tile + lbase + (dr * TW + dc) * spitch + (c0 - a.ca);
+				mstride = 16 * spitch;
+			}
+		} else {
+			ap = tile + lbase + (dr * TW + dc) * pitch + c0;
+			mstride = 16 * pitch;
+		}
 		u4v x[4];
 #pragma unroll
-		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(ap + 16 * mt * pitch);
+		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(ap + mt * mstride);
 #pragma unroll
 		for (int mt = 0; mt < 4; ++mt) {
 			const bf8 av = __builtin_bit_cast(bf8, LAST && !ok ? zero : x[mt]);
@@ -303,7 +351,8 @@ fx_status launch_conv_nt(const ConvArgs& a, hipStream_t st) {
 		attr = true;
 	}
 	dim3 grid((unsigned)((a.W + CTW - 1) / CTW), (unsigned)((a.H + CTH - 1) / CTH), (unsigned)a.B);
-	hipLaunchKernelGGL((k_conv3m<SRC, OUTK, NT>), grid, dim3(BLOCK), conv_lds_bytes(a.cin_p, a.cout_p), st, a);
+	hipLaunchKernelGGL((k_conv3m<SRC, OUTK, NT>), grid, dim3(BLOCK),
+	                   conv_lds_bytes(a.cin_p, a.cout_p, SRC == SRC_UPCAT ? a.ca : 0), st, a);
 	return fx::check_launch("unet conv");
 }
 template <int SRC, int OUTK>
