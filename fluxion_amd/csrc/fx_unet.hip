@@ -62,6 +62,7 @@ struct ConvArgs {
 	const bf16* w;         // [9][cout_p][cin_p]
 	const float* bias;     // [cout] or null
 	void* out;             // OUT_BF16 bf16 / OUT_F32 f32 [B][H][W][cout_p]; OUT_F32_CH0 f32 [B][H][W]
+	bf16* pool;            // OUT_BF16: also the 2×2 average of the output, bf16 [B][H/2][W/2][cout_p], or null
 	int B, H, W, cin_p, cout_p, cout, ca;
 	int relu, flip;
 	const int* skip;       // kernels return early while *skip != 0 (may be null)
@@ -327,6 +328,34 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * a.cout_p;
 			const int nchunk = nvalid * a.cout_p / 8;
 			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
+			if (a.pool && (wv & 1) == 0) {
+				// The encoder's 2×2 average pooling of rows orow and orow + 1 (this wave's and the next
+				// one's, both in LDS): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] +
+				// x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in BF16 storage).
+				const bf16* s1 = st + CTW * a.cout_p;
+				const int nck = a.cout_p >> 3, npc = nvalid >> 1;
+				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * a.cout_p;
+				for (int q = lane; q < npc * nck; q += 64) {
+					const int pc = q / nck, ch = (q - pc * nck) << 3;
+					const int e0 = 2 * pc * a.cout_p + ch;
+					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + a.cout_p);
+					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + a.cout_p);
+					u4v o;
+#pragma unroll
+					for (int e = 0; e < 4; ++e) {
+						uint32_t r2 = 0;
+#pragma unroll
+						for (int half = 0; half < 2; ++half) {
+							const int sh = 16 * half;
+							const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) +
+							                         bf2f((bf16)(a2[e] >> sh)) + bf2f((bf16)(a3[e] >> sh)));
+							r2 |= (uint32_t)f2bf(v) << sh;
+						}
+						o[e] = r2;
+					}
+					*(u4v*)(pdst + (size_t)pc * a.cout_p + ch) = o;
+				}
+			}
 		} else {
 			float* st = (float*)tile + (size_t)wv * CTW;
 			if (m == 0)
@@ -439,38 +468,6 @@ __global__ void k_bgrad(const float* dy, const bf16* y, int relu, float* db, lon
 	atomicAdd(&db[co], s);
 }
 
-// y[b][r][c][ch] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] + x[2r+1][2c+1])), 8 channels per
-// thread (16-byte loads and store); C is a multiple of 8.
-__global__ void k_pool2(const bf16* x, bf16* y, int B, int H, int W, int C, const int* skip) {
-	if (skip && *skip) return;
-	const int h = H / 2, w = W / 2, nch = C >> 3;
-	const long n = (long)B * h * w * nch;
-	for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-		const long p = q / nch;
-		const int ch = (int)(q - p * nch) << 3;
-		const int cc = (int)(p % w);
-		const long br = p / w;                 // b·h + rr
-		const int rr = (int)(br % h);
-		const long b = br / h;
-		const bf16* x0 = x + (((size_t)b * H + 2 * rr) * W + 2 * cc) * C + ch;
-		const u4v a0 = *(const u4v*)x0, a1 = *(const u4v*)(x0 + C);
-		const u4v a2 = *(const u4v*)(x0 + (size_t)W * C), a3 = *(const u4v*)(x0 + (size_t)W * C + C);
-		u4v o;
-#pragma unroll
-		for (int e = 0; e < 4; ++e) {
-			uint32_t r2 = 0;
-#pragma unroll
-			for (int half = 0; half < 2; ++half) {
-				const int sh = 16 * half;
-				const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) + bf2f((bf16)(a2[e] >> sh)) +
-				                         bf2f((bf16)(a3[e] >> sh)));
-				r2 |= (uint32_t)f2bf(v) << sh;
-			}
-			o[e] = r2;
-		}
-		*(u4v*)(y + (size_t)p * C + ch) = o;
-	}
-}
 // dx (f32, [B][H][W][C]) += 0.25·dy[B][H/2][W/2][C]
 __global__ void k_pool2_bwd(const float* dy, float* dx, int B, int H, int W, int C) {
 	long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -726,9 +723,10 @@ fx_status pack_weights(fx_unet* u) {
 // One convolution of layer li at level l: the forward (weights Wp, bias, ReLU on all but the output
 // layer) or, with data_grad, the data gradient (the transposed, flipped weights Wt).
 fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int ca, const bf16* mask, void* out,
-               int outk, int l, bool data_grad) {
+               int outk, int l, bool data_grad, bf16* pool = nullptr) {
 	ConvArgs a;
 	a.in = in;
+	a.pool = pool;
 	a.in2 = in2;
 	a.ca = ca;
 	a.mask = mask;
@@ -776,18 +774,15 @@ fx_status forward(fx_unet* u, bool infer, float* y, const float* xraw = nullptr)
 	int li = 0;
 	for (int l = 0; l < 4; ++l) {
 		for (int k = 0; k < 2; ++k, ++li) {
+			// The level's second convolution also writes the pooled map for the next level (l < 3).
+			bf16* pool = k == 1 && l < 3 ? u->pool[l] : nullptr;
 			if (li == 0 && xraw)   // the raw FP32 input, rounded and padded while staged
 				FX_CHECK(conv(u, li, SRC_F32IN, xraw, nullptr, u->cin, nullptr, u->act[li], OUT_BF16, l, false));
 			else
-				FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false));
+				FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false, pool));
 			h = u->act[li];
 		}
-		if (l < 3) {
-			long n = (long)u->B * (u->H >> (l + 1)) * (u->W >> (l + 1)) * (CH[l] / 8);
-			hipLaunchKernelGGL(k_pool2, dim3((unsigned)std::min<long>(nb(n), 65536)), dim3(256), 0, st, h, u->pool[l],
-			                   u->B, u->H >> l, u->W >> l, CH[l], u->skip);
-			h = u->pool[l];
-		}
+		if (l < 3) h = u->pool[l];
 	}
 	for (int l = 2; l >= 0; --l) {
 		int ca = CH[l + 1], cs = CH[l], cop = pad16(ca + cs);
