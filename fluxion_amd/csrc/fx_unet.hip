@@ -50,8 +50,9 @@ inline int pad16(int c) { return (c + 15) / 16 * 16; }
 
 // Source of a convolution's staged input tile, and the form of its output.  SRC_F32IN is the
 // network's raw FP32 input [B][H][W][ca], rounded to BF16 and zero-padded to cin_p as it is staged
-// (the bits k_pack_in would store), so inference needs no packed copy of it.
-enum { SRC_BF16 = 0, SRC_UPCAT = 1, SRC_F32 = 2, SRC_F32IN = 3 };
+// (the bits k_pack_in would store), so inference needs no packed copy of it; SRC_F32IN_K is the same
+// input (two channels) read by k_conv3p, whose K packs the nine taps' channels densely.
+enum { SRC_BF16 = 0, SRC_UPCAT = 1, SRC_F32 = 2, SRC_F32IN = 3, SRC_F32IN_K = 4 };
 enum { OUT_BF16 = 0, OUT_F32 = 1, OUT_F32_CH0 = 2 };
 
 struct ConvArgs {
@@ -87,11 +88,204 @@ inline size_t conv_lds_bytes(int cin_p, int cout_p, int ca = 0) {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
+// This lane's bias values, channel 16·t + (lane % 16) for t < NT (0 past cout or without a bias).
+template <int NT>
+__device__ __forceinline__ void load_bias(const ConvArgs& a, float (&bv)[NT]) {
+	const int m = threadIdx.x & 15;
+#pragma unroll
+	for (int t = 0; t < NT; ++t) bv[t] = a.bias && 16 * t + m < a.cout ? a.bias[16 * t + m] : 0.f;
+}
+
+// Bias, ReLU and the output form of a workgroup's CTH × CTW tile from the MFMA accumulators (wave wv
+// holds row row0 + wv; acc[mt][t][j] is pixel 16·mt + 4·g + j, channel 16·t + m of lane 16·g + m);
+// bv from load_bias.  Every thread of the workgroup calls it (it holds workgroup barriers, and returns
+// nowhere early); the input tile in LDS is reused for the output transpose.
+template <int OUTK, int NT>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT], const float (&bv)[NT], bf16* tile,
+                                              int b, int row0, int col0) {
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int m = lane & 15, g = lane >> 4;
+	const int orow = row0 + wv;
+	const bool rowok = orow < a.H;
+	auto value = [&](int mt, int t, int j) {
+		const int co = 16 * t + m;
+		float v = acc[mt][t][j];
+		if (co < a.cout && a.bias) v += bv[t];
+		if (co >= a.cout) v = 0.f;
+		if (a.relu && v < 0.f) v = 0.f;
+		return v;
+	};
+	if constexpr (OUTK == OUT_F32) {
+		// Training's data gradients: FP32 stores straight from the fragments (16 lanes write 64
+		// contiguous bytes).
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+			for (int t = 0; t < NT; ++t)
+#pragma unroll
+				for (int j = 0; j < 4; ++j) {
+					const int col = col0 + 16 * mt + 4 * g + j;
+					if (rowok && col < a.W)
+						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * a.cout_p + 16 * t + m] = value(mt, t, j);
+				}
+	} else {
+		// The wave's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS
+		// (the input tile is no longer read once every wave has passed the barrier) and store 16 bytes
+		// per lane.
+		__syncthreads();
+		if constexpr (OUTK == OUT_BF16) {
+			bf16* st = tile + (size_t)wv * CTW * a.cout_p;
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+				for (int t = 0; t < NT; ++t)
+#pragma unroll
+					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * a.cout_p + 16 * t + m] = f2bf(value(mt, t, j));
+			__syncthreads();
+			const int nvalid = min(CTW, a.W - col0);
+			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * a.cout_p;
+			const int nchunk = rowok ? nvalid * a.cout_p / 8 : 0;
+			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
+			if (a.pool && (wv & 1) == 0 && rowok) {
+				// The encoder's 2×2 average pooling of rows orow and orow + 1 (this wave's and the next
+				// one's, both in LDS): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] +
+				// x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in BF16 storage).
+				const bf16* s1 = st + CTW * a.cout_p;
+				const int nck = a.cout_p >> 3, npc = nvalid >> 1;
+				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * a.cout_p;
+				for (int q = lane; q < npc * nck; q += 64) {
+					const int pc = q / nck, ch = (q - pc * nck) << 3;
+					const int e0 = 2 * pc * a.cout_p + ch;
+					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + a.cout_p);
+					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + a.cout_p);
+					u4v o;
+#pragma unroll
+					for (int e = 0; e < 4; ++e) {
+						uint32_t r2 = 0;
+#pragma unroll
+						for (int half = 0; half < 2; ++half) {
+							const int sh = 16 * half;
+							const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) +
+							                         bf2f((bf16)(a2[e] >> sh)) + bf2f((bf16)(a3[e] >> sh)));
+							r2 |= (uint32_t)f2bf(v) << sh;
+						}
+						o[e] = r2;
+					}
+					*(u4v*)(pdst + (size_t)pc * a.cout_p + ch) = o;
+				}
+			}
+		} else {
+			float* st = (float*)tile + (size_t)wv * CTW;
+			if (m == 0)
+#pragma unroll
+				for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+					for (int j = 0; j < 4; ++j) st[16 * mt + 4 * g + j] = value(mt, 0, j);
+			__syncthreads();
+			const int col = col0 + lane;
+			if (rowok && col < a.W) ((float*)a.out)[((size_t)b * a.H + orow) * a.W + col] = st[lane];
+		}
+	}
+}
+
+// The K loop of a CTH × CTW output tile over the input tile staged in LDS (k_conv3m's layout: pitch
+// cin_p + 8 per pixel, or for SRC_UPCAT the coarse operand then the skip): acc = Σ_k A·B over
+// k = (tap, channel), weights read through the cache one K step ahead.
+template <int SRC, int NT>
+__device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
+	constexpr int TW = CTW + 2;
+	const int pitch = a.cin_p + 8;
+	const int cpitch = a.ca + 8, spitch = a.cin_p - a.ca + 8, soff = CUPX * cpitch;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int m = lane & 15, g = lane >> 4;
+#pragma unroll
+	for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+		for (int t = 0; t < NT; ++t) acc[mt][t] = f4{0.f, 0.f, 0.f, 0.f};
+	// This lane's k = 8g + j (j = 0..7) within each 32-wide K step, as (tap, first channel c0).
+	int tap = 0, c0 = 8 * g;
+	while (c0 >= a.cin_p) { c0 -= a.cin_p; ++tap; }
+	const int nsteps = (9 * a.cin_p + 31) >> 5;
+	const u4v zero = {0u, 0u, 0u, 0u};
+	// A fragment rows: tile pixel (wv + dr, 16·mt + m + dc) at LDS element lbase + (dr·TW + dc)·pitch
+	// + 16·mt·pitch; B fragment rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.
+	// SRC_UPCAT: a lane whose 8 channels lie in the upsampled operand (c0 < ca) reads the coarse pixel
+	// under that fine pixel, coarse tile row (wv + dr + 1)/2 and column 8·mt + (m + dc + 1)/2;
+	// otherwise the skip tile at channel c0 − ca.
+	const int lbase = SRC == SRC_UPCAT ? soff + (wv * TW + m) * spitch : (wv * TW + m) * pitch;
+	const bf16* wrow[NT];
+#pragma unroll
+	for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(16 * t + m) * a.cin_p;
+	const size_t wtap = (size_t)a.cout_p * a.cin_p;
+	// Weight fragments of the K step at (tap, c0); a step past 9·cin_p (only in the last step) reads
+	// tap 0, in bounds, and its fragments are replaced by zeros where they are used.
+	auto wload = [&](int tp, int c, u4v* bw) {
+		const int t9 = tp < 9 ? tp : 0;
+		const int wt = a.flip ? 8 - t9 : t9;
+#pragma unroll
+		for (int t = 0; t < NT; ++t) bw[t] = *(const u4v*)(wrow[t] + wt * wtap + c);
+	};
+	auto adv = [&](int& tp, int& c) {
+		c += 32;
+		while (c >= a.cin_p) { c -= a.cin_p; ++tp; }
+	};
+	// One K step; LAST (the final step, the only one whose k can pass 9·cin_p) masks the lanes past it.
+	auto kstep = [&](auto last, const u4v* bc) {
+		constexpr bool LAST = decltype(last)::value;
+		const bool ok = !LAST || tap < 9;
+		const int t9 = LAST ? (ok ? tap : 0) : tap;
+		const int dr = (t9 * 11) >> 5, dc = t9 - 3 * dr;   // t9 / 3 and t9 % 3 for t9 < 9
+		const bf16* ap;
+		int mstride;
+		if constexpr (SRC == SRC_UPCAT) {
+			if (c0 < a.ca) {
+				ap = tile + (((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1)) * cpitch + c0;
+				mstride = 8 * cpitch;
+			} else {
+				ap = tile + lbase + (dr * TW + dc) * spitch + (c0 - a.ca);
+				mstride = 16 * spitch;
+			}
+		} else {
+			ap = tile + lbase + (dr * TW + dc) * pitch + c0;
+			mstride = 16 * pitch;
+		}
+		u4v x[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(ap + mt * mstride);
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) {
+			const bf8 av = __builtin_bit_cast(bf8, LAST && !ok ? zero : x[mt]);
+#pragma unroll
+			for (int t = 0; t < NT; ++t) {
+				const u4v bv = LAST && !ok ? zero : bc[t];
+				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, bv), acc[mt][t], 0, 0, 0);
+			}
+		}
+	};
+	u4v bcur[NT];
+	wload(tap, c0, bcur);
+	for (int s1 = 1; s1 < nsteps; ++s1) {
+		// The next step's weights are fetched while this step computes.
+		int ntap = tap, nc0 = c0;
+		adv(ntap, nc0);
+		u4v bnext[NT];
+		wload(ntap, nc0, bnext);
+		kstep(std::false_type{}, bcur);
+#pragma unroll
+		for (int t = 0; t < NT; ++t) bcur[t] = bnext[t];
+		tap = ntap;
+		c0 = nc0;
+	}
+	kstep(std::true_type{}, bcur);
+}
+
 template <int SRC, int OUTK, int NT>
 __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	if (a.skip && *a.skip) return;
 	extern __shared__ __align__(16) bf16 tile[];
 	const int b = blockIdx.z, row0 = blockIdx.y * CTH, col0 = blockIdx.x * CTW;
+	float bv[NT];
+	load_bias<NT>(a, bv);
 	// Pixel pitch cin_p + 8 keeps the 16 lanes of an A fragment (16 neighbouring pixels) off a
 	// power-of-two stride in LDS.
 	const int pitch = a.cin_p + 8;
@@ -203,172 +397,173 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	}
 	__syncthreads();
 
+	f4 acc[4][NT];
+	conv_kloop<SRC, NT>(a, tile, acc);
+	conv_epilogue<OUTK, NT>(a, acc, bv, tile, b, row0, col0);
+}
+
+// Persistent convolutions with 16 output channels at full resolution, for two inputs: a 16-channel
+// BF16 map (SRC_BF16) and the network's raw 2-channel FP32 input (SRC_F32IN_K, rounded to BF16 as it
+// is staged, the bits k_pack_in stores), whose K packs the nine taps' two channels densely as
+// k = 2·tap + c: one 32-wide K step instead of five over channels padded to 16.  A workgroup loops over
+// output tiles t = blockIdx.x, blockIdx.x + gridDim.x, … with its weights and bias in registers, and
+// loads its next tile's input into registers while it computes and stores the current one, so the
+// staging latency of one tile overlaps the work of the previous one (the non-persistent k_conv3m runs
+// each workgroup's load, compute and store one after another).  Output tile, LDS layout, MFMA
+// fragments and epilogue are k_conv3m's.
+template <int SRC, int OUTK>
+__global__ __launch_bounds__(BLOCK, 4) void k_conv3p(ConvArgs a) {
+	if (a.skip && *a.skip) return;
+	extern __shared__ __align__(16) bf16 tile[];
+	constexpr bool PK = SRC == SRC_F32IN_K;
+	static_assert(PK || SRC == SRC_BF16, "16-channel BF16 or packed 2-channel FP32 input");
+	constexpr int TW = CTW + 2, TPX = (CTH + 2) * TW, PPT = (TPX + BLOCK - 1) / BLOCK;
+	constexpr int PITCH = PK ? 2 : 16 + 8;   // LDS elements per staged pixel
+	constexpr int NCH = PK ? 1 : 2;          // loads per staged pixel (PK: one 8-byte pair of floats)
+	constexpr int NSTEP = PK ? 1 : 5;        // 32-wide K steps (9·16 = 144 → 5, the last half zero)
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int m = lane & 15, g = lane >> 4;
-	f4 acc[4][NT];
+	const int ntx = (a.W + CTW - 1) / CTW, nty = (a.H + CTH - 1) / CTH, ntiles = ntx * nty * a.B;
+	float bv[1];
+	load_bias<1>(a, bv);
+	// B fragments of every K step (weights w[tap][16][16], output channel m), and this lane's A
+	// offsets from tile pixel (wv, m): SRC_BF16 step s covers taps 2s and 2s + 1, the lane's 8
+	// channels starting at c0 = 8(g % 2) of tap 2s + g/2; PK: the lane's k = 8g..8g+7 are taps
+	// 4g..4g+3, channels 0 and 1 of each (one dword of the staged pixel).
+	u4v wf[NSTEP];
+	int aoff[PK ? 4 : NSTEP];
+	bool aok[PK ? 4 : NSTEP];
+	if constexpr (PK) {
+		wf[0] = u4v{0u, 0u, 0u, 0u};
 #pragma unroll
-	for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-		for (int t = 0; t < NT; ++t) acc[mt][t] = f4{0.f, 0.f, 0.f, 0.f};
-	// This lane's k = 8g + j (j = 0..7) within each 32-wide K step, as (tap, first channel c0).
-	int tap = 0, c0 = 8 * g;
-	while (c0 >= a.cin_p) { c0 -= a.cin_p; ++tap; }
-	const int nsteps = (9 * a.cin_p + 31) >> 5;
-	const u4v zero = {0u, 0u, 0u, 0u};
-	// A fragment rows: tile pixel (wv + dr, 16·mt + m + dc) at LDS element lbase + (dr·TW + dc)·pitch
-	// + 16·mt·pitch; B fragment rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.
-	// SRC_UPCAT: a lane whose 8 channels lie in the upsampled operand (c0 < ca) reads the coarse pixel
-	// under that fine pixel, coarse tile row (wv + dr + 1)/2 and column 8·mt + (m + dc + 1)/2;
-	// otherwise the skip tile at channel c0 − ca.
-	const int lbase = SRC == SRC_UPCAT ? soff + (wv * TW + m) * spitch : (wv * TW + m) * pitch;
-	const bf16* wrow[NT];
-#pragma unroll
-	for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(16 * t + m) * a.cin_p;
-	const size_t wtap = (size_t)a.cout_p * a.cin_p;
-	// Weight fragments of the K step at (tap, c0); a step past 9·cin_p (only in the last step) reads
-	// tap 0, in bounds, and its fragments are replaced by zeros where they are used.
-	auto wload = [&](int tp, int c, u4v* bw) {
-		const int t9 = tp < 9 ? tp : 0;
-		const int wt = a.flip ? 8 - t9 : t9;
-#pragma unroll
-		for (int t = 0; t < NT; ++t) bw[t] = *(const u4v*)(wrow[t] + wt * wtap + c);
-	};
-	auto adv = [&](int& tp, int& c) {
-		c += 32;
-		while (c >= a.cin_p) { c -= a.cin_p; ++tp; }
-	};
-	// One K step; LAST (the final step, the only one whose k can pass 9·cin_p) masks the lanes past it.
-	auto kstep = [&](auto last, const u4v* bc) {
-		constexpr bool LAST = decltype(last)::value;
-		const bool ok = !LAST || tap < 9;
-		const int t9 = LAST ? (ok ? tap : 0) : tap;
-		const int dr = (t9 * 11) >> 5, dc = t9 - 3 * dr;   // t9 / 3 and t9 % 3 for t9 < 9
-		const bf16* ap;
-		int mstride;
-		if constexpr (SRC == SRC_UPCAT) {
-			if (c0 < a.ca) {
-				ap = tile + (((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1)) * cpitch + c0;
-				mstride = 8 * cpitch;
-			} else {
-				ap = tile + lbase + (dr * TW + dc) * spitch + (c0 - a.ca);
-				mstride = 16 * spitch;
-			}
-		} else {
-			ap = tile + lbase + (dr * TW + dc) * pitch + c0;
-			mstride = 16 * pitch;
+		for (int i = 0; i < 4; ++i) {
+			const int tap = 4 * g + i;
+			aok[i] = tap < 9;
+			const int t9 = aok[i] ? tap : 0, dr = t9 / 3, dc = t9 - 3 * dr;
+			aoff[i] = (dr * TW + dc) * PITCH;
+			const bf16 w0 = aok[i] ? a.w[(size_t)(t9 * 16 + m) * a.cin_p] : (bf16)0;
+			const bf16 w1 = aok[i] ? a.w[(size_t)(t9 * 16 + m) * a.cin_p + 1] : (bf16)0;
+			wf[0][i] = (uint32_t)w0 | ((uint32_t)w1 << 16);
 		}
-		u4v x[4];
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(ap + mt * mstride);
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) {
-			const bf8 av = __builtin_bit_cast(bf8, LAST && !ok ? zero : x[mt]);
-#pragma unroll
-			for (int t = 0; t < NT; ++t) {
-				const u4v bv = LAST && !ok ? zero : bc[t];
-				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, bv), acc[mt][t], 0, 0, 0);
-			}
-		}
-	};
-	u4v bcur[NT];
-	wload(tap, c0, bcur);
-	for (int s1 = 1; s1 < nsteps; ++s1) {
-		// The next step's weights are fetched while this step computes.
-		int ntap = tap, nc0 = c0;
-		adv(ntap, nc0);
-		u4v bnext[NT];
-		wload(ntap, nc0, bnext);
-		kstep(std::false_type{}, bcur);
-#pragma unroll
-		for (int t = 0; t < NT; ++t) bcur[t] = bnext[t];
-		tap = ntap;
-		c0 = nc0;
-	}
-	kstep(std::true_type{}, bcur);
-
-	const int orow = row0 + wv;
-	auto value = [&](int mt, int t, int j) {
-		const int co = 16 * t + m;
-		float v = acc[mt][t][j];
-		if (co < a.cout && a.bias) v += a.bias[co];
-		if (co >= a.cout) v = 0.f;
-		if (a.relu && v < 0.f) v = 0.f;
-		return v;
-	};
-	if constexpr (OUTK == OUT_F32) {
-		// Training's data gradients: FP32 stores straight from the fragments (16 lanes write 64
-		// contiguous bytes).
-		if (orow >= a.H) return;
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-			for (int t = 0; t < NT; ++t)
-#pragma unroll
-				for (int j = 0; j < 4; ++j) {
-					const int col = col0 + 16 * mt + 4 * g + j;
-					if (col < a.W)
-						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * a.cout_p + 16 * t + m] = value(mt, t, j);
-				}
 	} else {
-		// The wave's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS
-		// (the input tile is no longer read once every wave has passed the barrier) and store 16 bytes
-		// per lane.
-		__syncthreads();
-		if constexpr (OUTK == OUT_BF16) {
-			bf16* st = tile + (size_t)wv * CTW * a.cout_p;
 #pragma unroll
-			for (int mt = 0; mt < 4; ++mt)
+		for (int s = 0; s < NSTEP; ++s) {
+			const int tap = 2 * s + (g >> 1), c0 = 8 * (g & 1);
+			aok[s] = tap < 9;
+			const int t9 = aok[s] ? tap : 0, dr = t9 / 3, dc = t9 - 3 * dr;
+			aoff[s] = (dr * TW + dc) * PITCH + c0;
+			wf[s] = aok[s] ? *(const u4v*)(a.w + (size_t)(t9 * 16 + m) * 16 + c0) : u4v{0u, 0u, 0u, 0u};
+		}
+	}
+	const int lbase = (wv * TW + m) * PITCH;
+	// Staging registers of the next tile: pixel threadIdx.x + k·BLOCK of its (CTH + 2) × (CTW + 2)
+	// input window (PK: the raw FP32 pair in sv[k][0][0..1]).  The loads are unconditional, from pixel
+	// 0 of the image where the window pixel lies outside it (pin false: zeros are staged instead), so
+	// nothing waits for them before the next tile's staging stores.
+	u4v sv[PPT][NCH];
+	bool pin[PPT];
+	auto issue = [&](int t) {
+		const int b = t / (ntx * nty), rem = t - b * ntx * nty, ty = rem / ntx, tx = rem - ty * ntx;
+		const int row0 = ty * CTH, col0 = tx * CTW;
 #pragma unroll
-				for (int t = 0; t < NT; ++t)
+		for (int k = 0; k < PPT; ++k) {
+			const int p = threadIdx.x + k * BLOCK;
+			const int pr = p / TW, pc = p - pr * TW;
+			const int rr = row0 - 1 + pr, cc = col0 - 1 + pc;
+			pin[k] = p < TPX && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W;
+			const size_t pix = pin[k] ? ((size_t)b * a.H + rr) * a.W + cc : 0;
+			if constexpr (PK) {
+				const float2 f = *(const float2*)((const float*)a.in + pix * 2);
+				sv[k][0] = u4v{__float_as_uint(f.x), __float_as_uint(f.y), 0u, 0u};
+			} else {
 #pragma unroll
-					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * a.cout_p + 16 * t + m] = f2bf(value(mt, t, j));
-			__syncthreads();
-			if (orow >= a.H) return;
-			const int nvalid = min(CTW, a.W - col0);
-			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * a.cout_p;
-			const int nchunk = nvalid * a.cout_p / 8;
-			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
-			if (a.pool && (wv & 1) == 0) {
-				// The encoder's 2×2 average pooling of rows orow and orow + 1 (this wave's and the next
-				// one's, both in LDS): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] +
-				// x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in BF16 storage).
-				const bf16* s1 = st + CTW * a.cout_p;
-				const int nck = a.cout_p >> 3, npc = nvalid >> 1;
-				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * a.cout_p;
-				for (int q = lane; q < npc * nck; q += 64) {
-					const int pc = q / nck, ch = (q - pc * nck) << 3;
-					const int e0 = 2 * pc * a.cout_p + ch;
-					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + a.cout_p);
-					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + a.cout_p);
-					u4v o;
+				for (int q = 0; q < NCH; ++q) sv[k][q] = *(const u4v*)((const bf16*)a.in + pix * 16 + 8 * q);
+			}
+		}
+	};
+	int t = blockIdx.x;
+	if (t < ntiles) issue(t);
+	for (; t < ntiles; t += gridDim.x) {
+		const int b = t / (ntx * nty), rem = t - b * ntx * nty, ty = rem / ntx, tx = rem - ty * ntx;
+		const int row0 = ty * CTH, col0 = tx * CTW;
 #pragma unroll
-					for (int e = 0; e < 4; ++e) {
-						uint32_t r2 = 0;
+		for (int k = 0; k < PPT; ++k) {
+			const int p = threadIdx.x + k * BLOCK;
+			if (p < TPX) {
+				if constexpr (PK) {
+					const uint32_t v = (uint32_t)f2bf(__uint_as_float(sv[k][0][0])) |
+					                   ((uint32_t)f2bf(__uint_as_float(sv[k][0][1])) << 16);
+					*(uint32_t*)(tile + p * PITCH) = pin[k] ? v : 0u;
+				} else {
 #pragma unroll
-						for (int half = 0; half < 2; ++half) {
-							const int sh = 16 * half;
-							const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) +
-							                         bf2f((bf16)(a2[e] >> sh)) + bf2f((bf16)(a3[e] >> sh)));
-							r2 |= (uint32_t)f2bf(v) << sh;
-						}
-						o[e] = r2;
-					}
-					*(u4v*)(pdst + (size_t)pc * a.cout_p + ch) = o;
+					for (int q = 0; q < NCH; ++q)
+						*(u4v*)(tile + p * PITCH + 8 * q) = pin[k] ? sv[k][q] : u4v{0u, 0u, 0u, 0u};
 				}
 			}
+		}
+		__syncthreads();
+		if (t + (int)gridDim.x < ntiles) issue(t + gridDim.x);
+		f4 acc[4][1];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) acc[mt][0] = f4{0.f, 0.f, 0.f, 0.f};
+		if constexpr (PK) {
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) {
+				u4v x;
+#pragma unroll
+				for (int i = 0; i < 4; ++i) x[i] = aok[i] ? *(const uint32_t*)(tile + lbase + 16 * mt * PITCH + aoff[i]) : 0u;
+				acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, x),
+				                                                      __builtin_bit_cast(bf8, wf[0]), acc[mt][0], 0, 0, 0);
+			}
 		} else {
-			float* st = (float*)tile + (size_t)wv * CTW;
-			if (m == 0)
+#pragma unroll
+			for (int s = 0; s < NSTEP; ++s) {
+				u4v x[4];
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt)
+					x[mt] = aok[s] ? *(const u4v*)(tile + lbase + 16 * mt * PITCH + aoff[s]) : u4v{0u, 0u, 0u, 0u};
 #pragma unroll
-					for (int j = 0; j < 4; ++j) st[16 * mt + 4 * g + j] = value(mt, 0, j);
-			__syncthreads();
-			if (orow >= a.H) return;
-			const int col = col0 + lane;
-			if (col < a.W) ((float*)a.out)[((size_t)b * a.H + orow) * a.W + col] = st[lane];
+				for (int mt = 0; mt < 4; ++mt)
+					acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, x[mt]),
+					                                                      __builtin_bit_cast(bf8, wf[s]), acc[mt][0], 0, 0, 0);
+			}
 		}
+		conv_epilogue<OUTK, 1>(a, acc, bv, tile, b, row0, col0);
+		__syncthreads();   // the epilogue's LDS reads end before the next tile is staged
 	}
+}
+
+// Grid of a persistent convolution: as many workgroups as fit on the device at once (at most one per
+// tile); `resident` caches the count for the kernel.
+inline fx_status persistent_grid(const void* kern, size_t lds, const ConvArgs& a, int* resident, unsigned* grid) {
+	if (*resident == 0) {
+		int dev = 0, ncu = 0, per_cu = 0;
+		FX_HIP(hipGetDevice(&dev));
+		FX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+		FX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, lds));
+		*resident = std::max(1, ncu * std::max(1, per_cu));
+	}
+	const long ntiles = (long)((a.W + CTW - 1) / CTW) * ((a.H + CTH - 1) / CTH) * a.B;
+	FX_REQUIRE(ntiles < (1L << 31), FX_ERR_DIMENSION, "too many convolution tiles");
+	*grid = (unsigned)std::min<long>(ntiles, *resident);
+	return FX_OK;
+}
+
+inline size_t conv3p_lds_bytes(int src) {
+	const size_t in = (size_t)(CTH + 2) * (CTW + 2) * (src == SRC_F32IN_K ? 2 : 24) * sizeof(bf16);
+	const size_t out = (size_t)CTH * CTW * 16 * sizeof(bf16);
+	return in > out ? in : out;
+}
+
+template <int SRC, int OUTK>
+fx_status launch_conv3p(const ConvArgs& a, hipStream_t st) {
+	static int resident = 0;
+	const void* kern = (const void*)k_conv3p<SRC, OUTK>;
+	const size_t lds = conv3p_lds_bytes(SRC);
+	unsigned grid = 0;
+	FX_CHECK(persistent_grid(kern, lds, a, &resident, &grid));
+	hipLaunchKernelGGL((k_conv3p<SRC, OUTK>), dim3(grid), dim3(BLOCK), lds, st, a);
+	return fx::check_launch("unet persistent conv");
 }
 
 template <int SRC, int OUTK, int NT>
@@ -754,6 +949,14 @@ fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int
 	a.out = out;
 	a.skip = data_grad ? nullptr : u->skip;
 	hipStream_t st = u->be->stream;
+	// 16 → 16 channel forward convolutions (and the packed 2-channel input) run persistent.
+	if (src == SRC_F32IN_K && outk == OUT_BF16) {
+		FX_REQUIRE(a.ca == 2 && a.cin_p == 16 && a.cout_p == 16, FX_ERR_ARGUMENT, "packed-K first conv: 2 -> 16 channels");
+		return launch_conv3p<SRC_F32IN_K, OUT_BF16>(a, st);
+	}
+	const bool p16 = src == SRC_BF16 && a.cin_p == 16 && a.cout_p == 16 && !a.flip;
+	if (p16 && outk == OUT_BF16) return launch_conv3p<SRC_BF16, OUT_BF16>(a, st);
+	if (p16 && outk == OUT_F32_CH0) return launch_conv3p<SRC_BF16, OUT_F32_CH0>(a, st);
 	if (src == SRC_BF16 && outk == OUT_BF16) return launch_conv<SRC_BF16, OUT_BF16>(a, st);
 	if (src == SRC_F32IN && outk == OUT_BF16) return launch_conv<SRC_F32IN, OUT_BF16>(a, st);
 	if (src == SRC_UPCAT && outk == OUT_BF16) return launch_conv<SRC_UPCAT, OUT_BF16>(a, st);
@@ -776,8 +979,9 @@ fx_status forward(fx_unet* u, bool infer, float* y, const float* xraw = nullptr)
 		for (int k = 0; k < 2; ++k, ++li) {
 			// The level's second convolution also writes the pooled map for the next level (l < 3).
 			bf16* pool = k == 1 && l < 3 ? u->pool[l] : nullptr;
-			if (li == 0 && xraw)   // the raw FP32 input, rounded and padded while staged
-				FX_CHECK(conv(u, li, SRC_F32IN, xraw, nullptr, u->cin, nullptr, u->act[li], OUT_BF16, l, false));
+			if (li == 0 && xraw)   // the raw FP32 input, rounded (and padded) while staged
+				FX_CHECK(conv(u, li, u->cin == 2 ? SRC_F32IN_K : SRC_F32IN, xraw, nullptr, u->cin, nullptr,
+				              u->act[li], OUT_BF16, l, false));
 			else
 				FX_CHECK(conv(u, li, SRC_BF16, h, nullptr, 0, nullptr, u->act[li], OUT_BF16, l, false, pool));
 			h = u->act[li];
