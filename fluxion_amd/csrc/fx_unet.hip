@@ -75,14 +75,26 @@ constexpr int CTW = 64, CTH = 4;   // output tile of a workgroup: CTH rows (one 
 // even rows and columns).
 constexpr int CUW = CTW / 2 + 2, CUR = CTH / 2 + 2, CUPX = CUR * CUW;
 // LDS of a workgroup: the staged input tile, reused afterwards for the BF16 output transpose
-// (CTH waves × CTW pixels × cout_p).  ca > 0: the up+cat tile (coarse operand with pitch ca + 8,
-// then the fine skip with pitch cin_p − ca + 8).
+// (CTH waves × CTW pixels × cout_p).  ca > 0: the up+cat tile (coarse operand, ca channels per pixel,
+// then the fine skip, cin_p − ca).  Pixels are stored unpadded with their 16-byte channel chunks
+// swizzled (chunk_sw).
 inline size_t conv_lds_bytes(int cin_p, int cout_p, int ca = 0) {
 	const size_t tpx = (size_t)(CTH + 2) * (CTW + 2);
-	const size_t in = ca > 0 ? ((size_t)CUPX * (ca + 8) + tpx * (cin_p - ca + 8)) * sizeof(bf16)
-	                         : tpx * (cin_p + 8) * sizeof(bf16);
+	const size_t in = ca > 0 ? ((size_t)CUPX * ca + tpx * (cin_p - ca)) * sizeof(bf16) : tpx * cin_p * sizeof(bf16);
 	const size_t out = (size_t)CTH * CTW * cout_p * sizeof(bf16);
 	return in > out ? in : out;
+}
+
+// LDS swizzle of the staged tiles: 8-channel chunk c of staged pixel P, with n (even) chunks per
+// pixel, sits at chunk c ^ chunk_sw(P, n) of the pixel.  An A fragment's 16-byte reads by 8
+// consecutive pixels (one LDS pass) then fall on 8 different groups of four banks without padding the
+// pixels: with d = gcd(n, 8), pixels P and P + 8/d share a bank group unswizzled, and the XOR with
+// (P / (8/d)) mod d separates them (it permutes chunks within aligned blocks of d, so c ^ sw < n).
+// The value is the same for P and P + 16 (P + 8 on the coarse tile), i.e. for all four M tiles.
+__device__ __forceinline__ int chunk_sw(int P, int n) {
+	const int d = (n & -n) < 8 ? (n & -n) : 8;
+	const int sh = d == 8 ? 0 : (d == 4 ? 1 : 2);
+	return (P >> sh) & (d - 1);
 }
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -188,14 +200,14 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 	}
 }
 
-// The K loop of a CTH × CTW output tile over the input tile staged in LDS (k_conv3m's layout: pitch
-// cin_p + 8 per pixel, or for SRC_UPCAT the coarse operand then the skip): acc = Σ_k A·B over
-// k = (tap, channel), weights read through the cache one K step ahead.
+// The K loop of a CTH × CTW output tile over the input tile staged in LDS (k_conv3m's layout: cin_p
+// elements per pixel, or for SRC_UPCAT the coarse operand then the skip, chunks swizzled by
+// chunk_sw): acc = Σ_k A·B over k = (tap, channel), weights read through the cache one K step ahead.
 template <int SRC, int NT>
 __device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
 	constexpr int TW = CTW + 2;
-	const int pitch = a.cin_p + 8;
-	const int cpitch = a.ca + 8, spitch = a.cin_p - a.ca + 8, soff = CUPX * cpitch;
+	const int pitch = a.cin_p;
+	const int cpitch = a.ca, spitch = a.cin_p - a.ca, soff = CUPX * cpitch;
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int m = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -207,12 +219,11 @@ __device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, 
 	while (c0 >= a.cin_p) { c0 -= a.cin_p; ++tap; }
 	const int nsteps = (9 * a.cin_p + 31) >> 5;
 	const u4v zero = {0u, 0u, 0u, 0u};
-	// A fragment rows: tile pixel (wv + dr, 16·mt + m + dc) at LDS element lbase + (dr·TW + dc)·pitch
-	// + 16·mt·pitch; B fragment rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.
-	// SRC_UPCAT: a lane whose 8 channels lie in the upsampled operand (c0 < ca) reads the coarse pixel
-	// under that fine pixel, coarse tile row (wv + dr + 1)/2 and column 8·mt + (m + dc + 1)/2;
-	// otherwise the skip tile at channel c0 − ca.
-	const int lbase = SRC == SRC_UPCAT ? soff + (wv * TW + m) * spitch : (wv * TW + m) * pitch;
+	// A fragment rows: tile pixel P = (wv + dr)·TW + 16·mt + m + dc, its chunk c0/8 swizzled; B fragment
+	// rows: output channel 16·t + m of tap wt at wrow[t] + wt·cout_p·cin_p.  SRC_UPCAT: a lane whose 8
+	// channels lie in the upsampled operand (c0 < ca) reads the coarse pixel under that fine pixel,
+	// coarse tile row (wv + dr + 1)/2 and column 8·mt + (m + dc + 1)/2; otherwise the skip tile at
+	// channel c0 − ca.
 	const bf16* wrow[NT];
 #pragma unroll
 	for (int t = 0; t < NT; ++t) wrow[t] = a.w + (size_t)(16 * t + m) * a.cin_p;
@@ -239,14 +250,17 @@ __device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, 
 		int mstride;
 		if constexpr (SRC == SRC_UPCAT) {
 			if (c0 < a.ca) {
-				ap = tile + (((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1)) * cpitch + c0;
+				const int Q = ((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1);
+				ap = tile + Q * cpitch + (((c0 >> 3) ^ chunk_sw(Q, cpitch >> 3)) << 3);
 				mstride = 8 * cpitch;
 			} else {
-				ap = tile + lbase + (dr * TW + dc) * spitch + (c0 - a.ca);
+				const int P = (wv + dr) * TW + m + dc;
+				ap = tile + soff + P * spitch + ((((c0 - a.ca) >> 3) ^ chunk_sw(P, spitch >> 3)) << 3);
 				mstride = 16 * spitch;
 			}
 		} else {
-			ap = tile + lbase + (dr * TW + dc) * pitch + c0;
+			const int P = (wv + dr) * TW + m + dc;
+			ap = tile + P * pitch + (((c0 >> 3) ^ chunk_sw(P, pitch >> 3)) << 3);
 			mstride = 16 * pitch;
 		}
 		u4v x[4];
@@ -286,9 +300,8 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	const int b = blockIdx.z, row0 = blockIdx.y * CTH, col0 = blockIdx.x * CTW;
 	float bv[NT];
 	load_bias<NT>(a, bv);
-	// Pixel pitch cin_p + 8 keeps the 16 lanes of an A fragment (16 neighbouring pixels) off a
-	// power-of-two stride in LDS.
-	const int pitch = a.cin_p + 8;
+	// Pixels are stored unpadded, their chunks swizzled by chunk_sw.
+	const int pitch = a.cin_p;
 	constexpr int TW = CTW + 2, TPX = (CTH + 2) * TW;
 	// Staging, chunk-major: a thread owns tile pixels threadIdx.x + k·BLOCK (k < PPT), whose image
 	// position, bounds test and source pointers it forms once; the loop over 8-channel chunks is then
@@ -298,12 +311,12 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	// SRC_UPCAT: the upsampled operand's CUR × CUW coarse pixels at element 0 with pitch cpitch, the
 	// skip's fine pixels after them (element soff) with pitch spitch.  A fine pixel outside the image
 	// is exactly a coarse pixel outside the coarse image (H and W are even), so both stage zeros there.
-	const int cpitch = a.ca + 8, spitch = a.cin_p - a.ca + 8, soff = CUPX * cpitch;
+	const int cpitch = a.ca, spitch = a.cin_p - a.ca, soff = CUPX * cpitch;
 	const int nch = SRC == SRC_UPCAT ? (a.cin_p - a.ca) >> 3 : a.cin_p >> 3;
 	const char* src0[PPT];   // the pixel's first byte in `in` (SRC_UPCAT: in the skip `in2`)
 	const char* src1[PPT];   // SRC_F32: the pixel's mask row
 	bool pin[PPT], pst[PPT];
-	int lofs[PPT];
+	int lofs[PPT], swz[PPT];
 #pragma unroll
 	for (int k = 0; k < PPT; ++k) {
 		const int p = threadIdx.x + k * BLOCK;
@@ -313,6 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 		pin[k] = pst[k] && rr >= 0 && rr < a.H && cc >= 0 && cc < a.W;
 		const size_t pix = pin[k] ? ((size_t)b * a.H + rr) * a.W + cc : 0;
 		lofs[k] = SRC == SRC_UPCAT ? soff + p * spitch : p * pitch;
+		swz[k] = chunk_sw(p, nch);
 		if constexpr (SRC == SRC_BF16) {
 			src0[k] = (const char*)((const bf16*)a.in + pix * a.cin_p);
 			src1[k] = nullptr;
@@ -338,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 			const bool in = rr >= 0 && rr < Hc && cc >= 0 && cc < Wc;
 			const bf16* src = (const bf16*)a.in + (in ? (((size_t)b * Hc + rr) * Wc + cc) * a.ca : 0);
 			bf16* dst = tile + t * cpitch;
-			const int nc = a.ca >> 3;
+			const int nc = a.ca >> 3, sw = chunk_sw(t, nc);
 			for (int c8 = 0; c8 < nc; c8 += 4) {
 				u4v v[4];
 #pragma unroll
@@ -348,7 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 				}
 #pragma unroll
 				for (int q = 0; q < 4; ++q)
-					if (c8 + q < nc) *(u4v*)(dst + ((c8 + q) << 3)) = v[q];
+					if (c8 + q < nc) *(u4v*)(dst + (((c8 + q) ^ sw) << 3)) = v[q];
 			}
 		}
 	}
@@ -393,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 		for (int q = 0; q < CG; ++q)
 #pragma unroll
 			for (int k = 0; k < PPT; ++k)
-				if (c8 + q < nch && pst[k]) *(u4v*)(tile + lofs[k] + ((c8 + q) << 3)) = v[q][k];
+				if (c8 + q < nch && pst[k]) *(u4v*)(tile + lofs[k] + (((c8 + q) ^ swz[k]) << 3)) = v[q][k];
 	}
 	__syncthreads();
 
