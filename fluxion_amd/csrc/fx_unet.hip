@@ -35,12 +35,8 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;
 
-__device__ __forceinline__ bf16 f2bf(float f) {
-	uint32_t u = __float_as_uint(f);
-	if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);   // inf / nan
-	u += 0x7fffu + ((u >> 16) & 1u);                                    // round to nearest even
-	return (bf16)(u >> 16);
-}
+// FP32 → BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
+__device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 __device__ __forceinline__ float bf2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
 
 constexpr int NLAYER = 15;
@@ -293,7 +289,88 @@ __device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, 
 	kstep(std::true_type{}, bcur);
 }
 
-template <int SRC, int OUTK, int NT>
+// One K segment of the compile-time K loop: the NCH 8-channel chunks of every tap of one staged
+// operand, K step s covering chunks 4s .. 4s + 3 of the segment's (tap, chunk) order (lane g takes
+// chunk 4s + g).  A step spans at most two taps: lanes g ≥ NCH − (4s mod NCH) wrap to the next one,
+// so a lane's tap, chunk and LDS address are a compare and selects between constants.  COARSE reads
+// the up+cat tile's coarse operand (pixel under the fine pixel, rows and columns halved), otherwise a
+// fine tile at element BASE; CH0 is the segment's first channel in the weights.  Weights are fetched
+// two steps ahead.
+template <int NT, int CINP, int NCH, bool COARSE, int BASE, int CH0>
+__device__ __forceinline__ void conv_kseg(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
+	constexpr int TW = CTW + 2, PITCH = 8 * NCH, NSTEP = (9 * NCH + 3) / 4, PD = 2;
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int m = lane & 15, g = lane >> 4;
+	const bf16* wl = a.w + (size_t)m * CINP + CH0;   // + (tap·cout_p + 16t)·CINP + 8·chunk
+	const int wtap = a.cout_p * CINP;
+	auto step = [&](int s, int& tap, int& c8) {
+		const int q0 = 4 * s, t0 = q0 / NCH, r0 = q0 - t0 * NCH;
+		const bool wrap = g >= NCH - r0;
+		tap = t0 + (wrap ? 1 : 0);
+		c8 = r0 + g - (wrap ? NCH : 0);
+	};
+	u4v wb[NSTEP][NT];
+	auto wload = [&](int s) {
+		int tap, c8;
+		step(s, tap, c8);
+		const int t9 = tap < 9 ? tap : 0;
+		const int wt = a.flip ? 8 - t9 : t9;
+#pragma unroll
+		for (int t = 0; t < NT; ++t) wb[s][t] = *(const u4v*)(wl + (size_t)(wt * wtap + 16 * t * CINP + 8 * c8));
+	};
+#pragma unroll
+	for (int s = 0; s < PD && s < NSTEP; ++s) wload(s);
+	const u4v zero = {0u, 0u, 0u, 0u};
+#pragma unroll
+	for (int s = 0; s < NSTEP; ++s) {
+		if (s + PD < NSTEP) wload(s + PD);
+		int tap, c8;
+		step(s, tap, c8);
+		const bool ok = tap < 9;   // false only past the last tap, in the last step
+		const int t9 = ok ? tap : 0;
+		const int dr = t9 / 3, dc = t9 - 3 * dr;
+		int addr, mstride;
+		if constexpr (COARSE) {
+			const int Q = ((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1);
+			addr = BASE + Q * PITCH + ((c8 ^ chunk_sw(Q, NCH)) << 3);
+			mstride = 8 * PITCH;
+		} else {
+			const int P = (wv + dr) * TW + m + dc;
+			addr = BASE + P * PITCH + ((c8 ^ chunk_sw(P, NCH)) << 3);
+			mstride = 16 * PITCH;
+		}
+		u4v x[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(tile + addr + mt * mstride);
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt) {
+			const bf8 av = __builtin_bit_cast(bf8, ok ? x[mt] : zero);
+#pragma unroll
+			for (int t = 0; t < NT; ++t)
+				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, ok ? wb[s][t] : zero),
+				                                                     acc[mt][t], 0, 0, 0);
+		}
+	}
+}
+
+// conv_kloop with the channel counts known at compile time (CINP input channels, of which CA
+// upsampled for SRC_UPCAT): the K loop fully unrolled over conv_kseg segments — for up+cat first every
+// tap's coarse chunks, then every tap's skip chunks, so no step mixes the two tiles.
+template <int SRC, int NT, int CINP, int CA>
+__device__ __forceinline__ void conv_kloop_ct(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
+#pragma unroll
+	for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+		for (int t = 0; t < NT; ++t) acc[mt][t] = f4{0.f, 0.f, 0.f, 0.f};
+	if constexpr (SRC == SRC_UPCAT) {
+		conv_kseg<NT, CINP, CA / 8, true, 0, 0>(a, tile, acc);
+		conv_kseg<NT, CINP, (CINP - CA) / 8, false, CUPX * CA, CA>(a, tile, acc);
+	} else {
+		conv_kseg<NT, CINP, CINP / 8, false, 0, 0>(a, tile, acc);
+	}
+}
+
+template <int SRC, int OUTK, int NT, int CINP = 0, int CA = 0>
 __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	if (a.skip && *a.skip) return;
 	extern __shared__ __align__(16) bf16 tile[];
@@ -312,7 +389,8 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	// skip's fine pixels after them (element soff) with pitch spitch.  A fine pixel outside the image
 	// is exactly a coarse pixel outside the coarse image (H and W are even), so both stage zeros there.
 	const int cpitch = a.ca, spitch = a.cin_p - a.ca, soff = CUPX * cpitch;
-	const int nch = SRC == SRC_UPCAT ? (a.cin_p - a.ca) >> 3 : a.cin_p >> 3;
+	const int nch = CINP > 0 ? (SRC == SRC_UPCAT ? (CINP - CA) >> 3 : CINP >> 3)
+	                         : (SRC == SRC_UPCAT ? (a.cin_p - a.ca) >> 3 : a.cin_p >> 3);
 	const char* src0[PPT];   // the pixel's first byte in `in` (SRC_UPCAT: in the skip `in2`)
 	const char* src1[PPT];   // SRC_F32: the pixel's mask row
 	bool pin[PPT], pst[PPT];
@@ -352,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 			const bool in = rr >= 0 && rr < Hc && cc >= 0 && cc < Wc;
 			const bf16* src = (const bf16*)a.in + (in ? (((size_t)b * Hc + rr) * Wc + cc) * a.ca : 0);
 			bf16* dst = tile + t * cpitch;
-			const int nc = a.ca >> 3, sw = chunk_sw(t, nc);
+			const int nc = (CINP > 0 ? CA : a.ca) >> 3, sw = chunk_sw(t, nc);
 			for (int c8 = 0; c8 < nc; c8 += 4) {
 				u4v v[4];
 #pragma unroll
@@ -412,7 +490,8 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	__syncthreads();
 
 	f4 acc[4][NT];
-	conv_kloop<SRC, NT>(a, tile, acc);
+	if constexpr (CINP > 0) conv_kloop_ct<SRC, NT, CINP, CA>(a, tile, acc);
+	else conv_kloop<SRC, NT>(a, tile, acc);
 	conv_epilogue<OUTK, NT>(a, acc, bv, tile, b, row0, col0);
 }
 
@@ -593,6 +672,42 @@ fx_status launch_conv_nt(const ConvArgs& a, hipStream_t st) {
 	                   conv_lds_bytes(a.cin_p, a.cout_p, SRC == SRC_UPCAT ? a.ca : 0), st, a);
 	return fx::check_launch("unet conv");
 }
+// The forward convolutions of the network with their channel counts fixed at compile time
+// (conv_kloop_ct); launch_ct reports whether this shape has one.
+template <int SRC, int NT, int CINP, int CA>
+fx_status launch_ct1(const ConvArgs& a, hipStream_t st) {
+	static bool attr = false;
+	const void* kern = (const void*)k_conv3m<SRC, OUT_BF16, NT, CINP, CA>;
+	if (!attr) {
+		FX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds_bytes(128, 128)));
+		attr = true;
+	}
+	dim3 grid((unsigned)((a.W + CTW - 1) / CTW), (unsigned)((a.H + CTH - 1) / CTH), (unsigned)a.B);
+	hipLaunchKernelGGL((k_conv3m<SRC, OUT_BF16, NT, CINP, CA>), grid, dim3(BLOCK),
+	                   conv_lds_bytes(a.cin_p, a.cout_p, SRC == SRC_UPCAT ? a.ca : 0), st, a);
+	return fx::check_launch("unet conv");
+}
+fx_status launch_ct(int src, int outk, const ConvArgs& a, hipStream_t st, bool* done) {
+	*done = true;
+	if (outk == OUT_BF16 && src == SRC_BF16) {
+		const int nt = a.cout_p / 16, ci = a.cin_p;
+		if (nt == 2 && ci == 16) return launch_ct1<SRC_BF16, 2, 16, 0>(a, st);
+		if (nt == 2 && ci == 32) return launch_ct1<SRC_BF16, 2, 32, 0>(a, st);
+		if (nt == 3 && ci == 32) return launch_ct1<SRC_BF16, 3, 32, 0>(a, st);
+		if (nt == 3 && ci == 48) return launch_ct1<SRC_BF16, 3, 48, 0>(a, st);
+		if (nt == 4 && ci == 48) return launch_ct1<SRC_BF16, 4, 48, 0>(a, st);
+		if (nt == 4 && ci == 64) return launch_ct1<SRC_BF16, 4, 64, 0>(a, st);
+	}
+	if (outk == OUT_BF16 && src == SRC_UPCAT) {
+		const int nt = a.cout_p / 16, ci = a.cin_p, ca = a.ca;
+		if (nt == 1 && ci == 48 && ca == 32) return launch_ct1<SRC_UPCAT, 1, 48, 32>(a, st);
+		if (nt == 2 && ci == 80 && ca == 48) return launch_ct1<SRC_UPCAT, 2, 80, 48>(a, st);
+		if (nt == 3 && ci == 112 && ca == 64) return launch_ct1<SRC_UPCAT, 3, 112, 64>(a, st);
+	}
+	*done = false;
+	return FX_OK;
+}
+
 template <int SRC, int OUTK>
 fx_status launch_conv(const ConvArgs& a, hipStream_t st) {
 	switch (a.cout_p / 16) {
@@ -971,6 +1086,9 @@ fx_status conv(fx_unet* u, int li, int src, const void* in, const bf16* in2, int
 	const bool p16 = src == SRC_BF16 && a.cin_p == 16 && a.cout_p == 16 && !a.flip;
 	if (p16 && outk == OUT_BF16) return launch_conv3p<SRC_BF16, OUT_BF16>(a, st);
 	if (p16 && outk == OUT_F32_CH0) return launch_conv3p<SRC_BF16, OUT_F32_CH0>(a, st);
+	bool done = false;
+	const fx_status cs = launch_ct(src, outk, a, st, &done);
+	if (done) return cs;
 	if (src == SRC_BF16 && outk == OUT_BF16) return launch_conv<SRC_BF16, OUT_BF16>(a, st);
 	if (src == SRC_F32IN && outk == OUT_BF16) return launch_conv<SRC_F32IN, OUT_BF16>(a, st);
 	if (src == SRC_UPCAT && outk == OUT_BF16) return launch_conv<SRC_UPCAT, OUT_BF16>(a, st);
