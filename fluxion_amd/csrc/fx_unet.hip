@@ -115,13 +115,20 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 	const int m = lane & 15, g = lane >> 4;
 	const int orow = row0 + wv;
 	const bool rowok = orow < a.H;
+	// Output value of an accumulator: + bias, ReLU, and 0 for channels past cout, as one clamp per
+	// value, v_med3(acc + bias, lo, hi) with per-lane bounds: [0, ∞) under ReLU, (−∞, ∞) without, and
+	// [0, 0] past cout.  The outputs are cout_p = 16·NT channels (CP) in every form.
+	constexpr int CP = 16 * NT;
+	float badd[NT], lo[NT], hi[NT];
+#pragma unroll
+	for (int t = 0; t < NT; ++t) {
+		const bool keep = 16 * t + m < a.cout;
+		badd[t] = keep && a.bias ? bv[t] : 0.f;
+		lo[t] = keep ? (a.relu ? 0.f : -__builtin_inff()) : 0.f;
+		hi[t] = keep ? __builtin_inff() : 0.f;
+	}
 	auto value = [&](int mt, int t, int j) {
-		const int co = 16 * t + m;
-		float v = acc[mt][t][j];
-		if (co < a.cout && a.bias) v += bv[t];
-		if (co >= a.cout) v = 0.f;
-		if (a.relu && v < 0.f) v = 0.f;
-		return v;
+		return __builtin_amdgcn_fmed3f(acc[mt][t][j] + badd[t], lo[t], hi[t]);
 	};
 	if constexpr (OUTK == OUT_F32) {
 		// Training's data gradients: FP32 stores straight from the fragments (16 lanes write 64
@@ -134,7 +141,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 				for (int j = 0; j < 4; ++j) {
 					const int col = col0 + 16 * mt + 4 * g + j;
 					if (rowok && col < a.W)
-						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * a.cout_p + 16 * t + m] = value(mt, t, j);
+						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * CP + 16 * t + m] = value(mt, t, j);
 				}
 	} else {
 		// The wave's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS
@@ -142,30 +149,30 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 		// per lane.
 		__syncthreads();
 		if constexpr (OUTK == OUT_BF16) {
-			bf16* st = tile + (size_t)wv * CTW * a.cout_p;
+			bf16* st = tile + (size_t)wv * CTW * CP;
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
 				for (int t = 0; t < NT; ++t)
 #pragma unroll
-					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * a.cout_p + 16 * t + m] = f2bf(value(mt, t, j));
+					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * CP + 16 * t + m] = f2bf(value(mt, t, j));
 			__syncthreads();
 			const int nvalid = min(CTW, a.W - col0);
-			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * a.cout_p;
-			const int nchunk = rowok ? nvalid * a.cout_p / 8 : 0;
+			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * CP;
+			const int nchunk = rowok ? nvalid * CP / 8 : 0;
 			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
 			if (a.pool && (wv & 1) == 0 && rowok) {
 				// The encoder's 2×2 average pooling of rows orow and orow + 1 (this wave's and the next
 				// one's, both in LDS): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] +
 				// x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in BF16 storage).
-				const bf16* s1 = st + CTW * a.cout_p;
-				const int nck = a.cout_p >> 3, npc = nvalid >> 1;
-				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * a.cout_p;
+				const bf16* s1 = st + CTW * CP;
+				const int nck = CP >> 3, npc = nvalid >> 1;
+				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * CP;
 				for (int q = lane; q < npc * nck; q += 64) {
 					const int pc = q / nck, ch = (q - pc * nck) << 3;
-					const int e0 = 2 * pc * a.cout_p + ch;
-					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + a.cout_p);
-					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + a.cout_p);
+					const int e0 = 2 * pc * CP + ch;
+					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + CP);
+					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + CP);
 					u4v o;
 #pragma unroll
 					for (int e = 0; e < 4; ++e) {
@@ -179,7 +186,7 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 						}
 						o[e] = r2;
 					}
-					*(u4v*)(pdst + (size_t)pc * a.cout_p + ch) = o;
+					*(u4v*)(pdst + (size_t)pc * CP + ch) = o;
 				}
 			}
 		} else {
