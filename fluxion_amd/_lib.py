@@ -179,7 +179,7 @@ _SIGS = {
                                     _P(C.c_void_p)]),
     "fx_learned_destroy": (None, [C.c_void_p]),
     "fx_learned_fell_back": (C.c_int32, [C.c_void_p]),
-    "fx_learned_set_schedule": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fx_learned_set_schedule": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "fx_cn_create": (C.c_int, [c_be_p, _P(FxGrid), C.c_double, C.c_double, _P(FxBoundaries), C.c_double,
                                C.c_uint64, _P(FxCnOptions), _P(C.c_void_p)]),
     "fx_cn_destroy": (None, [C.c_void_p]),

@@ -56,7 +56,8 @@ struct SState {
 	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once the
 	// FP64 true residual ‖b − A·x‖² (ltres, evaluated after every x/r update while the network is in
 	// use) fails to drop below stall2·(its previous value).  lgate = 1 skips the network at this
-	// application (schedule: every `every`-th iteration, at most `max_apps` times; lapps counts them).
+	// application (schedule: from iteration `first`, every `every`-th, at most `max_apps` times; lapps
+	// counts them).
 	int32_t lmode, lplain, lskip, lgate;
 	double stall2;
 	double ltres;
@@ -760,9 +761,10 @@ struct FinLearnJudge {
 // Whether the network runs at this preconditioner application (lgate = 0) or only the V-cycle.
 struct FinLearnGate {
 	SState* st;
-	int every, max_apps;
+	int every, max_apps, first;
 	__device__ void operator()() const {
-		const bool on = !st->done && !st->lplain && st->iter % every == 0 && (max_apps <= 0 || st->lapps < max_apps);
+		const bool on = !st->done && !st->lplain && st->iter >= first && (st->iter - first) % every == 0 &&
+		                (max_apps <= 0 || st->lapps < max_apps);
 		st->lgate = on ? 0 : 1;
 		if (on) st->lapps += 1;
 	}
@@ -1505,7 +1507,12 @@ struct fx_learned {
 	double* ac = nullptr;      // A·c
 	float* xin = nullptr;
 	double* tr = nullptr;      // b − A·x of the fallback test
-	int every = 1, max_apps = 0;
+	// Network schedule.  first = 2 by default: on the first two applications the residual (the
+	// right-hand side, then one V-cycle-preconditioned step from it) is unlike the iterates' residuals
+	// the network was trained on; corrections there raised ‖r‖ up to 23-fold and cost flexible CG 9–14
+	// iterations on the log-normal-k family at 1024² and 4096², while from iteration 2 on the solve keeps
+	// the V-cycle's iteration count.
+	int every = 1, max_apps = 0, first = 2;
 	int fell_back = 0;
 };
 
@@ -1613,7 +1620,7 @@ struct Ctx {
 		(void)lskip;
 		FX_CHECK(eupd(FZero{z}, "pc zero", sk));
 		FX_CHECK(fx::mg_apply(L->mg, z, r, sk));
-		FX_CHECK(scalar(FinLearnGate{d, L->every, L->max_apps}, "learned gate", nullptr));
+		FX_CHECK(scalar(FinLearnGate{d, L->every, L->max_apps, L->first}, "learned gate", nullptr));
 		const int* lsk = &d->lgate;
 		FX_CHECK(apply_op(z, L->r1, lsk));
 		FX_CHECK(ered<1>(FLr1{r, L->r1}, FinScale{L->s_dev, (double)n}, "learned r1", lsk));
@@ -2893,11 +2900,13 @@ void fx_learned_destroy(fx_learned* L) {
 	delete L;
 }
 
-fx_status fx_learned_set_schedule(fx_learned* L, int32_t every, int32_t max_apps) {
+fx_status fx_learned_set_schedule(fx_learned* L, int32_t every, int32_t max_apps, int32_t first) {
 	FX_REQUIRE(L, FX_ERR_ARGUMENT, "learned preconditioner is null");
-	FX_REQUIRE(every >= 1 && max_apps >= 0, FX_ERR_ARGUMENT, "every must be >= 1 and max_apps >= 0");
+	FX_REQUIRE(every >= 1 && max_apps >= 0 && first >= 0, FX_ERR_ARGUMENT,
+	           "every must be >= 1, max_apps >= 0 and first >= 0");
 	L->every = every;
 	L->max_apps = max_apps;
+	L->first = first;
 	return FX_OK;
 }
 

@@ -349,9 +349,10 @@ void fx_learned_destroy(fx_learned* l);
 /* 1 if a solve since the previous call switched to the plain V-cycle (and clears the flag). */
 int32_t fx_learned_fell_back(fx_learned* l);
 /* Schedule of the network inside a solve: applied at the preconditioner applications made after
- * 0, every, 2·every, … iterations, at most max_apps times per solve pass (0: no limit); the other
- * applications are the V-cycle alone.  Default every = 1, max_apps = 0. */
-fx_status fx_learned_set_schedule(fx_learned* l, int32_t every, int32_t max_apps);
+ * first, first + every, first + 2·every, … iterations, at most max_apps times per solve pass (0: no
+ * limit); the other applications are the V-cycle alone.  Default every = 1, max_apps = 0, first = 2
+ * (the applications on the right-hand side and after the first iteration are the V-cycle's). */
+fx_status fx_learned_set_schedule(fx_learned* l, int32_t every, int32_t max_apps, int32_t first);
 
 /* Extension: A·x for an operator (same arithmetic as the solvers use). */
 fx_status fx_apply_operator(fx_backend* be, const fx_operator* op, const fx_buf* x, fx_buf* y,

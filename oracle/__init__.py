@@ -537,11 +537,11 @@ def unet_loss_grad(params, x, t, energy=False, bf16=False):
 
 
 # ── learned preconditioner inside flexible CG (composition of the pieces above) ──
-def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9, every=1, max_apps=0):
+def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9, every=1, max_apps=0, first=2):
     """Flexible CG (Notay's β = −z·Ap_old / p_old·Ap_old) with the learned preconditioner
     z = V(r) + α*·c, c = s·h²·N([r1/s, log k]) on interior points, r1 = r − A·V(r), s = rms(r1),
-    α* = (c·r1)/(c·A·c) clipped at 0, the network applied after 0, every, 2·every, … iterations (at most
-    max_apps times, 0 = no limit) until the FP64 true residual ‖b − A·x‖ fails to drop by `stall` in
+    α* = (c·r1)/(c·A·c) clipped at 0, the network applied after first, first + every, … iterations (at
+    most max_apps times, 0 = no limit) until the FP64 true residual ‖b − A·x‖ fails to drop by `stall` in
     one iteration.  The operation order of the device solver; network in BF16 storage mode.
     Returns (x, iterations, fell_back)."""
     n = grid.rows * grid.cols
@@ -554,7 +554,8 @@ def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9
 
     def pc(r):
         z = vcycle(np.zeros(n), r, grid, mg)
-        on = (not st["plain"]) and st["iter"] % every == 0 and (max_apps <= 0 or st["apps"] < max_apps)
+        on = ((not st["plain"]) and st["iter"] >= first and (st["iter"] - first) % every == 0
+              and (max_apps <= 0 or st["apps"] < max_apps))
         if not on:
             return z
         st["apps"] += 1

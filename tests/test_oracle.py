@@ -734,6 +734,21 @@ def test_learned_fcg_zero_network_is_fcg():
     assert rel_l2(xl, xf) < 1e-8
 
 
+def test_learned_fcg_schedule_first():
+    # The network runs from iteration `first` on: with `first` beyond the iteration count it never runs,
+    # and the trained-network solve is then flexible CG around the V-cycle step for step; with first = 0
+    # it also corrects the preconditioner application made on the right-hand side itself.
+    g, k, f, op, mg = _learned_problem()
+    tol = 1e-10 * np.linalg.norm(f)
+    params = np.random.default_rng(5).standard_normal(O.unet_nparams(2)) * 0.05
+    xn, itn, _ = O.learned_fcg(f, g, op, mg, params, np.log(k), tol, 200, first=1000)
+    xf, sf = O.fcg(np.zeros_like(f), f, g, op, tol, 200, mg=mg)
+    assert abs(itn - sf.iterations) <= 1 and rel_l2(xn, xf) < 1e-8
+    x0, it0, _ = O.learned_fcg(f, g, op, mg, params, np.log(k), tol, 200, first=0, stall=1e9)
+    x2, it2, _ = O.learned_fcg(f, g, op, mg, params, np.log(k), tol, 200, first=2, stall=1e9)
+    assert rel_l2(x0, x2) > 0 and it0 < 200 and it2 < 200
+
+
 def test_learned_fcg_random_network_still_converges():
     # α* = (c·r1)/(c·A·c), clipped at 0, minimises the A-norm of the error left after the V-cycle along c,
     # so each application is at least as good as the V-cycle alone, whatever the network.  With an
