@@ -295,8 +295,7 @@ def main():
                    "true_rel_residual": s_l.true_residual / fnorm, "fell_back": fb,
                    "speedup_vs_plain": t_plain / t_l, "speedup_vs_mixed": t_mixed / t_l,
                    "weights": Path(wpath).name,
-                   "note": "from its default first application (iteration 2) the network keeps the V-cycle's "
-                           "iteration count and adds its forward and the true-residual checks; the headline "
+                   "note": "the network correction costs more iterations and time than it saves; the headline "
                            "value is the mixed path"}
         # The network alone: one forward at this size.
         unet_rec = unet_forward_record(be, net, n)

@@ -53,11 +53,11 @@ struct SState {
 	int32_t pad2;
 	double gbeta, gcur, ghk;
 	double *H, *cc, *ss, *grs, *y;
-	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once the
-	// FP64 true residual ‖b − A·x‖² (ltres, evaluated after every x/r update while the network is in
-	// use) fails to drop below stall2·(its previous value).  lgate = 1 skips the network at this
-	// application (schedule: from iteration `first`, every `every`-th, at most `max_apps` times; lapps
-	// counts them).
+	// Learned preconditioner: lskip = done | lplain; lplain switches to the plain V-cycle once an
+	// iteration whose direction came from a network application fails to cut the FP64 true residual
+	// ‖b − A·x‖² (ltres, evaluated after every x/r update while the network is in use) below
+	// stall2·(its value before that iteration).  lgate = 1 skips the network at this application
+	// (schedule: from iteration `first`, every `every`-th, at most `max_apps` times; lapps counts them).
 	int32_t lmode, lplain, lskip, lgate;
 	double stall2;
 	double ltres;
@@ -746,12 +746,15 @@ struct FinPcgInitRR {
 		st->iter = 0;
 	}
 };
-// Learned preconditioner's fallback test on the FP64 true residual ‖b − A·x‖² after an x/r update.
+// Learned preconditioner's fallback test on the FP64 true residual ‖b − A·x‖² after an x/r update:
+// the iteration just taken followed the preconditioner application that lgate describes, and only
+// an iteration after a network application (lgate = 0) is judged (the V-cycle's own iterations need
+// not reduce ‖b − A·x‖ monotonically, and often raise it in the first step).
 struct FinLearnJudge {
 	SState* st;
 	__device__ void operator()(Acc* o) const {
 		const double tr = acc_value(o[0]);
-		if (!(tr < st->stall2 * st->ltres)) {
+		if (st->lgate == 0 && !(tr < st->stall2 * st->ltres)) {
 			st->lplain = 1;
 			st->lskip = 1;
 		}
@@ -1507,12 +1510,8 @@ struct fx_learned {
 	double* ac = nullptr;      // A·c
 	float* xin = nullptr;
 	double* tr = nullptr;      // b − A·x of the fallback test
-	// Network schedule.  first = 2 by default: on the first two applications the residual (the
-	// right-hand side, then one V-cycle-preconditioned step from it) is unlike the iterates' residuals
-	// the network was trained on; corrections there raised ‖r‖ up to 23-fold and cost flexible CG 9–14
-	// iterations on the log-normal-k family at 1024² and 4096², while from iteration 2 on the solve keeps
-	// the V-cycle's iteration count.
-	int every = 1, max_apps = 0, first = 2;
+	// Network schedule: from iteration `first` (default 0), every `every`-th, at most `max_apps` times.
+	int every = 1, max_apps = 0, first = 0;
 	int fell_back = 0;
 };
 

@@ -133,7 +133,7 @@ class LearnedPreconditioner:
     def fell_back(self) -> bool:
         return bool(self._lib.fx_learned_fell_back(self.h))
 
-    def set_schedule(self, every: int = 1, max_apps: int = 0, first: int = 2) -> None:
+    def set_schedule(self, every: int = 1, max_apps: int = 0, first: int = 0) -> None:
         """Network from iteration `first` on, at every `every`-th iteration, at most `max_apps` times
         per solve (0: no limit)."""
         L.check(self._lib.fx_learned_set_schedule(self.h, every, max_apps, first), "learned_set_schedule")

@@ -340,8 +340,8 @@ fx_status fx_unet_adam_step(fx_unet* u, float lr, float beta1, float beta2, floa
  * s = rms(r1), V the multigrid V-cycle, N the network (h² = dx·dy, logk may be NULL for k ≡ 1) and
  * α* = (c·r1)/(c·A·c) clipped at 0, the step that minimises the A-norm of the remaining error.  Used by
  * flexible CG.  While the network is in use the solve recomputes the FP64 true residual ‖b − A·x‖ after
- * every iteration, and falls back to z = V(r) for the rest of the solve once it fails to drop by the
- * factor `stall`. */
+ * every iteration, and falls back to z = V(r) for the rest of the solve once an iteration whose
+ * direction came from a network application fails to cut it by the factor `stall`. */
 typedef struct fx_learned fx_learned;
 fx_status fx_learned_create(fx_backend* be, fx_unet* net, fx_multigrid* mg, const fx_buf* logk, const fx_grid* grid,
                             double stall, fx_learned** out);
@@ -350,8 +350,7 @@ void fx_learned_destroy(fx_learned* l);
 int32_t fx_learned_fell_back(fx_learned* l);
 /* Schedule of the network inside a solve: applied at the preconditioner applications made after
  * first, first + every, first + 2·every, … iterations, at most max_apps times per solve pass (0: no
- * limit); the other applications are the V-cycle alone.  Default every = 1, max_apps = 0, first = 2
- * (the applications on the right-hand side and after the first iteration are the V-cycle's). */
+ * limit); the other applications are the V-cycle alone.  Default every = 1, max_apps = 0, first = 0. */
 fx_status fx_learned_set_schedule(fx_learned* l, int32_t every, int32_t max_apps, int32_t first);
 
 /* Extension: A·x for an operator (same arithmetic as the solvers use). */

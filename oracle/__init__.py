@@ -537,12 +537,12 @@ def unet_loss_grad(params, x, t, energy=False, bf16=False):
 
 
 # ── learned preconditioner inside flexible CG (composition of the pieces above) ──
-def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9, every=1, max_apps=0, first=2):
+def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9, every=1, max_apps=0, first=0):
     """Flexible CG (Notay's β = −z·Ap_old / p_old·Ap_old) with the learned preconditioner
     z = V(r) + α*·c, c = s·h²·N([r1/s, log k]) on interior points, r1 = r − A·V(r), s = rms(r1),
     α* = (c·r1)/(c·A·c) clipped at 0, the network applied after first, first + every, … iterations (at
-    most max_apps times, 0 = no limit) until the FP64 true residual ‖b − A·x‖ fails to drop by `stall` in
-    one iteration.  The operation order of the device solver; network in BF16 storage mode.
+    most max_apps times, 0 = no limit) until an iteration whose direction came from a network application
+    fails to cut the FP64 true residual ‖b − A·x‖ by the factor `stall`.  The operation order of the device solver; network in BF16 storage mode.
     Returns (x, iterations, fell_back)."""
     n = grid.rows * grid.cols
     h2 = grid.dx * grid.dy
@@ -550,12 +550,13 @@ def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9
     inner[1:-1, 1:-1] = True
     inner = inner.ravel()
     A = lambda v: apply_op(op, v, grid)
-    st = {"iter": 0, "plain": False, "apps": 0}
+    st = {"iter": 0, "plain": False, "apps": 0, "on": False}
 
     def pc(r):
         z = vcycle(np.zeros(n), r, grid, mg)
         on = ((not st["plain"]) and st["iter"] >= first and (st["iter"] - first) % every == 0
               and (max_apps <= 0 or st["apps"] < max_apps))
+        st["on"] = on
         if not on:
             return z
         st["apps"] += 1
@@ -595,7 +596,7 @@ def learned_fcg(f, grid, op: Op, mg: OMg, params, logk, tol, max_iter, stall=0.9
         if not st["plain"]:
             tr = f - A(x)
             t2 = float(tr @ tr)
-            if not t2 < stall * stall * ltres:
+            if st["on"] and not t2 < stall * stall * ltres:
                 st["plain"] = True
             ltres = t2
         z = pc(r)
