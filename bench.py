@@ -120,6 +120,34 @@ def unet_forward_record(be, net, n: int, reps: int = 10) -> dict:
             "frac_bf16_dense_peak": flop / t_f / BF16_PEAK_FLOPS}
 
 
+def unet_train_record(be, n: int = 256, batch: int = 8, reps: int = 10) -> dict:
+    """One training step as learned.train_vark takes it -- forward, relative energy-norm loss, backward
+    through all 15 layers (data and weight gradients), Adam update -- on a batch of n×n two-channel
+    inputs resident on the device, committed weights as the start; timed over `reps` steps after two
+    warm-ups (each step copies its loss to the host, as training does).  FLOP: the forward's 61,920 per
+    pixel and the same again for each of the two backward products (data and weight gradients)."""
+    from fluxion_amd.learned import DEFAULT_WEIGHTS, UNet, load_weights
+    net = UNet(be, 2)
+    net.set_params(load_weights(DEFAULT_WEIGHTS))
+    rng = np.random.default_rng(7)
+    XB = be.upload(rng.standard_normal(batch * n * n * 2).astype(np.float32))
+    TB = be.upload(rng.standard_normal(batch * n * n).astype(np.float32))
+    for _ in range(2):
+        net.loss_grad_device(XB, TB, batch, n, n, energy=True)
+        net.adam_step(1e-4)
+    be.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.loss_grad_device(XB, TB, batch, n, n, energy=True)
+        net.adam_step(1e-4)
+    be.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    flop = 3 * 61920.0 * batch * n * n
+    del XB, TB, net
+    return {"grid": n, "batch": batch, "ms_per_step": 1e3 * t, "pixels_per_s": batch * n * n / t,
+            "TFLOPs_approx": flop / t / 1e12, "frac_bf16_dense_peak_approx": flop / t / BF16_PEAK_FLOPS}
+
+
 # Which attainable ceiling each probed kernel is read against: the step kernel has the byte mix of the
 # 52 B/pt streaming kernel; the level-0 V-cycle kernels read 28-29 B/pt and write 4-9 B/pt, so the
 # read-only ceiling bounds them from above.
@@ -197,7 +225,10 @@ def main():
         net.set_params(load_weights(args.learned or DEFAULT_WEIGHTS))
         rec = unet_forward_record(be, net, args.size)
         log(f"bench: U-Net forward {args.size}^2 {rec['ms_per_forward']:.2f} ms, {rec['TFLOPs']:.0f} TFLOP/s")
-        print(json.dumps({"unet_forward": rec}), flush=True)
+        trec = unet_train_record(be)
+        log(f"bench: U-Net training step 8x256^2 {trec['ms_per_step']:.2f} ms "
+            f"({trec['pixels_per_s'] / 1e6:.0f} Mpixel/s, ~{trec['TFLOPs_approx']:.0f} TFLOP/s)")
+        print(json.dumps({"unet_forward": rec, "unet_train_step": trec}), flush=True)
         return
 
     if args.copy_calibration:
@@ -278,7 +309,7 @@ def main():
         f"true rel res {s_mixed.true_residual / fnorm:.2e}")
 
     # ── learned + mixed: the same FCG with the U-Net correction (committed weights) ──
-    learned, unet_rec, net, LK = None, None, None, None
+    learned, unet_rec, unet_train, net, LK = None, None, None, None, None
     t_value, s_value, label = t_mixed, s_mixed, "mixed"
     if not args.no_learned:
         from fluxion_amd.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, UNet, load_weights
@@ -303,6 +334,10 @@ def main():
         unet_rec["mixed_fcg_iterations_per_forward"] = t_f / (t_mixed / max(1, s_mixed.iterations))
         log(f"bench: U-Net forward {n}^2 {1e3 * t_f:.2f} ms, {unet_rec['TFLOPs']:.0f} TFLOP/s "
             f"({100 * unet_rec['frac_bf16_dense_peak']:.1f}% of BF16 dense peak)")
+        # One training step of the network at its training configuration (8 × 256², energy-norm loss).
+        unet_train = unet_train_record(be)
+        log(f"bench: U-Net training step 8x256^2 {unet_train['ms_per_step']:.2f} ms "
+            f"({unet_train['pixels_per_s'] / 1e6:.0f} Mpixel/s, ~{unet_train['TFLOPs_approx']:.0f} TFLOP/s)")
 
     # Per-kernel probes of the mixed solve (the x/r update was probed over the timed region).
     kb = kernel_bytes(True, mgk == MgKernels.WAVE and n % 2 == 0)
@@ -719,6 +754,7 @@ def main():
         "cn_vark_1024": cn,
         "explicit_diffusion_4096": explicit,
         "unet_forward": unet_rec,
+        "unet_train_step": unet_train,
         "kernels": kernels,
         "hbm_ceilings": ceilings,
         "stencil_axpy_GBs": st_gbs,

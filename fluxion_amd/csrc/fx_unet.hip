@@ -731,72 +731,103 @@ fx_status launch_conv(const ConvArgs& a, hipStream_t st) {
 }
 
 // dW[tap][co][ci] += Σ_p g[p][co]·x[p+off][ci], g = dy masked by the ReLU of y.
-// One wave: a 16(co) × 16(ci) tile of one tap over `rpw` image rows (rows counted over the batch),
+// One wave: a 16(co) × 16(ci) tile of all nine taps over `rpw` image rows (rows counted over the batch),
 // 32 pixels of a row per v_mfma_f32_16x16x32_bf16 (K = pixels): lane (m, q) feeds pixels 8q..8q+7 of
-// the chunk, g rounded to BF16 as the A row co = m and x as the B column ci = m.
+// the chunk, g rounded to BF16 as the A row co = m (loaded once for the nine taps) and x as the B
+// column ci = m.  For each of the three input rows the lane loads the 10 pixels its three column
+// shifts read, so a chunk costs 8 + 8 + 30 loads for 9 MFMAs.
 __global__ __launch_bounds__(BLOCK) void k_conv3_wgrad(const float* dy, const bf16* y, int relu, const bf16* x,
                                                        float* dW, int B, int H, int W, int cin_p, int cout_p,
                                                        int cin, int cout, int rpw) {
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-	const int tap = blockIdx.y, tile = blockIdx.z;
+	const int tile = blockIdx.z;
 	const int nco = cout_p / 16;
 	const int cot = tile % nco, cit = tile / nco;
 	const int m = lane & 15, q = lane >> 4;
-	const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
 	const int co = cot * 16 + m, ci = cit * 16 + m;
 	const long rows = (long)B * H;
 	const long rw0 = ((long)blockIdx.x * NWAVES + wv) * rpw;
-	f4 acc = {0.f, 0.f, 0.f, 0.f};
+	f4 acc[9];
+#pragma unroll
+	for (int t = 0; t < 9; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
 	for (long rw = rw0; rw < rw0 + rpw && rw < rows; ++rw) {
 		const long b = rw / H;
 		const int r = (int)(rw - b * H);
-		const int rs = r + dyo;
-		const bool row_ok = rs >= 0 && rs < H;
-		const size_t prow = (size_t)rw * W;                  // first pixel of this output row
-		const size_t srow = ((size_t)b * H + (row_ok ? rs : 0)) * W;   // first pixel of the shifted input row
+		const size_t prow = (size_t)rw * W;   // first pixel of this output row
 		for (int cb = 0; cb < W; cb += 32) {
-			uint16_t ga[8], xb[8];
-#pragma unroll
-			for (int j = 0; j < 8; ++j) {
-				const int c = cb + 8 * q + j;
-				float gv = 0.f;
-				uint16_t xv = 0;
-				if (c < W) {
-					gv = dy[(prow + c) * cout_p + co];
-					if (relu && !(bf2f(y[(prow + c) * cout_p + co]) > 0.f)) gv = 0.f;
-					const int cs = c + dxo;
-					if (row_ok && cs >= 0 && cs < W) xv = x[(srow + cs) * cin_p + ci];
-				}
-				ga[j] = f2bf(gv);
-				xb[j] = xv;
-			}
-			u4v av, bv;
+			const int c0 = cb + 8 * q;
+			u4v av;
 #pragma unroll
 			for (int e = 0; e < 4; ++e) {
-				av[e] = (uint32_t)ga[2 * e] | ((uint32_t)ga[2 * e + 1] << 16);
-				bv[e] = (uint32_t)xb[2 * e] | ((uint32_t)xb[2 * e + 1] << 16);
+				uint32_t pk = 0;
+#pragma unroll
+				for (int h = 0; h < 2; ++h) {
+					const int c = c0 + 2 * e + h;
+					float gv = 0.f;
+					if (c < W) {
+						gv = dy[(prow + c) * cout_p + co];
+						if (relu && !(bf2f(y[(prow + c) * cout_p + co]) > 0.f)) gv = 0.f;
+					}
+					pk |= (uint32_t)f2bf(gv) << (16 * h);
+				}
+				av[e] = pk;
 			}
-			acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv), acc, 0,
-			                                                0, 0);
+#pragma unroll
+			for (int dr = 0; dr < 3; ++dr) {
+				const int rs = r + dr - 1;
+				const bool row_ok = rs >= 0 && rs < H;
+				const size_t srow = ((size_t)b * H + (row_ok ? rs : 0)) * W;
+				uint16_t xv[10];   // pixels c0 − 1 .. c0 + 8 of input row rs, channel ci
+#pragma unroll
+				for (int j = 0; j < 10; ++j) {
+					const int cs = c0 - 1 + j;
+					xv[j] = row_ok && cs >= 0 && cs < W ? x[(srow + cs) * cin_p + ci] : (uint16_t)0;
+				}
+#pragma unroll
+				for (int dc = 0; dc < 3; ++dc) {
+					u4v bv;
+#pragma unroll
+					for (int e = 0; e < 4; ++e) {
+						// output pixels c0 + 2e, c0 + 2e + 1 read input columns + dc − 1; past the row's end
+						// the g values are zero, whatever x holds
+						bv[e] = (uint32_t)xv[2 * e + dc] | ((uint32_t)xv[2 * e + 1 + dc] << 16);
+					}
+					acc[3 * dr + dc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+					    __builtin_bit_cast(bf8, av), __builtin_bit_cast(bf8, bv), acc[3 * dr + dc], 0, 0, 0);
+				}
+			}
 		}
 	}
-	for (int j = 0; j < 4; ++j) {
-		const int o = cot * 16 + 4 * q + j;
-		if (o < cout && ci < cin) atomicAdd(&dW[((size_t)tap * cout + o) * cin + ci], acc[j]);
-	}
+#pragma unroll
+	for (int t = 0; t < 9; ++t)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const int o = cot * 16 + 4 * q + j;
+			if (o < cout && ci < cin) atomicAdd(&dW[((size_t)t * cout + o) * cin + ci], acc[t][j]);
+		}
 }
 
-// db[co] += Σ_p g[p][co].
-__global__ void k_bgrad(const float* dy, const bf16* y, int relu, float* db, long total, int cout_p, int cout) {
-	const int co = threadIdx.x & 63;
-	if (co >= cout) return;
+// db[co] += Σ_p g[p][co]: a block's threads cover `slots` pixels × cout_p channels of consecutive memory
+// per step (coalesced), sum their pixels, and merge the slots through LDS into one atomic per channel.
+__global__ __launch_bounds__(BLOCK) void k_bgrad(const float* dy, const bf16* y, int relu, float* db, long total,
+                                                 int cout_p, int cout) {
+	const int slots = BLOCK / cout_p;
+	const int c = threadIdx.x % cout_p, slot = threadIdx.x / cout_p;
 	float s = 0.f;
-	for (long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6); p < total; p += (long)gridDim.x * 4) {
-		float g = dy[p * cout_p + co];
-		if (relu && !(bf2f(y[p * cout_p + co]) > 0.f)) g = 0.f;
-		s += g;
+	if (slot < slots)
+		for (long p = (long)blockIdx.x * slots + slot; p < total; p += (long)gridDim.x * slots) {
+			float g = dy[p * cout_p + c];
+			if (relu && !(bf2f(y[p * cout_p + c]) > 0.f)) g = 0.f;
+			s += g;
+		}
+	__shared__ float sh[BLOCK];
+	sh[threadIdx.x] = slot < slots ? s : 0.f;
+	__syncthreads();
+	if (threadIdx.x < cout_p && threadIdx.x < cout) {
+		float t = 0.f;
+		for (int k = 0; k < slots; ++k) t += sh[k * cout_p + threadIdx.x];
+		atomicAdd(&db[threadIdx.x], t);
 	}
-	atomicAdd(&db[co], s);
 }
 
 // dx (f32, [B][H][W][C]) += 0.25·dy[B][H/2][W/2][C]
@@ -1159,10 +1190,10 @@ fx_status backward(fx_unet* u) {
 		int Hl = u->H >> l, Wl = u->W >> l;
 		long total = (long)u->B * Hl * Wl;
 		int cip = pad16(u->lcin[li]), cop = pad16(u->lcout[li]);
-		// Rows per wave: about 2048 pixels of work per wave, at least one row.
+		// Rows per wave: about 256 pixels (all nine taps) of work per wave, at least one row.
 		const long rows = (long)u->B * Hl;
-		const int rpw = (int)std::max<long>(1, 2048 / Wl);
-		dim3 grid((unsigned)((rows + (long)NWAVES * rpw - 1) / ((long)NWAVES * rpw)), 9, (unsigned)((cop / 16) * (cip / 16)));
+		const int rpw = (int)std::max<long>(1, 256 / Wl);
+		dim3 grid((unsigned)((rows + (long)NWAVES * rpw - 1) / ((long)NWAVES * rpw)), 1, (unsigned)((cop / 16) * (cip / 16)));
 		hipLaunchKernelGGL(k_conv3_wgrad, grid, dim3(BLOCK), 0, st, g, yact, relu, xin, u->G + u->woff[li], u->B, Hl, Wl,
 		                   cip, cop, u->lcin[li], u->lcout[li], rpw);
 		hipLaunchKernelGGL(k_bgrad, dim3(256), dim3(256), 0, st, g, yact, relu, u->G + u->boff[li], total, cop,
