@@ -67,17 +67,19 @@ struct ConvArgs {
 
 constexpr int CTW = 64, CTH = 4;   // output tile of a workgroup: CTH rows (one per wave) × CTW columns
 // SRC_UPCAT stages its upsampled operand at the coarse resolution: the coarse pixels under the
-// (CTH + 2) × (CTW + 2) fine input pixels, CUR rows × CUW columns (CTH and CTW even, tiles start on
-// even rows and columns).
+// (rows + 2) × (CTW + 2) fine input pixels of a tile of `rows` rows, rows / 2 + 2 rows × CUW columns
+// (rows and CTW even, tiles start on even rows and columns).
 constexpr int CUW = CTW / 2 + 2, CUR = CTH / 2 + 2, CUPX = CUR * CUW;
-// LDS of a workgroup: the staged input tile, reused afterwards for the BF16 output transpose
-// (CTH waves × CTW pixels × cout_p).  ca > 0: the up+cat tile (coarse operand, ca channels per pixel,
-// then the fine skip, cin_p − ca).  Pixels are stored unpadded with their 16-byte channel chunks
-// swizzled (chunk_sw).
-inline size_t conv_lds_bytes(int cin_p, int cout_p, int ca = 0) {
-	const size_t tpx = (size_t)(CTH + 2) * (CTW + 2);
-	const size_t in = ca > 0 ? ((size_t)CUPX * ca + tpx * (cin_p - ca)) * sizeof(bf16) : tpx * cin_p * sizeof(bf16);
-	const size_t out = (size_t)CTH * CTW * cout_p * sizeof(bf16);
+constexpr int coarse_px(int rows) { return (rows / 2 + 2) * CUW; }
+// LDS of a workgroup whose output tile has `rows` rows: the staged input tile, reused afterwards for
+// the BF16 output transpose (rows × CTW pixels × cout_p).  ca > 0: the up+cat tile (coarse operand,
+// ca channels per pixel, then the fine skip, cin_p − ca).  Pixels are stored unpadded with their
+// 16-byte channel chunks swizzled (chunk_sw).
+inline size_t conv_lds_bytes(int cin_p, int cout_p, int ca = 0, int rows = CTH) {
+	const size_t tpx = (size_t)(rows + 2) * (CTW + 2);
+	const size_t in = ca > 0 ? ((size_t)coarse_px(rows) * ca + tpx * (cin_p - ca)) * sizeof(bf16)
+	                         : tpx * cin_p * sizeof(bf16);
+	const size_t out = (size_t)rows * CTW * cout_p * sizeof(bf16);
 	return in > out ? in : out;
 }
 
@@ -104,17 +106,16 @@ __device__ __forceinline__ void load_bias(const ConvArgs& a, float (&bv)[NT]) {
 	for (int t = 0; t < NT; ++t) bv[t] = a.bias && 16 * t + m < a.cout ? a.bias[16 * t + m] : 0.f;
 }
 
-// Bias, ReLU and the output form of a workgroup's CTH × CTW tile from the MFMA accumulators (wave wv
-// holds row row0 + wv; acc[mt][t][j] is pixel 16·mt + 4·g + j, channel 16·t + m of lane 16·g + m);
-// bv from load_bias.  Every thread of the workgroup calls it (it holds workgroup barriers, and returns
-// nowhere early); the input tile in LDS is reused for the output transpose.
-template <int OUTK, int NT>
-__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT], const float (&bv)[NT], bf16* tile,
-                                              int b, int row0, int col0) {
+// Bias, ReLU and the output form of a workgroup's (CTH·RPW) × CTW tile from the MFMA accumulators
+// (wave wv holds rows row0 + RPW·wv + r, r < RPW; acc[4r + mt][t][j] is row r's pixel 16·mt + 4·g + j,
+// channel 16·t + m of lane 16·g + m); bv from load_bias.  Every thread of the workgroup calls it (it
+// holds workgroup barriers, and returns nowhere early); the input tile in LDS is reused for the output
+// transpose, tile row lr at element lr·CTW·cout_p.
+template <int OUTK, int NT, int RPW = 1>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4 * RPW][NT], const float (&bv)[NT],
+                                              bf16* tile, int b, int row0, int col0) {
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int m = lane & 15, g = lane >> 4;
-	const int orow = row0 + wv;
-	const bool rowok = orow < a.H;
 	// Output value of an accumulator: + bias, ReLU, and 0 for channels past cout, as one clamp per
 	// value, v_med3(acc + bias, lo, hi) with per-lane bounds: [0, ∞) under ReLU, (−∞, ∞) without, and
 	// [0, 0] past cout.  The outputs are cout_p = 16·NT channels (CP) in every form.
@@ -127,78 +128,98 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs& a, f4 (&acc)[4][NT
 		lo[t] = keep ? (a.relu ? 0.f : -__builtin_inff()) : 0.f;
 		hi[t] = keep ? __builtin_inff() : 0.f;
 	}
-	auto value = [&](int mt, int t, int j) {
-		return __builtin_amdgcn_fmed3f(acc[mt][t][j] + badd[t], lo[t], hi[t]);
+	auto value = [&](int i, int t, int j) {
+		return __builtin_amdgcn_fmed3f(acc[i][t][j] + badd[t], lo[t], hi[t]);
 	};
 	if constexpr (OUTK == OUT_F32) {
 		// Training's data gradients: FP32 stores straight from the fragments (16 lanes write 64
 		// contiguous bytes).
 #pragma unroll
-		for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-			for (int t = 0; t < NT; ++t)
-#pragma unroll
-				for (int j = 0; j < 4; ++j) {
-					const int col = col0 + 16 * mt + 4 * g + j;
-					if (rowok && col < a.W)
-						((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * CP + 16 * t + m] = value(mt, t, j);
-				}
-	} else {
-		// The wave's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS
-		// (the input tile is no longer read once every wave has passed the barrier) and store 16 bytes
-		// per lane.
-		__syncthreads();
-		if constexpr (OUTK == OUT_BF16) {
-			bf16* st = tile + (size_t)wv * CTW * CP;
+		for (int r = 0; r < RPW; ++r) {
+			const int orow = row0 + RPW * wv + r;
 #pragma unroll
 			for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
 				for (int t = 0; t < NT; ++t)
 #pragma unroll
-					for (int j = 0; j < 4; ++j) st[(16 * mt + 4 * g + j) * CP + 16 * t + m] = f2bf(value(mt, t, j));
-			__syncthreads();
-			const int nvalid = min(CTW, a.W - col0);
-			bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * CP;
-			const int nchunk = rowok ? nvalid * CP / 8 : 0;
-			for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(st + 8 * q);
-			if (a.pool && (wv & 1) == 0 && rowok) {
-				// The encoder's 2×2 average pooling of rows orow and orow + 1 (this wave's and the next
-				// one's, both in LDS): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] + x[2r+1][2c] +
-				// x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in BF16 storage).
-				const bf16* s1 = st + CTW * CP;
-				const int nck = CP >> 3, npc = nvalid >> 1;
-				bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * CP;
-				for (int q = lane; q < npc * nck; q += 64) {
-					const int pc = q / nck, ch = (q - pc * nck) << 3;
-					const int e0 = 2 * pc * CP + ch;
-					const u4v a0 = *(const u4v*)(st + e0), a1 = *(const u4v*)(st + e0 + CP);
-					const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + CP);
-					u4v o;
-#pragma unroll
-					for (int e = 0; e < 4; ++e) {
-						uint32_t r2 = 0;
-#pragma unroll
-						for (int half = 0; half < 2; ++half) {
-							const int sh = 16 * half;
-							const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) +
-							                         bf2f((bf16)(a2[e] >> sh)) + bf2f((bf16)(a3[e] >> sh)));
-							r2 |= (uint32_t)f2bf(v) << sh;
-						}
-						o[e] = r2;
+					for (int j = 0; j < 4; ++j) {
+						const int col = col0 + 16 * mt + 4 * g + j;
+						if (orow < a.H && col < a.W)
+							((float*)a.out)[(((size_t)b * a.H + orow) * a.W + col) * CP + 16 * t + m] = value(4 * r + mt, t, j);
 					}
-					*(u4v*)(pdst + (size_t)pc * CP + ch) = o;
-				}
-			}
-		} else {
-			float* st = (float*)tile + (size_t)wv * CTW;
-			if (m == 0)
+		}
+	} else {
+		// A row's 64 pixels × cout_p outputs are contiguous in memory: transpose them through LDS (the
+		// input tile is no longer read once every wave has passed the barrier) and store 16 bytes per
+		// lane.
+		__syncthreads();
+		const int nvalid = min(CTW, a.W - col0);
+		if constexpr (OUTK == OUT_BF16) {
+			bf16* st = tile + (size_t)wv * RPW * CTW * CP;
+#pragma unroll
+			for (int r = 0; r < RPW; ++r)
 #pragma unroll
 				for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-					for (int j = 0; j < 4; ++j) st[16 * mt + 4 * g + j] = value(mt, 0, j);
+					for (int t = 0; t < NT; ++t)
+#pragma unroll
+						for (int j = 0; j < 4; ++j)
+							st[(r * CTW + 16 * mt + 4 * g + j) * CP + 16 * t + m] = f2bf(value(4 * r + mt, t, j));
+			__syncthreads();
+#pragma unroll
+			for (int r = 0; r < RPW; ++r) {
+				const int lr = RPW * wv + r, orow = row0 + lr;
+				const bool rowok = orow < a.H;
+				const bf16* s0 = st + (size_t)r * CTW * CP;
+				bf16* dst = (bf16*)a.out + (((size_t)b * a.H + orow) * a.W + col0) * CP;
+				const int nchunk = rowok ? nvalid * CP / 8 : 0;
+				for (int q = lane; q < nchunk; q += 64) *(u4v*)(dst + 8 * q) = *(const u4v*)(s0 + 8 * q);
+				if (a.pool && (lr & 1) == 0 && rowok) {
+					// The encoder's 2×2 average pooling of tile rows lr and lr + 1 (both in LDS; the next
+					// wave's first row when RPW = 1): y[r][c] = bf16(0.25·(x[2r][2c] + x[2r][2c+1] +
+					// x[2r+1][2c] + x[2r+1][2c+1])), summed in FP32 left to right (the oracle's pool2 in
+					// BF16 storage).
+					const bf16* s1 = s0 + CTW * CP;
+					const int nck = CP >> 3, npc = nvalid >> 1;
+					bf16* pdst = a.pool + (((size_t)b * (a.H >> 1) + (orow >> 1)) * (a.W >> 1) + (col0 >> 1)) * CP;
+					for (int q = lane; q < npc * nck; q += 64) {
+						const int pc = q / nck, ch = (q - pc * nck) << 3;
+						const int e0 = 2 * pc * CP + ch;
+						const u4v a0 = *(const u4v*)(s0 + e0), a1 = *(const u4v*)(s0 + e0 + CP);
+						const u4v a2 = *(const u4v*)(s1 + e0), a3 = *(const u4v*)(s1 + e0 + CP);
+						u4v o;
+#pragma unroll
+						for (int e = 0; e < 4; ++e) {
+							uint32_t r2 = 0;
+#pragma unroll
+							for (int half = 0; half < 2; ++half) {
+								const int sh = 16 * half;
+								const float v = 0.25f * (bf2f((bf16)(a0[e] >> sh)) + bf2f((bf16)(a1[e] >> sh)) +
+								                         bf2f((bf16)(a2[e] >> sh)) + bf2f((bf16)(a3[e] >> sh)));
+								r2 |= (uint32_t)f2bf(v) << sh;
+							}
+							o[e] = r2;
+						}
+						*(u4v*)(pdst + (size_t)pc * CP + ch) = o;
+					}
+				}
+			}
+		} else {
+			float* st = (float*)tile + (size_t)wv * RPW * CTW;
+			if (m == 0)
+#pragma unroll
+				for (int r = 0; r < RPW; ++r)
+#pragma unroll
+					for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+						for (int j = 0; j < 4; ++j) st[r * CTW + 16 * mt + 4 * g + j] = value(4 * r + mt, 0, j);
 			__syncthreads();
 			const int col = col0 + lane;
-			if (rowok && col < a.W) ((float*)a.out)[((size_t)b * a.H + orow) * a.W + col] = st[lane];
+#pragma unroll
+			for (int r = 0; r < RPW; ++r) {
+				const int orow = row0 + RPW * wv + r;
+				if (orow < a.H && col < a.W) ((float*)a.out)[((size_t)b * a.H + orow) * a.W + col] = st[r * CTW + lane];
+			}
 		}
 	}
 }
@@ -302,9 +323,10 @@ __device__ __forceinline__ void conv_kloop(const ConvArgs& a, const bf16* tile, 
 // so a lane's tap, chunk and LDS address are a compare and selects between constants.  COARSE reads
 // the up+cat tile's coarse operand (pixel under the fine pixel, rows and columns halved), otherwise a
 // fine tile at element BASE; CH0 is the segment's first channel in the weights.  Weights are fetched
-// two steps ahead.
-template <int NT, int CINP, int NCH, bool COARSE, int BASE, int CH0>
-__device__ __forceinline__ void conv_kseg(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
+// two steps ahead.  A wave computes RPW output rows (tile rows RPW·wv + r), each weight fragment
+// feeding all of them.
+template <int NT, int CINP, int NCH, bool COARSE, int BASE, int CH0, int RPW>
+__device__ __forceinline__ void conv_kseg(const ConvArgs& a, const bf16* tile, f4 (&acc)[4 * RPW][NT]) {
 	constexpr int TW = CTW + 2, PITCH = 8 * NCH, NSTEP = (9 * NCH + 3) / 4, PD = 2;
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	const int m = lane & 15, g = lane >> 4;
@@ -336,66 +358,77 @@ __device__ __forceinline__ void conv_kseg(const ConvArgs& a, const bf16* tile, f
 		const bool ok = tap < 9;   // false only past the last tap, in the last step
 		const int t9 = ok ? tap : 0;
 		const int dr = t9 / 3, dc = t9 - 3 * dr;
-		int addr, mstride;
-		if constexpr (COARSE) {
-			const int Q = ((wv + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1);
-			addr = BASE + Q * PITCH + ((c8 ^ chunk_sw(Q, NCH)) << 3);
-			mstride = 8 * PITCH;
-		} else {
-			const int P = (wv + dr) * TW + m + dc;
-			addr = BASE + P * PITCH + ((c8 ^ chunk_sw(P, NCH)) << 3);
-			mstride = 16 * PITCH;
+		u4v x[4 * RPW];
+#pragma unroll
+		for (int r = 0; r < RPW; ++r) {
+			const int lr = RPW * wv + r;
+			int addr, mstride;
+			if constexpr (COARSE) {
+				const int Q = ((lr + dr + 1) >> 1) * CUW + ((m + dc + 1) >> 1);
+				addr = BASE + Q * PITCH + ((c8 ^ chunk_sw(Q, NCH)) << 3);
+				mstride = 8 * PITCH;
+			} else {
+				const int P = (lr + dr) * TW + m + dc;
+				addr = BASE + P * PITCH + ((c8 ^ chunk_sw(P, NCH)) << 3);
+				mstride = 16 * PITCH;
+			}
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt) x[4 * r + mt] = *(const u4v*)(tile + addr + mt * mstride);
 		}
-		u4v x[4];
 #pragma unroll
-		for (int mt = 0; mt < 4; ++mt) x[mt] = *(const u4v*)(tile + addr + mt * mstride);
-#pragma unroll
-		for (int mt = 0; mt < 4; ++mt) {
-			const bf8 av = __builtin_bit_cast(bf8, ok ? x[mt] : zero);
+		for (int i = 0; i < 4 * RPW; ++i) {
+			const bf8 av = __builtin_bit_cast(bf8, ok ? x[i] : zero);
 #pragma unroll
 			for (int t = 0; t < NT; ++t)
-				acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, ok ? wb[s][t] : zero),
-				                                                     acc[mt][t], 0, 0, 0);
+				acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, ok ? wb[s][t] : zero),
+				                                                    acc[i][t], 0, 0, 0);
 		}
 	}
 }
 
 // conv_kloop with the channel counts known at compile time (CINP input channels, of which CA
 // upsampled for SRC_UPCAT): the K loop fully unrolled over conv_kseg segments — for up+cat first every
-// tap's coarse chunks, then every tap's skip chunks, so no step mixes the two tiles.
-template <int SRC, int NT, int CINP, int CA>
-__device__ __forceinline__ void conv_kloop_ct(const ConvArgs& a, const bf16* tile, f4 (&acc)[4][NT]) {
+// tap's coarse chunks, then every tap's skip chunks, so no step mixes the two tiles.  The tile has
+// CTH·RPW rows.
+template <int SRC, int NT, int CINP, int CA, int RPW>
+__device__ __forceinline__ void conv_kloop_ct(const ConvArgs& a, const bf16* tile, f4 (&acc)[4 * RPW][NT]) {
 #pragma unroll
-	for (int mt = 0; mt < 4; ++mt)
+	for (int i = 0; i < 4 * RPW; ++i)
 #pragma unroll
-		for (int t = 0; t < NT; ++t) acc[mt][t] = f4{0.f, 0.f, 0.f, 0.f};
+		for (int t = 0; t < NT; ++t) acc[i][t] = f4{0.f, 0.f, 0.f, 0.f};
 	if constexpr (SRC == SRC_UPCAT) {
-		conv_kseg<NT, CINP, CA / 8, true, 0, 0>(a, tile, acc);
-		conv_kseg<NT, CINP, (CINP - CA) / 8, false, CUPX * CA, CA>(a, tile, acc);
+		conv_kseg<NT, CINP, CA / 8, true, 0, 0, RPW>(a, tile, acc);
+		conv_kseg<NT, CINP, (CINP - CA) / 8, false, coarse_px(CTH * RPW) * CA, CA, RPW>(a, tile, acc);
 	} else {
-		conv_kseg<NT, CINP, CINP / 8, false, 0, 0>(a, tile, acc);
+		conv_kseg<NT, CINP, CINP / 8, false, 0, 0, RPW>(a, tile, acc);
 	}
 }
 
-template <int SRC, int OUTK, int NT, int CINP = 0, int CA = 0>
-__global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
+// One (CTH·RPW) × CTW output tile per workgroup, wave wv computing rows RPW·wv .. RPW·wv + RPW − 1
+// (RPW > 1 only with the compile-time K loop, CINP > 0: each weight fragment then feeds RPW rows, and
+// the staged halo is 2 rows in CTH·RPW + 2 instead of in CTH + 2).
+template <int SRC, int OUTK, int NT, int CINP = 0, int CA = 0, int RPW = 1>
+__global__ __launch_bounds__(BLOCK, RPW > 1 ? 2 : 1) void k_conv3m(ConvArgs a) {
+	static_assert(RPW == 1 || CINP > 0, "several rows per wave need the compile-time K loop");
 	if (a.skip && *a.skip) return;
 	extern __shared__ __align__(16) bf16 tile[];
-	const int b = blockIdx.z, row0 = blockIdx.y * CTH, col0 = blockIdx.x * CTW;
+	constexpr int TR = CTH * RPW;
+	const int b = blockIdx.z, row0 = blockIdx.y * TR, col0 = blockIdx.x * CTW;
 	float bv[NT];
 	load_bias<NT>(a, bv);
 	// Pixels are stored unpadded, their chunks swizzled by chunk_sw.
 	const int pitch = a.cin_p;
-	constexpr int TW = CTW + 2, TPX = (CTH + 2) * TW;
+	constexpr int TW = CTW + 2, TPX = (TR + 2) * TW, CPX = coarse_px(TR);
 	// Staging, chunk-major: a thread owns tile pixels threadIdx.x + k·BLOCK (k < PPT), whose image
 	// position, bounds test and source pointers it forms once; the loop over 8-channel chunks is then
 	// wave-uniform, and CG chunks × PPT pixels of 16-byte loads are in flight together.  Pixels
 	// outside the image stage zeros.
 	constexpr int PPT = (TPX + BLOCK - 1) / BLOCK, CG = 2;
-	// SRC_UPCAT: the upsampled operand's CUR × CUW coarse pixels at element 0 with pitch cpitch, the
-	// skip's fine pixels after them (element soff) with pitch spitch.  A fine pixel outside the image
-	// is exactly a coarse pixel outside the coarse image (H and W are even), so both stage zeros there.
-	const int cpitch = a.ca, spitch = a.cin_p - a.ca, soff = CUPX * cpitch;
+	// SRC_UPCAT: the upsampled operand's CPX coarse pixels (TR / 2 + 2 rows × CUW) at element 0 with
+	// pitch cpitch, the skip's fine pixels after them (element soff) with pitch spitch.  A fine pixel
+	// outside the image is exactly a coarse pixel outside the coarse image (H and W are even), so both
+	// stage zeros there.
+	const int cpitch = a.ca, spitch = a.cin_p - a.ca, soff = CPX * cpitch;
 	const int nch = CINP > 0 ? (SRC == SRC_UPCAT ? (CINP - CA) >> 3 : CINP >> 3)
 	                         : (SRC == SRC_UPCAT ? (a.cin_p - a.ca) >> 3 : a.cin_p >> 3);
 	const char* src0[PPT];   // the pixel's first byte in `in` (SRC_UPCAT: in the skip `in2`)
@@ -427,10 +460,9 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 		}
 	}
 	if constexpr (SRC == SRC_UPCAT) {
-		// The coarse operand: thread t < CUPX stages coarse tile pixel t, four 16-byte loads in flight.
-		static_assert(CUPX <= BLOCK, "one coarse pixel per thread");
-		const int t = threadIdx.x;
-		if (t < CUPX) {
+		// The coarse operand: thread t stages coarse tile pixels t, t + BLOCK, …, four 16-byte loads in
+		// flight.
+		for (int t = threadIdx.x; t < CPX; t += BLOCK) {
 			const int pr = t / CUW, pc = t - pr * CUW;
 			const int Hc = a.H >> 1, Wc = a.W >> 1;
 			const int rr = (row0 >> 1) - 1 + pr, cc = (col0 >> 1) - 1 + pc;
@@ -496,10 +528,10 @@ __global__ __launch_bounds__(BLOCK) void k_conv3m(ConvArgs a) {
 	}
 	__syncthreads();
 
-	f4 acc[4][NT];
-	if constexpr (CINP > 0) conv_kloop_ct<SRC, NT, CINP, CA>(a, tile, acc);
+	f4 acc[4 * RPW][NT];
+	if constexpr (CINP > 0) conv_kloop_ct<SRC, NT, CINP, CA, RPW>(a, tile, acc);
 	else conv_kloop<SRC, NT>(a, tile, acc);
-	conv_epilogue<OUTK, NT>(a, acc, bv, tile, b, row0, col0);
+	conv_epilogue<OUTK, NT, RPW>(a, acc, bv, tile, b, row0, col0);
 }
 
 // Persistent convolutions with 16 output channels at full resolution, for two inputs: a 16-channel
@@ -681,25 +713,35 @@ fx_status launch_conv_nt(const ConvArgs& a, hipStream_t st) {
 }
 // The forward convolutions of the network with their channel counts fixed at compile time
 // (conv_kloop_ct); launch_ct reports whether this shape has one.
-template <int SRC, int NT, int CINP, int CA>
+template <int SRC, int NT, int CINP, int CA, int RPW = 1>
 fx_status launch_ct1(const ConvArgs& a, hipStream_t st) {
 	static bool attr = false;
-	const void* kern = (const void*)k_conv3m<SRC, OUT_BF16, NT, CINP, CA>;
+	const void* kern = (const void*)k_conv3m<SRC, OUT_BF16, NT, CINP, CA, RPW>;
+	constexpr int TR = CTH * RPW;
+	const size_t lds = conv_lds_bytes(CINP, 16 * NT, SRC == SRC_UPCAT ? CA : 0, TR);
 	if (!attr) {
-		FX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds_bytes(128, 128)));
+		FX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 		attr = true;
 	}
-	dim3 grid((unsigned)((a.W + CTW - 1) / CTW), (unsigned)((a.H + CTH - 1) / CTH), (unsigned)a.B);
-	hipLaunchKernelGGL((k_conv3m<SRC, OUT_BF16, NT, CINP, CA>), grid, dim3(BLOCK),
-	                   conv_lds_bytes(a.cin_p, a.cout_p, SRC == SRC_UPCAT ? a.ca : 0), st, a);
+	dim3 grid((unsigned)((a.W + CTW - 1) / CTW), (unsigned)((a.H + TR - 1) / TR), (unsigned)a.B);
+	hipLaunchKernelGGL((k_conv3m<SRC, OUT_BF16, NT, CINP, CA, RPW>), grid, dim3(BLOCK), lds, st, a);
 	return fx::check_launch("unet conv");
+}
+// Workgroups of a launch with tiles of `rows` rows.
+inline long conv_tiles(const ConvArgs& a, int rows) {
+	return (long)((a.W + CTW - 1) / CTW) * ((a.H + rows - 1) / rows) * a.B;
 }
 fx_status launch_ct(int src, int outk, const ConvArgs& a, hipStream_t st, bool* done) {
 	*done = true;
+	// Two rows per wave where the launch keeps at least 8 such tiles per CU (four rows per wave on the
+	// 16-channel up+cat layer measured the same as two, 386–407 against 393 µs at 4096²).
+	const bool tall2 = conv_tiles(a, 2 * CTH) >= 2048;
 	if (outk == OUT_BF16 && src == SRC_BF16) {
 		const int nt = a.cout_p / 16, ci = a.cin_p;
-		if (nt == 2 && ci == 16) return launch_ct1<SRC_BF16, 2, 16, 0>(a, st);
-		if (nt == 2 && ci == 32) return launch_ct1<SRC_BF16, 2, 32, 0>(a, st);
+		if (nt == 2 && ci == 16) return tall2 ? launch_ct1<SRC_BF16, 2, 16, 0, 2>(a, st) : launch_ct1<SRC_BF16, 2, 16, 0>(a, st);
+		if (nt == 2 && ci == 32) return tall2 ? launch_ct1<SRC_BF16, 2, 32, 0, 2>(a, st) : launch_ct1<SRC_BF16, 2, 32, 0>(a, st);
+		// (The 48-channel layers were slower with two rows per wave: 62 and 75 against 60 and 71 µs at
+		// 1024².)
 		if (nt == 3 && ci == 32) return launch_ct1<SRC_BF16, 3, 32, 0>(a, st);
 		if (nt == 3 && ci == 48) return launch_ct1<SRC_BF16, 3, 48, 0>(a, st);
 		if (nt == 4 && ci == 48) return launch_ct1<SRC_BF16, 4, 48, 0>(a, st);
@@ -707,8 +749,10 @@ fx_status launch_ct(int src, int outk, const ConvArgs& a, hipStream_t st, bool* 
 	}
 	if (outk == OUT_BF16 && src == SRC_UPCAT) {
 		const int nt = a.cout_p / 16, ci = a.cin_p, ca = a.ca;
-		if (nt == 1 && ci == 48 && ca == 32) return launch_ct1<SRC_UPCAT, 1, 48, 32>(a, st);
-		if (nt == 2 && ci == 80 && ca == 48) return launch_ct1<SRC_UPCAT, 2, 80, 48>(a, st);
+		if (nt == 1 && ci == 48 && ca == 32)
+			return tall2 ? launch_ct1<SRC_UPCAT, 1, 48, 32, 2>(a, st) : launch_ct1<SRC_UPCAT, 1, 48, 32>(a, st);
+		if (nt == 2 && ci == 80 && ca == 48)
+			return tall2 ? launch_ct1<SRC_UPCAT, 2, 80, 48, 2>(a, st) : launch_ct1<SRC_UPCAT, 2, 80, 48>(a, st);
 		if (nt == 3 && ci == 112 && ca == 64) return launch_ct1<SRC_UPCAT, 3, 112, 64>(a, st);
 	}
 	*done = false;
