@@ -71,14 +71,14 @@ def kernel_bytes(vk: bool, fused_update: bool) -> dict:
 # profiles/vark4096_mixed_fcg_wave.md.  Only valid for --size 4096 with the wave level kernels, and
 # only for the kernel sources the counters were read from: "sources_sha16" is the hash of
 # TRAFFIC_SOURCES at that run, and a bench whose sources differ reports traffic null.  The level-0
-# V-cycle kernels' fetch depends on the autotuned segment heights (lag rows): the restriction's
-# figure is at 16 coarse rows per wave, the post kernel's at 16 fine rows (its writes at 32).
+# V-cycle kernels' fetch depends on the autotuned segment heights (lag rows): both passes ran the
+# restriction at 8 coarse rows per wave and the post kernel at 32 fine rows, as the timed run did.
 TRAFFIC_SOURCES = ("fluxion_amd/csrc/fx_device.hpp", "fluxion_amd/csrc/fx_mgfast.hpp", "fluxion_amd/csrc/fx_solvers.hip")
 MEASURED_TRAFFIC_4096 = {
-    "sources_sha16": "64ac4c3bd22c999c",
-    "vc_restrict0": (272809 * 2 + 151979) * 1024,
-    "vc_post0": (307830 * 2 + 74923) * 1024,
-    "fcg_step": (244676 * 2 + 393248) * 1024,
+    "sources_sha16": "3d03464c8a56e57c",
+    "vc_restrict0": (275562 * 2 + 151982) * 1024,
+    "vc_post0": (292545 * 2 + 74924) * 1024,
+    "fcg_step": (244647 * 2 + 393248) * 1024,
 }
 
 
